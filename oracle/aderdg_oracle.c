@@ -1,0 +1,1230 @@
+/*
+ * aderdg_oracle.c -- CPU restatement of the reference ADER-DG element step
+ * (2-D Euler).  TEST INFRASTRUCTURE ONLY: the checker for the HIP path, never
+ * linked into the product.
+ *
+ * Build with -O2 -ffp-contract=off: every expression keeps the association and
+ * summation order of the Fortran it restates (matmul sums run over the inner
+ * index in increasing order from 0.0, Fortran's left-to-right evaluation of
+ * equal-precedence operators), so results are bit-identical to the reference
+ * built with flang -O3 -fdefault-real-8 on x86-64 (no FMA).  The CPU tests
+ * check that bit-identity routine by routine against oracle/_ref.
+ */
+#include "aderdg_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+static double g_ops;
+#define OPS(k) (g_ops += (double)(k))
+void orc_reset_opcount(void) { g_ops = 0.0; }
+double orc_get_opcount(void) { return g_ops; }
+
+#define NMAX 10
+#define SMAX (2 * NMAX + 1)
+#define VMAX 9
+
+struct orc_ctx {
+    int pde, N, n, V, s, imax, jmax, bc_mode;
+    double xl[2], xr[2], dx[2];
+    double gamma;
+    double cflpnpm; /* CFL*PNPMTable(N), Euler/calc_time_step.f90:41 */
+    double xi[NMAX], w[NMAX], mm[NMAX * NMAX], imm[NMAX * NMAX], kxi[NMAX * NMAX], dudx[NMAX * NMAX];
+    double k1[NMAX * NMAX], ik1[NMAX * NMAX], f0[NMAX], flc[NMAX], frc[NMAX];
+    double u2l[SMAX * NMAX], l2u[NMAX * SMAX], u2b[NMAX * NMAX], xlob[NMAX], wlob[NMAX];
+    double dt, time, tend;
+    /* state */
+    double *uh, *olduh, *qhi, *Fhi, *qbnd, *Fbnd, *duh, *Fu, *Gu;
+    int *recompute, *status, *oldstatus;
+    double *Lh, *NewLh, *lmin, *lmax, *lmin_old, *lmax_old;
+};
+
+/* PNPMTable of Euler/ader_dg.f90:28 */
+static const double PNPM[10] = {1.0, 0.33, 0.17, 0.1, 0.069, 0.045, 0.038, 0.03, 0.02, 0.015};
+
+/* ------------------------------------------------------------------ */
+/* quadrature, basis, linear algebra (setup)                            */
+/* ------------------------------------------------------------------ */
+
+/* gauleg: Euler/quadrature.f90:8-44 */
+static void gauleg(double x1, double x2, double *x, double *w, int n)
+{
+    const double EPS = 1.0e-15, PI = 4.0 * atan(1.0);
+    int m = (n + 1) / 2;
+    double xm = 0.5 * (x2 + x1), xl = 0.5 * (x2 - x1);
+    for (int i = 1; i <= m; i++) {
+        double z = cos(PI * (i - 0.25) / (n + 0.5)), z1, p1, p2, p3, pp;
+        do {
+            p1 = 1.0;
+            p2 = 0.0;
+            for (int j = 1; j <= n; j++) {
+                p3 = p2;
+                p2 = p1;
+                p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / (double)j;
+            }
+            pp = (double)n * (z * p1 - p2) / (z * z - 1.0);
+            z1 = z;
+            z = z1 - p1 / pp;
+        } while (fabs(z - z1) > EPS);
+        x[i - 1] = xm - xl * z;
+        x[n - i] = xm + xl * z;
+        w[i - 1] = 2.0 * xl / ((1.0 - z * z) * pp * pp);
+        w[n - i] = w[i - 1];
+    }
+}
+
+/* gaulob: Euler/quadrature.f90:47-101 */
+static void gaulob(double x1, double x2, double *x, double *w, int n1)
+{
+    const double EPS = 1.0e-15, PI = 4.0 * atan(1.0);
+    int N = n1 - 1;
+    double xold[NMAX], P[NMAX][NMAX + 1]; /* P[point][k], k = 1..n1 */
+    for (int i = 0; i <= N; i++) x[i] = cos(PI * (double)i / (double)N);
+    for (int iter = 1; iter <= 100000; iter++) {
+        double dmax = 0.0;
+        for (int a = 0; a < n1; a++) xold[a] = x[a];
+        for (int a = 0; a < n1; a++) {
+            P[a][1] = 1.0;
+            P[a][2] = x[a];
+            for (int k = 2; k <= N; k++)
+                P[a][k + 1] = ((double)(2 * k - 1) * x[a] * P[a][k] - (double)(k - 1) * P[a][k - 1]) / (double)k;
+        }
+        for (int a = 0; a < n1; a++) x[a] = xold[a] - (x[a] * P[a][n1] - P[a][N]) / ((double)n1 * P[a][n1]);
+        for (int a = 0; a < n1; a++) {
+            double d = fabs(x[a] - xold[a]);
+            if (d > dmax) dmax = d;
+        }
+        if (dmax <= EPS) break;
+    }
+    for (int a = 0; a < n1; a++) {
+        w[a] = 2.0 / ((double)(N * n1) * (P[a][n1] * P[a][n1]));
+        w[a] = 0.5 * w[a] * (x2 - x1);
+        x[a] = x1 + 0.5 * (x[a] + 1.0) * (x2 - x1);
+    }
+}
+
+/* basis_func_1d: Euler/basis.f90:9-41 (Lagrange basis through xin) */
+static void basis_1d(const double *xin, int n, double xi, double *phi, double *phi_xi)
+{
+    for (int m = 0; m < n; m++) {
+        phi[m] = 1.0;
+        phi_xi[m] = 0.0;
+    }
+    for (int m = 0; m < n; m++) {
+        for (int j = 0; j < n; j++) {
+            if (j == m) continue;
+            phi[m] = phi[m] * (xi - xin[j]) / (xin[m] - xin[j]);
+        }
+        for (int i = 0; i < n; i++) {
+            if (i == m) continue;
+            double tmp = 1.0;
+            for (int j = 0; j < n; j++) {
+                if (j == i || j == m) continue;
+                tmp = tmp * (xi - xin[j]) / (xin[m] - xin[j]);
+            }
+            phi_xi[m] = phi_xi[m] + tmp / (xin[m] - xin[i]);
+        }
+    }
+}
+
+/* MatrixInverse: Euler/la.f90:8-65 (Gauss-Jordan on the transposed matrix,
+ * column pivoting by maxloc).  A and iA column-major NxN. */
+static int matinv(int N, const double *A, double *iA)
+{
+    double C[2 * (NMAX + 2)][NMAX + 2]; /* C[row][col], rows 0..2N-1 */
+    for (int r = 0; r < N; r++)
+        for (int c = 0; c < N; c++) C[r][c] = A[c + N * r]; /* C(1:N,:) = transpose(A) */
+    for (int r = N; r < 2 * N; r++)
+        for (int c = 0; c < N; c++) C[r][c] = 0.0;
+    for (int i = 0; i < N; i++) C[N + i][i] = 1.0;
+    for (int i = 0; i < N; i++) {
+        int j = i;
+        double best = fabs(C[i][i]);
+        for (int c = i + 1; c < N; c++)
+            if (fabs(C[i][c]) > best) {
+                best = fabs(C[i][c]);
+                j = c;
+            }
+        for (int r = 0; r < 2 * N; r++) {
+            double t = C[r][j];
+            C[r][j] = C[r][i];
+            C[r][i] = t;
+        }
+        if (fabs(C[i][i]) <= 1.0e-15) return -1;
+        double piv = 1.0 / C[i][i];
+        for (int r = 0; r < 2 * N; r++) C[r][i] = C[r][i] * piv;
+        for (int c = i + 1; c < N; c++) {
+            double f = C[i][c];
+            for (int r = 0; r < 2 * N; r++) C[r][c] = C[r][c] - f * C[r][i];
+        }
+    }
+    for (int i = N - 1; i >= 0; i--)
+        for (int c = i - 1; c >= 0; c--) {
+            double f = C[i][c];
+            for (int r = 0; r < 2 * N; r++) C[r][c] = C[r][c] - f * C[r][i];
+        }
+    for (int r = 0; r < N; r++)
+        for (int c = 0; c < N; c++) iA[r + N * c] = C[N + c][r]; /* iA = transpose(C(N+1:2N,:)) */
+    return 0;
+}
+
+/* operator construction: Euler/initialize.f90:80-213 */
+static void build_ops(orc_ctx *c)
+{
+    int n = c->n, s = c->s;
+    double phi[NMAX], phx[NMAX], phi0[NMAX], phi1[NMAX];
+    gauleg(0.0, 1.0, c->xi, c->w, n);
+    memset(c->mm, 0, sizeof c->mm);
+    memset(c->kxi, 0, sizeof c->kxi);
+    for (int g = 0; g < n; g++) {
+        basis_1d(c->xi, n, c->xi[g], phi, phx);
+        for (int k = 0; k < n; k++)
+            for (int l = 0; l < n; l++) {
+                c->mm[k + n * l] = c->mm[k + n * l] + c->w[g] * phi[k] * phi[l];
+                c->kxi[k + n * l] = c->kxi[k + n * l] + c->w[g] * phx[k] * phi[l];
+            }
+    }
+    if (matinv(n, c->mm, c->imm)) fprintf(stderr, "orc: singular mass matrix\n");
+    for (int a = 0; a < n; a++) /* dudx = matmul(iMM, transpose(Kxi)) */
+        for (int b = 0; b < n; b++) {
+            double acc = 0.0;
+            for (int k = 0; k < n; k++) acc = acc + c->imm[a + n * k] * c->kxi[b + n * k];
+            c->dudx[a + n * b] = acc;
+        }
+    basis_1d(c->xi, n, 0.0, phi0, phx);
+    basis_1d(c->xi, n, 1.0, phi1, phx);
+    for (int k = 0; k < n; k++) {
+        c->f0[k] = phi0[k];
+        c->flc[k] = phi0[k];
+        c->frc[k] = phi1[k];
+        for (int l = 0; l < n; l++) c->k1[k + n * l] = phi1[k] * phi1[l] - c->kxi[k + n * l];
+    }
+    if (matinv(n, c->k1, c->ik1)) fprintf(stderr, "orc: singular K1\n");
+    /* subcell maps, Euler/initialize.f90:175-201 */
+    double dxi = 1.0 / (double)s;
+    memset(c->u2l, 0, sizeof c->u2l);
+    for (int i = 1; i <= s; i++) {
+        double xi1 = (double)(i - 1) * dxi;
+        for (int ii = 0; ii < n; ii++) {
+            double x = xi1 + dxi * c->xi[ii];
+            basis_1d(c->xi, n, x, phi, phx);
+            for (int a = 0; a < n; a++) c->u2l[(i - 1) + s * a] = c->u2l[(i - 1) + s * a] + c->w[ii] * phi[a];
+        }
+    }
+    int m = n + 1;
+    double lsq[(NMAX + 1) * (NMAX + 1)], ilsq[(NMAX + 1) * (NMAX + 1)], rhs[(NMAX + 1) * SMAX];
+    for (int a = 0; a < n; a++)
+        for (int b = 0; b < n; b++) {
+            double acc = 0.0;
+            for (int p = 0; p < s; p++) acc = acc + c->u2l[p + s * a] * c->u2l[p + s * b];
+            lsq[a + m * b] = 2.0 * acc;
+        }
+    for (int a = 0; a < n; a++) {
+        lsq[n + m * a] = c->w[a];
+        lsq[a + m * n] = -c->w[a];
+    }
+    lsq[n + m * n] = 0.0;
+    if (matinv(m, lsq, ilsq)) fprintf(stderr, "orc: singular LSQ matrix\n");
+    for (int a = 0; a < n; a++)
+        for (int p = 0; p < s; p++) rhs[a + m * p] = 2.0 * c->u2l[p + s * a];
+    for (int p = 0; p < s; p++) rhs[n + m * p] = dxi;
+    for (int a = 0; a < n; a++)
+        for (int p = 0; p < s; p++) {
+            double acc = 0.0;
+            for (int k = 0; k < m; k++) acc = acc + ilsq[a + m * k] * rhs[k + m * p];
+            c->l2u[a + n * p] = acc;
+        }
+    gaulob(0.0, 1.0, c->xlob, c->wlob, n);
+    for (int ii = 0; ii < n; ii++) {
+        basis_1d(c->xi, n, c->xlob[ii], phi, phx);
+        for (int a = 0; a < n; a++) c->u2b[ii + n * a] = phi[a];
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* PDE pointwise routines: Euler/pde.f90                                */
+/* ------------------------------------------------------------------ */
+
+/* PDEFlux: Euler/pde.f90:46-69.  F(V,2) column-major. */
+static void euler_flux(double g, const double *Q, double *F)
+{
+    double irho = 1.0 / Q[0];
+    double p = (g - 1.0) * (Q[3] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2]) * irho);
+    F[0] = Q[1];
+    F[1] = irho * Q[1] * Q[1] + p;
+    F[2] = irho * Q[1] * Q[2];
+    F[3] = irho * Q[1] * (Q[3] + p);
+    F[4] = Q[2];
+    F[5] = irho * Q[2] * Q[1];
+    F[6] = irho * Q[2] * Q[2] + p;
+    F[7] = irho * Q[2] * (Q[3] + p);
+    OPS(1 + 9 + 3 + 2 + 3 + 2 + 3 + 3);
+}
+
+/* PDEEigenvalues: Euler/pde.f90:75-91 */
+static void euler_eigen(double g, const double *Q, const double *nv, double *L)
+{
+    double u = (Q[1] * nv[0] + Q[2] * nv[1]) / Q[0];
+    double p = (g - 1.0) * (Q[3] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2]) / Q[0]);
+    double cs = sqrt(g * p / Q[0]);
+    L[0] = u - cs;
+    L[1] = u;
+    L[2] = u;
+    L[3] = u + cs;
+    OPS(4 + 8 + 3 + 2);
+}
+
+/* Fortran intrinsic semantics as compiled by flang (checked on a probe program):
+ * MAXVAL/MINVAL skip NaN elements (NaN only when every element is NaN), while the
+ * two-argument MAX(a,b)/MIN(a,b) evaluate as (a > b ? a : b) / (a < b ? a : b),
+ * so a NaN second argument wins.  Multi-argument MIN/MAX pair up left to right. */
+static double f_max(double a, double b) { return a > b ? a : b; }
+static double f_min(double a, double b) { return a < b ? a : b; }
+static double maxval_s(const double *x, int cnt, int stride)
+{
+    double m = NAN;
+    for (int k = 0; k < cnt; k++) {
+        double e = x[(size_t)k * stride];
+        if (isnan(e)) continue;
+        if (isnan(m) || e > m) m = e;
+    }
+    return m;
+}
+static double minval_s(const double *x, int cnt, int stride)
+{
+    double m = NAN;
+    for (int k = 0; k < cnt; k++) {
+        double e = x[(size_t)k * stride];
+        if (isnan(e)) continue;
+        if (isnan(m) || e < m) m = e;
+    }
+    return m;
+}
+
+/* maxval(abs(L)) */
+static double maxabs(const double *L, int V)
+{
+    double a[VMAX];
+    for (int v = 0; v < V; v++) a[v] = fabs(L[v]);
+    return maxval_s(a, V, 1);
+}
+
+/* RusanovFlux: Euler/pde.f90:97-114 */
+static void euler_rusanov(double g, const double *QL, const double *FL, const double *QR, const double *FR,
+                          const double *nv, double *flux)
+{
+    double LL[4], LR[4];
+    euler_eigen(g, QL, nv, LL);
+    euler_eigen(g, QR, nv, LR);
+    double smax = f_max(maxabs(LL, 4), maxabs(LR, 4));
+    for (int v = 0; v < 4; v++) flux[v] = 0.5 * (FL[v] + FR[v]) - 0.5 * smax * (QR[v] - QL[v]);
+    OPS(4 * 6);
+}
+
+/* PDEAssurePositivity: Euler/pde.f90:120-145 (returns iErr) */
+static int euler_positivity(double g, const double *Q)
+{
+    int ierr = 0;
+    double irho = 1.0 / Q[0];
+    double p = (g - 1.0) * (Q[3] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2]) * irho);
+    if (Q[0] <= 1.0e-12) ierr = -1;
+    if (p <= 1.0e-12) ierr = -2;
+    OPS(1 + 8);
+    return ierr;
+}
+
+/* PDEPrim2Cons: Euler/pde.f90:29-40 */
+static void euler_prim2cons(double g, const double *Vp, double *Q)
+{
+    Q[0] = Vp[0];
+    Q[1] = Vp[0] * Vp[1];
+    Q[2] = Vp[0] * Vp[2];
+    Q[3] = Vp[3] / (g - 1.0) + 0.5 * Vp[0] * (Vp[1] * Vp[1] + Vp[2] * Vp[2]);
+}
+
+/* ------------------------------------------------------------------ */
+/* context                                                             */
+/* ------------------------------------------------------------------ */
+
+orc_ctx *orc_create(int pde, int N, int imax, int jmax, const double xl[2], const double xr[2], int bc_mode,
+                    const double *eos)
+{
+    if (pde != ORC_PDE_EULER || N < 1 || N > 9 || imax < 1 || jmax < 1) return NULL;
+    orc_ctx *c = (orc_ctx *)calloc(1, sizeof(orc_ctx));
+    c->pde = pde;
+    c->N = N;
+    c->n = N + 1;
+    c->s = 2 * N + 1;
+    c->V = 4;
+    c->imax = imax;
+    c->jmax = jmax;
+    c->bc_mode = bc_mode;
+    c->gamma = eos[0];
+    for (int d = 0; d < 2; d++) {
+        c->xl[d] = xl[d];
+        c->xr[d] = xr[d];
+    }
+    c->dx[0] = (xr[0] - xl[0]) / (double)imax; /* dx = (xR-xL)/VMAX, Euler/initialize.f90:43 */
+    c->dx[1] = (xr[1] - xl[1]) / (double)jmax;
+    c->cflpnpm = 0.9 * PNPM[N];
+    build_ops(c);
+    size_t nc = (size_t)imax * jmax, V = c->V, n = c->n, s = c->s;
+    c->uh = (double *)calloc(V * n * n * nc, sizeof(double));
+    c->olduh = (double *)calloc(V * n * n * nc, sizeof(double));
+    c->qhi = (double *)calloc(V * n * n * nc, sizeof(double));
+    c->Fhi = (double *)calloc(2 * V * n * n * nc, sizeof(double));
+    c->duh = (double *)calloc(V * n * n * nc, sizeof(double));
+    c->qbnd = (double *)calloc(4 * V * n * nc, sizeof(double));
+    c->Fbnd = (double *)calloc(4 * V * n * nc, sizeof(double));
+    c->Fu = (double *)calloc(V * n * (imax + 1) * jmax, sizeof(double));
+    c->Gu = (double *)calloc(V * n * imax * (jmax + 1), sizeof(double));
+    c->recompute = (int *)calloc(nc, sizeof(int));
+    c->status = (int *)calloc(nc, sizeof(int));
+    c->oldstatus = (int *)calloc(nc, sizeof(int));
+    c->Lh = (double *)calloc(V * s * s * nc, sizeof(double));
+    c->NewLh = (double *)calloc(V * s * s * nc, sizeof(double));
+    c->lmin = (double *)calloc(V * nc, sizeof(double));
+    c->lmax = (double *)calloc(V * nc, sizeof(double));
+    c->lmin_old = (double *)calloc(V * nc, sizeof(double));
+    c->lmax_old = (double *)calloc(V * nc, sizeof(double));
+    return c;
+}
+
+void orc_destroy(orc_ctx *c)
+{
+    if (!c) return;
+    free(c->uh); free(c->olduh); free(c->qhi); free(c->Fhi); free(c->duh); free(c->qbnd); free(c->Fbnd);
+    free(c->Fu); free(c->Gu); free(c->recompute); free(c->status); free(c->oldstatus); free(c->Lh);
+    free(c->NewLh); free(c->lmin); free(c->lmax); free(c->lmin_old); free(c->lmax_old);
+    free(c);
+}
+
+void orc_get_ops(const orc_ctx *c, double *xi, double *w, double *mm, double *kxi, double *dudx, double *ik1,
+                 double *f0, double *flc, double *frc, double *u2l, double *l2u, double *u2b, double *xlob,
+                 double *wlob, double *k1)
+{
+    int n = c->n, s = c->s;
+    memcpy(xi, c->xi, n * sizeof(double));
+    memcpy(w, c->w, n * sizeof(double));
+    memcpy(mm, c->mm, n * n * sizeof(double));
+    memcpy(kxi, c->kxi, n * n * sizeof(double));
+    memcpy(dudx, c->dudx, n * n * sizeof(double));
+    memcpy(ik1, c->ik1, n * n * sizeof(double));
+    memcpy(f0, c->f0, n * sizeof(double));
+    memcpy(flc, c->flc, n * sizeof(double));
+    memcpy(frc, c->frc, n * sizeof(double));
+    memcpy(u2l, c->u2l, s * n * sizeof(double));
+    memcpy(l2u, c->l2u, n * s * sizeof(double));
+    memcpy(u2b, c->u2b, n * n * sizeof(double));
+    memcpy(xlob, c->xlob, n * sizeof(double));
+    memcpy(wlob, c->wlob, n * sizeof(double));
+    memcpy(k1, c->k1, n * n * sizeof(double));
+}
+
+void orc_set_scalars(orc_ctx *c, double dt, double time, double tend)
+{
+    c->dt = dt;
+    c->time = time;
+    c->tend = tend;
+}
+
+void orc_get_scalars(const orc_ctx *c, double *dt, double *time, double *tend)
+{
+    *dt = c->dt;
+    *time = c->time;
+    *tend = c->tend;
+}
+
+static size_t ncell(const orc_ctx *c) { return (size_t)c->imax * c->jmax; }
+
+void orc_set_uh(orc_ctx *c, const double *uh) { memcpy(c->uh, uh, c->V * c->n * c->n * ncell(c) * sizeof(double)); }
+void orc_get_uh(const orc_ctx *c, double *uh) { memcpy(uh, c->uh, c->V * c->n * c->n * ncell(c) * sizeof(double)); }
+
+void orc_set_limiter(orc_ctx *c, const int *status, const double *lh)
+{
+    size_t blk = (size_t)c->V * c->s * c->s;
+    for (size_t k = 0; k < ncell(c); k++) {
+        c->status[k] = c->oldstatus[k] = status[k] ? 1 : 0;
+        c->recompute[k] = 0;
+        if (status[k]) {
+            memcpy(c->Lh + k * blk, lh + k * blk, blk * sizeof(double));
+            memcpy(c->NewLh + k * blk, lh + k * blk, blk * sizeof(double));
+        }
+    }
+}
+
+void orc_get_limiter(const orc_ctx *c, int *status, int *recompute, double *lh, double *lmin, double *lmax)
+{
+    size_t blk = (size_t)c->V * c->s * c->s;
+    for (size_t k = 0; k < ncell(c); k++) {
+        status[k] = c->status[k];
+        recompute[k] = c->recompute[k];
+        if (c->status[k])
+            memcpy(lh + k * blk, c->Lh + k * blk, blk * sizeof(double));
+        else
+            memset(lh + k * blk, 0, blk * sizeof(double));
+    }
+    memcpy(lmin, c->lmin, c->V * ncell(c) * sizeof(double));
+    memcpy(lmax, c->lmax, c->V * ncell(c) * sizeof(double));
+}
+
+/* ------------------------------------------------------------------ */
+/* per-cell routines                                                    */
+/* ------------------------------------------------------------------ */
+
+void orc_flux(orc_ctx *c, const double *q, double *f) { euler_flux(c->gamma, q, f); }
+void orc_eigen(orc_ctx *c, const double *q, const double *nv, double *lam) { euler_eigen(c->gamma, q, nv, lam); }
+void orc_rusanov(orc_ctx *c, const double *ql, const double *fl, const double *qr, const double *fr,
+                 const double *nv, double *flux)
+{
+    euler_rusanov(c->gamma, ql, fl, qr, fr, nv, flux);
+}
+
+/* ader_space_time_predictor: Euler/space_time_predictor.f90:8-123 */
+void orc_predictor(orc_ctx *c, double *lqhi, double *lFhi, double *lqbnd, double *lFbnd, const double *luh)
+{
+    const int n = c->n, V = c->V;
+    const double *w = c->w;
+    double lqh[VMAX * NMAX * NMAX * NMAX], rhs0[VMAX * NMAX * NMAX * NMAX], rhs[VMAX * NMAX * NMAX * NMAX];
+    double lFh[2 * VMAX * NMAX * NMAX * NMAX];
+#define Q4(v, i, j, l) ((v) + V * ((i) + n * ((j) + n * (l))))
+#define F5(v, d, i, j, l) ((v) + V * ((d) + 2 * ((i) + n * ((j) + n * (l)))))
+    /* :29-45 initial guess and rhs0 */
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+            for (int v = 0; v < V; v++)
+                for (int l = 0; l < n; l++) lqh[Q4(v, i, j, l)] = luh[v + V * (i + n * j)];
+            for (int v = 0; v < V; v++)
+                for (int l = 0; l < n; l++) rhs0[Q4(v, i, j, l)] = w[i] * w[j] * c->f0[l] * luh[v + V * (i + n * j)];
+        }
+    OPS(3 * V * n * n * n);
+    /* :49-84 Picard iterations */
+    for (int iter = 0; iter < n; iter++) {
+        for (int l = 0; l < n; l++) {
+            for (int j = 0; j < n; j++)
+                for (int i = 0; i < n; i++) euler_flux(c->gamma, &lqh[Q4(0, i, j, l)], &lFh[F5(0, 0, i, j, l)]);
+            for (int j = 0; j < n; j++) { /* :65-67 */
+                double sx = w[l] * w[j] * c->dt / c->dx[0];
+                for (int i = 0; i < n; i++)
+                    for (int v = 0; v < V; v++) {
+                        double acc = 0.0;
+                        for (int k = 0; k < n; k++) acc = acc + lFh[F5(v, 0, k, j, l)] * c->kxi[k + n * i];
+                        rhs[Q4(v, i, j, l)] = rhs0[Q4(v, i, j, l)] - sx * acc;
+                    }
+            }
+            for (int i = 0; i < n; i++) { /* :71-73 */
+                double sy = w[l] * w[i] * c->dt / c->dx[1];
+                for (int j = 0; j < n; j++)
+                    for (int v = 0; v < V; v++) {
+                        double acc = 0.0;
+                        for (int k = 0; k < n; k++) acc = acc + lFh[F5(v, 1, i, k, l)] * c->kxi[k + n * j];
+                        rhs[Q4(v, i, j, l)] = rhs[Q4(v, i, j, l)] - sy * acc;
+                    }
+            }
+            OPS(2 * (3 * n) + 2 * V * n * n * (2 * n + 2));
+        }
+        for (int j = 0; j < n; j++) /* :79-83 */
+            for (int i = 0; i < n; i++) {
+                double f = 1.0 / (w[i] * w[j]);
+                double tmp[VMAX * NMAX];
+                for (int l = 0; l < n; l++)
+                    for (int v = 0; v < V; v++) {
+                        double acc = 0.0;
+                        for (int m = 0; m < n; m++) acc = acc + rhs[Q4(v, i, j, m)] * c->ik1[l + n * m];
+                        tmp[v + V * l] = f * acc;
+                    }
+                for (int l = 0; l < n; l++)
+                    for (int v = 0; v < V; v++) lqh[Q4(v, i, j, l)] = tmp[v + V * l];
+            }
+        OPS(n * n * (2 + V * n * (2 * n + 1)));
+    }
+    /* :88-95 time averages */
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+            for (int v = 0; v < V; v++) {
+                double acc = 0.0;
+                for (int l = 0; l < n; l++) acc = acc + lqh[Q4(v, i, j, l)] * w[l];
+                lqhi[v + V * (i + n * j)] = acc;
+            }
+            for (int d = 0; d < 2; d++)
+                for (int v = 0; v < V; v++) {
+                    double acc = 0.0;
+                    for (int l = 0; l < n; l++) acc = acc + lFh[F5(v, d, i, j, l)] * w[l];
+                    lFhi[v + V * (d + 2 * (i + n * j))] = acc;
+                }
+        }
+    OPS(3 * V * n * n * 2 * n);
+    /* :100-119 boundary extrapolation */
+    for (int j = 0; j < n; j++)
+        for (int v = 0; v < V; v++) {
+            double a = 0.0, b = 0.0, e = 0.0, g = 0.0;
+            for (int i = 0; i < n; i++) {
+                a = a + lqhi[v + V * (i + n * j)] * c->flc[i];
+                b = b + lqhi[v + V * (i + n * j)] * c->frc[i];
+                e = e + lFhi[v + V * (0 + 2 * (i + n * j))] * c->flc[i];
+                g = g + lFhi[v + V * (0 + 2 * (i + n * j))] * c->frc[i];
+            }
+            lqbnd[v + V * (0 + 4 * j)] = a;
+            lqbnd[v + V * (1 + 4 * j)] = b;
+            lFbnd[v + V * (0 + 4 * j)] = e;
+            lFbnd[v + V * (1 + 4 * j)] = g;
+        }
+    for (int i = 0; i < n; i++)
+        for (int v = 0; v < V; v++) {
+            double a = 0.0, b = 0.0, e = 0.0, g = 0.0;
+            for (int j = 0; j < n; j++) {
+                a = a + lqhi[v + V * (i + n * j)] * c->flc[j];
+                b = b + lqhi[v + V * (i + n * j)] * c->frc[j];
+                e = e + lFhi[v + V * (1 + 2 * (i + n * j))] * c->flc[j];
+                g = g + lFhi[v + V * (1 + 2 * (i + n * j))] * c->frc[j];
+            }
+            lqbnd[v + V * (2 + 4 * i)] = a;
+            lqbnd[v + V * (3 + 4 * i)] = b;
+            lFbnd[v + V * (2 + 4 * i)] = e;
+            lFbnd[v + V * (3 + 4 * i)] = g;
+        }
+    OPS(8 * V * n * 2 * n);
+#undef Q4
+#undef F5
+}
+
+/* volume_integral: Euler/volume_integral.f90:8-36 */
+void orc_volume(orc_ctx *c, double *lduh, const double *lFhi)
+{
+    const int n = c->n, V = c->V;
+    for (int k = 0; k < V * n * n; k++) lduh[k] = 0.0;
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++)
+            for (int v = 0; v < V; v++) {
+                double acc = 0.0;
+                for (int k = 0; k < n; k++) acc = acc + lFhi[v + V * (0 + 2 * (k + n * j))] * c->kxi[i + n * k];
+                lduh[v + V * (i + n * j)] = lduh[v + V * (i + n * j)] + acc * c->w[j] / c->dx[0];
+            }
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++)
+            for (int v = 0; v < V; v++) {
+                double acc = 0.0;
+                for (int k = 0; k < n; k++) acc = acc + lFhi[v + V * (1 + 2 * (i + n * k))] * c->kxi[j + n * k];
+                lduh[v + V * (i + n * j)] = lduh[v + V * (i + n * j)] + acc * c->w[i] / c->dx[1];
+            }
+    OPS(2 * V * n * n * (2 * n + 3));
+}
+
+/* surface_integral: Euler/surface_integral.f90:8-39 */
+void orc_surface(orc_ctx *c, double *lduh, const double *fl, const double *fr, const double *gb, const double *gt)
+{
+    const int n = c->n, V = c->V;
+    for (int j = 0; j < n; j++)
+        for (int v = 0; v < V; v++) {
+            double s = c->w[j] / c->dx[0];
+            for (int i = 0; i < n; i++)
+                lduh[v + V * (i + n * j)] =
+                    lduh[v + V * (i + n * j)] - s * (fr[v + V * j] * c->frc[i] - fl[v + V * j] * c->flc[i]);
+        }
+    for (int i = 0; i < n; i++)
+        for (int v = 0; v < V; v++) {
+            double s = c->w[i] / c->dx[1];
+            for (int j = 0; j < n; j++)
+                lduh[v + V * (i + n * j)] =
+                    lduh[v + V * (i + n * j)] - s * (gt[v + V * i] * c->frc[j] - gb[v + V * i] * c->flc[j]);
+        }
+    OPS(2 * V * n * (1 + 5 * n));
+}
+
+/* element_update: Euler/element_update.f90:8-30 */
+void orc_update(orc_ctx *c, double *luh, double *lduh)
+{
+    const int n = c->n, V = c->V;
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++)
+            for (int v = 0; v < V; v++) lduh[v + V * (i + n * j)] = lduh[v + V * (i + n * j)] / (c->w[i] * c->w[j]);
+    for (int k = 0; k < V * n * n; k++) luh[k] = luh[k] + c->dt * lduh[k];
+    OPS(4 * V * n * n);
+}
+
+/* get_subcell_data: Euler/limiter.f90:12-37 */
+void orc_subcell_data(orc_ctx *c, double *lim, const double *luh)
+{
+    const int n = c->n, V = c->V, s = c->s;
+    double limy[VMAX * NMAX * SMAX]; /* (V,n,s) */
+    for (int a = 0; a < n; a++)
+        for (int q = 0; q < s; q++)
+            for (int v = 0; v < V; v++) {
+                double acc = 0.0;
+                for (int b = 0; b < n; b++) acc = acc + luh[v + V * (a + n * b)] * c->u2l[q + s * b];
+                limy[v + V * (a + n * q)] = acc;
+            }
+    for (int q = 0; q < s; q++)
+        for (int p = 0; p < s; p++)
+            for (int v = 0; v < V; v++) {
+                double acc = 0.0;
+                for (int a = 0; a < n; a++) acc = acc + limy[v + V * (a + n * q)] * c->u2l[p + s * a];
+                lim[v + V * (p + s * q)] = acc;
+            }
+    OPS(2 * V * n * s * (n + s));
+}
+
+/* get_lobatto_data: Euler/limiter.f90:43-68 */
+void orc_lobatto_data(orc_ctx *c, double *lob, const double *luh)
+{
+    const int n = c->n, V = c->V;
+    double loby[VMAX * NMAX * NMAX];
+    for (int a = 0; a < n; a++)
+        for (int q = 0; q < n; q++)
+            for (int v = 0; v < V; v++) {
+                double acc = 0.0;
+                for (int b = 0; b < n; b++) acc = acc + luh[v + V * (a + n * b)] * c->u2b[q + n * b];
+                loby[v + V * (a + n * q)] = acc;
+            }
+    for (int q = 0; q < n; q++)
+        for (int p = 0; p < n; p++)
+            for (int v = 0; v < V; v++) {
+                double acc = 0.0;
+                for (int a = 0; a < n; a++) acc = acc + loby[v + V * (a + n * q)] * c->u2b[p + n * a];
+                lob[v + V * (p + n * q)] = acc;
+            }
+    OPS(4 * V * n * n * n);
+}
+
+/* put_subcell_data: Euler/limiter.f90:74-98 */
+void orc_put_subcell_data(orc_ctx *c, double *luh, const double *lim)
+{
+    const int n = c->n, V = c->V, s = c->s;
+    double luhy[VMAX * SMAX * NMAX]; /* (V,s,n) */
+    for (int p = 0; p < s; p++)
+        for (int b = 0; b < n; b++)
+            for (int v = 0; v < V; v++) {
+                double acc = 0.0;
+                for (int q = 0; q < s; q++) acc = acc + lim[v + V * (p + s * q)] * c->l2u[b + n * q];
+                luhy[v + V * (p + s * b)] = acc;
+            }
+    for (int b = 0; b < n; b++)
+        for (int a = 0; a < n; a++)
+            for (int v = 0; v < V; v++) {
+                double acc = 0.0;
+                for (int p = 0; p < s; p++) acc = acc + luhy[v + V * (p + s * b)] * c->l2u[a + n * p];
+                luh[v + V * (a + n * b)] = acc;
+            }
+    OPS(2 * V * n * s * (s + n));
+}
+
+/* min/max over nodal, subcell and Lobatto values of one variable.
+ * dmp_order = 0: min(minval(uh), minval(lob), minval(lim))   (get_min_max, Euler/limiter.f90:125-131)
+ * dmp_order = 1: min(minval(uh), minval(lim), minval(lob))   (DMP, Euler/limiter.f90:202-208) */
+static void cell_range(orc_ctx *c, const double *luh, const double *lim, const double *lob, int v, int dmp_order,
+                       double *mn, double *mx)
+{
+    const int n = c->n, V = c->V, s = c->s;
+    double u0 = minval_s(luh + v, n * n, V), u1 = maxval_s(luh + v, n * n, V);
+    double b0 = minval_s(lob + v, n * n, V), b1 = maxval_s(lob + v, n * n, V);
+    double l0 = minval_s(lim + v, s * s, V), l1 = maxval_s(lim + v, s * s, V);
+    if (dmp_order) {
+        *mn = f_min(f_min(u0, l0), b0);
+        *mx = f_max(f_max(u1, l1), b1);
+    } else {
+        *mn = f_min(f_min(u0, b0), l0);
+        *mx = f_max(f_max(u1, b1), l1);
+    }
+}
+
+/* DMP: Euler/limiter.f90:179-245; returns 1 if the cell is admissible */
+int orc_dmp(orc_ctx *c, const double *luh, const double *lmin, const double *lmax, double argmax)
+{
+    const int V = c->V, s = c->s;
+    double lim[VMAX * SMAX * SMAX], lob[VMAX * NMAX * NMAX];
+    orc_subcell_data(c, lim, luh);
+    orc_lobatto_data(c, lob, luh);
+    for (int v = 0; v < V; v++) {
+        double mn, mx;
+        cell_range(c, luh, lim, lob, v, 1, &mn, &mx);
+        double ldiff = f_max(f_max(1.0e-4, 1.0e-3 * (lmax[v] - lmin[v])), argmax);
+        OPS(2 + 2);
+        if (mn < lmin[v] - ldiff) return 0;
+        if (mx > lmax[v] + ldiff) return 0;
+    }
+    for (int k = 0; k < s * s; k++)
+        if (euler_positivity(c->gamma, &lim[V * k]) < 0) return 0;
+    return 1;
+}
+
+double orc_dmp_margin(orc_ctx *c, const double *luh, const double *lmin, const double *lmax, double argmax)
+{
+    const int V = c->V, s = c->s;
+    double lim[VMAX * SMAX * SMAX], lob[VMAX * NMAX * NMAX];
+    orc_subcell_data(c, lim, luh);
+    orc_lobatto_data(c, lob, luh);
+    double margin = INFINITY;
+    for (int v = 0; v < V; v++) {
+        double mn, mx;
+        cell_range(c, luh, lim, lob, v, 1, &mn, &mx);
+        double ldiff = f_max(f_max(1.0e-4, 1.0e-3 * (lmax[v] - lmin[v])), argmax);
+        double a = mn - (lmin[v] - ldiff), b = (lmax[v] + ldiff) - mx;
+        if (a < margin) margin = a;
+        if (b < margin) margin = b;
+    }
+    for (int k = 0; k < s * s; k++) {
+        const double *Q = &lim[V * k];
+        double irho = 1.0 / Q[0];
+        double p = (c->gamma - 1.0) * (Q[3] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2]) * irho);
+        if (Q[0] - 1.0e-12 < margin) margin = Q[0] - 1.0e-12;
+        if (p - 1.0e-12 < margin) margin = p - 1.0e-12;
+    }
+    return margin;
+}
+
+/* ------------------------------------------------------------------ */
+/* whole-mesh phases                                                    */
+/* ------------------------------------------------------------------ */
+
+#define UH(c, arr, ci, cj) (&(arr)[(size_t)(c)->V * (c)->n * (c)->n * ((size_t)(ci) + (size_t)(c)->imax * (cj))])
+#define BND(c, arr, ci, cj) (&(arr)[(size_t)(c)->V * 4 * (c)->n * ((size_t)(ci) + (size_t)(c)->imax * (cj))])
+
+/* neighbour table of Euler/initialize.f90:226-258: clamp, periodic wrap when the mode is periodic */
+static void neighbor(const orc_ctx *c, int i, int j, int di, int dj, int *in, int *jn)
+{
+    int a = i + di, b = j + dj;
+    int per = c->bc_mode == ORC_BC_PERIODIC;
+    if (a < 0) a = per ? c->imax - 1 : 0;
+    if (a > c->imax - 1) a = per ? 0 : c->imax - 1;
+    if (b < 0) b = per ? c->jmax - 1 : 0;
+    if (b > c->jmax - 1) b = per ? 0 : c->jmax - 1;
+    *in = a;
+    *jn = b;
+}
+
+/* calc_time_step: Euler/calc_time_step.f90:9-52 */
+double orc_calc_dt(orc_ctx *c)
+{
+    const int n = c->n, V = c->V;
+    const double nvx[2] = {1.0, 0.0}, nvy[2] = {0.0, 1.0};
+    double dt = 1.0e20;
+    for (int cj = 0; cj < c->jmax; cj++)
+        for (int ci = 0; ci < c->imax; ci++) {
+            const double *u = UH(c, c->uh, ci, cj);
+            for (int k = 0; k < n; k++)
+                for (int l = 0; l < n; l++) {
+                    double L[4], denom = 0.0;
+                    euler_eigen(c->gamma, &u[V * (k + n * l)], nvx, L);
+                    denom = denom + maxabs(L, 4) / c->dx[0];
+                    euler_eigen(c->gamma, &u[V * (k + n * l)], nvy, L);
+                    denom = denom + maxabs(L, 4) / c->dx[1];
+                    dt = f_min(dt, c->cflpnpm / denom);
+                    OPS(5);
+                }
+        }
+    if (c->time + dt > c->tend) dt = c->tend - c->time;
+    c->dt = dt;
+    return dt;
+}
+
+/* get_min_max: Euler/limiter.f90:104-171 */
+static void get_min_max(orc_ctx *c)
+{
+    const int V = c->V, s = c->s;
+    double lim[VMAX * SMAX * SMAX], lob[VMAX * NMAX * NMAX];
+    for (int cj = 0; cj < c->jmax; cj++)
+        for (int ci = 0; ci < c->imax; ci++) {
+            size_t k = (size_t)ci + (size_t)c->imax * cj;
+            if (c->status[k] == 0) {
+                const double *u = UH(c, c->uh, ci, cj);
+                orc_subcell_data(c, lim, u);
+                orc_lobatto_data(c, lob, u);
+                for (int v = 0; v < V; v++)
+                    cell_range(c, u, lim, lob, v, 0, &c->lmin_old[V * k + v], &c->lmax_old[V * k + v]);
+            } else {
+                const double *L = &c->Lh[(size_t)V * s * s * k];
+                for (int v = 0; v < V; v++) {
+                    c->lmin_old[V * k + v] = minval_s(L + v, s * s, V);
+                    c->lmax_old[V * k + v] = maxval_s(L + v, s * s, V);
+                }
+            }
+        }
+    for (int cj = 0; cj < c->jmax; cj++)
+        for (int ci = 0; ci < c->imax; ci++) {
+            size_t k = (size_t)ci + (size_t)c->imax * cj;
+            double qmn[VMAX], qmx[VMAX];
+            for (int v = 0; v < V; v++) {
+                qmn[v] = c->lmin_old[V * k + v];
+                qmx[v] = c->lmax_old[V * k + v];
+            }
+            for (int jj = -1; jj <= 1; jj++)
+                for (int ii = -1; ii <= 1; ii++) {
+                    int in, jn;
+                    neighbor(c, ci, cj, ii, jj, &in, &jn);
+                    size_t kn = (size_t)in + (size_t)c->imax * jn;
+                    for (int v = 0; v < V; v++) {
+                        qmn[v] = f_min(qmn[v], c->lmin_old[V * kn + v]);
+                        qmx[v] = f_max(qmx[v], c->lmax_old[V * kn + v]);
+                    }
+                }
+            for (int v = 0; v < V; v++) {
+                c->lmin[V * k + v] = qmn[v];
+                c->lmax[V * k + v] = qmx[v];
+            }
+        }
+}
+
+/* DMR boundary states of Euler/riemann_problem.f90:165-175 and the shock position of :1-8 */
+static void dmr_states(const orc_ctx *c, double *Qa, double *Qb, double *xsw)
+{
+    const double PI = 4.0 * atan(1.0);
+    double Va[4] = {8.0, 8.25 * cos(PI / 6.0), -8.25 * sin(PI / 6.0), 116.5};
+    double Vb[4] = {1.4, 0.0, 0.0, 1.0};
+    euler_prim2cons(c->gamma, Va, Qa);
+    euler_prim2cons(c->gamma, Vb, Qb);
+    *xsw = 1.0 / 6.0 + 1.0 / tan(PI / 3.0) + (10.0 / sin(PI / 3.0)) * c->time;
+}
+
+/* solve_riemann_problem: Euler/riemann_problem.f90:10-116 (DMR boundaries), or the
+ * periodic / transmissive boundary states (interior faces identical) */
+static void riemann(orc_ctx *c)
+{
+    const int n = c->n, V = c->V, I = c->imax, J = c->jmax;
+    double QL[4], QR[4], FL[4], FR[4], Qa[4], Qb[4], Fbc[8], xsw;
+    dmr_states(c, Qa, Qb, &xsw);
+    double nv[2] = {1.0, 0.0};
+    for (int j = 0; j < J; j++)
+        for (int i = 0; i <= I; i++)
+            for (int l = 0; l < n; l++) {
+                if (i == 0) {
+                    const double *qr = &BND(c, c->qbnd, 0, j)[V * (0 + 4 * l)];
+                    const double *fr = &BND(c, c->Fbnd, 0, j)[V * (0 + 4 * l)];
+                    if (c->bc_mode == ORC_BC_DMR) {
+                        euler_flux(c->gamma, Qa, Fbc);
+                        for (int v = 0; v < V; v++) {
+                            QL[v] = Qa[v];
+                            FL[v] = 0.0 + Fbc[v] * nv[0] + Fbc[v + V] * nv[1];
+                        }
+                    } else if (c->bc_mode == ORC_BC_PERIODIC) {
+                        for (int v = 0; v < V; v++) {
+                            QL[v] = BND(c, c->qbnd, I - 1, j)[v + V * (1 + 4 * l)];
+                            FL[v] = BND(c, c->Fbnd, I - 1, j)[v + V * (1 + 4 * l)];
+                        }
+                    } else {
+                        for (int v = 0; v < V; v++) {
+                            QL[v] = qr[v];
+                            FL[v] = fr[v];
+                        }
+                    }
+                    for (int v = 0; v < V; v++) {
+                        QR[v] = qr[v];
+                        FR[v] = fr[v];
+                    }
+                } else if (i == I) {
+                    for (int v = 0; v < V; v++) {
+                        QL[v] = BND(c, c->qbnd, I - 1, j)[v + V * (1 + 4 * l)];
+                        FL[v] = BND(c, c->Fbnd, I - 1, j)[v + V * (1 + 4 * l)];
+                    }
+                    if (c->bc_mode == ORC_BC_PERIODIC) {
+                        for (int v = 0; v < V; v++) {
+                            QR[v] = BND(c, c->qbnd, 0, j)[v + V * (0 + 4 * l)];
+                            FR[v] = BND(c, c->Fbnd, 0, j)[v + V * (0 + 4 * l)];
+                        }
+                    } else {
+                        for (int v = 0; v < V; v++) {
+                            QR[v] = QL[v];
+                            FR[v] = FL[v];
+                        }
+                    }
+                } else {
+                    for (int v = 0; v < V; v++) {
+                        QL[v] = BND(c, c->qbnd, i - 1, j)[v + V * (1 + 4 * l)];
+                        FL[v] = BND(c, c->Fbnd, i - 1, j)[v + V * (1 + 4 * l)];
+                        QR[v] = BND(c, c->qbnd, i, j)[v + V * (0 + 4 * l)];
+                        FR[v] = BND(c, c->Fbnd, i, j)[v + V * (0 + 4 * l)];
+                    }
+                }
+                euler_rusanov(c->gamma, QL, FL, QR, FR, nv, &c->Fu[V * (l + n * (i + (I + 1) * j))]);
+            }
+    nv[0] = 0.0;
+    nv[1] = 1.0;
+    for (int j = 0; j <= J; j++)
+        for (int i = 0; i < I; i++)
+            for (int k = 0; k < n; k++) {
+                if (j == 0) {
+                    const double *qr = &BND(c, c->qbnd, i, 0)[V * (2 + 4 * k)];
+                    const double *fr = &BND(c, c->Fbnd, i, 0)[V * (2 + 4 * k)];
+                    if (c->bc_mode == ORC_BC_PERIODIC) {
+                        for (int v = 0; v < V; v++) {
+                            QL[v] = BND(c, c->qbnd, i, J - 1)[v + V * (3 + 4 * k)];
+                            FL[v] = BND(c, c->Fbnd, i, J - 1)[v + V * (3 + 4 * k)];
+                        }
+                    } else {
+                        for (int v = 0; v < V; v++) {
+                            QL[v] = qr[v];
+                            FL[v] = fr[v];
+                        }
+                        if (c->bc_mode == ORC_BC_DMR && !((double)(i + 1) * c->dx[0] <= 1.0 / 6.0)) {
+                            QL[2] = -QL[2];
+                            FL[0] = -FL[0];
+                            FL[1] = -FL[1];
+                            FL[3] = -FL[3];
+                        }
+                    }
+                    for (int v = 0; v < V; v++) {
+                        QR[v] = qr[v];
+                        FR[v] = fr[v];
+                    }
+                } else if (j == J) {
+                    for (int v = 0; v < V; v++) {
+                        QL[v] = BND(c, c->qbnd, i, J - 1)[v + V * (3 + 4 * k)];
+                        FL[v] = BND(c, c->Fbnd, i, J - 1)[v + V * (3 + 4 * k)];
+                    }
+                    if (c->bc_mode == ORC_BC_DMR) {
+                        const double *Qbc = ((double)(i + 1) * c->dx[0] <= xsw) ? Qa : Qb;
+                        euler_flux(c->gamma, Qbc, Fbc);
+                        for (int v = 0; v < V; v++) {
+                            QR[v] = Qbc[v];
+                            FR[v] = 0.0 + Fbc[v] * nv[0] + Fbc[v + V] * nv[1];
+                        }
+                    } else if (c->bc_mode == ORC_BC_PERIODIC) {
+                        for (int v = 0; v < V; v++) {
+                            QR[v] = BND(c, c->qbnd, i, 0)[v + V * (2 + 4 * k)];
+                            FR[v] = BND(c, c->Fbnd, i, 0)[v + V * (2 + 4 * k)];
+                        }
+                    } else {
+                        for (int v = 0; v < V; v++) {
+                            QR[v] = QL[v];
+                            FR[v] = FL[v];
+                        }
+                    }
+                } else {
+                    for (int v = 0; v < V; v++) {
+                        QL[v] = BND(c, c->qbnd, i, j - 1)[v + V * (3 + 4 * k)];
+                        FL[v] = BND(c, c->Fbnd, i, j - 1)[v + V * (3 + 4 * k)];
+                        QR[v] = BND(c, c->qbnd, i, j)[v + V * (2 + 4 * k)];
+                        FR[v] = BND(c, c->Fbnd, i, j)[v + V * (2 + 4 * k)];
+                    }
+                }
+                euler_rusanov(c->gamma, QL, FL, QR, FR, nv, &c->Gu[V * (k + n * (i + I * j))]);
+            }
+}
+
+void orc_get_face_fluxes(const orc_ctx *c, double *fu, double *gu)
+{
+    memcpy(fu, c->Fu, sizeof(double) * c->V * c->n * (c->imax + 1) * c->jmax);
+    memcpy(gu, c->Gu, sizeof(double) * c->V * c->n * c->imax * (c->jmax + 1));
+}
+
+static void minmod(double *out, const double *a, const double *b, int V)
+{
+    for (int v = 0; v < V; v++) {
+        if (a[v] * b[v] <= 0.0)
+            out[v] = 0.0;
+        else
+            out[v] = fabs(a[v]) < fabs(b[v]) ? a[v] : b[v];
+    }
+}
+
+/* RusanovFluxS: Euler/limiter.f90:361-378 */
+static void rusanov_s(double g, double *flux, const double *QL, const double *QR, const double *nv)
+{
+    double LL[4], LR[4], FL[8], FR[8];
+    euler_eigen(g, QL, nv, LL);
+    euler_eigen(g, QR, nv, LR);
+    double smax = f_max(maxabs(LL, 4), maxabs(LR, 4));
+    euler_flux(g, QL, FL);
+    euler_flux(g, QR, FR);
+    for (int v = 0; v < 4; v++) {
+        double m = 0.0;
+        m = m + (FL[v] + FR[v]) * nv[0];
+        m = m + (FL[v + 4] + FR[v + 4]) * nv[1];
+        flux[v] = 0.5 * m - 0.5 * smax * (QR[v] - QL[v]);
+    }
+    OPS(4 * 11);
+}
+
+/* subcell_recompute: Euler/limiter.f90:411-647 */
+static void subcell_recompute(orc_ctx *c)
+{
+    const int V = c->V, s = c->s, I = c->imax, J = c->jmax;
+    const int W = 3 * s;             /* subuh0 index range 1-s .. 2s -> 0 .. 3s-1 */
+    const int H = s + 2;             /* slopes/w range 0 .. s+1 (dn = 1) */
+    double Qa[4], Qb[4], xsw;
+    dmr_states(c, Qa, Qb, &xsw);
+    double ldx[2] = {c->dx[0] / (double)s, c->dx[1] / (double)s};
+    const double nvx[2] = {1.0, 0.0}, nvy[2] = {0.0, 1.0};
+    double *subuh0 = (double *)malloc(sizeof(double) * V * W * W);
+    double *slx = (double *)calloc((size_t)V * H * H, sizeof(double));
+    double *sly = (double *)calloc((size_t)V * H * H, sizeof(double));
+    double *wLx = (double *)calloc((size_t)V * H * H, sizeof(double));
+    double *wRx = (double *)calloc((size_t)V * H * H, sizeof(double));
+    double *wLy = (double *)calloc((size_t)V * H * H, sizeof(double));
+    double *wRy = (double *)calloc((size_t)V * H * H, sizeof(double));
+    double *Fx = (double *)calloc((size_t)V * (s + 1) * s, sizeof(double));
+    double *Fy = (double *)calloc((size_t)V * s * (s + 1), sizeof(double));
+    double lim[VMAX * SMAX * SMAX], sub1[VMAX * SMAX * SMAX];
+#define S0(v, ii, jj) subuh0[(v) + V * (((ii) + s - 1) + W * ((jj) + s - 1))] /* ii,jj in 1-s..2s */
+#define HW(a, v, ii, jj) (a)[(v) + V * ((ii) + H * (jj))]                     /* ii,jj in 0..s+1 */
+    for (int cj = 0; cj < J; cj++)
+        for (int ci = 0; ci < I; ci++) {
+            size_t k = (size_t)ci + (size_t)I * cj;
+            if (c->recompute[k] <= 0) continue;
+            for (int jj = -1; jj <= 1; jj++)
+                for (int ii = -1; ii <= 1; ii++) {
+                    int in, jn;
+                    neighbor(c, ci, cj, ii, jj, &in, &jn);
+                    size_t kn = (size_t)in + (size_t)I * jn;
+                    const double *src;
+                    if (c->oldstatus[kn] == 0) {
+                        orc_subcell_data(c, lim, UH(c, c->olduh, in, jn));
+                        src = lim;
+                    } else {
+                        src = &c->Lh[(size_t)V * s * s * kn];
+                    }
+                    for (int q = 1; q <= s; q++)
+                        for (int p = 1; p <= s; p++)
+                            for (int v = 0; v < V; v++) S0(v, p + ii * s, q + jj * s) = src[v + V * ((p - 1) + s * (q - 1))];
+                }
+            if (c->bc_mode == ORC_BC_DMR) { /* :491-522 */
+                if (ci == 0)
+                    for (int jj = 1 - s; jj <= 2 * s; jj++)
+                        for (int ii = 1 - s; ii <= 0; ii++)
+                            for (int v = 0; v < V; v++) S0(v, ii, jj) = Qa[v];
+                if (cj == 0 && (double)(ci + 1) * c->dx[0] > 1.0 / 6.0)
+                    for (int jj = 1 - s; jj <= 0; jj++)
+                        for (int ii = 1 - s; ii <= 2 * s; ii++) S0(2, ii, jj) = -S0(2, ii, jj);
+                if (cj == J - 1) {
+                    const double *Qbc = ((double)(ci + 1) * c->dx[0] <= xsw) ? Qa : Qb;
+                    for (int jj = 1 + s; jj <= 2 * s; jj++)
+                        for (int ii = 1 - s; ii <= 2 * s; ii++)
+                            for (int v = 0; v < V; v++) S0(v, ii, jj) = Qbc[v];
+                }
+            }
+            /* :529-536 minmod slopes on the interior subcells only */
+            for (int jj = 1; jj <= s; jj++)
+                for (int ii = 1; ii <= s; ii++) {
+                    double a[VMAX], b[VMAX];
+                    for (int v = 0; v < V; v++) {
+                        a[v] = S0(v, ii + 1, jj) - S0(v, ii, jj);
+                        b[v] = S0(v, ii, jj) - S0(v, ii - 1, jj);
+                    }
+                    minmod(&HW(slx, 0, ii, jj), a, b, V);
+                    for (int v = 0; v < V; v++) {
+                        a[v] = S0(v, ii, jj + 1) - S0(v, ii, jj);
+                        b[v] = S0(v, ii, jj) - S0(v, ii, jj - 1);
+                    }
+                    minmod(&HW(sly, 0, ii, jj), a, b, V);
+                    OPS(4 * V);
+                }
+            /* :540-569 half-step prediction on interior plus one halo layer */
+            for (int jj = 0; jj <= s + 1; jj++)
+                for (int ii = 0; ii <= s + 1; ii++) {
+                    double FLx[8], FRx[8], FLy[8], FRy[8], Qt[VMAX];
+                    for (int v = 0; v < V; v++) {
+                        HW(wLx, v, ii, jj) = S0(v, ii, jj) - 0.5 * HW(slx, v, ii, jj);
+                        HW(wRx, v, ii, jj) = S0(v, ii, jj) + 0.5 * HW(slx, v, ii, jj);
+                    }
+                    euler_flux(c->gamma, &HW(wLx, 0, ii, jj), FLx);
+                    euler_flux(c->gamma, &HW(wRx, 0, ii, jj), FRx);
+                    for (int v = 0; v < V; v++) Qt[v] = -(FRx[v] - FLx[v]) / ldx[0];
+                    for (int v = 0; v < V; v++) {
+                        HW(wLy, v, ii, jj) = S0(v, ii, jj) - 0.5 * HW(sly, v, ii, jj);
+                        HW(wRy, v, ii, jj) = S0(v, ii, jj) + 0.5 * HW(sly, v, ii, jj);
+                    }
+                    euler_flux(c->gamma, &HW(wLy, 0, ii, jj), FLy);
+                    euler_flux(c->gamma, &HW(wRy, 0, ii, jj), FRy);
+                    for (int v = 0; v < V; v++) Qt[v] = Qt[v] - (FRy[v + 4] - FLy[v + 4]) / ldx[1];
+                    for (int v = 0; v < V; v++) {
+                        HW(wLx, v, ii, jj) = HW(wLx, v, ii, jj) + 0.5 * c->dt * Qt[v];
+                        HW(wRx, v, ii, jj) = HW(wRx, v, ii, jj) + 0.5 * c->dt * Qt[v];
+                        HW(wLy, v, ii, jj) = HW(wLy, v, ii, jj) + 0.5 * c->dt * Qt[v];
+                        HW(wRy, v, ii, jj) = HW(wRy, v, ii, jj) + 0.5 * c->dt * Qt[v];
+                    }
+                    OPS(V * (4 + 5 + 12));
+                }
+            /* :571-588 subcell Rusanov fluxes */
+            for (int jj = 1; jj <= s; jj++)
+                for (int ii = 1; ii <= s + 1; ii++)
+                    rusanov_s(c->gamma, &Fx[V * ((ii - 1) + (s + 1) * (jj - 1))], &HW(wRx, 0, ii - 1, jj),
+                              &HW(wLx, 0, ii, jj), nvx);
+            for (int jj = 1; jj <= s + 1; jj++)
+                for (int ii = 1; ii <= s; ii++)
+                    rusanov_s(c->gamma, &Fy[V * ((ii - 1) + s * (jj - 1))], &HW(wRy, 0, ii, jj - 1),
+                              &HW(wLy, 0, ii, jj), nvy);
+            /* :592-597 finite-volume update */
+            for (int jj = 1; jj <= s; jj++)
+                for (int ii = 1; ii <= s; ii++)
+                    for (int v = 0; v < V; v++)
+                        sub1[v + V * ((ii - 1) + s * (jj - 1))] =
+                            S0(v, ii, jj) -
+                            c->dt / ldx[0] * (Fx[v + V * (ii + (s + 1) * (jj - 1))] - Fx[v + V * ((ii - 1) + (s + 1) * (jj - 1))]) -
+                            c->dt / ldx[1] * (Fy[v + V * ((ii - 1) + s * jj)] - Fy[v + V * ((ii - 1) + s * (jj - 1))]);
+            OPS(V * s * s * 8);
+            double *u = UH(c, c->uh, ci, cj);
+            double *nl = &c->NewLh[(size_t)V * s * s * k];
+            if (c->recompute[k] == 1) { /* :599-611 */
+                c->status[k] = 1;
+                memcpy(nl, sub1, sizeof(double) * V * s * s);
+                orc_put_subcell_data(c, u, nl);
+            } else if (c->recompute[k] == 2) { /* :613-638 */
+                orc_put_subcell_data(c, u, sub1);
+                if (orc_dmp(c, u, &c->lmin[V * k], &c->lmax[V * k], 0.0)) {
+                    c->status[k] = 0;
+                } else {
+                    c->status[k] = 1;
+                    memcpy(nl, sub1, sizeof(double) * V * s * s);
+                }
+            }
+        }
+#undef S0
+#undef HW
+    free(subuh0); free(slx); free(sly); free(wLx); free(wRx); free(wLy); free(wRy); free(Fx); free(Fy);
+}
+
+/* one pass of the time loop body, Euler/main.f90:28-100 */
+int orc_step(orc_ctx *c)
+{
+    const int V = c->V, n = c->n, s = c->s, I = c->imax, J = c->jmax;
+    size_t nc = ncell(c), blk = (size_t)V * n * n;
+    int nrec = 0;
+    orc_calc_dt(c);
+    get_min_max(c);
+    memset(c->recompute, 0, nc * sizeof(int));
+    memcpy(c->olduh, c->uh, blk * nc * sizeof(double));
+    for (int cj = 0; cj < J; cj++)
+        for (int ci = 0; ci < I; ci++) {
+            size_t k = (size_t)ci + (size_t)I * cj;
+            orc_predictor(c, &c->qhi[blk * k], &c->Fhi[2 * blk * k], BND(c, c->qbnd, ci, cj), BND(c, c->Fbnd, ci, cj),
+                          UH(c, c->uh, ci, cj));
+        }
+    for (size_t k = 0; k < nc; k++) orc_volume(c, &c->duh[blk * k], &c->Fhi[2 * blk * k]);
+    riemann(c);
+    for (int cj = 0; cj < J; cj++)
+        for (int ci = 0; ci < I; ci++) {
+            size_t k = (size_t)ci + (size_t)I * cj;
+            orc_surface(c, &c->duh[blk * k], &c->Fu[V * n * (ci + (I + 1) * cj)], &c->Fu[V * n * (ci + 1 + (I + 1) * cj)],
+                        &c->Gu[V * n * (ci + I * cj)], &c->Gu[V * n * (ci + I * (cj + 1))]);
+        }
+    for (size_t k = 0; k < nc; k++) {
+        orc_update(c, &c->uh[blk * k], &c->duh[blk * k]);
+        if (!orc_dmp(c, &c->uh[blk * k], &c->lmin[V * k], &c->lmax[V * k], 0.0)) {
+            c->recompute[k] = 1;
+            nrec++;
+        }
+    }
+    /* spread_recompute: Euler/limiter.f90:329-354 */
+    static const int f2n[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+    for (int cj = 0; cj < J; cj++)
+        for (int ci = 0; ci < I; ci++) {
+            if (c->recompute[(size_t)ci + (size_t)I * cj] != 1) continue;
+            for (int f = 0; f < 4; f++) {
+                int in, jn;
+                neighbor(c, ci, cj, f2n[f][0], f2n[f][1], &in, &jn);
+                size_t kn = (size_t)in + (size_t)I * jn;
+                if (c->recompute[kn] == 0) c->recompute[kn] = 2;
+            }
+        }
+    /* allocate_limiter: Euler/limiter.f90:302-322 */
+    for (size_t k = 0; k < nc; k++) c->status[k] = c->recompute[k] == 1 ? 1 : 0;
+    subcell_recompute(c);
+    /* update_limiter: Euler/limiter.f90:272-296 */
+    for (size_t k = 0; k < nc; k++) {
+        if (c->status[k] != 0) memcpy(&c->Lh[(size_t)V * s * s * k], &c->NewLh[(size_t)V * s * s * k], sizeof(double) * V * s * s);
+        c->oldstatus[k] = c->status[k];
+    }
+    c->time = c->time + c->dt;
+    return nrec;
+}

@@ -1,0 +1,78 @@
+#!/bin/bash
+# Builds the reference Fortran solver (read from its own sources under the
+# reference checkout, never copied into this repository) into oracle/_ref/:
+#
+#   oracle/_ref/libref_<pde>_n<N>.so   reference routines + the ref_harness_<pde>.f90
+#                                      bind(C) harness (golden vectors, CPU baseline)
+#   oracle/_ref/aderdg_<pde>_n<N>      the reference's own program (main.f90)
+#
+# TEST INFRASTRUCTURE ONLY: the product never links or loads these files.
+#
+# N is a compile-time parameter of the reference (Euler/ader_dg.f90:17), so one
+# build is made per degree.  The sources are copied to a scratch directory under
+# /tmp, the `N = ...` parameter line of ader_dg.f90 is rewritten there, and
+# everything is compiled with the reference Makefile's flags
+# (Euler/Makefile:1-2: -O3 -fdefault-real-8) using LLVM flang, minus the
+# -Wall/-Wextra that flang ignores.  No -march: x86-64 baseline, no FMA.
+set -euo pipefail
+REF=${ADERDG_REFERENCE:-/root/reference}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT="$HERE/_ref"
+FC=${FC:-flang}
+FFLAGS="-O3 -fdefault-real-8 -fPIC"
+if [ ! -d "$REF" ]; then
+    echo "build_ref.sh: no reference checkout at $REF; nothing built" >&2
+    exit 0
+fi
+mkdir -p "$OUT"
+WORK=$(mktemp -d /tmp/aderdg_ref.XXXXXX)
+trap 'rm -rf "$WORK"' EXIT
+
+# build_variant <pde tag> <reference dir> <N> <harness file> [sed expressions for memory-safety fixes]
+build_variant() {
+    local tag=$1 dir=$2 n=$3 harness=$4
+    shift 4
+    local w="$WORK/${tag}_n${n}"
+    mkdir -p "$w"
+    cp "$REF/$dir"/*.f90 "$w/"
+    sed -i -E "s/^( *integer, parameter *:: *N *= *)[0-9]+/\1${n}/" "$w/ader_dg.f90"
+    grep -q -E "^ *integer, parameter *:: *N *= *${n}( |$)" "$w/ader_dg.f90"
+    for fix in "$@"; do
+        local file=${fix%%|*} expr=${fix#*|}
+        cp "$w/$file" "$w/$file.orig"
+        sed -i -E "$expr" "$w/$file"
+        if cmp -s "$w/$file" "$w/$file.orig"; then
+            echo "build_ref.sh: patch did not apply: $expr" >&2
+            exit 1
+        fi
+        rm "$w/$file.orig"
+    done
+    local srcs=""
+    for f in "$w"/*.f90; do
+        case $(basename "$f") in ader_dg.f90|main.f90) ;; *) srcs="$srcs $f";; esac
+    done
+    (cd "$w" && $FC $FFLAGS -c ader_dg.f90 && \
+        for f in $srcs; do $FC $FFLAGS -c "$f"; done && \
+        $FC $FFLAGS -c "$HERE/$harness" && \
+        $FC $FFLAGS -c main.f90)
+    local objs=""
+    for f in $srcs; do objs="$objs $w/$(basename "${f%.f90}").o"; done
+    local hobj="$w/$(basename "${harness%.f90}").o"
+    $FC $FFLAGS -shared -o "$OUT/libref_${tag}_n${n}.so" "$w/ader_dg.o" $objs "$hobj" \
+        -Wl,-Bsymbolic -Wl,--exclude-libs,ALL
+    $FC $FFLAGS -o "$OUT/aderdg_${tag}_n${n}" "$w/ader_dg.o" $objs "$w/main.o"
+    echo "built $OUT/libref_${tag}_n${n}.so"
+}
+
+# The Euler subcell recompute applies Double-Mach-Reflection ghost states on the
+# left, bottom and top boundaries unconditionally (Euler/limiter.f90:491-522).
+# Guarding them with ICType == 11 (DMR) leaves DMR runs unchanged and lets the
+# harness drive the other boundary modes through the same routine.
+EULER_DMR_GUARD=(
+  'limiter.f90|s/^( *)if \(i \.eq\. 1\) then ! Left/\1if (i .eq. 1 .and. ICType .eq. 11) then ! Left/'
+  'limiter.f90|s/^( *)if \(j \.eq\. 1\) then ! Bottom/\1if (j .eq. 1 .and. ICType .eq. 11) then ! Bottom/'
+  'limiter.f90|s/^( *)if \(j \.eq\. JMAX\) then ! Top/\1if (j .eq. JMAX .and. ICType .eq. 11) then ! Top/'
+)
+for n in 1 2 3 4; do
+    build_variant euler Euler "$n" ref_harness_euler.f90 "${EULER_DMR_GUARD[@]}"
+done

@@ -1,0 +1,412 @@
+"""ctypes front-ends of the oracle libraries.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product.
+
+* ``Oracle``  -- oracle/_build/liboracle.so, the C restatement (aderdg_oracle.c)
+* ``RefEuler`` -- oracle/_ref/libref_euler_n<N>.so, the reference Fortran routines
+  compiled from their own sources plus ref_harness_euler.f90.
+
+Arrays are numpy float64 in Fortran order with the reference's shapes
+(Euler/ader_dg.f90:85-93).
+"""
+import ctypes
+import os
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BC_DMR, BC_PERIODIC, BC_TRANSMISSIVE = 0, 1, 2
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _p(a):
+    assert a.dtype == np.float64 and (a.flags.f_contiguous or a.flags.c_contiguous)
+    return a.ctypes.data_as(_dp)
+
+
+def _ip_(a):
+    assert a.dtype == np.int32
+    return a.ctypes.data_as(_ip)
+
+
+def oracle_path():
+    return os.path.join(HERE, "_build", "liboracle.so")
+
+
+def ref_path(pde, N):
+    return os.path.join(HERE, "_ref", "libref_%s_n%d.so" % (pde, N))
+
+
+class Oracle:
+    """The C restatement of the reference element step."""
+
+    _lib = None
+
+    @classmethod
+    def lib(cls):
+        if cls._lib is None:
+            lib = ctypes.CDLL(oracle_path())
+            lib.orc_create.restype = ctypes.c_void_p
+            lib.orc_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                       ctypes.c_int, _dp]
+            lib.orc_destroy.argtypes = [ctypes.c_void_p]
+            lib.orc_calc_dt.restype = ctypes.c_double
+            lib.orc_calc_dt.argtypes = [ctypes.c_void_p]
+            lib.orc_step.restype = ctypes.c_int
+            lib.orc_step.argtypes = [ctypes.c_void_p]
+            lib.orc_dmp.restype = ctypes.c_int
+            lib.orc_dmp.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double]
+            lib.orc_dmp_margin.restype = ctypes.c_double
+            lib.orc_dmp_margin.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double]
+            lib.orc_set_scalars.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+            lib.orc_get_opcount.restype = ctypes.c_double
+            for name in ("orc_set_uh", "orc_get_uh", "orc_get_scalars", "orc_set_limiter", "orc_get_limiter",
+                         "orc_predictor", "orc_volume", "orc_surface", "orc_update", "orc_flux", "orc_eigen",
+                         "orc_rusanov", "orc_subcell_data", "orc_lobatto_data", "orc_put_subcell_data",
+                         "orc_get_ops", "orc_get_face_fluxes"):
+                getattr(lib, name).restype = None
+            cls._lib = lib
+        return cls._lib
+
+    def __init__(self, N, imax, jmax, xl, xr, bc_mode, gamma=1.4, pde=0):
+        self.N, self.n, self.s, self.V = N, N + 1, 2 * N + 1, 4
+        self.imax, self.jmax = imax, jmax
+        L = self.lib()
+        xl = np.asarray(xl, dtype=np.float64)
+        xr = np.asarray(xr, dtype=np.float64)
+        eos = np.array([gamma], dtype=np.float64)
+        self.h = L.orc_create(pde, N, imax, jmax, _p(xl), _p(xr), bc_mode, _p(eos))
+        if not self.h:
+            raise ValueError("orc_create failed")
+
+    def __del__(self):
+        if getattr(self, "h", None) and self._lib is not None:
+            self._lib.orc_destroy(self.h)
+            self.h = None
+
+    # shapes
+    def uh_shape(self):
+        return (self.V, self.n, self.n, self.imax, self.jmax)
+
+    def ops(self):
+        n, s = self.n, self.s
+        o = dict(xi=np.zeros(n), w=np.zeros(n), MM=np.zeros((n, n), order="F"), Kxi=np.zeros((n, n), order="F"),
+                 dudx=np.zeros((n, n), order="F"), iK1=np.zeros((n, n), order="F"), F0=np.zeros(n),
+                 FLcoeff=np.zeros(n), FRcoeff=np.zeros(n), uh2lim=np.zeros((s, n), order="F"),
+                 lim2uh=np.zeros((n, s), order="F"), uh2lob=np.zeros((n, n), order="F"), xiLob=np.zeros(n),
+                 wLob=np.zeros(n), K1=np.zeros((n, n), order="F"))
+        self.lib().orc_get_ops(ctypes.c_void_p(self.h), *[_p(o[k]) for k in
+                               ("xi", "w", "MM", "Kxi", "dudx", "iK1", "F0", "FLcoeff", "FRcoeff", "uh2lim",
+                                "lim2uh", "uh2lob", "xiLob", "wLob", "K1")])
+        return o
+
+    def set_scalars(self, dt, time, tend):
+        self.lib().orc_set_scalars(self.h, dt, time, tend)
+
+    def scalars(self):
+        a = (ctypes.c_double * 3)()
+        self.lib().orc_get_scalars(ctypes.c_void_p(self.h), ctypes.byref(a, 0), ctypes.byref(a, 8),
+                                   ctypes.byref(a, 16))
+        return a[0], a[1], a[2]
+
+    def set_uh(self, uh):
+        uh = np.asfortranarray(uh, dtype=np.float64)
+        assert uh.shape == self.uh_shape()
+        self.lib().orc_set_uh(ctypes.c_void_p(self.h), _p(uh))
+
+    def get_uh(self):
+        uh = np.zeros(self.uh_shape(), order="F")
+        self.lib().orc_get_uh(ctypes.c_void_p(self.h), _p(uh))
+        return uh
+
+    def set_limiter(self, status, lh):
+        st = np.asfortranarray(status, dtype=np.int32)
+        lh = np.asfortranarray(lh, dtype=np.float64)
+        self.lib().orc_set_limiter(ctypes.c_void_p(self.h), _ip_(st), _p(lh))
+
+    def get_limiter(self):
+        I, J, V, s = self.imax, self.jmax, self.V, self.s
+        st = np.zeros((I, J), dtype=np.int32, order="F")
+        rc = np.zeros((I, J), dtype=np.int32, order="F")
+        lh = np.zeros((V, s, s, I, J), order="F")
+        mn = np.zeros((V, I, J), order="F")
+        mx = np.zeros((V, I, J), order="F")
+        self.lib().orc_get_limiter(ctypes.c_void_p(self.h), _ip_(st), _ip_(rc), _p(lh), _p(mn), _p(mx))
+        return dict(status=st, recompute=rc, Lh=lh, lmin=mn, lmax=mx)
+
+    def calc_dt(self):
+        return self.lib().orc_calc_dt(self.h)
+
+    def step(self):
+        return self.lib().orc_step(self.h)
+
+    def face_fluxes(self):
+        V, n, I, J = self.V, self.n, self.imax, self.jmax
+        fu = np.zeros((V, n, I + 1, J), order="F")
+        gu = np.zeros((V, n, I, J + 1), order="F")
+        self.lib().orc_get_face_fluxes(ctypes.c_void_p(self.h), _p(fu), _p(gu))
+        return fu, gu
+
+    # per-cell
+    def predictor(self, luh):
+        V, n = self.V, self.n
+        luh = np.asfortranarray(luh, dtype=np.float64)
+        out = dict(qhi=np.zeros((V, n, n), order="F"), Fhi=np.zeros((V, 2, n, n), order="F"),
+                   qbnd=np.zeros((V, 4, n), order="F"), Fbnd=np.zeros((V, 4, n), order="F"))
+        self.lib().orc_predictor(ctypes.c_void_p(self.h), _p(out["qhi"]), _p(out["Fhi"]), _p(out["qbnd"]),
+                                 _p(out["Fbnd"]), _p(luh))
+        return out
+
+    def volume(self, Fhi):
+        V, n = self.V, self.n
+        d = np.zeros((V, n, n), order="F")
+        self.lib().orc_volume(ctypes.c_void_p(self.h), _p(d), _p(np.asfortranarray(Fhi)))
+        return d
+
+    def surface(self, lduh, fl, fr, gb, gt):
+        d = np.array(lduh, order="F", dtype=np.float64)
+        args = [np.asfortranarray(a, dtype=np.float64) for a in (fl, fr, gb, gt)]
+        self.lib().orc_surface(ctypes.c_void_p(self.h), _p(d), *[_p(a) for a in args])
+        return d
+
+    def update(self, luh, lduh):
+        u = np.array(luh, order="F", dtype=np.float64)
+        d = np.array(lduh, order="F", dtype=np.float64)
+        self.lib().orc_update(ctypes.c_void_p(self.h), _p(u), _p(d))
+        return u
+
+    def flux(self, q):
+        f = np.zeros((self.V, 2), order="F")
+        self.lib().orc_flux(ctypes.c_void_p(self.h), _p(np.ascontiguousarray(q, dtype=np.float64)), _p(f))
+        return f
+
+    def eigen(self, q, nv):
+        lam = np.zeros(self.V)
+        self.lib().orc_eigen(ctypes.c_void_p(self.h), _p(np.ascontiguousarray(q, dtype=np.float64)),
+                             _p(np.ascontiguousarray(nv, dtype=np.float64)), _p(lam))
+        return lam
+
+    def rusanov(self, ql, fl, qr, fr, nv):
+        out = np.zeros(self.V)
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (ql, fl, qr, fr, nv)]
+        self.lib().orc_rusanov(ctypes.c_void_p(self.h), *[_p(x) for x in a], _p(out))
+        return out
+
+    def subcell_data(self, luh):
+        lim = np.zeros((self.V, self.s, self.s), order="F")
+        self.lib().orc_subcell_data(ctypes.c_void_p(self.h), _p(lim), _p(np.asfortranarray(luh)))
+        return lim
+
+    def lobatto_data(self, luh):
+        lob = np.zeros((self.V, self.n, self.n), order="F")
+        self.lib().orc_lobatto_data(ctypes.c_void_p(self.h), _p(lob), _p(np.asfortranarray(luh)))
+        return lob
+
+    def put_subcell_data(self, lim):
+        luh = np.zeros((self.V, self.n, self.n), order="F")
+        self.lib().orc_put_subcell_data(ctypes.c_void_p(self.h), _p(luh), _p(np.asfortranarray(lim)))
+        return luh
+
+    def dmp(self, luh, lmin, lmax, argmax):
+        a = [np.asfortranarray(x, dtype=np.float64) for x in (luh, lmin, lmax)]
+        return bool(self.lib().orc_dmp(self.h, *[_p(x) for x in a], argmax))
+
+    def dmp_margin(self, luh, lmin, lmax, argmax):
+        a = [np.asfortranarray(x, dtype=np.float64) for x in (luh, lmin, lmax)]
+        return self.lib().orc_dmp_margin(self.h, *[_p(x) for x in a], argmax)
+
+    @classmethod
+    def reset_opcount(cls):
+        cls.lib().orc_reset_opcount()
+
+    @classmethod
+    def opcount(cls):
+        return cls.lib().orc_get_opcount()
+
+
+PARAMS_TEMPLATE = """{gamma}   ! GAMMA
+{imax}   ! IMAX
+{jmax}   ! JMAX
+{xl0}   ! xL(1)
+{xl1}   ! xL(2)
+{xr0}   ! xR(1)
+{xr1}   ! xR(2)
+{bl}   ! bL
+{br}   ! bR
+{bb}   ! bB
+{bt}   ! bT
+{ic}   ! IC
+{tend}   ! tend
+{wi}   ! WriteInterval
+"""
+
+
+class RefEuler:
+    """The reference Euler routines compiled from their own sources (oracle/_ref).
+
+    Only one instance per degree N may exist at a time: the reference keeps all
+    of its state in `module ader_dg` (Euler/ader_dg.f90:8-103).
+    """
+
+    _libs = {}
+
+    @classmethod
+    def available(cls, N):
+        return os.path.exists(ref_path("euler", N))
+
+    @classmethod
+    def lib(cls, N):
+        if N not in cls._libs:
+            lib = ctypes.CDLL(ref_path("euler", N))
+            lib.ref_set_scalars.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_int]
+            lib.ref_set_mesh_bc.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4 + [ctypes.c_int] * 4
+            lib.ref_dmp.argtypes = [ctypes.POINTER(ctypes.c_int), _dp, _dp, _dp, ctypes.c_double]
+            cls._libs[N] = lib
+        return cls._libs[N]
+
+    def __init__(self, N, imax, jmax, xl, xr, bc_mode, gamma=1.4, tend=1.0e9):
+        self.N, self.n, self.s, self.V = N, N + 1, 2 * N + 1, 4
+        self.imax, self.jmax = imax, jmax
+        self.bc_mode = bc_mode
+        L = self.lib(N)
+        code = 3 if bc_mode == BC_PERIODIC else 1
+        txt = PARAMS_TEMPLATE.format(gamma=repr(float(gamma)), imax=imax, jmax=jmax, xl0=repr(float(xl[0])),
+                                     xl1=repr(float(xl[1])), xr0=repr(float(xr[0])), xr1=repr(float(xr[1])),
+                                     bl=code, br=code, bb=code, bt=code, ic=11, tend=repr(float(tend)), wi=10 ** 8)
+        cwd = os.getcwd()
+        with tempfile.TemporaryDirectory() as d:
+            with open(os.path.join(d, "params.ini"), "w") as f:
+                f.write(txt)
+            os.chdir(d)
+            try:
+                L.ref_init()
+            finally:
+                os.chdir(cwd)
+        L.ref_reset_limiter()
+        L.ref_set_scalars(0.0, 0.0, float(tend), float(gamma), bc_mode)
+        self.lib_ = L
+
+    def uh_shape(self):
+        return (self.V, self.n, self.n, self.imax, self.jmax)
+
+    def ops(self):
+        n, s = self.n, self.s
+        o = dict(xi=np.zeros(n), w=np.zeros(n), MM=np.zeros((n, n), order="F"), Kxi=np.zeros((n, n), order="F"),
+                 dudx=np.zeros((n, n), order="F"), iK1=np.zeros((n, n), order="F"), F0=np.zeros(n),
+                 FLcoeff=np.zeros(n), FRcoeff=np.zeros(n), uh2lim=np.zeros((s, n), order="F"),
+                 lim2uh=np.zeros((n, s), order="F"), uh2lob=np.zeros((n, n), order="F"), xiLob=np.zeros(n),
+                 wLob=np.zeros(n), K1=np.zeros((n, n), order="F"))
+        self.lib_.ref_get_ops(*[_p(o[k]) for k in
+                                ("xi", "w", "MM", "Kxi", "dudx", "iK1", "F0", "FLcoeff", "FRcoeff", "uh2lim",
+                                 "lim2uh", "uh2lob", "xiLob", "wLob", "K1")])
+        return o
+
+    def set_scalars(self, dt, time, tend, gamma=1.4):
+        self.lib_.ref_set_scalars(dt, time, tend, gamma, self.bc_mode)
+
+    def scalars(self):
+        a = np.zeros(3)
+        self.lib_.ref_get_scalars(_p(a))
+        return tuple(a)
+
+    def set_uh(self, uh):
+        uh = np.asfortranarray(uh, dtype=np.float64)
+        assert uh.shape == self.uh_shape()
+        self.lib_.ref_set_uh(_p(uh))
+
+    def get_uh(self):
+        uh = np.zeros(self.uh_shape(), order="F")
+        self.lib_.ref_get_uh(_p(uh))
+        return uh
+
+    def set_limiter(self, status, lh):
+        self.lib_.ref_set_limiter(_ip_(np.asfortranarray(status, dtype=np.int32)),
+                                  _p(np.asfortranarray(lh, dtype=np.float64)))
+
+    def get_limiter(self):
+        I, J, V, s = self.imax, self.jmax, self.V, self.s
+        st = np.zeros((I, J), dtype=np.int32, order="F")
+        rc = np.zeros((I, J), dtype=np.int32, order="F")
+        lh = np.zeros((V, s, s, I, J), order="F")
+        mn = np.zeros((V, I, J), order="F")
+        mx = np.zeros((V, I, J), order="F")
+        self.lib_.ref_get_limiter(_ip_(st), _ip_(rc), _p(lh), _p(mn), _p(mx))
+        return dict(status=st, recompute=rc, Lh=lh, lmin=mn, lmax=mx)
+
+    def calc_dt(self):
+        d = ctypes.c_double()
+        self.lib_.ref_calc_dt(ctypes.byref(d))
+        return d.value
+
+    def step(self):
+        r = ctypes.c_int()
+        self.lib_.ref_step(ctypes.byref(r))
+        return r.value
+
+    def predictor(self, luh):
+        V, n = self.V, self.n
+        luh = np.asfortranarray(luh, dtype=np.float64)
+        out = dict(qhi=np.zeros((V, n, n), order="F"), Fhi=np.zeros((V, 2, n, n), order="F"),
+                   qbnd=np.zeros((V, 4, n), order="F"), Fbnd=np.zeros((V, 4, n), order="F"))
+        self.lib_.ref_predictor(_p(out["qhi"]), _p(out["Fhi"]), _p(out["qbnd"]), _p(out["Fbnd"]), _p(luh))
+        return out
+
+    def volume(self, Fhi):
+        d = np.zeros((self.V, self.n, self.n), order="F")
+        self.lib_.ref_volume(_p(d), _p(np.asfortranarray(Fhi)))
+        return d
+
+    def surface(self, lduh, fl, fr, gb, gt):
+        d = np.array(lduh, order="F", dtype=np.float64)
+        args = [np.asfortranarray(a, dtype=np.float64) for a in (fl, fr, gb, gt)]
+        self.lib_.ref_surface(_p(d), *[_p(a) for a in args])
+        return d
+
+    def update(self, luh, lduh):
+        u = np.array(luh, order="F", dtype=np.float64)
+        d = np.array(lduh, order="F", dtype=np.float64)
+        self.lib_.ref_update(_p(u), _p(d))
+        return u
+
+    def flux(self, q):
+        f = np.zeros((self.V, 2), order="F")
+        self.lib_.ref_flux(_p(np.ascontiguousarray(q, dtype=np.float64)), _p(f))
+        return f
+
+    def eigen(self, q, nv):
+        lam = np.zeros(self.V)
+        self.lib_.ref_eigen(_p(np.ascontiguousarray(q, dtype=np.float64)),
+                            _p(np.ascontiguousarray(nv, dtype=np.float64)), _p(lam))
+        return lam
+
+    def rusanov(self, ql, fl, qr, fr, nv):
+        out = np.zeros(self.V)
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (ql, fl, qr, fr, nv)]
+        self.lib_.ref_rusanov(*[_p(x) for x in a], _p(out))
+        return out
+
+    def subcell_data(self, luh):
+        lim = np.zeros((self.V, self.s, self.s), order="F")
+        self.lib_.ref_subcell_data(_p(lim), _p(np.asfortranarray(luh)))
+        return lim
+
+    def lobatto_data(self, luh):
+        lob = np.zeros((self.V, self.n, self.n), order="F")
+        self.lib_.ref_lobatto_data(_p(lob), _p(np.asfortranarray(luh)))
+        return lob
+
+    def put_subcell_data(self, lim):
+        luh = np.zeros((self.V, self.n, self.n), order="F")
+        self.lib_.ref_put_subcell_data(_p(luh), _p(np.asfortranarray(lim)))
+        return luh
+
+    def dmp(self, luh, lmin, lmax, argmax):
+        r = ctypes.c_int()
+        a = [np.asfortranarray(x, dtype=np.float64) for x in (luh, lmin, lmax)]
+        self.lib_.ref_dmp(ctypes.byref(r), *[_p(x) for x in a], argmax)
+        return bool(r.value)
