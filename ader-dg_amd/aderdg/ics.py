@@ -1,0 +1,109 @@
+"""Initial conditions of the workloads, as point values at the Gauss-Legendre
+nodes (the projection of Euler/initialize.f90:262-273: the nodal basis through
+the quadrature nodes makes the L2 projection a point evaluation).
+
+The node coordinates follow the reference arithmetic exactly:
+x_c = xL + (i - 1 + 0.5) dx (Euler/initialize.f90:151), corner = x_c - 0.5 dx
+(:265), x_GP = corner + xi_k dx (:268).
+
+* ``dmr``       Double Mach Reflection, the only InitialField of the reference
+                (Euler/initialize.f90:311-341)
+* ``vortex``    isentropic vortex on (rho,u,v,p) = (1,1,1,1), strength beta,
+                centre (xc, yc); not at reference HEAD
+* ``quadrant``  four-state 2-D Riemann problem split at (0.5, 0.5); not at
+                reference HEAD
+"""
+import math
+
+import numpy as np
+
+
+def prim2cons(gamma, rho, u, v, p):
+    """PDEPrim2Cons, Euler/pde.f90:29-40, elementwise."""
+    rho = np.asarray(rho, dtype=np.float64)
+    q = np.empty((4,) + rho.shape)
+    q[0] = rho
+    q[1] = rho * u
+    q[2] = rho * v
+    q[3] = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)
+    return q
+
+
+def node_coordinates(xi, imax, jmax, xl, xr):
+    """x, y of every Gauss node, shape (n, n, IMAX, JMAX) each."""
+    n = len(xi)
+    dx = (xr[0] - xl[0]) / imax
+    dy = (xr[1] - xl[1]) / jmax
+    xc = xl[0] + (np.arange(imax, dtype=np.float64) + 0.5) * dx
+    yc = xl[1] + (np.arange(jmax, dtype=np.float64) + 0.5) * dy
+    cx = xc - 0.5 * dx
+    cy = yc - 0.5 * dy
+    X = cx[None, None, :, None] + (np.asarray(xi)[:, None, None, None] * dx)
+    Y = cy[None, None, None, :] + (np.asarray(xi)[None, :, None, None] * dy)
+    X = np.broadcast_to(X, (n, n, imax, jmax))
+    Y = np.broadcast_to(Y, (n, n, imax, jmax))
+    return X, Y
+
+
+def dmr(X, Y, gamma=1.4):
+    """InitialField of Euler/initialize.f90:311-341."""
+    pi = 4.0 * math.atan(1.0)
+    a = (8.0, 8.25 * math.cos(pi / 6.0), -8.25 * math.sin(pi / 6.0), 116.5)
+    b = (1.4, 0.0, 0.0, 1.0)
+    left = X <= 1.0 / 6.0 + Y / math.tan(pi / 3.0)
+    qa = prim2cons(gamma, *[np.float64(c) for c in a])
+    qb = prim2cons(gamma, *[np.float64(c) for c in b])
+    return np.where(left[None], qa.reshape(4, 1, 1, 1, 1), qb.reshape(4, 1, 1, 1, 1))
+
+
+def vortex(X, Y, gamma=1.4, beta=5.0, xc=5.0, yc=5.0):
+    """Isentropic vortex: dT = -(g-1) b^2/(8 g pi^2) e^(1-r^2), rho = (1+dT)^(1/(g-1)),
+    (du, dv) = b/(2 pi) e^((1-r^2)/2) (-(y-yc), x-xc), p = rho^g, on (1,1,1,1)."""
+    pi = math.pi
+    dxr = X - xc
+    dyr = Y - yc
+    r2 = dxr * dxr + dyr * dyr
+    dT = -(gamma - 1.0) * beta * beta / (8.0 * gamma * pi * pi) * np.exp(1.0 - r2)
+    rho = (1.0 + dT) ** (1.0 / (gamma - 1.0))
+    f = beta / (2.0 * pi) * np.exp(0.5 * (1.0 - r2))
+    u = 1.0 - f * dyr
+    v = 1.0 + f * dxr
+    p = rho ** gamma
+    return prim2cons(gamma, rho, u, v, p)
+
+
+QUADRANT_STATES = {  # (rho, u, v, p), Lax-Liu configuration 3
+    "NE": (1.5, 0.0, 0.0, 1.5),
+    "NW": (0.5323, 1.206, 0.0, 0.3),
+    "SW": (0.138, 1.206, 1.206, 0.029),
+    "SE": (0.5323, 0.0, 1.206, 0.3),
+}
+
+
+def quadrant(X, Y, gamma=1.4, states=None, x0=0.5, y0=0.5):
+    st = states or QUADRANT_STATES
+    east = X > x0
+    north = Y > y0
+    out = np.empty((4,) + X.shape)
+    for key, mask in (("NE", east & north), ("NW", ~east & north), ("SW", ~east & ~north), ("SE", east & ~north)):
+        q = prim2cons(gamma, *[np.float64(c) for c in st[key]])
+        for v in range(4):
+            out[v][mask] = q[v]
+    return out
+
+
+def initial_uh(kind, xi, imax, jmax, xl, xr, gamma=1.4, **kw):
+    """uh(nVar, N+1, N+1, IMAX, JMAX) in Fortran order."""
+    X, Y = node_coordinates(xi, imax, jmax, xl, xr)
+    fn = {"dmr": dmr, "vortex": vortex, "quadrant": quadrant}[kind]
+    return np.asfortranarray(fn(X, Y, gamma, **kw))
+
+
+def cell_averages(uh, w):
+    """w-weighted cell average of every variable (Euler/write_data.f90:19-35), shape (nVar, IMAX, JMAX)."""
+    V, n = uh.shape[0], uh.shape[1]
+    q = np.zeros((V,) + uh.shape[3:])
+    for jj in range(n):
+        for ii in range(n):
+            q = q + w[ii] * w[jj] * uh[:, ii, jj]
+    return q

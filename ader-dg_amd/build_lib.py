@@ -1,0 +1,70 @@
+"""Build the HIP library of the ADER-DG element step for gfx950.
+
+Produces, in ader-dg_amd/lib/:
+  libaderdg_amd.so        floating-point contraction on (FMA), the product build
+  libaderdg_amd_exact.so  contraction off: bit-identical to the reference's
+                          expression order, used by the parity tests
+  aderdg_euler            the Fortran driver (fortran/aderdg_main.f90) linked
+                          against libaderdg_amd.so, when flang is available
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "lib")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("ADERDG_ARCH", "gfx950")
+
+SOURCES = ["operators.cpp", "euler_kernels.hip", "aderdg_capi.hip"]
+COMMON = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared",
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-I" + os.path.join(ROOT, "include")]
+
+
+def _run(cmd, cwd=None):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=cwd)
+
+
+def build_library(exact=False):
+    os.makedirs(LIB, exist_ok=True)
+    name = "libaderdg_amd_exact.so" if exact else "libaderdg_amd.so"
+    out = os.path.join(LIB, name)
+    flags = ["-ffp-contract=off", "-DADERDG_EXACT"] if exact else ["-ffp-contract=fast"]
+    tmp = out + ".tmp"
+    _run([HIPCC] + COMMON + flags + [os.path.join(CSRC, f) for f in SOURCES] + ["-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_fortran_driver():
+    fc = shutil.which("flang") or shutil.which("amdflang")
+    if fc is None:
+        print("flang not found: Fortran driver not built", flush=True)
+        return None
+    fdir = os.path.join(HERE, "fortran")
+    out = os.path.join(LIB, "aderdg_euler")
+    mods = os.path.join(LIB, "fmod")
+    os.makedirs(mods, exist_ok=True)
+    srcs = [os.path.join(fdir, f) for f in ("aderdg_hip.f90", "aderdg_main.f90")]
+    _run([fc, "-O2", "-fdefault-real-8", "-J", mods] + srcs +
+         ["-L" + LIB, "-laderdg_amd", "-Wl,-rpath,$ORIGIN", "-o", out])
+    return out
+
+
+def build_all():
+    build_library(exact=False)
+    build_library(exact=True)
+    if os.path.exists(os.path.join(HERE, "fortran", "aderdg_main.f90")):
+        build_fortran_driver()
+
+
+if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "fast":
+        build_library(False)
+    else:
+        build_all()

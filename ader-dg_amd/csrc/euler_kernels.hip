@@ -1,0 +1,960 @@
+// euler_kernels.hip -- CDNA4 (gfx950) kernels of the 2-D Euler ADER-DG step.
+//
+// Per time step (the order of Euler/main.f90:24-100):
+//   k_dt          calc_time_step: min-reduction of per-cell candidates
+//   k_predictor   ader_space_time_predictor + volume_integral + face
+//                 extrapolation + the 3x3 pass of get_min_max
+//   k_update      solve_riemann_problem + surface_integral + element_update
+//                 + DMP, and the next step's per-cell get_min_max pass and
+//                 time-step candidates
+//   k_spread      spread_recompute + allocate_limiter + wave-ballot
+//                 compaction of the cells to recompute
+//   k_subcell     subcell_recompute (MUSCL-Hancock on the (2N+1)^2 subgrid)
+//                 + put_subcell_data + re-DMP + update_limiter, over the
+//                 compacted list only
+//
+// Data layout in HBM (ncell = IMAX*JMAX, cell = i + IMAX*j):
+//   uh, lduh      [cell][j][i][v]   the reference's uh(V,n,n,IMAX,JMAX)
+//   fq, ff        [face][v][p][cell] face extrapolations of q and F*n,
+//                 structure of arrays so that the face pass reads
+//                 consecutive cells with consecutive lanes
+//   lmin.. etc.   [v][cell]
+//   Lh            [cell][q][p][v]   subcell averages of limited cells
+#include "euler_kernels.hpp"
+
+#include "euler_device.hpp"
+
+namespace aderdg {
+
+namespace {
+
+template <int N>
+struct Shape {
+    static constexpr int n = N + 1;
+    static constexpr int NN = n * n;
+    static constexpr int s = 2 * N + 1;
+    static constexpr int SS = s * s;
+    static constexpr int CPW = kBlock / NN;      // cells per predictor workgroup
+    static constexpr int CPWU = kUpdBlock / NN;  // cells per update workgroup
+    // per-cell LDS of the update: suh | (face fluxes, later limy + loby) | lim | lob | dt candidates
+    static constexpr int R1 = (4 * kV * n > kV * n * s + kV * NN) ? 4 * kV * n : kV * n * s + kV * NN;
+    static constexpr int UPD_LDS = kV * NN + R1 + kV * SS + kV * NN + NN;
+};
+
+ADG_DEV double dmr_shock_x(const MeshArgs &m, double time) { return m.dmr_a + m.dmr_b * time; }
+
+// ---------------------------------------------------------------------------
+// space-time predictor (+ volume integral, face extrapolation, min/max pass 2)
+// Thread (cell, i, j) owns the time line q(:, i, j, 0..N) of one spatial node.
+// The x- and y-contractions read the fluxes of the other nodes of the same
+// time slice from LDS (double buffered); the time contraction with iK1 is
+// thread-local.
+// ---------------------------------------------------------------------------
+template <int N, bool TEST>
+__global__ __launch_bounds__(kBlock) void k_predictor(MeshArgs m, StepBufs b, TestBufs t)
+{
+    using S = Shape<N>;
+    constexpr int n = S::n, NN = S::NN, CPW = S::CPW, V = kV;
+    __shared__ double sm[2][CPW][2][V][NN];
+    const Operators *__restrict__ op = b.op;
+    const int tid = threadIdx.x, lc = tid / NN, tt = tid - lc * NN;
+    const int i = tt % n, j = tt / n;
+    const int cell = blockIdx.x * CPW + lc;
+    const bool own = lc < CPW;
+    const bool act = own && cell < m.ncell;
+    const int lcs = own ? lc : CPW - 1;
+
+    double u[V];
+    const double *uh = TEST ? t.uh : b.uh_cur;
+#pragma unroll
+    for (int v = 0; v < V; v++) u[v] = act ? uh[(size_t)cell * V * NN + V * tt + v] : (v == 0 || v == 3 ? 1.0 : 0.0);
+    const double wi = op->w[i], wj = op->w[j];
+    double kx_i[n], kx_j[n];
+#pragma unroll
+    for (int k = 0; k < n; k++) {
+        kx_i[k] = op->kxi[k + n * i];
+        kx_j[k] = op->kxi[k + n * j];
+    }
+    const double dt = b.scal[0];
+
+    double q[V][n], fh[2][V];
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        fh[0][v] = 0.0;
+        fh[1][v] = 0.0;
+#pragma unroll
+        for (int l = 0; l < n; l++) q[v][l] = u[v];
+    }
+    int slot = 0;
+    // Picard iterations, Euler/space_time_predictor.f90:49-84
+    for (int it = 0; it <= N; it++) {
+        double r[V][n];
+#pragma unroll
+        for (int l = 0; l < n; l++) {
+            double Q[V], F[2 * V];
+#pragma unroll
+            for (int v = 0; v < V; v++) Q[v] = q[v][l];
+            euler_flux(m.gamma, Q, F);
+            if (own) {
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    sm[slot][lc][0][v][j * n + i] = F[v];
+                    sm[slot][lc][1][v][i * n + j] = F[V + v];
+                }
+            }
+            if (it == N) {  // time average of the last sweep's fluxes (:92)
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    fh[0][v] = fh[0][v] + F[v] * op->w[l];
+                    fh[1][v] = fh[1][v] + F[V + v] * op->w[l];
+                }
+            }
+            __syncthreads();
+            const double wl = op->w[l];
+            const double sx = wl * wj * dt / m.dx, sy = wl * wi * dt / m.dy;
+            const double c0 = wi * wj * op->f0[l];
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < n; k++) acc = acc + sm[slot][lcs][0][v][j * n + k] * kx_i[k];
+                const double rr = c0 * u[v] - sx * acc;  // :66
+                acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < n; k++) acc = acc + sm[slot][lcs][1][v][i * n + k] * kx_j[k];
+                r[v][l] = rr - sy * acc;  // :72
+            }
+            slot ^= 1;
+        }
+        const double f = 1.0 / (wi * wj);  // :81
+#pragma unroll
+        for (int v = 0; v < V; v++)
+#pragma unroll
+            for (int l = 0; l < n; l++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int mm = 0; mm < n; mm++) acc = acc + r[v][mm] * op->ik1[l + n * mm];
+                q[v][l] = f * acc;
+            }
+    }
+    double qh[V];  // :90
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int l = 0; l < n; l++) acc = acc + q[v][l] * op->w[l];
+        qh[v] = acc;
+    }
+    __syncthreads();
+    double *qhiL = &sm[0][lcs][0][0][0];
+    double *fhxL = &sm[0][lcs][1][0][0];
+    double *fhyL = &sm[1][lcs][0][0][0];
+    if (own) {
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            qhiL[v * NN + tt] = qh[v];
+            fhxL[v * NN + tt] = fh[0][v];
+            fhyL[v * NN + tt] = fh[1][v];
+        }
+    }
+    __syncthreads();
+
+    if (TEST) {
+        if (act) {
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                t.qhi[(size_t)cell * V * NN + v + V * tt] = qh[v];
+                t.fhi[(size_t)cell * 2 * V * NN + v + V * (0 + 2 * tt)] = fh[0][v];
+                t.fhi[(size_t)cell * 2 * V * NN + v + V * (1 + 2 * tt)] = fh[1][v];
+            }
+        }
+    } else {
+        // volume integral, Euler/volume_integral.f90:17-28
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < n; k++) acc = acc + fhxL[v * NN + k + n * j] * op->kxi[i + n * k];
+            double d = 0.0 + acc * wj / m.dx;
+            acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < n; k++) acc = acc + fhyL[v * NN + i + n * k] * op->kxi[j + n * k];
+            d = d + acc * wi / m.dy;
+            if (act) b.lduh[(size_t)cell * V * NN + V * tt + v] = d;
+        }
+    }
+
+    // boundary extrapolation, Euler/space_time_predictor.f90:100-119
+    for (int task = tt; task < 4 * n; task += NN) {
+        const int f = task / n, p = task - f * n;
+        const double *cf = (f & 1) ? op->frc : op->flc;
+        double Qb[V], Fb[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            double a = 0.0, c = 0.0;
+#pragma unroll
+            for (int k = 0; k < n; k++) {
+                const int idx = (f < 2) ? (k + n * p) : (p + n * k);
+                a = a + qhiL[v * NN + idx] * cf[k];
+                c = c + (f < 2 ? fhxL[v * NN + idx] : fhyL[v * NN + idx]) * cf[k];
+            }
+            Qb[v] = a;
+            Fb[v] = c;
+        }
+        if (act) {
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                if (TEST) {
+                    t.qbnd[(size_t)cell * V * 4 * n + v + V * (f + 4 * p)] = Qb[v];
+                    t.fbnd[(size_t)cell * V * 4 * n + v + V * (f + 4 * p)] = Fb[v];
+                } else {
+                    const size_t o = ((size_t)(f * V + v) * n + p) * m.ncell + cell;
+                    b.fq[o] = Qb[v];
+                    b.ff[o] = Fb[v];
+                }
+            }
+        }
+    }
+    if (TEST) return;
+
+    // get_min_max step 2: 3x3 neighbourhood, Euler/limiter.f90:147-167
+    if (act && tt < V) {
+        const int v = tt, ci = cell % m.imax, cj = cell / m.imax;
+        double mn = b.lmin_old[(size_t)v * m.ncell + cell], mx = b.lmax_old[(size_t)v * m.ncell + cell];
+        for (int jj = -1; jj <= 1; jj++)
+            for (int ii = -1; ii <= 1; ii++) {
+                const int nb = nb_index(ci, ii, m.imax, m.wrap) + m.imax * nb_index(cj, jj, m.jmax, m.wrap);
+                mn = f_min(mn, b.lmin_old[(size_t)v * m.ncell + nb]);
+                mx = f_max(mx, b.lmax_old[(size_t)v * m.ncell + nb]);
+            }
+        b.lmin[(size_t)v * m.ncell + cell] = mn;
+        b.lmax[(size_t)v * m.ncell + cell] = mx;
+    }
+    if (blockIdx.x == 0 && tid == 0) {
+        b.counters[kCntList] = 0;
+        b.counters[kCntTroubled] = 0;
+    }
+}
+
+// Face state of one Riemann problem, with the boundary treatment of the mode.
+// face: 0 left, 1 right (x faces), 2 bottom, 3 top (y faces) of cell (ci,cj).
+ADG_DEV void face_states(const MeshArgs &m, const StepBufs &b, int n, int face, int p, int cell, int ci, int cj,
+                         double time, double *QL, double *FL, double *QR, double *FR)
+{
+    const int V = kV;
+    auto ld = [&](int f, int c, double *Q, double *F) {
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            const size_t o = ((size_t)(f * V + v) * n + p) * m.ncell + c;
+            Q[v] = b.fq[o];
+            F[v] = b.ff[o];
+        }
+    };
+    auto copy = [&](const double *Qs, const double *Fs, double *Qd, double *Fd) {
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            Qd[v] = Qs[v];
+            Fd[v] = Fs[v];
+        }
+    };
+    const int I = m.imax, J = m.jmax;
+    if (face == 0) {
+        ld(0, cell, QR, FR);
+        if (ci > 0) {
+            ld(1, cell - 1, QL, FL);
+        } else if (m.bc_mode == kBcPeriodic) {
+            ld(1, cell + I - 1, QL, FL);
+        } else if (m.bc_mode == kBcTransmissive) {
+            copy(QR, FR, QL, FL);
+        } else {  // DMR inflow, Euler/riemann_problem.f90:186-193
+            double Qbc[V], Fbc[2 * V];
+#pragma unroll
+            for (int v = 0; v < V; v++) Qbc[v] = m.qa[v];
+            euler_flux(m.gamma, Qbc, Fbc);
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                QL[v] = Qbc[v];
+                FL[v] = 0.0 + Fbc[v] * 1.0 + Fbc[V + v] * 0.0;
+            }
+        }
+    } else if (face == 1) {
+        ld(1, cell, QL, FL);
+        if (ci < I - 1)
+            ld(0, cell + 1, QR, FR);
+        else if (m.bc_mode == kBcPeriodic)
+            ld(0, cell - (I - 1), QR, FR);
+        else
+            copy(QL, FL, QR, FR);  // transmissive, also the DMR right boundary (:195-198)
+    } else if (face == 2) {
+        ld(2, cell, QR, FR);
+        if (cj > 0) {
+            ld(3, cell - I, QL, FL);
+        } else if (m.bc_mode == kBcPeriodic) {
+            ld(3, cell + (J - 1) * I, QL, FL);
+        } else {
+            copy(QR, FR, QL, FL);
+            if (m.bc_mode == kBcDmr && !((double)(ci + 1) * m.dx <= 1.0 / 6.0)) {  // reflective, :225-228
+                QL[2] = -QL[2];
+                FL[0] = -FL[0];
+                FL[1] = -FL[1];
+                FL[3] = -FL[3];
+            }
+        }
+    } else {
+        ld(3, cell, QL, FL);
+        if (cj < J - 1) {
+            ld(2, cell + I, QR, FR);
+        } else if (m.bc_mode == kBcPeriodic) {
+            ld(2, cell - (J - 1) * I, QR, FR);
+        } else if (m.bc_mode == kBcTransmissive) {
+            copy(QL, FL, QR, FR);
+        } else {  // DMR top: exact shock position, :233-245
+            const bool pre = (double)(ci + 1) * m.dx <= dmr_shock_x(m, time);
+            double Qbc[V], Fbc[2 * V];
+#pragma unroll
+            for (int v = 0; v < V; v++) Qbc[v] = pre ? m.qa[v] : m.qb[v];
+            euler_flux(m.gamma, Qbc, Fbc);
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                QR[v] = Qbc[v];
+                FR[v] = 0.0 + Fbc[v] * 0.0 + Fbc[V + v] * 1.0;
+            }
+        }
+    }
+}
+
+// min/max of nodal values, Lobatto values and subcell averages of one cell
+// held in LDS (uhL [v][NN]); also flags inadmissible subcell averages.
+// Writes per-variable (minval, maxval) of the three groups to rng[v][6].
+template <int N>
+ADG_DEV void cell_ranges_stage1(const Operators *__restrict__ op, const double *uhL, double *tyL, double *byL, int tt,
+                                int nthr)
+{
+    using S = Shape<N>;
+    constexpr int n = S::n, NN = S::NN, s = S::s, V = kV;
+    for (int task = tt; task < n * s; task += nthr) {  // get_subcell_data first pass, Euler/limiter.f90:25-27
+        const int a = task % n, qq = task / n;
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int bb = 0; bb < n; bb++) acc = acc + uhL[v * NN + a + n * bb] * op->u2l[qq + s * bb];
+            tyL[v * n * s + a + n * qq] = acc;
+        }
+    }
+    for (int task = tt; task < NN; task += nthr) {  // get_lobatto_data first pass, :56-58
+        const int a = task % n, qq = task / n;
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int bb = 0; bb < n; bb++) acc = acc + uhL[v * NN + a + n * bb] * op->u2b[qq + n * bb];
+            byL[v * NN + a + n * qq] = acc;
+        }
+    }
+}
+
+template <int N>
+ADG_DEV bool cell_ranges_stage2(const Operators *__restrict__ op, double g, const double *tyL, const double *byL,
+                                double *limL, double *lobL, int tt, int nthr)
+{
+    using S = Shape<N>;
+    constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, V = kV;
+    bool bad = false;
+    for (int task = tt; task < SS; task += nthr) {  // second pass, :33-35
+        const int p = task % s, qq = task / s;
+        double L[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int a = 0; a < n; a++) acc = acc + tyL[v * n * s + a + n * qq] * op->u2l[p + s * a];
+            limL[v * SS + task] = acc;
+            L[v] = acc;
+        }
+        bad = bad || euler_inadmissible(g, L);  // physical detection, Euler/limiter.f90:227-241
+    }
+    for (int task = tt; task < NN; task += nthr) {  // :64-66
+        const int p = task % n, qq = task / n;
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int a = 0; a < n; a++) acc = acc + byL[v * NN + a + n * qq] * op->u2b[p + n * a];
+            lobL[v * NN + task] = acc;
+        }
+    }
+    return bad;
+}
+
+// (minval, maxval) of nodal, Lobatto, subcell groups of variable v
+template <int N>
+ADG_DEV void group_ranges(const double *uhL, const double *lobL, const double *limL, int v, double r[6])
+{
+    using S = Shape<N>;
+    constexpr int NN = S::NN, SS = S::SS;
+    double u0 = __builtin_nan(""), u1 = u0, b0 = u0, b1 = u0, l0 = u0, l1 = u0;
+    for (int k = 0; k < NN; k++) {
+        u0 = fold_min(u0, uhL[v * NN + k]);
+        u1 = fold_max(u1, uhL[v * NN + k]);
+        b0 = fold_min(b0, lobL[v * NN + k]);
+        b1 = fold_max(b1, lobL[v * NN + k]);
+    }
+    for (int k = 0; k < SS; k++) {
+        l0 = fold_min(l0, limL[v * SS + k]);
+        l1 = fold_max(l1, limL[v * SS + k]);
+    }
+    r[0] = u0; r[1] = u1; r[2] = b0; r[3] = b1; r[4] = l0; r[5] = l1;
+}
+
+// time-step candidate of one node, Euler/calc_time_step.f90:29-42
+ADG_DEV double node_dt(const MeshArgs &m, const double *Q)
+{
+    double denom = 0.0;
+    denom = denom + euler_maxabs_eig(m.gamma, Q, 1.0, 0.0) / m.dx;
+    denom = denom + euler_maxabs_eig(m.gamma, Q, 0.0, 1.0) / m.dy;
+    return m.cflpnpm / denom;
+}
+
+// ---------------------------------------------------------------------------
+// Riemann + surface integral + element update + DMP (SEED = false), or only
+// the per-cell limiter seeds of a freshly uploaded solution (SEED = true).
+// ---------------------------------------------------------------------------
+template <int N, bool SEED>
+__global__ __launch_bounds__(kUpdBlock) void k_update(MeshArgs m, StepBufs b)
+{
+    using S = Shape<N>;
+    constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, CPW = S::CPWU, V = kV;
+    __shared__ double lds[CPW][S::UPD_LDS];
+    __shared__ int sflag[CPW];
+    const Operators *__restrict__ op = b.op;
+    const int tid = threadIdx.x, lc = tid / NN, tt = tid - lc * NN;
+    const int i = tt % n, j = tt / n;
+    const int cell = blockIdx.x * CPW + lc;
+    const bool own = lc < CPW;
+    const bool act = own && cell < m.ncell;
+    const int lcs = own ? lc : CPW - 1;
+    const int ci = act ? cell % m.imax : 0, cj = act ? cell / m.imax : 0;
+    const double time = b.scal[1];
+    double *suh = &lds[lcs][0];
+    double *sflx = suh + V * NN;       // [f][v][p], dead once the update is done
+    double *sty = sflx;                // limy [v][a + n q] reuses the face-flux space
+    double *sby = sty + V * n * s;     // loby
+    double *slim = suh + V * NN + S::R1;
+    double *slob = slim + V * SS;
+    double *sdt = slob + V * NN;
+
+    if (!SEED && act) {  // Riemann problems of the cell's 4 faces, Euler/riemann_problem.f90:177-257
+        for (int task = tt; task < 4 * n; task += NN) {
+            const int f = task / n, p = task - f * n;
+            double QL[V], FL[V], QR[V], FR[V], out[V];
+            face_states(m, b, n, f, p, cell, ci, cj, time, QL, FL, QR, FR);
+            euler_rusanov(m.gamma, QL, FL, QR, FR, f < 2 ? 1.0 : 0.0, f < 2 ? 0.0 : 1.0, out);
+#pragma unroll
+            for (int v = 0; v < V; v++) sflx[(f * V + v) * n + p] = out[v];
+        }
+    }
+    if (own && tt == 0) sflag[lc] = 0;
+    __syncthreads();
+
+    double un[V];
+    if (SEED) {
+#pragma unroll
+        for (int v = 0; v < V; v++) un[v] = act ? b.uh_cur[(size_t)cell * V * NN + V * tt + v] : (v == 0 || v == 3 ? 1.0 : 0.0);
+    } else {
+        const double wi = op->w[i], wj = op->w[j], dt = b.scal[0];
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            double d = act ? b.lduh[(size_t)cell * V * NN + V * tt + v] : 0.0;
+            const double uo = act ? b.uh_cur[(size_t)cell * V * NN + V * tt + v] : (v == 0 || v == 3 ? 1.0 : 0.0);
+            if (act) {  // surface integral, Euler/surface_integral.f90:182-196
+                d = d - wj / m.dx * (sflx[(1 * V + v) * n + j] * op->frc[i] - sflx[(0 * V + v) * n + j] * op->flc[i]);
+                d = d - wi / m.dy * (sflx[(3 * V + v) * n + i] * op->frc[j] - sflx[(2 * V + v) * n + i] * op->flc[j]);
+            }
+            d = d / (wi * wj);  // element update, Euler/element_update.f90:218-226
+            un[v] = uo + dt * d;
+            if (act) b.uh_next[(size_t)cell * V * NN + V * tt + v] = un[v];
+        }
+    }
+    __syncthreads();  // face fluxes consumed: their space becomes limy/loby
+    if (own) {
+#pragma unroll
+        for (int v = 0; v < V; v++) suh[v * NN + tt] = un[v];
+        sdt[tt] = node_dt(m, un);
+    }
+    __syncthreads();
+    if (own) cell_ranges_stage1<N>(op, suh, sty, sby, tt, NN);
+    __syncthreads();
+    bool bad = false;
+    if (own) bad = cell_ranges_stage2<N>(op, m.gamma, sty, sby, slim, slob, tt, NN);
+    if (bad && !SEED) atomicOr(&sflag[lc], 1);
+    __syncthreads();
+    const int *st_cur = b.status_cur;
+    if (act && tt < V) {
+        const int v = tt;
+        double r[6];
+        group_ranges<N>(suh, slob, slim, v, r);
+        if (!SEED) {  // DMP, Euler/limiter.f90:198-223 (argmax = 0)
+            const double lmn = b.lmin[(size_t)v * m.ncell + cell], lmx = b.lmax[(size_t)v * m.ncell + cell];
+            const double mn = f_min(f_min(r[0], r[4]), r[2]);
+            const double mx = f_max(f_max(r[1], r[5]), r[3]);
+            const double ldiff = f_max(f_max(1.0e-4, 1.0e-3 * (lmx - lmn)), 0.0);
+            if (mn < lmn - ldiff || mx > lmx + ldiff) atomicOr(&sflag[lc], 2);
+        }
+        // next step's get_min_max step 1 for a cell without limiter, Euler/limiter.f90:123-132
+        double mn = f_min(f_min(r[0], r[2]), r[4]), mx = f_max(f_max(r[1], r[3]), r[5]);
+        if (SEED && st_cur[cell] != 0) {  // limited cell: range of its subcell data, :135-139
+            const double *L = b.lh_cur + (size_t)cell * V * SS;
+            mn = __builtin_nan("");
+            mx = mn;
+            for (int k = 0; k < SS; k++) {
+                mn = fold_min(mn, L[v + V * k]);
+                mx = fold_max(mx, L[v + V * k]);
+            }
+        }
+        b.lmin_old[(size_t)v * m.ncell + cell] = mn;
+        b.lmax_old[(size_t)v * m.ncell + cell] = mx;
+    }
+    if (act && tt == V) {
+        double d = __builtin_nan("");
+        for (int k = 0; k < NN; k++) d = fold_min(d, sdt[k]);
+        b.cell_dt[cell] = d;
+    }
+    __syncthreads();
+    if (!SEED && act && tt == 0) {
+        const int tr = sflag[lc] != 0;
+        b.rc[cell] = tr;
+        if (tr) atomicAdd(&b.counters[kCntTroubled], 1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// spread_recompute + allocate_limiter + compaction (one thread per cell)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_spread(MeshArgs m, StepBufs b)
+{
+    __shared__ int wcount[kBlock / 64], wbase[kBlock / 64], bbase;
+    const int cell = blockIdx.x * kBlock + threadIdx.x;
+    int flag = 0;
+    if (cell < m.ncell) {
+        if (b.rc[cell] == 1) {
+            flag = 1;
+        } else {
+            // a face neighbour marked 1 marks this cell 2 (Euler/limiter.f90:336-350; the
+            // face-neighbour relation of the clamped/wrapped table is symmetric)
+            const int ci = cell % m.imax, cj = cell / m.imax;
+            const int nbs[4] = {nb_index(ci, -1, m.imax, m.wrap) + m.imax * cj, nb_index(ci, 1, m.imax, m.wrap) + m.imax * cj,
+                                ci + m.imax * nb_index(cj, -1, m.jmax, m.wrap), ci + m.imax * nb_index(cj, 1, m.jmax, m.wrap)};
+#pragma unroll
+            for (int f = 0; f < 4; f++)
+                if (nbs[f] != cell && b.rc[nbs[f]] == 1) flag = 2;
+        }
+        b.rc_final[cell] = flag;
+        b.status_next[cell] = flag == 1 ? 1 : 0;  // allocate_limiter, Euler/limiter.f90:308-320
+    }
+    const bool take = flag > 0;
+    const unsigned long long mask = __ballot(take);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int below = __popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0) wcount[wave] = __popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int w = 0; w < kBlock / 64; w++) {
+            wbase[w] = tot;
+            tot += wcount[w];
+        }
+        bbase = tot ? atomicAdd(&b.counters[kCntList], tot) : 0;
+    }
+    __syncthreads();
+    if (take) b.list[bbase + wbase[wave] + below] = cell;
+}
+
+// ---------------------------------------------------------------------------
+// subcell_recompute over the compacted list, one wavefront per cell
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(64) void k_subcell(MeshArgs m, StepBufs b)
+{
+    using S = Shape<N>;
+    constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, V = kV, H = s + 2, HH = H * H;
+    __shared__ double sLy[5][V * n * s];
+    __shared__ double sS0[V * HH];
+    __shared__ double sSl[2][V * SS];
+    __shared__ double sW[4][V * HH];
+    __shared__ double sFx[V * (s + 1) * s];
+    __shared__ double sFy[V * s * (s + 1)];
+    __shared__ double sSub[V * SS];
+    __shared__ double sUhy[V * s * n];
+    __shared__ double sUh[V * NN];
+    __shared__ double sTy[V * n * s], sBy[V * NN], sLim[V * SS], sLob[V * NN];
+    __shared__ double sDt[NN];
+    __shared__ int sFlag;
+    const Operators *__restrict__ op = b.op;
+    const int lane = threadIdx.x;
+    const int count = b.counters[kCntList];
+    const double dt = b.scal[0], time = b.scal[1];
+    const double ldx = m.dx / (double)s, ldy = m.dy / (double)s;
+    for (int item = blockIdx.x; item < count; item += gridDim.x) {
+        const int cell = b.list[item];
+        const int flag = b.rc_final[cell];
+        const int ci = cell % m.imax, cj = cell / m.imax;
+        int nbk[5];
+        nbk[0] = cell;
+        nbk[1] = nb_index(ci, -1, m.imax, m.wrap) + m.imax * cj;
+        nbk[2] = nb_index(ci, 1, m.imax, m.wrap) + m.imax * cj;
+        nbk[3] = ci + m.imax * nb_index(cj, -1, m.jmax, m.wrap);
+        nbk[4] = ci + m.imax * nb_index(cj, 1, m.jmax, m.wrap);
+        int stk[5];
+#pragma unroll
+        for (int k = 0; k < 5; k++) stk[k] = b.status_cur[nbk[k]];
+        // subcell data of olduh for unlimited blocks, first pass (Euler/limiter.f90:25-27, :476-477)
+        for (int task = lane; task < 5 * n * s; task += 64) {
+            const int blk = task / (n * s), r = task - blk * n * s, a = r % n, qq = r / n;
+            if (stk[blk] != 0) continue;
+            const double *u = b.uh_cur + (size_t)nbk[blk] * V * NN;
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int bb = 0; bb < n; bb++) acc = acc + u[v + V * (a + n * bb)] * op->u2l[qq + s * bb];
+                sLy[blk][v * n * s + a + n * qq] = acc;
+            }
+        }
+        __syncthreads();
+        // subuh0 on the subgrid 0..s+1 (index P = ii): the cell and one ring of its face
+        // neighbours' subcells (:470-485), with the DMR ghost overrides (:491-522)
+        for (int task = lane; task < HH; task += 64) {
+            const int ii = task % H, jj = task / H;
+            const bool in_x = ii >= 1 && ii <= s, in_y = jj >= 1 && jj <= s;
+            int blk, p, qq;
+            if (in_x && in_y) { blk = 0; p = ii - 1; qq = jj - 1; }
+            else if (ii == 0 && in_y) { blk = 1; p = s - 1; qq = jj - 1; }
+            else if (ii == s + 1 && in_y) { blk = 2; p = 0; qq = jj - 1; }
+            else if (jj == 0 && in_x) { blk = 3; p = ii - 1; qq = s - 1; }
+            else if (jj == s + 1 && in_x) { blk = 4; p = ii - 1; qq = 0; }
+            else {  // corner subcells are never read
+#pragma unroll
+                for (int v = 0; v < V; v++) sS0[v * HH + task] = 0.0;
+                continue;
+            }
+            double val[V];
+            if (stk[blk] == 0) {
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int a = 0; a < n; a++) acc = acc + sLy[blk][v * n * s + a + n * qq] * op->u2l[p + s * a];
+                    val[v] = acc;
+                }
+            } else {
+                const double *L = b.lh_cur + (size_t)nbk[blk] * V * SS;
+#pragma unroll
+                for (int v = 0; v < V; v++) val[v] = L[v + V * (p + s * qq)];
+            }
+            if (m.bc_mode == kBcDmr) {
+                if (ci == 0 && ii == 0)
+#pragma unroll
+                    for (int v = 0; v < V; v++) val[v] = m.qa[v];
+                if (cj == 0 && jj == 0 && (double)(ci + 1) * m.dx > 1.0 / 6.0) val[2] = -val[2];
+                if (cj == m.jmax - 1 && jj == s + 1) {
+                    const bool pre = (double)(ci + 1) * m.dx <= dmr_shock_x(m, time);
+#pragma unroll
+                    for (int v = 0; v < V; v++) val[v] = pre ? m.qa[v] : m.qb[v];
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < V; v++) sS0[v * HH + task] = val[v];
+        }
+        __syncthreads();
+#define S0(v, ii, jj) sS0[(v) * HH + (ii) + H * (jj)]
+        // minmod slopes on the cell's own subcells, :529-536
+        for (int task = lane; task < SS; task += 64) {
+            const int ii = task % s + 1, jj = task / s + 1;
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                sSl[0][v * SS + task] = minmod(S0(v, ii + 1, jj) - S0(v, ii, jj), S0(v, ii, jj) - S0(v, ii - 1, jj));
+                sSl[1][v * SS + task] = minmod(S0(v, ii, jj + 1) - S0(v, ii, jj), S0(v, ii, jj) - S0(v, ii, jj - 1));
+            }
+        }
+        __syncthreads();
+        // Hancock half step, :540-569 (halo slopes are zero)
+        for (int task = lane; task < HH; task += 64) {
+            const int ii = task % H, jj = task / H;
+            const bool in_x = ii >= 1 && ii <= s, in_y = jj >= 1 && jj <= s;
+            if (!in_x && !in_y) continue;
+            const bool inner = in_x && in_y;
+            const int sidx = (ii - 1) + s * (jj - 1);
+            double wLx[V], wRx[V], wLy[V], wRy[V], FLx[2 * V], FRx[2 * V], FLy[2 * V], FRy[2 * V], Qt[V];
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                const double sl = inner ? sSl[0][v * SS + sidx] : 0.0;
+                wLx[v] = S0(v, ii, jj) - 0.5 * sl;
+                wRx[v] = S0(v, ii, jj) + 0.5 * sl;
+            }
+            euler_flux(m.gamma, wLx, FLx);
+            euler_flux(m.gamma, wRx, FRx);
+#pragma unroll
+            for (int v = 0; v < V; v++) Qt[v] = -(FRx[v] - FLx[v]) / ldx;
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                const double sl = inner ? sSl[1][v * SS + sidx] : 0.0;
+                wLy[v] = S0(v, ii, jj) - 0.5 * sl;
+                wRy[v] = S0(v, ii, jj) + 0.5 * sl;
+            }
+            euler_flux(m.gamma, wLy, FLy);
+            euler_flux(m.gamma, wRy, FRy);
+#pragma unroll
+            for (int v = 0; v < V; v++) Qt[v] = Qt[v] - (FRy[V + v] - FLy[V + v]) / ldy;
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                sW[0][v * HH + task] = wLx[v] + 0.5 * dt * Qt[v];
+                sW[1][v * HH + task] = wRx[v] + 0.5 * dt * Qt[v];
+                sW[2][v * HH + task] = wLy[v] + 0.5 * dt * Qt[v];
+                sW[3][v * HH + task] = wRy[v] + 0.5 * dt * Qt[v];
+            }
+        }
+        __syncthreads();
+        // subcell Rusanov fluxes, :571-588
+        for (int task = lane; task < (s + 1) * s; task += 64) {
+            const int ii = task % (s + 1) + 1, jj = task / (s + 1) + 1;
+            double QL[V], QR[V], out[V];
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                QL[v] = sW[1][v * HH + (ii - 1) + H * jj];
+                QR[v] = sW[0][v * HH + ii + H * jj];
+            }
+            euler_rusanov_s(m.gamma, QL, QR, 1.0, 0.0, out);
+#pragma unroll
+            for (int v = 0; v < V; v++) sFx[v * (s + 1) * s + task] = out[v];
+        }
+        for (int task = lane; task < s * (s + 1); task += 64) {
+            const int ii = task % s + 1, jj = task / s + 1;
+            double QL[V], QR[V], out[V];
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                QL[v] = sW[3][v * HH + ii + H * (jj - 1)];
+                QR[v] = sW[2][v * HH + ii + H * jj];
+            }
+            euler_rusanov_s(m.gamma, QL, QR, 0.0, 1.0, out);
+#pragma unroll
+            for (int v = 0; v < V; v++) sFy[v * s * (s + 1) + task] = out[v];
+        }
+        __syncthreads();
+        // finite-volume update, :592-597
+        for (int task = lane; task < SS; task += 64) {
+            const int ii = task % s + 1, jj = task / s + 1;
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                const double *fx = sFx + v * (s + 1) * s, *fy = sFy + v * s * (s + 1);
+                sSub[v * SS + task] = S0(v, ii, jj) - dt / ldx * (fx[ii + (s + 1) * (jj - 1)] - fx[(ii - 1) + (s + 1) * (jj - 1)]) -
+                                      dt / ldy * (fy[(ii - 1) + s * jj] - fy[(ii - 1) + s * (jj - 1)]);
+            }
+        }
+#undef S0
+        __syncthreads();
+        // put_subcell_data, Euler/limiter.f90:86-96
+        for (int task = lane; task < s * n; task += 64) {
+            const int p = task % s, bb = task / s;
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int qq = 0; qq < s; qq++) acc = acc + sSub[v * SS + p + s * qq] * op->l2u[bb + n * qq];
+                sUhy[v * s * n + p + s * bb] = acc;
+            }
+        }
+        __syncthreads();
+        for (int task = lane; task < NN; task += 64) {
+            const int a = task % n, bb = task / n;
+            double un[V];
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int p = 0; p < s; p++) acc = acc + sUhy[v * s * n + p + s * bb] * op->l2u[a + n * p];
+                un[v] = acc;
+                sUh[v * NN + task] = acc;
+                b.uh_next[(size_t)cell * V * NN + V * task + v] = acc;
+            }
+            sDt[task] = node_dt(m, un);
+        }
+        if (lane == 0) sFlag = 0;
+        __syncthreads();
+        // DMP of the recomputed polynomial (:619) and the next step's limiter seeds
+        cell_ranges_stage1<N>(op, sUh, sTy, sBy, lane, 64);
+        __syncthreads();
+        const bool bad = cell_ranges_stage2<N>(op, m.gamma, sTy, sBy, sLim, sLob, lane, 64);
+        if (bad) atomicOr(&sFlag, 1);
+        __syncthreads();
+        if (lane < V) {
+            const int v = lane;
+            double r[6];
+            group_ranges<N>(sUh, sLob, sLim, v, r);
+            const double lmn = b.lmin[(size_t)v * m.ncell + cell], lmx = b.lmax[(size_t)v * m.ncell + cell];
+            const double mn = f_min(f_min(r[0], r[4]), r[2]);
+            const double mx = f_max(f_max(r[1], r[5]), r[3]);
+            const double ldiff = f_max(f_max(1.0e-4, 1.0e-3 * (lmx - lmn)), 0.0);
+            if (mn < lmn - ldiff || mx > lmx + ldiff) atomicOr(&sFlag, 2);
+        }
+        __syncthreads();
+        const int status = (flag == 1) ? 1 : (sFlag != 0 ? 1 : 0);
+        if (lane < V) {
+            const int v = lane;
+            double mn, mx;
+            if (status) {  // next get_min_max uses Lh (:135-139)
+                mn = __builtin_nan("");
+                mx = mn;
+                for (int k = 0; k < SS; k++) {
+                    mn = fold_min(mn, sSub[v * SS + k]);
+                    mx = fold_max(mx, sSub[v * SS + k]);
+                }
+            } else {
+                double r[6];
+                group_ranges<N>(sUh, sLob, sLim, v, r);
+                mn = f_min(f_min(r[0], r[2]), r[4]);
+                mx = f_max(f_max(r[1], r[3]), r[5]);
+            }
+            b.lmin_old[(size_t)v * m.ncell + cell] = mn;
+            b.lmax_old[(size_t)v * m.ncell + cell] = mx;
+        }
+        if (lane == V) {
+            double d = __builtin_nan("");
+            for (int k = 0; k < NN; k++) d = fold_min(d, sDt[k]);
+            b.cell_dt[cell] = d;
+            b.status_next[cell] = status;
+        }
+        if (status) {  // NewLh, and Lh = NewLh at update_limiter (:287)
+            double *L = b.lh_next + (size_t)cell * V * SS;
+            for (int k = lane; k < V * SS; k += 64) L[k] = sSub[(k % V) * SS + k / V];
+        }
+        __syncthreads();
+    }
+}
+
+// calc_time_step reduction with the last-block pattern (no host sync, no memset)
+__global__ __launch_bounds__(kBlock) void k_dt(StepBufs b, int ncell, double host_time, double tend, int use_host_time)
+{
+    __shared__ double red[kBlock];
+    double mloc = __builtin_nan("");
+    for (int c = blockIdx.x * kBlock + threadIdx.x; c < ncell; c += gridDim.x * kBlock) mloc = fold_min(mloc, b.cell_dt[c]);
+    red[threadIdx.x] = mloc;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fold_min(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    __shared__ int last;
+    if (threadIdx.x == 0) {
+        b.dt_partial[blockIdx.x] = red[0];
+        __threadfence();
+        const int ticket = atomicAdd(&b.counters[kCntDtBlocks], 1);
+        last = ticket == (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double mm = __builtin_nan("");
+    for (int k = threadIdx.x; k < (int)gridDim.x; k += kBlock)
+        mm = fold_min(mm, __hip_atomic_load(&b.dt_partial[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    red[threadIdx.x] = mm;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fold_min(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double time = use_host_time ? host_time : b.scal[1];
+        if (!use_host_time) tend = b.scal[2];
+        double dt = f_min(1.0e20, red[0]);  // dt = 1e20 then min over nodes (:257-269)
+        if (time + dt > tend) dt = tend - time;  // :276-278
+        b.scal[0] = dt;
+        b.scal[1] = time;
+        b.scal[2] = tend;
+        b.counters[kCntDtBlocks] = 0;
+    }
+}
+
+__global__ void k_advance(StepBufs b)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) b.scal[1] = b.scal[1] + b.scal[0];  // time = time + dt
+}
+
+template <int N>
+hipError_t launch_step_impl(const MeshArgs &m, const StepBufs &b, hipStream_t st, int part)
+{
+    using S = Shape<N>;
+    const int gcell = (m.ncell + S::CPW - 1) / S::CPW;
+    const int gupd = (m.ncell + S::CPWU - 1) / S::CPWU;
+    switch (part) {
+    case kPartPredictor:
+        hipLaunchKernelGGL((k_predictor<N, false>), dim3(gcell), dim3(kBlock), 0, st, m, b, TestBufs{});
+        break;
+    case kPartUpdate:
+        hipLaunchKernelGGL((k_update<N, false>), dim3(gupd), dim3(kUpdBlock), 0, st, m, b);
+        break;
+    case kPartSpread:
+        hipLaunchKernelGGL(k_spread, dim3((m.ncell + kBlock - 1) / kBlock), dim3(kBlock), 0, st, m, b);
+        break;
+    case kPartSubcell: {
+        const int g = m.ncell < kSubcellGrid ? m.ncell : kSubcellGrid;
+        hipLaunchKernelGGL((k_subcell<N>), dim3(g), dim3(64), 0, st, m, b);
+        break;
+    }
+    case kPartSeed:
+        hipLaunchKernelGGL((k_update<N, true>), dim3(gupd), dim3(kUpdBlock), 0, st, m, b);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int N>
+hipError_t launch_test_predictor_impl(const MeshArgs &m, const StepBufs &b, const TestBufs &t, hipStream_t st)
+{
+    using S = Shape<N>;
+    const int gcell = (m.ncell + S::CPW - 1) / S::CPW;
+    hipLaunchKernelGGL((k_predictor<N, true>), dim3(gcell), dim3(kBlock), 0, st, m, b, t);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_part(int N, const MeshArgs &m, const StepBufs &b, hipStream_t st, int part)
+{
+    if (part == kPartDt) {
+        hipLaunchKernelGGL(k_dt, dim3(kDtGrid), dim3(kBlock), 0, st, b, m.ncell, 0.0, 0.0, 0);
+        return hipGetLastError();
+    }
+    if (part == kPartAdvance) {
+        hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, b);
+        return hipGetLastError();
+    }
+    switch (N) {
+    case 1: return launch_step_impl<1>(m, b, st, part);
+    case 2: return launch_step_impl<2>(m, b, st, part);
+    case 3: return launch_step_impl<3>(m, b, st, part);
+    case 4: return launch_step_impl<4>(m, b, st, part);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_dt(const MeshArgs &m, const StepBufs &b, hipStream_t st, double time, double tend, int use_host_time)
+{
+    hipLaunchKernelGGL(k_dt, dim3(kDtGrid), dim3(kBlock), 0, st, b, m.ncell, time, tend, use_host_time);
+    return hipGetLastError();
+}
+
+hipError_t launch_test_predictor(int N, const MeshArgs &m, const StepBufs &b, const TestBufs &t, hipStream_t st)
+{
+    switch (N) {
+    case 1: return launch_test_predictor_impl<1>(m, b, t, st);
+    case 2: return launch_test_predictor_impl<2>(m, b, t, st);
+    case 3: return launch_test_predictor_impl<3>(m, b, t, st);
+    case 4: return launch_test_predictor_impl<4>(m, b, t, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace aderdg

@@ -1,0 +1,166 @@
+/*
+ * aderdg.h -- C-ABI of the MI355X-native ADER-DG element step.
+ *
+ * The reference solver (dasikasunder/ADER-DG) is a Fortran program whose hot
+ * path is a sequence of external subroutines with implicit interfaces called
+ * by its driver (Euler/main.f90:24-100).  This library replaces those
+ * subroutines with hand-written HIP kernels for gfx950.  Every entry point
+ * below names the reference routine(s) it stands in for; the Fortran binding
+ * module (ader-dg_amd/fortran/aderdg_hip.f90) declares them with
+ * iso_c_binding so that a driver with main.f90's call order runs on the GPU.
+ *
+ * Conventions
+ *  - Host arrays cross the boundary in the reference's Fortran layouts
+ *    (Euler/ader_dg.f90:85-93): uh(nVar,N+1,N+1,IMAX,JMAX) etc., float64.
+ *  - Device state is owned by the context and stays resident between calls.
+ *  - Return 0 on success or a negative ADERDG_E* code; aderdg_last_error()
+ *    gives the message (the reference aborts on errors instead, e.g.
+ *    Euler/la.f90:39-44; the Fortran binding turns a non-zero code into `stop`).
+ *  - One host thread per context.  Calls are asynchronous on the context's
+ *    stream except those that return a value computed on the device.
+ */
+#ifndef ADERDG_H
+#define ADERDG_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ADERDG_OK 0
+#define ADERDG_EINVAL -1   /* invalid argument or descriptor */
+#define ADERDG_EHIP -2     /* HIP runtime error */
+#define ADERDG_ESINGULAR -3/* singular operator matrix (Euler/la.f90:39-44) */
+#define ADERDG_ESTATE -4   /* call out of order */
+#define ADERDG_ECAPACITY -5/* a fixed-size buffer is too small */
+
+enum aderdg_pde { ADERDG_PDE_EULER = 0 };
+/* Boundary modes.  DMR reproduces the reference at HEAD exactly
+ * (Euler/riemann_problem.f90:10-116 and the ghost overrides of
+ * Euler/limiter.f90:491-522).  PERIODIC wraps both the face coupling and the
+ * limiter neighbour table (Euler/initialize.f90:241-255, boundary code 3).
+ * TRANSMISSIVE copies the interior state into the ghost face state
+ * (Euler/riemann_problem.f90:50-53); the limiter neighbour table clamps to the
+ * cell itself (Euler/initialize.f90:231-237). */
+enum aderdg_bc { ADERDG_BC_DMR = 0, ADERDG_BC_PERIODIC = 1, ADERDG_BC_TRANSMISSIVE = 2 };
+
+typedef struct aderdg_ctx aderdg_ctx;
+
+/* Operator matrices of Euler/initialize.f90:80-213, column-major as in the
+ * reference module.  Pointers may be NULL in aderdg_desc.ops, in which case
+ * the library builds them with the same algorithm (quadrature.f90, basis.f90,
+ * la.f90). */
+typedef struct {
+    const double *wGPN;    /* (N+1)            Gauss-Legendre weights on [0,1] */
+    const double *Kxi;     /* (N+1,N+1)        stiffness                       */
+    const double *iK1;     /* (N+1,N+1)        (F1 - Kxi)^-1                   */
+    const double *F0;      /* (N+1)            basis at xi = 0                 */
+    const double *FLcoeff; /* (N+1)                                            */
+    const double *FRcoeff; /* (N+1)                                            */
+    const double *uh2lim;  /* (2N+1,N+1)       DG -> subcell averages          */
+    const double *lim2uh;  /* (N+1,2N+1)       subcell averages -> DG          */
+    const double *uh2lob;  /* (N+1,N+1)        DG -> Gauss-Lobatto values      */
+} aderdg_ops;
+
+typedef struct {
+    int pde;        /* enum aderdg_pde                                   */
+    int N;          /* polynomial degree, 1..4 (Euler/ader_dg.f90:17)    */
+    int imax, jmax; /* cells per direction                               */
+    double xl[2], xr[2];
+    int bc_mode;    /* enum aderdg_bc                                    */
+    double gamma;   /* EQN%GAMMA                                         */
+    double cfl;     /* CFL (Euler/ader_dg.f90:19), 0.9 in the reference  */
+    int exact;      /* reserved, must be 0                               */
+    int device;     /* HIP device ordinal                                */
+    aderdg_ops ops; /* optional host operator matrices                   */
+} aderdg_desc;
+
+/* Setup: replaces the allocation and operator part of `initialize`
+ * (Euler/initialize.f90:80-145).  The initial condition comes through
+ * aderdg_upload_uh. */
+int aderdg_create(const aderdg_desc *desc, aderdg_ctx **out);
+int aderdg_destroy(aderdg_ctx *ctx);
+const char *aderdg_last_error(void);
+/* 1 if the library was compiled with floating-point contraction off
+ * (bit-identical mode), 0 for the fused-multiply-add build */
+int aderdg_build_exact(void);
+
+/* Host-only construction of the operator matrices of Euler/initialize.f90:80-213
+ * for degree N (no device needed): xi, wGPN (N+1); Kxi, dudx, iK1 (N+1,N+1);
+ * F0, FLcoeff, FRcoeff (N+1); uh2lim (2N+1,N+1); lim2uh (N+1,2N+1); uh2lob (N+1,N+1). */
+int aderdg_operators(int N, double *xi, double *wGPN, double *Kxi, double *dudx, double *iK1, double *F0,
+                     double *FLcoeff, double *FRcoeff, double *uh2lim, double *lim2uh, double *uh2lob);
+
+/* Operators the context uses (host copies, same layout as aderdg_ops; any pointer may be NULL) */
+int aderdg_get_ops(aderdg_ctx *ctx, double *wGPN, double *Kxi, double *iK1, double *F0, double *FLcoeff,
+                   double *FRcoeff, double *uh2lim, double *lim2uh, double *uh2lob);
+
+/* Upload the solution and reset the limiter to "no troubled cell"
+ * (Euler/initialize.f90:160-165).  Seeds the per-cell min/max and time-step
+ * candidates that the next aderdg_calc_time_step reduces. */
+int aderdg_upload_uh(aderdg_ctx *ctx, const double *uh);
+int aderdg_download_uh(aderdg_ctx *ctx, double *uh);
+/* Limiter state in host layout: status(IMAX,JMAX), Lh(nVar,2N+1,2N+1,IMAX,JMAX).
+ * set: cells with status /= 0 become limited with the given Lh.
+ * get: status and recompute maps of the last step, Lh of limited cells (0
+ * elsewhere), lmin/lmax(nVar,IMAX,JMAX) of the last step's get_min_max. Any
+ * output pointer may be NULL. */
+int aderdg_set_limiter(aderdg_ctx *ctx, const int32_t *status, const double *Lh);
+int aderdg_get_limiter(aderdg_ctx *ctx, int32_t *status, int32_t *recompute, double *Lh, double *lmin,
+                       double *lmax);
+
+/* calc_time_step (Euler/calc_time_step.f90:9-52): dt = min over all nodes of
+ * CFL*PNPM(N)/sum_d max|lambda_d|/dx_d, clipped to tend - time.  Synchronous:
+ * returns dt in *dt and keeps it on the device for the following calls. */
+int aderdg_calc_time_step(aderdg_ctx *ctx, double time, double tend, double *dt);
+/* get_min_max + `recompute = 0` + save_olduh (Euler/main.f90:36-40,
+ * Euler/limiter.f90:104-171,251-265).  The per-cell pass of get_min_max is
+ * produced by the previous step's update, the 3x3 neighbourhood pass runs
+ * inside the predictor launch and olduh is a ping-pong buffer, so this call
+ * only opens the step. */
+int aderdg_limiter_begin(aderdg_ctx *ctx);
+/* ader_space_time_predictor + volume_integral over all cells
+ * (Euler/space_time_predictor.f90:8-123, Euler/volume_integral.f90:8-36,
+ * loops of Euler/main.f90:44-58). */
+int aderdg_space_time_predictor(aderdg_ctx *ctx);
+/* solve_riemann_problem + surface_integral + element_update + DMP over all
+ * cells (Euler/riemann_problem.f90:10-116, Euler/surface_integral.f90:8-39,
+ * Euler/element_update.f90:8-30, Euler/limiter.f90:179-245, loops of
+ * Euler/main.f90:62-88).  If nrecompute is not NULL the call waits and
+ * returns the number of troubled cells (`nRecompute` of Euler/main.f90). */
+int aderdg_riemann_surface_update(aderdg_ctx *ctx, int32_t *nrecompute);
+/* spread_recompute + allocate_limiter + subcell_recompute + update_limiter
+ * (Euler/limiter.f90:329-354,302-322,411-647,272-296).  Troubled cells are
+ * compacted into a work list on the device. */
+int aderdg_limiter_end(aderdg_ctx *ctx);
+/* One whole time step in the order of Euler/main.f90:24-100. */
+int aderdg_step(aderdg_ctx *ctx, double time, double tend, double *dt, int32_t *nrecompute);
+/* nsteps device-resident steps replayed from one captured hipGraph; time is
+ * advanced on the device (time = time + dt, Euler/main.f90:97).  Waits and
+ * returns the final time and the last step's dt / nRecompute. */
+int aderdg_run(aderdg_ctx *ctx, int nsteps, double time, double tend, double *time_out, double *dt_out,
+               int32_t *nrecompute_out);
+
+/* Batch form of ader_space_time_predictor with the reference argument list
+ * (Euler/space_time_predictor.f90:8-17) for ncells independent cells, host
+ * arrays in reference layout, dt given.  Used to check the kernel against the
+ * reference routine cell by cell. */
+int aderdg_predictor_cells(aderdg_ctx *ctx, int ncells, double dt, const double *luh, double *lqhi, double *lFhi,
+                           double *lqbnd, double *lFbnd);
+
+/* Kernel timing: when enabled, the kernels of each step are bracketed with
+ * hipEvents on the context stream.  Returns accumulated milliseconds and
+ * launch counts per kernel: [0] dt reduction, [1] predictor, [2] update,
+ * [3] spread/compact, [4] subcell recompute. */
+int aderdg_timing_enable(aderdg_ctx *ctx, int on);
+int aderdg_timing_get(aderdg_ctx *ctx, double ms[5], int64_t launches[5]);
+
+/* Algorithmic floating-point operations of one cell-step of the predictor
+ * launch (predictor + volume integral) and of the update launch, counted as
+ * + - * / sqrt = 1 (the convention of the oracle's op counter). */
+double aderdg_flops_predictor(int pde, int N);
+double aderdg_flops_update(int pde, int N);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
