@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "lib")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ADERDG_ARCH", "gfx950")
 
-SOURCES = ["operators.cpp", "euler_kernels.hip", "aderdg_capi.hip"]
+SOURCES = ["operators.cpp", "initial.cpp", "euler_kernels.hip", "aderdg_capi.hip"]
 COMMON = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-I" + os.path.join(ROOT, "include")]
@@ -41,18 +41,18 @@ def build_library(exact=False):
     return out
 
 
-def build_fortran_driver():
+def build_fortran_driver(exact=False):
     fc = shutil.which("flang") or shutil.which("amdflang")
     if fc is None:
         print("flang not found: Fortran driver not built", flush=True)
         return None
     fdir = os.path.join(HERE, "fortran")
-    out = os.path.join(LIB, "aderdg_euler")
+    out = os.path.join(LIB, "aderdg_euler_exact" if exact else "aderdg_euler")
     mods = os.path.join(LIB, "fmod")
     os.makedirs(mods, exist_ok=True)
     srcs = [os.path.join(fdir, f) for f in ("aderdg_hip.f90", "aderdg_main.f90")]
     _run([fc, "-O2", "-fdefault-real-8", "-J", mods] + srcs +
-         ["-L" + LIB, "-laderdg_amd", "-Wl,-rpath,$ORIGIN", "-o", out])
+         ["-L" + LIB, "-laderdg_amd_exact" if exact else "-laderdg_amd", "-Wl,-rpath,$ORIGIN", "-o", out])
     return out
 
 
@@ -60,7 +60,8 @@ def build_all():
     build_library(exact=False)
     build_library(exact=True)
     if os.path.exists(os.path.join(HERE, "fortran", "aderdg_main.f90")):
-        build_fortran_driver()
+        build_fortran_driver(False)
+        build_fortran_driver(True)
 
 
 if __name__ == "__main__":

@@ -90,6 +90,14 @@ int aderdg_build_exact(void);
 int aderdg_operators(int N, double *xi, double *wGPN, double *Kxi, double *dudx, double *iK1, double *F0,
                      double *FLcoeff, double *FRcoeff, double *uh2lim, double *lim2uh, double *uh2lob);
 
+/* Host-only initial condition at the Gauss nodes, uh(nVar,N+1,N+1,IMAX,JMAX), the
+ * point-value projection of Euler/initialize.f90:262-273.  ictype follows the
+ * reference catalogue (Euler/params.ini:17-85): 4 isentropic vortex (strength 5
+ * on (1,1,1,1), centred in the domain), 5 Lax-Liu 2-D Riemann problem 3 split at
+ * the domain centre, 11 Double Mach Reflection (Euler/initialize.f90:311-341). */
+int aderdg_initial_condition(int ictype, int N, int imax, int jmax, const double *xl, const double *xr, double gamma,
+                             double *uh);
+
 /* Operators the context uses (host copies, same layout as aderdg_ops; any pointer may be NULL) */
 int aderdg_get_ops(aderdg_ctx *ctx, double *wGPN, double *Kxi, double *iK1, double *F0, double *FLcoeff,
                    double *FRcoeff, double *uh2lim, double *lim2uh, double *uh2lob);
