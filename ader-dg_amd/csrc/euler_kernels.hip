@@ -218,8 +218,8 @@ __global__ __launch_bounds__(kBlock) void k_predictor(MeshArgs m, StepBufs b, Te
     if (TEST) return;
 
     // get_min_max step 2: 3x3 neighbourhood, Euler/limiter.f90:147-167
-    if (act && tt < V) {
-        const int v = tt, ci = cell % m.imax, cj = cell / m.imax;
+    for (int v = tt; act && v < V; v += NN) {
+        const int ci = cell % m.imax, cj = cell / m.imax;
         double mn = b.lmin_old[(size_t)v * m.ncell + cell], mx = b.lmax_old[(size_t)v * m.ncell + cell];
         for (int jj = -1; jj <= 1; jj++)
             for (int ii = -1; ii <= 1; ii++) {
@@ -490,8 +490,8 @@ __global__ __launch_bounds__(kUpdBlock) void k_update(MeshArgs m, StepBufs b)
     if (bad && !SEED) atomicOr(&sflag[lc], 1);
     __syncthreads();
     const int *st_cur = b.status_cur;
-    if (act && tt < V) {
-        const int v = tt;
+    // a cell has n^2 threads, possibly fewer than variables: stride over v
+    for (int v = tt; act && v < V; v += NN) {
         double r[6];
         group_ranges<N>(suh, slob, slim, v, r);
         if (!SEED) {  // DMP, Euler/limiter.f90:198-223 (argmax = 0)
@@ -515,7 +515,7 @@ __global__ __launch_bounds__(kUpdBlock) void k_update(MeshArgs m, StepBufs b)
         b.lmin_old[(size_t)v * m.ncell + cell] = mn;
         b.lmax_old[(size_t)v * m.ncell + cell] = mx;
     }
-    if (act && tt == V) {
+    if (act && tt == NN - 1) {
         double d = __builtin_nan("");
         for (int k = 0; k < NN; k++) d = fold_min(d, sdt[k]);
         b.cell_dt[cell] = d;
@@ -787,8 +787,7 @@ __global__ __launch_bounds__(64) void k_subcell(MeshArgs m, StepBufs b)
         const bool bad = cell_ranges_stage2<N>(op, m.gamma, sTy, sBy, sLim, sLob, lane, 64);
         if (bad) atomicOr(&sFlag, 1);
         __syncthreads();
-        if (lane < V) {
-            const int v = lane;
+        for (int v = lane; v < V; v += 64) {
             double r[6];
             group_ranges<N>(sUh, sLob, sLim, v, r);
             const double lmn = b.lmin[(size_t)v * m.ncell + cell], lmx = b.lmax[(size_t)v * m.ncell + cell];
@@ -799,8 +798,7 @@ __global__ __launch_bounds__(64) void k_subcell(MeshArgs m, StepBufs b)
         }
         __syncthreads();
         const int status = (flag == 1) ? 1 : (sFlag != 0 ? 1 : 0);
-        if (lane < V) {
-            const int v = lane;
+        for (int v = lane; v < V; v += 64) {
             double mn, mx;
             if (status) {  // next get_min_max uses Lh (:135-139)
                 mn = __builtin_nan("");
@@ -818,7 +816,7 @@ __global__ __launch_bounds__(64) void k_subcell(MeshArgs m, StepBufs b)
             b.lmin_old[(size_t)v * m.ncell + cell] = mn;
             b.lmax_old[(size_t)v * m.ncell + cell] = mx;
         }
-        if (lane == V) {
+        if (lane == 63) {
             double d = __builtin_nan("");
             for (int k = 0; k < NN; k++) d = fold_min(d, sDt[k]);
             b.cell_dt[cell] = d;

@@ -30,6 +30,7 @@ struct orc_ctx {
     int pde, N, n, V, s, imax, jmax, bc_mode;
     double xl[2], xr[2], dx[2];
     double gamma;
+    double eos[4];  /* Baer-Nunziato: GAMMA_S, GAMMA_G, PI_S, PI_G (Baer-Nunziato/initialize.f90:23-26) */
     double cflpnpm; /* CFL*PNPMTable(N), Euler/calc_time_step.f90:41 */
     double xi[NMAX], w[NMAX], mm[NMAX * NMAX], imm[NMAX * NMAX], kxi[NMAX * NMAX], dudx[NMAX * NMAX];
     double k1[NMAX * NMAX], ik1[NMAX * NMAX], f0[NMAX], flc[NMAX], frc[NMAX];
@@ -37,6 +38,7 @@ struct orc_ctx {
     double dt, time, tend;
     /* state */
     double *uh, *olduh, *qhi, *Fhi, *qbnd, *Fbnd, *duh, *Fu, *Gu;
+    double *Shi, *Fp, *Gp; /* Baer-Nunziato: time-averaged NCP source and the D+ fluctuations (Fu/Gu hold D-) */
     int *recompute, *status, *oldstatus;
     double *Lh, *NewLh, *lmin, *lmax, *lmin_old, *lmax_old;
 };
@@ -344,6 +346,213 @@ static void euler_prim2cons(double g, const double *Vp, double *Q)
     Q[3] = Vp[3] / (g - 1.0) + 0.5 * Vp[0] * (Vp[1] * Vp[1] + Vp[2] * Vp[2]);
 }
 
+
+/* ------------------------------------------------------------------ */
+/* Baer-Nunziato pointwise routines: Baer-Nunziato/pde.f90             */
+/* Q = (phi_s rho_s, .. u_s, .. v_s, phi_s E_s, phi_g rho_g, .. u_g, .. v_g, phi_g E_g, phi_s) */
+/* ------------------------------------------------------------------ */
+
+/* PDECons2Prim: Baer-Nunziato/pde.f90:8-43 */
+static void bn_cons2prim(const double *e, const double *Q, double *V)
+{
+    double phi_s = Q[8], phi_g = 1.0 - Q[8];
+    double rho_s = Q[0] / phi_s, rho_g = Q[4] / phi_g;
+    double u_s = Q[1] / Q[0], v_s = Q[2] / Q[0];
+    double u_g = Q[5] / Q[4], v_g = Q[6] / Q[4];
+    double E_s = Q[3] / phi_s, E_g = Q[7] / phi_g;
+    double p_s = (e[0] - 1.0) * (E_s - 0.5 * rho_s * (u_s * u_s + v_s * v_s)) - e[0] * e[2];
+    double p_g = (e[1] - 1.0) * (E_g - 0.5 * rho_g * (u_g * u_g + v_g * v_g)) - e[1] * e[3];
+    V[0] = rho_s; V[1] = u_s; V[2] = v_s; V[3] = p_s;
+    V[4] = rho_g; V[5] = u_g; V[6] = v_g; V[7] = p_g; V[8] = phi_s;
+    OPS(8 + 2 * 11);
+}
+
+/* PDEPrim2Cons: Baer-Nunziato/pde.f90:49-68 */
+static void bn_prim2cons(const double *e, const double *V, double *Q)
+{
+    Q[0] = V[8] * V[0];
+    Q[1] = Q[0] * V[1];
+    Q[2] = Q[0] * V[2];
+    Q[3] = Q[0] * (0.5 * (V[1] * V[1] + V[2] * V[2]) + (V[3] + e[0] * e[2]) / (V[0] * (e[0] - 1.0)));
+    Q[4] = (1.0 - V[8]) * V[4];
+    Q[5] = Q[4] * V[5];
+    Q[6] = Q[4] * V[6];
+    Q[7] = Q[4] * (0.5 * (V[5] * V[5] + V[6] * V[6]) + (V[7] + e[1] * e[3]) / (V[4] * (e[1] - 1.0)));
+    Q[8] = V[8];
+}
+
+/* PDEFlux: Baer-Nunziato/pde.f90:74-119.  F(9,2) column-major. */
+static void bn_flux(const double *e, const double *Q, double *F)
+{
+    double phi_s = Q[8], phi_g = 1.0 - Q[8];
+    double rho_s = Q[0] / phi_s, rho_g = Q[4] / phi_g;
+    double u_s = Q[1] / Q[0], v_s = Q[2] / Q[0];
+    double u_g = Q[5] / Q[4], v_g = Q[6] / Q[4];
+    double E_s = Q[3] / phi_s, E_g = Q[7] / phi_g;
+    double p_s = (e[0] - 1.0) * (E_s - 0.5 * rho_s * (u_s * u_s + v_s * v_s)) - e[0] * e[2];
+    double p_g = (e[1] - 1.0) * (E_g - 0.5 * rho_g * (u_g * u_g + v_g * v_g)) - e[1] * e[3];
+    F[0] = phi_s * rho_s * u_s;
+    F[1] = phi_s * (rho_s * u_s * u_s + p_s);
+    F[2] = phi_s * rho_s * u_s * v_s;
+    F[3] = phi_s * u_s * (E_s + p_s);
+    F[4] = phi_g * rho_g * u_g;
+    F[5] = phi_g * (rho_g * u_g * u_g + p_g);
+    F[6] = phi_g * rho_g * u_g * v_g;
+    F[7] = phi_g * u_g * (E_g + p_g);
+    F[8] = 0.0;
+    F[9] = phi_s * rho_s * v_s;
+    F[10] = phi_s * rho_s * u_s * v_s;
+    F[11] = phi_s * (rho_s * v_s * v_s + p_s);
+    F[12] = phi_s * v_s * (E_s + p_s);
+    F[13] = phi_g * rho_g * v_g;
+    F[14] = phi_g * rho_g * u_g * v_g;
+    F[15] = phi_g * (rho_g * v_g * v_g + p_g);
+    F[16] = phi_g * v_g * (E_g + p_g);
+    F[17] = 0.0;
+    OPS(2 + 8 + 2 * 9 + 2 * 3 * 8);
+}
+
+/* PDEEigenvalues: Baer-Nunziato/pde.f90:125-144 */
+static void bn_eigen(const double *e, const double *Q, const double *nv, double *L)
+{
+    double V[9];
+    bn_cons2prim(e, Q, V);
+    double u_s = V[1] * nv[0] + V[2] * nv[1];
+    double u_g = V[5] * nv[0] + V[6] * nv[1];
+    double a_s = sqrt(e[0] * (V[3] + e[2]) / V[0]);
+    double a_g = sqrt(e[1] * (V[7] + e[3]) / V[4]);
+    L[0] = u_g - a_g; L[1] = u_g + a_g; L[2] = u_s - a_s; L[3] = u_s + a_s;
+    L[4] = u_g; L[5] = u_g; L[6] = u_s; L[7] = u_s; L[8] = u_s;
+    OPS(6 + 8 + 4);
+}
+
+/* p_g and the normal solid velocity shared by PDEMatrixB and PDENCP */
+static double bn_gas_pressure(const double *e, const double *Q, double *u_s, double *v_s)
+{
+    double phi_g = 1.0 - Q[8];
+    double rho_g = Q[4] / phi_g;
+    *u_s = Q[1] / Q[0];
+    *v_s = Q[2] / Q[0];
+    double u_g = Q[5] / Q[4], v_g = Q[6] / Q[4];
+    double E_g = Q[7] / phi_g;
+    OPS(1 + 6 + 11);
+    return (e[1] - 1.0) * (E_g - 0.5 * rho_g * (u_g * u_g + v_g * v_g)) - e[1] * e[3];
+}
+
+/* PDEMatrixB: Baer-Nunziato/pde.f90:150-186.  Bn(9,9) column-major; only column 9 is non-zero. */
+static void bn_matrix_b(const double *e, const double *Q, const double *nv, double *Bn)
+{
+    double u_s, v_s;
+    for (int k = 0; k < 81; k++) Bn[k] = 0.0;
+    double phi_g = 1.0 - Q[8];
+    double rho_g = Q[4] / phi_g;
+    u_s = Q[1] / Q[0];
+    v_s = Q[2] / Q[0];
+    double u_n = nv[0] * u_s + nv[1] * v_s;
+    double u_g = Q[5] / Q[4], v_g = Q[6] / Q[4];
+    double E_g = Q[7] / phi_g;
+    double p_g = (e[1] - 1.0) * (E_g - 0.5 * rho_g * (u_g * u_g + v_g * v_g)) - e[1] * e[3];
+    Bn[1 + 9 * 8] = -nv[0] * p_g;
+    Bn[2 + 9 * 8] = -nv[1] * p_g;
+    Bn[3 + 9 * 8] = -p_g * u_n;
+    Bn[5 + 9 * 8] = nv[0] * p_g;
+    Bn[6 + 9 * 8] = nv[1] * p_g;
+    Bn[7 + 9 * 8] = p_g * u_n;
+    Bn[8 + 9 * 8] = u_n;
+    OPS(1 + 6 + 3 + 11 + 6);
+}
+
+/* PDEFusedSrcNCP = -PDENCP: Baer-Nunziato/pde.f90:277-290, 192-231.  gradQ(9,2) column-major. */
+static void bn_fused_src_ncp(const double *e, const double *Q, const double *gradQ, double *src)
+{
+    double u_s, v_s;
+    double p_g = bn_gas_pressure(e, Q, &u_s, &v_s);
+    const double *Qx = gradQ, *Qy = gradQ + 9;
+    double ncp[9];
+    ncp[0] = 0.0;
+    ncp[1] = -p_g * Qx[8];
+    ncp[2] = -p_g * Qy[8];
+    ncp[3] = -p_g * (u_s * Qx[8] + v_s * Qy[8]);
+    ncp[4] = 0.0;
+    ncp[5] = p_g * Qx[8];
+    ncp[6] = p_g * Qy[8];
+    ncp[7] = p_g * (u_s * Qx[8] + v_s * Qy[8]);
+    ncp[8] = u_s * Qx[8] + v_s * Qy[8];
+    for (int v = 0; v < 9; v++) src[v] = -ncp[v];
+    OPS(3 * 4 + 4);
+}
+
+/* RoeMatrix: Baer-Nunziato/pde.f90:296-320 (3-point Gauss-Legendre path integral) */
+static void bn_roe_matrix(const double *e, const double *Qa, const double *Qb, const double *nv, double *BRoe)
+{
+    const double sGP[3] = {0.5 - sqrt(15.0) / 10., 0.5, 0.5 + sqrt(15.) / 10.};
+    const double wGP[3] = {5. / 18.0, 8. / 18., 5. / 18.};
+    double B[81], Q[9];
+    for (int k = 0; k < 81; k++) BRoe[k] = 0.0;
+    for (int i = 0; i < 3; i++) {
+        for (int v = 0; v < 9; v++) Q[v] = Qa[v] + sGP[i] * (Qb[v] - Qa[v]);
+        bn_matrix_b(e, Q, nv, B);
+        for (int k = 0; k < 81; k++) BRoe[k] = BRoe[k] + wGP[i] * B[k];
+        OPS(9 * 3 + 7 * 2);
+    }
+}
+
+/* RusanovFlux (path-conservative): Baer-Nunziato/pde.f90:326-353 */
+static void bn_rusanov(const double *e, const double *QL, const double *FL, const double *QR, const double *FR,
+                       const double *nv, double *Dm, double *Dp)
+{
+    double LL[9], LR[9], Bn[81], temp[9];
+    bn_eigen(e, QL, nv, LL);
+    bn_eigen(e, QR, nv, LR);
+    double smax = f_max(maxabs(LL, 9), maxabs(LR, 9));
+    bn_roe_matrix(e, QL, QR, nv, Bn);
+    for (int v = 0; v < 9; v++) Dp[v] = 0.5 * (FL[v] + FR[v]) - 0.5 * smax * (QR[v] - QL[v]);
+    for (int r = 0; r < 9; r++) {
+        double acc = 0.0;
+        for (int k = 0; k < 9; k++) acc = acc + Bn[r + 9 * k] * (QR[k] - QL[k]);
+        temp[r] = acc;
+    }
+    for (int v = 0; v < 9; v++) {
+        Dm[v] = Dp[v] - 0.5 * temp[v];
+        Dp[v] = Dp[v] + 0.5 * temp[v];
+    }
+    OPS(9 * 6 + 9 * 3 + 9 * 4);
+}
+
+/* PDEAssurePositivity: Baer-Nunziato/pde.f90:237-266 (returns iErr) */
+static int bn_positivity(const double *e, const double *Q)
+{
+    double V[9];
+    int ierr = 0;
+    bn_cons2prim(e, Q, V);
+    if (V[0] <= 1.0e-12 || V[4] <= 1.0e-12) ierr = -1;
+    if ((V[3] + e[2]) <= 1.0e-12 || (V[7] + e[3]) <= 1.0e-12) ierr = -2;
+    if (V[8] <= 0.0 || V[8] >= 1.0) ierr = -3;
+    OPS(2);
+    return ierr;
+}
+
+static void pde_flux(const orc_ctx *c, const double *Q, double *F)
+{
+    if (c->pde == ORC_PDE_BN)
+        bn_flux(c->eos, Q, F);
+    else
+        euler_flux(c->gamma, Q, F);
+}
+
+static void pde_eigen(const orc_ctx *c, const double *Q, const double *nv, double *L)
+{
+    if (c->pde == ORC_PDE_BN)
+        bn_eigen(c->eos, Q, nv, L);
+    else
+        euler_eigen(c->gamma, Q, nv, L);
+}
+
+static int pde_positivity(const orc_ctx *c, const double *Q)
+{
+    return c->pde == ORC_PDE_BN ? bn_positivity(c->eos, Q) : euler_positivity(c->gamma, Q);
+}
+
 /* ------------------------------------------------------------------ */
 /* context                                                             */
 /* ------------------------------------------------------------------ */
@@ -351,17 +560,19 @@ static void euler_prim2cons(double g, const double *Vp, double *Q)
 orc_ctx *orc_create(int pde, int N, int imax, int jmax, const double xl[2], const double xr[2], int bc_mode,
                     const double *eos)
 {
-    if (pde != ORC_PDE_EULER || N < 1 || N > 9 || imax < 1 || jmax < 1) return NULL;
+    if ((pde != ORC_PDE_EULER && pde != ORC_PDE_BN) || N < 1 || N > 9 || imax < 1 || jmax < 1) return NULL;
     orc_ctx *c = (orc_ctx *)calloc(1, sizeof(orc_ctx));
     c->pde = pde;
     c->N = N;
     c->n = N + 1;
     c->s = 2 * N + 1;
-    c->V = 4;
+    c->V = pde == ORC_PDE_BN ? 9 : 4;
     c->imax = imax;
     c->jmax = jmax;
     c->bc_mode = bc_mode;
     c->gamma = eos[0];
+    if (pde == ORC_PDE_BN)
+        for (int k = 0; k < 4; k++) c->eos[k] = eos[k];
     for (int d = 0; d < 2; d++) {
         c->xl[d] = xl[d];
         c->xr[d] = xr[d];
@@ -380,6 +591,9 @@ orc_ctx *orc_create(int pde, int N, int imax, int jmax, const double xl[2], cons
     c->Fbnd = (double *)calloc(4 * V * n * nc, sizeof(double));
     c->Fu = (double *)calloc(V * n * (imax + 1) * jmax, sizeof(double));
     c->Gu = (double *)calloc(V * n * imax * (jmax + 1), sizeof(double));
+    c->Fp = (double *)calloc(V * n * (imax + 1) * jmax, sizeof(double));
+    c->Gp = (double *)calloc(V * n * imax * (jmax + 1), sizeof(double));
+    c->Shi = (double *)calloc(V * n * n * nc, sizeof(double));
     c->recompute = (int *)calloc(nc, sizeof(int));
     c->status = (int *)calloc(nc, sizeof(int));
     c->oldstatus = (int *)calloc(nc, sizeof(int));
@@ -398,6 +612,7 @@ void orc_destroy(orc_ctx *c)
     free(c->uh); free(c->olduh); free(c->qhi); free(c->Fhi); free(c->duh); free(c->qbnd); free(c->Fbnd);
     free(c->Fu); free(c->Gu); free(c->recompute); free(c->status); free(c->oldstatus); free(c->Lh);
     free(c->NewLh); free(c->lmin); free(c->lmax); free(c->lmin_old); free(c->lmax_old);
+    free(c->Fp); free(c->Gp); free(c->Shi);
     free(c);
 }
 
@@ -474,17 +689,39 @@ void orc_get_limiter(const orc_ctx *c, int *status, int *recompute, double *lh, 
 /* per-cell routines                                                    */
 /* ------------------------------------------------------------------ */
 
-void orc_flux(orc_ctx *c, const double *q, double *f) { euler_flux(c->gamma, q, f); }
-void orc_eigen(orc_ctx *c, const double *q, const double *nv, double *lam) { euler_eigen(c->gamma, q, nv, lam); }
+void orc_flux(orc_ctx *c, const double *q, double *f) { pde_flux(c, q, f); }
+void orc_eigen(orc_ctx *c, const double *q, const double *nv, double *lam) { pde_eigen(c, q, nv, lam); }
+void orc_rusanov_bn(orc_ctx *c, const double *ql, const double *fl, const double *qr, const double *fr,
+                    const double *nv, double *dm, double *dp)
+{
+    bn_rusanov(c->eos, ql, fl, qr, fr, nv, dm, dp);
+}
+void orc_ncp(orc_ctx *c, const double *q, const double *gradq, double *src) { bn_fused_src_ncp(c->eos, q, gradq, src); }
+void orc_roe(orc_ctx *c, const double *qa, const double *qb, const double *nv, double *b)
+{
+    bn_roe_matrix(c->eos, qa, qb, nv, b);
+}
+void orc_prim2cons(orc_ctx *c, const double *v, double *q)
+{
+    if (c->pde == ORC_PDE_BN)
+        bn_prim2cons(c->eos, v, q);
+    else
+        euler_prim2cons(c->gamma, v, q);
+}
 void orc_rusanov(orc_ctx *c, const double *ql, const double *fl, const double *qr, const double *fr,
                  const double *nv, double *flux)
 {
     euler_rusanov(c->gamma, ql, fl, qr, fr, nv, flux);
 }
 
-/* ader_space_time_predictor: Euler/space_time_predictor.f90:8-123 */
-void orc_predictor(orc_ctx *c, double *lqhi, double *lFhi, double *lqbnd, double *lFbnd, const double *luh)
+/* ader_space_time_predictor: Euler/space_time_predictor.f90:8-123; the Baer-Nunziato
+ * variant (Baer-Nunziato/space_time_predictor.f90:8-143) adds the gradients lqx, lqy
+ * (:72,79), the NCP source (:84-92) and its time average lShi (:111). */
+static void predictor_impl(orc_ctx *c, double *lqhi, double *lFhi, double *lShi, double *lqbnd, double *lFbnd,
+                           const double *luh)
 {
+    const int bn = c->pde == ORC_PDE_BN;
+    static double lqx[VMAX * NMAX * NMAX * NMAX], lqy[VMAX * NMAX * NMAX * NMAX], lSh[VMAX * NMAX * NMAX * NMAX];
     const int n = c->n, V = c->V;
     const double *w = c->w;
     double lqh[VMAX * NMAX * NMAX * NMAX], rhs0[VMAX * NMAX * NMAX * NMAX], rhs[VMAX * NMAX * NMAX * NMAX];
@@ -504,7 +741,7 @@ void orc_predictor(orc_ctx *c, double *lqhi, double *lFhi, double *lqbnd, double
     for (int iter = 0; iter < n; iter++) {
         for (int l = 0; l < n; l++) {
             for (int j = 0; j < n; j++)
-                for (int i = 0; i < n; i++) euler_flux(c->gamma, &lqh[Q4(0, i, j, l)], &lFh[F5(0, 0, i, j, l)]);
+                for (int i = 0; i < n; i++) pde_flux(c, &lqh[Q4(0, i, j, l)], &lFh[F5(0, 0, i, j, l)]);
             for (int j = 0; j < n; j++) { /* :65-67 */
                 double sx = w[l] * w[j] * c->dt / c->dx[0];
                 for (int i = 0; i < n; i++)
@@ -512,6 +749,11 @@ void orc_predictor(orc_ctx *c, double *lqhi, double *lFhi, double *lqbnd, double
                         double acc = 0.0;
                         for (int k = 0; k < n; k++) acc = acc + lFh[F5(v, 0, k, j, l)] * c->kxi[k + n * i];
                         rhs[Q4(v, i, j, l)] = rhs0[Q4(v, i, j, l)] - sx * acc;
+                        if (bn) { /* Baer-Nunziato/space_time_predictor.f90:72 */
+                            acc = 0.0;
+                            for (int k = 0; k < n; k++) acc = acc + lqh[Q4(v, k, j, l)] * c->dudx[i + n * k];
+                            lqx[Q4(v, i, j, l)] = 1.0 / c->dx[0] * acc;
+                        }
                     }
             }
             for (int i = 0; i < n; i++) { /* :71-73 */
@@ -521,7 +763,29 @@ void orc_predictor(orc_ctx *c, double *lqhi, double *lFhi, double *lqbnd, double
                         double acc = 0.0;
                         for (int k = 0; k < n; k++) acc = acc + lFh[F5(v, 1, i, k, l)] * c->kxi[k + n * j];
                         rhs[Q4(v, i, j, l)] = rhs[Q4(v, i, j, l)] - sy * acc;
+                        if (bn) { /* :79 */
+                            acc = 0.0;
+                            for (int k = 0; k < n; k++) acc = acc + lqh[Q4(v, i, k, l)] * c->dudx[j + n * k];
+                            lqy[Q4(v, i, j, l)] = 1.0 / c->dx[1] * acc;
+                        }
                     }
+            }
+            if (bn) { /* NCP source, Baer-Nunziato/space_time_predictor.f90:84-92 */
+                for (int j = 0; j < n; j++)
+                    for (int i = 0; i < n; i++) {
+                        double grad[18], src[9];
+                        for (int v = 0; v < V; v++) {
+                            grad[v] = lqx[Q4(v, i, j, l)];
+                            grad[9 + v] = lqy[Q4(v, i, j, l)];
+                        }
+                        bn_fused_src_ncp(c->eos, &lqh[Q4(0, i, j, l)], grad, src);
+                        const double f = w[i] * w[j] * w[l] * c->dt;
+                        for (int v = 0; v < V; v++) {
+                            lSh[Q4(v, i, j, l)] = src[v];
+                            rhs[Q4(v, i, j, l)] = rhs[Q4(v, i, j, l)] + f * src[v];
+                        }
+                    }
+                OPS(2 * V * n * n * (2 * n + 1) + n * n * (3 + 2 * V));
             }
             OPS(2 * (3 * n) + 2 * V * n * n * (2 * n + 2));
         }
@@ -548,6 +812,12 @@ void orc_predictor(orc_ctx *c, double *lqhi, double *lFhi, double *lqbnd, double
                 for (int l = 0; l < n; l++) acc = acc + lqh[Q4(v, i, j, l)] * w[l];
                 lqhi[v + V * (i + n * j)] = acc;
             }
+            if (bn)
+                for (int v = 0; v < V; v++) { /* Baer-Nunziato/space_time_predictor.f90:111 */
+                    double acc = 0.0;
+                    for (int l = 0; l < n; l++) acc = acc + lSh[Q4(v, i, j, l)] * w[l];
+                    lShi[v + V * (i + n * j)] = acc;
+                }
             for (int d = 0; d < 2; d++)
                 for (int v = 0; v < V; v++) {
                     double acc = 0.0;
@@ -555,7 +825,7 @@ void orc_predictor(orc_ctx *c, double *lqhi, double *lFhi, double *lqbnd, double
                     lFhi[v + V * (d + 2 * (i + n * j))] = acc;
                 }
         }
-    OPS(3 * V * n * n * 2 * n);
+    OPS((bn ? 4 : 3) * V * n * n * 2 * n);
     /* :100-119 boundary extrapolation */
     for (int j = 0; j < n; j++)
         for (int v = 0; v < V; v++) {
@@ -590,8 +860,24 @@ void orc_predictor(orc_ctx *c, double *lqhi, double *lFhi, double *lqbnd, double
 #undef F5
 }
 
-/* volume_integral: Euler/volume_integral.f90:8-36 */
-void orc_volume(orc_ctx *c, double *lduh, const double *lFhi)
+void orc_predictor(orc_ctx *c, double *lqhi, double *lFhi, double *lqbnd, double *lFbnd, const double *luh)
+{
+    double shi[VMAX * NMAX * NMAX];
+    predictor_impl(c, lqhi, lFhi, shi, lqbnd, lFbnd, luh);
+}
+
+void orc_predictor_bn(orc_ctx *c, double *lqhi, double *lFhi, double *lShi, double *lqbnd, double *lFbnd,
+                      const double *luh)
+{
+    predictor_impl(c, lqhi, lFhi, lShi, lqbnd, lFbnd, luh);
+}
+
+/* volume_integral: Euler/volume_integral.f90:8-36; Baer-Nunziato adds w_i w_j lShi
+ * (Baer-Nunziato/volume_integral.f90:36-40) when lShi is given */
+static void volume_impl(orc_ctx *c, double *lduh, const double *lFhi, const double *lShi);
+void orc_volume(orc_ctx *c, double *lduh, const double *lFhi) { volume_impl(c, lduh, lFhi, NULL); }
+void orc_volume_bn(orc_ctx *c, double *lduh, const double *lFhi, const double *lShi) { volume_impl(c, lduh, lFhi, lShi); }
+static void volume_impl(orc_ctx *c, double *lduh, const double *lFhi, const double *lShi)
 {
     const int n = c->n, V = c->V;
     for (int k = 0; k < V * n * n; k++) lduh[k] = 0.0;
@@ -609,7 +895,12 @@ void orc_volume(orc_ctx *c, double *lduh, const double *lFhi)
                 for (int k = 0; k < n; k++) acc = acc + lFhi[v + V * (1 + 2 * (i + n * k))] * c->kxi[j + n * k];
                 lduh[v + V * (i + n * j)] = lduh[v + V * (i + n * j)] + acc * c->w[i] / c->dx[1];
             }
-    OPS(2 * V * n * n * (2 * n + 3));
+    if (lShi)
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < n; i++)
+                for (int v = 0; v < V; v++)
+                    lduh[v + V * (i + n * j)] = lduh[v + V * (i + n * j)] + c->w[i] * c->w[j] * lShi[v + V * (i + n * j)];
+    OPS(2 * V * n * n * (2 * n + 3) + (lShi ? 3 * V * n * n : 0));
 }
 
 /* surface_integral: Euler/surface_integral.f90:8-39 */
@@ -745,7 +1036,7 @@ int orc_dmp(orc_ctx *c, const double *luh, const double *lmin, const double *lma
         if (mx > lmax[v] + ldiff) return 0;
     }
     for (int k = 0; k < s * s; k++)
-        if (euler_positivity(c->gamma, &lim[V * k]) < 0) return 0;
+        if (pde_positivity(c, &lim[V * k]) < 0) return 0;
     return 1;
 }
 
@@ -764,7 +1055,9 @@ double orc_dmp_margin(orc_ctx *c, const double *luh, const double *lmin, const d
         if (a < margin) margin = a;
         if (b < margin) margin = b;
     }
-    for (int k = 0; k < s * s; k++) {
+    for (int k = 0; k < s * s && c->pde == ORC_PDE_BN; k++)
+        if (bn_positivity(c->eos, &lim[V * k]) < 0 && margin > -1.0) margin = -1.0;
+    for (int k = 0; k < s * s && c->pde == ORC_PDE_EULER; k++) {
         const double *Q = &lim[V * k];
         double irho = 1.0 / Q[0];
         double p = (c->gamma - 1.0) * (Q[3] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2]) * irho);
@@ -805,11 +1098,11 @@ double orc_calc_dt(orc_ctx *c)
             const double *u = UH(c, c->uh, ci, cj);
             for (int k = 0; k < n; k++)
                 for (int l = 0; l < n; l++) {
-                    double L[4], denom = 0.0;
-                    euler_eigen(c->gamma, &u[V * (k + n * l)], nvx, L);
-                    denom = denom + maxabs(L, 4) / c->dx[0];
-                    euler_eigen(c->gamma, &u[V * (k + n * l)], nvy, L);
-                    denom = denom + maxabs(L, 4) / c->dx[1];
+                    double L[VMAX], denom = 0.0;
+                    pde_eigen(c, &u[V * (k + n * l)], nvx, L);
+                    denom = denom + maxabs(L, V) / c->dx[0];
+                    pde_eigen(c, &u[V * (k + n * l)], nvy, L);
+                    denom = denom + maxabs(L, V) / c->dx[1];
                     dt = f_min(dt, c->cflpnpm / denom);
                     OPS(5);
                 }
@@ -1174,7 +1467,199 @@ static void subcell_recompute(orc_ctx *c)
     free(subuh0); free(slx); free(sly); free(wLx); free(wRx); free(wLy); free(wRy); free(Fx); free(Fy);
 }
 
-/* one pass of the time loop body, Euler/main.f90:28-100 */
+
+/* solve_riemann_problem of Baer-Nunziato/riemann_problem.f90:9-80: transmissive on all
+ * sides; D- into Fu/Gu (Fm/Gm of the reference) and D+ into Fp/Gp.  The reference
+ * allocates Gm with the x-face shape (Baer-Nunziato/initialize.f90:136); the arrays here
+ * have the y-face shape, which keeps every element the reference reads and writes. */
+static void riemann_bn(orc_ctx *c)
+{
+    const int n = c->n, V = c->V, I = c->imax, J = c->jmax;
+    double QL[9], QR[9], FL[9], FR[9];
+    double nv[2] = {1.0, 0.0};
+    for (int j = 0; j < J; j++)
+        for (int i = 0; i <= I; i++)
+            for (int l = 0; l < n; l++) {
+                const int cl = i == 0 ? 0 : i - 1, fl = i == 0 ? 0 : 1;
+                const int cr = i == I ? I - 1 : i, fr = i == I ? 1 : 0;
+                for (int v = 0; v < V; v++) {
+                    QL[v] = BND(c, c->qbnd, cl, j)[v + V * (fl + 4 * l)];
+                    FL[v] = BND(c, c->Fbnd, cl, j)[v + V * (fl + 4 * l)];
+                    QR[v] = BND(c, c->qbnd, cr, j)[v + V * (fr + 4 * l)];
+                    FR[v] = BND(c, c->Fbnd, cr, j)[v + V * (fr + 4 * l)];
+                }
+                const size_t o = (size_t)V * (l + n * (i + (I + 1) * j));
+                bn_rusanov(c->eos, QL, FL, QR, FR, nv, &c->Fu[o], &c->Fp[o]);
+            }
+    nv[0] = 0.0;
+    nv[1] = 1.0;
+    for (int j = 0; j <= J; j++)
+        for (int i = 0; i < I; i++)
+            for (int k = 0; k < n; k++) {
+                const int cl = j == 0 ? 0 : j - 1, fl = j == 0 ? 2 : 3;
+                const int cr = j == J ? J - 1 : j, fr = j == J ? 3 : 2;
+                for (int v = 0; v < V; v++) {
+                    QL[v] = BND(c, c->qbnd, i, cl)[v + V * (fl + 4 * k)];
+                    FL[v] = BND(c, c->Fbnd, i, cl)[v + V * (fl + 4 * k)];
+                    QR[v] = BND(c, c->qbnd, i, cr)[v + V * (fr + 4 * k)];
+                    FR[v] = BND(c, c->Fbnd, i, cr)[v + V * (fr + 4 * k)];
+                }
+                const size_t o = (size_t)V * (k + n * (i + I * j));
+                bn_rusanov(c->eos, QL, FL, QR, FR, nv, &c->Gu[o], &c->Gp[o]);
+            }
+}
+
+/* RusanovFluxS of Baer-Nunziato/limiter.f90:359-383: flux and Roe fluctuation */
+static void bn_rusanov_s(const double *e, double *F, double *D, const double *QL, const double *QR, const double *nv)
+{
+    double LL[9], LR[9], FL[18], FR[18], B[81];
+    bn_eigen(e, QL, nv, LL);
+    bn_eigen(e, QR, nv, LR);
+    double smax = f_max(maxabs(LL, 9), maxabs(LR, 9));
+    bn_flux(e, QL, FL);
+    bn_flux(e, QR, FR);
+    for (int v = 0; v < 9; v++) {
+        double m = 0.0;
+        m = m + (FL[v] + FR[v]) * nv[0];
+        m = m + (FL[v + 9] + FR[v + 9]) * nv[1];
+        F[v] = 0.5 * m - 0.5 * smax * (QR[v] - QL[v]);
+    }
+    bn_roe_matrix(e, QL, QR, nv, B);
+    for (int r = 0; r < 9; r++) {
+        double acc = 0.0;
+        for (int k = 0; k < 9; k++) acc = acc + B[r + 9 * k] * (QR[k] - QL[k]);
+        D[r] = acc;
+    }
+    OPS(9 * 11 + 9 * 2);
+}
+
+/* subcell_recompute of Baer-Nunziato/limiter.f90:416-596: MUSCL-Hancock with the NCP
+ * source and Roe fluctuations; slopes also on the halo ring (:486-492), no boundary
+ * overrides (the neighbour table's clamp gives the ghost subcells). */
+static void subcell_recompute_bn(orc_ctx *c)
+{
+    const int V = c->V, s = c->s, I = c->imax, J = c->jmax;
+    const int W = 3 * s, H = s + 2;
+    const double ldx[2] = {c->dx[0] / (double)s, c->dx[1] / (double)s};
+    const double nvx[2] = {1.0, 0.0}, nvy[2] = {0.0, 1.0};
+    double *subuh0 = (double *)malloc(sizeof(double) * V * W * W);
+    double *slx = (double *)calloc((size_t)V * H * H, sizeof(double));
+    double *sly = (double *)calloc((size_t)V * H * H, sizeof(double));
+    double *slt = (double *)calloc((size_t)V * H * H, sizeof(double));
+    double *wLx = (double *)calloc((size_t)V * H * H, sizeof(double));
+    double *wRx = (double *)calloc((size_t)V * H * H, sizeof(double));
+    double *wLy = (double *)calloc((size_t)V * H * H, sizeof(double));
+    double *wRy = (double *)calloc((size_t)V * H * H, sizeof(double));
+    double lim[VMAX * SMAX * SMAX], sub1[VMAX * SMAX * SMAX];
+#define S0(v, ii, jj) subuh0[(v) + V * (((ii) + s - 1) + W * ((jj) + s - 1))]
+#define HW(a, v, ii, jj) (a)[(v) + V * ((ii) + H * (jj))]
+    for (int cj = 0; cj < J; cj++)
+        for (int ci = 0; ci < I; ci++) {
+            size_t k = (size_t)ci + (size_t)I * cj;
+            if (c->recompute[k] <= 0) continue;
+            for (int jj = -1; jj <= 1; jj++)
+                for (int ii = -1; ii <= 1; ii++) {
+                    int in, jn;
+                    neighbor(c, ci, cj, ii, jj, &in, &jn);
+                    size_t kn = (size_t)in + (size_t)I * jn;
+                    const double *src;
+                    if (c->oldstatus[kn] == 0) {
+                        orc_subcell_data(c, lim, UH(c, c->olduh, in, jn));
+                        src = lim;
+                    } else {
+                        src = &c->Lh[(size_t)V * s * s * kn];
+                    }
+                    for (int q = 1; q <= s; q++)
+                        for (int p = 1; p <= s; p++)
+                            for (int v = 0; v < V; v++) S0(v, p + ii * s, q + jj * s) = src[v + V * ((p - 1) + s * (q - 1))];
+                }
+            /* :486-513 slopes, NCP source, half step on interior and halo ring */
+            for (int jj = 0; jj <= s + 1; jj++)
+                for (int ii = 0; ii <= s + 1; ii++) {
+                    double a[VMAX], b[VMAX], grad[18], src[9], FLx[18], FRx[18], FLy[18], FRy[18], Qt[VMAX];
+                    for (int v = 0; v < V; v++) {
+                        a[v] = S0(v, ii + 1, jj) - S0(v, ii, jj);
+                        b[v] = S0(v, ii, jj) - S0(v, ii - 1, jj);
+                    }
+                    minmod(&HW(slx, 0, ii, jj), a, b, V);
+                    for (int v = 0; v < V; v++) {
+                        a[v] = S0(v, ii, jj + 1) - S0(v, ii, jj);
+                        b[v] = S0(v, ii, jj) - S0(v, ii, jj - 1);
+                    }
+                    minmod(&HW(sly, 0, ii, jj), a, b, V);
+                    for (int v = 0; v < V; v++) {
+                        grad[v] = HW(slx, v, ii, jj) / ldx[0];
+                        grad[9 + v] = HW(sly, v, ii, jj) / ldx[1];
+                    }
+                    double q0[VMAX];
+                    for (int v = 0; v < V; v++) q0[v] = S0(v, ii, jj);
+                    bn_fused_src_ncp(c->eos, q0, grad, src);
+                    for (int v = 0; v < V; v++) {
+                        HW(wLx, v, ii, jj) = S0(v, ii, jj) - 0.5 * HW(slx, v, ii, jj);
+                        HW(wRx, v, ii, jj) = S0(v, ii, jj) + 0.5 * HW(slx, v, ii, jj);
+                    }
+                    bn_flux(c->eos, &HW(wLx, 0, ii, jj), FLx);
+                    bn_flux(c->eos, &HW(wRx, 0, ii, jj), FRx);
+                    for (int v = 0; v < V; v++) Qt[v] = src[v] - (FRx[v] - FLx[v]) / ldx[0];
+                    for (int v = 0; v < V; v++) {
+                        HW(wLy, v, ii, jj) = S0(v, ii, jj) - 0.5 * HW(sly, v, ii, jj);
+                        HW(wRy, v, ii, jj) = S0(v, ii, jj) + 0.5 * HW(sly, v, ii, jj);
+                    }
+                    bn_flux(c->eos, &HW(wLy, 0, ii, jj), FLy);
+                    bn_flux(c->eos, &HW(wRy, 0, ii, jj), FRy);
+                    for (int v = 0; v < V; v++) Qt[v] = Qt[v] - (FRy[v + 9] - FLy[v + 9]) / ldx[1];
+                    for (int v = 0; v < V; v++) {
+                        HW(slt, v, ii, jj) = Qt[v];
+                        HW(wLx, v, ii, jj) = HW(wLx, v, ii, jj) + 0.5 * c->dt * Qt[v];
+                        HW(wRx, v, ii, jj) = HW(wRx, v, ii, jj) + 0.5 * c->dt * Qt[v];
+                        HW(wLy, v, ii, jj) = HW(wLy, v, ii, jj) + 0.5 * c->dt * Qt[v];
+                        HW(wRy, v, ii, jj) = HW(wRy, v, ii, jj) + 0.5 * c->dt * Qt[v];
+                    }
+                    OPS(V * (4 + 2 + 4 + 5 + 12));
+                }
+            /* :516-537 fluxes, fluctuations and the finite-volume update */
+            for (int jj = 1; jj <= s; jj++)
+                for (int ii = 1; ii <= s; ii++) {
+                    double grad[18], src[9], qh[VMAX], fpl[9], fmi[9], gpl[9], gmi[9], Dpx[9], Dmx[9], Dpy[9], Dmy[9];
+                    for (int v = 0; v < V; v++) {
+                        grad[v] = HW(slx, v, ii, jj) / ldx[0];
+                        grad[9 + v] = HW(sly, v, ii, jj) / ldx[1];
+                        qh[v] = S0(v, ii, jj) + 0.5 * c->dt * HW(slt, v, ii, jj);
+                    }
+                    bn_fused_src_ncp(c->eos, qh, grad, src);
+                    bn_rusanov_s(c->eos, fpl, Dpx, &HW(wRx, 0, ii, jj), &HW(wLx, 0, ii + 1, jj), nvx);
+                    bn_rusanov_s(c->eos, fmi, Dmx, &HW(wRx, 0, ii - 1, jj), &HW(wLx, 0, ii, jj), nvx);
+                    bn_rusanov_s(c->eos, gpl, Dpy, &HW(wRy, 0, ii, jj), &HW(wLy, 0, ii, jj + 1), nvy);
+                    bn_rusanov_s(c->eos, gmi, Dmy, &HW(wRy, 0, ii, jj - 1), &HW(wLy, 0, ii, jj), nvy);
+                    for (int v = 0; v < V; v++)
+                        sub1[v + V * ((ii - 1) + s * (jj - 1))] =
+                            S0(v, ii, jj) - c->dt / ldx[0] * (fpl[v] - fmi[v]) - c->dt / ldx[1] * (gpl[v] - gmi[v]) -
+                            c->dt / ldx[0] * 0.5 * (Dpx[v] + Dmx[v]) - c->dt / ldx[1] * 0.5 * (Dpy[v] + Dmy[v]) +
+                            c->dt * src[v];
+                    OPS(V * (2 + 3 + 20));
+                }
+            double *u = UH(c, c->uh, ci, cj);
+            double *nl = &c->NewLh[(size_t)V * s * s * k];
+            if (c->recompute[k] == 1) {
+                c->status[k] = 1;
+                memcpy(nl, sub1, sizeof(double) * V * s * s);
+                orc_put_subcell_data(c, u, nl);
+            } else if (c->recompute[k] == 2) {
+                orc_put_subcell_data(c, u, sub1);
+                if (orc_dmp(c, u, &c->lmin[V * k], &c->lmax[V * k], 0.0)) {
+                    c->status[k] = 0;
+                } else {
+                    c->status[k] = 1;
+                    memcpy(nl, sub1, sizeof(double) * V * s * s);
+                }
+            }
+        }
+#undef S0
+#undef HW
+    free(subuh0); free(slx); free(sly); free(slt); free(wLx); free(wRx); free(wLy); free(wRy);
+}
+
+/* one pass of the time loop body, Euler/main.f90:28-100 (Baer-Nunziato/main.f90:28-100) */
 int orc_step(orc_ctx *c)
 {
     const int V = c->V, n = c->n, s = c->s, I = c->imax, J = c->jmax;
@@ -1187,16 +1672,22 @@ int orc_step(orc_ctx *c)
     for (int cj = 0; cj < J; cj++)
         for (int ci = 0; ci < I; ci++) {
             size_t k = (size_t)ci + (size_t)I * cj;
-            orc_predictor(c, &c->qhi[blk * k], &c->Fhi[2 * blk * k], BND(c, c->qbnd, ci, cj), BND(c, c->Fbnd, ci, cj),
-                          UH(c, c->uh, ci, cj));
+            predictor_impl(c, &c->qhi[blk * k], &c->Fhi[2 * blk * k], &c->Shi[blk * k], BND(c, c->qbnd, ci, cj),
+                           BND(c, c->Fbnd, ci, cj), UH(c, c->uh, ci, cj));
         }
-    for (size_t k = 0; k < nc; k++) orc_volume(c, &c->duh[blk * k], &c->Fhi[2 * blk * k]);
-    riemann(c);
+    const int bn = c->pde == ORC_PDE_BN;
+    for (size_t k = 0; k < nc; k++) volume_impl(c, &c->duh[blk * k], &c->Fhi[2 * blk * k], bn ? &c->Shi[blk * k] : NULL);
+    if (bn)
+        riemann_bn(c);
+    else
+        riemann(c);
+    /* BN: surface_integral(duh, Fm(i), Fp(i+1), Gm(j), Gp(j+1)), Baer-Nunziato/main.f90:66-70 */
+    const double *FR_ = bn ? c->Fp : c->Fu, *GT_ = bn ? c->Gp : c->Gu;
     for (int cj = 0; cj < J; cj++)
         for (int ci = 0; ci < I; ci++) {
             size_t k = (size_t)ci + (size_t)I * cj;
-            orc_surface(c, &c->duh[blk * k], &c->Fu[V * n * (ci + (I + 1) * cj)], &c->Fu[V * n * (ci + 1 + (I + 1) * cj)],
-                        &c->Gu[V * n * (ci + I * cj)], &c->Gu[V * n * (ci + I * (cj + 1))]);
+            orc_surface(c, &c->duh[blk * k], &c->Fu[V * n * (ci + (I + 1) * cj)], &FR_[V * n * (ci + 1 + (I + 1) * cj)],
+                        &c->Gu[V * n * (ci + I * cj)], &GT_[V * n * (ci + I * (cj + 1))]);
         }
     for (size_t k = 0; k < nc; k++) {
         orc_update(c, &c->uh[blk * k], &c->duh[blk * k]);
@@ -1219,7 +1710,10 @@ int orc_step(orc_ctx *c)
         }
     /* allocate_limiter: Euler/limiter.f90:302-322 */
     for (size_t k = 0; k < nc; k++) c->status[k] = c->recompute[k] == 1 ? 1 : 0;
-    subcell_recompute(c);
+    if (bn)
+        subcell_recompute_bn(c);
+    else
+        subcell_recompute(c);
     /* update_limiter: Euler/limiter.f90:272-296 */
     for (size_t k = 0; k < nc; k++) {
         if (c->status[k] != 0) memcpy(&c->Lh[(size_t)V * s * s * k], &c->NewLh[(size_t)V * s * s * k], sizeof(double) * V * s * s);

@@ -59,6 +59,16 @@ int orc_dmp(orc_ctx *c, const double *luh, const double *lmin, const double *lma
 /* signed DMP margin (negative = troubled), used for the near-threshold rule */
 double orc_dmp_margin(orc_ctx *c, const double *luh, const double *lmin, const double *lmax, double argmax);
 
+/* Baer-Nunziato (pde = ORC_PDE_BN, eos = GAMMA_S, GAMMA_G, PI_S, PI_G) */
+void orc_predictor_bn(orc_ctx *c, double *lqhi, double *lFhi, double *lShi, double *lqbnd, double *lFbnd,
+                      const double *luh);
+void orc_volume_bn(orc_ctx *c, double *lduh, const double *lFhi, const double *lShi);
+void orc_rusanov_bn(orc_ctx *c, const double *ql, const double *fl, const double *qr, const double *fr,
+                    const double *nv, double *dm, double *dp);
+void orc_ncp(orc_ctx *c, const double *q, const double *gradq, double *src);  /* PDEFusedSrcNCP */
+void orc_roe(orc_ctx *c, const double *qa, const double *qb, const double *nv, double *b);
+void orc_prim2cons(orc_ctx *c, const double *v, double *q);
+
 /* whole-mesh phases */
 double orc_calc_dt(orc_ctx *c);
 int orc_step(orc_ctx *c);                       /* returns nRecompute */

@@ -76,3 +76,16 @@ EULER_DMR_GUARD=(
 for n in 1 2 3 4; do
     build_variant euler Euler "$n" ref_harness_euler.f90 "${EULER_DMR_GUARD[@]}"
 done
+
+# Baer-Nunziato: two memory-safety fixes.  Gm is allocated with the x-face shape
+# (Baer-Nunziato/initialize.f90:136) but written up to JMAX+1 in its last index
+# (Baer-Nunziato/riemann_problem.f90:72), a heap overflow; it gets the y-face shape of
+# Gp.  Every element the reference reads keeps its value.  The troubled-cell seeding at
+# t=0 uses the uninitialised x0 (:288); it gets the cell corner.
+BN_FIXES=(
+  'initialize.f90|s/allocate \( Gm\(nVar, nDOF\(2\), IMAX\+1 , JMAX \) \)/allocate ( Gm(nVar, nDOF(1), IMAX , JMAX+1 ) )/'
+  'initialize.f90|s#xGP = x0 \+ \(/ xilimbary#xGP = corner + (/ xilimbary#'
+)
+for n in 1 2 3 4; do
+    build_variant bn Baer-Nunziato "$n" ref_harness_bn.f90 "${BN_FIXES[@]}"
+done
