@@ -107,3 +107,47 @@ def cell_averages(uh, w):
         for ii in range(n):
             q = q + w[ii] * w[jj] * uh[:, ii, jj]
     return q
+
+
+# ---- Baer-Nunziato --------------------------------------------------------
+
+BN_DENSITY = (1000.0, 1.0)   # rho_s, rho_g on both sides of the interface
+BN_P_OUT = 1.0e5             # pressure outside the circle (both phases)
+BN_PHI_EPS = 1.0e-6          # volume fraction of the vanishing phase
+
+
+def bn_prim2cons(eos, V):
+    """PDEPrim2Cons, Baer-Nunziato/pde.f90:49-68; V = (rho_s,u_s,v_s,p_s,rho_g,u_g,v_g,p_g,phi_s)."""
+    gs, gg, ps, pg = eos
+    Q = np.empty_like(V)
+    Q[0] = V[8] * V[0]
+    Q[1] = Q[0] * V[1]
+    Q[2] = Q[0] * V[2]
+    Q[3] = Q[0] * (0.5 * (V[1] * V[1] + V[2] * V[2]) + (V[3] + gs * ps) / (V[0] * (gs - 1.0)))
+    Q[4] = (1.0 - V[8]) * V[4]
+    Q[5] = Q[4] * V[5]
+    Q[6] = Q[4] * V[6]
+    Q[7] = Q[4] * (0.5 * (V[5] * V[5] + V[6] * V[6]) + (V[7] + gg * pg) / (V[4] * (gg - 1.0)))
+    Q[8] = V[8]
+    return Q
+
+
+def bn_circle(X, Y, eos, radius=0.2, pratio=1.0e3, xc=0.5, yc=0.5):
+    """Circular material interface at rest: solid-dominated (phi_s = 1 - 1e-6) inside the
+    circle at pressure pratio * 1e5, gas-dominated (phi_s = 1e-6) outside at 1e5, both
+    phases at rest with densities BN_DENSITY.  Not at reference HEAD."""
+    inside = (X - xc) ** 2 + (Y - yc) ** 2 <= radius * radius
+    V = np.zeros((9,) + X.shape)
+    V[0] = BN_DENSITY[0]
+    V[4] = BN_DENSITY[1]
+    p = np.where(inside, pratio * BN_P_OUT, BN_P_OUT)
+    V[3] = p
+    V[7] = p
+    V[8] = np.where(inside, 1.0 - BN_PHI_EPS, BN_PHI_EPS)
+    return bn_prim2cons(eos, V)
+
+
+def initial_uh_bn(xi, imax, jmax, xl, xr, eos, **kw):
+    """uh(9, N+1, N+1, IMAX, JMAX) of the circular interface problem, Fortran order."""
+    X, Y = node_coordinates(xi, imax, jmax, xl, xr)
+    return np.asfortranarray(bn_circle(np.asarray(X), np.asarray(Y), eos, **kw))

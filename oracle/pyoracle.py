@@ -67,18 +67,21 @@ class Oracle:
             for name in ("orc_set_uh", "orc_get_uh", "orc_get_scalars", "orc_set_limiter", "orc_get_limiter",
                          "orc_predictor", "orc_volume", "orc_surface", "orc_update", "orc_flux", "orc_eigen",
                          "orc_rusanov", "orc_subcell_data", "orc_lobatto_data", "orc_put_subcell_data",
-                         "orc_get_ops", "orc_get_face_fluxes"):
+                         "orc_get_ops", "orc_get_face_fluxes", "orc_predictor_bn", "orc_volume_bn",
+                         "orc_rusanov_bn", "orc_ncp", "orc_roe", "orc_prim2cons"):
                 getattr(lib, name).restype = None
             cls._lib = lib
         return cls._lib
 
-    def __init__(self, N, imax, jmax, xl, xr, bc_mode, gamma=1.4, pde=0):
-        self.N, self.n, self.s, self.V = N, N + 1, 2 * N + 1, 4
+    def __init__(self, N, imax, jmax, xl, xr, bc_mode, gamma=1.4, pde=0, eos=None):
+        """pde 0: Euler (gamma); pde 1: Baer-Nunziato (eos = GAMMA_S, GAMMA_G, PI_S, PI_G)."""
+        self.pde = pde
+        self.N, self.n, self.s, self.V = N, N + 1, 2 * N + 1, (9 if pde == 1 else 4)
         self.imax, self.jmax = imax, jmax
         L = self.lib()
         xl = np.asarray(xl, dtype=np.float64)
         xr = np.asarray(xr, dtype=np.float64)
-        eos = np.array([gamma], dtype=np.float64)
+        eos = np.array([gamma] if pde == 0 else list(eos), dtype=np.float64)
         self.h = L.orc_create(pde, N, imax, jmax, _p(xl), _p(xr), bc_mode, _p(eos))
         if not self.h:
             raise ValueError("orc_create failed")
@@ -157,15 +160,47 @@ class Oracle:
         luh = np.asfortranarray(luh, dtype=np.float64)
         out = dict(qhi=np.zeros((V, n, n), order="F"), Fhi=np.zeros((V, 2, n, n), order="F"),
                    qbnd=np.zeros((V, 4, n), order="F"), Fbnd=np.zeros((V, 4, n), order="F"))
-        self.lib().orc_predictor(ctypes.c_void_p(self.h), _p(out["qhi"]), _p(out["Fhi"]), _p(out["qbnd"]),
-                                 _p(out["Fbnd"]), _p(luh))
+        if self.pde == 1:
+            out["Shi"] = np.zeros((V, n, n), order="F")
+            self.lib().orc_predictor_bn(ctypes.c_void_p(self.h), _p(out["qhi"]), _p(out["Fhi"]), _p(out["Shi"]),
+                                        _p(out["qbnd"]), _p(out["Fbnd"]), _p(luh))
+        else:
+            self.lib().orc_predictor(ctypes.c_void_p(self.h), _p(out["qhi"]), _p(out["Fhi"]), _p(out["qbnd"]),
+                                     _p(out["Fbnd"]), _p(luh))
         return out
 
-    def volume(self, Fhi):
+    def volume(self, Fhi, Shi=None):
         V, n = self.V, self.n
         d = np.zeros((V, n, n), order="F")
-        self.lib().orc_volume(ctypes.c_void_p(self.h), _p(d), _p(np.asfortranarray(Fhi)))
+        if Shi is not None:
+            self.lib().orc_volume_bn(ctypes.c_void_p(self.h), _p(d), _p(np.asfortranarray(Fhi)),
+                                     _p(np.asfortranarray(Shi)))
+        else:
+            self.lib().orc_volume(ctypes.c_void_p(self.h), _p(d), _p(np.asfortranarray(Fhi)))
         return d
+
+    def rusanov_bn(self, ql, fl, qr, fr, nv):
+        dm, dp = np.zeros(self.V), np.zeros(self.V)
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (ql, fl, qr, fr, nv)]
+        self.lib().orc_rusanov_bn(ctypes.c_void_p(self.h), *[_p(x) for x in a], _p(dm), _p(dp))
+        return dm, dp
+
+    def ncp(self, q, gradq):
+        out = np.zeros(self.V)
+        self.lib().orc_ncp(ctypes.c_void_p(self.h), _p(np.ascontiguousarray(q, dtype=np.float64)),
+                           _p(np.asfortranarray(gradq, dtype=np.float64)), _p(out))
+        return out
+
+    def roe(self, qa, qb, nv):
+        b = np.zeros((self.V, self.V), order="F")
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (qa, qb, nv)]
+        self.lib().orc_roe(ctypes.c_void_p(self.h), *[_p(x) for x in a], _p(b))
+        return b
+
+    def prim2cons(self, v):
+        q = np.zeros(self.V)
+        self.lib().orc_prim2cons(ctypes.c_void_p(self.h), _p(np.ascontiguousarray(v, dtype=np.float64)), _p(q))
+        return q
 
     def surface(self, lduh, fl, fr, gb, gt):
         d = np.array(lduh, order="F", dtype=np.float64)
@@ -410,3 +445,101 @@ class RefEuler:
         a = [np.asfortranarray(x, dtype=np.float64) for x in (luh, lmin, lmax)]
         self.lib_.ref_dmp(ctypes.byref(r), *[_p(x) for x in a], argmax)
         return bool(r.value)
+
+
+BN_PARAMS_TEMPLATE = """{gs}   ! GAMMA_S
+{gg}   ! GAMMA_G
+{ps}   ! PI_S
+{pg}   ! PI_G
+{imax}   ! IMAX
+{jmax}   ! JMAX
+{xl0}   ! xL(1)
+{xl1}   ! xL(2)
+{xr0}   ! xR(1)
+{xr1}   ! xR(2)
+1   ! bL
+1   ! bR
+1   ! bB
+1   ! bT
+1   ! IC
+{tend}   ! tend
+{wi}   ! WriteInterval
+"""
+
+
+class RefBN(RefEuler):
+    """The reference Baer-Nunziato routines (oracle/_ref/libref_bn_n<N>.so); transmissive
+    boundaries (Baer-Nunziato/riemann_problem.f90:9-80)."""
+
+    _libs = {}
+
+    @classmethod
+    def available(cls, N):
+        return os.path.exists(ref_path("bn", N))
+
+    @classmethod
+    def lib(cls, N):
+        if N not in cls._libs:
+            lib = ctypes.CDLL(ref_path("bn", N))
+            lib.ref_set_scalars.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_int]
+            lib.ref_dmp.argtypes = [ctypes.POINTER(ctypes.c_int), _dp, _dp, _dp, ctypes.c_double]
+            cls._libs[N] = lib
+        return cls._libs[N]
+
+    def __init__(self, N, imax, jmax, xl, xr, eos, tend=1.0e9):
+        self.N, self.n, self.s, self.V = N, N + 1, 2 * N + 1, 9
+        self.imax, self.jmax = imax, jmax
+        self.bc_mode = BC_TRANSMISSIVE
+        self.eos = np.array(eos, dtype=np.float64)
+        L = self.lib(N)
+        txt = BN_PARAMS_TEMPLATE.format(gs=repr(float(eos[0])), gg=repr(float(eos[1])), ps=repr(float(eos[2])),
+                                        pg=repr(float(eos[3])), imax=imax, jmax=jmax, xl0=repr(float(xl[0])),
+                                        xl1=repr(float(xl[1])), xr0=repr(float(xr[0])), xr1=repr(float(xr[1])),
+                                        tend=repr(float(tend)), wi=10 ** 8)
+        cwd = os.getcwd()
+        with tempfile.TemporaryDirectory() as d:
+            with open(os.path.join(d, "baer_nunziato_params.ini"), "w") as f:
+                f.write(txt)
+            os.chdir(d)
+            try:
+                L.ref_init()
+            finally:
+                os.chdir(cwd)
+        L.ref_reset_limiter()
+        L.ref_set_scalars(0.0, 0.0, float(tend), _p(self.eos), BC_TRANSMISSIVE)
+        self.lib_ = L
+
+    def set_scalars(self, dt, time, tend, gamma=None):
+        self.lib_.ref_set_scalars(dt, time, tend, _p(self.eos), BC_TRANSMISSIVE)
+
+    def predictor(self, luh):
+        V, n = self.V, self.n
+        luh = np.asfortranarray(luh, dtype=np.float64)
+        out = dict(qhi=np.zeros((V, n, n), order="F"), Fhi=np.zeros((V, 2, n, n), order="F"),
+                   Shi=np.zeros((V, n, n), order="F"), qbnd=np.zeros((V, 4, n), order="F"),
+                   Fbnd=np.zeros((V, 4, n), order="F"))
+        self.lib_.ref_predictor(_p(out["qhi"]), _p(out["Fhi"]), _p(out["Shi"]), _p(out["qbnd"]), _p(out["Fbnd"]),
+                                _p(luh))
+        return out
+
+    def volume(self, Fhi, Shi=None):
+        d = np.zeros((self.V, self.n, self.n), order="F")
+        self.lib_.ref_volume(_p(d), _p(np.asfortranarray(Fhi)), _p(np.asfortranarray(Shi)))
+        return d
+
+    def rusanov_bn(self, ql, fl, qr, fr, nv):
+        dm, dp = np.zeros(self.V), np.zeros(self.V)
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (ql, fl, qr, fr, nv)]
+        self.lib_.ref_rusanov(*[_p(x) for x in a], _p(dm), _p(dp))
+        return dm, dp
+
+    def ncp(self, q, gradq):
+        out = np.zeros(self.V)
+        self.lib_.ref_ncp(_p(np.ascontiguousarray(q, dtype=np.float64)), _p(np.asfortranarray(gradq)), _p(out))
+        return out
+
+    def roe(self, qa, qb, nv):
+        b = np.zeros((self.V, self.V), order="F")
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (qa, qb, nv)]
+        self.lib_.ref_roe(*[_p(x) for x in a], _p(b))
+        return b
