@@ -22,7 +22,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBDIR = os.path.join(os.path.dirname(HERE), "lib")
 
 BC = {"dmr": 0, "periodic": 1, "transmissive": 2}
-NVAR = 4
+PDE = {"euler": 0, "bn": 1}
+NVAR = {0: 4, 1: 9}
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -40,7 +41,7 @@ class _Desc(ctypes.Structure):
     _fields_ = [("pde", ctypes.c_int), ("N", ctypes.c_int), ("imax", ctypes.c_int), ("jmax", ctypes.c_int),
                 ("xl", ctypes.c_double * 2), ("xr", ctypes.c_double * 2), ("bc_mode", ctypes.c_int),
                 ("gamma", ctypes.c_double), ("cfl", ctypes.c_double), ("exact", ctypes.c_int),
-                ("device", ctypes.c_int), ("ops", _Ops)]
+                ("device", ctypes.c_int), ("ops", _Ops), ("bn_eos", ctypes.c_double * 4)]
 
 
 _libs = {}
@@ -77,6 +78,7 @@ def load_library(exact=False):
         "aderdg_step": [vp, ctypes.c_double, ctypes.c_double, _dp, _ip],
         "aderdg_run": [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp, _ip],
         "aderdg_predictor_cells": [vp, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp, _dp],
+        "aderdg_predictor_cells_bn": [vp, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp, _dp, _dp],
         "aderdg_timing_enable": [vp, ctypes.c_int],
         "aderdg_timing_get": [vp, _dp, ctypes.POINTER(ctypes.c_int64)],
         "aderdg_operators": [ctypes.c_int] + [_dp] * 11,
@@ -113,20 +115,28 @@ def operators(N, exact=False):
     return o
 
 
-def flops_per_cell(N):
+def flops_per_cell(N, pde="euler"):
     lib = load_library(False)
-    return lib.aderdg_flops_predictor(0, N), lib.aderdg_flops_update(0, N)
+    k = PDE[pde] if isinstance(pde, str) else int(pde)
+    return lib.aderdg_flops_predictor(k, N), lib.aderdg_flops_update(k, N)
 
 
 class Solver:
-    """One resident Euler solve on the GPU (context of include/aderdg.h)."""
+    """One resident solve on the GPU (context of include/aderdg.h).
 
-    def __init__(self, N, imax, jmax, xl, xr, bc="periodic", gamma=1.4, cfl=0.9, exact=False, device=0, ops=None):
+    pde "euler" (gamma) or "bn" (eos = GAMMA_S, GAMMA_G, PI_S, PI_G)."""
+
+    def __init__(self, N, imax, jmax, xl, xr, bc="periodic", gamma=1.4, cfl=0.9, exact=False, device=0, ops=None,
+                 pde="euler", eos=None):
         self.lib = load_library(exact)
-        self.N, self.n, self.s, self.V = N, N + 1, 2 * N + 1, NVAR
+        self.pde = PDE[pde] if isinstance(pde, str) else int(pde)
+        self.N, self.n, self.s, self.V = N, N + 1, 2 * N + 1, NVAR[self.pde]
         self.imax, self.jmax = imax, jmax
         d = _Desc()
-        d.pde, d.N, d.imax, d.jmax = 0, N, imax, jmax
+        d.pde, d.N, d.imax, d.jmax = self.pde, N, imax, jmax
+        if eos is not None:
+            for k in range(4):
+                d.bn_eos[k] = float(eos[k])
         d.xl[0], d.xl[1] = xl
         d.xr[0], d.xr[1] = xr
         d.bc_mode = BC[bc] if isinstance(bc, str) else int(bc)
@@ -231,6 +241,11 @@ class Solver:
         fhi = np.zeros((V, 2, n, n, nc), order="F")
         qb = np.zeros((V, 4, n, nc), order="F")
         fb = np.zeros((V, 4, n, nc), order="F")
+        if self.pde == 1:
+            shi = np.zeros((V, n, n, nc), order="F")
+            self._check(self.lib.aderdg_predictor_cells_bn(self.h, nc, dt, _p(luh), _p(qhi), _p(fhi), _p(shi), _p(qb),
+                                                           _p(fb)))
+            return dict(qhi=qhi, Fhi=fhi, Shi=shi, qbnd=qb, Fbnd=fb)
         self._check(self.lib.aderdg_predictor_cells(self.h, nc, dt, _p(luh), _p(qhi), _p(fhi), _p(qb), _p(fb)))
         return dict(qhi=qhi, Fhi=fhi, qbnd=qb, Fbnd=fb)
 

@@ -254,20 +254,25 @@ int aderdg_create(const aderdg_desc *d, aderdg_ctx **out)
 {
     if (!d || !out) return fail(ADERDG_EINVAL, "aderdg_create: null argument");
     *out = nullptr;
-    if (d->pde != ADERDG_PDE_EULER) return fail(ADERDG_EINVAL, "aderdg_create: unsupported pde %d", d->pde);
+    if (d->pde != ADERDG_PDE_EULER && d->pde != ADERDG_PDE_BN)
+        return fail(ADERDG_EINVAL, "aderdg_create: unsupported pde %d", d->pde);
+    if (d->pde == ADERDG_PDE_BN && d->bc_mode == ADERDG_BC_DMR)
+        return fail(ADERDG_EINVAL, "aderdg_create: the DMR boundaries exist for the Euler system only");
+    if (d->pde == ADERDG_PDE_BN && !(d->bn_eos[0] > 1.0 && d->bn_eos[1] > 1.0))
+        return fail(ADERDG_EINVAL, "aderdg_create: Baer-Nunziato GAMMA_S and GAMMA_G must exceed 1");
     if (d->N < 1 || d->N > kMaxDegree) return fail(ADERDG_EINVAL, "aderdg_create: N=%d outside 1..%d", d->N, kMaxDegree);
     if (d->imax < 1 || d->jmax < 1 || (long long)d->imax * d->jmax > (1LL << 30))
         return fail(ADERDG_EINVAL, "aderdg_create: bad mesh %d x %d", d->imax, d->jmax);
     if (d->bc_mode < 0 || d->bc_mode > 2) return fail(ADERDG_EINVAL, "aderdg_create: bad bc_mode %d", d->bc_mode);
     if (!(d->xr[0] > d->xl[0]) || !(d->xr[1] > d->xl[1])) return fail(ADERDG_EINVAL, "aderdg_create: empty domain");
-    if (!(d->gamma > 1.0)) return fail(ADERDG_EINVAL, "aderdg_create: gamma must exceed 1");
+    if (d->pde == ADERDG_PDE_EULER && !(d->gamma > 1.0)) return fail(ADERDG_EINVAL, "aderdg_create: gamma must exceed 1");
     if (d->exact != 0) return fail(ADERDG_EINVAL, "aderdg_create: exact must be 0");
     aderdg_ctx *c = new aderdg_ctx();
     c->desc = *d;
     c->N = d->N;
     c->n = d->N + 1;
     c->s = 2 * d->N + 1;
-    c->V = 4;
+    c->V = d->pde == ADERDG_PDE_BN ? 9 : 4;
     c->ncell = d->imax * d->jmax;
     if (!build_operators(d->N, c->ops)) {
         delete c;
@@ -286,6 +291,8 @@ int aderdg_create(const aderdg_desc *d, aderdg_ctx **out)
     if (o.uh2lob) std::memcpy(c->ops.u2b, o.uh2lob, n * n * sizeof(double));
 
     MeshArgs &m = c->m;
+    m.pde = d->pde;
+    for (int k = 0; k < 4; k++) m.eos[k] = d->bn_eos[k];
     m.imax = d->imax;
     m.jmax = d->jmax;
     m.ncell = c->ncell;
@@ -310,7 +317,7 @@ int aderdg_create(const aderdg_desc *d, aderdg_ctx **out)
     if (hipSetDevice(d->device) != hipSuccess) return bail(fail(ADERDG_EHIP, "hipSetDevice(%d) failed", d->device));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(ADERDG_EHIP, "hipStreamCreate failed"));
-    const size_t nc = c->ncell, V = 4, NN = (size_t)n * n, SS = (size_t)s * s;
+    const size_t nc = c->ncell, V = c->V, NN = (size_t)n * n, SS = (size_t)s * s;
     int r = 0;
     r = r ? r : dalloc(&c->ops_dev, 1);
     r = r ? r : dalloc(&c->scal, 4);
@@ -406,7 +413,7 @@ int aderdg_get_ops(aderdg_ctx *c, double *w, double *kxi, double *ik1, double *f
 int aderdg_upload_uh(aderdg_ctx *c, const double *uh)
 {
     if (!c || !uh) return fail(ADERDG_EINVAL, "aderdg_upload_uh: null argument");
-    const size_t bytes = sizeof(double) * 4 * c->n * c->n * (size_t)c->ncell;
+    const size_t bytes = sizeof(double) * c->V * c->n * c->n * (size_t)c->ncell;
     HIPCHK(hipMemcpyAsync(c->uh[c->cur], uh, bytes, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemsetAsync(c->status[c->cur], 0, sizeof(int) * c->ncell, c->stream));
     HIPCHK(hipMemsetAsync(c->rc_final, 0, sizeof(int) * c->ncell, c->stream));
@@ -419,7 +426,7 @@ int aderdg_upload_uh(aderdg_ctx *c, const double *uh)
 int aderdg_download_uh(aderdg_ctx *c, double *uh)
 {
     if (!c || !uh) return fail(ADERDG_EINVAL, "aderdg_download_uh: null argument");
-    const size_t bytes = sizeof(double) * 4 * c->n * c->n * (size_t)c->ncell;
+    const size_t bytes = sizeof(double) * c->V * c->n * c->n * (size_t)c->ncell;
     HIPCHK(hipMemcpyAsync(uh, c->uh[c->cur], bytes, hipMemcpyDeviceToHost, c->stream));
     return sync(c);
 }
@@ -428,7 +435,7 @@ int aderdg_set_limiter(aderdg_ctx *c, const int32_t *status, const double *Lh)
 {
     if (!c || !status || !Lh) return fail(ADERDG_EINVAL, "aderdg_set_limiter: null argument");
     if (!c->have_uh) return fail(ADERDG_ESTATE, "aderdg_set_limiter: upload uh first");
-    const size_t nc = c->ncell, blk = (size_t)4 * c->s * c->s;
+    const size_t nc = c->ncell, blk = (size_t)c->V * c->s * c->s;
     std::vector<int32_t> st(nc);
     for (size_t k = 0; k < nc; k++) st[k] = status[k] ? 1 : 0;
     HIPCHK(hipMemcpyAsync(c->status[c->cur], st.data(), nc * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
@@ -442,28 +449,28 @@ int aderdg_set_limiter(aderdg_ctx *c, const int32_t *status, const double *Lh)
 int aderdg_get_limiter(aderdg_ctx *c, int32_t *status, int32_t *recompute, double *Lh, double *lmin, double *lmax)
 {
     if (!c) return fail(ADERDG_EINVAL, "aderdg_get_limiter: null context");
-    const size_t nc = c->ncell, blk = (size_t)4 * c->s * c->s;
+    const size_t nc = c->ncell, blk = (size_t)c->V * c->s * c->s, V = c->V;
     std::vector<int32_t> st(nc);
     HIPCHK(hipMemcpyAsync(st.data(), c->status[c->cur], nc * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     if (recompute) HIPCHK(hipMemcpyAsync(recompute, c->rc_final, nc * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     if (Lh) HIPCHK(hipMemcpyAsync(Lh, c->lh[c->cur], nc * blk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     std::vector<double> mn, mx;
     if (lmin) {
-        mn.resize(4 * nc);
-        HIPCHK(hipMemcpyAsync(mn.data(), c->lmin, 4 * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        mn.resize(V * nc);
+        HIPCHK(hipMemcpyAsync(mn.data(), c->lmin, V * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     }
     if (lmax) {
-        mx.resize(4 * nc);
-        HIPCHK(hipMemcpyAsync(mx.data(), c->lmax, 4 * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        mx.resize(V * nc);
+        HIPCHK(hipMemcpyAsync(mx.data(), c->lmax, V * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     }
     int r = sync(c);
     if (r) return r;
     for (size_t k = 0; k < nc; k++) {
         if (status) status[k] = st[k];
         if (Lh && !st[k]) std::memset(Lh + k * blk, 0, blk * sizeof(double));
-        for (int v = 0; v < 4; v++) {
-            if (lmin) lmin[v + 4 * k] = mn[v * nc + k];
-            if (lmax) lmax[v + 4 * k] = mx[v * nc + k];
+        for (size_t v = 0; v < V; v++) {
+            if (lmin) lmin[v + V * k] = mn[v * nc + k];
+            if (lmax) lmax[v + V * k] = mx[v * nc + k];
         }
     }
     return 0;
@@ -571,37 +578,58 @@ int aderdg_run(aderdg_ctx *c, int nsteps, double time, double tend, double *time
     return 0;
 }
 
-int aderdg_predictor_cells(aderdg_ctx *c, int ncells, double dt, const double *luh, double *lqhi, double *lFhi,
-                           double *lqbnd, double *lFbnd)
+namespace {
+int predictor_cells(aderdg_ctx *c, int ncells, double dt, const double *luh, double *lqhi, double *lFhi, double *lShi,
+                    double *lqbnd, double *lFbnd)
 {
-    if (!c || !luh || !lqhi || !lFhi || !lqbnd || !lFbnd || ncells < 1)
-        return fail(ADERDG_EINVAL, "aderdg_predictor_cells: bad argument");
-    const size_t n = c->n, V = 4, nc = ncells;
-    double *d_uh = nullptr, *d_q = nullptr, *d_f = nullptr, *d_qb = nullptr, *d_fb = nullptr;
+    const size_t n = c->n, V = c->V, nc = ncells;
+    double *d_uh = nullptr, *d_q = nullptr, *d_f = nullptr, *d_qb = nullptr, *d_fb = nullptr, *d_s = nullptr;
     int r = dalloc(&d_uh, V * n * n * nc);
     r = r ? r : dalloc(&d_q, V * n * n * nc);
     r = r ? r : dalloc(&d_f, 2 * V * n * n * nc);
     r = r ? r : dalloc(&d_qb, 4 * V * n * nc);
     r = r ? r : dalloc(&d_fb, 4 * V * n * nc);
+    r = r ? r : dalloc(&d_s, V * n * n * nc);
     if (!r) {
         hipError_t e = hipMemcpyAsync(d_uh, luh, V * n * n * nc * sizeof(double), hipMemcpyHostToDevice, c->stream);
         c->pinned[0] = dt;
         if (e == hipSuccess) e = hipMemcpyAsync(c->scal, c->pinned, sizeof(double), hipMemcpyHostToDevice, c->stream);
         MeshArgs m = c->m;
         m.ncell = ncells;
-        TestBufs t{d_uh, d_q, d_f, d_qb, d_fb};
+        TestBufs t{d_uh, d_q, d_f, d_qb, d_fb, d_s};
         if (e == hipSuccess) e = launch_test_predictor(c->N, m, bufs(c), t, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(lqhi, d_q, V * n * n * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(lFhi, d_f, 2 * V * n * n * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(lqbnd, d_qb, 4 * V * n * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(lFbnd, d_fb, 4 * V * n * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess && lShi)
+            e = hipMemcpyAsync(lShi, d_s, V * n * n * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) r = fail(ADERDG_EHIP, "aderdg_predictor_cells: %s", hipGetErrorString(e));
     }
-    void *ptrs[] = {d_uh, d_q, d_f, d_qb, d_fb};
+    void *ptrs[] = {d_uh, d_q, d_f, d_qb, d_fb, d_s};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     return r;
+}
+}  // namespace
+
+int aderdg_predictor_cells(aderdg_ctx *c, int ncells, double dt, const double *luh, double *lqhi, double *lFhi,
+                           double *lqbnd, double *lFbnd)
+{
+    if (!c || !luh || !lqhi || !lFhi || !lqbnd || !lFbnd || ncells < 1)
+        return fail(ADERDG_EINVAL, "aderdg_predictor_cells: bad argument");
+    if (c->desc.pde != ADERDG_PDE_EULER) return fail(ADERDG_EINVAL, "aderdg_predictor_cells: use the _bn form");
+    return predictor_cells(c, ncells, dt, luh, lqhi, lFhi, nullptr, lqbnd, lFbnd);
+}
+
+int aderdg_predictor_cells_bn(aderdg_ctx *c, int ncells, double dt, const double *luh, double *lqhi, double *lFhi,
+                              double *lShi, double *lqbnd, double *lFbnd)
+{
+    if (!c || !luh || !lqhi || !lFhi || !lShi || !lqbnd || !lFbnd || ncells < 1)
+        return fail(ADERDG_EINVAL, "aderdg_predictor_cells_bn: bad argument");
+    if (c->desc.pde != ADERDG_PDE_BN) return fail(ADERDG_EINVAL, "aderdg_predictor_cells_bn: not a Baer-Nunziato context");
+    return predictor_cells(c, ncells, dt, luh, lqhi, lFhi, lShi, lqbnd, lFbnd);
 }
 
 int aderdg_timing_enable(aderdg_ctx *c, int on)
@@ -630,18 +658,21 @@ int aderdg_timing_get(aderdg_ctx *c, double ms[5], int64_t launches[5])
 }
 
 // Operation counts per cell-step with the oracle's convention (+ - * / sqrt = 1).
-// Predictor launch: Euler/space_time_predictor.f90 + Euler/volume_integral.f90.
+// Predictor launch: Euler/space_time_predictor.f90 + Euler/volume_integral.f90
+// (Baer-Nunziato: + gradients, NCP source and its time average).
 double aderdg_flops_predictor(int pde, int N)
 {
-    if (pde != ADERDG_PDE_EULER || N < 1) return 0.0;
-    const double n = N + 1, V = 4, cF = 26;
+    if ((pde != ADERDG_PDE_EULER && pde != ADERDG_PDE_BN) || N < 1) return 0.0;
+    const bool bn = pde == ADERDG_PDE_BN;
+    const double n = N + 1, V = bn ? 9 : 4, cF = bn ? 76 : 26, cNcp = 34;
     const double init = 3 * V * n * n * n;
-    const double per_slice = n * n * cF + 2 * (3 * n) + 2 * V * n * n * (2 * n + 2);
+    double per_slice = n * n * cF + 2 * (3 * n) + 2 * V * n * n * (2 * n + 2);
+    if (bn) per_slice += 2 * V * n * n * (2 * n + 1) + n * n * (3 + 2 * V) + n * n * cNcp;
     const double time_contraction = n * n * (2 + V * n * (2 * n + 1));
     const double picard = n * (n * per_slice + time_contraction);
-    const double averages = 3 * V * n * n * 2 * n;
+    const double averages = (bn ? 4 : 3) * V * n * n * 2 * n;
     const double extrapolation = 8 * V * n * 2 * n;
-    const double volume = 2 * V * n * n * (2 * n + 3);
+    const double volume = 2 * V * n * n * (2 * n + 3) + (bn ? 3 * V * n * n : 0);
     return init + picard + averages + extrapolation + volume;
 }
 
@@ -649,13 +680,16 @@ double aderdg_flops_predictor(int pde, int N)
 // element update, DMP and the next step's time-step candidates.
 double aderdg_flops_update(int pde, int N)
 {
-    if (pde != ADERDG_PDE_EULER || N < 1) return 0.0;
-    const double n = N + 1, V = 4, s = 2 * N + 1;
-    const double eig = 17, rusanov = 2 * eig + 4 * 6;
+    if ((pde != ADERDG_PDE_EULER && pde != ADERDG_PDE_BN) || N < 1) return 0.0;
+    const bool bn = pde == ADERDG_PDE_BN;
+    const double n = N + 1, V = bn ? 9 : 4, s = 2 * N + 1;
+    const double eig = bn ? 48 : 17;
+    const double rusanov = bn ? 2 * eig + 3 * (27 + 14 + 27) + 9 * 6 + 9 * 3 + 9 * 4 : 2 * eig + 4 * 6;
     const double faces = 2 * n * rusanov;
     const double surface = 2 * V * n * (1 + 5 * n);
     const double update = 4 * V * n * n;
-    const double dmp = 2 * V * n * s * (n + s) + 4 * V * n * n * n + V * 4 + s * s * 9;
+    const double positivity = bn ? 32 : 9;
+    const double dmp = 2 * V * n * s * (n + s) + 4 * V * n * n * n + V * 4 + s * s * positivity;
     const double dt = n * n * (2 * eig + 5);
     return faces + surface + update + dmp + dt;
 }

@@ -1,4 +1,5 @@
-// euler_kernels.hpp -- launch interface of the Euler step kernels.
+// euler_kernels.hpp -- launch interface of the element-step kernels (2-D Euler
+// and Baer-Nunziato), implemented in step_kernels.hip.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -6,11 +7,11 @@
 
 namespace aderdg {
 
-constexpr int kBlock = 256;        // predictor / spread / dt workgroup size
-constexpr int kUpdBlock = 128;     // update workgroup size (LDS-bound)
+constexpr int kBlock = 256;        // spread / dt workgroup size
 constexpr int kDtGrid = 256;       // workgroups of the time-step reduction
 constexpr int kSubcellGrid = 4096; // persistent workgroups of the subcell recompute
 
+enum { kPdeEuler = 0, kPdeBn = 1 };
 enum { kBcDmr = 0, kBcPeriodic = 1, kBcTransmissive = 2 };
 enum { kCntList = 0, kCntTroubled = 1, kCntDtBlocks = 2, kNumCounters = 4 };
 enum { kPartDt = 0, kPartPredictor = 1, kPartUpdate = 2, kPartSpread = 3, kPartSubcell = 4, kPartAdvance = 5,
@@ -18,8 +19,9 @@ enum { kPartDt = 0, kPartPredictor = 1, kPartUpdate = 2, kPartSpread = 3, kPartS
 
 // Mesh and equation constants, passed by value.
 struct MeshArgs {
-    int imax, jmax, ncell, bc_mode, wrap;
+    int pde, imax, jmax, ncell, bc_mode, wrap;
     double dx, dy, gamma, cflpnpm;
+    double eos[4];             // Baer-Nunziato: GAMMA_S, GAMMA_G, PI_S, PI_G
     double dmr_a, dmr_b;       // DMR shock position x_sw = dmr_a + dmr_b * t (Euler/riemann_problem.f90:1-8)
     double qa[4], qb[4];       // DMR inflow / pre-shock conserved states
 };
@@ -38,10 +40,10 @@ struct StepBufs {
     double *lh_next;
 };
 
-// Outputs of the standalone predictor (reference layouts per cell).
+// Outputs of the standalone predictor (reference layouts per cell; shi only for Baer-Nunziato).
 struct TestBufs {
     const double *uh;
-    double *qhi, *fhi, *qbnd, *fbnd;
+    double *qhi, *fhi, *qbnd, *fbnd, *shi;
 };
 
 hipError_t launch_part(int N, const MeshArgs &m, const StepBufs &b, hipStream_t st, int part);

@@ -1,0 +1,279 @@
+// pde_device.hpp -- pointwise device routines of the two PDE systems and the
+// Fortran intrinsic semantics the kernels reproduce.
+//
+// Association order follows the reference expressions exactly, so that the
+// build without floating-point contraction is bit-identical to the reference
+// routines (checked by the GPU parity tests).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "euler_kernels.hpp"
+
+#define ADG_DEV __device__ __forceinline__
+
+namespace aderdg {
+
+// Two-argument MAX/MIN as flang evaluates them: (a > b ? a : b), so a NaN
+// second argument wins and a NaN first argument loses.
+ADG_DEV double f_max(double a, double b) { return a > b ? a : b; }
+ADG_DEV double f_min(double a, double b) { return a < b ? a : b; }
+// One element of MAXVAL/MINVAL, which skip NaN elements; start from NaN.
+ADG_DEV double fold_max(double m, double e)
+{
+    return __builtin_isnan(e) ? m : ((__builtin_isnan(m) || e > m) ? e : m);
+}
+ADG_DEV double fold_min(double m, double e)
+{
+    return __builtin_isnan(e) ? m : ((__builtin_isnan(m) || e < m) ? e : m);
+}
+
+// minmod, Euler/limiter.f90:384-405
+ADG_DEV double minmod(double a, double b)
+{
+    if (a * b <= 0.0) return 0.0;
+    return __builtin_fabs(a) < __builtin_fabs(b) ? a : b;
+}
+
+// Neighbour along one axis: clamp to the mesh, or wrap when periodic
+// (Euler/initialize.f90:231-255).
+ADG_DEV int nb_index(int i, int d, int n, int wrap)
+{
+    int a = i + d;
+    if (a < 0) a = wrap ? n - 1 : 0;
+    if (a > n - 1) a = wrap ? 0 : n - 1;
+    return a;
+}
+
+// ---------------------------------------------------------------------------
+// 2-D Euler, Euler/pde.f90
+// ---------------------------------------------------------------------------
+struct EulerPDE {
+    static constexpr int V = 4;
+    static constexpr bool kNcp = false;
+
+    // PDEFlux, Euler/pde.f90:46-69.  F[0..3] x-flux, F[4..7] y-flux.
+    ADG_DEV static void flux(const MeshArgs &m, const double *Q, double *F)
+    {
+        const double g = m.gamma;
+        const double irho = 1.0 / Q[0];
+        const double p = (g - 1.0) * (Q[3] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2]) * irho);
+        F[0] = Q[1];
+        F[1] = irho * Q[1] * Q[1] + p;
+        F[2] = irho * Q[1] * Q[2];
+        F[3] = irho * Q[1] * (Q[3] + p);
+        F[4] = Q[2];
+        F[5] = irho * Q[2] * Q[1];
+        F[6] = irho * Q[2] * Q[2] + p;
+        F[7] = irho * Q[2] * (Q[3] + p);
+    }
+
+    // maxval(abs(Lambda)) of PDEEigenvalues, Euler/pde.f90:75-91, normal (nx, ny)
+    ADG_DEV static double maxabs_eig(const MeshArgs &m, const double *Q, double nx, double ny)
+    {
+        const double g = m.gamma;
+        const double u = (Q[1] * nx + Q[2] * ny) / Q[0];
+        const double p = (g - 1.0) * (Q[3] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2]) / Q[0]);
+        const double c = __builtin_sqrt(g * p / Q[0]);
+        double mx = __builtin_nan("");
+        mx = fold_max(mx, __builtin_fabs(u - c));
+        mx = fold_max(mx, __builtin_fabs(u));
+        mx = fold_max(mx, __builtin_fabs(u));
+        mx = fold_max(mx, __builtin_fabs(u + c));
+        return mx;
+    }
+
+    // RusanovFlux, Euler/pde.f90:97-114.  One flux serves both sides: Dm = Dp.
+    ADG_DEV static void face_flux(const MeshArgs &m, const double *QL, const double *FL, const double *QR,
+                                  const double *FR, double nx, double ny, double *Dm, double *Dp)
+    {
+        const double smax = f_max(maxabs_eig(m, QL, nx, ny), maxabs_eig(m, QR, nx, ny));
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            Dm[v] = 0.5 * (FL[v] + FR[v]) - 0.5 * smax * (QR[v] - QL[v]);
+            Dp[v] = Dm[v];
+        }
+    }
+
+    // PDEAssurePositivity, Euler/pde.f90:120-145: true if the state is rejected
+    ADG_DEV static bool inadmissible(const MeshArgs &m, const double *Q)
+    {
+        const double irho = 1.0 / Q[0];
+        const double p = (m.gamma - 1.0) * (Q[3] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2]) * irho);
+        return (Q[0] <= 1.0e-12) || (p <= 1.0e-12);
+    }
+
+    ADG_DEV static void ncp(const MeshArgs &, const double *, const double *, double *) {}
+};
+
+// ---------------------------------------------------------------------------
+// 2-D Baer-Nunziato, Baer-Nunziato/pde.f90 (eos = GAMMA_S, GAMMA_G, PI_S, PI_G)
+// Q = (phi_s rho_s, .. u_s, .. v_s, phi_s E_s, phi_g rho_g, .. u_g, .. v_g, phi_g E_g, phi_s)
+// ---------------------------------------------------------------------------
+struct BnPDE {
+    static constexpr int V = 9;
+    static constexpr bool kNcp = true;
+
+    // PDEFlux, Baer-Nunziato/pde.f90:74-119.  F[0..8] x-flux, F[9..17] y-flux.
+    ADG_DEV static void flux(const MeshArgs &m, const double *Q, double *F)
+    {
+        const double *e = m.eos;
+        const double phi_s = Q[8], phi_g = 1.0 - Q[8];
+        const double rho_s = Q[0] / phi_s, rho_g = Q[4] / phi_g;
+        const double u_s = Q[1] / Q[0], v_s = Q[2] / Q[0];
+        const double u_g = Q[5] / Q[4], v_g = Q[6] / Q[4];
+        const double E_s = Q[3] / phi_s, E_g = Q[7] / phi_g;
+        const double p_s = (e[0] - 1.0) * (E_s - 0.5 * rho_s * (u_s * u_s + v_s * v_s)) - e[0] * e[2];
+        const double p_g = (e[1] - 1.0) * (E_g - 0.5 * rho_g * (u_g * u_g + v_g * v_g)) - e[1] * e[3];
+        F[0] = phi_s * rho_s * u_s;
+        F[1] = phi_s * (rho_s * u_s * u_s + p_s);
+        F[2] = phi_s * rho_s * u_s * v_s;
+        F[3] = phi_s * u_s * (E_s + p_s);
+        F[4] = phi_g * rho_g * u_g;
+        F[5] = phi_g * (rho_g * u_g * u_g + p_g);
+        F[6] = phi_g * rho_g * u_g * v_g;
+        F[7] = phi_g * u_g * (E_g + p_g);
+        F[8] = 0.0;
+        F[9] = phi_s * rho_s * v_s;
+        F[10] = phi_s * rho_s * u_s * v_s;
+        F[11] = phi_s * (rho_s * v_s * v_s + p_s);
+        F[12] = phi_s * v_s * (E_s + p_s);
+        F[13] = phi_g * rho_g * v_g;
+        F[14] = phi_g * rho_g * u_g * v_g;
+        F[15] = phi_g * (rho_g * v_g * v_g + p_g);
+        F[16] = phi_g * v_g * (E_g + p_g);
+        F[17] = 0.0;
+    }
+
+    // maxval(abs(Lambda)) of PDEEigenvalues, Baer-Nunziato/pde.f90:125-144 (via PDECons2Prim :8-43)
+    ADG_DEV static double maxabs_eig(const MeshArgs &m, const double *Q, double nx, double ny)
+    {
+        const double *e = m.eos;
+        const double phi_s = Q[8], phi_g = 1.0 - Q[8];
+        const double rho_s = Q[0] / phi_s, rho_g = Q[4] / phi_g;
+        const double u_s = Q[1] / Q[0], v_s = Q[2] / Q[0];
+        const double u_g = Q[5] / Q[4], v_g = Q[6] / Q[4];
+        const double E_s = Q[3] / phi_s, E_g = Q[7] / phi_g;
+        const double p_s = (e[0] - 1.0) * (E_s - 0.5 * rho_s * (u_s * u_s + v_s * v_s)) - e[0] * e[2];
+        const double p_g = (e[1] - 1.0) * (E_g - 0.5 * rho_g * (u_g * u_g + v_g * v_g)) - e[1] * e[3];
+        const double un_s = u_s * nx + v_s * ny;
+        const double un_g = u_g * nx + v_g * ny;
+        const double a_s = __builtin_sqrt(e[0] * (p_s + e[2]) / rho_s);
+        const double a_g = __builtin_sqrt(e[1] * (p_g + e[3]) / rho_g);
+        double mx = __builtin_nan("");
+        mx = fold_max(mx, __builtin_fabs(un_g - a_g));
+        mx = fold_max(mx, __builtin_fabs(un_g + a_g));
+        mx = fold_max(mx, __builtin_fabs(un_s - a_s));
+        mx = fold_max(mx, __builtin_fabs(un_s + a_s));
+        mx = fold_max(mx, __builtin_fabs(un_g));
+        mx = fold_max(mx, __builtin_fabs(un_s));
+        return mx;
+    }
+
+    // p_g, u_s, v_s as in PDEMatrixB / PDENCP, Baer-Nunziato/pde.f90:163-176, 207-219
+    ADG_DEV static double gas_pressure(const MeshArgs &m, const double *Q, double &u_s, double &v_s)
+    {
+        const double *e = m.eos;
+        const double phi_g = 1.0 - Q[8];
+        const double rho_g = Q[4] / phi_g;
+        u_s = Q[1] / Q[0];
+        v_s = Q[2] / Q[0];
+        const double u_g = Q[5] / Q[4], v_g = Q[6] / Q[4];
+        const double E_g = Q[7] / phi_g;
+        return (e[1] - 1.0) * (E_g - 0.5 * rho_g * (u_g * u_g + v_g * v_g)) - e[1] * e[3];
+    }
+
+    // Column 9 of the Roe matrix: 3-point Gauss-Legendre path integral of PDEMatrixB
+    // (RoeMatrix, Baer-Nunziato/pde.f90:296-320; PDEMatrixB :150-186).  The other columns are 0.
+    ADG_DEV static void roe_col9(const MeshArgs &m, const double *Qa, const double *Qb, double nx, double ny,
+                                 double *B9)
+    {
+        const double s15 = __builtin_sqrt(15.0);
+        const double sGP[3] = {0.5 - s15 / 10., 0.5, 0.5 + s15 / 10.};
+        const double wGP[3] = {5. / 18.0, 8. / 18., 5. / 18.};
+#pragma unroll
+        for (int r = 0; r < V; r++) B9[r] = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            double Q[V];
+#pragma unroll
+            for (int v = 0; v < V; v++) Q[v] = Qa[v] + sGP[i] * (Qb[v] - Qa[v]);
+            double u_s, v_s;
+            const double p_g = gas_pressure(m, Q, u_s, v_s);
+            const double u_n = nx * u_s + ny * v_s;
+            double b[V];
+            b[0] = 0.0;
+            b[1] = -nx * p_g;
+            b[2] = -ny * p_g;
+            b[3] = -p_g * u_n;
+            b[4] = 0.0;
+            b[5] = nx * p_g;
+            b[6] = ny * p_g;
+            b[7] = p_g * u_n;
+            b[8] = u_n;
+#pragma unroll
+            for (int r = 0; r < V; r++) B9[r] = B9[r] + wGP[i] * b[r];
+        }
+    }
+
+    // matmul(BRoe, dQ) with a Roe matrix whose only non-zero column is 9: the
+    // zero columns add +0 terms, so the running sum starts at +0.
+    ADG_DEV static double roe_times(const double *B9, int r, const double *dQ)
+    {
+        double acc = 0.0;
+        acc = acc + B9[r] * dQ[8];
+        return acc;
+    }
+
+    // RusanovFlux (path-conservative), Baer-Nunziato/pde.f90:326-353
+    ADG_DEV static void face_flux(const MeshArgs &m, const double *QL, const double *FL, const double *QR,
+                                  const double *FR, double nx, double ny, double *Dm, double *Dp)
+    {
+        const double smax = f_max(maxabs_eig(m, QL, nx, ny), maxabs_eig(m, QR, nx, ny));
+        double B9[V], dQ[V];
+        roe_col9(m, QL, QR, nx, ny, B9);
+#pragma unroll
+        for (int v = 0; v < V; v++) dQ[v] = QR[v] - QL[v];
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            const double d = 0.5 * (FL[v] + FR[v]) - 0.5 * smax * dQ[v];
+            const double t = roe_times(B9, v, dQ);
+            Dm[v] = d - 0.5 * t;
+            Dp[v] = d + 0.5 * t;
+        }
+    }
+
+    // PDEAssurePositivity, Baer-Nunziato/pde.f90:237-266: true if the state is rejected
+    ADG_DEV static bool inadmissible(const MeshArgs &m, const double *Q)
+    {
+        const double *e = m.eos;
+        const double phi_s = Q[8], phi_g = 1.0 - Q[8];
+        const double rho_s = Q[0] / phi_s, rho_g = Q[4] / phi_g;
+        const double u_s = Q[1] / Q[0], v_s = Q[2] / Q[0];
+        const double u_g = Q[5] / Q[4], v_g = Q[6] / Q[4];
+        const double E_s = Q[3] / phi_s, E_g = Q[7] / phi_g;
+        const double p_s = (e[0] - 1.0) * (E_s - 0.5 * rho_s * (u_s * u_s + v_s * v_s)) - e[0] * e[2];
+        const double p_g = (e[1] - 1.0) * (E_g - 0.5 * rho_g * (u_g * u_g + v_g * v_g)) - e[1] * e[3];
+        return rho_s <= 1.0e-12 || rho_g <= 1.0e-12 || (p_s + e[2]) <= 1.0e-12 || (p_g + e[3]) <= 1.0e-12 ||
+               phi_s <= 0.0 || phi_s >= 1.0;
+    }
+
+    // PDEFusedSrcNCP = -PDENCP, Baer-Nunziato/pde.f90:277-290, 192-231.  Gx, Gy: gradient components.
+    ADG_DEV static void ncp(const MeshArgs &m, const double *Q, const double *G, double *src)
+    {
+        double u_s, v_s;
+        const double p_g = gas_pressure(m, Q, u_s, v_s);
+        const double qx = G[8], qy = G[V + 8];
+        const double un = u_s * qx + v_s * qy;
+        src[0] = -0.0;
+        src[1] = -(-p_g * qx);
+        src[2] = -(-p_g * qy);
+        src[3] = -(-p_g * un);
+        src[4] = -0.0;
+        src[5] = -(p_g * qx);
+        src[6] = -(p_g * qy);
+        src[7] = -(p_g * un);
+        src[8] = -un;
+    }
+};
+
+}  // namespace aderdg

@@ -1,6 +1,7 @@
-// euler_kernels.hip -- CDNA4 (gfx950) kernels of the 2-D Euler ADER-DG step.
+// step_kernels.hip -- CDNA4 (gfx950) kernels of the ADER-DG element step for the
+// 2-D Euler (nVar = 4) and Baer-Nunziato (nVar = 9) systems.
 //
-// Per time step (the order of Euler/main.f90:24-100):
+// Per time step (the order of Euler/main.f90:24-100, Baer-Nunziato/main.f90:24-100):
 //   k_dt          calc_time_step: min-reduction of per-cell candidates
 //   k_predictor   ader_space_time_predictor + volume_integral + face
 //                 extrapolation + the 3x3 pass of get_min_max
@@ -11,7 +12,7 @@
 //                 compaction of the cells to recompute
 //   k_subcell     subcell_recompute (MUSCL-Hancock on the (2N+1)^2 subgrid)
 //                 + put_subcell_data + re-DMP + update_limiter, over the
-//                 compacted list only
+//                 compacted list only (k_subcell_bn for Baer-Nunziato)
 //
 // Data layout in HBM (ncell = IMAX*JMAX, cell = i + IMAX*j):
 //   uh, lduh      [cell][j][i][v]   the reference's uh(V,n,n,IMAX,JMAX)
@@ -21,24 +22,34 @@
 //   lmin.. etc.   [v][cell]
 //   Lh            [cell][q][p][v]   subcell averages of limited cells
 #include "euler_kernels.hpp"
-
-#include "euler_device.hpp"
+#include "pde_device.hpp"
 
 namespace aderdg {
 
 namespace {
 
-template <int N>
+// workgroup sizes: Baer-Nunziato keeps three times the data per node
+template <class P>
+constexpr int kPredBlock = P::V == 4 ? 256 : 64;
+template <class P>
+constexpr int kUpdBlock = P::V == 4 ? 128 : 64;
+
+template <class P, int N>
 struct Shape {
+    static constexpr int V = P::V;
     static constexpr int n = N + 1;
     static constexpr int NN = n * n;
     static constexpr int s = 2 * N + 1;
     static constexpr int SS = s * s;
-    static constexpr int CPW = kBlock / NN;      // cells per predictor workgroup
-    static constexpr int CPWU = kUpdBlock / NN;  // cells per update workgroup
+    static constexpr int PRED_BLOCK = kPredBlock<P>;
+    static constexpr int UPD_BLOCK = kUpdBlock<P>;
+    static constexpr int CPW = PRED_BLOCK / NN;   // cells per predictor workgroup
+    static constexpr int CPWU = UPD_BLOCK / NN;   // cells per update workgroup
+    // predictor LDS per cell and time slice: Fx, Fy (and q for the NCP gradients)
+    static constexpr int SLOT = (P::kNcp ? 3 : 2) * V * NN;
     // per-cell LDS of the update: suh | (face fluxes, later limy + loby) | lim | lob | dt candidates
-    static constexpr int R1 = (4 * kV * n > kV * n * s + kV * NN) ? 4 * kV * n : kV * n * s + kV * NN;
-    static constexpr int UPD_LDS = kV * NN + R1 + kV * SS + kV * NN + NN;
+    static constexpr int R1 = (4 * V * n > V * n * s + V * NN) ? 4 * V * n : V * n * s + V * NN;
+    static constexpr int UPD_LDS = V * NN + R1 + V * SS + V * NN + NN;
 };
 
 ADG_DEV double dmr_shock_x(const MeshArgs &m, double time) { return m.dmr_a + m.dmr_b * time; }
@@ -47,15 +58,16 @@ ADG_DEV double dmr_shock_x(const MeshArgs &m, double time) { return m.dmr_a + m.
 // space-time predictor (+ volume integral, face extrapolation, min/max pass 2)
 // Thread (cell, i, j) owns the time line q(:, i, j, 0..N) of one spatial node.
 // The x- and y-contractions read the fluxes of the other nodes of the same
-// time slice from LDS (double buffered); the time contraction with iK1 is
-// thread-local.
+// time slice from LDS (double buffered, one barrier per slice); the time
+// contraction with iK1 is thread-local.  Baer-Nunziato also exchanges q for
+// the gradients of the non-conservative product.
 // ---------------------------------------------------------------------------
-template <int N, bool TEST>
-__global__ __launch_bounds__(kBlock) void k_predictor(MeshArgs m, StepBufs b, TestBufs t)
+template <class P, int N, bool TEST>
+__global__ __launch_bounds__(kPredBlock<P>) void k_predictor(MeshArgs m, StepBufs b, TestBufs t)
 {
-    using S = Shape<N>;
-    constexpr int n = S::n, NN = S::NN, CPW = S::CPW, V = kV;
-    __shared__ double sm[2][CPW][2][V][NN];
+    using S = Shape<P, N>;
+    constexpr int n = S::n, NN = S::NN, CPW = S::CPW, V = S::V, SLOT = S::SLOT;
+    __shared__ double sm[2][CPW][SLOT];
     const Operators *__restrict__ op = b.op;
     const int tid = threadIdx.x, lc = tid / NN, tt = tid - lc * NN;
     const int i = tt % n, j = tt / n;
@@ -67,26 +79,39 @@ __global__ __launch_bounds__(kBlock) void k_predictor(MeshArgs m, StepBufs b, Te
     double u[V];
     const double *uh = TEST ? t.uh : b.uh_cur;
 #pragma unroll
-    for (int v = 0; v < V; v++) u[v] = act ? uh[(size_t)cell * V * NN + V * tt + v] : (v == 0 || v == 3 ? 1.0 : 0.0);
+    for (int v = 0; v < V; v++) u[v] = act ? uh[(size_t)cell * V * NN + V * tt + v] : 0.0;
+    if (!act) {  // an admissible dummy state for idle lanes
+        if (V == 4) {
+            u[0] = 1.0;
+            u[3] = 2.5;
+        } else {
+            u[0] = 0.5; u[3] = 2.5; u[4] = 0.5; u[7] = 2.5; u[8] = 0.5;
+        }
+    }
     const double wi = op->w[i], wj = op->w[j];
-    double kx_i[n], kx_j[n];
+    double kx_i[n], kx_j[n], du_i[n], du_j[n];
 #pragma unroll
     for (int k = 0; k < n; k++) {
         kx_i[k] = op->kxi[k + n * i];
         kx_j[k] = op->kxi[k + n * j];
+        if (P::kNcp) {
+            du_i[k] = op->dudx[i + n * k];
+            du_j[k] = op->dudx[j + n * k];
+        }
     }
     const double dt = b.scal[0];
 
-    double q[V][n], fh[2][V];
+    double q[V][n], fh[2][V], sh[V];
 #pragma unroll
     for (int v = 0; v < V; v++) {
         fh[0][v] = 0.0;
         fh[1][v] = 0.0;
+        sh[v] = 0.0;
 #pragma unroll
         for (int l = 0; l < n; l++) q[v][l] = u[v];
     }
     int slot = 0;
-    // Picard iterations, Euler/space_time_predictor.f90:49-84
+    // Picard iterations, Euler/space_time_predictor.f90:49-84 (Baer-Nunziato/space_time_predictor.f90:54-104)
     for (int it = 0; it <= N; it++) {
         double r[V][n];
 #pragma unroll
@@ -94,12 +119,14 @@ __global__ __launch_bounds__(kBlock) void k_predictor(MeshArgs m, StepBufs b, Te
             double Q[V], F[2 * V];
 #pragma unroll
             for (int v = 0; v < V; v++) Q[v] = q[v][l];
-            euler_flux(m.gamma, Q, F);
+            P::flux(m, Q, F);
+            double *buf = sm[slot][lcs];
             if (own) {
 #pragma unroll
                 for (int v = 0; v < V; v++) {
-                    sm[slot][lc][0][v][j * n + i] = F[v];
-                    sm[slot][lc][1][v][i * n + j] = F[V + v];
+                    buf[v * NN + j * n + i] = F[v];
+                    buf[V * NN + v * NN + i * n + j] = F[V + v];
+                    if (P::kNcp) buf[2 * V * NN + v * NN + tt] = Q[v];
                 }
             }
             if (it == N) {  // time average of the last sweep's fluxes (:92)
@@ -113,16 +140,37 @@ __global__ __launch_bounds__(kBlock) void k_predictor(MeshArgs m, StepBufs b, Te
             const double wl = op->w[l];
             const double sx = wl * wj * dt / m.dx, sy = wl * wi * dt / m.dy;
             const double c0 = wi * wj * op->f0[l];
+            double G[2 * V];
 #pragma unroll
             for (int v = 0; v < V; v++) {
                 double acc = 0.0;
 #pragma unroll
-                for (int k = 0; k < n; k++) acc = acc + sm[slot][lcs][0][v][j * n + k] * kx_i[k];
+                for (int k = 0; k < n; k++) acc = acc + buf[v * NN + j * n + k] * kx_i[k];
                 const double rr = c0 * u[v] - sx * acc;  // :66
                 acc = 0.0;
 #pragma unroll
-                for (int k = 0; k < n; k++) acc = acc + sm[slot][lcs][1][v][i * n + k] * kx_j[k];
+                for (int k = 0; k < n; k++) acc = acc + buf[V * NN + v * NN + i * n + k] * kx_j[k];
                 r[v][l] = rr - sy * acc;  // :72
+                if (P::kNcp) {            // gradients, Baer-Nunziato/space_time_predictor.f90:72,79
+                    acc = 0.0;
+#pragma unroll
+                    for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + k + n * j] * du_i[k];
+                    G[v] = 1.0 / m.dx * acc;
+                    acc = 0.0;
+#pragma unroll
+                    for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + i + n * k] * du_j[k];
+                    G[V + v] = 1.0 / m.dy * acc;
+                }
+            }
+            if (P::kNcp) {  // NCP source, Baer-Nunziato/space_time_predictor.f90:84-92
+                double src[V];
+                P::ncp(m, Q, G, src);
+                const double f = wi * wj * wl * dt;
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    r[v][l] = r[v][l] + f * src[v];
+                    if (it == N) sh[v] = sh[v] + src[v] * wl;  // lShi (:111)
+                }
             }
             slot ^= 1;
         }
@@ -146,9 +194,9 @@ __global__ __launch_bounds__(kBlock) void k_predictor(MeshArgs m, StepBufs b, Te
         qh[v] = acc;
     }
     __syncthreads();
-    double *qhiL = &sm[0][lcs][0][0][0];
-    double *fhxL = &sm[0][lcs][1][0][0];
-    double *fhyL = &sm[1][lcs][0][0][0];
+    double *qhiL = &sm[0][lcs][0];
+    double *fhxL = &sm[0][lcs][V * NN];
+    double *fhyL = &sm[1][lcs][0];
     if (own) {
 #pragma unroll
         for (int v = 0; v < V; v++) {
@@ -166,10 +214,11 @@ __global__ __launch_bounds__(kBlock) void k_predictor(MeshArgs m, StepBufs b, Te
                 t.qhi[(size_t)cell * V * NN + v + V * tt] = qh[v];
                 t.fhi[(size_t)cell * 2 * V * NN + v + V * (0 + 2 * tt)] = fh[0][v];
                 t.fhi[(size_t)cell * 2 * V * NN + v + V * (1 + 2 * tt)] = fh[1][v];
+                if (P::kNcp) t.shi[(size_t)cell * V * NN + v + V * tt] = sh[v];
             }
         }
     } else {
-        // volume integral, Euler/volume_integral.f90:17-28
+        // volume integral, Euler/volume_integral.f90:17-28 (+ Baer-Nunziato/volume_integral.f90:36-40)
 #pragma unroll
         for (int v = 0; v < V; v++) {
             double acc = 0.0;
@@ -180,6 +229,7 @@ __global__ __launch_bounds__(kBlock) void k_predictor(MeshArgs m, StepBufs b, Te
 #pragma unroll
             for (int k = 0; k < n; k++) acc = acc + fhyL[v * NN + i + n * k] * op->kxi[j + n * k];
             d = d + acc * wi / m.dy;
+            if (P::kNcp) d = d + wi * wj * sh[v];
             if (act) b.lduh[(size_t)cell * V * NN + V * tt + v] = d;
         }
     }
@@ -236,12 +286,13 @@ __global__ __launch_bounds__(kBlock) void k_predictor(MeshArgs m, StepBufs b, Te
     }
 }
 
-// Face state of one Riemann problem, with the boundary treatment of the mode.
+// Face states of one Riemann problem, with the boundary treatment of the mode.
 // face: 0 left, 1 right (x faces), 2 bottom, 3 top (y faces) of cell (ci,cj).
+template <class P>
 ADG_DEV void face_states(const MeshArgs &m, const StepBufs &b, int n, int face, int p, int cell, int ci, int cj,
                          double time, double *QL, double *FL, double *QR, double *FR)
 {
-    const int V = kV;
+    constexpr int V = P::V;
     auto ld = [&](int f, int c, double *Q, double *F) {
 #pragma unroll
         for (int v = 0; v < V; v++) {
@@ -258,19 +309,20 @@ ADG_DEV void face_states(const MeshArgs &m, const StepBufs &b, int n, int face, 
         }
     };
     const int I = m.imax, J = m.jmax;
+    const bool dmr = V == 4 && m.bc_mode == kBcDmr;
     if (face == 0) {
         ld(0, cell, QR, FR);
         if (ci > 0) {
             ld(1, cell - 1, QL, FL);
         } else if (m.bc_mode == kBcPeriodic) {
             ld(1, cell + I - 1, QL, FL);
-        } else if (m.bc_mode == kBcTransmissive) {
+        } else if (!dmr) {
             copy(QR, FR, QL, FL);
         } else {  // DMR inflow, Euler/riemann_problem.f90:186-193
             double Qbc[V], Fbc[2 * V];
 #pragma unroll
-            for (int v = 0; v < V; v++) Qbc[v] = m.qa[v];
-            euler_flux(m.gamma, Qbc, Fbc);
+            for (int v = 0; v < V; v++) Qbc[v] = m.qa[v % 4];
+            P::flux(m, Qbc, Fbc);
 #pragma unroll
             for (int v = 0; v < V; v++) {
                 QL[v] = Qbc[v];
@@ -293,7 +345,7 @@ ADG_DEV void face_states(const MeshArgs &m, const StepBufs &b, int n, int face, 
             ld(3, cell + (J - 1) * I, QL, FL);
         } else {
             copy(QR, FR, QL, FL);
-            if (m.bc_mode == kBcDmr && !((double)(ci + 1) * m.dx <= 1.0 / 6.0)) {  // reflective, :225-228
+            if (dmr && !((double)(ci + 1) * m.dx <= 1.0 / 6.0)) {  // reflective, :225-228
                 QL[2] = -QL[2];
                 FL[0] = -FL[0];
                 FL[1] = -FL[1];
@@ -306,14 +358,14 @@ ADG_DEV void face_states(const MeshArgs &m, const StepBufs &b, int n, int face, 
             ld(2, cell + I, QR, FR);
         } else if (m.bc_mode == kBcPeriodic) {
             ld(2, cell - (J - 1) * I, QR, FR);
-        } else if (m.bc_mode == kBcTransmissive) {
+        } else if (!dmr) {
             copy(QL, FL, QR, FR);
         } else {  // DMR top: exact shock position, :233-245
             const bool pre = (double)(ci + 1) * m.dx <= dmr_shock_x(m, time);
             double Qbc[V], Fbc[2 * V];
 #pragma unroll
-            for (int v = 0; v < V; v++) Qbc[v] = pre ? m.qa[v] : m.qb[v];
-            euler_flux(m.gamma, Qbc, Fbc);
+            for (int v = 0; v < V; v++) Qbc[v] = pre ? m.qa[v % 4] : m.qb[v % 4];
+            P::flux(m, Qbc, Fbc);
 #pragma unroll
             for (int v = 0; v < V; v++) {
                 QR[v] = Qbc[v];
@@ -323,16 +375,14 @@ ADG_DEV void face_states(const MeshArgs &m, const StepBufs &b, int n, int face, 
     }
 }
 
-// min/max of nodal values, Lobatto values and subcell averages of one cell
-// held in LDS (uhL [v][NN]); also flags inadmissible subcell averages.
-// Writes per-variable (minval, maxval) of the three groups to rng[v][6].
-template <int N>
+// First passes of get_subcell_data / get_lobatto_data for one cell held in LDS (uhL [v][NN])
+template <class P, int N>
 ADG_DEV void cell_ranges_stage1(const Operators *__restrict__ op, const double *uhL, double *tyL, double *byL, int tt,
                                 int nthr)
 {
-    using S = Shape<N>;
-    constexpr int n = S::n, NN = S::NN, s = S::s, V = kV;
-    for (int task = tt; task < n * s; task += nthr) {  // get_subcell_data first pass, Euler/limiter.f90:25-27
+    using S = Shape<P, N>;
+    constexpr int n = S::n, NN = S::NN, s = S::s, V = S::V;
+    for (int task = tt; task < n * s; task += nthr) {  // Euler/limiter.f90:25-27
         const int a = task % n, qq = task / n;
 #pragma unroll
         for (int v = 0; v < V; v++) {
@@ -342,7 +392,7 @@ ADG_DEV void cell_ranges_stage1(const Operators *__restrict__ op, const double *
             tyL[v * n * s + a + n * qq] = acc;
         }
     }
-    for (int task = tt; task < NN; task += nthr) {  // get_lobatto_data first pass, :56-58
+    for (int task = tt; task < NN; task += nthr) {  // :56-58
         const int a = task % n, qq = task / n;
 #pragma unroll
         for (int v = 0; v < V; v++) {
@@ -354,14 +404,15 @@ ADG_DEV void cell_ranges_stage1(const Operators *__restrict__ op, const double *
     }
 }
 
-template <int N>
-ADG_DEV bool cell_ranges_stage2(const Operators *__restrict__ op, double g, const double *tyL, const double *byL,
-                                double *limL, double *lobL, int tt, int nthr)
+// Second passes; returns true if a subcell average is inadmissible (Euler/limiter.f90:227-241)
+template <class P, int N>
+ADG_DEV bool cell_ranges_stage2(const Operators *__restrict__ op, const MeshArgs &m, const double *tyL,
+                                const double *byL, double *limL, double *lobL, int tt, int nthr)
 {
-    using S = Shape<N>;
-    constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, V = kV;
+    using S = Shape<P, N>;
+    constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, V = S::V;
     bool bad = false;
-    for (int task = tt; task < SS; task += nthr) {  // second pass, :33-35
+    for (int task = tt; task < SS; task += nthr) {  // :33-35
         const int p = task % s, qq = task / s;
         double L[V];
 #pragma unroll
@@ -372,7 +423,7 @@ ADG_DEV bool cell_ranges_stage2(const Operators *__restrict__ op, double g, cons
             limL[v * SS + task] = acc;
             L[v] = acc;
         }
-        bad = bad || euler_inadmissible(g, L);  // physical detection, Euler/limiter.f90:227-241
+        bad = bad || P::inadmissible(m, L);
     }
     for (int task = tt; task < NN; task += nthr) {  // :64-66
         const int p = task % n, qq = task / n;
@@ -388,10 +439,10 @@ ADG_DEV bool cell_ranges_stage2(const Operators *__restrict__ op, double g, cons
 }
 
 // (minval, maxval) of nodal, Lobatto, subcell groups of variable v
-template <int N>
+template <class P, int N>
 ADG_DEV void group_ranges(const double *uhL, const double *lobL, const double *limL, int v, double r[6])
 {
-    using S = Shape<N>;
+    using S = Shape<P, N>;
     constexpr int NN = S::NN, SS = S::SS;
     double u0 = __builtin_nan(""), u1 = u0, b0 = u0, b1 = u0, l0 = u0, l1 = u0;
     for (int k = 0; k < NN; k++) {
@@ -407,12 +458,22 @@ ADG_DEV void group_ranges(const double *uhL, const double *lobL, const double *l
     r[0] = u0; r[1] = u1; r[2] = b0; r[3] = b1; r[4] = l0; r[5] = l1;
 }
 
+// DMP range test of one variable, Euler/limiter.f90:198-223 (argmax = 0): true if violated
+ADG_DEV bool dmp_violated(const double r[6], double lmn, double lmx)
+{
+    const double mn = f_min(f_min(r[0], r[4]), r[2]);
+    const double mx = f_max(f_max(r[1], r[5]), r[3]);
+    const double ldiff = f_max(f_max(1.0e-4, 1.0e-3 * (lmx - lmn)), 0.0);
+    return mn < lmn - ldiff || mx > lmx + ldiff;
+}
+
 // time-step candidate of one node, Euler/calc_time_step.f90:29-42
+template <class P>
 ADG_DEV double node_dt(const MeshArgs &m, const double *Q)
 {
     double denom = 0.0;
-    denom = denom + euler_maxabs_eig(m.gamma, Q, 1.0, 0.0) / m.dx;
-    denom = denom + euler_maxabs_eig(m.gamma, Q, 0.0, 1.0) / m.dy;
+    denom = denom + P::maxabs_eig(m, Q, 1.0, 0.0) / m.dx;
+    denom = denom + P::maxabs_eig(m, Q, 0.0, 1.0) / m.dy;
     return m.cflpnpm / denom;
 }
 
@@ -420,11 +481,11 @@ ADG_DEV double node_dt(const MeshArgs &m, const double *Q)
 // Riemann + surface integral + element update + DMP (SEED = false), or only
 // the per-cell limiter seeds of a freshly uploaded solution (SEED = true).
 // ---------------------------------------------------------------------------
-template <int N, bool SEED>
-__global__ __launch_bounds__(kUpdBlock) void k_update(MeshArgs m, StepBufs b)
+template <class P, int N, bool SEED>
+__global__ __launch_bounds__(kUpdBlock<P>) void k_update(MeshArgs m, StepBufs b)
 {
-    using S = Shape<N>;
-    constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, CPW = S::CPWU, V = kV;
+    using S = Shape<P, N>;
+    constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, CPW = S::CPWU, V = S::V;
     __shared__ double lds[CPW][S::UPD_LDS];
     __shared__ int sflag[CPW];
     const Operators *__restrict__ op = b.op;
@@ -437,9 +498,9 @@ __global__ __launch_bounds__(kUpdBlock) void k_update(MeshArgs m, StepBufs b)
     const int ci = act ? cell % m.imax : 0, cj = act ? cell / m.imax : 0;
     const double time = b.scal[1];
     double *suh = &lds[lcs][0];
-    double *sflx = suh + V * NN;       // [f][v][p], dead once the update is done
-    double *sty = sflx;                // limy [v][a + n q] reuses the face-flux space
-    double *sby = sty + V * n * s;     // loby
+    double *sflx = suh + V * NN;     // [f][v][p], dead once the update is done
+    double *sty = sflx;              // limy [v][a + n q] reuses the face-flux space
+    double *sby = sty + V * n * s;   // loby
     double *slim = suh + V * NN + S::R1;
     double *slob = slim + V * SS;
     double *sdt = slob + V * NN;
@@ -447,11 +508,13 @@ __global__ __launch_bounds__(kUpdBlock) void k_update(MeshArgs m, StepBufs b)
     if (!SEED && act) {  // Riemann problems of the cell's 4 faces, Euler/riemann_problem.f90:177-257
         for (int task = tt; task < 4 * n; task += NN) {
             const int f = task / n, p = task - f * n;
-            double QL[V], FL[V], QR[V], FR[V], out[V];
-            face_states(m, b, n, f, p, cell, ci, cj, time, QL, FL, QR, FR);
-            euler_rusanov(m.gamma, QL, FL, QR, FR, f < 2 ? 1.0 : 0.0, f < 2 ? 0.0 : 1.0, out);
+            double QL[V], FL[V], QR[V], FR[V], Dm[V], Dp[V];
+            face_states<P>(m, b, n, f, p, cell, ci, cj, time, QL, FL, QR, FR);
+            P::face_flux(m, QL, FL, QR, FR, f < 2 ? 1.0 : 0.0, f < 2 ? 0.0 : 1.0, Dm, Dp);
+            // left/bottom faces take D- of their face, right/top faces D+ (Baer-Nunziato/main.f90:68)
+            const bool minus = (f == 0 || f == 2);
 #pragma unroll
-            for (int v = 0; v < V; v++) sflx[(f * V + v) * n + p] = out[v];
+            for (int v = 0; v < V; v++) sflx[(f * V + v) * n + p] = minus ? Dm[v] : Dp[v];
         }
     }
     if (own && tt == 0) sflag[lc] = 0;
@@ -480,30 +543,24 @@ __global__ __launch_bounds__(kUpdBlock) void k_update(MeshArgs m, StepBufs b)
     if (own) {
 #pragma unroll
         for (int v = 0; v < V; v++) suh[v * NN + tt] = un[v];
-        sdt[tt] = node_dt(m, un);
+        sdt[tt] = act ? node_dt<P>(m, un) : 1.0;
     }
     __syncthreads();
-    if (own) cell_ranges_stage1<N>(op, suh, sty, sby, tt, NN);
+    if (own) cell_ranges_stage1<P, N>(op, suh, sty, sby, tt, NN);
     __syncthreads();
     bool bad = false;
-    if (own) bad = cell_ranges_stage2<N>(op, m.gamma, sty, sby, slim, slob, tt, NN);
+    if (act) bad = cell_ranges_stage2<P, N>(op, m, sty, sby, slim, slob, tt, NN);
     if (bad && !SEED) atomicOr(&sflag[lc], 1);
     __syncthreads();
-    const int *st_cur = b.status_cur;
     // a cell has n^2 threads, possibly fewer than variables: stride over v
     for (int v = tt; act && v < V; v += NN) {
         double r[6];
-        group_ranges<N>(suh, slob, slim, v, r);
-        if (!SEED) {  // DMP, Euler/limiter.f90:198-223 (argmax = 0)
-            const double lmn = b.lmin[(size_t)v * m.ncell + cell], lmx = b.lmax[(size_t)v * m.ncell + cell];
-            const double mn = f_min(f_min(r[0], r[4]), r[2]);
-            const double mx = f_max(f_max(r[1], r[5]), r[3]);
-            const double ldiff = f_max(f_max(1.0e-4, 1.0e-3 * (lmx - lmn)), 0.0);
-            if (mn < lmn - ldiff || mx > lmx + ldiff) atomicOr(&sflag[lc], 2);
-        }
+        group_ranges<P, N>(suh, slob, slim, v, r);
+        if (!SEED && dmp_violated(r, b.lmin[(size_t)v * m.ncell + cell], b.lmax[(size_t)v * m.ncell + cell]))
+            atomicOr(&sflag[lc], 2);
         // next step's get_min_max step 1 for a cell without limiter, Euler/limiter.f90:123-132
         double mn = f_min(f_min(r[0], r[2]), r[4]), mx = f_max(f_max(r[1], r[3]), r[5]);
-        if (SEED && st_cur[cell] != 0) {  // limited cell: range of its subcell data, :135-139
+        if (SEED && b.status_cur[cell] != 0) {  // limited cell: range of its subcell data, :135-139
             const double *L = b.lh_cur + (size_t)cell * V * SS;
             mn = __builtin_nan("");
             mx = mn;
@@ -570,14 +627,96 @@ __global__ __launch_bounds__(kBlock) void k_spread(MeshArgs m, StepBufs b)
     if (take) b.list[bbase + wbase[wave] + below] = cell;
 }
 
+// Common tail of both subcell kernels: put_subcell_data (Euler/limiter.f90:86-96),
+// re-DMP of cells marked 2 (:619-638), next-step limiter seeds and dt candidate.
+template <class P, int N>
+ADG_DEV void subcell_finish(const MeshArgs &m, const StepBufs &b, int cell, int flag, const double *sSub, double *sUhy,
+                            double *sUh, double *sTy, double *sBy, double *sLim, double *sLob, double *sDt, int *sFlag)
+{
+    using S = Shape<P, N>;
+    constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, V = S::V;
+    const Operators *__restrict__ op = b.op;
+    const int lane = threadIdx.x;
+    for (int task = lane; task < s * n; task += 64) {
+        const int p = task % s, bb = task / s;
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int qq = 0; qq < s; qq++) acc = acc + sSub[v * SS + p + s * qq] * op->l2u[bb + n * qq];
+            sUhy[v * s * n + p + s * bb] = acc;
+        }
+    }
+    __syncthreads();
+    for (int task = lane; task < NN; task += 64) {
+        const int a = task % n, bb = task / n;
+        double un[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int p = 0; p < s; p++) acc = acc + sUhy[v * s * n + p + s * bb] * op->l2u[a + n * p];
+            un[v] = acc;
+            sUh[v * NN + task] = acc;
+            b.uh_next[(size_t)cell * V * NN + V * task + v] = acc;
+        }
+        sDt[task] = node_dt<P>(m, un);
+    }
+    if (lane == 0) *sFlag = 0;
+    __syncthreads();
+    cell_ranges_stage1<P, N>(op, sUh, sTy, sBy, lane, 64);
+    __syncthreads();
+    const bool bad = cell_ranges_stage2<P, N>(op, m, sTy, sBy, sLim, sLob, lane, 64);
+    if (bad) atomicOr(sFlag, 1);
+    __syncthreads();
+    for (int v = lane; v < V; v += 64) {
+        double r[6];
+        group_ranges<P, N>(sUh, sLob, sLim, v, r);
+        if (dmp_violated(r, b.lmin[(size_t)v * m.ncell + cell], b.lmax[(size_t)v * m.ncell + cell])) atomicOr(sFlag, 2);
+    }
+    __syncthreads();
+    const int status = (flag == 1) ? 1 : (*sFlag != 0 ? 1 : 0);
+    for (int v = lane; v < V; v += 64) {
+        double mn, mx;
+        if (status) {  // next get_min_max uses Lh (:135-139)
+            mn = __builtin_nan("");
+            mx = mn;
+            for (int k = 0; k < SS; k++) {
+                mn = fold_min(mn, sSub[v * SS + k]);
+                mx = fold_max(mx, sSub[v * SS + k]);
+            }
+        } else {
+            double r[6];
+            group_ranges<P, N>(sUh, sLob, sLim, v, r);
+            mn = f_min(f_min(r[0], r[2]), r[4]);
+            mx = f_max(f_max(r[1], r[3]), r[5]);
+        }
+        b.lmin_old[(size_t)v * m.ncell + cell] = mn;
+        b.lmax_old[(size_t)v * m.ncell + cell] = mx;
+    }
+    if (lane == 63) {
+        double d = __builtin_nan("");
+        for (int k = 0; k < NN; k++) d = fold_min(d, sDt[k]);
+        b.cell_dt[cell] = d;
+        b.status_next[cell] = status;
+    }
+    if (status) {  // NewLh, and Lh = NewLh at update_limiter (:287)
+        double *L = b.lh_next + (size_t)cell * V * SS;
+        for (int k = lane; k < V * SS; k += 64) L[k] = sSub[(k % V) * SS + k / V];
+    }
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
-// subcell_recompute over the compacted list, one wavefront per cell
+// Euler subcell_recompute over the compacted list, one wavefront per cell
+// (Euler/limiter.f90:411-647)
 // ---------------------------------------------------------------------------
 template <int N>
 __global__ __launch_bounds__(64) void k_subcell(MeshArgs m, StepBufs b)
 {
-    using S = Shape<N>;
-    constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, V = kV, H = s + 2, HH = H * H;
+    using P = EulerPDE;
+    using S = Shape<P, N>;
+    constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, V = S::V, H = s + 2, HH = H * H;
     __shared__ double sLy[5][V * n * s];
     __shared__ double sS0[V * HH];
     __shared__ double sSl[2][V * SS];
@@ -692,8 +831,8 @@ __global__ __launch_bounds__(64) void k_subcell(MeshArgs m, StepBufs b)
                 wLx[v] = S0(v, ii, jj) - 0.5 * sl;
                 wRx[v] = S0(v, ii, jj) + 0.5 * sl;
             }
-            euler_flux(m.gamma, wLx, FLx);
-            euler_flux(m.gamma, wRx, FRx);
+            P::flux(m, wLx, FLx);
+            P::flux(m, wRx, FRx);
 #pragma unroll
             for (int v = 0; v < V; v++) Qt[v] = -(FRx[v] - FLx[v]) / ldx;
 #pragma unroll
@@ -702,8 +841,8 @@ __global__ __launch_bounds__(64) void k_subcell(MeshArgs m, StepBufs b)
                 wLy[v] = S0(v, ii, jj) - 0.5 * sl;
                 wRy[v] = S0(v, ii, jj) + 0.5 * sl;
             }
-            euler_flux(m.gamma, wLy, FLy);
-            euler_flux(m.gamma, wRy, FRy);
+            P::flux(m, wLy, FLy);
+            P::flux(m, wRy, FRy);
 #pragma unroll
             for (int v = 0; v < V; v++) Qt[v] = Qt[v] - (FRy[V + v] - FLy[V + v]) / ldy;
 #pragma unroll
@@ -715,30 +854,44 @@ __global__ __launch_bounds__(64) void k_subcell(MeshArgs m, StepBufs b)
             }
         }
         __syncthreads();
-        // subcell Rusanov fluxes, :571-588
-        for (int task = lane; task < (s + 1) * s; task += 64) {
-            const int ii = task % (s + 1) + 1, jj = task / (s + 1) + 1;
-            double QL[V], QR[V], out[V];
+        // subcell Rusanov fluxes, :571-588 (RusanovFluxS, :361-378)
+        for (int task = lane; task < (s + 1) * s + s * (s + 1); task += 64) {
+            const bool xdir = task < (s + 1) * s;
+            const int t2 = xdir ? task : task - (s + 1) * s;
+            double QL[V], QR[V], FL[2 * V], FR[2 * V];
+            int ii, jj;
+            if (xdir) {
+                ii = t2 % (s + 1) + 1;
+                jj = t2 / (s + 1) + 1;
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    QL[v] = sW[1][v * HH + (ii - 1) + H * jj];
+                    QR[v] = sW[0][v * HH + ii + H * jj];
+                }
+            } else {
+                ii = t2 % s + 1;
+                jj = t2 / s + 1;
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    QL[v] = sW[3][v * HH + ii + H * (jj - 1)];
+                    QR[v] = sW[2][v * HH + ii + H * jj];
+                }
+            }
+            const double nx = xdir ? 1.0 : 0.0, ny = xdir ? 0.0 : 1.0;
+            const double smax = f_max(P::maxabs_eig(m, QL, nx, ny), P::maxabs_eig(m, QR, nx, ny));
+            P::flux(m, QL, FL);
+            P::flux(m, QR, FR);
 #pragma unroll
             for (int v = 0; v < V; v++) {
-                QL[v] = sW[1][v * HH + (ii - 1) + H * jj];
-                QR[v] = sW[0][v * HH + ii + H * jj];
+                double mm = 0.0;
+                mm = mm + (FL[v] + FR[v]) * nx;
+                mm = mm + (FL[V + v] + FR[V + v]) * ny;
+                const double out = 0.5 * mm - 0.5 * smax * (QR[v] - QL[v]);
+                if (xdir)
+                    sFx[v * (s + 1) * s + t2] = out;
+                else
+                    sFy[v * s * (s + 1) + t2] = out;
             }
-            euler_rusanov_s(m.gamma, QL, QR, 1.0, 0.0, out);
-#pragma unroll
-            for (int v = 0; v < V; v++) sFx[v * (s + 1) * s + task] = out[v];
-        }
-        for (int task = lane; task < s * (s + 1); task += 64) {
-            const int ii = task % s + 1, jj = task / s + 1;
-            double QL[V], QR[V], out[V];
-#pragma unroll
-            for (int v = 0; v < V; v++) {
-                QL[v] = sW[3][v * HH + ii + H * (jj - 1)];
-                QR[v] = sW[2][v * HH + ii + H * jj];
-            }
-            euler_rusanov_s(m.gamma, QL, QR, 0.0, 1.0, out);
-#pragma unroll
-            for (int v = 0; v < V; v++) sFy[v * s * (s + 1) + task] = out[v];
         }
         __syncthreads();
         // finite-volume update, :592-597
@@ -753,80 +906,188 @@ __global__ __launch_bounds__(64) void k_subcell(MeshArgs m, StepBufs b)
         }
 #undef S0
         __syncthreads();
-        // put_subcell_data, Euler/limiter.f90:86-96
-        for (int task = lane; task < s * n; task += 64) {
-            const int p = task % s, bb = task / s;
+        subcell_finish<P, N>(m, b, cell, flag, sSub, sUhy, sUh, sTy, sBy, sLim, sLob, sDt, &sFlag);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Baer-Nunziato subcell_recompute over the compacted list, one wavefront per
+// cell (Baer-Nunziato/limiter.f90:416-596): slopes also on the halo ring, so
+// the subgrid spans -1..s+2 and takes all 8 neighbours; NCP source in the
+// half step and the update; Roe fluctuations at the subcell faces; no
+// boundary overrides.  The half-step states w are recomputed from (S0,
+// slopes, Qt) where needed instead of being stored.
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(64) void k_subcell_bn(MeshArgs m, StepBufs b)
+{
+    using P = BnPDE;
+    using S = Shape<P, N>;
+    constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, V = S::V;
+    constexpr int G = s + 4, GG = G * G;  // subgrid index -1..s+2
+    constexpr int H = s + 2, HH = H * H;  // halo ring 0..s+1
+    constexpr int LY = 9 * V * n * s, SL = 3 * V * HH;
+    constexpr int R0 = LY > SL ? LY : SL;
+    constexpr int DMPB = V * n * s + V * NN + V * SS + V * NN;
+    static_assert(DMPB <= R0, "DMP scratch must fit in the slope region");
+    __shared__ double sS0[V * GG];
+    __shared__ double sR0[R0];     // limy of 9 blocks, then slopes x / y / t, then DMP scratch
+    __shared__ double sSub[V * SS];
+    __shared__ double sUhy[V * s * n];
+    __shared__ double sUh[V * NN];
+    __shared__ double sDt[NN];
+    __shared__ int sFlag;
+    const Operators *__restrict__ op = b.op;
+    const int lane = threadIdx.x;
+    const int count = b.counters[kCntList];
+    const double dt = b.scal[0];
+    const double ldx = m.dx / (double)s, ldy = m.dy / (double)s;
+    double *sLy = sR0;
+    double *slx = sR0, *sly = sR0 + V * HH, *slt = sR0 + 2 * V * HH;
+    for (int item = blockIdx.x; item < count; item += gridDim.x) {
+        const int cell = b.list[item];
+        const int flag = b.rc_final[cell];
+        const int ci = cell % m.imax, cj = cell / m.imax;
+        // blocks (di,dj) of the 3x3 Voronoi neighbourhood, :463-479
+        for (int task = lane; task < 9 * n * s; task += 64) {
+            const int blk = task / (n * s), r = task - blk * n * s, a = r % n, qq = r / n;
+            const int di = blk % 3 - 1, dj = blk / 3 - 1;
+            const int nb = nb_index(ci, di, m.imax, m.wrap) + m.imax * nb_index(cj, dj, m.jmax, m.wrap);
+            if (b.status_cur[nb] != 0) continue;
+            const double *u = b.uh_cur + (size_t)nb * V * NN;
 #pragma unroll
             for (int v = 0; v < V; v++) {
                 double acc = 0.0;
 #pragma unroll
-                for (int qq = 0; qq < s; qq++) acc = acc + sSub[v * SS + p + s * qq] * op->l2u[bb + n * qq];
-                sUhy[v * s * n + p + s * bb] = acc;
+                for (int bb = 0; bb < n; bb++) acc = acc + u[v + V * (a + n * bb)] * op->u2l[qq + s * bb];
+                sLy[(blk * V + v) * n * s + a + n * qq] = acc;
             }
         }
         __syncthreads();
-        for (int task = lane; task < NN; task += 64) {
-            const int a = task % n, bb = task / n;
-            double un[V];
+        for (int task = lane; task < GG; task += 64) {  // subuh0 on -1..s+2
+            const int ii = task % G - 1, jj = task / G - 1;
+            const int di = ii < 1 ? -1 : (ii > s ? 1 : 0), dj = jj < 1 ? -1 : (jj > s ? 1 : 0);
+            const int p = ii - 1 - di * s, qq = jj - 1 - dj * s;
+            const int blk = (di + 1) + 3 * (dj + 1);
+            const int nb = nb_index(ci, di, m.imax, m.wrap) + m.imax * nb_index(cj, dj, m.jmax, m.wrap);
+            if (b.status_cur[nb] == 0) {
 #pragma unroll
-            for (int v = 0; v < V; v++) {
-                double acc = 0.0;
+                for (int v = 0; v < V; v++) {
+                    double acc = 0.0;
 #pragma unroll
-                for (int p = 0; p < s; p++) acc = acc + sUhy[v * s * n + p + s * bb] * op->l2u[a + n * p];
-                un[v] = acc;
-                sUh[v * NN + task] = acc;
-                b.uh_next[(size_t)cell * V * NN + V * task + v] = acc;
-            }
-            sDt[task] = node_dt(m, un);
-        }
-        if (lane == 0) sFlag = 0;
-        __syncthreads();
-        // DMP of the recomputed polynomial (:619) and the next step's limiter seeds
-        cell_ranges_stage1<N>(op, sUh, sTy, sBy, lane, 64);
-        __syncthreads();
-        const bool bad = cell_ranges_stage2<N>(op, m.gamma, sTy, sBy, sLim, sLob, lane, 64);
-        if (bad) atomicOr(&sFlag, 1);
-        __syncthreads();
-        for (int v = lane; v < V; v += 64) {
-            double r[6];
-            group_ranges<N>(sUh, sLob, sLim, v, r);
-            const double lmn = b.lmin[(size_t)v * m.ncell + cell], lmx = b.lmax[(size_t)v * m.ncell + cell];
-            const double mn = f_min(f_min(r[0], r[4]), r[2]);
-            const double mx = f_max(f_max(r[1], r[5]), r[3]);
-            const double ldiff = f_max(f_max(1.0e-4, 1.0e-3 * (lmx - lmn)), 0.0);
-            if (mn < lmn - ldiff || mx > lmx + ldiff) atomicOr(&sFlag, 2);
-        }
-        __syncthreads();
-        const int status = (flag == 1) ? 1 : (sFlag != 0 ? 1 : 0);
-        for (int v = lane; v < V; v += 64) {
-            double mn, mx;
-            if (status) {  // next get_min_max uses Lh (:135-139)
-                mn = __builtin_nan("");
-                mx = mn;
-                for (int k = 0; k < SS; k++) {
-                    mn = fold_min(mn, sSub[v * SS + k]);
-                    mx = fold_max(mx, sSub[v * SS + k]);
+                    for (int a = 0; a < n; a++) acc = acc + sLy[(blk * V + v) * n * s + a + n * qq] * op->u2l[p + s * a];
+                    sS0[v * GG + task] = acc;
                 }
             } else {
-                double r[6];
-                group_ranges<N>(sUh, sLob, sLim, v, r);
-                mn = f_min(f_min(r[0], r[2]), r[4]);
-                mx = f_max(f_max(r[1], r[3]), r[5]);
+                const double *L = b.lh_cur + (size_t)nb * V * SS;
+#pragma unroll
+                for (int v = 0; v < V; v++) sS0[v * GG + task] = L[v + V * (p + s * qq)];
             }
-            b.lmin_old[(size_t)v * m.ncell + cell] = mn;
-            b.lmax_old[(size_t)v * m.ncell + cell] = mx;
-        }
-        if (lane == 63) {
-            double d = __builtin_nan("");
-            for (int k = 0; k < NN; k++) d = fold_min(d, sDt[k]);
-            b.cell_dt[cell] = d;
-            b.status_next[cell] = status;
-        }
-        if (status) {  // NewLh, and Lh = NewLh at update_limiter (:287)
-            double *L = b.lh_next + (size_t)cell * V * SS;
-            for (int k = lane; k < V * SS; k += 64) L[k] = sSub[(k % V) * SS + k / V];
         }
         __syncthreads();
+#define S0(v, ii, jj) sS0[(v) * GG + ((ii) + 1) + G * ((jj) + 1)]
+#define HIX(ii, jj) ((ii) + H * (jj))
+        // slopes, NCP source and half-step time derivative on 0..s+1, :486-513
+        for (int task = lane; task < HH; task += 64) {
+            const int ii = task % H, jj = task / H;
+            double sx[V], sy[V], Gr[2 * V], src[V], q0[V], wL[V], wR[V], FL[2 * V], FR[2 * V], Qt[V];
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                sx[v] = minmod(S0(v, ii + 1, jj) - S0(v, ii, jj), S0(v, ii, jj) - S0(v, ii - 1, jj));
+                sy[v] = minmod(S0(v, ii, jj + 1) - S0(v, ii, jj), S0(v, ii, jj) - S0(v, ii, jj - 1));
+                Gr[v] = sx[v] / ldx;
+                Gr[V + v] = sy[v] / ldy;
+                q0[v] = S0(v, ii, jj);
+            }
+            P::ncp(m, q0, Gr, src);
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                wL[v] = q0[v] - 0.5 * sx[v];
+                wR[v] = q0[v] + 0.5 * sx[v];
+            }
+            P::flux(m, wL, FL);
+            P::flux(m, wR, FR);
+#pragma unroll
+            for (int v = 0; v < V; v++) Qt[v] = src[v] - (FR[v] - FL[v]) / ldx;
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                wL[v] = q0[v] - 0.5 * sy[v];
+                wR[v] = q0[v] + 0.5 * sy[v];
+            }
+            P::flux(m, wL, FL);
+            P::flux(m, wR, FR);
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                Qt[v] = Qt[v] - (FR[V + v] - FL[V + v]) / ldy;
+                slx[v * HH + task] = sx[v];
+                sly[v * HH + task] = sy[v];
+                slt[v * HH + task] = Qt[v];
+            }
+        }
+        __syncthreads();
+        // half-step face states: w = (S0 -/+ 0.5 slope) + 0.5 dt Qt, as the reference stores them
+        auto wstate = [&](int v, int ii, int jj, int dir, int right) {
+            const double sl = dir == 0 ? slx[v * HH + HIX(ii, jj)] : sly[v * HH + HIX(ii, jj)];
+            const double w0 = right ? S0(v, ii, jj) + 0.5 * sl : S0(v, ii, jj) - 0.5 * sl;
+            return w0 + 0.5 * dt * slt[v * HH + HIX(ii, jj)];
+        };
+        // update of the interior subcells, :516-537: four RusanovFluxS per subcell
+        for (int task = lane; task < SS; task += 64) {
+            const int ii = task % s + 1, jj = task / s + 1;
+            double Gr[2 * V], qh[V], src[V], acc_f[V], acc_d[V];
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                Gr[v] = slx[v * HH + HIX(ii, jj)] / ldx;
+                Gr[V + v] = sly[v * HH + HIX(ii, jj)] / ldy;
+                qh[v] = S0(v, ii, jj) + 0.5 * dt * slt[v * HH + HIX(ii, jj)];
+            }
+            P::ncp(m, qh, Gr, src);
+            double Fp_[V], Fm_[V], Gp_[V], Gm_[V], Dpx[V], Dmx[V], Dpy[V], Dmy[V];
+            for (int fc = 0; fc < 4; fc++) {
+                double QL[V], QR[V], FL[2 * V], FR[2 * V], B9[V], dQ[V];
+                const int dir = fc < 2 ? 0 : 1;
+                const double nx = dir == 0 ? 1.0 : 0.0, ny = dir == 0 ? 0.0 : 1.0;
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    if (fc == 0) { QL[v] = wstate(v, ii, jj, 0, 1); QR[v] = wstate(v, ii + 1, jj, 0, 0); }
+                    else if (fc == 1) { QL[v] = wstate(v, ii - 1, jj, 0, 1); QR[v] = wstate(v, ii, jj, 0, 0); }
+                    else if (fc == 2) { QL[v] = wstate(v, ii, jj, 1, 1); QR[v] = wstate(v, ii, jj + 1, 1, 0); }
+                    else { QL[v] = wstate(v, ii, jj - 1, 1, 1); QR[v] = wstate(v, ii, jj, 1, 0); }
+                }
+                // RusanovFluxS, Baer-Nunziato/limiter.f90:359-383
+                const double smax = f_max(P::maxabs_eig(m, QL, nx, ny), P::maxabs_eig(m, QR, nx, ny));
+                P::flux(m, QL, FL);
+                P::flux(m, QR, FR);
+                P::roe_col9(m, QL, QR, nx, ny, B9);
+#pragma unroll
+                for (int v = 0; v < V; v++) dQ[v] = QR[v] - QL[v];
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    double mm = 0.0;
+                    mm = mm + (FL[v] + FR[v]) * nx;
+                    mm = mm + (FL[V + v] + FR[V + v]) * ny;
+                    acc_f[v] = 0.5 * mm - 0.5 * smax * dQ[v];
+                    acc_d[v] = P::roe_times(B9, v, dQ);
+                }
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    if (fc == 0) { Fp_[v] = acc_f[v]; Dpx[v] = acc_d[v]; }
+                    else if (fc == 1) { Fm_[v] = acc_f[v]; Dmx[v] = acc_d[v]; }
+                    else if (fc == 2) { Gp_[v] = acc_f[v]; Dpy[v] = acc_d[v]; }
+                    else { Gm_[v] = acc_f[v]; Dmy[v] = acc_d[v]; }
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < V; v++)
+                sSub[v * SS + task] = S0(v, ii, jj) - dt / ldx * (Fp_[v] - Fm_[v]) - dt / ldy * (Gp_[v] - Gm_[v]) -
+                                      dt / ldx * 0.5 * (Dpx[v] + Dmx[v]) - dt / ldy * 0.5 * (Dpy[v] + Dmy[v]) +
+                                      dt * src[v];
+        }
+#undef S0
+#undef HIX
+        __syncthreads();
+        double *sTy = sR0, *sBy = sTy + V * n * s, *sLim = sBy + V * NN, *sLob = sLim + V * SS;
+        subcell_finish<P, N>(m, b, cell, flag, sSub, sUhy, sUh, sTy, sBy, sLim, sLob, sDt, &sFlag);
     }
 }
 
@@ -878,29 +1139,32 @@ __global__ void k_advance(StepBufs b)
     if (threadIdx.x == 0 && blockIdx.x == 0) b.scal[1] = b.scal[1] + b.scal[0];  // time = time + dt
 }
 
-template <int N>
+template <class P, int N>
 hipError_t launch_step_impl(const MeshArgs &m, const StepBufs &b, hipStream_t st, int part)
 {
-    using S = Shape<N>;
+    using S = Shape<P, N>;
     const int gcell = (m.ncell + S::CPW - 1) / S::CPW;
     const int gupd = (m.ncell + S::CPWU - 1) / S::CPWU;
     switch (part) {
     case kPartPredictor:
-        hipLaunchKernelGGL((k_predictor<N, false>), dim3(gcell), dim3(kBlock), 0, st, m, b, TestBufs{});
+        hipLaunchKernelGGL((k_predictor<P, N, false>), dim3(gcell), dim3(S::PRED_BLOCK), 0, st, m, b, TestBufs{});
         break;
     case kPartUpdate:
-        hipLaunchKernelGGL((k_update<N, false>), dim3(gupd), dim3(kUpdBlock), 0, st, m, b);
+        hipLaunchKernelGGL((k_update<P, N, false>), dim3(gupd), dim3(S::UPD_BLOCK), 0, st, m, b);
         break;
     case kPartSpread:
         hipLaunchKernelGGL(k_spread, dim3((m.ncell + kBlock - 1) / kBlock), dim3(kBlock), 0, st, m, b);
         break;
     case kPartSubcell: {
         const int g = m.ncell < kSubcellGrid ? m.ncell : kSubcellGrid;
-        hipLaunchKernelGGL((k_subcell<N>), dim3(g), dim3(64), 0, st, m, b);
+        if (P::kNcp)
+            hipLaunchKernelGGL((k_subcell_bn<N>), dim3(g), dim3(64), 0, st, m, b);
+        else
+            hipLaunchKernelGGL((k_subcell<N>), dim3(g), dim3(64), 0, st, m, b);
         break;
     }
     case kPartSeed:
-        hipLaunchKernelGGL((k_update<N, true>), dim3(gupd), dim3(kUpdBlock), 0, st, m, b);
+        hipLaunchKernelGGL((k_update<P, N, true>), dim3(gupd), dim3(S::UPD_BLOCK), 0, st, m, b);
         break;
     default:
         return hipErrorInvalidValue;
@@ -908,13 +1172,37 @@ hipError_t launch_step_impl(const MeshArgs &m, const StepBufs &b, hipStream_t st
     return hipGetLastError();
 }
 
-template <int N>
+template <class P, int N>
 hipError_t launch_test_predictor_impl(const MeshArgs &m, const StepBufs &b, const TestBufs &t, hipStream_t st)
 {
-    using S = Shape<N>;
+    using S = Shape<P, N>;
     const int gcell = (m.ncell + S::CPW - 1) / S::CPW;
-    hipLaunchKernelGGL((k_predictor<N, true>), dim3(gcell), dim3(kBlock), 0, st, m, b, t);
+    hipLaunchKernelGGL((k_predictor<P, N, true>), dim3(gcell), dim3(S::PRED_BLOCK), 0, st, m, b, t);
     return hipGetLastError();
+}
+
+template <class P>
+hipError_t launch_part_pde(int N, const MeshArgs &m, const StepBufs &b, hipStream_t st, int part)
+{
+    switch (N) {
+    case 1: return launch_step_impl<P, 1>(m, b, st, part);
+    case 2: return launch_step_impl<P, 2>(m, b, st, part);
+    case 3: return launch_step_impl<P, 3>(m, b, st, part);
+    case 4: return launch_step_impl<P, 4>(m, b, st, part);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <class P>
+hipError_t launch_test_pde(int N, const MeshArgs &m, const StepBufs &b, const TestBufs &t, hipStream_t st)
+{
+    switch (N) {
+    case 1: return launch_test_predictor_impl<P, 1>(m, b, t, st);
+    case 2: return launch_test_predictor_impl<P, 2>(m, b, t, st);
+    case 3: return launch_test_predictor_impl<P, 3>(m, b, t, st);
+    case 4: return launch_test_predictor_impl<P, 4>(m, b, t, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace
@@ -929,13 +1217,7 @@ hipError_t launch_part(int N, const MeshArgs &m, const StepBufs &b, hipStream_t 
         hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, b);
         return hipGetLastError();
     }
-    switch (N) {
-    case 1: return launch_step_impl<1>(m, b, st, part);
-    case 2: return launch_step_impl<2>(m, b, st, part);
-    case 3: return launch_step_impl<3>(m, b, st, part);
-    case 4: return launch_step_impl<4>(m, b, st, part);
-    default: return hipErrorInvalidValue;
-    }
+    return m.pde == kPdeBn ? launch_part_pde<BnPDE>(N, m, b, st, part) : launch_part_pde<EulerPDE>(N, m, b, st, part);
 }
 
 hipError_t launch_dt(const MeshArgs &m, const StepBufs &b, hipStream_t st, double time, double tend, int use_host_time)
@@ -946,13 +1228,7 @@ hipError_t launch_dt(const MeshArgs &m, const StepBufs &b, hipStream_t st, doubl
 
 hipError_t launch_test_predictor(int N, const MeshArgs &m, const StepBufs &b, const TestBufs &t, hipStream_t st)
 {
-    switch (N) {
-    case 1: return launch_test_predictor_impl<1>(m, b, t, st);
-    case 2: return launch_test_predictor_impl<2>(m, b, t, st);
-    case 3: return launch_test_predictor_impl<3>(m, b, t, st);
-    case 4: return launch_test_predictor_impl<4>(m, b, t, st);
-    default: return hipErrorInvalidValue;
-    }
+    return m.pde == kPdeBn ? launch_test_pde<BnPDE>(N, m, b, t, st) : launch_test_pde<EulerPDE>(N, m, b, t, st);
 }
 
 }  // namespace aderdg

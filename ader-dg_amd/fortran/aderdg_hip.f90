@@ -13,7 +13,7 @@ module aderdg_hip
     use iso_c_binding
     implicit none
 
-    integer(c_int), parameter :: ADERDG_PDE_EULER = 0
+    integer(c_int), parameter :: ADERDG_PDE_EULER = 0, ADERDG_PDE_BN = 1
     integer(c_int), parameter :: ADERDG_BC_DMR = 0, ADERDG_BC_PERIODIC = 1, ADERDG_BC_TRANSMISSIVE = 2
 
     type, bind(C) :: aderdg_ops_t
@@ -30,6 +30,7 @@ module aderdg_hip
         real(c_double) :: gamma = 1.4d0, cfl = 0.9d0
         integer(c_int) :: exact = 0, device = 0
         type(aderdg_ops_t) :: ops
+        real(c_double) :: bn_eos(4) = (/ 1.4d0, 1.4d0, 0.0d0, 0.0d0 /)   ! GAMMA_S, GAMMA_G, PI_S, PI_G
     end type
 
     interface

@@ -69,6 +69,22 @@ def cpu_baseline(wl, target_seconds):
             "sample": "%s IC, %dx%d cells, N=%d, %s, %d steps in %.1f s, one process" % (ic, si, sj, N, bc, steps, el)}
 
 
+def copy_probe(nbytes):
+    """One device-to-device copy of known size (the HIP runtime's copy kernel): the
+    FETCH_SIZE / WRITE_SIZE it reads under rocprofv3 --pmc calibrate the counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log("copy probe: %d bytes device to device" % nbytes)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +94,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true", help="time graph replay instead of evented launches")
+    ap.add_argument("--copy-probe-bytes", type=int, default=0,
+                    help="after the run, copy this many bytes device to device (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; --gpus %d is not supported "
@@ -134,6 +152,8 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds)
+    if args.copy_probe_bytes:
+        copy_probe(args.copy_probe_bytes)
     print(json.dumps(out), flush=True)
 
 

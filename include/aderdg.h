@@ -33,7 +33,9 @@ extern "C" {
 #define ADERDG_ESTATE -4   /* call out of order */
 #define ADERDG_ECAPACITY -5/* a fixed-size buffer is too small */
 
-enum aderdg_pde { ADERDG_PDE_EULER = 0 };
+/* 2-D Euler (nVar 4, Euler/) and 2-D Baer-Nunziato two-phase flow (nVar 9,
+ * Baer-Nunziato/, path-conservative fluxes, transmissive or periodic boundaries) */
+enum aderdg_pde { ADERDG_PDE_EULER = 0, ADERDG_PDE_BN = 1 };
 /* Boundary modes.  DMR reproduces the reference at HEAD exactly
  * (Euler/riemann_problem.f90:10-116 and the ghost overrides of
  * Euler/limiter.f90:491-522).  PERIODIC wraps both the face coupling and the
@@ -67,11 +69,13 @@ typedef struct {
     int imax, jmax; /* cells per direction                               */
     double xl[2], xr[2];
     int bc_mode;    /* enum aderdg_bc                                    */
-    double gamma;   /* EQN%GAMMA                                         */
+    double gamma;   /* EQN%GAMMA (Euler)                                 */
     double cfl;     /* CFL (Euler/ader_dg.f90:19), 0.9 in the reference  */
     int exact;      /* reserved, must be 0                               */
     int device;     /* HIP device ordinal                                */
     aderdg_ops ops; /* optional host operator matrices                   */
+    double bn_eos[4]; /* Baer-Nunziato EQN: GAMMA_S, GAMMA_G, PI_S, PI_G
+                         (Baer-Nunziato/initialize.f90:23-26)            */
 } aderdg_desc;
 
 /* Setup: replaces the allocation and operator part of `initialize`
@@ -102,7 +106,7 @@ int aderdg_initial_condition(int ictype, int N, int imax, int jmax, const double
 int aderdg_get_ops(aderdg_ctx *ctx, double *wGPN, double *Kxi, double *iK1, double *F0, double *FLcoeff,
                    double *FRcoeff, double *uh2lim, double *lim2uh, double *uh2lob);
 
-/* Upload the solution and reset the limiter to "no troubled cell"
+/* Upload the solution (nVar = 4 Euler, 9 Baer-Nunziato) and reset the limiter to "no troubled cell"
  * (Euler/initialize.f90:160-165).  Seeds the per-cell min/max and time-step
  * candidates that the next aderdg_calc_time_step reduces. */
 int aderdg_upload_uh(aderdg_ctx *ctx, const double *uh);
@@ -154,6 +158,11 @@ int aderdg_run(aderdg_ctx *ctx, int nsteps, double time, double tend, double *ti
  * reference routine cell by cell. */
 int aderdg_predictor_cells(aderdg_ctx *ctx, int ncells, double dt, const double *luh, double *lqhi, double *lFhi,
                            double *lqbnd, double *lFbnd);
+
+/* Baer-Nunziato form of aderdg_predictor_cells with the argument list of
+ * Baer-Nunziato/space_time_predictor.f90:8-18 (lShi = time-averaged NCP source). */
+int aderdg_predictor_cells_bn(aderdg_ctx *ctx, int ncells, double dt, const double *luh, double *lqhi, double *lFhi,
+                              double *lShi, double *lqbnd, double *lFbnd);
 
 /* Kernel timing: when enabled, the kernels of each step are bracketed with
  * hipEvents on the context stream.  Returns accumulated milliseconds and

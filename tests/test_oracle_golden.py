@@ -151,3 +151,18 @@ def test_flop_model_matches_op_counter():
         counted = Oracle.opcount()
         fp, fu = aderdg.flops_per_cell(N)
         assert counted == fp, (N, counted, fp)
+
+
+def test_bn_flop_model_matches_op_counter():
+    """Baer-Nunziato predictor + volume flop count used by the roofline equals the oracle's counter."""
+    from helpers import BN_EOS_IDEAL, random_bn_cells
+    for N in (1, 2, 3, 4):
+        o = Oracle(N, 4, 4, (0.0, 0.0), (1.0, 1.0), 2, pde=1, eos=BN_EOS_IDEAL)
+        o.set_scalars(0.001, 0.0, 1e9)
+        luh = random_bn_cells(np.random.default_rng(N), N, 1)[..., 0]
+        Oracle.reset_opcount()
+        p = o.predictor(luh)
+        o.volume(p["Fhi"], p["Shi"])
+        counted = Oracle.opcount()
+        fp, fu = aderdg.flops_per_cell(N, "bn")
+        assert counted == fp, (N, counted, fp)
