@@ -24,6 +24,8 @@ LIBDIR = os.path.join(os.path.dirname(HERE), "lib")
 BC = {"dmr": 0, "periodic": 1, "transmissive": 2}
 PDE = {"euler": 0, "bn": 1}
 NVAR = {0: 4, 1: 9}
+# kernels timed by aderdg_timing_get, in its slot order
+TIMING_SLOTS = ("dt", "predictor", "update", "spread", "subcell", "face")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -253,7 +255,8 @@ class Solver:
         self._check(self.lib.aderdg_timing_enable(self.h, 1 if on else 0))
 
     def timing_get(self):
-        ms = (ctypes.c_double * 5)()
-        ln = (ctypes.c_int64 * 5)()
+        """(ms, launches) per kernel in the order of TIMING_SLOTS."""
+        ms = (ctypes.c_double * len(TIMING_SLOTS))()
+        ln = (ctypes.c_int64 * len(TIMING_SLOTS))()
         self._check(self.lib.aderdg_timing_get(self.h, ms, ln))
         return list(ms), list(ln)

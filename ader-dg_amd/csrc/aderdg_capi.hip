@@ -57,6 +57,7 @@ struct aderdg_ctx {
     hipStream_t stream = nullptr;
     double *scal = nullptr, *pinned = nullptr;
     double *uh[2] = {nullptr, nullptr}, *lduh = nullptr, *fq = nullptr, *ff = nullptr;
+    double *xflux = nullptr, *yflux = nullptr;
     double *lmin_old = nullptr, *lmax_old = nullptr, *lmin = nullptr, *lmax = nullptr, *cell_dt = nullptr,
            *dt_partial = nullptr;
     int *rc = nullptr, *rc_final = nullptr, *list = nullptr, *counters = nullptr;
@@ -70,8 +71,8 @@ struct aderdg_ctx {
     bool timing = false;
     std::vector<TimedLaunch> pending;
     std::vector<hipEvent_t> free_events;
-    double ms[5] = {0, 0, 0, 0, 0};
-    int64_t launches[5] = {0, 0, 0, 0, 0};
+    double ms[ADERDG_TIMING_SLOTS] = {};
+    int64_t launches[ADERDG_TIMING_SLOTS] = {};
 };
 
 namespace {
@@ -87,6 +88,8 @@ StepBufs bufs(aderdg_ctx *c)
     b.lduh = c->lduh;
     b.fq = c->fq;
     b.ff = c->ff;
+    b.xflux = c->xflux;
+    b.yflux = c->yflux;
     b.lmin_old = c->lmin_old;
     b.lmax_old = c->lmax_old;
     b.lmin = c->lmin;
@@ -119,8 +122,15 @@ hipEvent_t get_event(aderdg_ctx *c)
 int launch(aderdg_ctx *c, int part, double time = 0.0, double tend = 0.0, int use_host = 0)
 {
     const StepBufs b = bufs(c);
-    const int slot = part == kPartDt ? 0 : part == kPartPredictor ? 1 : part == kPartUpdate ? 2 : part == kPartSpread ? 3
-                                                                                                                   : part == kPartSubcell ? 4 : -1;
+    int slot = -1;  // timing slots: dt, predictor, update, spread, subcell, face
+    switch (part) {
+    case kPartDt: slot = 0; break;
+    case kPartPredictor: slot = 1; break;
+    case kPartUpdate: slot = 2; break;
+    case kPartSpread: slot = 3; break;
+    case kPartSubcell: slot = 4; break;
+    case kPartFace: slot = 5; break;
+    }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->timing && slot >= 0) {
         e0 = get_event(c);
@@ -188,6 +198,7 @@ int step_parts(aderdg_ctx *c)
 {
     int r;
     if ((r = launch(c, kPartPredictor))) return r;
+    if ((r = launch(c, kPartFace))) return r;
     if ((r = launch(c, kPartUpdate))) return r;
     if ((r = launch(c, kPartSpread))) return r;
     if ((r = launch(c, kPartSubcell))) return r;
@@ -326,6 +337,9 @@ int aderdg_create(const aderdg_desc *d, aderdg_ctx **out)
     r = r ? r : dalloc(&c->lduh, V * NN * nc);
     r = r ? r : dalloc(&c->fq, 4 * V * n * nc);
     r = r ? r : dalloc(&c->ff, 4 * V * n * nc);
+    const size_t nside = d->pde == ADERDG_PDE_BN ? 2 : 1;
+    r = r ? r : dalloc(&c->xflux, nside * V * n * (size_t)(d->imax + 1) * d->jmax);
+    r = r ? r : dalloc(&c->yflux, nside * V * n * (size_t)d->imax * (d->jmax + 1));
     r = r ? r : dalloc(&c->lmin_old, V * nc);
     r = r ? r : dalloc(&c->lmax_old, V * nc);
     r = r ? r : dalloc(&c->lmin, V * nc);
@@ -361,7 +375,7 @@ int aderdg_destroy(aderdg_ctx *c)
     destroy_graphs(c);
     collect_timing(c);
     for (auto e : c->free_events) (void)hipEventDestroy(e);
-    void *ptrs[] = {c->ops_dev, c->scal, c->uh[0], c->uh[1], c->lduh, c->fq, c->ff, c->lmin_old, c->lmax_old, c->lmin,
+    void *ptrs[] = {c->ops_dev, c->scal, c->uh[0], c->uh[1], c->lduh, c->fq, c->ff, c->xflux, c->yflux, c->lmin_old, c->lmax_old, c->lmin,
                     c->lmax, c->cell_dt, c->dt_partial, c->rc, c->rc_final, c->list, c->counters, c->status[0],
                     c->status[1], c->lh[0], c->lh[1]};
     for (void *p : ptrs)
@@ -507,7 +521,8 @@ int aderdg_space_time_predictor(aderdg_ctx *c)
 int aderdg_riemann_surface_update(aderdg_ctx *c, int32_t *nrec)
 {
     if (!c) return fail(ADERDG_EINVAL, "aderdg_riemann_surface_update: null context");
-    int r = launch(c, kPartUpdate);
+    int r = launch(c, kPartFace);
+    if (!r) r = launch(c, kPartUpdate);
     if (r) return r;
     if (nrec) {
         HIPCHK(hipMemcpyAsync(c->pinned + 4, c->counters, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -638,19 +653,19 @@ int aderdg_timing_enable(aderdg_ctx *c, int on)
     int r = sync(c);
     if (r) return r;
     c->timing = on != 0;
-    for (int k = 0; k < 5; k++) {
+    for (int k = 0; k < ADERDG_TIMING_SLOTS; k++) {
         c->ms[k] = 0.0;
         c->launches[k] = 0;
     }
     return 0;
 }
 
-int aderdg_timing_get(aderdg_ctx *c, double ms[5], int64_t launches[5])
+int aderdg_timing_get(aderdg_ctx *c, double ms[ADERDG_TIMING_SLOTS], int64_t launches[ADERDG_TIMING_SLOTS])
 {
     if (!c) return fail(ADERDG_EINVAL, "aderdg_timing_get: null context");
     int r = sync(c);
     if (r) return r;
-    for (int k = 0; k < 5; k++) {
+    for (int k = 0; k < ADERDG_TIMING_SLOTS; k++) {
         if (ms) ms[k] = c->ms[k];
         if (launches) launches[k] = c->launches[k];
     }

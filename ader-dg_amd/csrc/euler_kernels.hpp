@@ -15,7 +15,7 @@ enum { kPdeEuler = 0, kPdeBn = 1 };
 enum { kBcDmr = 0, kBcPeriodic = 1, kBcTransmissive = 2 };
 enum { kCntList = 0, kCntTroubled = 1, kCntDtBlocks = 2, kNumCounters = 4 };
 enum { kPartDt = 0, kPartPredictor = 1, kPartUpdate = 2, kPartSpread = 3, kPartSubcell = 4, kPartAdvance = 5,
-       kPartSeed = 6 };
+       kPartSeed = 6, kPartFace = 7 };
 
 // Mesh and equation constants, passed by value.
 struct MeshArgs {
@@ -32,6 +32,7 @@ struct StepBufs {
     double *scal;  // [0] dt, [1] time, [2] tend
     const double *uh_cur;
     double *uh_next, *lduh, *fq, *ff;
+    double *xflux, *yflux;  // [side][v][p][face]: side 0 D- (Euler: the flux), side 1 D+ (Baer-Nunziato)
     double *lmin_old, *lmax_old, *lmin, *lmax, *cell_dt, *dt_partial;
     int *rc, *rc_final, *list, *counters;
     const int *status_cur;

@@ -47,9 +47,9 @@ struct Shape {
     static constexpr int CPWU = UPD_BLOCK / NN;   // cells per update workgroup
     // predictor LDS per cell and time slice: Fx, Fy (and q for the NCP gradients)
     static constexpr int SLOT = (P::kNcp ? 3 : 2) * V * NN;
-    // per-cell LDS of the update: suh | (face fluxes, later limy + loby) | lim | lob | dt candidates
-    static constexpr int R1 = (4 * V * n > V * n * s + V * NN) ? 4 * V * n : V * n * s + V * NN;
-    static constexpr int UPD_LDS = V * NN + R1 + V * SS + V * NN + NN;
+    // per-cell LDS of the update: suh | limy | loby | per-thread partial ranges | reduced ranges
+    static constexpr int NPART = 6 * V + 1;  // (min,max) of nodal / Lobatto / subcell values per v, dt
+    static constexpr int UPD_LDS = V * NN + V * n * s + V * NN + NPART * NN + NPART;
 };
 
 ADG_DEV double dmr_shock_x(const MeshArgs &m, double time) { return m.dmr_a + m.dmr_b * time; }
@@ -111,14 +111,22 @@ __global__ __launch_bounds__(kPredBlock<P>) void k_predictor(MeshArgs m, StepBuf
         for (int l = 0; l < n; l++) q[v][l] = u[v];
     }
     int slot = 0;
-    // Picard iterations, Euler/space_time_predictor.f90:49-84 (Baer-Nunziato/space_time_predictor.f90:54-104)
+    // Picard iterations, Euler/space_time_predictor.f90:49-84 (Baer-Nunziato/space_time_predictor.f90:54-104).
+    // Both loops stay rolled: the time line q rotates so that the current slice is always q[:,0],
+    // and the time contraction with iK1 accumulates slice by slice (summation order over m
+    // unchanged), which keeps the register footprint at two time lines.
+#pragma unroll 1
     for (int it = 0; it <= N; it++) {
-        double r[V][n];
+        double qn[V][n];
 #pragma unroll
+        for (int v = 0; v < V; v++)
+#pragma unroll
+            for (int ll = 0; ll < n; ll++) qn[v][ll] = 0.0;
+#pragma unroll 1
         for (int l = 0; l < n; l++) {
             double Q[V], F[2 * V];
 #pragma unroll
-            for (int v = 0; v < V; v++) Q[v] = q[v][l];
+            for (int v = 0; v < V; v++) Q[v] = q[v][0];
             P::flux(m, Q, F);
             double *buf = sm[slot][lcs];
             if (own) {
@@ -129,18 +137,18 @@ __global__ __launch_bounds__(kPredBlock<P>) void k_predictor(MeshArgs m, StepBuf
                     if (P::kNcp) buf[2 * V * NN + v * NN + tt] = Q[v];
                 }
             }
+            const double wl = op->w[l];
             if (it == N) {  // time average of the last sweep's fluxes (:92)
 #pragma unroll
                 for (int v = 0; v < V; v++) {
-                    fh[0][v] = fh[0][v] + F[v] * op->w[l];
-                    fh[1][v] = fh[1][v] + F[V + v] * op->w[l];
+                    fh[0][v] = fh[0][v] + F[v] * wl;
+                    fh[1][v] = fh[1][v] + F[V + v] * wl;
                 }
             }
             __syncthreads();
-            const double wl = op->w[l];
             const double sx = wl * wj * dt / m.dx, sy = wl * wi * dt / m.dy;
             const double c0 = wi * wj * op->f0[l];
-            double G[2 * V];
+            double G[2 * V], rv[V];
 #pragma unroll
             for (int v = 0; v < V; v++) {
                 double acc = 0.0;
@@ -150,8 +158,8 @@ __global__ __launch_bounds__(kPredBlock<P>) void k_predictor(MeshArgs m, StepBuf
                 acc = 0.0;
 #pragma unroll
                 for (int k = 0; k < n; k++) acc = acc + buf[V * NN + v * NN + i * n + k] * kx_j[k];
-                r[v][l] = rr - sy * acc;  // :72
-                if (P::kNcp) {            // gradients, Baer-Nunziato/space_time_predictor.f90:72,79
+                rv[v] = rr - sy * acc;  // :72
+                if (P::kNcp) {          // gradients, Baer-Nunziato/space_time_predictor.f90:72,79
                     acc = 0.0;
 #pragma unroll
                     for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + k + n * j] * du_i[k];
@@ -168,9 +176,22 @@ __global__ __launch_bounds__(kPredBlock<P>) void k_predictor(MeshArgs m, StepBuf
                 const double f = wi * wj * wl * dt;
 #pragma unroll
                 for (int v = 0; v < V; v++) {
-                    r[v][l] = r[v][l] + f * src[v];
+                    rv[v] = rv[v] + f * src[v];
                     if (it == N) sh[v] = sh[v] + src[v] * wl;  // lShi (:111)
                 }
+            }
+            // time contraction, :79-83: q_new(v,ll) = sum over m of rhs(v,m) iK1(ll,m), m = l here
+#pragma unroll
+            for (int v = 0; v < V; v++)
+#pragma unroll
+                for (int ll = 0; ll < n; ll++) qn[v][ll] = qn[v][ll] + rv[v] * op->ik1[ll + n * l];
+            // rotate the time line: slice l+1 moves to position 0
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                const double q0 = q[v][0];
+#pragma unroll
+                for (int ll = 0; ll + 1 < n; ll++) q[v][ll] = q[v][ll + 1];
+                q[v][n - 1] = q0;
             }
             slot ^= 1;
         }
@@ -178,12 +199,7 @@ __global__ __launch_bounds__(kPredBlock<P>) void k_predictor(MeshArgs m, StepBuf
 #pragma unroll
         for (int v = 0; v < V; v++)
 #pragma unroll
-            for (int l = 0; l < n; l++) {
-                double acc = 0.0;
-#pragma unroll
-                for (int mm = 0; mm < n; mm++) acc = acc + r[v][mm] * op->ik1[l + n * mm];
-                q[v][l] = f * acc;
-            }
+            for (int ll = 0; ll < n; ll++) q[v][ll] = f * qn[v][ll];
     }
     double qh[V];  // :90
 #pragma unroll
@@ -286,91 +302,114 @@ __global__ __launch_bounds__(kPredBlock<P>) void k_predictor(MeshArgs m, StepBuf
     }
 }
 
-// Face states of one Riemann problem, with the boundary treatment of the mode.
-// face: 0 left, 1 right (x faces), 2 bottom, 3 top (y faces) of cell (ci,cj).
-template <class P>
-ADG_DEV void face_states(const MeshArgs &m, const StepBufs &b, int n, int face, int p, int cell, int ci, int cj,
-                         double time, double *QL, double *FL, double *QR, double *FR)
+// ---------------------------------------------------------------------------
+// Face fluxes (solve_riemann_problem, Euler/riemann_problem.f90:10-116 and
+// Baer-Nunziato/riemann_problem.f90:9-80): one thread per face point, reading
+// the structure-of-arrays face extrapolations of the two neighbouring cells
+// with consecutive lanes on consecutive cells, writing the flux (Euler) or
+// the fluctuations D-/D+ (Baer-Nunziato) in the same order.
+// ---------------------------------------------------------------------------
+template <class P, int N>
+__global__ __launch_bounds__(256) void k_face(MeshArgs m, StepBufs b)
 {
-    constexpr int V = P::V;
-    auto ld = [&](int f, int c, double *Q, double *F) {
-#pragma unroll
-        for (int v = 0; v < V; v++) {
-            const size_t o = ((size_t)(f * V + v) * n + p) * m.ncell + c;
-            Q[v] = b.fq[o];
-            F[v] = b.ff[o];
-        }
-    };
-    auto copy = [&](const double *Qs, const double *Fs, double *Qd, double *Fd) {
-#pragma unroll
-        for (int v = 0; v < V; v++) {
-            Qd[v] = Qs[v];
-            Fd[v] = Fs[v];
-        }
-    };
+    constexpr int V = P::V, n = N + 1;
     const int I = m.imax, J = m.jmax;
+    const long nfx = (long)(I + 1) * J, nfy = (long)I * (J + 1);
+    const long total = (nfx + nfy) * n;
+    const double time = b.scal[1];
     const bool dmr = V == 4 && m.bc_mode == kBcDmr;
-    if (face == 0) {
-        ld(0, cell, QR, FR);
-        if (ci > 0) {
-            ld(1, cell - 1, QL, FL);
-        } else if (m.bc_mode == kBcPeriodic) {
-            ld(1, cell + I - 1, QL, FL);
-        } else if (!dmr) {
-            copy(QR, FR, QL, FL);
-        } else {  // DMR inflow, Euler/riemann_problem.f90:186-193
-            double Qbc[V], Fbc[2 * V];
-#pragma unroll
-            for (int v = 0; v < V; v++) Qbc[v] = m.qa[v % 4];
-            P::flux(m, Qbc, Fbc);
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+        const bool xdir = t < nfx * n;
+        const long t2 = xdir ? t : t - nfx * n;
+        const long nf = xdir ? nfx : nfy;
+        const int p = (int)(t2 / nf);
+        const long fidx = t2 - (long)p * nf;
+        double QL[V], FL[V], QR[V], FR[V], Dm[V], Dp[V];
+        auto ld = [&](int f, long c, double *Q, double *F) {
 #pragma unroll
             for (int v = 0; v < V; v++) {
-                QL[v] = Qbc[v];
-                FL[v] = 0.0 + Fbc[v] * 1.0 + Fbc[V + v] * 0.0;
+                const size_t o = ((size_t)(f * V + v) * n + p) * m.ncell + c;
+                Q[v] = b.fq[o];
+                F[v] = b.ff[o];
             }
-        }
-    } else if (face == 1) {
-        ld(1, cell, QL, FL);
-        if (ci < I - 1)
-            ld(0, cell + 1, QR, FR);
-        else if (m.bc_mode == kBcPeriodic)
-            ld(0, cell - (I - 1), QR, FR);
-        else
-            copy(QL, FL, QR, FR);  // transmissive, also the DMR right boundary (:195-198)
-    } else if (face == 2) {
-        ld(2, cell, QR, FR);
-        if (cj > 0) {
-            ld(3, cell - I, QL, FL);
-        } else if (m.bc_mode == kBcPeriodic) {
-            ld(3, cell + (J - 1) * I, QL, FL);
+        };
+        auto copy = [&](const double *Qs, const double *Fs, double *Qd, double *Fd) {
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                Qd[v] = Qs[v];
+                Fd[v] = Fs[v];
+            }
+        };
+        if (xdir) {
+            const int fi = (int)(fidx % (I + 1)), j = (int)(fidx / (I + 1));
+            if (fi > 0) ld(1, (fi - 1) + (long)I * j, QL, FL);
+            if (fi < I) ld(0, fi + (long)I * j, QR, FR);
+            if (fi == 0) {
+                if (m.bc_mode == kBcPeriodic) {
+                    ld(1, (I - 1) + (long)I * j, QL, FL);
+                } else if (!dmr) {
+                    copy(QR, FR, QL, FL);
+                } else {  // DMR inflow, Euler/riemann_problem.f90:186-193
+                    double Qbc[V], Fbc[2 * V];
+#pragma unroll
+                    for (int v = 0; v < V; v++) Qbc[v] = m.qa[v % 4];
+                    P::flux(m, Qbc, Fbc);
+#pragma unroll
+                    for (int v = 0; v < V; v++) {
+                        QL[v] = Qbc[v];
+                        FL[v] = 0.0 + Fbc[v] * 1.0 + Fbc[V + v] * 0.0;
+                    }
+                }
+            }
+            if (fi == I) {
+                if (m.bc_mode == kBcPeriodic)
+                    ld(0, (long)I * j, QR, FR);
+                else
+                    copy(QL, FL, QR, FR);  // transmissive, also the DMR right boundary (:195-198)
+            }
+            P::face_flux(m, QL, FL, QR, FR, 1.0, 0.0, Dm, Dp);
         } else {
-            copy(QR, FR, QL, FL);
-            if (dmr && !((double)(ci + 1) * m.dx <= 1.0 / 6.0)) {  // reflective, :225-228
-                QL[2] = -QL[2];
-                FL[0] = -FL[0];
-                FL[1] = -FL[1];
-                FL[3] = -FL[3];
+            const int i = (int)(fidx % I), fj = (int)(fidx / I);
+            if (fj > 0) ld(3, i + (long)I * (fj - 1), QL, FL);
+            if (fj < J) ld(2, i + (long)I * fj, QR, FR);
+            if (fj == 0) {
+                if (m.bc_mode == kBcPeriodic) {
+                    ld(3, i + (long)I * (J - 1), QL, FL);
+                } else {
+                    copy(QR, FR, QL, FL);
+                    if (dmr && !((double)(i + 1) * m.dx <= 1.0 / 6.0)) {  // reflective, :225-228
+                        QL[2] = -QL[2];
+                        FL[0] = -FL[0];
+                        FL[1] = -FL[1];
+                        FL[3] = -FL[3];
+                    }
+                }
             }
+            if (fj == J) {
+                if (m.bc_mode == kBcPeriodic) {
+                    ld(2, i, QR, FR);
+                } else if (!dmr) {
+                    copy(QL, FL, QR, FR);
+                } else {  // DMR top: exact shock position, :233-245
+                    const bool pre = (double)(i + 1) * m.dx <= dmr_shock_x(m, time);
+                    double Qbc[V], Fbc[2 * V];
+#pragma unroll
+                    for (int v = 0; v < V; v++) Qbc[v] = pre ? m.qa[v % 4] : m.qb[v % 4];
+                    P::flux(m, Qbc, Fbc);
+#pragma unroll
+                    for (int v = 0; v < V; v++) {
+                        QR[v] = Qbc[v];
+                        FR[v] = 0.0 + Fbc[v] * 0.0 + Fbc[V + v] * 1.0;
+                    }
+                }
+            }
+            P::face_flux(m, QL, FL, QR, FR, 0.0, 1.0, Dm, Dp);
         }
-    } else {
-        ld(3, cell, QL, FL);
-        if (cj < J - 1) {
-            ld(2, cell + I, QR, FR);
-        } else if (m.bc_mode == kBcPeriodic) {
-            ld(2, cell - (J - 1) * I, QR, FR);
-        } else if (!dmr) {
-            copy(QL, FL, QR, FR);
-        } else {  // DMR top: exact shock position, :233-245
-            const bool pre = (double)(ci + 1) * m.dx <= dmr_shock_x(m, time);
-            double Qbc[V], Fbc[2 * V];
+        double *out = xdir ? b.xflux : b.yflux;
 #pragma unroll
-            for (int v = 0; v < V; v++) Qbc[v] = pre ? m.qa[v % 4] : m.qb[v % 4];
-            P::flux(m, Qbc, Fbc);
-#pragma unroll
-            for (int v = 0; v < V; v++) {
-                QR[v] = Qbc[v];
-                FR[v] = 0.0 + Fbc[v] * 0.0 + Fbc[V + v] * 1.0;
-            }
+        for (int v = 0; v < V; v++) {
+            out[((size_t)(0 * V + v) * n + p) * nf + fidx] = Dm[v];
+            if (P::kNcp) out[((size_t)(1 * V + v) * n + p) * nf + fidx] = Dp[v];
         }
     }
 }
@@ -478,14 +517,17 @@ ADG_DEV double node_dt(const MeshArgs &m, const double *Q)
 }
 
 // ---------------------------------------------------------------------------
-// Riemann + surface integral + element update + DMP (SEED = false), or only
-// the per-cell limiter seeds of a freshly uploaded solution (SEED = true).
+// surface integral + element update + DMP (SEED = false), or only the
+// per-cell limiter seeds of a freshly uploaded solution (SEED = true).
+// Thread (cell, i, j) updates node (i,j).  The ranges the DMP needs (min/max
+// of nodal, Lobatto and subcell values) are folded per thread while the maps
+// are evaluated and reduced across the cell's threads afterwards.
 // ---------------------------------------------------------------------------
 template <class P, int N, bool SEED>
 __global__ __launch_bounds__(kUpdBlock<P>) void k_update(MeshArgs m, StepBufs b)
 {
     using S = Shape<P, N>;
-    constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, CPW = S::CPWU, V = S::V;
+    constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, CPW = S::CPWU, V = S::V, NP = S::NPART;
     __shared__ double lds[CPW][S::UPD_LDS];
     __shared__ int sflag[CPW];
     const Operators *__restrict__ op = b.op;
@@ -495,67 +537,105 @@ __global__ __launch_bounds__(kUpdBlock<P>) void k_update(MeshArgs m, StepBufs b)
     const bool own = lc < CPW;
     const bool act = own && cell < m.ncell;
     const int lcs = own ? lc : CPW - 1;
-    const int ci = act ? cell % m.imax : 0, cj = act ? cell / m.imax : 0;
-    const double time = b.scal[1];
     double *suh = &lds[lcs][0];
-    double *sflx = suh + V * NN;     // [f][v][p], dead once the update is done
-    double *sty = sflx;              // limy [v][a + n q] reuses the face-flux space
-    double *sby = sty + V * n * s;   // loby
-    double *slim = suh + V * NN + S::R1;
-    double *slob = slim + V * SS;
-    double *sdt = slob + V * NN;
-
-    if (!SEED && act) {  // Riemann problems of the cell's 4 faces, Euler/riemann_problem.f90:177-257
-        for (int task = tt; task < 4 * n; task += NN) {
-            const int f = task / n, p = task - f * n;
-            double QL[V], FL[V], QR[V], FR[V], Dm[V], Dp[V];
-            face_states<P>(m, b, n, f, p, cell, ci, cj, time, QL, FL, QR, FR);
-            P::face_flux(m, QL, FL, QR, FR, f < 2 ? 1.0 : 0.0, f < 2 ? 0.0 : 1.0, Dm, Dp);
-            // left/bottom faces take D- of their face, right/top faces D+ (Baer-Nunziato/main.f90:68)
-            const bool minus = (f == 0 || f == 2);
-#pragma unroll
-            for (int v = 0; v < V; v++) sflx[(f * V + v) * n + p] = minus ? Dm[v] : Dp[v];
-        }
-    }
+    double *sty = suh + V * NN;     // limy [v][a + n q]
+    double *sby = sty + V * n * s;  // loby
+    double *spart = sby + V * NN;   // [slot][tt]
+    double *sred = spart + NP * NN; // [slot]
     if (own && tt == 0) sflag[lc] = 0;
-    __syncthreads();
 
     double un[V];
-    if (SEED) {
+    if (SEED || !act) {
 #pragma unroll
         for (int v = 0; v < V; v++) un[v] = act ? b.uh_cur[(size_t)cell * V * NN + V * tt + v] : (v == 0 || v == 3 ? 1.0 : 0.0);
     } else {
+        const int ci = cell % m.imax, cj = cell / m.imax;
+        const long nfx = (long)(m.imax + 1) * m.jmax, nfy = (long)m.imax * (m.jmax + 1);
+        const long fl = ci + (long)(m.imax + 1) * cj, fb = ci + (long)m.imax * cj;
+        const int sp = P::kNcp ? 1 : 0;  // right/top faces take D+ (Baer-Nunziato/main.f90:68)
         const double wi = op->w[i], wj = op->w[j], dt = b.scal[0];
 #pragma unroll
         for (int v = 0; v < V; v++) {
-            double d = act ? b.lduh[(size_t)cell * V * NN + V * tt + v] : 0.0;
-            const double uo = act ? b.uh_cur[(size_t)cell * V * NN + V * tt + v] : (v == 0 || v == 3 ? 1.0 : 0.0);
-            if (act) {  // surface integral, Euler/surface_integral.f90:182-196
-                d = d - wj / m.dx * (sflx[(1 * V + v) * n + j] * op->frc[i] - sflx[(0 * V + v) * n + j] * op->flc[i]);
-                d = d - wi / m.dy * (sflx[(3 * V + v) * n + i] * op->frc[j] - sflx[(2 * V + v) * n + i] * op->flc[j]);
-            }
+            double d = b.lduh[(size_t)cell * V * NN + V * tt + v];
+            const double uo = b.uh_cur[(size_t)cell * V * NN + V * tt + v];
+            const double Fl = b.xflux[((size_t)(0 * V + v) * n + j) * nfx + fl];
+            const double Fr = b.xflux[((size_t)(sp * V + v) * n + j) * nfx + fl + 1];
+            const double Gb = b.yflux[((size_t)(0 * V + v) * n + i) * nfy + fb];
+            const double Gt = b.yflux[((size_t)(sp * V + v) * n + i) * nfy + fb + m.imax];
+            // surface integral, Euler/surface_integral.f90:182-196
+            d = d - wj / m.dx * (Fr * op->frc[i] - Fl * op->flc[i]);
+            d = d - wi / m.dy * (Gt * op->frc[j] - Gb * op->flc[j]);
             d = d / (wi * wj);  // element update, Euler/element_update.f90:218-226
             un[v] = uo + dt * d;
-            if (act) b.uh_next[(size_t)cell * V * NN + V * tt + v] = un[v];
+            b.uh_next[(size_t)cell * V * NN + V * tt + v] = un[v];
         }
     }
-    __syncthreads();  // face fluxes consumed: their space becomes limy/loby
     if (own) {
 #pragma unroll
         for (int v = 0; v < V; v++) suh[v * NN + tt] = un[v];
-        sdt[tt] = act ? node_dt<P>(m, un) : 1.0;
     }
+    const double dtc = act ? node_dt<P>(m, un) : __builtin_nan("");
     __syncthreads();
     if (own) cell_ranges_stage1<P, N>(op, suh, sty, sby, tt, NN);
     __syncthreads();
+    // second passes folded into per-thread (min, max) of Lobatto and subcell values
+    double b0[V], b1[V], l0[V], l1[V];
     bool bad = false;
-    if (act) bad = cell_ranges_stage2<P, N>(op, m, sty, sby, slim, slob, tt, NN);
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        b0[v] = b1[v] = l0[v] = l1[v] = __builtin_nan("");
+    }
+    if (act) {
+        for (int task = tt; task < SS; task += NN) {  // get_subcell_data :33-35, positivity :227-241
+            const int p = task % s, qq = task / s;
+            double L[V];
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int a = 0; a < n; a++) acc = acc + sty[v * n * s + a + n * qq] * op->u2l[p + s * a];
+                L[v] = acc;
+                l0[v] = fold_min(l0[v], acc);
+                l1[v] = fold_max(l1[v], acc);
+            }
+            bad = bad || P::inadmissible(m, L);
+        }
+        {  // get_lobatto_data :64-66, one node per thread
+            const int p = tt % n, qq = tt / n;
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int a = 0; a < n; a++) acc = acc + sby[v * NN + a + n * qq] * op->u2b[p + n * a];
+                b0[v] = acc;
+                b1[v] = acc;
+            }
+        }
+    }
+    if (own) {
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            spart[(0 * V + v) * NN + tt] = un[v];
+            spart[(1 * V + v) * NN + tt] = un[v];
+            spart[(2 * V + v) * NN + tt] = b0[v];
+            spart[(3 * V + v) * NN + tt] = b1[v];
+            spart[(4 * V + v) * NN + tt] = l0[v];
+            spart[(5 * V + v) * NN + tt] = l1[v];
+        }
+        spart[6 * V * NN + tt] = dtc;
+    }
     if (bad && !SEED) atomicOr(&sflag[lc], 1);
     __syncthreads();
-    // a cell has n^2 threads, possibly fewer than variables: stride over v
+    for (int slot = tt; own && slot < NP; slot += NN) {  // reduce the partials of the cell's threads
+        const bool is_max = slot < 6 * V && ((slot / V) & 1);
+        double r = __builtin_nan("");
+        for (int k = 0; k < NN; k++) r = is_max ? fold_max(r, spart[slot * NN + k]) : fold_min(r, spart[slot * NN + k]);
+        sred[slot] = r;
+    }
+    __syncthreads();
     for (int v = tt; act && v < V; v += NN) {
-        double r[6];
-        group_ranges<P, N>(suh, slob, slim, v, r);
+        const double r[6] = {sred[0 * V + v], sred[1 * V + v], sred[2 * V + v],
+                             sred[3 * V + v], sred[4 * V + v], sred[5 * V + v]};
         if (!SEED && dmp_violated(r, b.lmin[(size_t)v * m.ncell + cell], b.lmax[(size_t)v * m.ncell + cell]))
             atomicOr(&sflag[lc], 2);
         // next step's get_min_max step 1 for a cell without limiter, Euler/limiter.f90:123-132
@@ -572,11 +652,7 @@ __global__ __launch_bounds__(kUpdBlock<P>) void k_update(MeshArgs m, StepBufs b)
         b.lmin_old[(size_t)v * m.ncell + cell] = mn;
         b.lmax_old[(size_t)v * m.ncell + cell] = mx;
     }
-    if (act && tt == NN - 1) {
-        double d = __builtin_nan("");
-        for (int k = 0; k < NN; k++) d = fold_min(d, sdt[k]);
-        b.cell_dt[cell] = d;
-    }
+    if (act && tt == NN - 1) b.cell_dt[cell] = sred[6 * V];
     __syncthreads();
     if (!SEED && act && tt == 0) {
         const int tr = sflag[lc] != 0;
@@ -1149,6 +1225,12 @@ hipError_t launch_step_impl(const MeshArgs &m, const StepBufs &b, hipStream_t st
     case kPartPredictor:
         hipLaunchKernelGGL((k_predictor<P, N, false>), dim3(gcell), dim3(S::PRED_BLOCK), 0, st, m, b, TestBufs{});
         break;
+    case kPartFace: {
+        const long pts = ((long)(m.imax + 1) * m.jmax + (long)m.imax * (m.jmax + 1)) * S::n;
+        const long g = (pts + 255) / 256;
+        hipLaunchKernelGGL((k_face<P, N>), dim3(g < 65536 ? g : 65536), dim3(256), 0, st, m, b);
+        break;
+    }
     case kPartUpdate:
         hipLaunchKernelGGL((k_update<P, N, false>), dim3(gupd), dim3(S::UPD_BLOCK), 0, st, m, b);
         break;

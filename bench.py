@@ -25,6 +25,10 @@ sys.path.insert(0, os.path.join(ROOT, "ader-dg_amd"))
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix) dense peak
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per k_predictor launch, from rocprofv3 --pmc FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0),
+# scaled by a 256 MiB device copy; profiles/hbm_traffic_euler_vortex_512_n4.md holds the run.
+PREDICTOR_TRAFFIC_BYTES = {"euler_vortex_512_n4": 965.7e6}
+
 WORKLOADS = {
     # name: (ic, N, imax, jmax, xl, xr, bc)
     "euler_vortex_512_n4": ("vortex", 4, 512, 512, (0.0, 0.0), (10.0, 10.0), "periodic"),
@@ -122,15 +126,15 @@ def main():
     dof_per_cell = 4 * (N + 1) ** 2
     value = args.steps * ncell * dof_per_cell / el
     fl_pred, fl_upd = aderdg.flops_per_cell(N)
-    names = ["dt", "predictor", "update", "spread", "subcell"]
-    per = {names[k]: (ms[k] / launches[k] if launches[k] else None) for k in range(5)}
+    per = {name: (ms[k] / launches[k] if launches[k] else None) for k, name in enumerate(aderdg.TIMING_SLOTS)}
     log("per-kernel average ms:", json.dumps(per))
     log("simulated time %.6g after %d steps, last dt %.6g, nRecompute %d" % (t1_sim, args.steps, dt, nrec))
     roofline = None
     if per["predictor"]:
         achieved = fl_pred * ncell / (per["predictor"] * 1e-3) / 1e12
         roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_predictor",
+                    "frac": achieved / FP64_PEAK_TFLOPS, "traffic": PREDICTOR_TRAFFIC_BYTES.get(args.workload),
+                    "traffic_unit": "bytes/launch", "kernel": "k_predictor",
                     "flop_per_cell": fl_pred, "cells_per_launch": ncell,
                     "note": "FP64 vector ALU; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s"}
     out = {

@@ -166,10 +166,12 @@ int aderdg_predictor_cells_bn(aderdg_ctx *ctx, int ncells, double dt, const doub
 
 /* Kernel timing: when enabled, the kernels of each step are bracketed with
  * hipEvents on the context stream.  Returns accumulated milliseconds and
- * launch counts per kernel: [0] dt reduction, [1] predictor, [2] update,
- * [3] spread/compact, [4] subcell recompute. */
+ * launch counts per kernel: [0] dt reduction, [1] predictor, [2] surface
+ * integral + update + DMP, [3] spread/compact, [4] subcell recompute,
+ * [5] face Riemann fluxes. */
+#define ADERDG_TIMING_SLOTS 6
 int aderdg_timing_enable(aderdg_ctx *ctx, int on);
-int aderdg_timing_get(aderdg_ctx *ctx, double ms[5], int64_t launches[5]);
+int aderdg_timing_get(aderdg_ctx *ctx, double ms[ADERDG_TIMING_SLOTS], int64_t launches[ADERDG_TIMING_SLOTS]);
 
 /* Algorithmic floating-point operations of one cell-step of the predictor
  * launch (predictor + volume integral) and of the update launch, counted as

@@ -58,7 +58,9 @@ def run_pair(N, I, J, uh0, steps, exact, eos=BN_EOS_STIFF, bitwise=True):
                 np.testing.assert_array_equal(la[key], lb[key], err_msg="%s step %d" % (key, k))
             np.testing.assert_array_equal(a, b, err_msg="uh step %d" % k)
         else:
-            assert abs(dt - dto) <= TOL * dto
+            # dt comes from p = (g-1)(E - rho u^2/2) - g pi with pi = 6e8 here: the cancellation
+            # leaves the time step a few 1e-12 apart in relative terms; the DOF bar stays 1e-12
+            assert abs(dt - dto) <= 1e-10 * dto, (k, dt, dto)
             np.testing.assert_array_equal(la["status"], lb["status"], err_msg="status step %d" % k)
             assert rel_l2(a, b) <= TOL, (k, rel_l2(a, b))
     s.close()
