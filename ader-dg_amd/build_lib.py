@@ -7,6 +7,7 @@ Produces, in ader-dg_amd/lib/:
   aderdg_euler            the Fortran driver (fortran/aderdg_main.f90) linked
                           against libaderdg_amd.so, when flang is available
 """
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -19,8 +20,8 @@ LIB = os.path.join(HERE, "lib")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ADERDG_ARCH", "gfx950")
 
-SOURCES = ["operators.cpp", "initial.cpp", "step_kernels.hip", "aderdg_capi.hip"]
-COMMON = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared",
+SOURCES = ["operators.cpp", "initial.cpp", "step_kernels.hip", "group.hip", "aderdg_capi.hip"]
+COMMON = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-I" + os.path.join(ROOT, "include")]
 
@@ -30,13 +31,35 @@ def _run(cmd, cwd=None):
     subprocess.run(cmd, check=True, cwd=cwd)
 
 
-def build_library(exact=False):
+def _objects(exact):
+    """Compile every source of one variant to objects, in parallel; returns their paths."""
+    flags = ["-ffp-contract=off", "-DADERDG_EXACT"] if exact else ["-ffp-contract=fast"]
+    odir = os.path.join(LIB, "obj", "exact" if exact else "fast")
+    os.makedirs(odir, exist_ok=True)
+    jobs = []
+    for f in SOURCES:
+        obj = os.path.join(odir, os.path.splitext(f)[0] + ".o")
+        jobs.append(([HIPCC] + COMMON + flags + ["-c", os.path.join(CSRC, f), "-o", obj], obj))
+    return jobs
+
+
+def _compile_all(jobs):
+    workers = min(len(jobs), int(os.environ.get("MAX_JOBS", "8")))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as ex:
+        for fut in [ex.submit(_run, cmd) for cmd, _ in jobs]:
+            fut.result()
+
+
+def build_library(exact=False, compiled=False):
+    """Link libaderdg_amd[_exact].so; compiles its objects first unless compiled."""
     os.makedirs(LIB, exist_ok=True)
     name = "libaderdg_amd_exact.so" if exact else "libaderdg_amd.so"
     out = os.path.join(LIB, name)
-    flags = ["-ffp-contract=off", "-DADERDG_EXACT"] if exact else ["-ffp-contract=fast"]
+    jobs = _objects(exact)
+    if not compiled:
+        _compile_all(jobs)
     tmp = out + ".tmp"
-    _run([HIPCC] + COMMON + flags + [os.path.join(CSRC, f) for f in SOURCES] + ["-o", tmp])
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC"] + [o for _, o in jobs] + ["-o", tmp])
     os.replace(tmp, out)
     return out
 
@@ -57,8 +80,9 @@ def build_fortran_driver(exact=False):
 
 
 def build_all():
-    build_library(exact=False)
-    build_library(exact=True)
+    _compile_all(_objects(False) + _objects(True))
+    build_library(exact=False, compiled=True)
+    build_library(exact=True, compiled=True)
     if os.path.exists(os.path.join(HERE, "fortran", "aderdg_main.f90")):
         build_fortran_driver(False)
         build_fortran_driver(True)

@@ -1,7 +1,9 @@
-// aderdg_capi.hip -- the C-ABI of include/aderdg.h: context, resident device
-// state, the step sequence of Euler/main.f90:24-100 and graph replay.
+// aderdg_capi.hip -- the C-ABI of include/aderdg.h for one solve: context,
+// resident device state, the step sequence of Euler/main.f90:24-100 and graph
+// replay.  The context is a solve group (group.hpp) holding a single solve.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -10,14 +12,37 @@
 #include <vector>
 
 #include "../../include/aderdg.h"
-#include "euler_kernels.hpp"
+#include "capi_common.hpp"
+#include "group.hpp"
 #include "operators.hpp"
+#include "step_kernels.hpp"
 
 using namespace aderdg;
 
 namespace {
 
 thread_local std::string g_err;
+
+// PNPMTable of Euler/ader_dg.f90:28
+const double kPNPM[10] = {1.0, 0.33, 0.17, 0.1, 0.069, 0.045, 0.038, 0.03, 0.02, 0.015};
+
+struct TimedLaunch {
+    int part;
+    hipEvent_t a, b;
+};
+
+// DMR states of Euler/riemann_problem.f90:165-175, PDEPrim2Cons of Euler/pde.f90:29-40
+void prim2cons(double g, const double *V, double *Q)
+{
+    Q[0] = V[0];
+    Q[1] = V[0] * V[1];
+    Q[2] = V[0] * V[2];
+    Q[3] = V[3] / (g - 1.0) + 0.5 * V[0] * (V[1] * V[1] + V[2] * V[2]);
+}
+
+}  // namespace
+
+namespace aderdg {
 
 int fail(int code, const char *fmt, ...)
 {
@@ -30,43 +55,80 @@ int fail(int code, const char *fmt, ...)
     return code;
 }
 
-#define HIPCHK(expr)                                                                                   \
-    do {                                                                                               \
-        hipError_t e_ = (expr);                                                                        \
-        if (e_ != hipSuccess) return fail(ADERDG_EHIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
-                                          __FILE__, __LINE__);                                         \
-    } while (0)
+int check_desc(const aderdg_desc *d, const char *who)
+{
+    if (!d) return fail(ADERDG_EINVAL, "%s: null descriptor", who);
+    if (d->pde != ADERDG_PDE_EULER && d->pde != ADERDG_PDE_BN)
+        return fail(ADERDG_EINVAL, "%s: unsupported pde %d", who, d->pde);
+    if (d->pde == ADERDG_PDE_BN && d->bc_mode == ADERDG_BC_DMR)
+        return fail(ADERDG_EINVAL, "%s: the DMR boundaries exist for the Euler system only", who);
+    if (d->pde == ADERDG_PDE_BN && !(d->bn_eos[0] > 1.0 && d->bn_eos[1] > 1.0))
+        return fail(ADERDG_EINVAL, "%s: Baer-Nunziato GAMMA_S and GAMMA_G must exceed 1", who);
+    if (d->N < 1 || d->N > kMaxDegree) return fail(ADERDG_EINVAL, "%s: N=%d outside 1..%d", who, d->N, kMaxDegree);
+    if (d->imax < 1 || d->jmax < 1 || (long long)d->imax * d->jmax > (1LL << 28))
+        return fail(ADERDG_EINVAL, "%s: bad mesh %d x %d", who, d->imax, d->jmax);
+    if (d->bc_mode < 0 || d->bc_mode > 2) return fail(ADERDG_EINVAL, "%s: bad bc_mode %d", who, d->bc_mode);
+    if (!(d->xr[0] > d->xl[0]) || !(d->xr[1] > d->xl[1])) return fail(ADERDG_EINVAL, "%s: empty domain", who);
+    if (d->pde == ADERDG_PDE_EULER && !(d->gamma > 1.0)) return fail(ADERDG_EINVAL, "%s: gamma must exceed 1", who);
+    if (!(d->cfl > 0.0)) return fail(ADERDG_EINVAL, "%s: cfl must be positive", who);
+    if (d->exact != 0) return fail(ADERDG_EINVAL, "%s: exact must be 0", who);
+    return 0;
+}
 
-// PNPMTable of Euler/ader_dg.f90:28
-const double kPNPM[10] = {1.0, 0.33, 0.17, 0.1, 0.069, 0.045, 0.038, 0.03, 0.02, 0.015};
+MeshArgs mesh_from_desc(const aderdg_desc *d)
+{
+    MeshArgs m{};
+    m.pde = d->pde;
+    for (int k = 0; k < 4; k++) m.eos[k] = d->bn_eos[k];
+    m.imax = d->imax;
+    m.jmax = d->jmax;
+    m.ncell = d->imax * d->jmax;
+    m.bc_mode = d->bc_mode;
+    m.wrap = d->bc_mode == ADERDG_BC_PERIODIC;
+    m.dx = (d->xr[0] - d->xl[0]) / (double)d->imax;  // Euler/initialize.f90:43
+    m.dy = (d->xr[1] - d->xl[1]) / (double)d->jmax;
+    m.gamma = d->gamma;
+    m.cflpnpm = d->cfl * kPNPM[d->N];
+    const double pi = 4.0 * std::atan(1.0);
+    m.dmr_a = 1.0 / 6.0 + 1.0 / std::tan(pi / 3.0);
+    m.dmr_b = 10.0 / std::sin(pi / 3.0);
+    const double Va[4] = {8.0, 8.25 * std::cos(pi / 6.0), -8.25 * std::sin(pi / 6.0), 116.5};
+    const double Vb[4] = {1.4, 0.0, 0.0, 1.0};
+    prim2cons(d->gamma, Va, m.qa);
+    prim2cons(d->gamma, Vb, m.qb);
+    return m;
+}
 
-struct TimedLaunch {
-    int part;
-    hipEvent_t a, b;
-};
+int ops_from_desc(const aderdg_desc *d, Operators *o, const char *who)
+{
+    if (!build_operators(d->N, *o)) return fail(ADERDG_ESINGULAR, "%s: operator construction failed (singular matrix)", who);
+    const int n = d->N + 1, s = 2 * d->N + 1;
+    const aderdg_ops &h = d->ops;
+    if (h.wGPN) std::memcpy(o->w, h.wGPN, n * sizeof(double));
+    if (h.Kxi) std::memcpy(o->kxi, h.Kxi, n * n * sizeof(double));
+    if (h.iK1) std::memcpy(o->ik1, h.iK1, n * n * sizeof(double));
+    if (h.F0) std::memcpy(o->f0, h.F0, n * sizeof(double));
+    if (h.FLcoeff) std::memcpy(o->flc, h.FLcoeff, n * sizeof(double));
+    if (h.FRcoeff) std::memcpy(o->frc, h.FRcoeff, n * sizeof(double));
+    if (h.uh2lim) std::memcpy(o->u2l, h.uh2lim, s * n * sizeof(double));
+    if (h.lim2uh) std::memcpy(o->l2u, h.lim2uh, n * s * sizeof(double));
+    if (h.uh2lob) std::memcpy(o->u2b, h.uh2lob, n * n * sizeof(double));
+    return 0;
+}
 
-}  // namespace
+}  // namespace aderdg
 
 struct aderdg_ctx {
     aderdg_desc desc;
     int N, n, s, V;
     int ncell;
-    MeshArgs m;
-    Operators ops;
-    Operators *ops_dev = nullptr;
     hipStream_t stream = nullptr;
-    double *scal = nullptr, *pinned = nullptr;
-    double *uh[2] = {nullptr, nullptr}, *lduh = nullptr, *fq = nullptr, *ff = nullptr;
-    double *xflux = nullptr, *yflux = nullptr;
-    double *lmin_old = nullptr, *lmax_old = nullptr, *lmin = nullptr, *lmax = nullptr, *cell_dt = nullptr,
-           *dt_partial = nullptr;
-    int *rc = nullptr, *rc_final = nullptr, *list = nullptr, *counters = nullptr;
-    int *status[2] = {nullptr, nullptr};
-    double *lh[2] = {nullptr, nullptr};
-    int cur = 0;
+    Group g;
+    int slot = 0;
+    SolveState st{};  // host view of the solve's state; parity mirrors the device
+    double *pinned = nullptr;
     bool have_uh = false;
-    // graph replay, one graph per ping-pong parity
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t graph = nullptr;  // one device-resident step
     // timing
     bool timing = false;
     std::vector<TimedLaunch> pending;
@@ -76,36 +138,6 @@ struct aderdg_ctx {
 };
 
 namespace {
-
-StepBufs bufs(aderdg_ctx *c)
-{
-    const int a = c->cur, z = 1 - c->cur;
-    StepBufs b;
-    b.op = c->ops_dev;
-    b.scal = c->scal;
-    b.uh_cur = c->uh[a];
-    b.uh_next = c->uh[z];
-    b.lduh = c->lduh;
-    b.fq = c->fq;
-    b.ff = c->ff;
-    b.xflux = c->xflux;
-    b.yflux = c->yflux;
-    b.lmin_old = c->lmin_old;
-    b.lmax_old = c->lmax_old;
-    b.lmin = c->lmin;
-    b.lmax = c->lmax;
-    b.cell_dt = c->cell_dt;
-    b.dt_partial = c->dt_partial;
-    b.rc = c->rc;
-    b.rc_final = c->rc_final;
-    b.list = c->list;
-    b.counters = c->counters;
-    b.status_cur = c->status[a];
-    b.status_next = c->status[z];
-    b.lh_cur = c->lh[a];
-    b.lh_next = c->lh[z];
-    return b;
-}
 
 hipEvent_t get_event(aderdg_ctx *c)
 {
@@ -119,18 +151,22 @@ hipEvent_t get_event(aderdg_ctx *c)
     return e;
 }
 
+int timing_slot(int part)
+{
+    switch (part) {  // dt, predictor, update, spread, subcell, face
+    case kPartDt: return 0;
+    case kPartPredictor: return 1;
+    case kPartUpdate: return 2;
+    case kPartSpread: return 3;
+    case kPartSubcell: return 4;
+    case kPartFace: return 5;
+    default: return -1;
+    }
+}
+
 int launch(aderdg_ctx *c, int part, double time = 0.0, double tend = 0.0, int use_host = 0)
 {
-    const StepBufs b = bufs(c);
-    int slot = -1;  // timing slots: dt, predictor, update, spread, subcell, face
-    switch (part) {
-    case kPartDt: slot = 0; break;
-    case kPartPredictor: slot = 1; break;
-    case kPartUpdate: slot = 2; break;
-    case kPartSpread: slot = 3; break;
-    case kPartSubcell: slot = 4; break;
-    case kPartFace: slot = 5; break;
-    }
+    const int slot = timing_slot(part);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->timing && slot >= 0) {
         e0 = get_event(c);
@@ -140,9 +176,9 @@ int launch(aderdg_ctx *c, int part, double time = 0.0, double tend = 0.0, int us
     }
     hipError_t e;
     if (part == kPartDt)
-        e = launch_dt(c->m, b, c->stream, time, tend, use_host);
+        e = launch_dt(c->g.b, c->stream, time, tend, use_host);
     else
-        e = launch_part(c->N, c->m, b, c->stream, part);
+        e = c->g.launch(part);
     if (e != hipSuccess) return fail(ADERDG_EHIP, "kernel launch (part %d): %s", part, hipGetErrorString(e));
     if (e1) {
         HIPCHK(hipEventRecord(e1, c->stream));
@@ -173,25 +209,20 @@ int sync(aderdg_ctx *c)
     return 0;
 }
 
-// DMR states of Euler/riemann_problem.f90:165-175, PDEPrim2Cons of Euler/pde.f90:29-40
-void prim2cons(double g, const double *V, double *Q)
-{
-    Q[0] = V[0];
-    Q[1] = V[0] * V[1];
-    Q[2] = V[0] * V[2];
-    Q[3] = V[3] / (g - 1.0) + 0.5 * V[0] * (V[1] * V[1] + V[2] * V[2]);
-}
-
-template <class T>
-int dalloc(T **p, size_t count)
-{
-    HIPCHK(hipMalloc((void **)p, count * sizeof(T) > 0 ? count * sizeof(T) : 8));
-    return 0;
-}
-
+// Per-cell limiter seeds and time-step candidates of the current solution.
 int seed(aderdg_ctx *c)
 {
-    return launch(c, kPartSeed);
+    c->st.seed = 1;
+    c->st.active = 1;
+    c->st.dtmin = ~0ull;
+    c->st.max_steps = INT_MAX;
+    c->st.stop_at_tend = 0;
+    int r = c->g.write_state(c->slot, c->st);
+    if (r) return fail(r, "aderdg: state upload failed");
+    if ((r = launch(c, kPartSeed))) return r;
+    if ((r = launch(c, kPartSeedDone))) return r;
+    c->st.seed = 0;
+    return 0;
 }
 
 int step_parts(aderdg_ctx *c)
@@ -205,44 +236,23 @@ int step_parts(aderdg_ctx *c)
     return 0;
 }
 
-void destroy_graphs(aderdg_ctx *c)
+// One device-resident step (time, tend, parity read from the device).
+int build_graph(aderdg_ctx *c)
 {
-    for (int k = 0; k < 2; k++)
-        if (c->graph[k]) {
-            (void)hipGraphExecDestroy(c->graph[k]);
-            c->graph[k] = nullptr;
-        }
-}
-
-// One device-resident step (time and tend read from the device), captured per parity.
-int build_graphs(aderdg_ctx *c)
-{
-    const int cur0 = c->cur;
-    for (int k = 0; k < 2; k++) {
-        c->cur = k;
-        hipGraph_t g = nullptr;
-        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int r = launch(c, kPartDt);
-        if (!r) r = step_parts(c);
-        if (!r) r = launch(c, kPartAdvance);
-        hipError_t e = hipStreamEndCapture(c->stream, &g);
-        if (r) {
-            if (g) (void)hipGraphDestroy(g);
-            c->cur = cur0;
-            return r;
-        }
-        if (e != hipSuccess) {
-            c->cur = cur0;
-            return fail(ADERDG_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-        }
-        e = hipGraphInstantiate(&c->graph[k], g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) {
-            c->cur = cur0;
-            return fail(ADERDG_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
-        }
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int r = launch(c, kPartDt);
+    if (!r) r = step_parts(c);
+    if (!r) r = launch(c, kPartAdvance);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (r) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
     }
-    c->cur = cur0;
+    if (e != hipSuccess) return fail(ADERDG_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(ADERDG_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
     return 0;
 }
 
@@ -265,19 +275,8 @@ int aderdg_create(const aderdg_desc *d, aderdg_ctx **out)
 {
     if (!d || !out) return fail(ADERDG_EINVAL, "aderdg_create: null argument");
     *out = nullptr;
-    if (d->pde != ADERDG_PDE_EULER && d->pde != ADERDG_PDE_BN)
-        return fail(ADERDG_EINVAL, "aderdg_create: unsupported pde %d", d->pde);
-    if (d->pde == ADERDG_PDE_BN && d->bc_mode == ADERDG_BC_DMR)
-        return fail(ADERDG_EINVAL, "aderdg_create: the DMR boundaries exist for the Euler system only");
-    if (d->pde == ADERDG_PDE_BN && !(d->bn_eos[0] > 1.0 && d->bn_eos[1] > 1.0))
-        return fail(ADERDG_EINVAL, "aderdg_create: Baer-Nunziato GAMMA_S and GAMMA_G must exceed 1");
-    if (d->N < 1 || d->N > kMaxDegree) return fail(ADERDG_EINVAL, "aderdg_create: N=%d outside 1..%d", d->N, kMaxDegree);
-    if (d->imax < 1 || d->jmax < 1 || (long long)d->imax * d->jmax > (1LL << 30))
-        return fail(ADERDG_EINVAL, "aderdg_create: bad mesh %d x %d", d->imax, d->jmax);
-    if (d->bc_mode < 0 || d->bc_mode > 2) return fail(ADERDG_EINVAL, "aderdg_create: bad bc_mode %d", d->bc_mode);
-    if (!(d->xr[0] > d->xl[0]) || !(d->xr[1] > d->xl[1])) return fail(ADERDG_EINVAL, "aderdg_create: empty domain");
-    if (d->pde == ADERDG_PDE_EULER && !(d->gamma > 1.0)) return fail(ADERDG_EINVAL, "aderdg_create: gamma must exceed 1");
-    if (d->exact != 0) return fail(ADERDG_EINVAL, "aderdg_create: exact must be 0");
+    int r = check_desc(d, "aderdg_create");
+    if (r) return r;
     aderdg_ctx *c = new aderdg_ctx();
     c->desc = *d;
     c->N = d->N;
@@ -285,42 +284,12 @@ int aderdg_create(const aderdg_desc *d, aderdg_ctx **out)
     c->s = 2 * d->N + 1;
     c->V = d->pde == ADERDG_PDE_BN ? 9 : 4;
     c->ncell = d->imax * d->jmax;
-    if (!build_operators(d->N, c->ops)) {
+    Operators ops;
+    if ((r = ops_from_desc(d, &ops, "aderdg_create"))) {
         delete c;
-        return fail(ADERDG_ESINGULAR, "aderdg_create: operator construction failed (singular matrix)");
+        return r;
     }
-    const int n = c->n, s = c->s;
-    const aderdg_ops &o = d->ops;
-    if (o.wGPN) std::memcpy(c->ops.w, o.wGPN, n * sizeof(double));
-    if (o.Kxi) std::memcpy(c->ops.kxi, o.Kxi, n * n * sizeof(double));
-    if (o.iK1) std::memcpy(c->ops.ik1, o.iK1, n * n * sizeof(double));
-    if (o.F0) std::memcpy(c->ops.f0, o.F0, n * sizeof(double));
-    if (o.FLcoeff) std::memcpy(c->ops.flc, o.FLcoeff, n * sizeof(double));
-    if (o.FRcoeff) std::memcpy(c->ops.frc, o.FRcoeff, n * sizeof(double));
-    if (o.uh2lim) std::memcpy(c->ops.u2l, o.uh2lim, s * n * sizeof(double));
-    if (o.lim2uh) std::memcpy(c->ops.l2u, o.lim2uh, n * s * sizeof(double));
-    if (o.uh2lob) std::memcpy(c->ops.u2b, o.uh2lob, n * n * sizeof(double));
-
-    MeshArgs &m = c->m;
-    m.pde = d->pde;
-    for (int k = 0; k < 4; k++) m.eos[k] = d->bn_eos[k];
-    m.imax = d->imax;
-    m.jmax = d->jmax;
-    m.ncell = c->ncell;
-    m.bc_mode = d->bc_mode;
-    m.wrap = d->bc_mode == ADERDG_BC_PERIODIC;
-    m.dx = (d->xr[0] - d->xl[0]) / (double)d->imax;  // Euler/initialize.f90:43
-    m.dy = (d->xr[1] - d->xl[1]) / (double)d->jmax;
-    m.gamma = d->gamma;
-    m.cflpnpm = d->cfl * kPNPM[d->N];
-    const double pi = 4.0 * std::atan(1.0);
-    m.dmr_a = 1.0 / 6.0 + 1.0 / std::tan(pi / 3.0);
-    m.dmr_b = 10.0 / std::sin(pi / 3.0);
-    const double Va[4] = {8.0, 8.25 * std::cos(pi / 6.0), -8.25 * std::sin(pi / 6.0), 116.5};
-    const double Vb[4] = {1.4, 0.0, 0.0, 1.0};
-    prim2cons(d->gamma, Va, m.qa);
-    prim2cons(d->gamma, Vb, m.qb);
-
+    const MeshArgs m = mesh_from_desc(d);
     auto bail = [&](int code) {
         aderdg_destroy(c);
         return code;
@@ -328,42 +297,15 @@ int aderdg_create(const aderdg_desc *d, aderdg_ctx **out)
     if (hipSetDevice(d->device) != hipSuccess) return bail(fail(ADERDG_EHIP, "hipSetDevice(%d) failed", d->device));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(ADERDG_EHIP, "hipStreamCreate failed"));
-    const size_t nc = c->ncell, V = c->V, NN = (size_t)n * n, SS = (size_t)s * s;
-    int r = 0;
-    r = r ? r : dalloc(&c->ops_dev, 1);
-    r = r ? r : dalloc(&c->scal, 4);
-    r = r ? r : dalloc(&c->uh[0], V * NN * nc);
-    r = r ? r : dalloc(&c->uh[1], V * NN * nc);
-    r = r ? r : dalloc(&c->lduh, V * NN * nc);
-    r = r ? r : dalloc(&c->fq, 4 * V * n * nc);
-    r = r ? r : dalloc(&c->ff, 4 * V * n * nc);
-    const size_t nside = d->pde == ADERDG_PDE_BN ? 2 : 1;
-    r = r ? r : dalloc(&c->xflux, nside * V * n * (size_t)(d->imax + 1) * d->jmax);
-    r = r ? r : dalloc(&c->yflux, nside * V * n * (size_t)d->imax * (d->jmax + 1));
-    r = r ? r : dalloc(&c->lmin_old, V * nc);
-    r = r ? r : dalloc(&c->lmax_old, V * nc);
-    r = r ? r : dalloc(&c->lmin, V * nc);
-    r = r ? r : dalloc(&c->lmax, V * nc);
-    r = r ? r : dalloc(&c->cell_dt, nc);
-    r = r ? r : dalloc(&c->dt_partial, (size_t)kDtGrid);
-    r = r ? r : dalloc(&c->rc, nc);
-    r = r ? r : dalloc(&c->rc_final, nc);
-    r = r ? r : dalloc(&c->list, nc);
-    r = r ? r : dalloc(&c->counters, (size_t)kNumCounters);
-    r = r ? r : dalloc(&c->status[0], nc);
-    r = r ? r : dalloc(&c->status[1], nc);
-    r = r ? r : dalloc(&c->lh[0], V * SS * nc);
-    r = r ? r : dalloc(&c->lh[1], V * SS * nc);
-    if (r) return bail(r);
+    const GroupCaps caps{c->ncell, (int64_t)(d->imax + 1) * d->jmax, (int64_t)d->imax * (d->jmax + 1), 1};
+    if ((r = c->g.init(d->pde, d->N, caps, ops, 0, c->stream))) return bail(fail(r, "aderdg_create: device allocation failed"));
+    SolveState st0{};
+    st0.dtmin = ~0ull;
+    st0.max_steps = INT_MAX;
+    if ((r = c->g.add(m, st0, &c->slot))) return bail(fail(r, "aderdg_create: solve placement failed"));
+    c->st = st0;
     if (hipHostMalloc((void **)&c->pinned, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess)
         return bail(fail(ADERDG_EHIP, "hipHostMalloc failed"));
-    if (hipMemcpy(c->ops_dev, &c->ops, sizeof(Operators), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(c->counters, 0, kNumCounters * sizeof(int)) != hipSuccess ||
-        hipMemset(c->scal, 0, 4 * sizeof(double)) != hipSuccess ||
-        hipMemset(c->rc, 0, nc * sizeof(int)) != hipSuccess || hipMemset(c->rc_final, 0, nc * sizeof(int)) != hipSuccess ||
-        hipMemset(c->status[0], 0, nc * sizeof(int)) != hipSuccess ||
-        hipMemset(c->status[1], 0, nc * sizeof(int)) != hipSuccess)
-        return bail(fail(ADERDG_EHIP, "device initialisation failed"));
     *out = c;
     return 0;
 }
@@ -372,14 +314,10 @@ int aderdg_destroy(aderdg_ctx *c)
 {
     if (!c) return 0;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    destroy_graphs(c);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
     collect_timing(c);
     for (auto e : c->free_events) (void)hipEventDestroy(e);
-    void *ptrs[] = {c->ops_dev, c->scal, c->uh[0], c->uh[1], c->lduh, c->fq, c->ff, c->xflux, c->yflux, c->lmin_old, c->lmax_old, c->lmin,
-                    c->lmax, c->cell_dt, c->dt_partial, c->rc, c->rc_final, c->list, c->counters, c->status[0],
-                    c->status[1], c->lh[0], c->lh[1]};
-    for (void *p : ptrs)
-        if (p) (void)hipFree(p);
+    c->g.release();
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -412,27 +350,28 @@ int aderdg_get_ops(aderdg_ctx *c, double *w, double *kxi, double *ik1, double *f
 {
     if (!c) return fail(ADERDG_EINVAL, "aderdg_get_ops: null context");
     const int n = c->n, s = c->s;
-    if (w) std::memcpy(w, c->ops.w, n * sizeof(double));
-    if (kxi) std::memcpy(kxi, c->ops.kxi, n * n * sizeof(double));
-    if (ik1) std::memcpy(ik1, c->ops.ik1, n * n * sizeof(double));
-    if (f0) std::memcpy(f0, c->ops.f0, n * sizeof(double));
-    if (flc) std::memcpy(flc, c->ops.flc, n * sizeof(double));
-    if (frc) std::memcpy(frc, c->ops.frc, n * sizeof(double));
-    if (u2l) std::memcpy(u2l, c->ops.u2l, s * n * sizeof(double));
-    if (l2u) std::memcpy(l2u, c->ops.l2u, n * s * sizeof(double));
-    if (u2b) std::memcpy(u2b, c->ops.u2b, n * n * sizeof(double));
+    const Operators &o = c->g.ops;
+    if (w) std::memcpy(w, o.w, n * sizeof(double));
+    if (kxi) std::memcpy(kxi, o.kxi, n * n * sizeof(double));
+    if (ik1) std::memcpy(ik1, o.ik1, n * n * sizeof(double));
+    if (f0) std::memcpy(f0, o.f0, n * sizeof(double));
+    if (flc) std::memcpy(flc, o.flc, n * sizeof(double));
+    if (frc) std::memcpy(frc, o.frc, n * sizeof(double));
+    if (u2l) std::memcpy(u2l, o.u2l, s * n * sizeof(double));
+    if (l2u) std::memcpy(l2u, o.l2u, n * s * sizeof(double));
+    if (u2b) std::memcpy(u2b, o.u2b, n * n * sizeof(double));
     return 0;
 }
 
 int aderdg_upload_uh(aderdg_ctx *c, const double *uh)
 {
     if (!c || !uh) return fail(ADERDG_EINVAL, "aderdg_upload_uh: null argument");
-    const size_t bytes = sizeof(double) * c->V * c->n * c->n * (size_t)c->ncell;
-    HIPCHK(hipMemcpyAsync(c->uh[c->cur], uh, bytes, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemsetAsync(c->status[c->cur], 0, sizeof(int) * c->ncell, c->stream));
-    HIPCHK(hipMemsetAsync(c->rc_final, 0, sizeof(int) * c->ncell, c->stream));
-    int r = seed(c);
-    if (r) return r;
+    const int par = c->st.parity;
+    int r = c->g.upload_uh(c->slot, uh, par);
+    if (r) return fail(r, "aderdg_upload_uh: copy failed");
+    HIPCHK(hipMemsetAsync(c->g.b.status[par], 0, sizeof(int) * c->ncell, c->stream));
+    HIPCHK(hipMemsetAsync(c->g.b.rc_final, 0, sizeof(int) * c->ncell, c->stream));
+    if ((r = seed(c))) return r;
     c->have_uh = true;
     return sync(c);
 }
@@ -440,8 +379,8 @@ int aderdg_upload_uh(aderdg_ctx *c, const double *uh)
 int aderdg_download_uh(aderdg_ctx *c, double *uh)
 {
     if (!c || !uh) return fail(ADERDG_EINVAL, "aderdg_download_uh: null argument");
-    const size_t bytes = sizeof(double) * c->V * c->n * c->n * (size_t)c->ncell;
-    HIPCHK(hipMemcpyAsync(uh, c->uh[c->cur], bytes, hipMemcpyDeviceToHost, c->stream));
+    int r = c->g.download_uh(c->slot, uh, c->st.parity);
+    if (r) return fail(r, "aderdg_download_uh: copy failed");
     return sync(c);
 }
 
@@ -450,10 +389,11 @@ int aderdg_set_limiter(aderdg_ctx *c, const int32_t *status, const double *Lh)
     if (!c || !status || !Lh) return fail(ADERDG_EINVAL, "aderdg_set_limiter: null argument");
     if (!c->have_uh) return fail(ADERDG_ESTATE, "aderdg_set_limiter: upload uh first");
     const size_t nc = c->ncell, blk = (size_t)c->V * c->s * c->s;
+    const int par = c->st.parity;
     std::vector<int32_t> st(nc);
     for (size_t k = 0; k < nc; k++) st[k] = status[k] ? 1 : 0;
-    HIPCHK(hipMemcpyAsync(c->status[c->cur], st.data(), nc * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->lh[c->cur], Lh, nc * blk * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->g.b.status[par], st.data(), nc * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->g.b.lh[par], Lh, nc * blk * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     int r = seed(c);
     if (r) return r;
@@ -464,18 +404,20 @@ int aderdg_get_limiter(aderdg_ctx *c, int32_t *status, int32_t *recompute, doubl
 {
     if (!c) return fail(ADERDG_EINVAL, "aderdg_get_limiter: null context");
     const size_t nc = c->ncell, blk = (size_t)c->V * c->s * c->s, V = c->V;
+    const int par = c->st.parity;
     std::vector<int32_t> st(nc);
-    HIPCHK(hipMemcpyAsync(st.data(), c->status[c->cur], nc * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-    if (recompute) HIPCHK(hipMemcpyAsync(recompute, c->rc_final, nc * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-    if (Lh) HIPCHK(hipMemcpyAsync(Lh, c->lh[c->cur], nc * blk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(st.data(), c->g.b.status[par], nc * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    if (recompute)
+        HIPCHK(hipMemcpyAsync(recompute, c->g.b.rc_final, nc * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    if (Lh) HIPCHK(hipMemcpyAsync(Lh, c->g.b.lh[par], nc * blk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     std::vector<double> mn, mx;
     if (lmin) {
         mn.resize(V * nc);
-        HIPCHK(hipMemcpyAsync(mn.data(), c->lmin, V * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(mn.data(), c->g.b.lmin, V * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     }
     if (lmax) {
         mx.resize(V * nc);
-        HIPCHK(hipMemcpyAsync(mx.data(), c->lmax, V * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(mx.data(), c->g.b.lmax, V * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     }
     int r = sync(c);
     if (r) return r;
@@ -497,7 +439,7 @@ int aderdg_calc_time_step(aderdg_ctx *c, double time, double tend, double *dt)
     int r = launch(c, kPartDt, time, tend, 1);
     if (r) return r;
     if (dt) {
-        HIPCHK(hipMemcpyAsync(c->pinned, c->scal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(c->pinned, &c->g.b.state[c->slot].dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         if ((r = sync(c))) return r;
         *dt = c->pinned[0];
     }
@@ -525,7 +467,7 @@ int aderdg_riemann_surface_update(aderdg_ctx *c, int32_t *nrec)
     if (!r) r = launch(c, kPartUpdate);
     if (r) return r;
     if (nrec) {
-        HIPCHK(hipMemcpyAsync(c->pinned + 4, c->counters, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(c->pinned + 4, c->g.b.counters, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
         if ((r = sync(c))) return r;
         int tmp[2];
         std::memcpy(tmp, c->pinned + 4, sizeof tmp);
@@ -539,8 +481,9 @@ int aderdg_limiter_end(aderdg_ctx *c)
     if (!c) return fail(ADERDG_EINVAL, "aderdg_limiter_end: null context");
     int r = launch(c, kPartSpread);
     if (!r) r = launch(c, kPartSubcell);
+    if (!r) r = launch(c, kPartFlip);  // update_limiter: Lh = NewLh, oldstatus = status, uh <- new
     if (r) return r;
-    c->cur = 1 - c->cur;  // update_limiter: Lh = NewLh, oldstatus = status, uh <- new
+    c->st.parity ^= 1;
     return 0;
 }
 
@@ -560,14 +503,17 @@ int aderdg_run(aderdg_ctx *c, int nsteps, double time, double tend, double *time
     if (!c) return fail(ADERDG_EINVAL, "aderdg_run: null context");
     if (nsteps < 0) return fail(ADERDG_EINVAL, "aderdg_run: negative step count");
     if (!c->have_uh) return fail(ADERDG_ESTATE, "aderdg_run: no solution uploaded");
-    c->pinned[0] = 0.0;
-    c->pinned[1] = time;
-    c->pinned[2] = tend;
-    HIPCHK(hipMemcpyAsync(c->scal, c->pinned, 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    int r = sync(c);
-    if (r) return r;
-    if (!c->timing && !c->graph[0]) {
-        if ((r = build_graphs(c))) return r;
+    c->st.dt = 0.0;
+    c->st.time = time;
+    c->st.tend = tend;
+    c->st.dtmin = ~0ull;
+    c->st.active = 1;
+    c->st.steps = 0;
+    int r = c->g.write_state(c->slot, c->st);
+    if (r) return fail(r, "aderdg_run: state upload failed");
+    if ((r = sync(c))) return r;
+    if (!c->timing && !c->graph) {
+        if ((r = build_graph(c))) return r;
     }
     for (int k = 0; k < nsteps; k++) {
         if (c->timing) {
@@ -576,24 +522,32 @@ int aderdg_run(aderdg_ctx *c, int nsteps, double time, double tend, double *time
             if (!r) r = launch(c, kPartAdvance);
             if (r) return r;
         } else {
-            HIPCHK(hipGraphLaunch(c->graph[c->cur], c->stream));
+            HIPCHK(hipGraphLaunch(c->graph, c->stream));
         }
-        c->cur = 1 - c->cur;
+        c->st.parity ^= 1;
     }
-    HIPCHK(hipMemcpyAsync(c->pinned, c->scal, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(c->pinned + 4, c->counters, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    SolveState *hs = reinterpret_cast<SolveState *>(c->pinned);
+    static_assert(sizeof(SolveState) <= 4 * sizeof(double) + 8 * sizeof(int), "pinned staging too small");
+    HIPCHK(hipMemcpyAsync(hs, &c->g.b.state[c->slot], sizeof(SolveState), hipMemcpyDeviceToHost, c->stream));
+    int cnt[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(cnt, c->g.b.counters, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     if ((r = sync(c))) return r;
-    if (time_out) *time_out = c->pinned[1];
-    if (dt_out) *dt_out = c->pinned[0];
-    if (nrec_out) {
-        int tmp[2];
-        std::memcpy(tmp, c->pinned + 4, sizeof tmp);
-        *nrec_out = tmp[kCntTroubled];
-    }
+    c->st.time = hs->time;
+    c->st.dt = hs->dt;
+    c->st.tend = hs->tend;
+    if (time_out) *time_out = hs->time;
+    if (dt_out) *dt_out = hs->dt;
+    if (nrec_out) *nrec_out = cnt[kCntTroubled];
     return 0;
 }
 
 namespace {
+int dalloc(double **p, size_t count)
+{
+    HIPCHK(hipMalloc((void **)p, count > 0 ? count * sizeof(double) : 8));
+    return 0;
+}
+
 int predictor_cells(aderdg_ctx *c, int ncells, double dt, const double *luh, double *lqhi, double *lFhi, double *lShi,
                     double *lqbnd, double *lFbnd)
 {
@@ -607,12 +561,10 @@ int predictor_cells(aderdg_ctx *c, int ncells, double dt, const double *luh, dou
     r = r ? r : dalloc(&d_s, V * n * n * nc);
     if (!r) {
         hipError_t e = hipMemcpyAsync(d_uh, luh, V * n * n * nc * sizeof(double), hipMemcpyHostToDevice, c->stream);
-        c->pinned[0] = dt;
-        if (e == hipSuccess) e = hipMemcpyAsync(c->scal, c->pinned, sizeof(double), hipMemcpyHostToDevice, c->stream);
-        MeshArgs m = c->m;
+        MeshArgs m = c->g.mesh(c->slot);
         m.ncell = ncells;
         TestBufs t{d_uh, d_q, d_f, d_qb, d_fb, d_s};
-        if (e == hipSuccess) e = launch_test_predictor(c->N, m, bufs(c), t, c->stream);
+        if (e == hipSuccess) e = launch_test_predictor(c->N, m, c->g.b, t, dt, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(lqhi, d_q, V * n * n * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(lFhi, d_f, 2 * V * n * n * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(lqbnd, d_qb, 4 * V * n * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream);

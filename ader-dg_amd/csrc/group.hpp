@@ -1,0 +1,91 @@
+// group.hpp -- host side of a solve group: the device arena of one (PDE, N),
+// the solves placed in it, and the work tables the kernels loop over.
+//
+// A group owns flat device arrays sized by its capacities (cells, x-faces,
+// y-faces, solve slots).  Adding a solve takes first-fit ranges of cells and
+// faces and a slot, writes the solve's SolveArgs / SolveState and rebuilds the
+// work tables on the device; removing it returns the ranges.  Neither changes
+// a kernel's launch configuration, so a captured step graph stays valid.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <vector>
+
+#include "operators.hpp"
+#include "step_kernels.hpp"
+
+namespace aderdg {
+
+// First-fit allocator of index ranges in [0, cap), coalescing on release.
+class RangeAlloc {
+public:
+    void reset(int64_t cap);
+    bool alloc(int64_t len, int64_t *start);
+    void release(int64_t start, int64_t len);
+    int64_t largest_free() const;
+    int64_t capacity() const { return cap_; }
+
+private:
+    int64_t cap_ = 0;
+    std::map<int64_t, int64_t> free_;  // start -> length
+};
+
+struct GroupCaps {
+    int64_t cells, xfaces, yfaces;
+    int slots;
+};
+
+class Group {
+public:
+    int pde = 0, N = 0, V = 0, n = 0, s = 0;
+    GroupCaps caps{};
+    Operators ops{};
+    int quanta[kNumTabs] = {};
+    int grid[kNumTabs] = {};
+    int64_t tab_cap[kNumTabs] = {};
+    StepBufs b{};
+
+    ~Group() { release(); }
+    // Allocate the arena.  persist_blocks: workgroups of the table-driven kernels
+    // (0: one per table entry of a full arena).
+    int init(int pde, int N, const GroupCaps &caps, const Operators &ops, int persist_blocks, hipStream_t st);
+    void release();
+
+    // Place a solve; returns 0, or ADERDG_EFULL when no slot or range is free.
+    int add(const MeshArgs &m, const SolveState &st0, int *slot);
+    int remove(int slot);
+    const MeshArgs &mesh(int slot) const { return solves_[slot].m; }
+    int cell0(int slot) const { return solves_[slot].cell0; }
+    bool used(int slot) const { return slot >= 0 && slot < caps.slots && used_[slot]; }
+    int nslot() const { return nslot_; }
+    int64_t cells_in_use() const { return cells_in_use_; }
+
+    // Host <-> device for one solve; uh in the reference layout uh(V,n,n,IMAX,JMAX).
+    int upload_uh(int slot, const double *uh, int parity);
+    int download_uh(int slot, double *uh, int parity);
+    int write_state(int slot, const SolveState &st);
+    int read_states(SolveState *out);  // all nslot() states (synchronous)
+
+    hipError_t launch(int part) const;
+    // Bytes of device memory per cell / x-face / y-face and per slot of a (PDE, N).
+    static void footprint(int pde, int N, int64_t *per_cell, int64_t *per_xface, int64_t *per_yface,
+                          int64_t *per_slot);
+
+private:
+    hipStream_t st_ = nullptr;
+    std::vector<void *> allocs_;
+    std::vector<SolveArgs> solves_;
+    std::vector<char> used_;
+    RangeAlloc cell_alloc_, fx_alloc_, fy_alloc_;
+    int nslot_ = 0;
+    int64_t cells_in_use_ = 0;
+    int *prefix_dev_ = nullptr;
+    int *prefix_host_ = nullptr;  // pinned
+    int rebuild_tables();
+    template <class T>
+    int dalloc(T **p, size_t count);
+};
+
+}  // namespace aderdg

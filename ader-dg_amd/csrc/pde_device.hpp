@@ -7,7 +7,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include "euler_kernels.hpp"
+#include "step_kernels.hpp"
 
 #define ADG_DEV __device__ __forceinline__
 

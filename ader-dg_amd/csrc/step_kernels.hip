@@ -2,26 +2,33 @@
 // 2-D Euler (nVar = 4) and Baer-Nunziato (nVar = 9) systems.
 //
 // Per time step (the order of Euler/main.f90:24-100, Baer-Nunziato/main.f90:24-100):
-//   k_dt          calc_time_step: min-reduction of per-cell candidates
+//   k_dt          calc_time_step: per-solve min-reduction of per-cell candidates
 //   k_predictor   ader_space_time_predictor + volume_integral + face
 //                 extrapolation + the 3x3 pass of get_min_max
-//   k_update      solve_riemann_problem + surface_integral + element_update
-//                 + DMP, and the next step's per-cell get_min_max pass and
-//                 time-step candidates
+//   k_face        solve_riemann_problem: one thread per face point
+//   k_update      surface_integral + element_update + DMP, and the next
+//                 step's per-cell get_min_max pass and time-step candidates
 //   k_spread      spread_recompute + allocate_limiter + wave-ballot
 //                 compaction of the cells to recompute
 //   k_subcell     subcell_recompute (MUSCL-Hancock on the (2N+1)^2 subgrid)
 //                 + put_subcell_data + re-DMP + update_limiter, over the
 //                 compacted list only (k_subcell_bn for Baer-Nunziato)
 //
-// Data layout in HBM (ncell = IMAX*JMAX, cell = i + IMAX*j):
+// All kernels serve a group of independent solves of one (PDE, N): a solve's
+// cells are a contiguous range of the flat cell arrays (local cell
+// i + IMAX*j at cell0 + i + IMAX*j) and its faces ranges of the face arrays.
+// Data layout in HBM (G = cell capacity of the group):
 //   uh, lduh      [cell][j][i][v]   the reference's uh(V,n,n,IMAX,JMAX)
 //   fq, ff        [face][v][p][cell] face extrapolations of q and F*n,
 //                 structure of arrays so that the face pass reads
 //                 consecutive cells with consecutive lanes
+//   xflux, yflux  [side][v][p][face]
 //   lmin.. etc.   [v][cell]
 //   Lh            [cell][q][p][v]   subcell averages of limited cells
-#include "euler_kernels.hpp"
+// The step kernels loop over work tables of (solve, first item) entries
+// (persistent workgroups); solves that are not active are skipped, and each
+// solve has its own ping-pong parity, time and dt.
+#include "step_kernels.hpp"
 #include "pde_device.hpp"
 
 namespace aderdg {
@@ -54,6 +61,22 @@ struct Shape {
 
 ADG_DEV double dmr_shock_x(const MeshArgs &m, double time) { return m.dmr_a + m.dmr_b * time; }
 
+// Loads of data that no kernel writes while it runs (solve table, work tables,
+// the per-solve state outside k_dt / k_solves) through the constant address
+// space: uniform addresses then become scalar loads into SGPRs.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define ADG_CONST __attribute__((address_space(4)))
+#else  // the host pass of this file only type-checks the device code
+#define ADG_CONST
+#endif
+template <class T>
+ADG_DEV T ld_const(const T *p)
+{
+    return *(const ADG_CONST T *)p;
+}
+// the operator matrices, read-only for the whole run
+using OpPtr = const ADG_CONST Operators *;
+
 // ---------------------------------------------------------------------------
 // space-time predictor (+ volume integral, face extrapolation, min/max pass 2)
 // Thread (cell, i, j) owns the time line q(:, i, j, 0..N) of one spatial node.
@@ -62,24 +85,29 @@ ADG_DEV double dmr_shock_x(const MeshArgs &m, double time) { return m.dmr_a + m.
 // contraction with iK1 is thread-local.  Baer-Nunziato also exchanges q for
 // the gradients of the non-conservative product.
 // ---------------------------------------------------------------------------
+// CPW cells from local cell `first` of one solve (cell0, parity par); smem holds
+// 2 * CPW * SLOT doubles.  TEST: cells of t.uh, outputs in the reference layouts.
 template <class P, int N, bool TEST>
-__global__ __launch_bounds__(kPredBlock<P>) void k_predictor(MeshArgs m, StepBufs b, TestBufs t)
+ADG_DEV void predictor_cells(const MeshArgs &m, const StepBufs &b, const TestBufs &t, double *smem, int first,
+                             int cell0, int par, double dt)
 {
     using S = Shape<P, N>;
     constexpr int n = S::n, NN = S::NN, CPW = S::CPW, V = S::V, SLOT = S::SLOT;
-    __shared__ double sm[2][CPW][SLOT];
-    const Operators *__restrict__ op = b.op;
+    const OpPtr op = (OpPtr)b.op;
     const int tid = threadIdx.x, lc = tid / NN, tt = tid - lc * NN;
     const int i = tt % n, j = tt / n;
-    const int cell = blockIdx.x * CPW + lc;
+    const int cell = first + lc;                 // local cell of the solve
+    const size_t g = (size_t)cell0 + cell;       // cell of the group arrays
     const bool own = lc < CPW;
     const bool act = own && cell < m.ncell;
     const int lcs = own ? lc : CPW - 1;
+    auto SM = [&](int sl) { return smem + (sl * CPW + lcs) * SLOT; };
 
     double u[V];
-    const double *uh = TEST ? t.uh : b.uh_cur;
+    const double *uh = TEST ? t.uh : b.uh[par];
+    const size_t gu = TEST ? (size_t)cell : g;
 #pragma unroll
-    for (int v = 0; v < V; v++) u[v] = act ? uh[(size_t)cell * V * NN + V * tt + v] : 0.0;
+    for (int v = 0; v < V; v++) u[v] = act ? uh[gu * V * NN + V * tt + v] : 0.0;
     if (!act) {  // an admissible dummy state for idle lanes
         if (V == 4) {
             u[0] = 1.0;
@@ -99,7 +127,6 @@ __global__ __launch_bounds__(kPredBlock<P>) void k_predictor(MeshArgs m, StepBuf
             du_j[k] = op->dudx[j + n * k];
         }
     }
-    const double dt = b.scal[0];
 
     double q[V][n], fh[2][V], sh[V];
 #pragma unroll
@@ -128,7 +155,7 @@ __global__ __launch_bounds__(kPredBlock<P>) void k_predictor(MeshArgs m, StepBuf
 #pragma unroll
             for (int v = 0; v < V; v++) Q[v] = q[v][0];
             P::flux(m, Q, F);
-            double *buf = sm[slot][lcs];
+            double *buf = SM(slot);
             if (own) {
 #pragma unroll
                 for (int v = 0; v < V; v++) {
@@ -210,9 +237,9 @@ __global__ __launch_bounds__(kPredBlock<P>) void k_predictor(MeshArgs m, StepBuf
         qh[v] = acc;
     }
     __syncthreads();
-    double *qhiL = &sm[0][lcs][0];
-    double *fhxL = &sm[0][lcs][V * NN];
-    double *fhyL = &sm[1][lcs][0];
+    double *qhiL = SM(0);
+    double *fhxL = SM(0) + V * NN;
+    double *fhyL = SM(1);
     if (own) {
 #pragma unroll
         for (int v = 0; v < V; v++) {
@@ -246,7 +273,7 @@ __global__ __launch_bounds__(kPredBlock<P>) void k_predictor(MeshArgs m, StepBuf
             for (int k = 0; k < n; k++) acc = acc + fhyL[v * NN + i + n * k] * op->kxi[j + n * k];
             d = d + acc * wi / m.dy;
             if (P::kNcp) d = d + wi * wj * sh[v];
-            if (act) b.lduh[(size_t)cell * V * NN + V * tt + v] = d;
+            if (act) b.lduh[g * V * NN + V * tt + v] = d;
         }
     }
 
@@ -274,7 +301,7 @@ __global__ __launch_bounds__(kPredBlock<P>) void k_predictor(MeshArgs m, StepBuf
                     t.qbnd[(size_t)cell * V * 4 * n + v + V * (f + 4 * p)] = Qb[v];
                     t.fbnd[(size_t)cell * V * 4 * n + v + V * (f + 4 * p)] = Fb[v];
                 } else {
-                    const size_t o = ((size_t)(f * V + v) * n + p) * m.ncell + cell;
+                    const size_t o = ((size_t)(f * V + v) * n + p) * b.G + g;
                     b.fq[o] = Qb[v];
                     b.ff[o] = Fb[v];
                 }
@@ -286,20 +313,45 @@ __global__ __launch_bounds__(kPredBlock<P>) void k_predictor(MeshArgs m, StepBuf
     // get_min_max step 2: 3x3 neighbourhood, Euler/limiter.f90:147-167
     for (int v = tt; act && v < V; v += NN) {
         const int ci = cell % m.imax, cj = cell / m.imax;
-        double mn = b.lmin_old[(size_t)v * m.ncell + cell], mx = b.lmax_old[(size_t)v * m.ncell + cell];
+        const size_t row = (size_t)v * b.G + cell0;
+        double mn = b.lmin_old[row + cell], mx = b.lmax_old[row + cell];
         for (int jj = -1; jj <= 1; jj++)
             for (int ii = -1; ii <= 1; ii++) {
                 const int nb = nb_index(ci, ii, m.imax, m.wrap) + m.imax * nb_index(cj, jj, m.jmax, m.wrap);
-                mn = f_min(mn, b.lmin_old[(size_t)v * m.ncell + nb]);
-                mx = f_max(mx, b.lmax_old[(size_t)v * m.ncell + nb]);
+                mn = f_min(mn, b.lmin_old[row + nb]);
+                mx = f_max(mx, b.lmax_old[row + nb]);
             }
-        b.lmin[(size_t)v * m.ncell + cell] = mn;
-        b.lmax[(size_t)v * m.ncell + cell] = mx;
+        b.lmin[row + cell] = mn;
+        b.lmax[row + cell] = mx;
     }
-    if (blockIdx.x == 0 && tid == 0) {
+}
+
+template <class P, int N>
+__global__ __launch_bounds__(kPredBlock<P>) void k_predictor(StepBufs b)
+{
+    using S = Shape<P, N>;
+    __shared__ double smem[2 * S::CPW * S::SLOT];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         b.counters[kCntList] = 0;
         b.counters[kCntTroubled] = 0;
     }
+    const int ntab = ld_const(b.meta + kTabPred);
+    for (int e = blockIdx.x; e < ntab; e += gridDim.x) {
+        const int2 w = ld_const(b.tab[kTabPred] + e);
+        const SolveState st = ld_const(b.state + w.x);
+        if (!st.active) continue;
+        const SolveArgs sa = ld_const(b.solves + w.x);
+        predictor_cells<P, N, false>(sa.m, b, TestBufs{}, smem, w.y, sa.cell0, st.parity, st.dt);
+        __syncthreads();
+    }
+}
+
+template <class P, int N>
+__global__ __launch_bounds__(kPredBlock<P>) void k_predictor_test(MeshArgs m, StepBufs b, TestBufs t, double dt)
+{
+    using S = Shape<P, N>;
+    __shared__ double smem[2 * S::CPW * S::SLOT];
+    predictor_cells<P, N, true>(m, b, t, smem, blockIdx.x * S::CPW, 0, 0, dt);
 }
 
 // ---------------------------------------------------------------------------
@@ -310,15 +362,23 @@ __global__ __launch_bounds__(kPredBlock<P>) void k_predictor(MeshArgs m, StepBuf
 // the fluctuations D-/D+ (Baer-Nunziato) in the same order.
 // ---------------------------------------------------------------------------
 template <class P, int N>
-__global__ __launch_bounds__(256) void k_face(MeshArgs m, StepBufs b)
+__global__ __launch_bounds__(kBlock) void k_face(StepBufs b)
 {
     constexpr int V = P::V, n = N + 1;
-    const int I = m.imax, J = m.jmax;
-    const long nfx = (long)(I + 1) * J, nfy = (long)I * (J + 1);
-    const long total = (nfx + nfy) * n;
-    const double time = b.scal[1];
-    const bool dmr = V == 4 && m.bc_mode == kBcDmr;
-    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    const int ntab = ld_const(b.meta + kTabFace);
+    for (int e = blockIdx.x; e < ntab; e += gridDim.x) {
+        const int2 w = ld_const(b.tab[kTabFace] + e);
+        const SolveState st = ld_const(b.state + w.x);
+        if (!st.active) continue;
+        const SolveArgs sa = ld_const(b.solves + w.x);
+        const MeshArgs &m = sa.m;
+        const int I = m.imax, J = m.jmax;
+        const long nfx = (long)(I + 1) * J, nfy = (long)I * (J + 1);
+        const long t = (long)w.y + threadIdx.x;
+        if (t >= (nfx + nfy) * n) continue;
+        const double time = st.time;
+        const bool dmr = V == 4 && m.bc_mode == kBcDmr;
+        const size_t c0 = sa.cell0;
         const bool xdir = t < nfx * n;
         const long t2 = xdir ? t : t - nfx * n;
         const long nf = xdir ? nfx : nfy;
@@ -328,7 +388,7 @@ __global__ __launch_bounds__(256) void k_face(MeshArgs m, StepBufs b)
         auto ld = [&](int f, long c, double *Q, double *F) {
 #pragma unroll
             for (int v = 0; v < V; v++) {
-                const size_t o = ((size_t)(f * V + v) * n + p) * m.ncell + c;
+                const size_t o = ((size_t)(f * V + v) * n + p) * b.G + c0 + c;
                 Q[v] = b.fq[o];
                 F[v] = b.ff[o];
             }
@@ -406,17 +466,19 @@ __global__ __launch_bounds__(256) void k_face(MeshArgs m, StepBufs b)
             P::face_flux(m, QL, FL, QR, FR, 0.0, 1.0, Dm, Dp);
         }
         double *out = xdir ? b.xflux : b.yflux;
+        const size_t stride = xdir ? b.GX : b.GY;
+        const size_t f0 = (size_t)(xdir ? sa.fx0 : sa.fy0) + fidx;
 #pragma unroll
         for (int v = 0; v < V; v++) {
-            out[((size_t)(0 * V + v) * n + p) * nf + fidx] = Dm[v];
-            if (P::kNcp) out[((size_t)(1 * V + v) * n + p) * nf + fidx] = Dp[v];
+            out[((size_t)(0 * V + v) * n + p) * stride + f0] = Dm[v];
+            if (P::kNcp) out[((size_t)(1 * V + v) * n + p) * stride + f0] = Dp[v];
         }
     }
 }
 
 // First passes of get_subcell_data / get_lobatto_data for one cell held in LDS (uhL [v][NN])
 template <class P, int N>
-ADG_DEV void cell_ranges_stage1(const Operators *__restrict__ op, const double *uhL, double *tyL, double *byL, int tt,
+ADG_DEV void cell_ranges_stage1(OpPtr op, const double *uhL, double *tyL, double *byL, int tt,
                                 int nthr)
 {
     using S = Shape<P, N>;
@@ -445,7 +507,7 @@ ADG_DEV void cell_ranges_stage1(const Operators *__restrict__ op, const double *
 
 // Second passes; returns true if a subcell average is inadmissible (Euler/limiter.f90:227-241)
 template <class P, int N>
-ADG_DEV bool cell_ranges_stage2(const Operators *__restrict__ op, const MeshArgs &m, const double *tyL,
+ADG_DEV bool cell_ranges_stage2(OpPtr op, const MeshArgs &m, const double *tyL,
                                 const double *byL, double *limL, double *lobL, int tt, int nthr)
 {
     using S = Shape<P, N>;
@@ -524,19 +586,29 @@ ADG_DEV double node_dt(const MeshArgs &m, const double *Q)
 // are evaluated and reduced across the cell's threads afterwards.
 // ---------------------------------------------------------------------------
 template <class P, int N, bool SEED>
-__global__ __launch_bounds__(kUpdBlock<P>) void k_update(MeshArgs m, StepBufs b)
+__global__ __launch_bounds__(kUpdBlock<P>) void k_update(StepBufs b)
 {
     using S = Shape<P, N>;
     constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, CPW = S::CPWU, V = S::V, NP = S::NPART;
     __shared__ double lds[CPW][S::UPD_LDS];
     __shared__ int sflag[CPW];
-    const Operators *__restrict__ op = b.op;
+    const OpPtr op = (OpPtr)b.op;
     const int tid = threadIdx.x, lc = tid / NN, tt = tid - lc * NN;
     const int i = tt % n, j = tt / n;
-    const int cell = blockIdx.x * CPW + lc;
     const bool own = lc < CPW;
-    const bool act = own && cell < m.ncell;
     const int lcs = own ? lc : CPW - 1;
+    const int ntab = ld_const(b.meta + kTabUpd);
+    for (int e = blockIdx.x; e < ntab; e += gridDim.x) {
+    const int2 w = ld_const(b.tab[kTabUpd] + e);
+    const SolveState st = ld_const(b.state + w.x);
+    if (SEED ? !st.seed : !st.active) continue;
+    const SolveArgs sa = ld_const(b.solves + w.x);
+    const MeshArgs &m = sa.m;
+    const int par = st.parity;
+    const int cell = w.y + lc;                 // local cell of the solve
+    const size_t g = (size_t)sa.cell0 + cell;  // cell of the group arrays
+    const bool act = own && cell < m.ncell;
+    const double *uh_cur = b.uh[par];
     double *suh = &lds[lcs][0];
     double *sty = suh + V * NN;     // limy [v][a + n q]
     double *sby = sty + V * n * s;  // loby
@@ -547,17 +619,17 @@ __global__ __launch_bounds__(kUpdBlock<P>) void k_update(MeshArgs m, StepBufs b)
     double un[V];
     if (SEED || !act) {
 #pragma unroll
-        for (int v = 0; v < V; v++) un[v] = act ? b.uh_cur[(size_t)cell * V * NN + V * tt + v] : (v == 0 || v == 3 ? 1.0 : 0.0);
+        for (int v = 0; v < V; v++) un[v] = act ? uh_cur[g * V * NN + V * tt + v] : (v == 0 || v == 3 ? 1.0 : 0.0);
     } else {
         const int ci = cell % m.imax, cj = cell / m.imax;
-        const long nfx = (long)(m.imax + 1) * m.jmax, nfy = (long)m.imax * (m.jmax + 1);
-        const long fl = ci + (long)(m.imax + 1) * cj, fb = ci + (long)m.imax * cj;
+        const size_t nfx = b.GX, nfy = b.GY;
+        const size_t fl = sa.fx0 + ci + (size_t)(m.imax + 1) * cj, fb = sa.fy0 + ci + (size_t)m.imax * cj;
         const int sp = P::kNcp ? 1 : 0;  // right/top faces take D+ (Baer-Nunziato/main.f90:68)
-        const double wi = op->w[i], wj = op->w[j], dt = b.scal[0];
+        const double wi = op->w[i], wj = op->w[j], dt = st.dt;
 #pragma unroll
         for (int v = 0; v < V; v++) {
-            double d = b.lduh[(size_t)cell * V * NN + V * tt + v];
-            const double uo = b.uh_cur[(size_t)cell * V * NN + V * tt + v];
+            double d = b.lduh[g * V * NN + V * tt + v];
+            const double uo = uh_cur[g * V * NN + V * tt + v];
             const double Fl = b.xflux[((size_t)(0 * V + v) * n + j) * nfx + fl];
             const double Fr = b.xflux[((size_t)(sp * V + v) * n + j) * nfx + fl + 1];
             const double Gb = b.yflux[((size_t)(0 * V + v) * n + i) * nfy + fb];
@@ -567,7 +639,7 @@ __global__ __launch_bounds__(kUpdBlock<P>) void k_update(MeshArgs m, StepBufs b)
             d = d - wi / m.dy * (Gt * op->frc[j] - Gb * op->flc[j]);
             d = d / (wi * wj);  // element update, Euler/element_update.f90:218-226
             un[v] = uo + dt * d;
-            b.uh_next[(size_t)cell * V * NN + V * tt + v] = un[v];
+            b.uh[par ^ 1][g * V * NN + V * tt + v] = un[v];
         }
     }
     if (own) {
@@ -636,12 +708,12 @@ __global__ __launch_bounds__(kUpdBlock<P>) void k_update(MeshArgs m, StepBufs b)
     for (int v = tt; act && v < V; v += NN) {
         const double r[6] = {sred[0 * V + v], sred[1 * V + v], sred[2 * V + v],
                              sred[3 * V + v], sred[4 * V + v], sred[5 * V + v]};
-        if (!SEED && dmp_violated(r, b.lmin[(size_t)v * m.ncell + cell], b.lmax[(size_t)v * m.ncell + cell]))
+        if (!SEED && dmp_violated(r, b.lmin[(size_t)v * b.G + g], b.lmax[(size_t)v * b.G + g]))
             atomicOr(&sflag[lc], 2);
         // next step's get_min_max step 1 for a cell without limiter, Euler/limiter.f90:123-132
         double mn = f_min(f_min(r[0], r[2]), r[4]), mx = f_max(f_max(r[1], r[3]), r[5]);
-        if (SEED && b.status_cur[cell] != 0) {  // limited cell: range of its subcell data, :135-139
-            const double *L = b.lh_cur + (size_t)cell * V * SS;
+        if (SEED && b.status[par][g] != 0) {  // limited cell: range of its subcell data, :135-139
+            const double *L = b.lh[par] + g * V * SS;
             mn = __builtin_nan("");
             mx = mn;
             for (int k = 0; k < SS; k++) {
@@ -649,69 +721,85 @@ __global__ __launch_bounds__(kUpdBlock<P>) void k_update(MeshArgs m, StepBufs b)
                 mx = fold_max(mx, L[v + V * k]);
             }
         }
-        b.lmin_old[(size_t)v * m.ncell + cell] = mn;
-        b.lmax_old[(size_t)v * m.ncell + cell] = mx;
+        b.lmin_old[(size_t)v * b.G + g] = mn;
+        b.lmax_old[(size_t)v * b.G + g] = mx;
     }
-    if (act && tt == NN - 1) b.cell_dt[cell] = sred[6 * V];
+    if (act && tt == NN - 1) b.cell_dt[g] = sred[6 * V];
     __syncthreads();
     if (!SEED && act && tt == 0) {
         const int tr = sflag[lc] != 0;
-        b.rc[cell] = tr;
+        b.rc[g] = tr;
         if (tr) atomicAdd(&b.counters[kCntTroubled], 1);
+    }
+    __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
 // spread_recompute + allocate_limiter + compaction (one thread per cell)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_spread(MeshArgs m, StepBufs b)
+__global__ __launch_bounds__(kBlock) void k_spread(StepBufs b)
 {
     __shared__ int wcount[kBlock / 64], wbase[kBlock / 64], bbase;
-    const int cell = blockIdx.x * kBlock + threadIdx.x;
-    int flag = 0;
-    if (cell < m.ncell) {
-        if (b.rc[cell] == 1) {
-            flag = 1;
-        } else {
-            // a face neighbour marked 1 marks this cell 2 (Euler/limiter.f90:336-350; the
-            // face-neighbour relation of the clamped/wrapped table is symmetric)
-            const int ci = cell % m.imax, cj = cell / m.imax;
-            const int nbs[4] = {nb_index(ci, -1, m.imax, m.wrap) + m.imax * cj, nb_index(ci, 1, m.imax, m.wrap) + m.imax * cj,
-                                ci + m.imax * nb_index(cj, -1, m.jmax, m.wrap), ci + m.imax * nb_index(cj, 1, m.jmax, m.wrap)};
+    const int ntab = ld_const(b.meta + kTabCell);
+    for (int e = blockIdx.x; e < ntab; e += gridDim.x) {
+        const int2 w = ld_const(b.tab[kTabCell] + e);
+        const SolveState st = ld_const(b.state + w.x);
+        if (!st.active) continue;
+        const SolveArgs sa = ld_const(b.solves + w.x);
+        const MeshArgs &m = sa.m;
+        const int par = st.parity;
+        const int cell = w.y + threadIdx.x;  // local cell of the solve
+        const int g = sa.cell0 + cell;
+        int flag = 0;
+        if (cell < m.ncell) {
+            if (b.rc[g] == 1) {
+                flag = 1;
+            } else {
+                // a face neighbour marked 1 marks this cell 2 (Euler/limiter.f90:336-350; the
+                // face-neighbour relation of the clamped/wrapped table is symmetric)
+                const int ci = cell % m.imax, cj = cell / m.imax;
+                const int nbs[4] = {nb_index(ci, -1, m.imax, m.wrap) + m.imax * cj,
+                                    nb_index(ci, 1, m.imax, m.wrap) + m.imax * cj,
+                                    ci + m.imax * nb_index(cj, -1, m.jmax, m.wrap),
+                                    ci + m.imax * nb_index(cj, 1, m.jmax, m.wrap)};
 #pragma unroll
-            for (int f = 0; f < 4; f++)
-                if (nbs[f] != cell && b.rc[nbs[f]] == 1) flag = 2;
+                for (int f = 0; f < 4; f++)
+                    if (nbs[f] != cell && b.rc[sa.cell0 + nbs[f]] == 1) flag = 2;
+            }
+            b.rc_final[g] = flag;
+            b.status[par ^ 1][g] = flag == 1 ? 1 : 0;  // allocate_limiter, Euler/limiter.f90:308-320
         }
-        b.rc_final[cell] = flag;
-        b.status_next[cell] = flag == 1 ? 1 : 0;  // allocate_limiter, Euler/limiter.f90:308-320
-    }
-    const bool take = flag > 0;
-    const unsigned long long mask = __ballot(take);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int below = __popcll(mask & ((1ull << lane) - 1ull));
-    if (lane == 0) wcount[wave] = __popcll(mask);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int tot = 0;
-        for (int w = 0; w < kBlock / 64; w++) {
-            wbase[w] = tot;
-            tot += wcount[w];
+        const bool take = flag > 0;
+        const unsigned long long mask = __ballot(take);
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int below = __popcll(mask & ((1ull << lane) - 1ull));
+        if (lane == 0) wcount[wave] = __popcll(mask);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int tot = 0;
+            for (int k = 0; k < kBlock / 64; k++) {
+                wbase[k] = tot;
+                tot += wcount[k];
+            }
+            bbase = tot ? atomicAdd(&b.counters[kCntList], tot) : 0;
         }
-        bbase = tot ? atomicAdd(&b.counters[kCntList], tot) : 0;
+        __syncthreads();
+        if (take) b.list[bbase + wbase[wave] + below] = g;
+        __syncthreads();
     }
-    __syncthreads();
-    if (take) b.list[bbase + wbase[wave] + below] = cell;
 }
 
 // Common tail of both subcell kernels: put_subcell_data (Euler/limiter.f90:86-96),
 // re-DMP of cells marked 2 (:619-638), next-step limiter seeds and dt candidate.
 template <class P, int N>
-ADG_DEV void subcell_finish(const MeshArgs &m, const StepBufs &b, int cell, int flag, const double *sSub, double *sUhy,
-                            double *sUh, double *sTy, double *sBy, double *sLim, double *sLob, double *sDt, int *sFlag)
+ADG_DEV void subcell_finish(const MeshArgs &m, const StepBufs &b, size_t cell, int par, int flag, const double *sSub,
+                            double *sUhy, double *sUh, double *sTy, double *sBy, double *sLim, double *sLob, double *sDt,
+                            int *sFlag)
 {
     using S = Shape<P, N>;
     constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, V = S::V;
-    const Operators *__restrict__ op = b.op;
+    const OpPtr op = (OpPtr)b.op;
     const int lane = threadIdx.x;
     for (int task = lane; task < s * n; task += 64) {
         const int p = task % s, bb = task / s;
@@ -734,7 +822,7 @@ ADG_DEV void subcell_finish(const MeshArgs &m, const StepBufs &b, int cell, int 
             for (int p = 0; p < s; p++) acc = acc + sUhy[v * s * n + p + s * bb] * op->l2u[a + n * p];
             un[v] = acc;
             sUh[v * NN + task] = acc;
-            b.uh_next[(size_t)cell * V * NN + V * task + v] = acc;
+            b.uh[par ^ 1][cell * V * NN + V * task + v] = acc;
         }
         sDt[task] = node_dt<P>(m, un);
     }
@@ -748,7 +836,7 @@ ADG_DEV void subcell_finish(const MeshArgs &m, const StepBufs &b, int cell, int 
     for (int v = lane; v < V; v += 64) {
         double r[6];
         group_ranges<P, N>(sUh, sLob, sLim, v, r);
-        if (dmp_violated(r, b.lmin[(size_t)v * m.ncell + cell], b.lmax[(size_t)v * m.ncell + cell])) atomicOr(sFlag, 2);
+        if (dmp_violated(r, b.lmin[(size_t)v * b.G + cell], b.lmax[(size_t)v * b.G + cell])) atomicOr(sFlag, 2);
     }
     __syncthreads();
     const int status = (flag == 1) ? 1 : (*sFlag != 0 ? 1 : 0);
@@ -767,17 +855,17 @@ ADG_DEV void subcell_finish(const MeshArgs &m, const StepBufs &b, int cell, int 
             mn = f_min(f_min(r[0], r[2]), r[4]);
             mx = f_max(f_max(r[1], r[3]), r[5]);
         }
-        b.lmin_old[(size_t)v * m.ncell + cell] = mn;
-        b.lmax_old[(size_t)v * m.ncell + cell] = mx;
+        b.lmin_old[(size_t)v * b.G + cell] = mn;
+        b.lmax_old[(size_t)v * b.G + cell] = mx;
     }
     if (lane == 63) {
         double d = __builtin_nan("");
         for (int k = 0; k < NN; k++) d = fold_min(d, sDt[k]);
         b.cell_dt[cell] = d;
-        b.status_next[cell] = status;
+        b.status[par ^ 1][cell] = status;
     }
     if (status) {  // NewLh, and Lh = NewLh at update_limiter (:287)
-        double *L = b.lh_next + (size_t)cell * V * SS;
+        double *L = b.lh[par ^ 1] + cell * V * SS;
         for (int k = lane; k < V * SS; k += 64) L[k] = sSub[(k % V) * SS + k / V];
     }
     __syncthreads();
@@ -788,7 +876,7 @@ ADG_DEV void subcell_finish(const MeshArgs &m, const StepBufs &b, int cell, int 
 // (Euler/limiter.f90:411-647)
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ __launch_bounds__(64) void k_subcell(MeshArgs m, StepBufs b)
+__global__ __launch_bounds__(64) void k_subcell(StepBufs b)
 {
     using P = EulerPDE;
     using S = Shape<P, N>;
@@ -805,29 +893,35 @@ __global__ __launch_bounds__(64) void k_subcell(MeshArgs m, StepBufs b)
     __shared__ double sTy[V * n * s], sBy[V * NN], sLim[V * SS], sLob[V * NN];
     __shared__ double sDt[NN];
     __shared__ int sFlag;
-    const Operators *__restrict__ op = b.op;
+    const OpPtr op = (OpPtr)b.op;
     const int lane = threadIdx.x;
     const int count = b.counters[kCntList];
-    const double dt = b.scal[0], time = b.scal[1];
-    const double ldx = m.dx / (double)s, ldy = m.dy / (double)s;
     for (int item = blockIdx.x; item < count; item += gridDim.x) {
-        const int cell = b.list[item];
-        const int flag = b.rc_final[cell];
+        const int gcell = b.list[item];
+        const int slot = ld_const(b.cell_solve + gcell);
+        const SolveArgs sa = ld_const(b.solves + slot);
+        const MeshArgs &m = sa.m;
+        const SolveState st = ld_const(b.state + slot);
+        const int par = st.parity;
+        const double dt = st.dt, time = st.time;
+        const double ldx = m.dx / (double)s, ldy = m.dy / (double)s;
+        const int cell = gcell - sa.cell0;  // local cell of the solve
+        const int flag = b.rc_final[gcell];
         const int ci = cell % m.imax, cj = cell / m.imax;
-        int nbk[5];
-        nbk[0] = cell;
-        nbk[1] = nb_index(ci, -1, m.imax, m.wrap) + m.imax * cj;
-        nbk[2] = nb_index(ci, 1, m.imax, m.wrap) + m.imax * cj;
-        nbk[3] = ci + m.imax * nb_index(cj, -1, m.jmax, m.wrap);
-        nbk[4] = ci + m.imax * nb_index(cj, 1, m.jmax, m.wrap);
+        int nbk[5];  // cells of the group arrays
+        nbk[0] = gcell;
+        nbk[1] = sa.cell0 + nb_index(ci, -1, m.imax, m.wrap) + m.imax * cj;
+        nbk[2] = sa.cell0 + nb_index(ci, 1, m.imax, m.wrap) + m.imax * cj;
+        nbk[3] = sa.cell0 + ci + m.imax * nb_index(cj, -1, m.jmax, m.wrap);
+        nbk[4] = sa.cell0 + ci + m.imax * nb_index(cj, 1, m.jmax, m.wrap);
         int stk[5];
 #pragma unroll
-        for (int k = 0; k < 5; k++) stk[k] = b.status_cur[nbk[k]];
+        for (int k = 0; k < 5; k++) stk[k] = b.status[par][nbk[k]];
         // subcell data of olduh for unlimited blocks, first pass (Euler/limiter.f90:25-27, :476-477)
         for (int task = lane; task < 5 * n * s; task += 64) {
             const int blk = task / (n * s), r = task - blk * n * s, a = r % n, qq = r / n;
             if (stk[blk] != 0) continue;
-            const double *u = b.uh_cur + (size_t)nbk[blk] * V * NN;
+            const double *u = b.uh[par] + (size_t)nbk[blk] * V * NN;
 #pragma unroll
             for (int v = 0; v < V; v++) {
                 double acc = 0.0;
@@ -863,7 +957,7 @@ __global__ __launch_bounds__(64) void k_subcell(MeshArgs m, StepBufs b)
                     val[v] = acc;
                 }
             } else {
-                const double *L = b.lh_cur + (size_t)nbk[blk] * V * SS;
+                const double *L = b.lh[par] + (size_t)nbk[blk] * V * SS;
 #pragma unroll
                 for (int v = 0; v < V; v++) val[v] = L[v + V * (p + s * qq)];
             }
@@ -982,7 +1076,7 @@ __global__ __launch_bounds__(64) void k_subcell(MeshArgs m, StepBufs b)
         }
 #undef S0
         __syncthreads();
-        subcell_finish<P, N>(m, b, cell, flag, sSub, sUhy, sUh, sTy, sBy, sLim, sLob, sDt, &sFlag);
+        subcell_finish<P, N>(m, b, gcell, par, flag, sSub, sUhy, sUh, sTy, sBy, sLim, sLob, sDt, &sFlag);
     }
 }
 
@@ -995,7 +1089,7 @@ __global__ __launch_bounds__(64) void k_subcell(MeshArgs m, StepBufs b)
 // slopes, Qt) where needed instead of being stored.
 // ---------------------------------------------------------------------------
 template <int N>
-__global__ __launch_bounds__(64) void k_subcell_bn(MeshArgs m, StepBufs b)
+__global__ __launch_bounds__(64) void k_subcell_bn(StepBufs b)
 {
     using P = BnPDE;
     using S = Shape<P, N>;
@@ -1013,24 +1107,30 @@ __global__ __launch_bounds__(64) void k_subcell_bn(MeshArgs m, StepBufs b)
     __shared__ double sUh[V * NN];
     __shared__ double sDt[NN];
     __shared__ int sFlag;
-    const Operators *__restrict__ op = b.op;
+    const OpPtr op = (OpPtr)b.op;
     const int lane = threadIdx.x;
     const int count = b.counters[kCntList];
-    const double dt = b.scal[0];
-    const double ldx = m.dx / (double)s, ldy = m.dy / (double)s;
     double *sLy = sR0;
     double *slx = sR0, *sly = sR0 + V * HH, *slt = sR0 + 2 * V * HH;
     for (int item = blockIdx.x; item < count; item += gridDim.x) {
-        const int cell = b.list[item];
-        const int flag = b.rc_final[cell];
+        const int gcell = b.list[item];
+        const int slot = ld_const(b.cell_solve + gcell);
+        const SolveArgs sa = ld_const(b.solves + slot);
+        const MeshArgs &m = sa.m;
+        const SolveState st = ld_const(b.state + slot);
+        const int par = st.parity;
+        const double dt = st.dt;
+        const double ldx = m.dx / (double)s, ldy = m.dy / (double)s;
+        const int cell = gcell - sa.cell0;  // local cell of the solve
+        const int flag = b.rc_final[gcell];
         const int ci = cell % m.imax, cj = cell / m.imax;
         // blocks (di,dj) of the 3x3 Voronoi neighbourhood, :463-479
         for (int task = lane; task < 9 * n * s; task += 64) {
             const int blk = task / (n * s), r = task - blk * n * s, a = r % n, qq = r / n;
             const int di = blk % 3 - 1, dj = blk / 3 - 1;
-            const int nb = nb_index(ci, di, m.imax, m.wrap) + m.imax * nb_index(cj, dj, m.jmax, m.wrap);
-            if (b.status_cur[nb] != 0) continue;
-            const double *u = b.uh_cur + (size_t)nb * V * NN;
+            const int nb = sa.cell0 + nb_index(ci, di, m.imax, m.wrap) + m.imax * nb_index(cj, dj, m.jmax, m.wrap);
+            if (b.status[par][nb] != 0) continue;
+            const double *u = b.uh[par] + (size_t)nb * V * NN;
 #pragma unroll
             for (int v = 0; v < V; v++) {
                 double acc = 0.0;
@@ -1045,8 +1145,8 @@ __global__ __launch_bounds__(64) void k_subcell_bn(MeshArgs m, StepBufs b)
             const int di = ii < 1 ? -1 : (ii > s ? 1 : 0), dj = jj < 1 ? -1 : (jj > s ? 1 : 0);
             const int p = ii - 1 - di * s, qq = jj - 1 - dj * s;
             const int blk = (di + 1) + 3 * (dj + 1);
-            const int nb = nb_index(ci, di, m.imax, m.wrap) + m.imax * nb_index(cj, dj, m.jmax, m.wrap);
-            if (b.status_cur[nb] == 0) {
+            const int nb = sa.cell0 + nb_index(ci, di, m.imax, m.wrap) + m.imax * nb_index(cj, dj, m.jmax, m.wrap);
+            if (b.status[par][nb] == 0) {
 #pragma unroll
                 for (int v = 0; v < V; v++) {
                     double acc = 0.0;
@@ -1055,7 +1155,7 @@ __global__ __launch_bounds__(64) void k_subcell_bn(MeshArgs m, StepBufs b)
                     sS0[v * GG + task] = acc;
                 }
             } else {
-                const double *L = b.lh_cur + (size_t)nb * V * SS;
+                const double *L = b.lh[par] + (size_t)nb * V * SS;
 #pragma unroll
                 for (int v = 0; v < V; v++) sS0[v * GG + task] = L[v + V * (p + s * qq)];
             }
@@ -1163,25 +1263,38 @@ __global__ __launch_bounds__(64) void k_subcell_bn(MeshArgs m, StepBufs b)
 #undef HIX
         __syncthreads();
         double *sTy = sR0, *sBy = sTy + V * n * s, *sLim = sBy + V * NN, *sLob = sLim + V * SS;
-        subcell_finish<P, N>(m, b, cell, flag, sSub, sUhy, sUh, sTy, sBy, sLim, sLob, sDt, &sFlag);
+        subcell_finish<P, N>(m, b, gcell, par, flag, sSub, sUhy, sUh, sTy, sBy, sLim, sLob, sDt, &sFlag);
     }
 }
 
-// calc_time_step reduction with the last-block pattern (no host sync, no memset)
-__global__ __launch_bounds__(kBlock) void k_dt(StepBufs b, int ncell, double host_time, double tend, int use_host_time)
+// calc_time_step (Euler/calc_time_step.f90:9-52) for every active solve: each
+// workgroup reduces the candidates of a piece of one solve and folds the result
+// into the solve's minimum with an integer atomic on the bit pattern (positive
+// doubles order like their bits; NaN candidates are skipped as MINVAL skips
+// them); the last workgroup to finish turns the minima into dt.  No host sync,
+// no memset.
+__global__ __launch_bounds__(kBlock) void k_dt(StepBufs b, double host_time, double host_tend, int use_host_time)
 {
     __shared__ double red[kBlock];
-    double mloc = __builtin_nan("");
-    for (int c = blockIdx.x * kBlock + threadIdx.x; c < ncell; c += gridDim.x * kBlock) mloc = fold_min(mloc, b.cell_dt[c]);
-    red[threadIdx.x] = mloc;
-    __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] = fold_min(red[threadIdx.x], red[threadIdx.x + w]);
+    __shared__ int last;
+    const int tid = threadIdx.x;
+    const int ntab = b.meta[kTabCell];
+    for (int e = blockIdx.x; e < ntab; e += gridDim.x) {
+        const int2 w = b.tab[kTabCell][e];
+        if (!b.state[w.x].active) continue;
+        const SolveArgs &sa = b.solves[w.x];
+        const int cell = w.y + tid;
+        red[tid] = cell < sa.m.ncell ? b.cell_dt[sa.cell0 + cell] : __builtin_nan("");
+        __syncthreads();
+        for (int k = kBlock / 2; k > 0; k >>= 1) {
+            if (tid < k) red[tid] = fold_min(red[tid], red[tid + k]);
+            __syncthreads();
+        }
+        if (tid == 0 && !__builtin_isnan(red[0]))
+            atomicMin(&b.state[w.x].dtmin, (unsigned long long)__double_as_longlong(red[0]));
         __syncthreads();
     }
-    __shared__ int last;
-    if (threadIdx.x == 0) {
-        b.dt_partial[blockIdx.x] = red[0];
+    if (tid == 0) {
         __threadfence();
         const int ticket = atomicAdd(&b.counters[kCntDtBlocks], 1);
         last = ticket == (int)gridDim.x - 1;
@@ -1189,128 +1302,195 @@ __global__ __launch_bounds__(kBlock) void k_dt(StepBufs b, int ncell, double hos
     __syncthreads();
     if (!last) return;
     __threadfence();
-    double mm = __builtin_nan("");
-    for (int k = threadIdx.x; k < (int)gridDim.x; k += kBlock)
-        mm = fold_min(mm, __hip_atomic_load(&b.dt_partial[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    red[threadIdx.x] = mm;
-    __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] = fold_min(red[threadIdx.x], red[threadIdx.x + w]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        const double time = use_host_time ? host_time : b.scal[1];
-        if (!use_host_time) tend = b.scal[2];
-        double dt = f_min(1.0e20, red[0]);  // dt = 1e20 then min over nodes (:257-269)
+    const int ns = b.meta[kMetaSolves];
+    for (int k = tid; k < ns; k += kBlock) {
+        SolveState &st = b.state[k];
+        if (!st.active) continue;
+        const unsigned long long bits = __hip_atomic_load(&st.dtmin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const double mn = bits == ~0ull ? __builtin_nan("") : __longlong_as_double((long long)bits);
+        const double time = use_host_time ? host_time : st.time;
+        const double tend = use_host_time ? host_tend : st.tend;
+        double dt = f_min(1.0e20, mn);           // dt = 1e20 then min over nodes (:257-269)
         if (time + dt > tend) dt = tend - time;  // :276-278
-        b.scal[0] = dt;
-        b.scal[1] = time;
-        b.scal[2] = tend;
-        b.counters[kCntDtBlocks] = 0;
+        st.dt = dt;
+        st.time = time;
+        st.tend = tend;
+        st.dtmin = ~0ull;
+    }
+    if (tid == 0) b.counters[kCntDtBlocks] = 0;
+}
+
+// Per-solve bookkeeping after a step: kPartAdvance (time = time + dt, count the
+// step, switch buffers, end the solve at its step count or end time),
+// kPartFlip (switch buffers only: update_limiter, Euler/limiter.f90:272-296),
+// kPartSeedDone (the seeds of freshly uploaded solutions are in place).
+__global__ __launch_bounds__(kBlock) void k_solves(StepBufs b, int part)
+{
+    const int ns = b.meta[kMetaSolves];
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < ns; k += gridDim.x * kBlock) {
+        SolveState &st = b.state[k];
+        if (part == kPartSeedDone) {
+            st.seed = 0;
+            continue;
+        }
+        if (!st.active) continue;
+        st.parity ^= 1;
+        if (part == kPartAdvance) {
+            st.time = st.time + st.dt;
+            st.steps += 1;
+            if (st.steps >= st.max_steps || (st.stop_at_tend && !(st.time < st.tend))) st.active = 0;
+        }
     }
 }
 
-__global__ void k_advance(StepBufs b)
+// Work tables from per-table prefix counts over the solve slots: entry e of
+// table k belongs to the last slot s with prefix[s] <= e.
+__global__ __launch_bounds__(kBlock) void k_tables(StepBufs b, const int *prefix, int nslot, int4 quanta)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) b.scal[1] = b.scal[1] + b.scal[0];  // time = time + dt
+    const int q[kNumTabs] = {quanta.x, quanta.y, quanta.z, quanta.w};
+    int *meta = const_cast<int *>(b.meta);
+    for (int k = 0; k < kNumTabs; k++) {
+        const int *pk = prefix + k * (nslot + 1);
+        const int total = pk[nslot];
+        int2 *tab = const_cast<int2 *>(b.tab[k]);
+        for (int e = blockIdx.x * kBlock + threadIdx.x; e < total; e += gridDim.x * kBlock) {
+            int lo = 0, hi = nslot - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (pk[mid] <= e)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            tab[e] = make_int2(lo, (e - pk[lo]) * q[k]);
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) meta[k] = total;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) meta[kMetaSolves] = nslot;
 }
 
 template <class P, int N>
-hipError_t launch_step_impl(const MeshArgs &m, const StepBufs &b, hipStream_t st, int part)
+hipError_t launch_step_impl(const StepBufs &b, hipStream_t st, int part, const int grid[kNumTabs])
 {
     using S = Shape<P, N>;
-    const int gcell = (m.ncell + S::CPW - 1) / S::CPW;
-    const int gupd = (m.ncell + S::CPWU - 1) / S::CPWU;
     switch (part) {
     case kPartPredictor:
-        hipLaunchKernelGGL((k_predictor<P, N, false>), dim3(gcell), dim3(S::PRED_BLOCK), 0, st, m, b, TestBufs{});
+        hipLaunchKernelGGL((k_predictor<P, N>), dim3(grid[kTabPred]), dim3(S::PRED_BLOCK), 0, st, b);
         break;
-    case kPartFace: {
-        const long pts = ((long)(m.imax + 1) * m.jmax + (long)m.imax * (m.jmax + 1)) * S::n;
-        const long g = (pts + 255) / 256;
-        hipLaunchKernelGGL((k_face<P, N>), dim3(g < 65536 ? g : 65536), dim3(256), 0, st, m, b);
+    case kPartFace:
+        hipLaunchKernelGGL((k_face<P, N>), dim3(grid[kTabFace]), dim3(kBlock), 0, st, b);
         break;
-    }
     case kPartUpdate:
-        hipLaunchKernelGGL((k_update<P, N, false>), dim3(gupd), dim3(S::UPD_BLOCK), 0, st, m, b);
+        hipLaunchKernelGGL((k_update<P, N, false>), dim3(grid[kTabUpd]), dim3(S::UPD_BLOCK), 0, st, b);
+        break;
+    case kPartSeed:
+        hipLaunchKernelGGL((k_update<P, N, true>), dim3(grid[kTabUpd]), dim3(S::UPD_BLOCK), 0, st, b);
         break;
     case kPartSpread:
-        hipLaunchKernelGGL(k_spread, dim3((m.ncell + kBlock - 1) / kBlock), dim3(kBlock), 0, st, m, b);
+        hipLaunchKernelGGL(k_spread, dim3(grid[kTabCell]), dim3(kBlock), 0, st, b);
         break;
     case kPartSubcell: {
-        const int g = m.ncell < kSubcellGrid ? m.ncell : kSubcellGrid;
+        const int g = b.G < kSubcellGrid ? b.G : kSubcellGrid;
         if (P::kNcp)
-            hipLaunchKernelGGL((k_subcell_bn<N>), dim3(g), dim3(64), 0, st, m, b);
+            hipLaunchKernelGGL((k_subcell_bn<N>), dim3(g), dim3(64), 0, st, b);
         else
-            hipLaunchKernelGGL((k_subcell<N>), dim3(g), dim3(64), 0, st, m, b);
+            hipLaunchKernelGGL((k_subcell<N>), dim3(g), dim3(64), 0, st, b);
         break;
     }
-    case kPartSeed:
-        hipLaunchKernelGGL((k_update<P, N, true>), dim3(gupd), dim3(S::UPD_BLOCK), 0, st, m, b);
-        break;
     default:
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
+template <class P>
+hipError_t launch_part_pde(int N, const StepBufs &b, hipStream_t st, int part, const int grid[kNumTabs])
+{
+    switch (N) {
+    case 1: return launch_step_impl<P, 1>(b, st, part, grid);
+    case 2: return launch_step_impl<P, 2>(b, st, part, grid);
+    case 3: return launch_step_impl<P, 3>(b, st, part, grid);
+    case 4: return launch_step_impl<P, 4>(b, st, part, grid);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 template <class P, int N>
-hipError_t launch_test_predictor_impl(const MeshArgs &m, const StepBufs &b, const TestBufs &t, hipStream_t st)
+hipError_t launch_test_impl(const MeshArgs &m, const StepBufs &b, const TestBufs &t, double dt, hipStream_t st)
 {
     using S = Shape<P, N>;
-    const int gcell = (m.ncell + S::CPW - 1) / S::CPW;
-    hipLaunchKernelGGL((k_predictor<P, N, true>), dim3(gcell), dim3(S::PRED_BLOCK), 0, st, m, b, t);
+    const int g = (m.ncell + S::CPW - 1) / S::CPW;
+    hipLaunchKernelGGL((k_predictor_test<P, N>), dim3(g), dim3(S::PRED_BLOCK), 0, st, m, b, t, dt);
     return hipGetLastError();
 }
 
 template <class P>
-hipError_t launch_part_pde(int N, const MeshArgs &m, const StepBufs &b, hipStream_t st, int part)
+hipError_t launch_test_pde(int N, const MeshArgs &m, const StepBufs &b, const TestBufs &t, double dt, hipStream_t st)
 {
     switch (N) {
-    case 1: return launch_step_impl<P, 1>(m, b, st, part);
-    case 2: return launch_step_impl<P, 2>(m, b, st, part);
-    case 3: return launch_step_impl<P, 3>(m, b, st, part);
-    case 4: return launch_step_impl<P, 4>(m, b, st, part);
+    case 1: return launch_test_impl<P, 1>(m, b, t, dt, st);
+    case 2: return launch_test_impl<P, 2>(m, b, t, dt, st);
+    case 3: return launch_test_impl<P, 3>(m, b, t, dt, st);
+    case 4: return launch_test_impl<P, 4>(m, b, t, dt, st);
     default: return hipErrorInvalidValue;
     }
 }
 
 template <class P>
-hipError_t launch_test_pde(int N, const MeshArgs &m, const StepBufs &b, const TestBufs &t, hipStream_t st)
+void quanta_pde(int N, int q[kNumTabs])
 {
     switch (N) {
-    case 1: return launch_test_predictor_impl<P, 1>(m, b, t, st);
-    case 2: return launch_test_predictor_impl<P, 2>(m, b, t, st);
-    case 3: return launch_test_predictor_impl<P, 3>(m, b, t, st);
-    case 4: return launch_test_predictor_impl<P, 4>(m, b, t, st);
-    default: return hipErrorInvalidValue;
+    case 1: q[kTabPred] = Shape<P, 1>::CPW; q[kTabUpd] = Shape<P, 1>::CPWU; break;
+    case 2: q[kTabPred] = Shape<P, 2>::CPW; q[kTabUpd] = Shape<P, 2>::CPWU; break;
+    case 3: q[kTabPred] = Shape<P, 3>::CPW; q[kTabUpd] = Shape<P, 3>::CPWU; break;
+    default: q[kTabPred] = Shape<P, 4>::CPW; q[kTabUpd] = Shape<P, 4>::CPWU; break;
     }
+    q[kTabFace] = kBlock;
+    q[kTabCell] = kBlock;
 }
 
 }  // namespace
 
-hipError_t launch_part(int N, const MeshArgs &m, const StepBufs &b, hipStream_t st, int part)
+void table_quanta(int pde, int N, int quanta[kNumTabs])
 {
-    if (part == kPartDt) {
-        hipLaunchKernelGGL(k_dt, dim3(kDtGrid), dim3(kBlock), 0, st, b, m.ncell, 0.0, 0.0, 0);
-        return hipGetLastError();
-    }
-    if (part == kPartAdvance) {
-        hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, b);
-        return hipGetLastError();
-    }
-    return m.pde == kPdeBn ? launch_part_pde<BnPDE>(N, m, b, st, part) : launch_part_pde<EulerPDE>(N, m, b, st, part);
+    if (pde == kPdeBn)
+        quanta_pde<BnPDE>(N, quanta);
+    else
+        quanta_pde<EulerPDE>(N, quanta);
 }
 
-hipError_t launch_dt(const MeshArgs &m, const StepBufs &b, hipStream_t st, double time, double tend, int use_host_time)
+hipError_t launch_part(int pde, int N, const StepBufs &b, hipStream_t st, int part, const int grid[kNumTabs],
+                       int nslot_cap)
 {
-    hipLaunchKernelGGL(k_dt, dim3(kDtGrid), dim3(kBlock), 0, st, b, m.ncell, time, tend, use_host_time);
+    if (part == kPartDt) return launch_dt(b, st, 0.0, 0.0, 0);
+    if (part == kPartAdvance || part == kPartFlip || part == kPartSeedDone) {
+        const int g = (nslot_cap + kBlock - 1) / kBlock;
+        hipLaunchKernelGGL(k_solves, dim3(g > 0 ? g : 1), dim3(kBlock), 0, st, b, part);
+        return hipGetLastError();
+    }
+    return pde == kPdeBn ? launch_part_pde<BnPDE>(N, b, st, part, grid) : launch_part_pde<EulerPDE>(N, b, st, part, grid);
+}
+
+hipError_t launch_dt(const StepBufs &b, hipStream_t st, double time, double tend, int use_host_time)
+{
+    hipLaunchKernelGGL(k_dt, dim3(kDtGrid), dim3(kBlock), 0, st, b, time, tend, use_host_time);
     return hipGetLastError();
 }
 
-hipError_t launch_test_predictor(int N, const MeshArgs &m, const StepBufs &b, const TestBufs &t, hipStream_t st)
+hipError_t launch_tables(const StepBufs &b, const int *prefix, int nslot, const int quanta[kNumTabs], int max_entries,
+                         hipStream_t st)
 {
-    return m.pde == kPdeBn ? launch_test_pde<BnPDE>(N, m, b, t, st) : launch_test_pde<EulerPDE>(N, m, b, t, st);
+    int g = (max_entries + kBlock - 1) / kBlock;
+    g = g < 1 ? 1 : (g > 4096 ? 4096 : g);
+    hipLaunchKernelGGL(k_tables, dim3(g), dim3(kBlock), 0, st, b, prefix, nslot,
+                       make_int4(quanta[0], quanta[1], quanta[2], quanta[3]));
+    return hipGetLastError();
+}
+
+hipError_t launch_test_predictor(int N, const MeshArgs &m, const StepBufs &b, const TestBufs &t, double dt,
+                                 hipStream_t st)
+{
+    return m.pde == kPdeBn ? launch_test_pde<BnPDE>(N, m, b, t, dt, st) : launch_test_pde<EulerPDE>(N, m, b, t, dt, st);
 }
 
 }  // namespace aderdg

@@ -1,0 +1,104 @@
+// step_kernels.hpp -- launch interface of the element-step kernels (2-D Euler
+// and Baer-Nunziato), implemented in step_kernels.hip.
+//
+// The kernels work on a group: any number of independent solves of one
+// (PDE, N), concatenated in flat device arrays.  A solve is described by
+// SolveArgs (static: mesh, boundaries, equation of state, offsets into the
+// flat arrays) and SolveState (dynamic: dt, time, ping-pong parity, active
+// flag).  Workgroups take their work from device-resident tables of
+// (solve, first item) entries that never straddle two solves, so everything a
+// workgroup reads about its solve is uniform.  The single-instance solver is a
+// group of one solve.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "operators.hpp"
+
+namespace aderdg {
+
+constexpr int kBlock = 256;        // spread / dt / face workgroup size
+constexpr int kDtGrid = 256;       // workgroups of the time-step reduction
+constexpr int kSubcellGrid = 4096; // persistent workgroups of the subcell recompute
+
+enum { kPdeEuler = 0, kPdeBn = 1 };
+enum { kBcDmr = 0, kBcPeriodic = 1, kBcTransmissive = 2 };
+enum { kCntList = 0, kCntTroubled = 1, kCntDtBlocks = 2, kNumCounters = 4 };
+enum { kPartDt = 0, kPartPredictor = 1, kPartUpdate = 2, kPartSpread = 3, kPartSubcell = 4, kPartAdvance = 5,
+       kPartSeed = 6, kPartFace = 7, kPartFlip = 8, kPartSeedDone = 9 };
+// work tables: items per entry are cells (predictor, update, cell) or face points (face)
+enum { kTabPred = 0, kTabUpd = 1, kTabFace = 2, kTabCell = 3, kNumTabs = 4 };
+// device meta words: [0, kNumTabs) table lengths, then the number of solve slots in use
+enum { kMetaSolves = kNumTabs, kNumMeta = kNumTabs + 1 };
+
+// Mesh and equation constants of one solve.
+struct MeshArgs {
+    int pde, imax, jmax, ncell, bc_mode, wrap;
+    double dx, dy, gamma, cflpnpm;
+    double eos[4];             // Baer-Nunziato: GAMMA_S, GAMMA_G, PI_S, PI_G
+    double dmr_a, dmr_b;       // DMR shock position x_sw = dmr_a + dmr_b * t (Euler/riemann_problem.f90:1-8)
+    double qa[4], qb[4];       // DMR inflow / pre-shock conserved states
+};
+
+// Static description of one solve of a group.
+struct SolveArgs {
+    MeshArgs m;
+    int cell0;      // first cell in the group's cell arrays
+    int fx0, fy0;   // first x-face ((IMAX+1)*JMAX of them) and y-face (IMAX*(JMAX+1))
+    int pad;
+};
+
+// Dynamic per-solve scalars, written by the kernels.
+struct SolveState {
+    double dt, time, tend;
+    unsigned long long dtmin;  // ordered bits of the smallest time-step candidate, ~0 = none
+    int parity;                // buffer (0/1) that holds the current solution
+    int active;                // 1 while the solve is stepping
+    int seed;                  // 1 until the limiter seeds of a fresh solution are computed
+    int steps, max_steps;      // steps taken, and the count that ends the solve
+    int stop_at_tend;          // 1: the solve also ends once time >= tend
+    int pad[2];
+};
+
+// Device buffers of a group.  Cell arrays are strided by G (cell capacity), face
+// arrays by GX / GY (x- / y-face capacity).
+struct StepBufs {
+    const Operators *op;
+    const SolveArgs *solves;
+    SolveState *state;
+    const int2 *tab[kNumTabs];  // (solve, first item) per workgroup-sized piece of work
+    const int *meta;
+    int G, GX, GY;
+    const int *cell_solve;      // solve slot of every cell (compacted-list kernels)
+    double *uh[2];              // [cell][j][i][v], the reference's uh(V,n,n,IMAX,JMAX) per cell
+    double *lduh;
+    double *fq, *ff;            // [face][v][p][cell] face extrapolations of q and F*n
+    double *xflux, *yflux;      // [side][v][p][face]: side 0 D- (Euler: the flux), side 1 D+ (Baer-Nunziato)
+    double *lmin_old, *lmax_old, *lmin, *lmax, *cell_dt;  // [v][cell], [cell]
+    int *rc, *rc_final, *list, *counters;
+    int *status[2];
+    double *lh[2];              // [cell][q][p][v] subcell averages of limited cells
+};
+
+// Outputs of the standalone predictor (reference layouts per cell; shi only for Baer-Nunziato).
+struct TestBufs {
+    const double *uh;
+    double *qhi, *fhi, *qbnd, *fbnd, *shi;
+};
+
+// Items (cells or face points) per work-table entry of each table, for a (PDE, N).
+void table_quanta(int pde, int N, int quanta[kNumTabs]);
+
+// One part of the step for every solve of the group.  grid: workgroups per table-driven
+// kernel (they loop over the table); nslot_cap: solve slots of the group.
+hipError_t launch_part(int pde, int N, const StepBufs &b, hipStream_t st, int part, const int grid[kNumTabs],
+                       int nslot_cap);
+// calc_time_step of every active solve; use_host: time and tend given here instead of the solve state.
+hipError_t launch_dt(const StepBufs &b, hipStream_t st, double time, double tend, int use_host);
+// Rebuild the work tables from per-table prefix counts of entries over the solve slots
+// (prefix[k * (nslot + 1) + s], device memory); max_entries bounds every table's length.
+hipError_t launch_tables(const StepBufs &b, const int *prefix, int nslot, const int quanta[kNumTabs], int max_entries,
+                         hipStream_t st);
+hipError_t launch_test_predictor(int N, const MeshArgs &m, const StepBufs &b, const TestBufs &t, double dt,
+                                 hipStream_t st);
+
+}  // namespace aderdg

@@ -32,6 +32,7 @@ extern "C" {
 #define ADERDG_ESINGULAR -3/* singular operator matrix (Euler/la.f90:39-44) */
 #define ADERDG_ESTATE -4   /* call out of order */
 #define ADERDG_ECAPACITY -5/* a fixed-size buffer is too small */
+#define ADERDG_EFULL -6    /* no room for another solve in a batch arena (admission refused) */
 
 /* 2-D Euler (nVar 4, Euler/) and 2-D Baer-Nunziato two-phase flow (nVar 9,
  * Baer-Nunziato/, path-conservative fluxes, transmissive or periodic boundaries) */
