@@ -46,6 +46,23 @@ class _Desc(ctypes.Structure):
                 ("device", ctypes.c_int), ("ops", _Ops), ("bn_eos", ctypes.c_double * 4)]
 
 
+class _BatchDesc(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("persist_blocks", ctypes.c_int)]
+
+
+def _desc(pde, N, imax, jmax, xl, xr, bc, gamma, cfl, device, eos):
+    d = _Desc()
+    d.pde, d.N, d.imax, d.jmax = pde, N, imax, jmax
+    if eos is not None:
+        for k in range(4):
+            d.bn_eos[k] = float(eos[k])
+    d.xl[0], d.xl[1] = xl
+    d.xr[0], d.xr[1] = xr
+    d.bc_mode = BC[bc] if isinstance(bc, str) else int(bc)
+    d.gamma, d.cfl, d.exact, d.device = gamma, cfl, 0, device
+    return d
+
+
 _libs = {}
 
 
@@ -89,6 +106,30 @@ def load_library(exact=False):
         f = getattr(lib, name)
         f.argtypes = args
         f.restype = ctypes.c_int
+    i64 = ctypes.c_int64
+    bsig = {
+        "aderdg_batch_create": [ctypes.POINTER(_BatchDesc), ctypes.POINTER(ctypes.c_void_p)],
+        "aderdg_batch_destroy": [vp],
+        "aderdg_batch_reserve": [vp, ctypes.c_int, ctypes.c_int, i64, ctypes.c_int],
+        "aderdg_batch_add": [vp, ctypes.POINTER(_Desc), _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                             ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+        "aderdg_batch_step": [vp, ctypes.c_int],
+        "aderdg_batch_sync": [vp],
+        "aderdg_batch_query": [vp, ctypes.c_int, _dp, _dp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+        "aderdg_batch_finished": [vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+        "aderdg_batch_download": [vp, ctypes.c_int, _dp],
+        "aderdg_batch_limiter": [vp, ctypes.c_int, _ip, _ip],
+        "aderdg_batch_remove": [vp, ctypes.c_int],
+        "aderdg_batch_usage": [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(i64)],
+    }
+    for name, args in bsig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    lib.aderdg_solve_footprint.restype = i64
+    lib.aderdg_solve_footprint.argtypes = [ctypes.c_int] * 4
+    lib.aderdg_batch_reserved_bytes.restype = i64
+    lib.aderdg_batch_reserved_bytes.argtypes = [vp]
     lib.aderdg_flops_predictor.restype = ctypes.c_double
     lib.aderdg_flops_predictor.argtypes = [ctypes.c_int, ctypes.c_int]
     lib.aderdg_flops_update.restype = ctypes.c_double
@@ -134,15 +175,7 @@ class Solver:
         self.pde = PDE[pde] if isinstance(pde, str) else int(pde)
         self.N, self.n, self.s, self.V = N, N + 1, 2 * N + 1, NVAR[self.pde]
         self.imax, self.jmax = imax, jmax
-        d = _Desc()
-        d.pde, d.N, d.imax, d.jmax = self.pde, N, imax, jmax
-        if eos is not None:
-            for k in range(4):
-                d.bn_eos[k] = float(eos[k])
-        d.xl[0], d.xl[1] = xl
-        d.xr[0], d.xr[1] = xr
-        d.bc_mode = BC[bc] if isinstance(bc, str) else int(bc)
-        d.gamma, d.cfl, d.exact, d.device = gamma, cfl, 0, device
+        d = _desc(self.pde, N, imax, jmax, xl, xr, bc, gamma, cfl, device, eos)
         self._keep = []
         if ops is not None:
             for k in _Ops._fields_:
@@ -260,3 +293,118 @@ class Solver:
         ln = (ctypes.c_int64 * len(TIMING_SLOTS))()
         self._check(self.lib.aderdg_timing_get(self.h, ms, ln))
         return list(ms), list(ln)
+
+
+def solve_footprint(pde, N, imax, jmax, exact=False):
+    """Device bytes one solve of this shape occupies in a batch arena."""
+    k = PDE[pde] if isinstance(pde, str) else int(pde)
+    return int(load_library(exact).aderdg_solve_footprint(k, N, imax, jmax))
+
+
+class ArenaFull(AderdgError):
+    """aderdg_batch_add found no free slot or range (ADERDG_EFULL)."""
+
+
+class Batch:
+    """Independent solves of any mesh size, degree and PDE stepped together
+    (aderdg_batch_* of include/aderdg.h): one arena per (pde, N), one step
+    graph for all of them.
+
+        b = Batch()
+        b.reserve("bn", 3, max_cells=4_000_000, max_solves=128)
+        i = b.add(uh, N=3, imax=96, jmax=96, xl=(0, 0), xr=(1, 1), bc="transmissive",
+                  pde="bn", eos=eos, max_steps=50)
+        b.step(50); b.sync()
+        uh_i = b.download(i)
+    """
+
+    EFULL = -6
+
+    def __init__(self, exact=False, device=0, persist_blocks=0):
+        self.lib = load_library(exact)
+        self.device = device
+        d = _BatchDesc()
+        d.device, d.persist_blocks = device, persist_blocks
+        h = ctypes.c_void_p()
+        self._check(self.lib.aderdg_batch_create(ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+        self.shapes = {}
+
+    def _check(self, rc):
+        if rc == self.EFULL:
+            raise ArenaFull(self.lib.aderdg_last_error().decode())
+        if rc != 0:
+            raise AderdgError("aderdg error %d: %s" % (rc, self.lib.aderdg_last_error().decode()))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.aderdg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reserve(self, pde, N, max_cells, max_solves):
+        k = PDE[pde] if isinstance(pde, str) else int(pde)
+        self._check(self.lib.aderdg_batch_reserve(self.h, k, N, int(max_cells), int(max_solves)))
+
+    def reserved_bytes(self):
+        return int(self.lib.aderdg_batch_reserved_bytes(self.h))
+
+    def add(self, uh, N, imax, jmax, xl, xr, bc="periodic", pde="euler", gamma=1.4, cfl=0.9, eos=None, time=0.0,
+            tend=1e30, max_steps=1 << 30, stop_at_tend=False):
+        k = PDE[pde] if isinstance(pde, str) else int(pde)
+        shape = (NVAR[k], N + 1, N + 1, imax, jmax)
+        uh = np.asfortranarray(uh, dtype=np.float64)
+        if uh.shape != shape:
+            raise ValueError("uh shape %s, expected %s" % (uh.shape, shape))
+        d = _desc(k, N, imax, jmax, xl, xr, bc, gamma, cfl, self.device, eos)
+        i = ctypes.c_int()
+        self._check(self.lib.aderdg_batch_add(self.h, ctypes.byref(d), _p(uh), float(time), float(tend),
+                                              int(max_steps), 1 if stop_at_tend else 0, ctypes.byref(i)))
+        self.shapes[i.value] = shape
+        return i.value
+
+    def step(self, nsteps=1):
+        self._check(self.lib.aderdg_batch_step(self.h, int(nsteps)))
+
+    def sync(self):
+        self._check(self.lib.aderdg_batch_sync(self.h))
+
+    def query(self, i):
+        t, dt = ctypes.c_double(), ctypes.c_double()
+        steps, active = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.aderdg_batch_query(self.h, i, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(steps),
+                                                ctypes.byref(active)))
+        return dict(time=t.value, dt=dt.value, steps=steps.value, active=bool(active.value))
+
+    def finished(self):
+        n = max(len(self.shapes), 1)
+        ids = (ctypes.c_int * n)()
+        c = ctypes.c_int()
+        self._check(self.lib.aderdg_batch_finished(self.h, ids, n, ctypes.byref(c)))
+        return [ids[k] for k in range(c.value)]
+
+    def download(self, i):
+        uh = np.zeros(self.shapes[i], order="F")
+        self._check(self.lib.aderdg_batch_download(self.h, i, _p(uh)))
+        return uh
+
+    def limiter(self, i):
+        I, J = self.shapes[i][3:]
+        st = np.zeros((I, J), dtype=np.int32, order="F")
+        rc = np.zeros((I, J), dtype=np.int32, order="F")
+        self._check(self.lib.aderdg_batch_limiter(self.h, i, st.ctypes.data_as(_ip), rc.ctypes.data_as(_ip)))
+        return st, rc
+
+    def remove(self, i):
+        self._check(self.lib.aderdg_batch_remove(self.h, i))
+        self.shapes.pop(i, None)
+
+    def usage(self):
+        n, c = ctypes.c_int(), ctypes.c_int64()
+        self._check(self.lib.aderdg_batch_usage(self.h, ctypes.byref(n), ctypes.byref(c)))
+        return n.value, c.value

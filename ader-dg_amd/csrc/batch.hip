@@ -1,0 +1,366 @@
+// batch.hip -- the batched engine of include/aderdg.h (aderdg_batch_*): one
+// solve group per (pde, N), a single captured graph that steps every group,
+// admission into the groups' arenas and retirement of finished solves.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "../../include/aderdg.h"
+#include "batch.hpp"
+#include "capi_common.hpp"
+
+using namespace aderdg;
+
+namespace aderdg {
+
+int Batch::init(int device_, int persist_)
+{
+    device = device_;
+    HIPCHK(hipSetDevice(device));
+    int cus = 0;
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    persist = persist_ > 0 ? persist_ : 8 * std::max(cus, 1);
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&copy, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&upload_done, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&removed, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&states_ready, hipEventDisableTiming));
+    return 0;
+}
+
+void Batch::release()
+{
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (copy) (void)hipStreamSynchronize(copy);
+    if (graph) (void)hipGraphExecDestroy(graph);
+    graph = nullptr;
+    for (auto &g : groups) g->release();
+    groups.clear();
+    for (auto p : pinned_states)
+        if (p) (void)hipHostFree(p);
+    pinned_states.clear();
+    hipEvent_t evs[] = {upload_done, removed, states_ready};
+    for (auto e : evs)
+        if (e) (void)hipEventDestroy(e);
+    upload_done = removed = states_ready = nullptr;
+    if (copy) (void)hipStreamDestroy(copy);
+    if (stream) (void)hipStreamDestroy(stream);
+    copy = stream = nullptr;
+}
+
+int Batch::group_of(int pde, int N) const
+{
+    for (size_t k = 0; k < groups.size(); k++)
+        if (groups[k]->pde == pde && groups[k]->N == N) return (int)k;
+    return -1;
+}
+
+int64_t Batch::footprint(int pde, int N, int imax, int jmax)
+{
+    int64_t pc, px, py, ps;
+    Group::footprint(pde, N, &pc, &px, &py, &ps);
+    return pc * imax * jmax + px * (int64_t)(imax + 1) * jmax + py * (int64_t)imax * (jmax + 1) + ps;
+}
+
+int Batch::reserve(int pde, int N, int64_t max_cells, int max_solves)
+{
+    if (pde != ADERDG_PDE_EULER && pde != ADERDG_PDE_BN) return fail(ADERDG_EINVAL, "aderdg_batch_reserve: bad pde %d", pde);
+    if (N < 1 || N > kMaxDegree) return fail(ADERDG_EINVAL, "aderdg_batch_reserve: N=%d outside 1..%d", N, kMaxDegree);
+    if (max_cells < 1 || max_cells > (1LL << 28) || max_solves < 1 || max_solves > (1 << 20))
+        return fail(ADERDG_EINVAL, "aderdg_batch_reserve: bad capacity (%lld cells, %d solves)", (long long)max_cells,
+                    max_solves);
+    if (group_of(pde, N) >= 0) return fail(ADERDG_EINVAL, "aderdg_batch_reserve: (pde %d, N %d) already reserved", pde, N);
+    Operators ops;
+    if (!build_operators(N, ops)) return fail(ADERDG_ESINGULAR, "aderdg_batch_reserve: operator construction failed");
+    // faces: (IMAX+1)*JMAX = cells + JMAX <= cells * 9/8 when IMAX >= 8, and alike for y
+    const int64_t faces = max_cells + max_cells / 8 + 1;
+    const GroupCaps caps{max_cells, faces, faces, max_solves};
+    auto g = std::make_unique<Group>();
+    int r = g->init(pde, N, caps, ops, persist, stream);
+    if (r) {
+        g->release();
+        return fail(r == ADERDG_EINVAL ? r : ADERDG_EHIP, "aderdg_batch_reserve: arena allocation failed");
+    }
+    SolveState *ps = nullptr;
+    if (hipHostMalloc((void **)&ps, sizeof(SolveState) * max_solves, hipHostMallocDefault) != hipSuccess) {
+        g->release();
+        return fail(ADERDG_EHIP, "aderdg_batch_reserve: hipHostMalloc failed");
+    }
+    int64_t pc, px, py, psl;
+    Group::footprint(pde, N, &pc, &px, &py, &psl);
+    reserved += pc * max_cells + (px + py) * faces + psl * max_solves;
+    groups.push_back(std::move(g));
+    pinned_states.push_back(ps);
+    graph_stale = true;
+    return 0;
+}
+
+int Batch::add(const aderdg_desc *d, const double *uh, double time, double tend, int max_steps, int stop_at_tend,
+               int *id)
+{
+    int r = check_desc(d, "aderdg_batch_add");
+    if (r) return r;
+    if (!uh || !id || max_steps < 1) return fail(ADERDG_EINVAL, "aderdg_batch_add: bad argument");
+    const aderdg_ops &o = d->ops;
+    if (o.wGPN || o.Kxi || o.iK1 || o.F0 || o.FLcoeff || o.FRcoeff || o.uh2lim || o.lim2uh || o.uh2lob)
+        return fail(ADERDG_EINVAL, "aderdg_batch_add: batched solves use the built-in operators (ops must be NULL)");
+    if (d->device != device) return fail(ADERDG_EINVAL, "aderdg_batch_add: solve on device %d, batch on %d", d->device, device);
+    const int gi = group_of(d->pde, d->N);
+    if (gi < 0) return fail(ADERDG_EINVAL, "aderdg_batch_add: no arena reserved for (pde %d, N %d)", d->pde, d->N);
+    Group &g = *groups[gi];
+    SolveState st{};
+    st.time = time;
+    st.tend = tend;
+    st.dtmin = ~0ull;
+    st.active = 1;
+    st.seed = 1;
+    st.max_steps = max_steps;
+    st.stop_at_tend = stop_at_tend ? 1 : 0;
+    int slot = -1;
+    r = g.add(mesh_from_desc(d), st, &slot);
+    if (r == ADERDG_EFULL) return fail(r, "aderdg_batch_add: arena (pde %d, N %d) full", d->pde, d->N);
+    if (r) return fail(r, "aderdg_batch_add: placement failed");
+    // the copy may reuse cells of a removed solve: order it after the removal
+    HIPCHK(hipStreamWaitEvent(copy, removed, 0));
+    if ((r = g.upload_uh(slot, uh, 0, copy))) return fail(r, "aderdg_batch_add: upload failed");
+    HIPCHK(hipEventRecord(upload_done, copy));
+    HIPCHK(hipStreamWaitEvent(stream, upload_done, 0));
+    hipError_t e = g.launch(kPartSeed);
+    if (e == hipSuccess) e = g.launch(kPartSeedDone);
+    if (e != hipSuccess) return fail(ADERDG_EHIP, "aderdg_batch_add: seed launch: %s", hipGetErrorString(e));
+    const int h = next_id++;
+    solves[h] = Handle{gi, slot};
+    *id = h;
+    return 0;
+}
+
+int Batch::build_graph()
+{
+    if (graph) (void)hipGraphExecDestroy(graph);
+    graph = nullptr;
+    hipGraph_t gr = nullptr;
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    static const int parts[] = {kPartPredictor, kPartFace, kPartUpdate, kPartSpread, kPartSubcell, kPartAdvance};
+    for (auto &g : groups) {
+        if (e == hipSuccess) e = launch_dt(g->b, stream, 0.0, 0.0, 0);
+        for (int p : parts)
+            if (e == hipSuccess) e = g->launch(p);
+    }
+    hipError_t e2 = hipStreamEndCapture(stream, &gr);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return fail(ADERDG_EHIP, "aderdg_batch: step capture failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    e = hipGraphInstantiate(&graph, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) return fail(ADERDG_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    graph_stale = false;
+    return 0;
+}
+
+int Batch::step(int nsteps)
+{
+    if (nsteps < 0) return fail(ADERDG_EINVAL, "aderdg_batch_step: negative step count");
+    if (groups.empty()) return fail(ADERDG_ESTATE, "aderdg_batch_step: nothing reserved");
+    if (graph_stale) {
+        int r = build_graph();
+        if (r) return r;
+    }
+    for (int k = 0; k < nsteps; k++) HIPCHK(hipGraphLaunch(graph, stream));
+    return 0;
+}
+
+int Batch::snapshot_async()
+{
+    for (size_t k = 0; k < groups.size(); k++)
+        if (groups[k]->read_states_async(pinned_states[k])) return fail(ADERDG_EHIP, "aderdg_batch: state readback failed");
+    HIPCHK(hipEventRecord(states_ready, stream));
+    return 0;
+}
+
+bool Batch::snapshot_ready() const { return hipEventQuery(states_ready) == hipSuccess; }
+
+int Batch::snapshot()
+{
+    int r = snapshot_async();
+    if (r) return r;
+    HIPCHK(hipEventSynchronize(states_ready));
+    return 0;
+}
+
+const SolveState *Batch::state_of(int id) const
+{
+    auto it = solves.find(id);
+    if (it == solves.end()) return nullptr;
+    return &pinned_states[it->second.group][it->second.slot];
+}
+
+int Batch::download(int id, double *uh)
+{
+    auto it = solves.find(id);
+    if (it == solves.end()) return fail(ADERDG_EINVAL, "aderdg_batch_download: unknown id %d", id);
+    int r = snapshot();
+    if (r) return r;
+    Group &g = *groups[it->second.group];
+    const int par = pinned_states[it->second.group][it->second.slot].parity;
+    if ((r = g.download_uh(it->second.slot, uh, par))) return fail(r, "aderdg_batch_download: copy failed");
+    HIPCHK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+int Batch::limiter(int id, int32_t *status, int32_t *recompute)
+{
+    auto it = solves.find(id);
+    if (it == solves.end()) return fail(ADERDG_EINVAL, "aderdg_batch_limiter: unknown id %d", id);
+    int r = snapshot();
+    if (r) return r;
+    Group &g = *groups[it->second.group];
+    const int sl = it->second.slot;
+    const int par = pinned_states[it->second.group][sl].parity;
+    const size_t c0 = g.cell0(sl), nc = g.mesh(sl).ncell;
+    if (status) HIPCHK(hipMemcpyAsync(status, g.b.status[par] + c0, nc * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    if (recompute) HIPCHK(hipMemcpyAsync(recompute, g.b.rc_final + c0, nc * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+int Batch::remove(int id)
+{
+    auto it = solves.find(id);
+    if (it == solves.end()) return fail(ADERDG_EINVAL, "aderdg_batch_remove: unknown id %d", id);
+    int r = groups[it->second.group]->remove(it->second.slot);
+    if (r) return fail(r, "aderdg_batch_remove: failed");
+    HIPCHK(hipEventRecord(removed, stream));
+    solves.erase(it);
+    return 0;
+}
+
+}  // namespace aderdg
+
+struct aderdg_batch {
+    Batch eng;
+};
+
+extern "C" {
+
+int aderdg_batch_create(const aderdg_batch_desc *d, aderdg_batch **out)
+{
+    if (!d || !out) return fail(ADERDG_EINVAL, "aderdg_batch_create: null argument");
+    *out = nullptr;
+    auto *b = new aderdg_batch();
+    int r = b->eng.init(d->device, d->persist_blocks);
+    if (r) {
+        b->eng.release();
+        delete b;
+        return r;
+    }
+    *out = b;
+    return 0;
+}
+
+int aderdg_batch_destroy(aderdg_batch *b)
+{
+    if (!b) return 0;
+    b->eng.release();
+    delete b;
+    return 0;
+}
+
+int aderdg_batch_reserve(aderdg_batch *b, int pde, int N, int64_t max_cells, int max_solves)
+{
+    if (!b) return fail(ADERDG_EINVAL, "aderdg_batch_reserve: null batch");
+    return b->eng.reserve(pde, N, max_cells, max_solves);
+}
+
+int64_t aderdg_solve_footprint(int pde, int N, int imax, int jmax)
+{
+    if ((pde != ADERDG_PDE_EULER && pde != ADERDG_PDE_BN) || N < 1 || N > kMaxDegree || imax < 1 || jmax < 1) return -1;
+    return Batch::footprint(pde, N, imax, jmax);
+}
+
+int64_t aderdg_batch_reserved_bytes(aderdg_batch *b) { return b ? b->eng.reserved : -1; }
+
+int aderdg_batch_add(aderdg_batch *b, const aderdg_desc *solve, const double *uh, double time, double tend,
+                     int max_steps, int stop_at_tend, int *id)
+{
+    if (!b) return fail(ADERDG_EINVAL, "aderdg_batch_add: null batch");
+    return b->eng.add(solve, uh, time, tend, max_steps, stop_at_tend, id);
+}
+
+int aderdg_batch_step(aderdg_batch *b, int nsteps)
+{
+    if (!b) return fail(ADERDG_EINVAL, "aderdg_batch_step: null batch");
+    return b->eng.step(nsteps);
+}
+
+int aderdg_batch_sync(aderdg_batch *b)
+{
+    if (!b) return fail(ADERDG_EINVAL, "aderdg_batch_sync: null batch");
+    HIPCHK(hipStreamSynchronize(b->eng.copy));
+    HIPCHK(hipStreamSynchronize(b->eng.stream));
+    return 0;
+}
+
+int aderdg_batch_query(aderdg_batch *b, int id, double *time, double *dt, int *steps, int *active)
+{
+    if (!b) return fail(ADERDG_EINVAL, "aderdg_batch_query: null batch");
+    if (!b->eng.state_of(id)) return fail(ADERDG_EINVAL, "aderdg_batch_query: unknown id %d", id);
+    int r = b->eng.snapshot();
+    if (r) return r;
+    const SolveState *s = b->eng.state_of(id);
+    if (time) *time = s->time;
+    if (dt) *dt = s->dt;
+    if (steps) *steps = s->steps;
+    if (active) *active = s->active;
+    return 0;
+}
+
+int aderdg_batch_finished(aderdg_batch *b, int *ids, int max_ids, int *count)
+{
+    if (!b || !count || (max_ids > 0 && !ids)) return fail(ADERDG_EINVAL, "aderdg_batch_finished: bad argument");
+    int r = b->eng.snapshot();
+    if (r) return r;
+    int c = 0;
+    for (const auto &kv : b->eng.solves) {
+        const SolveState *s = b->eng.state_of(kv.first);
+        if (!s->active && c < max_ids) ids[c++] = kv.first;
+    }
+    *count = c;
+    return 0;
+}
+
+int aderdg_batch_download(aderdg_batch *b, int id, double *uh)
+{
+    if (!b || !uh) return fail(ADERDG_EINVAL, "aderdg_batch_download: null argument");
+    return b->eng.download(id, uh);
+}
+
+int aderdg_batch_limiter(aderdg_batch *b, int id, int32_t *status, int32_t *recompute)
+{
+    if (!b) return fail(ADERDG_EINVAL, "aderdg_batch_limiter: null batch");
+    return b->eng.limiter(id, status, recompute);
+}
+
+int aderdg_batch_remove(aderdg_batch *b, int id)
+{
+    if (!b) return fail(ADERDG_EINVAL, "aderdg_batch_remove: null batch");
+    return b->eng.remove(id);
+}
+
+int aderdg_batch_usage(aderdg_batch *b, int *solves, int64_t *cells)
+{
+    if (!b) return fail(ADERDG_EINVAL, "aderdg_batch_usage: null batch");
+    int64_t c = 0;
+    for (auto &g : b->eng.groups) c += g->cells_in_use();
+    if (solves) *solves = (int)b->eng.solves.size();
+    if (cells) *cells = c;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,53 @@
+// batch.hpp -- the batched engine behind aderdg_batch_* (batch.hip), also
+// driven directly by the serving loop (server.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "../../include/aderdg.h"
+#include "group.hpp"
+
+namespace aderdg {
+
+class Batch {
+public:
+    struct Handle {
+        int group, slot;
+    };
+    int device = 0, persist = 0;
+    hipStream_t stream = nullptr;  // kernels, state traffic
+    hipStream_t copy = nullptr;    // solution uploads
+    std::vector<std::unique_ptr<Group>> groups;
+    std::vector<SolveState *> pinned_states;  // per group, caps.slots entries
+    std::map<int, Handle> solves;
+    int next_id = 0;
+    int64_t reserved = 0;
+    hipGraphExec_t graph = nullptr;
+    bool graph_stale = true;
+    hipEvent_t upload_done = nullptr, removed = nullptr, states_ready = nullptr;
+
+    int init(int device, int persist);
+    void release();
+    int reserve(int pde, int N, int64_t max_cells, int max_solves);
+    int add(const aderdg_desc *d, const double *uh, double time, double tend, int max_steps, int stop_at_tend, int *id);
+    int step(int nsteps);
+    // Solve states into pinned memory: asynchronous (ready when snapshot_ready()) or synchronous.
+    int snapshot_async();
+    bool snapshot_ready() const;
+    int snapshot();
+    const SolveState *state_of(int id) const;  // from the last snapshot
+    int download(int id, double *uh);
+    int limiter(int id, int32_t *status, int32_t *recompute);
+    int remove(int id);
+    int group_of(int pde, int N) const;
+    static int64_t footprint(int pde, int N, int imax, int jmax);
+
+private:
+    int build_graph();
+};
+
+}  // namespace aderdg

@@ -8,6 +8,67 @@
 
 namespace aderdg {
 
+int Staging::init(size_t bytes)
+{
+    release();
+    if (hipHostMalloc((void **)&buf_, bytes, hipHostMallocDefault) != hipSuccess) return ADERDG_EHIP;
+    cap_ = bytes;
+    pos_ = 0;
+    return 0;
+}
+
+void Staging::release()
+{
+    for (auto &p : pending_) {
+        (void)hipEventSynchronize(p.ev);
+        (void)hipEventDestroy(p.ev);
+    }
+    pending_.clear();
+    for (auto e : free_) (void)hipEventDestroy(e);
+    free_.clear();
+    if (buf_) (void)hipHostFree(buf_);
+    buf_ = nullptr;
+    cap_ = pos_ = 0;
+}
+
+hipError_t Staging::put(void *dst, const void *src, size_t n, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    if (n > cap_) return hipErrorInvalidValue;
+    const size_t len = (n + 255) & ~(size_t)255;
+    if (pos_ + len > cap_) pos_ = 0;
+    const size_t start = pos_, end = pos_ + len;
+    // wait for the chunks that overlap [start, end); retire every completed chunk
+    std::vector<Pending> keep;
+    for (const Pending &p : pending_) {
+        bool done = hipEventQuery(p.ev) == hipSuccess;
+        if (!done && p.start < end && start < p.end) {
+            hipError_t e = hipEventSynchronize(p.ev);
+            if (e != hipSuccess) return e;
+            done = true;
+        }
+        if (done)
+            free_.push_back(p.ev);
+        else
+            keep.push_back(p);
+    }
+    pending_.swap(keep);
+    std::memcpy(buf_ + start, src, n);
+    hipError_t e = hipMemcpyAsync(dst, buf_ + start, n, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return e;
+    hipEvent_t ev = nullptr;
+    if (!free_.empty()) {
+        ev = free_.back();
+        free_.pop_back();
+    } else if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) {
+        return e;
+    }
+    if ((e = hipEventRecord(ev, st)) != hipSuccess) return e;
+    pending_.push_back({start, end, ev});
+    pos_ = end;
+    return hipSuccess;
+}
+
 void RangeAlloc::reset(int64_t cap)
 {
     cap_ = cap;
@@ -147,9 +208,8 @@ int Group::init(int pde_, int N_, const GroupCaps &c, const Operators &ops_, int
     b.solves = sad;
     b.meta = meta;
     b.cell_solve = cs;
-    if (hipHostMalloc((void **)&prefix_host_, sizeof(int) * kNumTabs * (c.slots + 1), hipHostMallocDefault) !=
-        hipSuccess)
-        return ADERDG_EHIP;
+    prefix_host_.assign((size_t)kNumTabs * (c.slots + 1), 0);
+    if ((r = staging_.init(std::max<size_t>(1 << 20, 8 * sizeof(int) * kNumTabs * (c.slots + 1)))) != 0) return r;
     solves_.assign(c.slots, SolveArgs{});
     used_.assign(c.slots, 0);
     cell_alloc_.reset(c.cells);
@@ -171,10 +231,10 @@ int Group::init(int pde_, int N_, const GroupCaps &c, const Operators &ops_, int
 void Group::release()
 {
     if (st_) (void)hipStreamSynchronize(st_);
+    staging_.release();
     for (void *p : allocs_) (void)hipFree(p);
     allocs_.clear();
-    if (prefix_host_) (void)hipHostFree(prefix_host_);
-    prefix_host_ = nullptr;
+    prefix_host_.clear();
     prefix_dev_ = nullptr;
     b = StepBufs{};
 }
@@ -182,7 +242,7 @@ void Group::release()
 int Group::rebuild_tables()
 {
     const int ns = nslot_;
-    int *p = prefix_host_;
+    int *p = prefix_host_.data();
     for (int k = 0; k < kNumTabs; k++) {
         int *pk = p + k * (ns + 1);
         pk[0] = 0;
@@ -199,11 +259,8 @@ int Group::rebuild_tables()
     }
     int64_t mx = 0;
     for (int k = 0; k < kNumTabs; k++) mx = std::max<int64_t>(mx, p[k * (ns + 1) + ns]);
-    if (hipMemcpyAsync(prefix_dev_, p, sizeof(int) * kNumTabs * (ns + 1), hipMemcpyHostToDevice, st_) != hipSuccess)
-        return ADERDG_EHIP;
+    if (staging_.put(prefix_dev_, p, sizeof(int) * kNumTabs * (ns + 1), st_) != hipSuccess) return ADERDG_EHIP;
     if (launch_tables(b, prefix_dev_, ns, quanta, (int)std::max<int64_t>(mx, 1), st_) != hipSuccess) return ADERDG_EHIP;
-    // the pinned prefix buffer is rewritten by the next rebuild: wait for this copy
-    if (hipStreamSynchronize(st_) != hipSuccess) return ADERDG_EHIP;
     return 0;
 }
 
@@ -239,8 +296,8 @@ int Group::add(const MeshArgs &m, const SolveState &st0, int *slot)
     cells_in_use_ += m.ncell;
     SolveArgs *sad = const_cast<SolveArgs *>(b.solves);
     int *cs = const_cast<int *>(b.cell_solve);
-    if (hipMemcpyAsync(sad + sl, &sa, sizeof(SolveArgs), hipMemcpyHostToDevice, st_) != hipSuccess ||
-        hipMemcpyAsync(b.state + sl, &st0, sizeof(SolveState), hipMemcpyHostToDevice, st_) != hipSuccess ||
+    if (staging_.put(sad + sl, &sa, sizeof(SolveArgs), st_) != hipSuccess ||
+        staging_.put(b.state + sl, &st0, sizeof(SolveState), st_) != hipSuccess ||
         hipMemsetD32Async((hipDeviceptr_t)(cs + c0), sl, (size_t)m.ncell, st_) != hipSuccess ||
         hipMemsetAsync(b.rc + c0, 0, sizeof(int) * m.ncell, st_) != hipSuccess ||
         hipMemsetAsync(b.rc_final + c0, 0, sizeof(int) * m.ncell, st_) != hipSuccess ||
@@ -258,8 +315,7 @@ int Group::remove(int sl)
     if (!used(sl)) return ADERDG_EINVAL;
     const MeshArgs &m = solves_[sl].m;
     SolveState off{};  // inactive, no seed pending
-    if (hipMemcpyAsync(b.state + sl, &off, sizeof(SolveState), hipMemcpyHostToDevice, st_) != hipSuccess)
-        return ADERDG_EHIP;
+    if (staging_.put(b.state + sl, &off, sizeof(SolveState), st_) != hipSuccess) return ADERDG_EHIP;
     cell_alloc_.release(solves_[sl].cell0, m.ncell);
     fx_alloc_.release(solves_[sl].fx0, (int64_t)(m.imax + 1) * m.jmax);
     fy_alloc_.release(solves_[sl].fy0, (int64_t)m.imax * (m.jmax + 1));
@@ -269,24 +325,24 @@ int Group::remove(int sl)
     return rebuild_tables();
 }
 
-int Group::upload_uh(int sl, const double *uh, int parity)
+int Group::upload_uh(int sl, const double *uh, int parity, hipStream_t cs)
 {
     if (!used(sl)) return ADERDG_EINVAL;
     const size_t blk = (size_t)V * n * n;
     const size_t c0 = solves_[sl].cell0, nc = solves_[sl].m.ncell;
-    if (hipMemcpyAsync(b.uh[parity] + c0 * blk, uh, nc * blk * sizeof(double), hipMemcpyHostToDevice, st_) !=
-        hipSuccess)
+    if (hipMemcpyAsync(b.uh[parity] + c0 * blk, uh, nc * blk * sizeof(double), hipMemcpyHostToDevice,
+                       cs ? cs : st_) != hipSuccess)
         return ADERDG_EHIP;
     return 0;
 }
 
-int Group::download_uh(int sl, double *uh, int parity)
+int Group::download_uh(int sl, double *uh, int parity, hipStream_t cs)
 {
     if (!used(sl)) return ADERDG_EINVAL;
     const size_t blk = (size_t)V * n * n;
     const size_t c0 = solves_[sl].cell0, nc = solves_[sl].m.ncell;
-    if (hipMemcpyAsync(uh, b.uh[parity] + c0 * blk, nc * blk * sizeof(double), hipMemcpyDeviceToHost, st_) !=
-        hipSuccess)
+    if (hipMemcpyAsync(uh, b.uh[parity] + c0 * blk, nc * blk * sizeof(double), hipMemcpyDeviceToHost,
+                       cs ? cs : st_) != hipSuccess)
         return ADERDG_EHIP;
     return 0;
 }
@@ -294,7 +350,14 @@ int Group::download_uh(int sl, double *uh, int parity)
 int Group::write_state(int sl, const SolveState &st)
 {
     if (!used(sl)) return ADERDG_EINVAL;
-    if (hipMemcpyAsync(b.state + sl, &st, sizeof(SolveState), hipMemcpyHostToDevice, st_) != hipSuccess)
+    if (staging_.put(b.state + sl, &st, sizeof(SolveState), st_) != hipSuccess) return ADERDG_EHIP;
+    return 0;
+}
+
+int Group::read_states_async(SolveState *pinned_out)
+{
+    if (nslot_ == 0) return 0;
+    if (hipMemcpyAsync(pinned_out, b.state, sizeof(SolveState) * nslot_, hipMemcpyDeviceToHost, st_) != hipSuccess)
         return ADERDG_EHIP;
     return 0;
 }

@@ -18,6 +18,27 @@
 
 namespace aderdg {
 
+// Pinned host ring for small host-to-device uploads on one stream: a chunk is
+// reused only after the event recorded behind its copy has completed, so an
+// upload never waits for the kernels queued before it.
+class Staging {
+public:
+    int init(size_t bytes);
+    void release();
+    // Copy n bytes of src to device dst on stream st through the ring.
+    hipError_t put(void *dst, const void *src, size_t n, hipStream_t st);
+
+private:
+    struct Pending {
+        size_t start, end;
+        hipEvent_t ev;
+    };
+    char *buf_ = nullptr;
+    size_t cap_ = 0, pos_ = 0;
+    std::vector<Pending> pending_;  // oldest first
+    std::vector<hipEvent_t> free_;
+};
+
 // First-fit allocator of index ranges in [0, cap), coalescing on release.
 class RangeAlloc {
 public:
@@ -63,10 +84,13 @@ public:
     int64_t cells_in_use() const { return cells_in_use_; }
 
     // Host <-> device for one solve; uh in the reference layout uh(V,n,n,IMAX,JMAX).
-    int upload_uh(int slot, const double *uh, int parity);
-    int download_uh(int slot, double *uh, int parity);
+    // Solution copies on stream cs (default: the group's stream).
+    int upload_uh(int slot, const double *uh, int parity, hipStream_t cs = nullptr);
+    int download_uh(int slot, double *uh, int parity, hipStream_t cs = nullptr);
     int write_state(int slot, const SolveState &st);
-    int read_states(SolveState *out);  // all nslot() states (synchronous)
+    int read_states(SolveState *out);                  // all nslot() states (synchronous)
+    int read_states_async(SolveState *pinned_out);     // enqueued on the group's stream
+    hipStream_t stream() const { return st_; }
 
     hipError_t launch(int part) const;
     // Bytes of device memory per cell / x-face / y-face and per slot of a (PDE, N).
@@ -82,7 +106,8 @@ private:
     int nslot_ = 0;
     int64_t cells_in_use_ = 0;
     int *prefix_dev_ = nullptr;
-    int *prefix_host_ = nullptr;  // pinned
+    std::vector<int> prefix_host_;
+    Staging staging_;
     int rebuild_tables();
     template <class T>
     int dalloc(T **p, size_t count);

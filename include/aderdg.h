@@ -180,6 +180,60 @@ int aderdg_timing_get(aderdg_ctx *ctx, double ms[ADERDG_TIMING_SLOTS], int64_t l
 double aderdg_flops_predictor(int pde, int N);
 double aderdg_flops_update(int pde, int N);
 
+/* ------------------------------------------------------------------------
+ * Batched engine.  No counterpart in the reference, which runs one solve per
+ * process (Euler/main.f90:1-107); each solve of a batch follows the same
+ * per-step routines with its own mesh, degree, boundaries, time and dt
+ * (the state of Euler/ader_dg.f90:30-101, one copy per solve).
+ *
+ * Solves of one (pde, N) share an arena (a group): flat device arrays sized
+ * at aderdg_batch_reserve, in which a solve takes contiguous ranges of cells
+ * and faces.  One captured graph steps every active solve of every group;
+ * the kernels loop over device-resident work tables, so adding or removing
+ * solves never changes the graph.  A solve stops after max_steps steps or,
+ * with stop_at_tend, once its time reaches tend; it keeps its state until
+ * it is removed.  All calls come from one host thread.
+ * ------------------------------------------------------------------------ */
+typedef struct aderdg_batch aderdg_batch;
+
+typedef struct {
+    int device;          /* HIP device ordinal                                   */
+    int persist_blocks;  /* workgroups of each table-driven kernel per group;
+                            0 = eight per compute unit                           */
+} aderdg_batch_desc;
+
+int aderdg_batch_create(const aderdg_batch_desc *desc, aderdg_batch **out);
+int aderdg_batch_destroy(aderdg_batch *b);
+/* Arena for up to max_cells cells in up to max_solves solves of one (pde, N).
+ * Face capacity is sized for solves of at least 8 cells per side. */
+int aderdg_batch_reserve(aderdg_batch *b, int pde, int N, int64_t max_cells, int max_solves);
+/* Device bytes a solve of this shape occupies in an arena (admission by HBM footprint). */
+int64_t aderdg_solve_footprint(int pde, int N, int imax, int jmax);
+/* Device bytes reserved by all arenas of the batch. */
+int64_t aderdg_batch_reserved_bytes(aderdg_batch *b);
+/* Add a solve: descriptor (ops must be NULL: built-in operators), initial
+ * uh(nVar,N+1,N+1,IMAX,JMAX), start time, end time, step limit (> 0) and
+ * whether reaching tend ends it.  *id receives a handle.  ADERDG_EFULL: no
+ * free slot or range in the (pde, N) arena (nothing was added). */
+int aderdg_batch_add(aderdg_batch *b, const aderdg_desc *solve, const double *uh, double time, double tend,
+                     int max_steps, int stop_at_tend, int *id);
+/* Enqueue nsteps replays of the step graph (asynchronous). */
+int aderdg_batch_step(aderdg_batch *b, int nsteps);
+int aderdg_batch_sync(aderdg_batch *b);
+/* State of one solve (synchronous): time, last dt, steps taken, still active. */
+int aderdg_batch_query(aderdg_batch *b, int id, double *time, double *dt, int *steps, int *active);
+/* Handles of solves that have stopped and are not removed yet (synchronous);
+ * *count receives how many were written (at most max_ids). */
+int aderdg_batch_finished(aderdg_batch *b, int *ids, int max_ids, int *count);
+/* Current solution of a solve in the reference layout (synchronous). */
+int aderdg_batch_download(aderdg_batch *b, int id, double *uh);
+/* Limiter status and recompute flags of a solve's last step (synchronous). */
+int aderdg_batch_limiter(aderdg_batch *b, int id, int32_t *status, int32_t *recompute);
+/* Free a solve's slot and ranges. */
+int aderdg_batch_remove(aderdg_batch *b, int id);
+/* Solves placed and cells in use over all arenas. */
+int aderdg_batch_usage(aderdg_batch *b, int *solves, int64_t *cells);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,3 +79,19 @@ def test_cpp_initial_condition_matches_python():
                 np.testing.assert_allclose(a, b, rtol=2e-15, atol=0)
             else:
                 np.testing.assert_array_equal(a, b)
+
+
+def test_solve_footprint_host():
+    """Admission footprint (host only): grows with the mesh, the degree and nVar,
+    and covers the arena arrays of a cell (uh x2, lduh, face extrapolations, Lh x2)."""
+    f = aderdg.solve_footprint
+    assert f("euler", 4, 512, 512) > f("euler", 4, 256, 256) > 0
+    assert f("euler", 4, 64, 64) > f("euler", 3, 64, 64)
+    assert f("bn", 3, 64, 64) > f("euler", 3, 64, 64)
+    assert f("euler", 5, 8, 8) == -1 and f("euler", 2, 0, 8) == -1
+    for pde, V in (("euler", 4), ("bn", 9)):
+        for N in (1, 2, 3, 4):
+            n, s = N + 1, 2 * N + 1
+            floor = 8 * (3 * V * n * n + 8 * V * n + 2 * V * s * s)
+            per_cell = f(pde, N, 200, 200) / 200.0 ** 2
+            assert floor < per_cell < 1.25 * floor + 512, (pde, N, per_cell, floor)
