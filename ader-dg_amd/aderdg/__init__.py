@@ -121,6 +121,8 @@ def load_library(exact=False):
         "aderdg_batch_limiter": [vp, ctypes.c_int, _ip, _ip],
         "aderdg_batch_remove": [vp, ctypes.c_int],
         "aderdg_batch_usage": [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(i64)],
+        "aderdg_batch_timing_enable": [vp, ctypes.c_int],
+        "aderdg_batch_timing_get": [vp, ctypes.c_int, ctypes.c_int, _dp, ctypes.POINTER(i64)],
     }
     for name, args in bsig.items():
         f = getattr(lib, name)
@@ -408,3 +410,14 @@ class Batch:
         n, c = ctypes.c_int(), ctypes.c_int64()
         self._check(self.lib.aderdg_batch_usage(self.h, ctypes.byref(n), ctypes.byref(c)))
         return n.value, c.value
+
+    def timing(self, on):
+        self._check(self.lib.aderdg_batch_timing_enable(self.h, 1 if on else 0))
+
+    def timing_get(self, pde, N):
+        """(ms, launches) of the (pde, N) arena per kernel, in the order of TIMING_SLOTS."""
+        k = PDE[pde] if isinstance(pde, str) else int(pde)
+        ms = (ctypes.c_double * len(TIMING_SLOTS))()
+        ln = (ctypes.c_int64 * len(TIMING_SLOTS))()
+        self._check(self.lib.aderdg_batch_timing_get(self.h, k, N, ms, ln))
+        return list(ms), list(ln)

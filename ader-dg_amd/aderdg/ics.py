@@ -151,3 +151,24 @@ def initial_uh_bn(xi, imax, jmax, xl, xr, eos, **kw):
     """uh(9, N+1, N+1, IMAX, JMAX) of the circular interface problem, Fortran order."""
     X, Y = node_coordinates(xi, imax, jmax, xl, xr)
     return np.asfortranarray(bn_circle(np.asarray(X), np.asarray(Y), eos, **kw))
+
+
+# ---- ensembles ------------------------------------------------------------
+
+BN_EOS_ENSEMBLE = (4.4, 1.4, 6.0e8, 0.0)  # GAMMA_S, GAMMA_G, PI_S, PI_G: stiffened-gas solid, ideal gas
+
+
+def bn_ensemble(count=256, seed=2026, side=(64, 384), degrees=(2, 3)):
+    """Seeded ensemble of Baer-Nunziato circle problems on [0,1]^2, transmissive:
+    mesh side log-uniform in `side`, degree uniform over `degrees`, radius
+    U(0.1, 0.3), pressure ratio log-uniform in [1e3, 1e4].  Returns specs
+    (dicts of N, imax, jmax, radius, pratio); initial_uh_bn builds each state."""
+    rng = np.random.default_rng(seed)
+    lo, hi = side
+    out = []
+    for _ in range(count):
+        s = int(round(lo * (hi / lo) ** rng.uniform()))
+        N = int(rng.choice(degrees))
+        out.append(dict(N=N, imax=s, jmax=s, radius=float(rng.uniform(0.1, 0.3)),
+                        pratio=float(10.0 ** rng.uniform(3.0, 4.0))))
+    return out

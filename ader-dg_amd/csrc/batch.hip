@@ -41,6 +41,13 @@ void Batch::release()
     graph = nullptr;
     for (auto &g : groups) g->release();
     groups.clear();
+    for (auto &t : pending) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    pending.clear();
+    for (auto e : free_events) (void)hipEventDestroy(e);
+    free_events.clear();
     for (auto p : pinned_states)
         if (p) (void)hipHostFree(p);
     pinned_states.clear();
@@ -96,6 +103,8 @@ int Batch::reserve(int pde, int N, int64_t max_cells, int max_solves)
     reserved += pc * max_cells + (px + py) * faces + psl * max_solves;
     groups.push_back(std::move(g));
     pinned_states.push_back(ps);
+    ms.emplace_back(ADERDG_TIMING_SLOTS, 0.0);
+    launches.emplace_back(ADERDG_TIMING_SLOTS, 0);
     graph_stale = true;
     return 0;
 }
@@ -164,10 +173,57 @@ int Batch::build_graph()
     return 0;
 }
 
+int Batch::collect_timing()
+{
+    HIPCHK(hipStreamSynchronize(stream));
+    for (auto &t : pending) {
+        float m = 0.f;
+        if (hipEventElapsedTime(&m, t.a, t.b) == hipSuccess) {
+            ms[t.group][t.slot] += m;
+            launches[t.group][t.slot] += 1;
+        }
+        free_events.push_back(t.a);
+        free_events.push_back(t.b);
+    }
+    pending.clear();
+    return 0;
+}
+
 int Batch::step(int nsteps)
 {
     if (nsteps < 0) return fail(ADERDG_EINVAL, "aderdg_batch_step: negative step count");
     if (groups.empty()) return fail(ADERDG_ESTATE, "aderdg_batch_step: nothing reserved");
+    if (timing) {  // the graph's launches one by one, each between two events
+        // timing slots of include/aderdg.h: dt, predictor, update, spread, subcell, face; -1 untimed
+        static const int parts[] = {kPartDt, kPartPredictor, kPartFace, kPartUpdate, kPartSpread, kPartSubcell,
+                                    kPartAdvance};
+        static const int slots[] = {0, 1, 5, 2, 3, 4, -1};
+        for (int k = 0; k < nsteps; k++)
+            for (size_t gi = 0; gi < groups.size(); gi++)
+                for (int p = 0; p < 7; p++) {
+                    hipEvent_t e0 = nullptr, e1 = nullptr;
+                    for (hipEvent_t *e : {&e0, &e1}) {
+                        if (!free_events.empty()) {
+                            *e = free_events.back();
+                            free_events.pop_back();
+                        } else {
+                            HIPCHK(hipEventCreate(e));
+                        }
+                    }
+                    HIPCHK(hipEventRecord(e0, stream));
+                    hipError_t e = parts[p] == kPartDt ? launch_dt(groups[gi]->b, stream, 0.0, 0.0, 0)
+                                                       : groups[gi]->launch(parts[p]);
+                    if (e != hipSuccess) return fail(ADERDG_EHIP, "aderdg_batch_step: launch: %s", hipGetErrorString(e));
+                    HIPCHK(hipEventRecord(e1, stream));
+                    if (slots[p] >= 0)
+                        pending.push_back({(int)gi, slots[p], e0, e1});
+                    else {
+                        free_events.push_back(e0);
+                        free_events.push_back(e1);
+                    }
+                }
+        return 0;
+    }
     if (graph_stale) {
         int r = build_graph();
         if (r) return r;
@@ -351,6 +407,32 @@ int aderdg_batch_remove(aderdg_batch *b, int id)
 {
     if (!b) return fail(ADERDG_EINVAL, "aderdg_batch_remove: null batch");
     return b->eng.remove(id);
+}
+
+int aderdg_batch_timing_enable(aderdg_batch *b, int on)
+{
+    if (!b) return fail(ADERDG_EINVAL, "aderdg_batch_timing_enable: null batch");
+    int r = b->eng.collect_timing();
+    if (r) return r;
+    b->eng.timing = on != 0;
+    for (auto &v : b->eng.ms) std::fill(v.begin(), v.end(), 0.0);
+    for (auto &v : b->eng.launches) std::fill(v.begin(), v.end(), 0);
+    return 0;
+}
+
+int aderdg_batch_timing_get(aderdg_batch *b, int pde, int N, double ms[ADERDG_TIMING_SLOTS],
+                            int64_t launches[ADERDG_TIMING_SLOTS])
+{
+    if (!b) return fail(ADERDG_EINVAL, "aderdg_batch_timing_get: null batch");
+    const int gi = b->eng.group_of(pde, N);
+    if (gi < 0) return fail(ADERDG_EINVAL, "aderdg_batch_timing_get: no arena (pde %d, N %d)", pde, N);
+    int r = b->eng.collect_timing();
+    if (r) return r;
+    for (int k = 0; k < ADERDG_TIMING_SLOTS; k++) {
+        if (ms) ms[k] = b->eng.ms[gi][k];
+        if (launches) launches[k] = b->eng.launches[gi][k];
+    }
+    return 0;
 }
 
 int aderdg_batch_usage(aderdg_batch *b, int *solves, int64_t *cells)

@@ -29,6 +29,17 @@ public:
     hipGraphExec_t graph = nullptr;
     bool graph_stale = true;
     hipEvent_t upload_done = nullptr, removed = nullptr, states_ready = nullptr;
+    // per-kernel timing (steps launched part by part with events instead of the graph)
+    struct Timed {
+        int group, slot;
+        hipEvent_t a, b;
+    };
+    bool timing = false;
+    std::vector<Timed> pending;
+    std::vector<hipEvent_t> free_events;
+    std::vector<std::vector<double>> ms;        // [group][timing slot]
+    std::vector<std::vector<int64_t>> launches;
+    int collect_timing();
 
     int init(int device, int persist);
     void release();

@@ -1,20 +1,32 @@
 """Benchmark of the ADER-DG element step on one MI355X.
 
-Workload (default): 2-D Euler isentropic vortex, periodic [0,10]^2, 512x512
-cells, degree N=4, gamma=1.4, vortex strength 5 at (5,5) on (rho,u,v,p) =
-(1,1,1,1), FP64, limiter enabled (untriggered), 200 steps.
+Default workload `bn_ensemble_256`: 256 independent Baer-Nunziato two-phase
+solves (9 variables, path-conservative fluxes, subcell limiter on) of a seeded
+ensemble: mesh side log-uniform 64..384 on [0,1]^2, degree N in {2,3},
+transmissive boundaries, stiffened-gas solid (gamma 4.4, pi 6e8) / ideal gas
+(gamma 1.4), circular interface of radius U(0.1,0.3), pressure ratio
+log-uniform 1e3..1e4, volume fraction 1e-6 / 1-1e-6; FP64.  The same K steps
+of every solve run two ways:
+  batched  all 256 solves in one batch (one arena per N, one step graph per
+           step for all solves); `value`
+  plain    one single-instance context per solve, one after the other
+           (graph replay per step); `plain_value`
 
-A step is one pass of the reference time loop body (Euler/main.f90:24-100):
-time step, predictor + volume integral, Riemann + surface + update + DMP,
-limiter post-processing.  `value` = element-DOF updates per second over the
-timed steps, a DOF update being one coefficient (v,i,j) of one cell advanced
-one step (nVar*(N+1)^2 per cell-step).  Inputs are resident in HBM before
-the timed region starts.
+A step is one pass of the reference time loop body (Euler/main.f90:24-100,
+Baer-Nunziato/main.f90): time step, predictor + volume integral, Riemann +
+surface + update + DMP, limiter.  `value` = element-DOF updates per second of
+the timed steps, a DOF update being one coefficient (v,i,j) of one cell
+advanced one step (nVar*(N+1)^2 per cell-step).  Inputs are resident in HBM
+before the timed region starts; uploads are not timed.
+
+Workloads `euler_vortex_512_n4` and `euler_quadrant_1024_n3` time one plain
+solve.
 
 Prints ONE JSON line.  Per-kernel details go to stderr.
 """
 import argparse
 import json
+import multiprocessing as mp
 import os
 import sys
 import time
@@ -26,51 +38,211 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix) dense peak
 HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per k_predictor launch, from rocprofv3 --pmc FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0),
-# scaled by a 256 MiB device copy; profiles/hbm_traffic_euler_vortex_512_n4.md holds the run.
+# scaled by a 256 MiB device copy; profiles/hbm_traffic_*.md holds the runs.
 PREDICTOR_TRAFFIC_BYTES = {"euler_vortex_512_n4": 965.7e6}
 
-WORKLOADS = {
+PLAIN = {
     # name: (ic, N, imax, jmax, xl, xr, bc)
     "euler_vortex_512_n4": ("vortex", 4, 512, 512, (0.0, 0.0), (10.0, 10.0), "periodic"),
     "euler_quadrant_1024_n3": ("quadrant", 3, 1024, 1024, (0.0, 0.0), (1.0, 1.0), "transmissive"),
 }
+ENSEMBLES = {"bn_ensemble_256": dict(count=256, seed=2026)}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(wl, target_seconds):
-    """Reference Fortran routines (oracle/_ref, flang -O3, one core) on a bounded sample of the
-    same workload: the same IC and boundaries on a 64x64 mesh, steps until target_seconds."""
+# ---------------------------------------------------------------------------
+# CPU baselines: the reference Fortran routines built from its sources
+# (oracle/_ref, flang -O3), or the oracle's C restatement where they are absent
+# ---------------------------------------------------------------------------
+def cpu_plain(wl, target_seconds):
+    """One process on a 64x64 sample of a plain workload, steps until target_seconds."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import aderdg
     from aderdg import ics
     import pyoracle
-    ic, N, I, J, xl, xr, bc = WORKLOADS[wl]
+    ic, N, I, J, xl, xr, bc = PLAIN[wl]
     si = sj = 64
-    xi = aderdg.operators(N)["xi"]
-    uh = ics.initial_uh(ic, xi, si, sj, xl, xr)
+    uh = ics.initial_uh(ic, aderdg.operators(N)["xi"], si, sj, xl, xr)
     mode = {"periodic": pyoracle.BC_PERIODIC, "transmissive": pyoracle.BC_TRANSMISSIVE}[bc]
     if pyoracle.RefEuler.available(N):
         solver, kind = pyoracle.RefEuler(N, si, sj, xl, xr, mode), "reference"
-        solver.set_uh(uh)
-        solver.set_scalars(0.0, 0.0, 1e9)
     else:
         solver, kind = pyoracle.Oracle(N, si, sj, xl, xr, mode), "port"
-        solver.set_uh(uh)
-        solver.set_scalars(0.0, 0.0, 1e9)
-    steps = 0
-    t0 = time.perf_counter()
-    while True:
+    solver.set_uh(uh)
+    solver.set_scalars(0.0, 0.0, 1e9)
+    steps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < target_seconds:
         solver.step()
         steps += 1
-        el = time.perf_counter() - t0
-        if el >= target_seconds:
-            break
+    el = time.perf_counter() - t0
     dof = steps * si * sj * 4 * (N + 1) ** 2
     return {"value": dof / el, "unit": "element-DOF updates/s", "cores": 1, "kind": kind,
             "sample": "%s IC, %dx%d cells, N=%d, %s, %d steps in %.1f s, one process" % (ic, si, sj, N, bc, steps, el)}
+
+
+def _cpu_bn_worker(args):
+    """One reference Baer-Nunziato solve of the ensemble distribution on a 64x64 mesh."""
+    spec, target_seconds = args
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import aderdg
+    from aderdg import ics
+    import pyoracle
+    N = spec["N"]
+    eos = ics.BN_EOS_ENSEMBLE
+    uh = ics.initial_uh_bn(aderdg.operators(N)["xi"], 64, 64, (0.0, 0.0), (1.0, 1.0), eos,
+                           radius=spec["radius"], pratio=spec["pratio"])
+    if pyoracle.RefBN.available(N):
+        s, kind = pyoracle.RefBN(N, 64, 64, (0.0, 0.0), (1.0, 1.0), eos), "reference"
+    else:
+        s, kind = pyoracle.Oracle(N, 64, 64, (0.0, 0.0), (1.0, 1.0), pyoracle.BC_TRANSMISSIVE, pde=1, eos=eos), "port"
+    s.set_uh(uh)
+    s.set_scalars(0.0, 0.0, 1e9)
+    steps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < target_seconds:
+        s.step()
+        steps += 1
+    return steps * 64 * 64 * 9 * (N + 1) ** 2, time.perf_counter() - t0, kind
+
+
+def cpu_ensemble(specs, target_seconds, procs):
+    """One reference process per core (the reference is single-threaded), each on one
+    ensemble member's initial condition sampled on a 64x64 mesh; runs before any GPU use."""
+    sample = [specs[k % len(specs)] for k in range(procs)]
+    with mp.get_context("fork").Pool(procs) as pool:
+        res = pool.map(_cpu_bn_worker, [(s, target_seconds) for s in sample])
+    value = sum(d / e for d, e, _ in res)
+    kinds = {k for _, _, k in res}
+    return {"value": value, "unit": "element-DOF updates/s", "cores": procs,
+            "kind": "reference" if kinds == {"reference"} else "port",
+            "sample": "%d processes, one per core, each one ensemble member (N=%s) on 64x64 cells for %.0f s"
+                      % (procs, "/".join(str(s["N"]) for s in sample[:4]) + ("..." if procs > 4 else ""),
+                         target_seconds)}
+
+
+# ---------------------------------------------------------------------------
+def roofline_of(flop_per_cell, cells, ms_per_launch, kernel, traffic=None):
+    achieved = flop_per_cell * cells / (ms_per_launch * 1e-3) / 1e12
+    return {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic, "traffic_unit": "bytes/launch",
+            "kernel": kernel, "flop_per_cell": flop_per_cell, "cells_per_launch": cells,
+            "note": "FP64 vector ALU; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s"}
+
+
+def run_plain(args):
+    import aderdg
+    from aderdg import ics
+    ic, N, I, J, xl, xr, bc = PLAIN[args.workload]
+    ops = aderdg.operators(N)
+    uh0 = ics.initial_uh(ic, ops["xi"], I, J, xl, xr)
+    s = aderdg.Solver(N, I, J, xl, xr, bc)
+    s.upload(uh0)
+    t, dt, nrec = s.run(args.warmup, 0.0, 1e30) if args.warmup else (0.0, 0.0, 0)
+    s.timing(not args.graph)
+    t0 = time.perf_counter()
+    t1_sim, dt, nrec = s.run(args.steps, t, 1e30)
+    el = time.perf_counter() - t0
+    ms, launches = s.timing_get()
+    s.close()
+    ncell = I * J
+    value = args.steps * ncell * 4 * (N + 1) ** 2 / el
+    per = {name: (ms[k] / launches[k] if launches[k] else None) for k, name in enumerate(aderdg.TIMING_SLOTS)}
+    log("per-kernel average ms:", json.dumps(per))
+    log("simulated time %.6g after %d steps, last dt %.6g, nRecompute %d" % (t1_sim, args.steps, dt, nrec))
+    fl_pred, _ = aderdg.flops_per_cell(N)
+    roof = roofline_of(fl_pred, ncell, per["predictor"], "k_predictor<EulerPDE,%d>" % N,
+                       PREDICTOR_TRAFFIC_BYTES.get(args.workload)) if per["predictor"] else None
+    return {
+        "metric": "element-DOF updates/s (plain single-instance Euler step)",
+        "value": value, "unit": "element-DOF updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (analytic initial condition)",
+        "config": {"workload": args.workload, "pde": "euler", "ic": ic, "N": N, "cells": [I, J], "bc": bc,
+                   "cell_updates_per_s": args.steps * ncell / el},
+        "roofline": roof,
+    }
+
+
+def run_ensemble(args, specs):
+    import aderdg
+    from aderdg import ics
+    eos = ics.BN_EOS_ENSEMBLE
+    xis = {N: aderdg.operators(N)["xi"] for N in {s["N"] for s in specs}}
+
+    def uh_of(sp):
+        return ics.initial_uh_bn(xis[sp["N"]], sp["imax"], sp["jmax"], (0.0, 0.0), (1.0, 1.0), eos,
+                                 radius=sp["radius"], pratio=sp["pratio"])
+
+    K, W, T = args.steps, args.warmup, 2
+    dof_step = sum(9 * (s["N"] + 1) ** 2 * s["imax"] * s["jmax"] for s in specs)
+    cells = sum(s["imax"] * s["jmax"] for s in specs)
+
+    # plain: one context per solve, one after the other; only the K step replays are timed
+    plain_s = None
+    if not args.no_plain:
+        plain_s = 0.0
+        for k, sp in enumerate(specs):
+            s = aderdg.Solver(sp["N"], sp["imax"], sp["jmax"], (0.0, 0.0), (1.0, 1.0), "transmissive", pde="bn",
+                              eos=eos)
+            s.upload(uh_of(sp))
+            t, _, _ = s.run(W, 0.0, 1e30) if W else (0.0, 0.0, 0)
+            t0 = time.perf_counter()
+            s.run(K, t, 1e30)
+            plain_s += time.perf_counter() - t0
+            s.close()
+            if k % 64 == 63:
+                log("plain: %d solves done, %.2f s timed" % (k + 1, plain_s))
+
+    # batched: every solve in one batch; W warm-up, K timed graph replays, T evented steps
+    b = aderdg.Batch()
+    for N in sorted({s["N"] for s in specs}):
+        mine = [s for s in specs if s["N"] == N]
+        b.reserve("bn", N, max_cells=sum(s["imax"] * s["jmax"] for s in mine), max_solves=len(mine))
+    log("batch arenas: %.1f GB reserved" % (b.reserved_bytes() / 1e9))
+    ids = [b.add(uh_of(sp), sp["N"], sp["imax"], sp["jmax"], (0.0, 0.0), (1.0, 1.0), bc="transmissive", pde="bn",
+                 eos=eos, max_steps=W + K + T) for sp in specs]
+    b.sync()
+    if W:
+        b.step(W)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(K)
+    b.sync()
+    batched_s = time.perf_counter() - t0
+    b.timing(True)
+    b.step(T)
+    b.sync()
+    per_group = {}
+    for N in sorted({s["N"] for s in specs}):
+        ms, ln = b.timing_get("bn", N)
+        gcells = sum(s["imax"] * s["jmax"] for s in specs if s["N"] == N)
+        per_group[N] = dict(cells=gcells, **{name: (ms[k] / ln[k] if ln[k] else None)
+                                              for k, name in enumerate(aderdg.TIMING_SLOTS)})
+    done = b.finished()
+    b.close()
+    log("batched per-kernel average ms per arena:", json.dumps(per_group))
+    assert len(done) == len(ids), "every solve should have stopped after W+K+T steps"
+
+    value = dof_step * K / batched_s
+    dom = max(per_group, key=lambda N: per_group[N]["predictor"] or 0.0)
+    fl_pred, _ = aderdg.flops_per_cell(dom, "bn")
+    roof = roofline_of(fl_pred, per_group[dom]["cells"], per_group[dom]["predictor"],
+                       "k_predictor<BnPDE,%d>" % dom, PREDICTOR_TRAFFIC_BYTES.get("%s_n%d" % (args.workload, dom)))
+    out = {
+        "metric": "element-DOF updates/s batched vs plain; p50/p99 solve latency at fixed QPS",
+        "value": value, "unit": "element-DOF updates/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": batched_s / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded analytic initial conditions)",
+        "config": {"workload": args.workload, "pde": "baer-nunziato", "solves": len(specs), "cells": cells,
+                   "N": sorted({s["N"] for s in specs}), "side": [64, 384], "bc": "transmissive",
+                   "seed": ENSEMBLES[args.workload]["seed"], "dof_per_step": dof_step},
+        "plain_value": dof_step * K / plain_s if plain_s else None,
+        "batched_vs_plain": (plain_s / batched_s) if plain_s else None,
+        "roofline": roof,
+    }
+    return out
 
 
 def copy_probe(nbytes):
@@ -92,12 +264,15 @@ def copy_probe(nbytes):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="euler_vortex_512_n4", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20 ensemble, 200 plain)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (default 2 ensemble, 10 plain)")
+    ap.add_argument("--workload", default="bn_ensemble_256", choices=sorted(PLAIN) + sorted(ENSEMBLES))
+    ap.add_argument("--count", type=int, default=None, help="ensemble size (default 256)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-procs", type=int, default=None, help="reference processes (default min(16, cores))")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--graph", action="store_true", help="time graph replay instead of evented launches")
+    ap.add_argument("--no-plain", action="store_true", help="ensemble: skip the plain (one-by-one) leg")
+    ap.add_argument("--graph", action="store_true", help="plain: time graph replay instead of evented launches")
     ap.add_argument("--copy-probe-bytes", type=int, default=0,
                     help="after the run, copy this many bytes device to device (counter calibration)")
     args = ap.parse_args()
@@ -105,57 +280,30 @@ def main():
         print("bench.py: this build measures one GPU only; --gpus %d is not supported "
               "(independent solves shard across GPUs without communication)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    ens = args.workload in ENSEMBLES
+    if args.steps is None:
+        args.steps = 20 if ens else 200
+    if args.warmup is None:
+        args.warmup = 2 if ens else 10
 
-    import aderdg
-    from aderdg import ics
-
-    ic, N, I, J, xl, xr, bc = WORKLOADS[args.workload]
-    ops = aderdg.operators(N)
-    uh0 = ics.initial_uh(ic, ops["xi"], I, J, xl, xr)
-    s = aderdg.Solver(N, I, J, xl, xr, bc)
-    s.upload(uh0)
-    t, dt, nrec = s.run(args.warmup, 0.0, 1e30) if args.warmup else (0.0, 0.0, 0)
-    s.timing(not args.graph)
-    t0 = time.perf_counter()
-    t1_sim, dt, nrec = s.run(args.steps, t, 1e30)
-    el = time.perf_counter() - t0
-    ms, launches = s.timing_get()
-    s.close()
-
-    ncell = I * J
-    dof_per_cell = 4 * (N + 1) ** 2
-    value = args.steps * ncell * dof_per_cell / el
-    fl_pred, fl_upd = aderdg.flops_per_cell(N)
-    per = {name: (ms[k] / launches[k] if launches[k] else None) for k, name in enumerate(aderdg.TIMING_SLOTS)}
-    log("per-kernel average ms:", json.dumps(per))
-    log("simulated time %.6g after %d steps, last dt %.6g, nRecompute %d" % (t1_sim, args.steps, dt, nrec))
-    roofline = None
-    if per["predictor"]:
-        achieved = fl_pred * ncell / (per["predictor"] * 1e-3) / 1e12
-        roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": achieved / FP64_PEAK_TFLOPS, "traffic": PREDICTOR_TRAFFIC_BYTES.get(args.workload),
-                    "traffic_unit": "bytes/launch", "kernel": "k_predictor",
-                    "flop_per_cell": fl_pred, "cells_per_launch": ncell,
-                    "note": "FP64 vector ALU; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s"}
-    out = {
-        "metric": "element-DOF updates/s (plain single-instance Euler step)",
-        "value": value,
-        "unit": "element-DOF updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": el / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (analytic initial condition)",
-        "config": {"workload": args.workload, "pde": "euler", "ic": ic, "N": N, "cells": [I, J], "bc": bc,
-                   "cell_updates_per_s": args.steps * ncell / el},
-        "roofline": roofline,
-    }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_seconds)
+    cpu = None
+    if ens:
+        from aderdg import ics
+        cfg = dict(ENSEMBLES[args.workload])
+        if args.count:
+            cfg["count"] = args.count
+        specs = ics.bn_ensemble(**cfg)
+        if not args.no_cpu:  # before any GPU use: the workers are forked
+            procs = args.cpu_procs or max(1, min(16, os.cpu_count() or 1))
+            cpu = cpu_ensemble(specs, args.cpu_seconds, procs)
+            log("cpu baseline:", json.dumps(cpu))
+        out = run_ensemble(args, specs)
+    else:
+        out = run_plain(args)
+        if not args.no_cpu:
+            cpu = cpu_plain(args.workload, args.cpu_seconds)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     if args.copy_probe_bytes:
         copy_probe(args.copy_probe_bytes)
     print(json.dumps(out), flush=True)

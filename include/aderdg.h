@@ -233,6 +233,13 @@ int aderdg_batch_limiter(aderdg_batch *b, int id, int32_t *status, int32_t *reco
 int aderdg_batch_remove(aderdg_batch *b, int id);
 /* Solves placed and cells in use over all arenas. */
 int aderdg_batch_usage(aderdg_batch *b, int *solves, int64_t *cells);
+/* Kernel timing of batch steps: when enabled, aderdg_batch_step launches the
+ * step's kernels one by one between hipEvents instead of replaying the graph.
+ * aderdg_batch_timing_get returns the totals of the (pde, N) arena in the
+ * slot order of aderdg_timing_get. */
+int aderdg_batch_timing_enable(aderdg_batch *b, int on);
+int aderdg_batch_timing_get(aderdg_batch *b, int pde, int N, double ms[ADERDG_TIMING_SLOTS],
+                            int64_t launches[ADERDG_TIMING_SLOTS]);
 
 #ifdef __cplusplus
 }
