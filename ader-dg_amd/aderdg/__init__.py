@@ -50,6 +50,64 @@ class _BatchDesc(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("persist_blocks", ctypes.c_int)]
 
 
+class _Ic(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("pad", ctypes.c_int), ("p", ctypes.c_double * 24)]
+
+
+class _Request(ctypes.Structure):
+    _fields_ = [("solve", _Desc), ("ic", _Ic), ("tend", ctypes.c_double), ("max_steps", ctypes.c_int),
+                ("keep_result", ctypes.c_int)]
+
+
+class _ServerDesc(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("steps_per_round", ctypes.c_int), ("persist_blocks", ctypes.c_int),
+                ("pad", ctypes.c_int), ("hbm_budget", ctypes.c_int64)]
+
+
+class _ServerStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int64) for k in ("submitted", "admitted", "completed", "failed", "queued", "running",
+                                               "dof_updates", "bytes_in_use", "peak_bytes")] + \
+               [("busy_seconds", ctypes.c_double)]
+
+
+IC_KIND = {"vortex": 1, "quadrant": 2, "dmr": 3, "bn_circle": 4}
+
+
+def ic_spec(kind, *params):
+    """Formula initial condition (aderdg_ic of include/aderdg.h); parameters in the
+    documented order of the kind."""
+    ic = _Ic()
+    ic.kind = IC_KIND[kind] if isinstance(kind, str) else int(kind)
+    if len(params) > 24:
+        raise ValueError("at most 24 parameters")
+    for k, v in enumerate(params):
+        ic.p[k] = float(v)
+    return ic
+
+
+def ic_vortex(beta=5.0, xc=5.0, yc=5.0, u0=1.0, v0=1.0):
+    return ic_spec("vortex", beta, xc, yc, u0, v0)
+
+
+def ic_quadrant(x0=0.5, y0=0.5, states=None):
+    from .ics import QUADRANT_STATES
+    st = states or QUADRANT_STATES
+    return ic_spec("quadrant", x0, y0, *st["NE"], *st["NW"], *st["SW"], *st["SE"])
+
+
+def ic_dmr():
+    import math
+    pi = 4.0 * math.atan(1.0)
+    return ic_spec("dmr", 1.0 / 6.0, math.tan(pi / 3.0), 8.0, 8.25 * math.cos(pi / 6.0), -8.25 * math.sin(pi / 6.0),
+                   116.5, 1.4, 0.0, 0.0, 1.0)
+
+
+def ic_bn_circle(radius=0.2, pratio=1.0e3, xc=0.5, yc=0.5):
+    from .ics import BN_DENSITY, BN_P_OUT, BN_PHI_EPS
+    return ic_spec("bn_circle", xc, yc, radius, pratio * BN_P_OUT, BN_P_OUT, BN_DENSITY[0], BN_DENSITY[1],
+                   1.0 - BN_PHI_EPS, BN_PHI_EPS)
+
+
 def _desc(pde, N, imax, jmax, xl, xr, bc, gamma, cfl, device, eos):
     d = _Desc()
     d.pde, d.N, d.imax, d.jmax = pde, N, imax, jmax
@@ -122,6 +180,19 @@ def load_library(exact=False):
         "aderdg_batch_remove": [vp, ctypes.c_int],
         "aderdg_batch_usage": [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(i64)],
         "aderdg_batch_timing_enable": [vp, ctypes.c_int],
+        "aderdg_batch_add_ic": [vp, ctypes.POINTER(_Desc), ctypes.POINTER(_Ic), ctypes.c_double, ctypes.c_double,
+                                ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+        "aderdg_ic_eval": [ctypes.POINTER(_Desc), ctypes.POINTER(_Ic), _dp],
+        "aderdg_server_create": [ctypes.POINTER(_ServerDesc), ctypes.POINTER(ctypes.c_void_p)],
+        "aderdg_server_reserve": [vp, ctypes.c_int, ctypes.c_int, i64, ctypes.c_int],
+        "aderdg_server_start": [vp],
+        "aderdg_server_submit": [vp, ctypes.POINTER(_Request), ctypes.POINTER(i64)],
+        "aderdg_server_poll": [vp, ctypes.POINTER(i64), _dp, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                               ctypes.POINTER(ctypes.c_int)],
+        "aderdg_server_result": [vp, i64, _dp, _dp, ctypes.POINTER(ctypes.c_int)],
+        "aderdg_server_drain": [vp, ctypes.c_double],
+        "aderdg_server_stats_get": [vp, ctypes.POINTER(_ServerStats)],
+        "aderdg_server_destroy": [vp],
         "aderdg_batch_timing_get": [vp, ctypes.c_int, ctypes.c_int, _dp, ctypes.POINTER(i64)],
     }
     for name, args in bsig.items():
@@ -370,6 +441,17 @@ class Batch:
         self.shapes[i.value] = shape
         return i.value
 
+    def add_ic(self, ic, N, imax, jmax, xl, xr, bc="periodic", pde="euler", gamma=1.4, cfl=0.9, eos=None, time=0.0,
+               tend=1e30, max_steps=1 << 30, stop_at_tend=False):
+        """Add a solve whose initial state is computed on the device from a formula (ic_spec)."""
+        k = PDE[pde] if isinstance(pde, str) else int(pde)
+        d = _desc(k, N, imax, jmax, xl, xr, bc, gamma, cfl, self.device, eos)
+        i = ctypes.c_int()
+        self._check(self.lib.aderdg_batch_add_ic(self.h, ctypes.byref(d), ctypes.byref(ic), float(time), float(tend),
+                                                 int(max_steps), 1 if stop_at_tend else 0, ctypes.byref(i)))
+        self.shapes[i.value] = (NVAR[k], N + 1, N + 1, imax, jmax)
+        return i.value
+
     def step(self, nsteps=1):
         self._check(self.lib.aderdg_batch_step(self.h, int(nsteps)))
 
@@ -421,3 +503,100 @@ class Batch:
         ln = (ctypes.c_int64 * len(TIMING_SLOTS))()
         self._check(self.lib.aderdg_batch_timing_get(self.h, k, N, ms, ln))
         return list(ms), list(ln)
+
+
+def ic_eval(ic, N, imax, jmax, xl, xr, pde="euler", gamma=1.4, eos=None, exact=False):
+    """Host evaluation of a formula initial condition: uh(nVar, N+1, N+1, IMAX, JMAX)."""
+    lib = load_library(exact)
+    k = PDE[pde] if isinstance(pde, str) else int(pde)
+    d = _desc(k, N, imax, jmax, xl, xr, "transmissive", gamma, 0.9, 0, eos)
+    uh = np.zeros((NVAR[k], N + 1, N + 1, imax, jmax), order="F")
+    rc = lib.aderdg_ic_eval(ctypes.byref(d), ctypes.byref(ic), _p(uh))
+    if rc != 0:
+        raise AderdgError("aderdg_ic_eval: error %d" % rc)
+    return uh
+
+
+class Server:
+    """Continuous-batching request loop (aderdg_server_* of include/aderdg.h).
+
+        srv = Server(hbm_budget=100e9)
+        srv.reserve("euler", 3, max_cells=4_000_000, max_solves=256)
+        srv.start()
+        t = srv.submit(ic_vortex(), N=3, imax=128, jmax=128, xl=(0, 0), xr=(10, 10), tend=0.1)
+        srv.drain(); done = srv.poll()   # [(ticket, latency_s, status)]
+    """
+
+    def __init__(self, device=0, steps_per_round=2, hbm_budget=0, persist_blocks=0, exact=False):
+        self.lib = load_library(exact)
+        self.device = device
+        d = _ServerDesc()
+        d.device, d.steps_per_round, d.persist_blocks, d.hbm_budget = device, steps_per_round, persist_blocks, \
+            int(hbm_budget)
+        h = ctypes.c_void_p()
+        self._check(self.lib.aderdg_server_create(ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+        self.shapes = {}
+
+    def _check(self, rc):
+        if rc != 0:
+            raise AderdgError("aderdg error %d: %s" % (rc, self.lib.aderdg_last_error().decode()))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.aderdg_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reserve(self, pde, N, max_cells, max_solves):
+        k = PDE[pde] if isinstance(pde, str) else int(pde)
+        self._check(self.lib.aderdg_server_reserve(self.h, k, N, int(max_cells), int(max_solves)))
+
+    def start(self):
+        self._check(self.lib.aderdg_server_start(self.h))
+
+    def request(self, ic, N, imax, jmax, xl, xr, tend, bc="periodic", pde="euler", gamma=1.4, cfl=0.9, eos=None,
+                max_steps=1 << 30, keep_result=False):
+        k = PDE[pde] if isinstance(pde, str) else int(pde)
+        r = _Request()
+        r.solve = _desc(k, N, imax, jmax, xl, xr, bc, gamma, cfl, self.device, eos)
+        r.ic = ic
+        r.tend, r.max_steps, r.keep_result = float(tend), int(max_steps), 1 if keep_result else 0
+        return r
+
+    def submit_request(self, r):
+        t = ctypes.c_int64()
+        self._check(self.lib.aderdg_server_submit(self.h, ctypes.byref(r), ctypes.byref(t)))
+        s = r.solve
+        self.shapes[t.value] = (NVAR[s.pde], s.N + 1, s.N + 1, s.imax, s.jmax)
+        return t.value
+
+    def submit(self, ic, N, imax, jmax, xl, xr, tend, **kw):
+        return self.submit_request(self.request(ic, N, imax, jmax, xl, xr, tend, **kw))
+
+    def poll(self, max_items=4096):
+        t = (ctypes.c_int64 * max_items)()
+        lat = (ctypes.c_double * max_items)()
+        st = (ctypes.c_int * max_items)()
+        c = ctypes.c_int()
+        self._check(self.lib.aderdg_server_poll(self.h, t, lat, st, max_items, ctypes.byref(c)))
+        return [(t[k], lat[k], st[k]) for k in range(c.value)]
+
+    def result(self, ticket):
+        uh = np.zeros(self.shapes[ticket], order="F")
+        tm, steps = ctypes.c_double(), ctypes.c_int()
+        self._check(self.lib.aderdg_server_result(self.h, ticket, _p(uh), ctypes.byref(tm), ctypes.byref(steps)))
+        return uh, tm.value, steps.value
+
+    def drain(self, timeout_s=0.0):
+        self._check(self.lib.aderdg_server_drain(self.h, float(timeout_s)))
+
+    def stats(self):
+        st = _ServerStats()
+        self._check(self.lib.aderdg_server_stats_get(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in _ServerStats._fields_}

@@ -85,6 +85,8 @@ MeshArgs mesh_from_desc(const aderdg_desc *d)
     m.ncell = d->imax * d->jmax;
     m.bc_mode = d->bc_mode;
     m.wrap = d->bc_mode == ADERDG_BC_PERIODIC;
+    m.xl[0] = d->xl[0];
+    m.xl[1] = d->xl[1];
     m.dx = (d->xr[0] - d->xl[0]) / (double)d->imax;  // Euler/initialize.f90:43
     m.dy = (d->xr[1] - d->xl[1]) / (double)d->jmax;
     m.gamma = d->gamma;

@@ -109,12 +109,16 @@ int Batch::reserve(int pde, int N, int64_t max_cells, int max_solves)
     return 0;
 }
 
-int Batch::add(const aderdg_desc *d, const double *uh, double time, double tend, int max_steps, int stop_at_tend,
-               int *id)
+int Batch::add(const aderdg_desc *d, const double *uh, const aderdg_ic *ic, double time, double tend, int max_steps,
+               int stop_at_tend, int *id)
 {
     int r = check_desc(d, "aderdg_batch_add");
     if (r) return r;
-    if (!uh || !id || max_steps < 1) return fail(ADERDG_EINVAL, "aderdg_batch_add: bad argument");
+    if ((!uh && !ic) || !id || max_steps < 1) return fail(ADERDG_EINVAL, "aderdg_batch_add: bad argument");
+    if (ic && ((d->pde == ADERDG_PDE_BN) != (ic->kind == ADERDG_IC_BN_CIRCLE) || ic->kind < ADERDG_IC_VORTEX ||
+               ic->kind > ADERDG_IC_BN_CIRCLE))
+        return fail(ADERDG_EINVAL, "aderdg_batch_add_ic: initial condition kind %d does not fit pde %d", ic->kind,
+                    d->pde);
     const aderdg_ops &o = d->ops;
     if (o.wGPN || o.Kxi || o.iK1 || o.F0 || o.FLcoeff || o.FRcoeff || o.uh2lim || o.lim2uh || o.uh2lob)
         return fail(ADERDG_EINVAL, "aderdg_batch_add: batched solves use the built-in operators (ops must be NULL)");
@@ -134,12 +138,20 @@ int Batch::add(const aderdg_desc *d, const double *uh, double time, double tend,
     r = g.add(mesh_from_desc(d), st, &slot);
     if (r == ADERDG_EFULL) return fail(r, "aderdg_batch_add: arena (pde %d, N %d) full", d->pde, d->N);
     if (r) return fail(r, "aderdg_batch_add: placement failed");
-    // the copy may reuse cells of a removed solve: order it after the removal
-    HIPCHK(hipStreamWaitEvent(copy, removed, 0));
-    if ((r = g.upload_uh(slot, uh, 0, copy))) return fail(r, "aderdg_batch_add: upload failed");
-    HIPCHK(hipEventRecord(upload_done, copy));
-    HIPCHK(hipStreamWaitEvent(stream, upload_done, 0));
-    hipError_t e = g.launch(kPartSeed);
+    hipError_t e = hipSuccess;
+    if (uh) {
+        // the copy may reuse cells of a removed solve: order it after the removal
+        HIPCHK(hipStreamWaitEvent(copy, removed, 0));
+        if ((r = g.upload_uh(slot, uh, 0, copy))) return fail(r, "aderdg_batch_add: upload failed");
+        HIPCHK(hipEventRecord(upload_done, copy));
+        HIPCHK(hipStreamWaitEvent(stream, upload_done, 0));
+    } else {
+        IcSpec s{};
+        s.kind = ic->kind;
+        for (int k = 0; k < kIcParams; k++) s.p[k] = ic->p[k];
+        e = launch_init(g.pde, g.N, g.b, s, slot, d->imax * d->jmax, 0, stream);
+    }
+    if (e == hipSuccess) e = g.launch(kPartSeed);
     if (e == hipSuccess) e = g.launch(kPartSeedDone);
     if (e != hipSuccess) return fail(ADERDG_EHIP, "aderdg_batch_add: seed launch: %s", hipGetErrorString(e));
     const int h = next_id++;
@@ -346,7 +358,14 @@ int aderdg_batch_add(aderdg_batch *b, const aderdg_desc *solve, const double *uh
                      int max_steps, int stop_at_tend, int *id)
 {
     if (!b) return fail(ADERDG_EINVAL, "aderdg_batch_add: null batch");
-    return b->eng.add(solve, uh, time, tend, max_steps, stop_at_tend, id);
+    return b->eng.add(solve, uh, nullptr, time, tend, max_steps, stop_at_tend, id);
+}
+
+int aderdg_batch_add_ic(aderdg_batch *b, const aderdg_desc *solve, const aderdg_ic *ic, double time, double tend,
+                        int max_steps, int stop_at_tend, int *id)
+{
+    if (!b || !ic) return fail(ADERDG_EINVAL, "aderdg_batch_add_ic: null argument");
+    return b->eng.add(solve, nullptr, ic, time, tend, max_steps, stop_at_tend, id);
 }
 
 int aderdg_batch_step(aderdg_batch *b, int nsteps)

@@ -44,7 +44,9 @@ public:
     int init(int device, int persist);
     void release();
     int reserve(int pde, int N, int64_t max_cells, int max_solves);
-    int add(const aderdg_desc *d, const double *uh, double time, double tend, int max_steps, int stop_at_tend, int *id);
+    // initial state from the host array uh, or computed on the device from ic
+    int add(const aderdg_desc *d, const double *uh, const aderdg_ic *ic, double time, double tend, int max_steps,
+            int stop_at_tend, int *id);
     int step(int nsteps);
     // Solve states into pinned memory: asynchronous (ready when snapshot_ready()) or synchronous.
     int snapshot_async();

@@ -12,76 +12,76 @@
 #include <cmath>
 
 #include "../../include/aderdg.h"
+#include "ic_spec.hpp"
 #include "operators.hpp"
 
 #pragma clang fp contract(off)
 
-namespace {
+namespace aderdg {
 
-void prim2cons(double g, const double *V, double *Q)  // PDEPrim2Cons, Euler/pde.f90:29-40
+// The reference catalogue entries as IcSpecs.
+bool ic_of_ictype(int ictype, const double *xl, const double *xr, IcSpec *ic)
 {
-    Q[0] = V[0];
-    Q[1] = V[0] * V[1];
-    Q[2] = V[0] * V[2];
-    Q[3] = V[3] / (g - 1.0) + 0.5 * V[0] * (V[1] * V[1] + V[2] * V[2]);
-}
-
-void field(int ictype, double x, double y, const double *xl, const double *xr, double g, double *Q)
-{
-    double V[4];
+    *ic = IcSpec{};
     if (ictype == 11) {
         const double pi = 4.0 * std::atan(1.0);
-        if (x <= 1.0 / 6.0 + y / std::tan(pi / 3.0)) {
-            V[0] = 8.0;
-            V[1] = 8.25 * std::cos(pi / 6.0);
-            V[2] = -8.25 * std::sin(pi / 6.0);
-            V[3] = 116.5;
-        } else {
-            V[0] = 1.4;
-            V[1] = 0.0;
-            V[2] = 0.0;
-            V[3] = 1.0;
-        }
+        ic->kind = kIcDmr;
+        const double p[10] = {1.0 / 6.0, std::tan(pi / 3.0), 8.0, 8.25 * std::cos(pi / 6.0), -8.25 * std::sin(pi / 6.0),
+                              116.5, 1.4, 0.0, 0.0, 1.0};
+        for (int k = 0; k < 10; k++) ic->p[k] = p[k];
     } else if (ictype == 4) {
-        const double pi = M_PI, beta = 5.0;
-        const double xc = 0.5 * (xl[0] + xr[0]), yc = 0.5 * (xl[1] + xr[1]);
-        const double dxr = x - xc, dyr = y - yc, r2 = dxr * dxr + dyr * dyr;
-        const double dT = -(g - 1.0) * beta * beta / (8.0 * g * pi * pi) * std::exp(1.0 - r2);
-        const double rho = std::pow(1.0 + dT, 1.0 / (g - 1.0));
-        const double f = beta / (2.0 * pi) * std::exp(0.5 * (1.0 - r2));
-        V[0] = rho;
-        V[1] = 1.0 - f * dyr;
-        V[2] = 1.0 + f * dxr;
-        V[3] = std::pow(rho, g);
-    } else {  // 5: Lax-Liu configuration 3
-        const double x0 = 0.5 * (xl[0] + xr[0]), y0 = 0.5 * (xl[1] + xr[1]);
-        const bool east = x > x0, north = y > y0;
-        static const double NE[4] = {1.5, 0.0, 0.0, 1.5}, NW[4] = {0.5323, 1.206, 0.0, 0.3},
-                            SW[4] = {0.138, 1.206, 1.206, 0.029}, SE[4] = {0.5323, 0.0, 1.206, 0.3};
-        const double *s = east ? (north ? NE : SE) : (north ? NW : SW);
-        for (int v = 0; v < 4; v++) V[v] = s[v];
+        ic->kind = kIcVortex;
+        const double p[5] = {5.0, 0.5 * (xl[0] + xr[0]), 0.5 * (xl[1] + xr[1]), 1.0, 1.0};
+        for (int k = 0; k < 5; k++) ic->p[k] = p[k];
+    } else if (ictype == 5) {  // Lax-Liu configuration 3
+        ic->kind = kIcQuadrant;
+        const double p[18] = {0.5 * (xl[0] + xr[0]), 0.5 * (xl[1] + xr[1]), 1.5, 0.0, 0.0, 1.5, 0.5323, 1.206, 0.0, 0.3,
+                              0.138, 1.206, 1.206, 0.029, 0.5323, 0.0, 1.206, 0.3};
+        for (int k = 0; k < 18; k++) ic->p[k] = p[k];
+    } else {
+        return false;
     }
-    prim2cons(g, V, Q);
+    return true;
 }
 
-}  // namespace
-
-extern "C" int aderdg_initial_condition(int ictype, int N, int imax, int jmax, const double *xl, const double *xr,
-                                        double gamma, double *uh)
+// uh(V,n,n,IMAX,JMAX) of an IcSpec on the host.
+int ic_fill(const IcSpec &ic, int N, int V, int imax, int jmax, const double *xl, const double *xr, double gamma,
+            const double *eos, double *uh)
 {
-    if (ictype != 4 && ictype != 5 && ictype != 11) return ADERDG_EINVAL;
-    if (N < 1 || N > aderdg::kMaxDegree || imax < 1 || jmax < 1 || !xl || !xr || !uh) return ADERDG_EINVAL;
-    aderdg::Operators op;
-    if (!aderdg::build_operators(N, op)) return ADERDG_ESINGULAR;
+    Operators op;
+    if (!build_operators(N, op)) return ADERDG_ESINGULAR;
     const int n = N + 1;
     const double dx = (xr[0] - xl[0]) / imax, dy = (xr[1] - xl[1]) / jmax;
     for (int j = 0; j < jmax; j++)
         for (int i = 0; i < imax; i++) {
             const double cx = (xl[0] + ((double)i + 0.5) * dx) - 0.5 * dx;
             const double cy = (xl[1] + ((double)j + 0.5) * dy) - 0.5 * dy;
-            double *u = uh + (size_t)4 * n * n * ((size_t)i + (size_t)imax * j);
+            double *u = uh + (size_t)V * n * n * ((size_t)i + (size_t)imax * j);
             for (int l = 0; l < n; l++)
-                for (int k = 0; k < n; k++) field(ictype, cx + op.xi[k] * dx, cy + op.xi[l] * dy, xl, xr, gamma, u + 4 * (k + n * l));
+                for (int k = 0; k < n; k++) ic_field(ic, gamma, eos, cx + op.xi[k] * dx, cy + op.xi[l] * dy, u + V * (k + n * l));
         }
     return 0;
+}
+
+}  // namespace aderdg
+
+extern "C" int aderdg_initial_condition(int ictype, int N, int imax, int jmax, const double *xl, const double *xr,
+                                        double gamma, double *uh)
+{
+    if (N < 1 || N > aderdg::kMaxDegree || imax < 1 || jmax < 1 || !xl || !xr || !uh) return ADERDG_EINVAL;
+    aderdg::IcSpec ic;
+    if (!aderdg::ic_of_ictype(ictype, xl, xr, &ic)) return ADERDG_EINVAL;
+    return aderdg::ic_fill(ic, N, 4, imax, jmax, xl, xr, gamma, nullptr, uh);
+}
+
+extern "C" int aderdg_ic_eval(const aderdg_desc *d, const aderdg_ic *ic, double *uh)
+{
+    if (!d || !ic || !uh || d->N < 1 || d->N > aderdg::kMaxDegree || d->imax < 1 || d->jmax < 1) return ADERDG_EINVAL;
+    const bool bn = d->pde == ADERDG_PDE_BN;
+    if (bn != (ic->kind == ADERDG_IC_BN_CIRCLE) || ic->kind < ADERDG_IC_VORTEX || ic->kind > ADERDG_IC_BN_CIRCLE)
+        return ADERDG_EINVAL;
+    aderdg::IcSpec s{};
+    s.kind = ic->kind;
+    for (int k = 0; k < aderdg::kIcParams; k++) s.p[k] = ic->p[k];
+    return aderdg::ic_fill(s, d->N, bn ? 9 : 4, d->imax, d->jmax, d->xl, d->xr, d->gamma, d->bn_eos, uh);
 }

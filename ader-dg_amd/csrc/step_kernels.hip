@@ -1369,6 +1369,28 @@ __global__ __launch_bounds__(kBlock) void k_tables(StepBufs b, const int *prefix
     if (blockIdx.x == 0 && threadIdx.x == 0) meta[kMetaSolves] = nslot;
 }
 
+// Initial state of one solve from a formula: one thread per (cell, node).
+template <class P, int N>
+__global__ __launch_bounds__(kBlock) void k_init(StepBufs b, IcSpec ic, int slot, int parity)
+{
+#pragma clang fp contract(off)
+    constexpr int V = P::V, n = N + 1, NN = n * n;
+    const SolveArgs sa = ld_const(b.solves + slot);
+    const MeshArgs &m = sa.m;
+    const OpPtr op = (OpPtr)b.op;
+    const long t = blockIdx.x * (long)kBlock + threadIdx.x;
+    if (t >= (long)m.ncell * NN) return;
+    const int cell = (int)(t / NN), node = (int)(t - (long)cell * NN);
+    const int i = cell % m.imax, j = cell / m.imax, k = node % n, l = node / n;
+    const double cx = (m.xl[0] + ((double)i + 0.5) * m.dx) - 0.5 * m.dx;  // Euler/initialize.f90:151,265
+    const double cy = (m.xl[1] + ((double)j + 0.5) * m.dy) - 0.5 * m.dy;
+    double Q[V];
+    ic_field(ic, m.gamma, m.eos, cx + op->xi[k] * m.dx, cy + op->xi[l] * m.dy, Q);
+    double *u = b.uh[parity] + ((size_t)sa.cell0 + cell) * V * NN + V * node;
+#pragma unroll
+    for (int v = 0; v < V; v++) u[v] = Q[v];
+}
+
 template <class P, int N>
 hipError_t launch_step_impl(const StepBufs &b, hipStream_t st, int part, const int grid[kNumTabs])
 {
@@ -1484,6 +1506,31 @@ hipError_t launch_tables(const StepBufs &b, const int *prefix, int nslot, const 
     g = g < 1 ? 1 : (g > 4096 ? 4096 : g);
     hipLaunchKernelGGL(k_tables, dim3(g), dim3(kBlock), 0, st, b, prefix, nslot,
                        make_int4(quanta[0], quanta[1], quanta[2], quanta[3]));
+    return hipGetLastError();
+}
+
+hipError_t launch_init(int pde, int N, const StepBufs &b, const IcSpec &ic, int slot, int ncell, int parity,
+                       hipStream_t st)
+{
+    const long pts = (long)ncell * (N + 1) * (N + 1);
+    const dim3 grid((unsigned)((pts + kBlock - 1) / kBlock)), block(kBlock);
+    if (pde == kPdeBn) {
+        switch (N) {
+        case 1: hipLaunchKernelGGL((k_init<BnPDE, 1>), grid, block, 0, st, b, ic, slot, parity); break;
+        case 2: hipLaunchKernelGGL((k_init<BnPDE, 2>), grid, block, 0, st, b, ic, slot, parity); break;
+        case 3: hipLaunchKernelGGL((k_init<BnPDE, 3>), grid, block, 0, st, b, ic, slot, parity); break;
+        case 4: hipLaunchKernelGGL((k_init<BnPDE, 4>), grid, block, 0, st, b, ic, slot, parity); break;
+        default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch (N) {
+        case 1: hipLaunchKernelGGL((k_init<EulerPDE, 1>), grid, block, 0, st, b, ic, slot, parity); break;
+        case 2: hipLaunchKernelGGL((k_init<EulerPDE, 2>), grid, block, 0, st, b, ic, slot, parity); break;
+        case 3: hipLaunchKernelGGL((k_init<EulerPDE, 3>), grid, block, 0, st, b, ic, slot, parity); break;
+        case 4: hipLaunchKernelGGL((k_init<EulerPDE, 4>), grid, block, 0, st, b, ic, slot, parity); break;
+        default: return hipErrorInvalidValue;
+        }
+    }
     return hipGetLastError();
 }
 

@@ -12,6 +12,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "ic_spec.hpp"
 #include "operators.hpp"
 
 namespace aderdg {
@@ -34,6 +35,7 @@ enum { kMetaSolves = kNumTabs, kNumMeta = kNumTabs + 1 };
 struct MeshArgs {
     int pde, imax, jmax, ncell, bc_mode, wrap;
     double dx, dy, gamma, cflpnpm;
+    double xl[2];              // lower-left corner of the domain
     double eos[4];             // Baer-Nunziato: GAMMA_S, GAMMA_G, PI_S, PI_G
     double dmr_a, dmr_b;       // DMR shock position x_sw = dmr_a + dmr_b * t (Euler/riemann_problem.f90:1-8)
     double qa[4], qb[4];       // DMR inflow / pre-shock conserved states
@@ -98,6 +100,9 @@ hipError_t launch_dt(const StepBufs &b, hipStream_t st, double time, double tend
 // (prefix[k * (nslot + 1) + s], device memory); max_entries bounds every table's length.
 hipError_t launch_tables(const StepBufs &b, const int *prefix, int nslot, const int quanta[kNumTabs], int max_entries,
                          hipStream_t st);
+// Initial state of solve `slot` into uh[parity] from a formula (ic_spec.hpp).
+hipError_t launch_init(int pde, int N, const StepBufs &b, const IcSpec &ic, int slot, int ncell, int parity,
+                       hipStream_t st);
 hipError_t launch_test_predictor(int N, const MeshArgs &m, const StepBufs &b, const TestBufs &t, double dt,
                                  hipStream_t st);
 

@@ -196,6 +196,25 @@ double aderdg_flops_update(int pde, int N);
  * ------------------------------------------------------------------------ */
 typedef struct aderdg_batch aderdg_batch;
 
+/* Initial condition given by a formula, evaluated where the solve lives (on
+ * the device for batched solves) at the Gauss-Legendre nodes with the node
+ * arithmetic of Euler/initialize.f90:151,262-273.  Parameters p[]:
+ *   VORTEX    (Euler)  beta, xc, yc, u0, v0: isentropic vortex on rho = p = 1
+ *   QUADRANT  (Euler)  x0, y0, then (rho,u,v,p) of NE, NW, SW, SE
+ *   DMR       (Euler)  x_split0, tan(pi/3), (rho,u,v,p) left, (rho,u,v,p) right
+ *                      (InitialField, Euler/initialize.f90:311-341)
+ *   BN_CIRCLE (Baer-Nunziato) xc, yc, radius, p_in, p_out, rho_s, rho_g,
+ *                      phi_s inside, phi_s outside; both phases at rest */
+enum aderdg_ic_kind { ADERDG_IC_VORTEX = 1, ADERDG_IC_QUADRANT = 2, ADERDG_IC_DMR = 3, ADERDG_IC_BN_CIRCLE = 4 };
+typedef struct {
+    int kind;
+    int pad;
+    double p[24];
+} aderdg_ic;
+
+/* Host evaluation of an initial condition into uh(nVar,N+1,N+1,IMAX,JMAX). */
+int aderdg_ic_eval(const aderdg_desc *solve, const aderdg_ic *ic, double *uh);
+
 typedef struct {
     int device;          /* HIP device ordinal                                   */
     int persist_blocks;  /* workgroups of each table-driven kernel per group;
@@ -217,6 +236,10 @@ int64_t aderdg_batch_reserved_bytes(aderdg_batch *b);
  * free slot or range in the (pde, N) arena (nothing was added). */
 int aderdg_batch_add(aderdg_batch *b, const aderdg_desc *solve, const double *uh, double time, double tend,
                      int max_steps, int stop_at_tend, int *id);
+/* aderdg_batch_add with the initial state computed on the device from ic
+ * (no host array, no transfer). */
+int aderdg_batch_add_ic(aderdg_batch *b, const aderdg_desc *solve, const aderdg_ic *ic, double time, double tend,
+                        int max_steps, int stop_at_tend, int *id);
 /* Enqueue nsteps replays of the step graph (asynchronous). */
 int aderdg_batch_step(aderdg_batch *b, int nsteps);
 int aderdg_batch_sync(aderdg_batch *b);
@@ -240,6 +263,58 @@ int aderdg_batch_usage(aderdg_batch *b, int *solves, int64_t *cells);
 int aderdg_batch_timing_enable(aderdg_batch *b, int on);
 int aderdg_batch_timing_get(aderdg_batch *b, int pde, int N, double ms[ADERDG_TIMING_SLOTS],
                             int64_t launches[ADERDG_TIMING_SLOTS]);
+
+/* ------------------------------------------------------------------------
+ * Serving loop.  No counterpart in the reference.  A worker thread owns a
+ * batch: requests (a solve, a formula initial condition, an end time) wait
+ * in arrival order until the (pde, N) arena has room and the admitted
+ * footprint stays within hbm_budget, are then added to the running batch
+ * (continuous batching: the step graph keeps running while solves join and
+ * leave), and are retired when they reach their end time.  Submission and
+ * polling are thread-safe; everything on the device is done by the worker.
+ * ------------------------------------------------------------------------ */
+typedef struct aderdg_server aderdg_server;
+
+typedef struct {
+    int device;
+    int steps_per_round;  /* step-graph replays between completion checks (0: 2)  */
+    int persist_blocks;   /* as in aderdg_batch_desc                              */
+    int pad;
+    int64_t hbm_budget;   /* bytes of admitted solve footprint (0: arenas only)   */
+} aderdg_server_desc;
+
+typedef struct {
+    aderdg_desc solve;    /* ops must be NULL                                     */
+    aderdg_ic ic;
+    double tend;          /* the solve ends once its time reaches tend            */
+    int max_steps;        /* and at the latest after this many steps (> 0)        */
+    int keep_result;      /* 1: keep the final uh until aderdg_server_result      */
+} aderdg_request;
+
+typedef struct {
+    int64_t submitted, admitted, completed, failed, queued, running;
+    int64_t dof_updates;  /* nVar (N+1)^2 cells x steps of completed requests     */
+    int64_t bytes_in_use, peak_bytes;
+    double busy_seconds;  /* worker time with solves on the device                */
+} aderdg_server_stats;
+
+int aderdg_server_create(const aderdg_server_desc *desc, aderdg_server **out);
+/* Arenas, before aderdg_server_start (aderdg_batch_reserve). */
+int aderdg_server_reserve(aderdg_server *s, int pde, int N, int64_t max_cells, int max_solves);
+int aderdg_server_start(aderdg_server *s);
+/* Queue a request; *ticket identifies it in poll and result. */
+int aderdg_server_submit(aderdg_server *s, const aderdg_request *req, int64_t *ticket);
+/* Requests finished since the last poll: ticket, seconds from submit to
+ * completion, status (0 done, else the error code); *count written. */
+int aderdg_server_poll(aderdg_server *s, int64_t *tickets, double *latency_s, int *status, int max, int *count);
+/* Final state of a finished request kept with keep_result (released here). */
+int aderdg_server_result(aderdg_server *s, int64_t ticket, double *uh, double *time, int *steps);
+/* Wait until no request is queued or running (timeout_s <= 0: no limit);
+ * ADERDG_ESTATE on timeout. */
+int aderdg_server_drain(aderdg_server *s, double timeout_s);
+int aderdg_server_stats_get(aderdg_server *s, aderdg_server_stats *st);
+/* Stop the worker (queued requests are dropped) and free everything. */
+int aderdg_server_destroy(aderdg_server *s);
 
 #ifdef __cplusplus
 }

@@ -95,3 +95,22 @@ def test_solve_footprint_host():
             floor = 8 * (3 * V * n * n + 8 * V * n + 2 * V * s * s)
             per_cell = f(pde, N, 200, 200) / 200.0 ** 2
             assert floor < per_cell < 1.25 * floor + 512, (pde, N, per_cell, floor)
+
+
+def test_formula_ic_host_matches_python():
+    """aderdg_ic_eval (the formula library the device also evaluates) against the
+    Python initial conditions: bitwise where no transcendental function enters."""
+    from aderdg import ics
+    from helpers import BN_EOS_STIFF
+    xi3, xi2, xi4 = (aderdg.operators(N)["xi"] for N in (3, 2, 4))
+    a = aderdg.ic_eval(aderdg.ic_quadrant(), 3, 20, 16, (0.0, 0.0), (1.0, 1.0))
+    np.testing.assert_array_equal(a, ics.initial_uh("quadrant", xi3, 20, 16, (0.0, 0.0), (1.0, 1.0)))
+    a = aderdg.ic_eval(aderdg.ic_dmr(), 2, 24, 8, (0.0, 0.0), (4.0, 1.0))
+    np.testing.assert_array_equal(a, ics.initial_uh("dmr", xi2, 24, 8, (0.0, 0.0), (4.0, 1.0)))
+    a = aderdg.ic_eval(aderdg.ic_bn_circle(0.25, 3.0e3), 2, 16, 16, (0.0, 0.0), (1.0, 1.0), pde="bn", eos=BN_EOS_STIFF)
+    np.testing.assert_array_equal(a, ics.initial_uh_bn(xi2, 16, 16, (0.0, 0.0), (1.0, 1.0), BN_EOS_STIFF,
+                                                       radius=0.25, pratio=3.0e3))
+    a = aderdg.ic_eval(aderdg.ic_vortex(), 4, 12, 12, (0.0, 0.0), (10.0, 10.0))
+    np.testing.assert_allclose(a, ics.initial_uh("vortex", xi4, 12, 12, (0.0, 0.0), (10.0, 10.0)), rtol=2e-15, atol=0)
+    with pytest.raises(aderdg.AderdgError):  # a Baer-Nunziato formula for an Euler solve
+        aderdg.ic_eval(aderdg.ic_bn_circle(), 2, 4, 4, (0.0, 0.0), (1.0, 1.0))
