@@ -70,7 +70,7 @@ class _ServerStats(ctypes.Structure):
                [("busy_seconds", ctypes.c_double)]
 
 
-IC_KIND = {"vortex": 1, "quadrant": 2, "dmr": 3, "bn_circle": 4}
+IC_KIND = {"vortex": 1, "quadrant": 2, "dmr": 3, "bn_circle": 4, "bn_disc": 5}
 
 
 def ic_spec(kind, *params):
@@ -100,6 +100,11 @@ def ic_dmr():
     pi = 4.0 * math.atan(1.0)
     return ic_spec("dmr", 1.0 / 6.0, math.tan(pi / 3.0), 8.0, 8.25 * math.cos(pi / 6.0), -8.25 * math.sin(pi / 6.0),
                    116.5, 1.4, 0.0, 0.0, 1.0)
+
+
+def ic_bn_disc(v_in, v_out, radius=0.2, xc=0.5, yc=0.5):
+    """Disc of primitive state v_in (rho_s,u_s,v_s,p_s,rho_g,u_g,v_g,p_g,phi_s) in v_out."""
+    return ic_spec("bn_disc", xc, yc, radius, *v_in, *v_out)
 
 
 def ic_bn_circle(radius=0.2, pratio=1.0e3, xc=0.5, yc=0.5):
@@ -187,7 +192,8 @@ def load_library(exact=False):
         "aderdg_server_reserve": [vp, ctypes.c_int, ctypes.c_int, i64, ctypes.c_int],
         "aderdg_server_start": [vp],
         "aderdg_server_submit": [vp, ctypes.POINTER(_Request), ctypes.POINTER(i64)],
-        "aderdg_server_poll": [vp, ctypes.POINTER(i64), _dp, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+        "aderdg_server_poll": [vp, ctypes.POINTER(i64), _dp, ctypes.POINTER(ctypes.c_int),
+                               ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_int,
                                ctypes.POINTER(ctypes.c_int)],
         "aderdg_server_result": [vp, i64, _dp, _dp, ctypes.POINTER(ctypes.c_int)],
         "aderdg_server_drain": [vp, ctypes.c_double],
@@ -579,12 +585,18 @@ class Server:
     def submit(self, ic, N, imax, jmax, xl, xr, tend, **kw):
         return self.submit_request(self.request(ic, N, imax, jmax, xl, xr, tend, **kw))
 
-    def poll(self, max_items=4096):
+    def poll(self, max_items=4096, details=False):
+        """Finished requests: (ticket, latency_s, status), or with details
+        (ticket, latency_s, status, steps, reached_tend)."""
         t = (ctypes.c_int64 * max_items)()
         lat = (ctypes.c_double * max_items)()
         st = (ctypes.c_int * max_items)()
+        steps = (ctypes.c_int * max_items)()
+        rt = (ctypes.c_int * max_items)()
         c = ctypes.c_int()
-        self._check(self.lib.aderdg_server_poll(self.h, t, lat, st, max_items, ctypes.byref(c)))
+        self._check(self.lib.aderdg_server_poll(self.h, t, lat, st, steps, rt, max_items, ctypes.byref(c)))
+        if details:
+            return [(t[k], lat[k], st[k], steps[k], bool(rt[k])) for k in range(c.value)]
         return [(t[k], lat[k], st[k]) for k in range(c.value)]
 
     def result(self, ticket):

@@ -153,6 +153,23 @@ def initial_uh_bn(xi, imax, jmax, xl, xr, eos, **kw):
     return np.asfortranarray(bn_circle(np.asarray(X), np.asarray(Y), eos, **kw))
 
 
+# Riemann problems of Baer-Nunziato/initialize.f90:314-356 (ICType 1 and 2): left and right
+# primitive states (rho_s, u_s, v_s, p_s, rho_g, u_g, v_g, p_g, phi_s), ideal-gas phases
+BN_RP_STATES = {
+    1: ((1.0, 0.0, 0.0, 1.0, 0.5, 0.0, 0.0, 1.0, 0.4), (2.0, 0.0, 0.0, 2.0, 1.5, 0.0, 0.0, 2.0, 0.8)),
+    2: ((800.0, 0.0, 0.0, 500.0, 1.5, 0.0, 0.0, 2.0, 0.4), (1000.0, 0.0, 0.0, 600.0, 1.0, 0.0, 0.0, 1.0, 0.3)),
+}
+BN_EOS_REFERENCE = (1.4, 1.4, 0.0, 0.0)  # Baer-Nunziato/baer_nunziato_params.ini:1-4
+
+
+def bn_disc(X, Y, eos, v_in, v_out, radius=0.2, xc=0.5, yc=0.5):
+    """Disc of primitive state v_in inside v_out (both 9-tuples)."""
+    inside = (X - xc) ** 2 + (Y - yc) ** 2 <= radius * radius
+    V = np.where(inside[None], np.asarray(v_in, dtype=np.float64).reshape((9,) + (1,) * X.ndim),
+                 np.asarray(v_out, dtype=np.float64).reshape((9,) + (1,) * X.ndim))
+    return bn_prim2cons(eos, V)
+
+
 # ---- ensembles ------------------------------------------------------------
 
 BN_EOS_ENSEMBLE = (4.4, 1.4, 6.0e8, 0.0)  # GAMMA_S, GAMMA_G, PI_S, PI_G: stiffened-gas solid, ideal gas
@@ -171,4 +188,91 @@ def bn_ensemble(count=256, seed=2026, side=(64, 384), degrees=(2, 3)):
         N = int(rng.choice(degrees))
         out.append(dict(N=N, imax=s, jmax=s, radius=float(rng.uniform(0.1, 0.3)),
                         pratio=float(10.0 ** rng.uniform(3.0, 4.0))))
+    return out
+
+
+PNPM = (1.0, 0.33, 0.17, 0.1, 0.069)  # PNPMTable, Euler/ader_dg.f90:28
+
+
+def _dt_estimate(pde, N, side, xl, xr, q, eos=None, gamma=1.4, cfl=0.9):
+    """calc_time_step (Euler/calc_time_step.f90:29-42) on conserved point values q
+    (V, ...) sampled over the domain: cfl PNPM / (max(|u|+c)/dx + max(|v|+c)/dy)."""
+    dx = (xr[0] - xl[0]) / side
+    dy = (xr[1] - xl[1]) / side
+    if pde == "euler":
+        rho = q[0]
+        u, v = q[1] / rho, q[2] / rho
+        p = (gamma - 1.0) * (q[3] - 0.5 * rho * (u * u + v * v))
+        c = np.sqrt(gamma * p / rho)
+        sx, sy = np.abs(u) + c, np.abs(v) + c
+    else:
+        gs, gg, ps, pg = eos
+        sx = sy = 0.0
+        for a, g, pi in ((0, gs, ps), (4, gg, pg)):
+            ar = q[a]
+            u, v = q[a + 1] / ar, q[a + 2] / ar
+            phi = q[8] if a == 0 else 1.0 - q[8]
+            rho = ar / phi
+            p = (g - 1.0) * (q[a + 3] / phi - 0.5 * rho * (u * u + v * v)) - g * pi
+            c = np.sqrt(g * (p + pi) / rho)
+            sx = np.maximum(sx, np.abs(u) + c)
+            sy = np.maximum(sy, np.abs(v) + c)
+    return cfl * PNPM[N] / (np.max(sx) / dx + np.max(sy) / dy)
+
+
+def serving_requests(count=4000, seed=7, steps=20, side=(32, 512), degrees=(2, 3, 4)):
+    """Seeded request stream of the serving workload: 70% Euler (half isentropic
+    vortices on [0,10]^2 periodic with random strength, centre and background
+    velocity, half four-quadrant Riemann problems on [0,1]^2 transmissive with the
+    Lax-Liu 3 states scaled by U(0.8,1.2) and a random split point), 30%
+    Baer-Nunziato discs (the two states of the reference's Riemann problem 1 or
+    2 inside / outside a circle of random radius and centre, ideal-gas phases);
+    mesh side
+    log-uniform in `side`, N uniform over `degrees`.  Each request's end time is
+    `steps` initial time steps (estimated from its initial state), so requests of
+    every size run a comparable number of steps.  Returns dicts with pde, N,
+    side, xl, xr, bc, kind, params (aderdg_ic layout), eos, tend."""
+    rng = np.random.default_rng(seed)
+    lo, hi = side
+    out = []
+    for _ in range(count):
+        s = int(round(lo * (hi / lo) ** rng.uniform()))
+        N = int(rng.choice(degrees))
+        r = rng.uniform()
+        n_est = min(s, 48)
+        if r < 0.35:
+            beta, xc, yc = rng.uniform(3.0, 6.0), rng.uniform(3.5, 6.5), rng.uniform(3.5, 6.5)
+            u0, v0 = rng.uniform(-1.0, 1.0, 2)
+            xl, xr = (0.0, 0.0), (10.0, 10.0)
+            X, Y = np.meshgrid((np.arange(n_est) + 0.5) * 10.0 / n_est, (np.arange(n_est) + 0.5) * 10.0 / n_est)
+            q = vortex(X, Y, 1.4, beta, xc, yc)
+            q[1] += q[0] * (u0 - 1.0)  # background velocity (u0, v0) instead of (1, 1), p and rho unchanged
+            q[2] += q[0] * (v0 - 1.0)
+            rho = q[0]
+            q[3] = rho ** 1.4 / 0.4 + 0.5 * (q[1] ** 2 + q[2] ** 2) / rho
+            req = dict(pde="euler", bc="periodic", kind="vortex", params=(beta, xc, yc, u0, v0), eos=None)
+        elif r < 0.70:
+            x0, y0 = rng.uniform(0.4, 0.6, 2)
+            st = {}
+            for key in ("NE", "NW", "SW", "SE"):
+                rho, u, v, p = QUADRANT_STATES[key]
+                f = rng.uniform(0.8, 1.2, 4)
+                st[key] = (rho * f[0], u * f[1], v * f[2], p * f[3])
+            xl, xr = (0.0, 0.0), (1.0, 1.0)
+            X, Y = np.meshgrid((np.arange(n_est) + 0.5) / n_est, (np.arange(n_est) + 0.5) / n_est)
+            q = quadrant(X, Y, 1.4, st, x0, y0)
+            params = (x0, y0) + st["NE"] + st["NW"] + st["SW"] + st["SE"]
+            req = dict(pde="euler", bc="transmissive", kind="quadrant", params=params, eos=None)
+        else:
+            radius = rng.uniform(0.1, 0.3)
+            xc, yc = rng.uniform(0.4, 0.6, 2)
+            v_in, v_out = BN_RP_STATES[int(rng.integers(1, 3))]
+            xl, xr = (0.0, 0.0), (1.0, 1.0)
+            X, Y = np.meshgrid((np.arange(n_est) + 0.5) / n_est, (np.arange(n_est) + 0.5) / n_est)
+            q = bn_disc(X, Y, BN_EOS_REFERENCE, v_in, v_out, radius, xc, yc)
+            params = (xc, yc, radius) + tuple(v_in) + tuple(v_out)
+            req = dict(pde="bn", bc="transmissive", kind="bn_disc", params=params, eos=BN_EOS_REFERENCE)
+        dt = _dt_estimate(req["pde"], N, s, xl, xr, q, req["eos"])
+        req.update(N=N, side=s, xl=xl, xr=xr, tend=float(steps * dt), steps=steps)
+        out.append(req)
     return out

@@ -115,8 +115,8 @@ int Batch::add(const aderdg_desc *d, const double *uh, const aderdg_ic *ic, doub
     int r = check_desc(d, "aderdg_batch_add");
     if (r) return r;
     if ((!uh && !ic) || !id || max_steps < 1) return fail(ADERDG_EINVAL, "aderdg_batch_add: bad argument");
-    if (ic && ((d->pde == ADERDG_PDE_BN) != (ic->kind == ADERDG_IC_BN_CIRCLE) || ic->kind < ADERDG_IC_VORTEX ||
-               ic->kind > ADERDG_IC_BN_CIRCLE))
+    if (ic && ((d->pde == ADERDG_PDE_BN) != (ic->kind == ADERDG_IC_BN_CIRCLE || ic->kind == ADERDG_IC_BN_DISC) ||
+               ic->kind < ADERDG_IC_VORTEX || ic->kind > ADERDG_IC_BN_DISC))
         return fail(ADERDG_EINVAL, "aderdg_batch_add_ic: initial condition kind %d does not fit pde %d", ic->kind,
                     d->pde);
     const aderdg_ops &o = d->ops;

@@ -15,6 +15,8 @@
 //             x <= x_split0 + y/tan(pi/3) (InitialField, Euler/initialize.f90:311-341)
 //   BN_CIRCLE (Baer-Nunziato) xc, yc, radius, p_in, p_out, rho_s, rho_g,
 //             phi_in, phi_out: both phases at rest
+//   BN_DISC   (Baer-Nunziato) xc, yc, radius, then the primitive states
+//             (rho_s,u_s,v_s,p_s,rho_g,u_g,v_g,p_g,phi_s) inside and outside
 #pragma once
 #include <cmath>
 
@@ -27,7 +29,7 @@
 
 namespace aderdg {
 
-enum { kIcVortex = 1, kIcQuadrant = 2, kIcDmr = 3, kIcBnCircle = 4 };
+enum { kIcVortex = 1, kIcQuadrant = 2, kIcDmr = 3, kIcBnCircle = 4, kIcBnDisc = 5 };
 constexpr int kIcParams = 24;
 
 struct IcSpec {
@@ -65,6 +67,11 @@ ADG_HD void ic_field(const IcSpec &ic, double gamma, const double *eos, double x
 {
 #pragma clang fp contract(off)
     const double *p = ic.p;
+    if (ic.kind == kIcBnDisc) {
+        const bool inside = (x - p[0]) * (x - p[0]) + (y - p[1]) * (y - p[1]) <= p[2] * p[2];
+        ic_bn_cons(eos, p + (inside ? 3 : 12), Q);
+        return;
+    }
     if (ic.kind == kIcBnCircle) {
         const bool inside = (x - p[0]) * (x - p[0]) + (y - p[1]) * (y - p[1]) <= p[2] * p[2];
         double V[9];

@@ -78,8 +78,8 @@ extern "C" int aderdg_ic_eval(const aderdg_desc *d, const aderdg_ic *ic, double 
 {
     if (!d || !ic || !uh || d->N < 1 || d->N > aderdg::kMaxDegree || d->imax < 1 || d->jmax < 1) return ADERDG_EINVAL;
     const bool bn = d->pde == ADERDG_PDE_BN;
-    if (bn != (ic->kind == ADERDG_IC_BN_CIRCLE) || ic->kind < ADERDG_IC_VORTEX || ic->kind > ADERDG_IC_BN_CIRCLE)
-        return ADERDG_EINVAL;
+    const bool bn_kind = ic->kind == ADERDG_IC_BN_CIRCLE || ic->kind == ADERDG_IC_BN_DISC;
+    if (bn != bn_kind || ic->kind < ADERDG_IC_VORTEX || ic->kind > ADERDG_IC_BN_DISC) return ADERDG_EINVAL;
     aderdg::IcSpec s{};
     s.kind = ic->kind;
     for (int k = 0; k < aderdg::kIcParams; k++) s.p[k] = ic->p[k];

@@ -43,6 +43,8 @@ struct Done {
     int64_t ticket;
     double latency;
     int status;
+    int steps = 0;
+    int reached_tend = 0;
 };
 
 struct Result {
@@ -101,7 +103,7 @@ int admit(aderdg_server *s)
         if (rc == ADERDG_EFULL) {
             if (s->running.empty()) {  // can never fit: fail it rather than wait forever
                 std::lock_guard<std::mutex> lk(s->mu);
-                s->done.push_back({r.ticket, std::chrono::duration<double>(Clock::now() - r.submitted).count(), rc});
+                s->done.push_back({r.ticket, std::chrono::duration<double>(Clock::now() - r.submitted).count(), rc, 0, 0});
                 s->stats.failed++;
                 s->stats.queued--;
                 s->pending.pop_front();
@@ -111,7 +113,7 @@ int admit(aderdg_server *s)
         }
         if (rc) {
             std::lock_guard<std::mutex> lk(s->mu);
-            s->done.push_back({r.ticket, std::chrono::duration<double>(Clock::now() - r.submitted).count(), rc});
+            s->done.push_back({r.ticket, std::chrono::duration<double>(Clock::now() - r.submitted).count(), rc, 0, 0});
             s->stats.failed++;
             s->stats.queued--;
             s->pending.pop_front();
@@ -166,7 +168,7 @@ int retire(aderdg_server *s, int k)
         const double lat = std::chrono::duration<double>(Clock::now() - r.submitted).count();
         std::lock_guard<std::mutex> lk(s->mu);
         if (r.req.keep_result) s->results[r.ticket] = std::move(res);
-        s->done.push_back({r.ticket, lat, 0});
+        s->done.push_back({r.ticket, lat, 0, st.steps, st.time < r.req.tend ? 0 : 1});
         s->stats.completed++;
         s->stats.running--;
         s->stats.dof_updates += dof;
@@ -301,7 +303,8 @@ int aderdg_server_submit(aderdg_server *s, const aderdg_request *req, int64_t *t
     return 0;
 }
 
-int aderdg_server_poll(aderdg_server *s, int64_t *tickets, double *latency_s, int *status, int max, int *count)
+int aderdg_server_poll(aderdg_server *s, int64_t *tickets, double *latency_s, int *status, int *steps,
+                       int *reached_tend, int max, int *count)
 {
     if (!s || !count || max < 0 || (max > 0 && !tickets)) return fail(ADERDG_EINVAL, "aderdg_server_poll: bad argument");
     std::lock_guard<std::mutex> lk(s->mu);
@@ -310,6 +313,8 @@ int aderdg_server_poll(aderdg_server *s, int64_t *tickets, double *latency_s, in
         tickets[c] = s->done[c].ticket;
         if (latency_s) latency_s[c] = s->done[c].latency;
         if (status) status[c] = s->done[c].status;
+        if (steps) steps[c] = s->done[c].steps;
+        if (reached_tend) reached_tend[c] = s->done[c].reached_tend;
         c++;
     }
     s->done.erase(s->done.begin(), s->done.begin() + c);

@@ -47,6 +47,7 @@ PLAIN = {
     "euler_quadrant_1024_n3": ("quadrant", 3, 1024, 1024, (0.0, 0.0), (1.0, 1.0), "transmissive"),
 }
 ENSEMBLES = {"bn_ensemble_256": dict(count=256, seed=2026)}
+SERVING = {"serving_4000": dict(count=4000, seed=7, steps=20)}
 
 
 def log(*a):
@@ -245,6 +246,80 @@ def run_ensemble(args, specs):
     return out
 
 
+# ---------------------------------------------------------------------------
+# serving: Poisson arrivals into the continuous-batching server
+# ---------------------------------------------------------------------------
+SERVING_CELLS = {"euler": 2_000_000, "bn": 1_500_000}  # arena cells per (pde, N)
+
+
+def serving_setup(specs):
+    import aderdg
+    srv = aderdg.Server(steps_per_round=2)
+    for pde, N in sorted({(s["pde"], s["N"]) for s in specs}):
+        srv.reserve(pde, N, max_cells=SERVING_CELLS[pde], max_solves=512)
+    srv.start()
+    # a step cap beside each end time: a solve whose time step collapses stops at 3x the planned steps
+    reqs = [srv.request(aderdg.ic_spec(s["kind"], *s["params"]), s["N"], s["side"], s["side"], s["xl"], s["xr"],
+                        s["tend"], bc=s["bc"], pde=s["pde"], eos=s["eos"], max_steps=3 * s["steps"]) for s in specs]
+    return srv, reqs
+
+
+def serve(srv, reqs, rate, seed):
+    """Submit reqs at Poisson arrival times of `rate` per second (all at once when rate
+    is None) from a producer thread; returns latencies (s), wall seconds, DOF updates."""
+    import threading
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs))) if rate else np.zeros(len(reqs))
+    dof0 = srv.stats()["dof_updates"]
+    t0 = time.perf_counter()
+
+    def producer():
+        for k, r in enumerate(reqs):
+            delay = arrivals[k] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            srv.submit_request(r)
+
+    th = threading.Thread(target=producer)
+    th.start()
+    lat, capped = [], 0
+    while len(lat) < len(reqs):
+        for t, l, st, steps, reached in srv.poll(details=True):
+            if st != 0:
+                raise RuntimeError("request %d failed with %d" % (t, st))
+            lat.append(l)
+            capped += 0 if reached else 1
+            if len(lat) % 500 == 0:
+                log("  %d / %d requests done after %.1f s" % (len(lat), len(reqs), time.perf_counter() - t0))
+        time.sleep(0.001)
+    wall = time.perf_counter() - t0
+    th.join()
+    if capped:
+        log("  %d requests stopped at the step cap before their end time" % capped)
+    return np.array(lat), wall, srv.stats()["dof_updates"] - dof0
+
+
+def run_serving(specs, factors, seed=11):
+    import numpy as np
+    srv, reqs = serving_setup(specs)
+    lat, wall, dof = serve(srv, reqs, None, seed)
+    sat = len(reqs) / wall
+    log("serving saturation: %d requests in %.2f s = %.1f req/s, %.3g DOF/s" % (len(reqs), wall, sat, dof / wall))
+    out = {"requests": len(reqs), "saturation_req_per_s": sat, "saturation_dof_per_s": dof / wall, "rates": []}
+    for f in factors:
+        lat, wall, dof = serve(srv, reqs, f * sat, seed + int(100 * f))
+        r = {"rate_factor": f, "offered_req_per_s": f * sat, "achieved_req_per_s": len(reqs) / wall,
+             "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+             "mean_s": float(lat.mean()), "max_s": float(lat.max()), "dof_per_s": dof / wall}
+        log("serving at %.1fx saturation: %s" % (f, json.dumps(r)))
+        out["rates"].append(r)
+    st = srv.stats()
+    srv.close()
+    out["peak_admitted_bytes"] = st["peak_bytes"]
+    return out
+
+
 def copy_probe(nbytes):
     """One device-to-device copy of known size (the HIP runtime's copy kernel): the
     FETCH_SIZE / WRITE_SIZE it reads under rocprofv3 --pmc calibrate the counters."""
@@ -266,7 +341,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20 ensemble, 200 plain)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (default 2 ensemble, 10 plain)")
-    ap.add_argument("--workload", default="bn_ensemble_256", choices=sorted(PLAIN) + sorted(ENSEMBLES))
+    ap.add_argument("--workload", default="bn_ensemble_256", choices=sorted(PLAIN) + sorted(ENSEMBLES) + sorted(SERVING))
+    ap.add_argument("--serving-requests", type=int, default=400,
+                    help="ensemble workload: requests of the serving stream timed beside it (0: none)")
     ap.add_argument("--count", type=int, default=None, help="ensemble size (default 256)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-procs", type=int, default=None, help="reference processes (default min(16, cores))")
@@ -287,6 +364,24 @@ def main():
         args.warmup = 2 if ens else 10
 
     cpu = None
+    if args.workload in SERVING:
+        from aderdg import ics
+        cfg = dict(SERVING[args.workload])
+        if args.count:
+            cfg["count"] = args.count
+        specs = ics.serving_requests(**cfg)
+        sv = run_serving(specs, (0.5, 0.8, 1.0))
+        r08 = [r for r in sv["rates"] if r["rate_factor"] == 0.8][0]
+        out = {"metric": "p50/p99 solve latency at fixed QPS (serving)", "value": r08["p99_s"], "unit": "s",
+               "n_gpus": 1, "steps": cfg["steps"], "warmup": 0, "ms_per_step": None, "higher_is_better": False,
+               "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+               "data": "synthetic (seeded formula initial conditions)",
+               "config": {"workload": args.workload, "requests": cfg["count"], "seed": cfg["seed"],
+                          "mix": "35% Euler vortex, 35% Euler quadrant, 30% Baer-Nunziato disc (reference Riemann states 1/2)",
+                          "side": [32, 512], "N": [2, 3, 4], "steps_per_request": cfg["steps"]},
+               "serving": sv}
+        print(json.dumps(out), flush=True)
+        return
     if ens:
         from aderdg import ics
         cfg = dict(ENSEMBLES[args.workload])
@@ -298,6 +393,9 @@ def main():
             cpu = cpu_ensemble(specs, args.cpu_seconds, procs)
             log("cpu baseline:", json.dumps(cpu))
         out = run_ensemble(args, specs)
+        if args.serving_requests:
+            sspecs = ics.serving_requests(**dict(SERVING["serving_4000"], count=args.serving_requests))
+            out["serving"] = run_serving(sspecs, (0.8,))
     else:
         out = run_plain(args)
         if not args.no_cpu:

@@ -204,8 +204,13 @@ typedef struct aderdg_batch aderdg_batch;
  *   DMR       (Euler)  x_split0, tan(pi/3), (rho,u,v,p) left, (rho,u,v,p) right
  *                      (InitialField, Euler/initialize.f90:311-341)
  *   BN_CIRCLE (Baer-Nunziato) xc, yc, radius, p_in, p_out, rho_s, rho_g,
- *                      phi_s inside, phi_s outside; both phases at rest */
-enum aderdg_ic_kind { ADERDG_IC_VORTEX = 1, ADERDG_IC_QUADRANT = 2, ADERDG_IC_DMR = 3, ADERDG_IC_BN_CIRCLE = 4 };
+ *                      phi_s inside, phi_s outside; both phases at rest
+ *   BN_DISC   (Baer-Nunziato) xc, yc, radius, then the primitive states
+ *                      (rho_s,u_s,v_s,p_s,rho_g,u_g,v_g,p_g,phi_s) inside and
+ *                      outside (e.g. the two sides of a Riemann problem of
+ *                      Baer-Nunziato/initialize.f90:314-390) */
+enum aderdg_ic_kind { ADERDG_IC_VORTEX = 1, ADERDG_IC_QUADRANT = 2, ADERDG_IC_DMR = 3, ADERDG_IC_BN_CIRCLE = 4,
+                      ADERDG_IC_BN_DISC = 5 };
 typedef struct {
     int kind;
     int pad;
@@ -305,8 +310,11 @@ int aderdg_server_start(aderdg_server *s);
 /* Queue a request; *ticket identifies it in poll and result. */
 int aderdg_server_submit(aderdg_server *s, const aderdg_request *req, int64_t *ticket);
 /* Requests finished since the last poll: ticket, seconds from submit to
- * completion, status (0 done, else the error code); *count written. */
-int aderdg_server_poll(aderdg_server *s, int64_t *tickets, double *latency_s, int *status, int max, int *count);
+ * completion, status (0 done, else the error code), steps taken and whether
+ * the solve reached tend (1) or stopped at max_steps (0); *count written.
+ * Any of the output arrays except tickets may be NULL. */
+int aderdg_server_poll(aderdg_server *s, int64_t *tickets, double *latency_s, int *status, int *steps,
+                       int *reached_tend, int max, int *count);
 /* Final state of a finished request kept with keep_result (released here). */
 int aderdg_server_result(aderdg_server *s, int64_t ticket, double *uh, double *time, int *steps);
 /* Wait until no request is queued or running (timeout_s <= 0: no limit);

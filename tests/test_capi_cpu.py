@@ -112,5 +112,11 @@ def test_formula_ic_host_matches_python():
                                                        radius=0.25, pratio=3.0e3))
     a = aderdg.ic_eval(aderdg.ic_vortex(), 4, 12, 12, (0.0, 0.0), (10.0, 10.0))
     np.testing.assert_allclose(a, ics.initial_uh("vortex", xi4, 12, 12, (0.0, 0.0), (10.0, 10.0)), rtol=2e-15, atol=0)
+    vin, vout = ics.BN_RP_STATES[2]
+    a = aderdg.ic_eval(aderdg.ic_bn_disc(vin, vout, 0.3, 0.45, 0.55), 3, 12, 14, (0.0, 0.0), (1.0, 1.0), pde="bn",
+                       eos=ics.BN_EOS_REFERENCE)
+    X, Y = ics.node_coordinates(xi3, 12, 14, (0.0, 0.0), (1.0, 1.0))
+    np.testing.assert_array_equal(a, np.asfortranarray(ics.bn_disc(np.asarray(X), np.asarray(Y), ics.BN_EOS_REFERENCE,
+                                                                   vin, vout, 0.3, 0.45, 0.55)))
     with pytest.raises(aderdg.AderdgError):  # a Baer-Nunziato formula for an Euler solve
         aderdg.ic_eval(aderdg.ic_bn_circle(), 2, 4, 4, (0.0, 0.0), (1.0, 1.0))

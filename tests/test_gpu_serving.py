@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import aderdg
+from aderdg import ics
 from helpers import BN_EOS_STIFF
 from pyoracle import BC_TRANSMISSIVE, Oracle
 
@@ -25,7 +26,9 @@ def test_device_ic_matches_host():
              ("euler", 2, aderdg.ic_dmr(), (24, 8), (0.0, 0.0), (4.0, 1.0), "dmr", 0.0),
              ("euler", 4, aderdg.ic_vortex(4.0, 4.5, 5.5, 0.5, -0.25), (12, 12), (0.0, 0.0), (10.0, 10.0),
               "periodic", 4e-15),
-             ("bn", 2, aderdg.ic_bn_circle(0.25, 3.0e3), (16, 16), (0.0, 0.0), (1.0, 1.0), "transmissive", 0.0)]
+             ("bn", 2, aderdg.ic_bn_circle(0.25, 3.0e3), (16, 16), (0.0, 0.0), (1.0, 1.0), "transmissive", 0.0),
+             ("bn", 3, aderdg.ic_bn_disc(*ics.BN_RP_STATES[1], 0.2, 0.55, 0.45), (14, 10), (0.0, 0.0), (1.0, 1.0),
+              "transmissive", 0.0)]
     for pde, N in sorted({(c[0], c[1]) for c in cases}):
         b.reserve(pde, N, max_cells=400, max_solves=2)
     for pde, N, ic, (I, J), xl, xr, bc, rtol in cases:
