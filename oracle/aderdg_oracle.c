@@ -41,6 +41,10 @@ struct orc_ctx {
     double *Shi, *Fp, *Gp; /* Baer-Nunziato: time-averaged NCP source and the D+ fluctuations (Fu/Gu hold D-) */
     int *recompute, *status, *oldstatus;
     double *Lh, *NewLh, *lmin, *lmax, *lmin_old, *lmax_old;
+    /* advection-diffusion: EQN%cx, EQN%cy, EQN%a, bC(1..4), predictor flux variant */
+    double ad_c[2], ad_a;
+    int ad_bc[4], ad_fixed;
+    double *gradQbnd; /* (nVar, nDim, 4, n) per cell */
 };
 
 /* PNPMTable of Euler/ader_dg.f90:28 */
@@ -560,13 +564,14 @@ static int pde_positivity(const orc_ctx *c, const double *Q)
 orc_ctx *orc_create(int pde, int N, int imax, int jmax, const double xl[2], const double xr[2], int bc_mode,
                     const double *eos)
 {
-    if ((pde != ORC_PDE_EULER && pde != ORC_PDE_BN) || N < 1 || N > 9 || imax < 1 || jmax < 1) return NULL;
+    if ((pde != ORC_PDE_EULER && pde != ORC_PDE_BN && pde != ORC_PDE_AD) || N < 1 || N > 9 || imax < 1 || jmax < 1)
+        return NULL;
     orc_ctx *c = (orc_ctx *)calloc(1, sizeof(orc_ctx));
     c->pde = pde;
     c->N = N;
     c->n = N + 1;
     c->s = 2 * N + 1;
-    c->V = pde == ORC_PDE_BN ? 9 : 4;
+    c->V = pde == ORC_PDE_BN ? 9 : (pde == ORC_PDE_AD ? 1 : 4);
     c->imax = imax;
     c->jmax = jmax;
     c->bc_mode = bc_mode;
@@ -603,6 +608,7 @@ orc_ctx *orc_create(int pde, int N, int imax, int jmax, const double xl[2], cons
     c->lmax = (double *)calloc(V * nc, sizeof(double));
     c->lmin_old = (double *)calloc(V * nc, sizeof(double));
     c->lmax_old = (double *)calloc(V * nc, sizeof(double));
+    c->gradQbnd = (double *)calloc(2 * 4 * V * n * nc, sizeof(double));
     return c;
 }
 
@@ -612,7 +618,7 @@ void orc_destroy(orc_ctx *c)
     free(c->uh); free(c->olduh); free(c->qhi); free(c->Fhi); free(c->duh); free(c->qbnd); free(c->Fbnd);
     free(c->Fu); free(c->Gu); free(c->recompute); free(c->status); free(c->oldstatus); free(c->Lh);
     free(c->NewLh); free(c->lmin); free(c->lmax); free(c->lmin_old); free(c->lmax_old);
-    free(c->Fp); free(c->Gp); free(c->Shi);
+    free(c->Fp); free(c->Gp); free(c->Shi); free(c->gradQbnd);
     free(c);
 }
 
@@ -1088,8 +1094,10 @@ static void neighbor(const orc_ctx *c, int i, int j, int di, int dj, int *in, in
 }
 
 /* calc_time_step: Euler/calc_time_step.f90:9-52 */
+static double calc_dt_ad(orc_ctx *c);
 double orc_calc_dt(orc_ctx *c)
 {
+    if (c->pde == ORC_PDE_AD) return calc_dt_ad(c);
     const int n = c->n, V = c->V;
     const double nvx[2] = {1.0, 0.0}, nvy[2] = {0.0, 1.0};
     double dt = 1.0e20;
@@ -1659,9 +1667,317 @@ static void subcell_recompute_bn(orc_ctx *c)
     free(subuh0); free(slx); free(sly); free(slt); free(wLx); free(wRx); free(wLy); free(wRy);
 }
 
+/* ------------------------------------------------------------------ */
+/* advection-diffusion (Advection-Diffusion/), nVar = 1, no limiter     */
+/* ------------------------------------------------------------------ */
+
+void orc_set_ad(orc_ctx *c, double cx, double cy, double a, const int bc[4], int fixed)
+{
+    c->ad_c[0] = cx;
+    c->ad_c[1] = cy;
+    c->ad_a = a;
+    for (int k = 0; k < 4; k++) c->ad_bc[k] = bc[k];
+    c->ad_fixed = fixed;
+}
+
+/* PDEViscFlux (Advection-Diffusion/pde.f90:54-64) or PDEFusedViscConvFlux (:115-129) */
+static void ad_flux(const orc_ctx *c, int fused, double q, const double *grad, double *F)
+{
+    const double fv0 = -(c->ad_a * grad[0]), fv1 = -(c->ad_a * grad[1]);
+    if (fused) {
+        F[0] = c->ad_c[0] * q + fv0;
+        F[1] = c->ad_c[1] * q + fv1;
+    } else {
+        F[0] = fv0;
+        F[1] = fv1;
+    }
+}
+
+/* ader_space_time_predictor, Advection-Diffusion/space_time_predictor.f90:10-155 */
+void orc_predictor_ad(orc_ctx *c, double *lqhi, double *lFhi, double *lqbnd, double *lgradqbnd, const double *luh)
+{
+    const int n = c->n;
+    const double *w = c->w;
+    double lqh[NMAX * NMAX * NMAX], rhs0[NMAX * NMAX * NMAX], rhs[NMAX * NMAX * NMAX];
+    double lqx[NMAX * NMAX * NMAX], lqy[NMAX * NMAX * NMAX], lFh[2 * NMAX * NMAX * NMAX];
+    double lgx[NMAX * NMAX], lgy[NMAX * NMAX];
+#define A3(i, j, l) ((i) + n * ((j) + n * (l)))
+    for (int j = 0; j < n; j++) /* :42-58 */
+        for (int i = 0; i < n; i++)
+            for (int l = 0; l < n; l++) {
+                lqh[A3(i, j, l)] = luh[i + n * j];
+                rhs0[A3(i, j, l)] = w[i] * w[j] * c->f0[l] * luh[i + n * j];
+            }
+    for (int iter = 0; iter < n; iter++) { /* :62-112 */
+        for (int l = 0; l < n; l++) {
+            for (int j = 0; j < n; j++) /* :67-69 */
+                for (int i = 0; i < n; i++) {
+                    double acc = 0.0;
+                    for (int k = 0; k < n; k++) acc = acc + lqh[A3(k, j, l)] * c->dudx[i + n * k];
+                    lqx[A3(i, j, l)] = 1.0 / c->dx[0] * acc;
+                }
+            for (int i = 0; i < n; i++) /* :73-75 */
+                for (int j = 0; j < n; j++) {
+                    double acc = 0.0;
+                    for (int k = 0; k < n; k++) acc = acc + lqh[A3(i, k, l)] * c->dudx[j + n * k];
+                    lqy[A3(i, j, l)] = 1.0 / c->dx[1] * acc;
+                }
+            for (int j = 0; j < n; j++) /* :78-85 */
+                for (int i = 0; i < n; i++) {
+                    const double g[2] = {lqx[A3(i, j, l)], lqy[A3(i, j, l)]};
+                    ad_flux(c, c->ad_fixed, lqh[A3(i, j, l)], g, &lFh[2 * A3(i, j, l)]);
+                }
+            for (int j = 0; j < n; j++) { /* :91-93 */
+                const double sx = w[l] * w[j] * c->dt / c->dx[0];
+                for (int i = 0; i < n; i++) {
+                    double acc = 0.0;
+                    for (int k = 0; k < n; k++) acc = acc + lFh[2 * A3(k, j, l)] * c->kxi[k + n * i];
+                    rhs[A3(i, j, l)] = rhs0[A3(i, j, l)] - sx * acc;
+                }
+            }
+            for (int i = 0; i < n; i++) { /* :97-99 */
+                const double sy = w[l] * w[i] * c->dt / c->dx[1];
+                for (int j = 0; j < n; j++) {
+                    double acc = 0.0;
+                    for (int k = 0; k < n; k++) acc = acc + lFh[1 + 2 * A3(i, k, l)] * c->kxi[k + n * j];
+                    rhs[A3(i, j, l)] = rhs[A3(i, j, l)] - sy * acc;
+                }
+            }
+        }
+        for (int j = 0; j < n; j++) /* :105-109 */
+            for (int i = 0; i < n; i++) {
+                const double f = 1.0 / (w[i] * w[j]);
+                double tmp[NMAX];
+                for (int l = 0; l < n; l++) {
+                    double acc = 0.0;
+                    for (int m = 0; m < n; m++) acc = acc + rhs[A3(i, j, m)] * c->ik1[l + n * m];
+                    tmp[l] = f * acc;
+                }
+                for (int l = 0; l < n; l++) lqh[A3(i, j, l)] = tmp[l];
+            }
+    }
+    for (int j = 0; j < n; j++) /* :114-124 time averages */
+        for (int i = 0; i < n; i++) {
+            double a = 0.0, gx = 0.0, gy = 0.0, f0 = 0.0, f1 = 0.0;
+            for (int l = 0; l < n; l++) a = a + lqh[A3(i, j, l)] * w[l];
+            for (int l = 0; l < n; l++) gx = gx + lqx[A3(i, j, l)] * w[l];
+            for (int l = 0; l < n; l++) gy = gy + lqy[A3(i, j, l)] * w[l];
+            for (int l = 0; l < n; l++) f0 = f0 + lFh[2 * A3(i, j, l)] * w[l];
+            for (int l = 0; l < n; l++) f1 = f1 + lFh[1 + 2 * A3(i, j, l)] * w[l];
+            lqhi[i + n * j] = a;
+            lgx[i + n * j] = gx;
+            lgy[i + n * j] = gy;
+            lFhi[0 + 2 * (i + n * j)] = f0;
+            lFhi[1 + 2 * (i + n * j)] = f1;
+        }
+    for (int k = 0; k < 4 * n; k++) lqbnd[k] = 0.0;
+    /* :130-152 boundary extrapolation; lqbnd(1,f,p), lgradqbnd(1,d,f,p) */
+    for (int j = 0; j < n; j++) {
+        double q0 = 0.0, q1 = 0.0, a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+        for (int i = 0; i < n; i++) q0 = q0 + lqhi[i + n * j] * c->flc[i];
+        for (int i = 0; i < n; i++) q1 = q1 + lqhi[i + n * j] * c->frc[i];
+        for (int i = 0; i < n; i++) a0 = a0 + lgx[i + n * j] * c->flc[i];
+        for (int i = 0; i < n; i++) b0 = b0 + lgy[i + n * j] * c->flc[i];
+        for (int i = 0; i < n; i++) a1 = a1 + lgx[i + n * j] * c->frc[i];
+        for (int i = 0; i < n; i++) b1 = b1 + lgy[i + n * j] * c->frc[i];
+        lqbnd[0 + 4 * j] = q0;
+        lqbnd[1 + 4 * j] = q1;
+        lgradqbnd[0 + 2 * (0 + 4 * j)] = a0;
+        lgradqbnd[1 + 2 * (0 + 4 * j)] = b0;
+        lgradqbnd[0 + 2 * (1 + 4 * j)] = a1;
+        lgradqbnd[1 + 2 * (1 + 4 * j)] = b1;
+    }
+    for (int i = 0; i < n; i++) {
+        double q0 = 0.0, q1 = 0.0, a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+        for (int j = 0; j < n; j++) q0 = q0 + lqhi[i + n * j] * c->flc[j];
+        for (int j = 0; j < n; j++) q1 = q1 + lqhi[i + n * j] * c->frc[j];
+        for (int j = 0; j < n; j++) a0 = a0 + lgx[i + n * j] * c->flc[j];
+        for (int j = 0; j < n; j++) b0 = b0 + lgy[i + n * j] * c->flc[j];
+        for (int j = 0; j < n; j++) a1 = a1 + lgx[i + n * j] * c->frc[j];
+        for (int j = 0; j < n; j++) b1 = b1 + lgy[i + n * j] * c->frc[j];
+        lqbnd[2 + 4 * i] = q0;
+        lqbnd[3 + 4 * i] = q1;
+        lgradqbnd[0 + 2 * (2 + 4 * i)] = a0;
+        lgradqbnd[1 + 2 * (2 + 4 * i)] = b0;
+        lgradqbnd[0 + 2 * (3 + 4 * i)] = a1;
+        lgradqbnd[1 + 2 * (3 + 4 * i)] = b1;
+    }
+#undef A3
+}
+
+/* PDEViscLLFRiemannSolver, Advection-Diffusion/pde.f90:135-154 */
+static double ad_llf(const orc_ctx *c, double QL, const double *gL, double QR, const double *gR, const double *nv)
+{
+    const double m_pi = 4.0 * atan(1.0);
+    const double LL = c->ad_c[0] * nv[0] + c->ad_c[1] * nv[1], LR = LL;
+    const double factor = (double)(2 * c->N + 1);
+    const double eta = factor / (c->dx[0] * sqrt(0.5 * m_pi));
+    const double smax_c = f_max(fabs(LL), fabs(LR));
+    const double smax_v = f_max(c->ad_a, c->ad_a);
+    const double smax = smax_c + 2.0 * eta * smax_v;
+    double FL[2], FR[2];
+    ad_flux(c, 1, QL, gL, FL);
+    ad_flux(c, 1, QR, gR, FR);
+    double mm = 0.0;
+    mm = mm + (FL[0] + FR[0]) * nv[0];
+    mm = mm + (FL[1] + FR[1]) * nv[1];
+    return 0.5 * mm - 0.5 * smax * (QR - QL);
+}
+
+/* solve_riemann_problem, Advection-Diffusion/riemann_problem.f90:8-113 */
+static void riemann_ad(orc_ctx *c)
+{
+    const int n = c->n, I = c->imax, J = c->jmax;
+    const double m_pi = 4.0 * atan(1.0);
+#define QB(ci, cj, f, p) (c->qbnd[(f) + 4 * ((p) + n * ((size_t)(ci) + (size_t)I * (cj)))])
+#define GB(ci, cj, f, p) (&c->gradQbnd[2 * ((f) + 4 * ((p) + n * ((size_t)(ci) + (size_t)I * (cj))))])
+    double nv[2] = {1.0, 0.0};
+    for (int j = 0; j < J; j++)
+        for (int i = 0; i <= I; i++)
+            for (int l = 0; l < n; l++) {
+                double QL, QR, gL[2], gR[2];
+                const double *a, *b;
+                if (i == 0) {
+                    const int bc = c->ad_bc[0];
+                    if (bc == 1) {
+                        QL = QR = QB(0, j, 0, l);
+                        a = b = GB(0, j, 0, l);
+                    } else if (bc == 2) { /* :30-33: the state of the last cell, a reference quirk */
+                        const double y = ((double)j + c->xi[l]) * c->dx[1];
+                        QR = QB(I - 1, j, 1, l);
+                        b = GB(I - 1, j, 1, l);
+                        QL = -QR + 2.0 * sin(m_pi * y);
+                        a = b;
+                    } else {
+                        QL = QB(I - 1, j, 1, l);
+                        a = GB(I - 1, j, 1, l);
+                        QR = QB(0, j, 0, l);
+                        b = GB(0, j, 0, l);
+                    }
+                } else if (i == I) {
+                    const int bc = c->ad_bc[1];
+                    QL = QB(I - 1, j, 1, l);
+                    a = GB(I - 1, j, 1, l);
+                    if (bc == 1) {
+                        QR = QL;
+                        b = a;
+                    } else if (bc == 2) {
+                        QR = -QL;
+                        b = a;
+                    } else {
+                        QR = QB(0, j, 0, l);
+                        b = GB(0, j, 0, l);
+                    }
+                } else {
+                    QL = QB(i - 1, j, 1, l);
+                    a = GB(i - 1, j, 1, l);
+                    QR = QB(i, j, 0, l);
+                    b = GB(i, j, 0, l);
+                }
+                gL[0] = a[0]; gL[1] = a[1];
+                gR[0] = b[0]; gR[1] = b[1];
+                c->Fu[l + n * ((size_t)i + (size_t)(I + 1) * j)] = ad_llf(c, QL, gL, QR, gR, nv);
+            }
+    nv[0] = 0.0;
+    nv[1] = 1.0;
+    for (int j = 0; j <= J; j++)
+        for (int i = 0; i < I; i++)
+            for (int k = 0; k < n; k++) {
+                double QL, QR, gL[2], gR[2];
+                const double *a, *b;
+                if (j == 0) {
+                    const int bc = c->ad_bc[2];
+                    if (bc == 1) {
+                        QL = QR = QB(i, 0, 2, k);
+                        a = b = GB(i, 0, 2, k);
+                    } else if (bc == 2) {
+                        QR = QB(i, 0, 2, k);
+                        b = GB(i, 0, 2, k);
+                        QL = -QR;
+                        a = b;
+                    } else {
+                        QL = QB(i, J - 1, 3, k);
+                        a = GB(i, J - 1, 3, k);
+                        QR = QB(i, 0, 2, k);
+                        b = GB(i, 0, 2, k);
+                    }
+                } else if (j == J) {
+                    const int bc = c->ad_bc[3];
+                    QL = QB(i, J - 1, 3, k);
+                    a = GB(i, J - 1, 3, k);
+                    if (bc == 1) {
+                        QR = QL;
+                        b = a;
+                    } else if (bc == 2) {
+                        QR = -QL;
+                        b = a;
+                    } else {
+                        QR = QB(i, 0, 2, k);
+                        b = GB(i, 0, 2, k);
+                    }
+                } else {
+                    QL = QB(i, j - 1, 3, k);
+                    a = GB(i, j - 1, 3, k);
+                    QR = QB(i, j, 2, k);
+                    b = GB(i, j, 2, k);
+                }
+                gL[0] = a[0]; gL[1] = a[1];
+                gR[0] = b[0]; gR[1] = b[1];
+                c->Gu[k + n * ((size_t)i + (size_t)I * j)] = ad_llf(c, QL, gL, QR, gR, nv);
+            }
+#undef QB
+#undef GB
+}
+
+/* calc_time_step, Advection-Diffusion/calc_time_step.f90:9-57 */
+static double calc_dt_ad(orc_ctx *c)
+{
+    const int n = c->n;
+    const double factor = (double)(2 * c->N + 1);
+    double dt = 1.0e20;
+    for (size_t k = 0; k < ncell(c) * n * n; k++) {
+        double denom = 0.0;
+        for (int d = 0; d < 2; d++) {
+            const double lam = c->ad_c[0] * (d == 0 ? 1.0 : 0.0) + c->ad_c[1] * (d == 0 ? 0.0 : 1.0);
+            const double smax_c = fabs(lam), smax_v = fabs(c->ad_a);
+            denom = denom + (smax_c + 2.0 * smax_v * factor / c->dx[d]) / c->dx[d];
+        }
+        dt = f_min(dt, c->cflpnpm / denom);
+    }
+    if (c->time + dt > c->tend) dt = c->tend - c->time;
+    c->dt = dt;
+    return dt;
+}
+
+/* one pass of Advection-Diffusion/main.f90:22-74 */
+static int step_ad(orc_ctx *c)
+{
+    const int n = c->n, I = c->imax, J = c->jmax;
+    const size_t blk = (size_t)n * n;
+    calc_dt_ad(c);
+    for (int cj = 0; cj < J; cj++)
+        for (int ci = 0; ci < I; ci++) {
+            const size_t k = (size_t)ci + (size_t)I * cj;
+            orc_predictor_ad(c, &c->qhi[blk * k], &c->Fhi[2 * blk * k], &c->qbnd[4 * n * k], &c->gradQbnd[8 * n * k],
+                             &c->uh[blk * k]);
+        }
+    for (size_t k = 0; k < ncell(c); k++) volume_impl(c, &c->duh[blk * k], &c->Fhi[2 * blk * k], NULL);
+    riemann_ad(c);
+    for (int cj = 0; cj < J; cj++)
+        for (int ci = 0; ci < I; ci++) {
+            const size_t k = (size_t)ci + (size_t)I * cj;
+            orc_surface(c, &c->duh[blk * k], &c->Fu[n * (ci + (I + 1) * cj)], &c->Fu[n * (ci + 1 + (I + 1) * cj)],
+                        &c->Gu[n * (ci + I * cj)], &c->Gu[n * (ci + I * (cj + 1))]);
+        }
+    for (size_t k = 0; k < ncell(c); k++) orc_update(c, &c->uh[blk * k], &c->duh[blk * k]);
+    c->time = c->time + c->dt;
+    return 0;
+}
+
 /* one pass of the time loop body, Euler/main.f90:28-100 (Baer-Nunziato/main.f90:28-100) */
 int orc_step(orc_ctx *c)
 {
+    if (c->pde == ORC_PDE_AD) return step_ad(c);
     const int V = c->V, n = c->n, s = c->s, I = c->imax, J = c->jmax;
     size_t nc = ncell(c), blk = (size_t)V * n * n;
     int nrec = 0;

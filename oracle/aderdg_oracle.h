@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-enum { ORC_PDE_EULER = 0, ORC_PDE_BN = 1 };
+enum { ORC_PDE_EULER = 0, ORC_PDE_BN = 1, ORC_PDE_AD = 2 };
 enum { ORC_BC_DMR = 0, ORC_BC_PERIODIC = 1, ORC_BC_TRANSMISSIVE = 2 };
 
 typedef struct orc_ctx orc_ctx;
@@ -75,6 +75,13 @@ int orc_step(orc_ctx *c);                       /* returns nRecompute */
 void orc_get_face_fluxes(const orc_ctx *c, double *fu, double *gu);
 
 /* algorithmic floating-point operation counter (+ - * / sqrt count 1 each) */
+/* Advection-diffusion (pde = ORC_PDE_AD, nVar = 1): advection speeds, diffusivity,
+ * boundary codes bC(1..4) of Advection-Diffusion/params.ini (1 Neumann, 2 Dirichlet,
+ * 3 periodic) and the predictor flux: fixed = 0 the reference's PDEViscFlux,
+ * fixed = 1 PDEFusedViscConvFlux.  Call right after orc_create. */
+void orc_set_ad(orc_ctx *c, double cx, double cy, double a, const int bc[4], int fixed);
+void orc_predictor_ad(orc_ctx *c, double *lqhi, double *lFhi, double *lqbnd, double *lgradqbnd, const double *luh);
+
 void orc_reset_opcount(void);
 double orc_get_opcount(void);
 

@@ -89,3 +89,15 @@ BN_FIXES=(
 for n in 1 2 3 4; do
     build_variant bn Baer-Nunziato "$n" ref_harness_bn.f90 "${BN_FIXES[@]}"
 done
+
+# Advection-diffusion: the reference as it is (its predictor evaluates only the
+# viscous flux, Advection-Diffusion/space_time_predictor.f90:78), and a second
+# build whose predictor calls the fused advective + viscous flux
+# (PDEFusedViscConvFlux, Advection-Diffusion/pde.f90:115-129) at that line.
+AD_FIXED=(
+  'space_time_predictor.f90|s/call PDEViscFlux\(lFh/call PDEFusedViscConvFlux(lFh/'
+)
+for n in 1 2 3 4; do
+    build_variant ad Advection-Diffusion "$n" ref_harness_ad.f90
+    build_variant adfix Advection-Diffusion "$n" ref_harness_ad.f90 "${AD_FIXED[@]}"
+done

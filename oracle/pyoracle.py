@@ -64,27 +64,45 @@ class Oracle:
             lib.orc_dmp_margin.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double]
             lib.orc_set_scalars.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double]
             lib.orc_get_opcount.restype = ctypes.c_double
+            lib.orc_set_ad.restype = None
+            lib.orc_set_ad.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       ctypes.POINTER(ctypes.c_int), ctypes.c_int]
             for name in ("orc_set_uh", "orc_get_uh", "orc_get_scalars", "orc_set_limiter", "orc_get_limiter",
                          "orc_predictor", "orc_volume", "orc_surface", "orc_update", "orc_flux", "orc_eigen",
                          "orc_rusanov", "orc_subcell_data", "orc_lobatto_data", "orc_put_subcell_data",
                          "orc_get_ops", "orc_get_face_fluxes", "orc_predictor_bn", "orc_volume_bn",
-                         "orc_rusanov_bn", "orc_ncp", "orc_roe", "orc_prim2cons"):
+                         "orc_rusanov_bn", "orc_ncp", "orc_roe", "orc_prim2cons", "orc_predictor_ad"):
                 getattr(lib, name).restype = None
             cls._lib = lib
         return cls._lib
 
-    def __init__(self, N, imax, jmax, xl, xr, bc_mode, gamma=1.4, pde=0, eos=None):
-        """pde 0: Euler (gamma); pde 1: Baer-Nunziato (eos = GAMMA_S, GAMMA_G, PI_S, PI_G)."""
+    def __init__(self, N, imax, jmax, xl, xr, bc_mode, gamma=1.4, pde=0, eos=None, ad=None):
+        """pde 0: Euler (gamma); pde 1: Baer-Nunziato (eos = GAMMA_S, GAMMA_G, PI_S, PI_G);
+        pde 2: advection-diffusion, ad = dict(c=(cx, cy), a=diffusivity, bc=(bL, bR, bB, bT),
+        fixed=0/1) with the boundary codes of Advection-Diffusion/params.ini."""
         self.pde = pde
-        self.N, self.n, self.s, self.V = N, N + 1, 2 * N + 1, (9 if pde == 1 else 4)
+        self.N, self.n, self.s, self.V = N, N + 1, 2 * N + 1, {0: 4, 1: 9, 2: 1}[pde]
         self.imax, self.jmax = imax, jmax
         L = self.lib()
         xl = np.asarray(xl, dtype=np.float64)
         xr = np.asarray(xr, dtype=np.float64)
-        eos = np.array([gamma] if pde == 0 else list(eos), dtype=np.float64)
+        eos = np.array([gamma] if pde != 1 else list(eos), dtype=np.float64)
         self.h = L.orc_create(pde, N, imax, jmax, _p(xl), _p(xr), bc_mode, _p(eos))
         if not self.h:
             raise ValueError("orc_create failed")
+        if pde == 2:
+            bc = (ctypes.c_int * 4)(*ad["bc"])
+            L.orc_set_ad(ctypes.c_void_p(self.h), float(ad["c"][0]), float(ad["c"][1]), float(ad["a"]), bc,
+                         int(ad.get("fixed", 0)))
+
+    def predictor_ad(self, luh):
+        n = self.n
+        luh = np.asfortranarray(luh, dtype=np.float64)
+        out = dict(qhi=np.zeros((1, n, n), order="F"), Fhi=np.zeros((1, 2, n, n), order="F"),
+                   qbnd=np.zeros((1, 4, n), order="F"), gradqbnd=np.zeros((1, 2, 4, n), order="F"))
+        self.lib().orc_predictor_ad(ctypes.c_void_p(self.h), _p(out["qhi"]), _p(out["Fhi"]), _p(out["qbnd"]),
+                                    _p(out["gradqbnd"]), _p(luh))
+        return out
 
     def __del__(self):
         if getattr(self, "h", None) and self._lib is not None:
@@ -543,3 +561,111 @@ class RefBN(RefEuler):
         a = [np.ascontiguousarray(x, dtype=np.float64) for x in (qa, qb, nv)]
         self.lib_.ref_roe(*[_p(x) for x in a], _p(b))
         return b
+
+
+AD_PARAMS_TEMPLATE = """{cx}   ! cx
+{cy}   ! cy
+{a}   ! a
+{imax}   ! IMAX
+{jmax}   ! JMAX
+{xl0}   ! xL(1)
+{xl1}   ! xL(2)
+{xr0}   ! xR(1)
+{xr1}   ! xR(2)
+{tend}   ! tend
+{b1}   ! bL
+{b2}   ! bR
+{b3}   ! bB
+{b4}   ! bT
+{wi}   ! WriteInterval
+"""
+
+
+class RefAD:
+    """The reference advection-diffusion routines (oracle/_ref/libref_ad_n<N>.so, or
+    libref_adfix_n<N>.so whose predictor evaluates the fused advective + viscous flux).
+    One instance per (variant, N) at a time: the state lives in `module ader_dg`."""
+
+    _libs = {}
+
+    @classmethod
+    def available(cls, N, fixed=False):
+        return os.path.exists(ref_path("adfix" if fixed else "ad", N))
+
+    @classmethod
+    def lib(cls, N, fixed):
+        key = (N, bool(fixed))
+        if key not in cls._libs:
+            lib = ctypes.CDLL(ref_path("adfix" if fixed else "ad", N))
+            lib.ref_set_scalars.argtypes = [ctypes.c_double] * 3
+            lib.ref_set_eqn.argtypes = [ctypes.c_double] * 3 + [ctypes.c_int] * 4
+            cls._libs[key] = lib
+        return cls._libs[key]
+
+    def __init__(self, N, imax, jmax, xl, xr, c, a, bc, fixed=False, tend=1.0e9):
+        self.N, self.n, self.V = N, N + 1, 1
+        self.imax, self.jmax = imax, jmax
+        L = self.lib(N, fixed)
+        txt = AD_PARAMS_TEMPLATE.format(cx=repr(float(c[0])), cy=repr(float(c[1])), a=repr(float(a)), imax=imax,
+                                        jmax=jmax, xl0=repr(float(xl[0])), xl1=repr(float(xl[1])),
+                                        xr0=repr(float(xr[0])), xr1=repr(float(xr[1])), tend=repr(float(tend)),
+                                        b1=bc[0], b2=bc[1], b3=bc[2], b4=bc[3], wi=10 ** 8)
+        cwd = os.getcwd()
+        with tempfile.TemporaryDirectory() as d:
+            with open(os.path.join(d, "params.ini"), "w") as f:
+                f.write(txt)
+            os.chdir(d)
+            try:
+                L.ref_init()
+            finally:
+                os.chdir(cwd)
+        L.ref_set_eqn(float(c[0]), float(c[1]), float(a), *[int(b) for b in bc])
+        L.ref_set_scalars(0.0, 0.0, float(tend))
+        self.lib_ = L
+
+    def uh_shape(self):
+        return (1, self.n, self.n, self.imax, self.jmax)
+
+    def ops(self):
+        n = self.n
+        o = dict(xi=np.zeros(n), w=np.zeros(n), Kxi=np.zeros((n, n), order="F"), dudx=np.zeros((n, n), order="F"),
+                 iK1=np.zeros((n, n), order="F"), F0=np.zeros(n), FLcoeff=np.zeros(n), FRcoeff=np.zeros(n))
+        self.lib_.ref_get_ops(*[_p(o[k]) for k in ("xi", "w", "Kxi", "dudx", "iK1", "F0", "FLcoeff", "FRcoeff")])
+        return o
+
+    def set_scalars(self, dt, time, tend):
+        self.lib_.ref_set_scalars(dt, time, tend)
+
+    def scalars(self):
+        a = np.zeros(3)
+        self.lib_.ref_get_scalars(_p(a))
+        return tuple(a)
+
+    def set_uh(self, uh):
+        uh = np.asfortranarray(uh, dtype=np.float64)
+        assert uh.shape == self.uh_shape()
+        self.lib_.ref_set_uh(_p(uh))
+
+    def get_uh(self):
+        uh = np.zeros(self.uh_shape(), order="F")
+        self.lib_.ref_get_uh(_p(uh))
+        return uh
+
+    def predictor(self, luh):
+        n = self.n
+        luh = np.asfortranarray(luh, dtype=np.float64)
+        out = dict(qhi=np.zeros((1, n, n), order="F"), Fhi=np.zeros((1, 2, n, n), order="F"),
+                   qbnd=np.zeros((1, 4, n), order="F"), gradqbnd=np.zeros((1, 2, 4, n), order="F"))
+        self.lib_.ref_predictor(_p(out["qhi"]), _p(out["Fhi"]), _p(out["qbnd"]), _p(out["gradqbnd"]), _p(luh))
+        return out
+
+    def step(self):
+        self.lib_.ref_step()
+        return 0
+
+    def face_fluxes(self):
+        n, I, J = self.n, self.imax, self.jmax
+        fu = np.zeros((1, n, I + 1, J), order="F")
+        gu = np.zeros((1, n, I, J + 1), order="F")
+        self.lib_.ref_get_face_fluxes(_p(fu), _p(gu))
+        return fu, gu
