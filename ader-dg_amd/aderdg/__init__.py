@@ -22,8 +22,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBDIR = os.path.join(os.path.dirname(HERE), "lib")
 
 BC = {"dmr": 0, "periodic": 1, "transmissive": 2}
-PDE = {"euler": 0, "bn": 1}
-NVAR = {0: 4, 1: 9}
+PDE = {"euler": 0, "bn": 1, "ad": 2}
+NVAR = {0: 4, 1: 9, 2: 1}
 # kernels timed by aderdg_timing_get, in its slot order
 TIMING_SLOTS = ("dt", "predictor", "update", "spread", "subcell", "face")
 
@@ -43,7 +43,9 @@ class _Desc(ctypes.Structure):
     _fields_ = [("pde", ctypes.c_int), ("N", ctypes.c_int), ("imax", ctypes.c_int), ("jmax", ctypes.c_int),
                 ("xl", ctypes.c_double * 2), ("xr", ctypes.c_double * 2), ("bc_mode", ctypes.c_int),
                 ("gamma", ctypes.c_double), ("cfl", ctypes.c_double), ("exact", ctypes.c_int),
-                ("device", ctypes.c_int), ("ops", _Ops), ("bn_eos", ctypes.c_double * 4)]
+                ("device", ctypes.c_int), ("ops", _Ops), ("bn_eos", ctypes.c_double * 4),
+                ("ad_c", ctypes.c_double * 2), ("ad_a", ctypes.c_double), ("ad_bc", ctypes.c_int * 4),
+                ("ad_fixed_predictor", ctypes.c_int), ("ad_pad", ctypes.c_int)]
 
 
 class _BatchDesc(ctypes.Structure):
@@ -70,7 +72,7 @@ class _ServerStats(ctypes.Structure):
                [("busy_seconds", ctypes.c_double)]
 
 
-IC_KIND = {"vortex": 1, "quadrant": 2, "dmr": 3, "bn_circle": 4, "bn_disc": 5}
+IC_KIND = {"vortex": 1, "quadrant": 2, "dmr": 3, "bn_circle": 4, "bn_disc": 5, "sine": 6}
 
 
 def ic_spec(kind, *params):
@@ -102,6 +104,11 @@ def ic_dmr():
                    116.5, 1.4, 0.0, 0.0, 1.0)
 
 
+def ic_sine(amplitude=1.0, kx=1.0, ky=1.0):
+    """A sin(2 pi kx x) sin(2 pi ky y) (advection-diffusion)."""
+    return ic_spec("sine", amplitude, kx, ky)
+
+
 def ic_bn_disc(v_in, v_out, radius=0.2, xc=0.5, yc=0.5):
     """Disc of primitive state v_in (rho_s,u_s,v_s,p_s,rho_g,u_g,v_g,p_g,phi_s) in v_out."""
     return ic_spec("bn_disc", xc, yc, radius, *v_in, *v_out)
@@ -113,12 +120,20 @@ def ic_bn_circle(radius=0.2, pratio=1.0e3, xc=0.5, yc=0.5):
                    1.0 - BN_PHI_EPS, BN_PHI_EPS)
 
 
-def _desc(pde, N, imax, jmax, xl, xr, bc, gamma, cfl, device, eos):
+def _desc(pde, N, imax, jmax, xl, xr, bc, gamma, cfl, device, eos, ad=None):
+    """eos: Baer-Nunziato (GAMMA_S, GAMMA_G, PI_S, PI_G); ad: advection-diffusion
+    dict(c=(cx, cy), a=diffusivity, bc=(bL, bR, bB, bT) codes 1/2/3, fixed=0/1)."""
     d = _Desc()
     d.pde, d.N, d.imax, d.jmax = pde, N, imax, jmax
     if eos is not None:
         for k in range(4):
             d.bn_eos[k] = float(eos[k])
+    if ad is not None:
+        d.ad_c[0], d.ad_c[1] = float(ad["c"][0]), float(ad["c"][1])
+        d.ad_a = float(ad["a"])
+        for k in range(4):
+            d.ad_bc[k] = int(ad["bc"][k])
+        d.ad_fixed_predictor = int(ad.get("fixed", 0))
     d.xl[0], d.xl[1] = xl
     d.xr[0], d.xr[1] = xr
     d.bc_mode = BC[bc] if isinstance(bc, str) else int(bc)
@@ -211,6 +226,8 @@ def load_library(exact=False):
     lib.aderdg_batch_reserved_bytes.argtypes = [vp]
     lib.aderdg_flops_predictor.restype = ctypes.c_double
     lib.aderdg_flops_predictor.argtypes = [ctypes.c_int, ctypes.c_int]
+    lib.aderdg_flops_predictor_ad.restype = ctypes.c_double
+    lib.aderdg_flops_predictor_ad.argtypes = [ctypes.c_int, ctypes.c_int]
     lib.aderdg_flops_update.restype = ctypes.c_double
     lib.aderdg_flops_update.argtypes = [ctypes.c_int, ctypes.c_int]
     lib.aderdg_build_exact.restype = ctypes.c_int
@@ -237,10 +254,13 @@ def operators(N, exact=False):
     return o
 
 
-def flops_per_cell(N, pde="euler"):
+def flops_per_cell(N, pde="euler", fused=False):
+    """(predictor + volume, update) operations per cell-step; `fused` selects the
+    advection-diffusion predictor with the convective flux."""
     lib = load_library(False)
     k = PDE[pde] if isinstance(pde, str) else int(pde)
-    return lib.aderdg_flops_predictor(k, N), lib.aderdg_flops_update(k, N)
+    fp = lib.aderdg_flops_predictor_ad(N, int(fused)) if k == 2 else lib.aderdg_flops_predictor(k, N)
+    return fp, lib.aderdg_flops_update(k, N)
 
 
 class Solver:
@@ -249,12 +269,12 @@ class Solver:
     pde "euler" (gamma) or "bn" (eos = GAMMA_S, GAMMA_G, PI_S, PI_G)."""
 
     def __init__(self, N, imax, jmax, xl, xr, bc="periodic", gamma=1.4, cfl=0.9, exact=False, device=0, ops=None,
-                 pde="euler", eos=None):
+                 pde="euler", eos=None, ad=None):
         self.lib = load_library(exact)
         self.pde = PDE[pde] if isinstance(pde, str) else int(pde)
         self.N, self.n, self.s, self.V = N, N + 1, 2 * N + 1, NVAR[self.pde]
         self.imax, self.jmax = imax, jmax
-        d = _desc(self.pde, N, imax, jmax, xl, xr, bc, gamma, cfl, device, eos)
+        d = _desc(self.pde, N, imax, jmax, xl, xr, bc, gamma, cfl, device, eos, ad)
         self._keep = []
         if ops is not None:
             for k in _Ops._fields_:
@@ -434,13 +454,13 @@ class Batch:
         return int(self.lib.aderdg_batch_reserved_bytes(self.h))
 
     def add(self, uh, N, imax, jmax, xl, xr, bc="periodic", pde="euler", gamma=1.4, cfl=0.9, eos=None, time=0.0,
-            tend=1e30, max_steps=1 << 30, stop_at_tend=False):
+            tend=1e30, max_steps=1 << 30, stop_at_tend=False, ad=None):
         k = PDE[pde] if isinstance(pde, str) else int(pde)
         shape = (NVAR[k], N + 1, N + 1, imax, jmax)
         uh = np.asfortranarray(uh, dtype=np.float64)
         if uh.shape != shape:
             raise ValueError("uh shape %s, expected %s" % (uh.shape, shape))
-        d = _desc(k, N, imax, jmax, xl, xr, bc, gamma, cfl, self.device, eos)
+        d = _desc(k, N, imax, jmax, xl, xr, bc, gamma, cfl, self.device, eos, ad)
         i = ctypes.c_int()
         self._check(self.lib.aderdg_batch_add(self.h, ctypes.byref(d), _p(uh), float(time), float(tend),
                                               int(max_steps), 1 if stop_at_tend else 0, ctypes.byref(i)))
@@ -448,10 +468,10 @@ class Batch:
         return i.value
 
     def add_ic(self, ic, N, imax, jmax, xl, xr, bc="periodic", pde="euler", gamma=1.4, cfl=0.9, eos=None, time=0.0,
-               tend=1e30, max_steps=1 << 30, stop_at_tend=False):
+               tend=1e30, max_steps=1 << 30, stop_at_tend=False, ad=None):
         """Add a solve whose initial state is computed on the device from a formula (ic_spec)."""
         k = PDE[pde] if isinstance(pde, str) else int(pde)
-        d = _desc(k, N, imax, jmax, xl, xr, bc, gamma, cfl, self.device, eos)
+        d = _desc(k, N, imax, jmax, xl, xr, bc, gamma, cfl, self.device, eos, ad)
         i = ctypes.c_int()
         self._check(self.lib.aderdg_batch_add_ic(self.h, ctypes.byref(d), ctypes.byref(ic), float(time), float(tend),
                                                  int(max_steps), 1 if stop_at_tend else 0, ctypes.byref(i)))
@@ -515,7 +535,8 @@ def ic_eval(ic, N, imax, jmax, xl, xr, pde="euler", gamma=1.4, eos=None, exact=F
     """Host evaluation of a formula initial condition: uh(nVar, N+1, N+1, IMAX, JMAX)."""
     lib = load_library(exact)
     k = PDE[pde] if isinstance(pde, str) else int(pde)
-    d = _desc(k, N, imax, jmax, xl, xr, "transmissive", gamma, 0.9, 0, eos)
+    d = _desc(k, N, imax, jmax, xl, xr, "transmissive", gamma, 0.9, 0, eos,
+              dict(c=(0.0, 0.0), a=0.0, bc=(3, 3, 3, 3)) if k == 2 else None)
     uh = np.zeros((NVAR[k], N + 1, N + 1, imax, jmax), order="F")
     rc = lib.aderdg_ic_eval(ctypes.byref(d), ctypes.byref(ic), _p(uh))
     if rc != 0:
@@ -567,10 +588,10 @@ class Server:
         self._check(self.lib.aderdg_server_start(self.h))
 
     def request(self, ic, N, imax, jmax, xl, xr, tend, bc="periodic", pde="euler", gamma=1.4, cfl=0.9, eos=None,
-                max_steps=1 << 30, keep_result=False):
+                max_steps=1 << 30, keep_result=False, ad=None):
         k = PDE[pde] if isinstance(pde, str) else int(pde)
         r = _Request()
-        r.solve = _desc(k, N, imax, jmax, xl, xr, bc, gamma, cfl, self.device, eos)
+        r.solve = _desc(k, N, imax, jmax, xl, xr, bc, gamma, cfl, self.device, eos, ad)
         r.ic = ic
         r.tend, r.max_steps, r.keep_result = float(tend), int(max_steps), 1 if keep_result else 0
         return r

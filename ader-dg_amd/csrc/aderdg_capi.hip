@@ -58,8 +58,16 @@ int fail(int code, const char *fmt, ...)
 int check_desc(const aderdg_desc *d, const char *who)
 {
     if (!d) return fail(ADERDG_EINVAL, "%s: null descriptor", who);
-    if (d->pde != ADERDG_PDE_EULER && d->pde != ADERDG_PDE_BN)
+    if (d->pde != ADERDG_PDE_EULER && d->pde != ADERDG_PDE_BN && d->pde != ADERDG_PDE_AD)
         return fail(ADERDG_EINVAL, "%s: unsupported pde %d", who, d->pde);
+    if (d->pde == ADERDG_PDE_AD) {
+        for (int k = 0; k < 4; k++)
+            if (d->ad_bc[k] < 1 || d->ad_bc[k] > 3)
+                return fail(ADERDG_EINVAL, "%s: advection-diffusion boundary code %d outside 1..3", who, d->ad_bc[k]);
+        if (!(d->ad_a >= 0.0)) return fail(ADERDG_EINVAL, "%s: negative diffusivity", who);
+        if (d->ad_fixed_predictor != 0 && d->ad_fixed_predictor != 1)
+            return fail(ADERDG_EINVAL, "%s: ad_fixed_predictor must be 0 or 1", who);
+    }
     if (d->pde == ADERDG_PDE_BN && d->bc_mode == ADERDG_BC_DMR)
         return fail(ADERDG_EINVAL, "%s: the DMR boundaries exist for the Euler system only", who);
     if (d->pde == ADERDG_PDE_BN && !(d->bn_eos[0] > 1.0 && d->bn_eos[1] > 1.0))
@@ -87,6 +95,11 @@ MeshArgs mesh_from_desc(const aderdg_desc *d)
     m.wrap = d->bc_mode == ADERDG_BC_PERIODIC;
     m.xl[0] = d->xl[0];
     m.xl[1] = d->xl[1];
+    m.adv[0] = d->ad_c[0];
+    m.adv[1] = d->ad_c[1];
+    m.diff = d->ad_a;
+    m.ad_fixed = d->ad_fixed_predictor;
+    for (int k = 0; k < 4; k++) m.ad_bc[k] = d->ad_bc[k];
     m.dx = (d->xr[0] - d->xl[0]) / (double)d->imax;  // Euler/initialize.f90:43
     m.dy = (d->xr[1] - d->xl[1]) / (double)d->jmax;
     m.gamma = d->gamma;
@@ -284,7 +297,7 @@ int aderdg_create(const aderdg_desc *d, aderdg_ctx **out)
     c->N = d->N;
     c->n = d->N + 1;
     c->s = 2 * d->N + 1;
-    c->V = d->pde == ADERDG_PDE_BN ? 9 : 4;
+    c->V = d->pde == ADERDG_PDE_BN ? 9 : (d->pde == ADERDG_PDE_AD ? 1 : 4);
     c->ncell = d->imax * d->jmax;
     Operators ops;
     if ((r = ops_from_desc(d, &ops, "aderdg_create"))) {
@@ -626,11 +639,29 @@ int aderdg_timing_get(aderdg_ctx *c, double ms[ADERDG_TIMING_SLOTS], int64_t lau
     return 0;
 }
 
+namespace {
+// Advection-diffusion predictor + volume integral (Advection-Diffusion/space_time_predictor.f90:10-155,
+// Advection-Diffusion/volume_integral.f90); `fused` adds the convective part of the predictor flux.
+double ad_flops_predictor(int N, bool fused)
+{
+    const double n = N + 1;
+    const double per_slice = 2 * n * n * (2 * n + 2) + n * n * (fused ? 6 : 2) + 2 * n * (3 + n * (2 * n + 2));
+    const double picard = n * (n * per_slice + n * n * (2 + n * (2 * n + 1)));
+    return 3 * n * n * n + picard + 5 * n * n * 2 * n + 2 * n * 6 * 2 * n + 2 * n * n * (2 * n + 3);
+}
+}  // namespace
+
+double aderdg_flops_predictor_ad(int N, int fused)
+{
+    return N >= 1 ? ad_flops_predictor(N, fused != 0) : 0.0;
+}
+
 // Operation counts per cell-step with the oracle's convention (+ - * / sqrt = 1).
 // Predictor launch: Euler/space_time_predictor.f90 + Euler/volume_integral.f90
 // (Baer-Nunziato: + gradients, NCP source and its time average).
 double aderdg_flops_predictor(int pde, int N)
 {
+    if (pde == ADERDG_PDE_AD && N >= 1) return ad_flops_predictor(N, false);
     if ((pde != ADERDG_PDE_EULER && pde != ADERDG_PDE_BN) || N < 1) return 0.0;
     const bool bn = pde == ADERDG_PDE_BN;
     const double n = N + 1, V = bn ? 9 : 4, cF = bn ? 76 : 26, cNcp = 34;

@@ -76,7 +76,8 @@ int64_t Batch::footprint(int pde, int N, int imax, int jmax)
 
 int Batch::reserve(int pde, int N, int64_t max_cells, int max_solves)
 {
-    if (pde != ADERDG_PDE_EULER && pde != ADERDG_PDE_BN) return fail(ADERDG_EINVAL, "aderdg_batch_reserve: bad pde %d", pde);
+    if (pde != ADERDG_PDE_EULER && pde != ADERDG_PDE_BN && pde != ADERDG_PDE_AD)
+        return fail(ADERDG_EINVAL, "aderdg_batch_reserve: bad pde %d", pde);
     if (N < 1 || N > kMaxDegree) return fail(ADERDG_EINVAL, "aderdg_batch_reserve: N=%d outside 1..%d", N, kMaxDegree);
     if (max_cells < 1 || max_cells > (1LL << 28) || max_solves < 1 || max_solves > (1 << 20))
         return fail(ADERDG_EINVAL, "aderdg_batch_reserve: bad capacity (%lld cells, %d solves)", (long long)max_cells,
@@ -115,8 +116,7 @@ int Batch::add(const aderdg_desc *d, const double *uh, const aderdg_ic *ic, doub
     int r = check_desc(d, "aderdg_batch_add");
     if (r) return r;
     if ((!uh && !ic) || !id || max_steps < 1) return fail(ADERDG_EINVAL, "aderdg_batch_add: bad argument");
-    if (ic && ((d->pde == ADERDG_PDE_BN) != (ic->kind == ADERDG_IC_BN_CIRCLE || ic->kind == ADERDG_IC_BN_DISC) ||
-               ic->kind < ADERDG_IC_VORTEX || ic->kind > ADERDG_IC_BN_DISC))
+    if (ic && !ic_fits(d->pde, ic->kind))
         return fail(ADERDG_EINVAL, "aderdg_batch_add_ic: initial condition kind %d does not fit pde %d", ic->kind,
                     d->pde);
     const aderdg_ops &o = d->ops;
@@ -348,7 +348,9 @@ int aderdg_batch_reserve(aderdg_batch *b, int pde, int N, int64_t max_cells, int
 
 int64_t aderdg_solve_footprint(int pde, int N, int imax, int jmax)
 {
-    if ((pde != ADERDG_PDE_EULER && pde != ADERDG_PDE_BN) || N < 1 || N > kMaxDegree || imax < 1 || jmax < 1) return -1;
+    if ((pde != ADERDG_PDE_EULER && pde != ADERDG_PDE_BN && pde != ADERDG_PDE_AD) || N < 1 || N > kMaxDegree ||
+        imax < 1 || jmax < 1)
+        return -1;
     return Batch::footprint(pde, N, imax, jmax);
 }
 

@@ -131,9 +131,10 @@ int Group::dalloc(T **p, size_t count)
 
 void Group::footprint(int pde, int N, int64_t *per_cell, int64_t *per_xface, int64_t *per_yface, int64_t *per_slot)
 {
-    const int64_t V = pde == kPdeBn ? 9 : 4, n = N + 1, s = 2 * N + 1, sides = pde == kPdeBn ? 2 : 1;
+    const int64_t V = pde == kPdeBn ? 9 : (pde == kPdeAd ? 1 : 4), n = N + 1, s = 2 * N + 1;
+    const int64_t sides = pde == kPdeBn ? 2 : 1, ffc = pde == kPdeAd ? 2 : V;
     // uh x2, lduh, fq + ff, lmin/lmax old and new, cell_dt, Lh x2; ints: rc, rc_final, list, status x2, cell_solve
-    const int64_t dbl = 3 * V * n * n + 2 * 4 * V * n + 4 * V + 1 + 2 * V * s * s;
+    const int64_t dbl = 3 * V * n * n + 4 * (V + ffc) * n + 4 * V + 1 + 2 * V * s * s;
     const int64_t ints = 6;
     // work-table entries are a few bytes per cell; counted per cell at one entry per 4 cells
     if (per_cell) *per_cell = 8 * dbl + 4 * ints + 4 * 8 / 4;
@@ -146,7 +147,7 @@ int Group::init(int pde_, int N_, const GroupCaps &c, const Operators &ops_, int
 {
     pde = pde_;
     N = N_;
-    V = pde == kPdeBn ? 9 : 4;
+    V = pde == kPdeBn ? 9 : (pde == kPdeAd ? 1 : 4);
     n = N + 1;
     s = 2 * N + 1;
     caps = c;
@@ -191,7 +192,7 @@ int Group::init(int pde_, int N_, const GroupCaps &c, const Operators &ops_, int
     }
     r = r ? r : dalloc(&b.lduh, V * NN * G);
     r = r ? r : dalloc(&b.fq, 4 * V * n * G);
-    r = r ? r : dalloc(&b.ff, 4 * V * n * G);
+    r = r ? r : dalloc(&b.ff, 4 * (size_t)(pde == kPdeAd ? 2 : V) * n * G);  // advection-diffusion: grad q
     r = r ? r : dalloc(&b.xflux, sides * V * n * (size_t)b.GX);
     r = r ? r : dalloc(&b.yflux, sides * V * n * (size_t)b.GY);
     r = r ? r : dalloc(&b.lmin_old, V * G);

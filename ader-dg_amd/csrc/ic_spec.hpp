@@ -17,6 +17,7 @@
 //             phi_in, phi_out: both phases at rest
 //   BN_DISC   (Baer-Nunziato) xc, yc, radius, then the primitive states
 //             (rho_s,u_s,v_s,p_s,rho_g,u_g,v_g,p_g,phi_s) inside and outside
+//   SINE      (advection-diffusion) A, kx, ky: A sin(2 pi kx x) sin(2 pi ky y)
 #pragma once
 #include <cmath>
 
@@ -29,7 +30,7 @@
 
 namespace aderdg {
 
-enum { kIcVortex = 1, kIcQuadrant = 2, kIcDmr = 3, kIcBnCircle = 4, kIcBnDisc = 5 };
+enum { kIcVortex = 1, kIcQuadrant = 2, kIcDmr = 3, kIcBnCircle = 4, kIcBnDisc = 5, kIcSine = 6 };
 constexpr int kIcParams = 24;
 
 struct IcSpec {
@@ -62,11 +63,24 @@ ADG_HD void ic_bn_cons(const double *eos, const double *V, double *Q)
     Q[8] = V[8];
 }
 
+// Whether a formula kind belongs to a system (0 Euler, 1 Baer-Nunziato, 2 advection-diffusion).
+ADG_HD bool ic_fits(int pde, int kind)
+{
+    if (pde == 1) return kind == kIcBnCircle || kind == kIcBnDisc;
+    if (pde == 2) return kind == kIcSine;
+    return kind == kIcVortex || kind == kIcQuadrant || kind == kIcDmr;
+}
+
 // Conserved state at (x, y); gamma (Euler) or eos (Baer-Nunziato).
 ADG_HD void ic_field(const IcSpec &ic, double gamma, const double *eos, double x, double y, double *Q)
 {
 #pragma clang fp contract(off)
     const double *p = ic.p;
+    if (ic.kind == kIcSine) {
+        const double pi = 3.141592653589793;
+        Q[0] = p[0] * sin(2.0 * pi * p[1] * x) * sin(2.0 * pi * p[2] * y);
+        return;
+    }
     if (ic.kind == kIcBnDisc) {
         const bool inside = (x - p[0]) * (x - p[0]) + (y - p[1]) * (y - p[1]) <= p[2] * p[2];
         ic_bn_cons(eos, p + (inside ? 3 : 12), Q);

@@ -77,11 +77,10 @@ extern "C" int aderdg_initial_condition(int ictype, int N, int imax, int jmax, c
 extern "C" int aderdg_ic_eval(const aderdg_desc *d, const aderdg_ic *ic, double *uh)
 {
     if (!d || !ic || !uh || d->N < 1 || d->N > aderdg::kMaxDegree || d->imax < 1 || d->jmax < 1) return ADERDG_EINVAL;
-    const bool bn = d->pde == ADERDG_PDE_BN;
-    const bool bn_kind = ic->kind == ADERDG_IC_BN_CIRCLE || ic->kind == ADERDG_IC_BN_DISC;
-    if (bn != bn_kind || ic->kind < ADERDG_IC_VORTEX || ic->kind > ADERDG_IC_BN_DISC) return ADERDG_EINVAL;
+    if (!aderdg::ic_fits(d->pde, ic->kind)) return ADERDG_EINVAL;
     aderdg::IcSpec s{};
     s.kind = ic->kind;
     for (int k = 0; k < aderdg::kIcParams; k++) s.p[k] = ic->p[k];
-    return aderdg::ic_fill(s, d->N, bn ? 9 : 4, d->imax, d->jmax, d->xl, d->xr, d->gamma, d->bn_eos, uh);
+    const int V = d->pde == ADERDG_PDE_BN ? 9 : (d->pde == ADERDG_PDE_AD ? 1 : 4);
+    return aderdg::ic_fill(s, d->N, V, d->imax, d->jmax, d->xl, d->xr, d->gamma, d->bn_eos, uh);
 }

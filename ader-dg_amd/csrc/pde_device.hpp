@@ -13,6 +13,22 @@
 
 namespace aderdg {
 
+// Loads of data that no kernel writes while it runs (solve table, work tables,
+// the per-solve state outside k_dt / k_solves, operators) through the constant
+// address space: uniform addresses then become scalar loads into SGPRs.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define ADG_CONST __attribute__((address_space(4)))
+#else  // the host pass only type-checks the device code
+#define ADG_CONST
+#endif
+template <class T>
+ADG_DEV T ld_const(const T *p)
+{
+    return *(const ADG_CONST T *)p;
+}
+// the operator matrices, read-only for the whole run
+using OpPtr = const ADG_CONST Operators *;
+
 // Two-argument MAX/MIN as flang evaluates them: (a > b ? a : b), so a NaN
 // second argument wins and a NaN first argument loses.
 ADG_DEV double f_max(double a, double b) { return a > b ? a : b; }
@@ -43,6 +59,13 @@ ADG_DEV int nb_index(int i, int d, int n, int wrap)
     if (a > n - 1) a = wrap ? 0 : n - 1;
     return a;
 }
+
+// Linear advection-diffusion, Advection-Diffusion/pde.f90 (its kernels are in
+// ad_kernels.hip; this type only sizes the shared initial-condition kernel).
+struct AdPDE {
+    static constexpr int V = 1;
+    static constexpr bool kNcp = false;
+};
 
 // ---------------------------------------------------------------------------
 // 2-D Euler, Euler/pde.f90

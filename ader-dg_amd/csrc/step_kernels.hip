@@ -61,21 +61,7 @@ struct Shape {
 
 ADG_DEV double dmr_shock_x(const MeshArgs &m, double time) { return m.dmr_a + m.dmr_b * time; }
 
-// Loads of data that no kernel writes while it runs (solve table, work tables,
-// the per-solve state outside k_dt / k_solves) through the constant address
-// space: uniform addresses then become scalar loads into SGPRs.
-#if defined(__HIP_DEVICE_COMPILE__)
-#define ADG_CONST __attribute__((address_space(4)))
-#else  // the host pass of this file only type-checks the device code
-#define ADG_CONST
-#endif
-template <class T>
-ADG_DEV T ld_const(const T *p)
-{
-    return *(const ADG_CONST T *)p;
-}
-// the operator matrices, read-only for the whole run
-using OpPtr = const ADG_CONST Operators *;
+
 
 // ---------------------------------------------------------------------------
 // space-time predictor (+ volume integral, face extrapolation, min/max pass 2)
@@ -1475,7 +1461,9 @@ void quanta_pde(int N, int q[kNumTabs])
 
 void table_quanta(int pde, int N, int quanta[kNumTabs])
 {
-    if (pde == kPdeBn)
+    if (pde == kPdeAd)
+        table_quanta_ad(N, quanta);
+    else if (pde == kPdeBn)
         quanta_pde<BnPDE>(N, quanta);
     else
         quanta_pde<EulerPDE>(N, quanta);
@@ -1490,6 +1478,7 @@ hipError_t launch_part(int pde, int N, const StepBufs &b, hipStream_t st, int pa
         hipLaunchKernelGGL(k_solves, dim3(g > 0 ? g : 1), dim3(kBlock), 0, st, b, part);
         return hipGetLastError();
     }
+    if (pde == kPdeAd) return launch_part_ad(N, b, st, part, grid);
     return pde == kPdeBn ? launch_part_pde<BnPDE>(N, b, st, part, grid) : launch_part_pde<EulerPDE>(N, b, st, part, grid);
 }
 
@@ -1514,7 +1503,15 @@ hipError_t launch_init(int pde, int N, const StepBufs &b, const IcSpec &ic, int 
 {
     const long pts = (long)ncell * (N + 1) * (N + 1);
     const dim3 grid((unsigned)((pts + kBlock - 1) / kBlock)), block(kBlock);
-    if (pde == kPdeBn) {
+    if (pde == kPdeAd) {
+        switch (N) {
+        case 1: hipLaunchKernelGGL((k_init<AdPDE, 1>), grid, block, 0, st, b, ic, slot, parity); break;
+        case 2: hipLaunchKernelGGL((k_init<AdPDE, 2>), grid, block, 0, st, b, ic, slot, parity); break;
+        case 3: hipLaunchKernelGGL((k_init<AdPDE, 3>), grid, block, 0, st, b, ic, slot, parity); break;
+        case 4: hipLaunchKernelGGL((k_init<AdPDE, 4>), grid, block, 0, st, b, ic, slot, parity); break;
+        default: return hipErrorInvalidValue;
+        }
+    } else if (pde == kPdeBn) {
         switch (N) {
         case 1: hipLaunchKernelGGL((k_init<BnPDE, 1>), grid, block, 0, st, b, ic, slot, parity); break;
         case 2: hipLaunchKernelGGL((k_init<BnPDE, 2>), grid, block, 0, st, b, ic, slot, parity); break;

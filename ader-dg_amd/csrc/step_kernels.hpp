@@ -21,7 +21,7 @@ constexpr int kBlock = 256;        // spread / dt / face workgroup size
 constexpr int kDtGrid = 256;       // workgroups of the time-step reduction
 constexpr int kSubcellGrid = 4096; // persistent workgroups of the subcell recompute
 
-enum { kPdeEuler = 0, kPdeBn = 1 };
+enum { kPdeEuler = 0, kPdeBn = 1, kPdeAd = 2 };
 enum { kBcDmr = 0, kBcPeriodic = 1, kBcTransmissive = 2 };
 enum { kCntList = 0, kCntTroubled = 1, kCntDtBlocks = 2, kNumCounters = 4 };
 enum { kPartDt = 0, kPartPredictor = 1, kPartUpdate = 2, kPartSpread = 3, kPartSubcell = 4, kPartAdvance = 5,
@@ -34,7 +34,11 @@ enum { kMetaSolves = kNumTabs, kNumMeta = kNumTabs + 1 };
 // Mesh and equation constants of one solve.
 struct MeshArgs {
     int pde, imax, jmax, ncell, bc_mode, wrap;
+    int ad_fixed;              // advection-diffusion: predictor flux (0 viscous only, 1 fused)
+    int ad_bc[4];              // advection-diffusion bC(1..4): 1 Neumann, 2 Dirichlet, 3 periodic
+    int pad;
     double dx, dy, gamma, cflpnpm;
+    double adv[2], diff;       // advection-diffusion EQN%cx, EQN%cy, EQN%a
     double xl[2];              // lower-left corner of the domain
     double eos[4];             // Baer-Nunziato: GAMMA_S, GAMMA_G, PI_S, PI_G
     double dmr_a, dmr_b;       // DMR shock position x_sw = dmr_a + dmr_b * t (Euler/riemann_problem.f90:1-8)
@@ -89,6 +93,9 @@ struct TestBufs {
 
 // Items (cells or face points) per work-table entry of each table, for a (PDE, N).
 void table_quanta(int pde, int N, int quanta[kNumTabs]);
+// the advection-diffusion kernels (ad_kernels.hip)
+void table_quanta_ad(int N, int quanta[kNumTabs]);
+hipError_t launch_part_ad(int N, const StepBufs &b, hipStream_t st, int part, const int grid[kNumTabs]);
 
 // One part of the step for every solve of the group.  grid: workgroups per table-driven
 // kernel (they loop over the table); nslot_cap: solve slots of the group.

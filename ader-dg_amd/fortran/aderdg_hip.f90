@@ -13,7 +13,7 @@ module aderdg_hip
     use iso_c_binding
     implicit none
 
-    integer(c_int), parameter :: ADERDG_PDE_EULER = 0, ADERDG_PDE_BN = 1
+    integer(c_int), parameter :: ADERDG_PDE_EULER = 0, ADERDG_PDE_BN = 1, ADERDG_PDE_AD = 2
     integer(c_int), parameter :: ADERDG_BC_DMR = 0, ADERDG_BC_PERIODIC = 1, ADERDG_BC_TRANSMISSIVE = 2
 
     type, bind(C) :: aderdg_ops_t
@@ -31,6 +31,9 @@ module aderdg_hip
         integer(c_int) :: exact = 0, device = 0
         type(aderdg_ops_t) :: ops
         real(c_double) :: bn_eos(4) = (/ 1.4d0, 1.4d0, 0.0d0, 0.0d0 /)   ! GAMMA_S, GAMMA_G, PI_S, PI_G
+        real(c_double) :: ad_c(2) = 0.0d0, ad_a = 0.0d0                   ! EQN%cx, EQN%cy, EQN%a
+        integer(c_int) :: ad_bc(4) = 3                                     ! bC(1..4)
+        integer(c_int) :: ad_fixed_predictor = 0, ad_pad = 0
     end type
 
     interface

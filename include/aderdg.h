@@ -34,9 +34,11 @@ extern "C" {
 #define ADERDG_ECAPACITY -5/* a fixed-size buffer is too small */
 #define ADERDG_EFULL -6    /* no room for another solve in a batch arena (admission refused) */
 
-/* 2-D Euler (nVar 4, Euler/) and 2-D Baer-Nunziato two-phase flow (nVar 9,
- * Baer-Nunziato/, path-conservative fluxes, transmissive or periodic boundaries) */
-enum aderdg_pde { ADERDG_PDE_EULER = 0, ADERDG_PDE_BN = 1 };
+/* 2-D Euler (nVar 4, Euler/), 2-D Baer-Nunziato two-phase flow (nVar 9,
+ * Baer-Nunziato/, path-conservative fluxes, transmissive or periodic
+ * boundaries) and 2-D linear advection-diffusion (nVar 1,
+ * Advection-Diffusion/, viscous LLF fluxes, no limiter) */
+enum aderdg_pde { ADERDG_PDE_EULER = 0, ADERDG_PDE_BN = 1, ADERDG_PDE_AD = 2 };
 /* Boundary modes.  DMR reproduces the reference at HEAD exactly
  * (Euler/riemann_problem.f90:10-116 and the ghost overrides of
  * Euler/limiter.f90:491-522).  PERIODIC wraps both the face coupling and the
@@ -77,6 +79,15 @@ typedef struct {
     aderdg_ops ops; /* optional host operator matrices                   */
     double bn_eos[4]; /* Baer-Nunziato EQN: GAMMA_S, GAMMA_G, PI_S, PI_G
                          (Baer-Nunziato/initialize.f90:23-26)            */
+    double ad_c[2];   /* advection-diffusion EQN%cx, EQN%cy              */
+    double ad_a;      /* EQN%a, the diffusivity (Advection-Diffusion/ader_dg.f90:74) */
+    int ad_bc[4];     /* bC(1..4) left, right, bottom, top: 1 Neumann, 2 Dirichlet,
+                         3 periodic (Advection-Diffusion/riemann_problem.f90:8-113);
+                         bc_mode is not used for this system             */
+    int ad_fixed_predictor; /* 0: the reference predictor, which evaluates only
+                         the viscous flux (Advection-Diffusion/space_time_predictor.f90:78);
+                         1: PDEFusedViscConvFlux there                    */
+    int ad_pad;
 } aderdg_desc;
 
 /* Setup: replaces the allocation and operator part of `initialize`
@@ -176,8 +187,12 @@ int aderdg_timing_get(aderdg_ctx *ctx, double ms[ADERDG_TIMING_SLOTS], int64_t l
 
 /* Algorithmic floating-point operations of one cell-step of the predictor
  * launch (predictor + volume integral) and of the update launch, counted as
- * + - * / sqrt = 1 (the convention of the oracle's op counter). */
+ * + - * / sqrt = 1 (the convention of the oracle's op counter).  For
+ * advection-diffusion the predictor count is that of the reference's
+ * viscous-only predictor flux; aderdg_flops_predictor_ad(N, 1) counts the
+ * fused convective + viscous flux, and the update count is 0 (not modelled). */
 double aderdg_flops_predictor(int pde, int N);
+double aderdg_flops_predictor_ad(int N, int fused);
 double aderdg_flops_update(int pde, int N);
 
 /* ------------------------------------------------------------------------
@@ -208,9 +223,10 @@ typedef struct aderdg_batch aderdg_batch;
  *   BN_DISC   (Baer-Nunziato) xc, yc, radius, then the primitive states
  *                      (rho_s,u_s,v_s,p_s,rho_g,u_g,v_g,p_g,phi_s) inside and
  *                      outside (e.g. the two sides of a Riemann problem of
- *                      Baer-Nunziato/initialize.f90:314-390) */
+ *                      Baer-Nunziato/initialize.f90:314-390)
+ *   SINE      (advection-diffusion) amplitude, kx, ky: A sin(2 pi kx x) sin(2 pi ky y) */
 enum aderdg_ic_kind { ADERDG_IC_VORTEX = 1, ADERDG_IC_QUADRANT = 2, ADERDG_IC_DMR = 3, ADERDG_IC_BN_CIRCLE = 4,
-                      ADERDG_IC_BN_DISC = 5 };
+                      ADERDG_IC_BN_DISC = 5, ADERDG_IC_SINE = 6 };
 typedef struct {
     int kind;
     int pad;

@@ -1708,8 +1708,10 @@ void orc_predictor_ad(orc_ctx *c, double *lqhi, double *lFhi, double *lqbnd, dou
                 lqh[A3(i, j, l)] = luh[i + n * j];
                 rhs0[A3(i, j, l)] = w[i] * w[j] * c->f0[l] * luh[i + n * j];
             }
+    OPS(3 * n * n * n);
     for (int iter = 0; iter < n; iter++) { /* :62-112 */
         for (int l = 0; l < n; l++) {
+            OPS(2 * n * n * (2 * n + 2) + n * n * (c->ad_fixed ? 6 : 2) + 2 * n * (3 + n * (2 * n + 2)));
             for (int j = 0; j < n; j++) /* :67-69 */
                 for (int i = 0; i < n; i++) {
                     double acc = 0.0;
@@ -1755,7 +1757,9 @@ void orc_predictor_ad(orc_ctx *c, double *lqhi, double *lFhi, double *lqbnd, dou
                 }
                 for (int l = 0; l < n; l++) lqh[A3(i, j, l)] = tmp[l];
             }
+        OPS(n * n * (2 + n * (2 * n + 1)));
     }
+    OPS(5 * n * n * 2 * n + 2 * n * 6 * 2 * n);
     for (int j = 0; j < n; j++) /* :114-124 time averages */
         for (int i = 0; i < n; i++) {
             double a = 0.0, gx = 0.0, gy = 0.0, f0 = 0.0, f1 = 0.0;

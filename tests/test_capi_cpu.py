@@ -120,3 +120,27 @@ def test_formula_ic_host_matches_python():
                                                                    vin, vout, 0.3, 0.45, 0.55)))
     with pytest.raises(aderdg.AderdgError):  # a Baer-Nunziato formula for an Euler solve
         aderdg.ic_eval(aderdg.ic_bn_circle(), 2, 4, 4, (0.0, 0.0), (1.0, 1.0))
+
+
+def test_ad_descriptor_validation():
+    """Advection-diffusion descriptors: boundary codes outside 1..3 (Advection-Diffusion/params.ini),
+    a negative diffusivity and a predictor switch other than 0/1 are refused before any device work."""
+    ok = dict(c=(1.0, 0.5), a=0.01, bc=(3, 3, 3, 3), fixed=0)
+    for bad in (dict(ok, bc=(0, 3, 3, 3)), dict(ok, bc=(3, 3, 3, 4)), dict(ok, a=-1e-3), dict(ok, fixed=2)):
+        with pytest.raises(aderdg.AderdgError, match="advection-diffusion|diffusivity|ad_fixed"):
+            aderdg.Solver(2, 8, 8, (0.0, 0.0), (1.0, 1.0), pde="ad", ad=bad)
+
+
+def test_ad_sine_ic_and_footprint():
+    """The sine initial condition of the advection-diffusion case, A sin(2 pi kx x) sin(2 pi ky y)
+    at the nodes, and a one-variable arena footprint below the Euler one."""
+    N, I, J = 3, 10, 6
+    xi = aderdg.operators(N)["xi"]
+    u = aderdg.ic_eval(aderdg.ic_sine(0.7, 1.0, 2.0), N, I, J, (0.0, 0.0), (1.0, 1.0), pde="ad")
+    assert u.shape == (1, N + 1, N + 1, I, J)
+    x = (np.arange(I)[None, None, :, None] + xi[:, None, None, None]) / I
+    y = (np.arange(J)[None, None, None, :] + xi[None, :, None, None]) / J
+    want = 0.7 * np.sin(2 * np.pi * x) * np.sin(4 * np.pi * y)
+    np.testing.assert_allclose(u[0], want, rtol=0, atol=1e-15)
+    f = aderdg.solve_footprint
+    assert 0 < f("ad", 3, 64, 64) < f("euler", 3, 64, 64)

@@ -166,3 +166,19 @@ def test_bn_flop_model_matches_op_counter():
         counted = Oracle.opcount()
         fp, fu = aderdg.flops_per_cell(N, "bn")
         assert counted == fp, (N, counted, fp)
+
+
+def test_ad_flop_model_matches_op_counter():
+    """Advection-diffusion predictor + volume flop count (both predictor fluxes) equals the oracle's counter."""
+    for N in (1, 2, 3, 4):
+        for fused in (0, 1):
+            o = Oracle(N, 4, 4, (0.0, 0.0), (1.0, 1.0), 1, pde=2,
+                       ad=dict(c=(1.0, 0.5), a=0.01, bc=(3, 3, 3, 3), fixed=fused))
+            o.set_scalars(0.001, 0.0, 1e9)
+            luh = np.random.default_rng(N).standard_normal((1, N + 1, N + 1))
+            Oracle.reset_opcount()
+            p = o.predictor_ad(luh)
+            o.volume(p["Fhi"])
+            counted = Oracle.opcount()
+            fp, fu = aderdg.flops_per_cell(N, "ad", fused=bool(fused))
+            assert counted == fp, (N, fused, counted, fp)
