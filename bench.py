@@ -20,7 +20,11 @@ advanced one step (nVar*(N+1)^2 per cell-step).  Inputs are resident in HBM
 before the timed region starts; uploads are not timed.
 
 Workloads `euler_vortex_512_n4` and `euler_quadrant_1024_n3` time one plain
-solve.
+solve.  `ad_sine_64_n3` runs the linear advection-diffusion problem (periodic
+[0,1]^2, 64x64 cells, N=3, c=(1,1), diffusivity 0.01, u0 = sin 2 pi x sin 2 pi y,
+CFL 0.9) to T=1 with the reference's viscous-only predictor flux and with the
+fused advective + viscous one, and reports each run's L2 error against the exact
+decaying solution.
 
 Prints ONE JSON line.  Per-kernel details go to stderr.
 """
@@ -46,6 +50,8 @@ PLAIN = {
     "euler_vortex_512_n4": ("vortex", 4, 512, 512, (0.0, 0.0), (10.0, 10.0), "periodic"),
     "euler_quadrant_1024_n3": ("quadrant", 3, 1024, 1024, (0.0, 0.0), (1.0, 1.0), "transmissive"),
 }
+# name: (N, cells per side, advection velocity, diffusivity, end time)
+AD = {"ad_sine_64_n3": (3, 64, (1.0, 1.0), 0.01, 1.0)}
 ENSEMBLES = {"bn_ensemble_256": dict(count=256, seed=2026)}
 SERVING = {"serving_4000": dict(count=4000, seed=7, steps=20)}
 
@@ -82,6 +88,37 @@ def cpu_plain(wl, target_seconds):
     dof = steps * si * sj * 4 * (N + 1) ** 2
     return {"value": dof / el, "unit": "element-DOF updates/s", "cores": 1, "kind": kind,
             "sample": "%s IC, %dx%d cells, N=%d, %s, %d steps in %.1f s, one process" % (ic, si, sj, N, bc, steps, el)}
+
+
+def cpu_ad(wl, target_seconds):
+    """The reference advection-diffusion routines on the workload itself (64x64 is the
+    workload's own mesh), both predictor builds in turn, steps until target_seconds."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import aderdg
+    import pyoracle
+    N, side, c, a, T = AD[wl]
+    uh = aderdg.ic_eval(aderdg.ic_sine(1.0, 1.0, 1.0), N, side, side, (0.0, 0.0), (1.0, 1.0), pde="ad")
+    steps, el, kinds = 0, 0.0, set()
+    for fixed in (False, True):
+        if pyoracle.RefAD.available(N, fixed):
+            s = pyoracle.RefAD(N, side, side, (0.0, 0.0), (1.0, 1.0), c, a, (3, 3, 3, 3), fixed=fixed)
+            kinds.add("reference")
+        else:
+            s = pyoracle.Oracle(N, side, side, (0.0, 0.0), (1.0, 1.0), 1, pde=2,
+                                ad=dict(c=c, a=a, bc=(3, 3, 3, 3), fixed=int(fixed)))
+            kinds.add("port")
+        s.set_uh(uh)
+        s.set_scalars(0.0, 0.0, T)
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < target_seconds / 2:
+            s.step()
+            steps += 1
+        el += time.perf_counter() - t0
+    dof = steps * side * side * (N + 1) ** 2
+    return {"value": dof / el, "unit": "element-DOF updates/s", "cores": 1,
+            "kind": "reference" if kinds == {"reference"} else "port",
+            "sample": "the workload's first %d steps (both predictor builds, half the time each) in %.1f s, one process"
+                      % (steps, el)}
 
 
 def _cpu_bn_worker(args):
@@ -163,6 +200,64 @@ def run_plain(args):
         "config": {"workload": args.workload, "pde": "euler", "ic": ic, "N": N, "cells": [I, J], "bc": bc,
                    "cell_updates_per_s": args.steps * ncell / el},
         "roofline": roof,
+    }
+
+
+def run_ad(args):
+    """Both predictor fluxes to the end time; `value` over the two runs together."""
+    import math
+    import numpy as np
+    import aderdg
+    N, side, c, a, T = AD[args.workload]
+    ops = aderdg.operators(N)
+    xi, w = ops["xi"], ops["w"]
+    uh0 = aderdg.ic_eval(aderdg.ic_sine(1.0, 1.0, 1.0), N, side, side, (0.0, 0.0), (1.0, 1.0), pde="ad")
+    ii = np.arange(side)
+    X = (ii[None, None, :, None] + xi[:, None, None, None]) / side
+    Y = (ii[None, None, None, :] + xi[None, :, None, None]) / side
+    avg = lambda q: np.einsum("k,l,klij->ij", w, w, q)
+    runs, total_steps, total_s = {}, 0, 0.0
+    for name, fixed in (("reference_predictor", 0), ("fused", 1)):
+        s = aderdg.Solver(N, side, side, (0.0, 0.0), (1.0, 1.0), "periodic", pde="ad",
+                          ad=dict(c=c, a=a, bc=(3, 3, 3, 3), fixed=fixed))
+        s.upload(uh0)
+        s.run(args.warmup, 0.0, T)  # warm-up on a copy of the start state, then restart
+        s.upload(uh0)
+        t0 = time.perf_counter()
+        t, dt, _ = s.run(1, 0.0, T)
+        steps = 1
+        while t < T:  # dt is the same every step until the last one is clipped to T
+            k = max(1, int((T - t) / dt) - 1)
+            t, dt, _ = s.run(k, t, T)
+            steps += k
+        el = time.perf_counter() - t0
+        uh = s.download()
+        s.timing(True)
+        s.run(20, 0.0, 1e30)
+        ms, launches = s.timing_get()
+        s.close()
+        ex = math.exp(-8 * math.pi ** 2 * a * t) * np.sin(2 * math.pi * (X - c[0] * t)) * np.sin(2 * math.pi * (Y - c[1] * t))
+        per = {nm: (ms[k] / launches[k] if launches[k] else None) for k, nm in enumerate(aderdg.TIMING_SLOTS)}
+        runs[name] = {"steps": steps, "time": t, "seconds": el, "ms_per_step": el / steps * 1e3,
+                      "l2_nodal": float(np.sqrt(np.mean((uh[0] - ex) ** 2))),
+                      "l2_cell_average": float(np.sqrt(np.mean((avg(uh[0]) - avg(ex)) ** 2))),
+                      "kernel_ms": per}
+        log("%s: %d steps to t=%.6g in %.3f s; L2 %.4g; kernels %s"
+            % (name, steps, t, el, runs[name]["l2_cell_average"], json.dumps(per)))
+        total_steps += steps
+        total_s += el
+    ncell = side * side
+    fl, _ = aderdg.flops_per_cell(N, "ad", fused=True)
+    pm = runs["fused"]["kernel_ms"]["predictor"]
+    return {
+        "metric": "element-DOF updates/s (plain single-instance advection-diffusion step)",
+        "value": total_steps * ncell * (N + 1) ** 2 / total_s, "unit": "element-DOF updates/s", "n_gpus": 1,
+        "steps": total_steps, "warmup": args.warmup, "ms_per_step": total_s / total_steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (analytic initial condition)",
+        "config": {"workload": args.workload, "pde": "advection-diffusion", "N": N, "cells": [side, side],
+                   "c": list(c), "diffusivity": a, "tend": T, "bc": "periodic", "runs": runs},
+        "roofline": roofline_of(fl, ncell, pm, "k_predictor_ad<%d> (fused flux)" % N) if pm else None,
     }
 
 
@@ -341,7 +436,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20 ensemble, 200 plain)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (default 2 ensemble, 10 plain)")
-    ap.add_argument("--workload", default="bn_ensemble_256", choices=sorted(PLAIN) + sorted(ENSEMBLES) + sorted(SERVING))
+    ap.add_argument("--workload", default="bn_ensemble_256", choices=sorted(PLAIN) + sorted(AD) + sorted(ENSEMBLES) + sorted(SERVING))
     ap.add_argument("--serving-requests", type=int, default=400,
                     help="ensemble workload: requests of the serving stream timed beside it (0: none)")
     ap.add_argument("--count", type=int, default=None, help="ensemble size (default 256)")
@@ -396,6 +491,10 @@ def main():
         if args.serving_requests:
             sspecs = ics.serving_requests(**dict(SERVING["serving_4000"], count=args.serving_requests))
             out["serving"] = run_serving(sspecs, (0.8,))
+    elif args.workload in AD:
+        out = run_ad(args)
+        if not args.no_cpu:
+            cpu = cpu_ad(args.workload, args.cpu_seconds)
     else:
         out = run_plain(args)
         if not args.no_cpu:
