@@ -73,7 +73,7 @@ __global__ __launch_bounds__(kAdBlock) void k_predictor_ad(StepBufs b)
         du_j[k] = op->dudx[j + n * k];
     }
     const int ntab = ld_const(b.meta + kTabPred);
-    for (int e = blockIdx.x; e < ntab; e += gridDim.x) {
+    for (int e = xcd_entry(blockIdx.x, gridDim.x); e < ntab; e += gridDim.x) {
         const int2 w = ld_const(b.tab[kTabPred] + e);
         const SolveState st = ld_const(b.state + w.x);
         if (!st.active) continue;
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(kBlock) void k_face_ad(StepBufs b)
 {
     constexpr int n = N + 1;
     const int ntab = ld_const(b.meta + kTabFace);
-    for (int e = blockIdx.x; e < ntab; e += gridDim.x) {
+    for (int e = xcd_entry(blockIdx.x, gridDim.x); e < ntab; e += gridDim.x) {
         const int2 w = ld_const(b.tab[kTabFace] + e);
         const SolveState st = ld_const(b.state + w.x);
         if (!st.active) continue;
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(kAdBlock) void k_update_ad(StepBufs b)
     const int tid = threadIdx.x, lc = tid / NN, tt = tid - lc * NN;
     const int i = tt % n, j = tt / n;
     const int ntab = ld_const(b.meta + kTabUpd);
-    for (int e = blockIdx.x; e < ntab; e += gridDim.x) {
+    for (int e = xcd_entry(blockIdx.x, gridDim.x); e < ntab; e += gridDim.x) {
         const int2 w = ld_const(b.tab[kTabUpd] + e);
         const SolveState st = ld_const(b.state + w.x);
         if (SEED ? !st.seed : !st.active) continue;

@@ -29,6 +29,19 @@ ADG_DEV T ld_const(const T *p)
 // the operator matrices, read-only for the whole run
 using OpPtr = const ADG_CONST Operators *;
 
+// Workgroups of a launch go to the chip's 8 XCDs in turn (workgroup k to XCD
+// k mod 8), each XCD with its own L2.  Work-table entries next to each other
+// touch the same cache lines (neighbouring cells of the per-cell and face
+// planes), so each round of gridDim.x entries is dealt out as 8 contiguous
+// ranges, one per XCD: workgroup k takes entry (k mod 8) * (G/8) + k / 8 of the
+// round (a permutation of the round; the last G mod 8 workgroups keep their own).
+constexpr int kXcds = 8;
+ADG_DEV int xcd_entry(int blk, int nblk)
+{
+    const int per = nblk / kXcds;
+    return blk < per * kXcds ? (blk % kXcds) * per + blk / kXcds : blk;
+}
+
 // Two-argument MAX/MIN as flang evaluates them: (a > b ? a : b), so a NaN
 // second argument wins and a NaN first argument loses.
 ADG_DEV double f_max(double a, double b) { return a > b ? a : b; }

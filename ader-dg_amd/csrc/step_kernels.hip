@@ -54,9 +54,14 @@ struct Shape {
     static constexpr int CPWU = UPD_BLOCK / NN;   // cells per update workgroup
     // predictor LDS per cell and time slice: Fx, Fy (and q for the NCP gradients)
     static constexpr int SLOT = (P::kNcp ? 3 : 2) * V * NN;
-    // per-cell LDS of the update: suh | limy | loby | per-thread partial ranges | reduced ranges
-    static constexpr int NPART = 6 * V + 1;  // (min,max) of nodal / Lobatto / subcell values per v, dt
-    static constexpr int UPD_LDS = V * NN + V * n * s + V * NN + NPART * NN + NPART;
+    // per-cell LDS of the update: [suh | limy | loby], reused for the per-thread partials
+    // (nodal, Lobatto, subcell min, subcell max per v, dt) once the maps are read, then the
+    // reduced ranges (min,max) of nodal / Lobatto / subcell values per v and dt
+    static constexpr int NPART = 6 * V + 1;
+    static constexpr int NSRC = 4 * V + 1;
+    static constexpr int UPD_MAPS = V * NN + V * n * s + V * NN;
+    static constexpr int UPD_PART = (UPD_MAPS > NSRC * NN ? UPD_MAPS : NSRC * NN);
+    static constexpr int UPD_LDS = UPD_PART + NPART;
 };
 
 ADG_DEV double dmr_shock_x(const MeshArgs &m, double time) { return m.dmr_a + m.dmr_b * time; }
@@ -322,7 +327,7 @@ __global__ __launch_bounds__(kPredBlock<P>) void k_predictor(StepBufs b)
         b.counters[kCntTroubled] = 0;
     }
     const int ntab = ld_const(b.meta + kTabPred);
-    for (int e = blockIdx.x; e < ntab; e += gridDim.x) {
+    for (int e = xcd_entry(blockIdx.x, gridDim.x); e < ntab; e += gridDim.x) {
         const int2 w = ld_const(b.tab[kTabPred] + e);
         const SolveState st = ld_const(b.state + w.x);
         if (!st.active) continue;
@@ -352,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void k_face(StepBufs b)
 {
     constexpr int V = P::V, n = N + 1;
     const int ntab = ld_const(b.meta + kTabFace);
-    for (int e = blockIdx.x; e < ntab; e += gridDim.x) {
+    for (int e = xcd_entry(blockIdx.x, gridDim.x); e < ntab; e += gridDim.x) {
         const int2 w = ld_const(b.tab[kTabFace] + e);
         const SolveState st = ld_const(b.state + w.x);
         if (!st.active) continue;
@@ -584,7 +589,7 @@ __global__ __launch_bounds__(kUpdBlock<P>) void k_update(StepBufs b)
     const bool own = lc < CPW;
     const int lcs = own ? lc : CPW - 1;
     const int ntab = ld_const(b.meta + kTabUpd);
-    for (int e = blockIdx.x; e < ntab; e += gridDim.x) {
+    for (int e = xcd_entry(blockIdx.x, gridDim.x); e < ntab; e += gridDim.x) {
     const int2 w = ld_const(b.tab[kTabUpd] + e);
     const SolveState st = ld_const(b.state + w.x);
     if (SEED ? !st.seed : !st.active) continue;
@@ -598,8 +603,8 @@ __global__ __launch_bounds__(kUpdBlock<P>) void k_update(StepBufs b)
     double *suh = &lds[lcs][0];
     double *sty = suh + V * NN;     // limy [v][a + n q]
     double *sby = sty + V * n * s;  // loby
-    double *spart = sby + V * NN;   // [slot][tt]
-    double *sred = spart + NP * NN; // [slot]
+    double *spart = suh;            // [source][tt], after the maps are read
+    double *sred = suh + S::UPD_PART; // [slot]
     if (own && tt == 0) sflag[lc] = 0;
 
     double un[V];
@@ -637,11 +642,11 @@ __global__ __launch_bounds__(kUpdBlock<P>) void k_update(StepBufs b)
     if (own) cell_ranges_stage1<P, N>(op, suh, sty, sby, tt, NN);
     __syncthreads();
     // second passes folded into per-thread (min, max) of Lobatto and subcell values
-    double b0[V], b1[V], l0[V], l1[V];
+    double b0[V], l0[V], l1[V];
     bool bad = false;
 #pragma unroll
     for (int v = 0; v < V; v++) {
-        b0[v] = b1[v] = l0[v] = l1[v] = __builtin_nan("");
+        b0[v] = l0[v] = l1[v] = __builtin_nan("");
     }
     if (act) {
         for (int task = tt; task < SS; task += NN) {  // get_subcell_data :33-35, positivity :227-241
@@ -666,28 +671,29 @@ __global__ __launch_bounds__(kUpdBlock<P>) void k_update(StepBufs b)
 #pragma unroll
                 for (int a = 0; a < n; a++) acc = acc + sby[v * NN + a + n * qq] * op->u2b[p + n * a];
                 b0[v] = acc;
-                b1[v] = acc;
             }
         }
     }
+    __syncthreads();  // limy / loby are read: their space takes the partials
     if (own) {
 #pragma unroll
         for (int v = 0; v < V; v++) {
             spart[(0 * V + v) * NN + tt] = un[v];
-            spart[(1 * V + v) * NN + tt] = un[v];
-            spart[(2 * V + v) * NN + tt] = b0[v];
-            spart[(3 * V + v) * NN + tt] = b1[v];
-            spart[(4 * V + v) * NN + tt] = l0[v];
-            spart[(5 * V + v) * NN + tt] = l1[v];
+            spart[(1 * V + v) * NN + tt] = b0[v];  // = b1: one Lobatto node per thread
+            spart[(2 * V + v) * NN + tt] = l0[v];
+            spart[(3 * V + v) * NN + tt] = l1[v];
         }
-        spart[6 * V * NN + tt] = dtc;
+        spart[4 * V * NN + tt] = dtc;
     }
     if (bad && !SEED) atomicOr(&sflag[lc], 1);
     __syncthreads();
     for (int slot = tt; own && slot < NP; slot += NN) {  // reduce the partials of the cell's threads
-        const bool is_max = slot < 6 * V && ((slot / V) & 1);
+        // slot kind / V: 0 min, 1 max nodal; 2 min, 3 max Lobatto; 4 min, 5 max subcell; 6V: dt
+        const int kind = slot / V, v = slot - kind * V;
+        const bool is_max = slot < 6 * V && (kind & 1);
+        const int src = slot < 6 * V ? (kind < 4 ? kind >> 1 : kind - 2) * V + v : 4 * V;
         double r = __builtin_nan("");
-        for (int k = 0; k < NN; k++) r = is_max ? fold_max(r, spart[slot * NN + k]) : fold_min(r, spart[slot * NN + k]);
+        for (int k = 0; k < NN; k++) r = is_max ? fold_max(r, spart[src * NN + k]) : fold_min(r, spart[src * NN + k]);
         sred[slot] = r;
     }
     __syncthreads();
@@ -728,7 +734,7 @@ __global__ __launch_bounds__(kBlock) void k_spread(StepBufs b)
 {
     __shared__ int wcount[kBlock / 64], wbase[kBlock / 64], bbase;
     const int ntab = ld_const(b.meta + kTabCell);
-    for (int e = blockIdx.x; e < ntab; e += gridDim.x) {
+    for (int e = xcd_entry(blockIdx.x, gridDim.x); e < ntab; e += gridDim.x) {
         const int2 w = ld_const(b.tab[kTabCell] + e);
         const SolveState st = ld_const(b.state + w.x);
         if (!st.active) continue;
@@ -1265,7 +1271,7 @@ __global__ __launch_bounds__(kBlock) void k_dt(StepBufs b, double host_time, dou
     __shared__ int last;
     const int tid = threadIdx.x;
     const int ntab = b.meta[kTabCell];
-    for (int e = blockIdx.x; e < ntab; e += gridDim.x) {
+    for (int e = xcd_entry(blockIdx.x, gridDim.x); e < ntab; e += gridDim.x) {
         const int2 w = b.tab[kTabCell][e];
         if (!b.state[w.x].active) continue;
         const SolveArgs &sa = b.solves[w.x];
