@@ -215,6 +215,8 @@ def load_library(exact=False):
         "aderdg_server_stats_get": [vp, ctypes.POINTER(_ServerStats)],
         "aderdg_server_destroy": [vp],
         "aderdg_batch_timing_get": [vp, ctypes.c_int, ctypes.c_int, _dp, ctypes.POINTER(i64)],
+        "aderdg_tune_force": [ctypes.c_int, ctypes.c_int, ctypes.c_int],
+        "aderdg_tune_query": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp],
     }
     for name, args in bsig.items():
         f = getattr(lib, name)
@@ -392,6 +394,35 @@ class Solver:
         ln = (ctypes.c_int64 * len(TIMING_SLOTS))()
         self._check(self.lib.aderdg_timing_get(self.h, ms, ln))
         return list(ms), list(ln)
+
+
+PREDICTOR_VARIANTS = ("time-line", "node-64", "node-128", "node-256")
+
+
+def tune_force(pde, N, variant, exact=False):
+    """Predictor kernel variant for contexts / arenas created afterwards (-1: the autotuner)."""
+    lib = load_library(exact)
+    k = PDE[pde] if isinstance(pde, str) else int(pde)
+    if lib.aderdg_tune_force(k, N, int(variant)) != 0:
+        raise AderdgError(lib.aderdg_last_error().decode())
+
+
+def tune_query(pde, N, persistent, device=0, exact=False):
+    """(chosen variant, ms per launch of each variant) the autotuner measured, or None."""
+    lib = load_library(exact)
+    k = PDE[pde] if isinstance(pde, str) else int(pde)
+    v = ctypes.c_int()
+    ms = (ctypes.c_double * len(PREDICTOR_VARIANTS))()
+    if lib.aderdg_tune_query(device, k, N, 1 if persistent else 0, ctypes.byref(v), ms) != 0:
+        return None
+    return v.value, list(ms)
+
+
+def pred_variant_valid(pde, N, variant):
+    k = PDE[pde] if isinstance(pde, str) else int(pde)
+    if k == 2:
+        return variant == 0
+    return variant in (0, 2, 3) or (variant == 1 and (N + 1) ** 3 <= 64)
 
 
 def solve_footprint(pde, N, imax, jmax, exact=False):

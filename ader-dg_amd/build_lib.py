@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "lib")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ADERDG_ARCH", "gfx950")
 
-SOURCES = ["operators.cpp", "initial.cpp", "step_kernels.hip", "ad_kernels.hip", "group.hip", "aderdg_capi.hip", "batch.hip", "server.hip"]
+SOURCES = ["operators.cpp", "initial.cpp", "step_kernels.hip", "ad_kernels.hip", "group.hip", "tune.hip", "aderdg_capi.hip", "batch.hip", "server.hip"]
 COMMON = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-I" + os.path.join(ROOT, "include")]
@@ -33,7 +33,10 @@ def _run(cmd, cwd=None):
 
 def _objects(exact):
     """Compile every source of one variant to objects, in parallel; returns their paths."""
-    flags = ["-ffp-contract=off", "-DADERDG_EXACT"] if exact else ["-ffp-contract=fast"]
+    # product build: FMA contraction and, in device code, a/b as a * (1/b) with the reciprocal shared
+    # between divisions by the same value (both within the 1e-12 parity bar); exact
+    # build: the oracle's operation sequence, IEEE division, no contraction
+    flags = ["-ffp-contract=off", "-DADERDG_EXACT"] if exact else ["-ffp-contract=fast", "-Xarch_device", "-freciprocal-math"]
     odir = os.path.join(LIB, "obj", "exact" if exact else "fast")
     os.makedirs(odir, exist_ok=True)
     jobs = []

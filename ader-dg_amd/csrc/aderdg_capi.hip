@@ -579,7 +579,7 @@ int predictor_cells(aderdg_ctx *c, int ncells, double dt, const double *luh, dou
         MeshArgs m = c->g.mesh(c->slot);
         m.ncell = ncells;
         TestBufs t{d_uh, d_q, d_f, d_qb, d_fb, d_s};
-        if (e == hipSuccess) e = launch_test_predictor(c->N, m, c->g.b, t, dt, c->stream);
+        if (e == hipSuccess) e = launch_test_predictor(c->N, c->g.variant, m, c->g.b, t, dt, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(lqhi, d_q, V * n * n * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(lFhi, d_f, 2 * V * n * n * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(lqbnd, d_qb, 4 * V * n * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream);

@@ -143,7 +143,8 @@ void Group::footprint(int pde, int N, int64_t *per_cell, int64_t *per_xface, int
     if (per_slot) *per_slot = (int64_t)(sizeof(SolveArgs) + sizeof(SolveState)) + 4 * 8 * kNumTabs;
 }
 
-int Group::init(int pde_, int N_, const GroupCaps &c, const Operators &ops_, int persist_blocks, hipStream_t st)
+int Group::init(int pde_, int N_, const GroupCaps &c, const Operators &ops_, int persist_blocks, hipStream_t st,
+                 int variant_)
 {
     pde = pde_;
     N = N_;
@@ -155,7 +156,9 @@ int Group::init(int pde_, int N_, const GroupCaps &c, const Operators &ops_, int
     st_ = st;
     if (c.cells < 1 || c.slots < 1 || c.cells > (1LL << 30) || c.xfaces > (1LL << 30) || c.yfaces > (1LL << 30))
         return ADERDG_EINVAL;
-    table_quanta(pde, N, quanta);
+    variant = variant_ >= 0 ? variant_ : tune_predictor(pde, N, persist_blocks, ops_, st);
+    if (!pred_variant_valid(pde, N, variant)) return ADERDG_EINVAL;
+    table_quanta(pde, N, variant, quanta);
     const int64_t items[kNumTabs] = {c.cells, c.cells, (c.xfaces + c.yfaces) * n, c.cells};
     int64_t max_entries = 0;
     for (int k = 0; k < kNumTabs; k++) {
@@ -374,7 +377,7 @@ int Group::read_states(SolveState *out)
 
 hipError_t Group::launch(int part) const
 {
-    return launch_part(pde, N, b, st_, part, grid, caps.slots);
+    return launch_part(pde, N, variant, b, st_, part, grid, caps.slots);
 }
 
 }  // namespace aderdg

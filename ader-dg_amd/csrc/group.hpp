@@ -63,6 +63,7 @@ public:
     int pde = 0, N = 0, V = 0, n = 0, s = 0;
     GroupCaps caps{};
     Operators ops{};
+    int variant = kPredLine;  // predictor kernel variant (step_kernels.hpp)
     int quanta[kNumTabs] = {};
     int grid[kNumTabs] = {};
     int64_t tab_cap[kNumTabs] = {};
@@ -70,8 +71,10 @@ public:
 
     ~Group() { release(); }
     // Allocate the arena.  persist_blocks: workgroups of the table-driven kernels
-    // (0: one per table entry of a full arena).
-    int init(int pde, int N, const GroupCaps &caps, const Operators &ops, int persist_blocks, hipStream_t st);
+    // (0: one per table entry of a full arena); variant: predictor kernel variant,
+    // or -1 for the autotuner's choice (tune.hip).
+    int init(int pde, int N, const GroupCaps &caps, const Operators &ops, int persist_blocks, hipStream_t st,
+             int variant = -1);
     void release();
 
     // Place a solve; returns 0, or ADERDG_EFULL when no slot or range is free.
@@ -112,5 +115,9 @@ private:
     template <class T>
     int dalloc(T **p, size_t count);
 };
+
+// The autotuner's predictor variant for a (PDE, N) and launch mode on the current
+// device (tune.hip); times the variants the first time it is asked.
+int tune_predictor(int pde, int N, int persist_blocks, const Operators &ops, hipStream_t st);
 
 }  // namespace aderdg

@@ -28,6 +28,8 @@
 // The step kernels loop over work tables of (solve, first item) entries
 // (persistent workgroups); solves that are not active are skipped, and each
 // solve has its own ping-pong parity, time and dt.
+#include <type_traits>
+
 #include "step_kernels.hpp"
 #include "pde_device.hpp"
 
@@ -76,6 +78,97 @@ ADG_DEV double dmr_shock_x(const MeshArgs &m, double time) { return m.dmr_a + m.
 // contraction with iK1 is thread-local.  Baer-Nunziato also exchanges q for
 // the gradients of the non-conservative product.
 // ---------------------------------------------------------------------------
+// Volume integral, boundary extrapolation and the 3x3 pass of get_min_max of one
+// cell from its time averages in LDS (qhiL, fhxL, fhyL, shL: [v][j][i]; shL is
+// read for Baer-Nunziato only), spread over the cell's nthr threads.  TEST:
+// write the averages and boundary values in the reference layouts instead.
+template <class P, int N, bool TEST>
+ADG_DEV void predictor_finish(const MeshArgs &m, const StepBufs &b, const TestBufs &t, const double *qhiL,
+                              const double *fhxL, const double *fhyL, const double *shL, int tt, int nthr,
+                              bool act, int cell, int cell0)
+{
+    using S = Shape<P, N>;
+    constexpr int n = S::n, NN = S::NN, V = S::V;
+    const OpPtr op = (OpPtr)b.op;
+    const size_t g = (size_t)cell0 + cell;
+    for (int node = tt; act && node < NN; node += nthr) {
+        if (TEST) {
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                t.qhi[(size_t)cell * V * NN + v + V * node] = qhiL[v * NN + node];
+                t.fhi[(size_t)cell * 2 * V * NN + v + V * (0 + 2 * node)] = fhxL[v * NN + node];
+                t.fhi[(size_t)cell * 2 * V * NN + v + V * (1 + 2 * node)] = fhyL[v * NN + node];
+                if (P::kNcp) t.shi[(size_t)cell * V * NN + v + V * node] = shL[v * NN + node];
+            }
+        } else {
+            // volume integral, Euler/volume_integral.f90:17-28 (+ Baer-Nunziato/volume_integral.f90:36-40)
+            const int i = node % n, j = node / n;
+            const double wi = op->w[i], wj = op->w[j];
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < n; k++) acc = acc + fhxL[v * NN + k + n * j] * op->kxi[i + n * k];
+                double d = 0.0 + acc * wj / m.dx;
+                acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < n; k++) acc = acc + fhyL[v * NN + i + n * k] * op->kxi[j + n * k];
+                d = d + acc * wi / m.dy;
+                if (P::kNcp) d = d + wi * wj * shL[v * NN + node];
+                b.lduh[g * V * NN + V * node + v] = d;
+            }
+        }
+    }
+
+    // boundary extrapolation, Euler/space_time_predictor.f90:100-119
+    for (int task = tt; task < 4 * n; task += nthr) {
+        const int f = task / n, p = task - f * n;
+        const double *cf = (f & 1) ? op->frc : op->flc;
+        double Qb[V], Fb[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            double a = 0.0, c = 0.0;
+#pragma unroll
+            for (int k = 0; k < n; k++) {
+                const int idx = (f < 2) ? (k + n * p) : (p + n * k);
+                a = a + qhiL[v * NN + idx] * cf[k];
+                c = c + (f < 2 ? fhxL[v * NN + idx] : fhyL[v * NN + idx]) * cf[k];
+            }
+            Qb[v] = a;
+            Fb[v] = c;
+        }
+        if (act) {
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                if (TEST) {
+                    t.qbnd[(size_t)cell * V * 4 * n + v + V * (f + 4 * p)] = Qb[v];
+                    t.fbnd[(size_t)cell * V * 4 * n + v + V * (f + 4 * p)] = Fb[v];
+                } else {
+                    const size_t o = ((size_t)(f * V + v) * n + p) * b.G + g;
+                    b.fq[o] = Qb[v];
+                    b.ff[o] = Fb[v];
+                }
+            }
+        }
+    }
+    if (TEST) return;
+
+    // get_min_max step 2: 3x3 neighbourhood, Euler/limiter.f90:147-167
+    for (int v = tt; act && v < V; v += nthr) {
+        const int ci = cell % m.imax, cj = cell / m.imax;
+        const size_t row = (size_t)v * b.G + cell0;
+        double mn = b.lmin_old[row + cell], mx = b.lmax_old[row + cell];
+        for (int jj = -1; jj <= 1; jj++)
+            for (int ii = -1; ii <= 1; ii++) {
+                const int nb = nb_index(ci, ii, m.imax, m.wrap) + m.imax * nb_index(cj, jj, m.jmax, m.wrap);
+                mn = f_min(mn, b.lmin_old[row + nb]);
+                mx = f_max(mx, b.lmax_old[row + nb]);
+            }
+        b.lmin[row + cell] = mn;
+        b.lmax[row + cell] = mx;
+    }
+}
+
 // CPW cells from local cell `first` of one solve (cell0, parity par); smem holds
 // 2 * CPW * SLOT doubles.  TEST: cells of t.uh, outputs in the reference layouts.
 template <class P, int N, bool TEST>
@@ -231,90 +324,240 @@ ADG_DEV void predictor_cells(const MeshArgs &m, const StepBufs &b, const TestBuf
     double *qhiL = SM(0);
     double *fhxL = SM(0) + V * NN;
     double *fhyL = SM(1);
+    double *shL = SM(1) + V * NN;
     if (own) {
 #pragma unroll
         for (int v = 0; v < V; v++) {
             qhiL[v * NN + tt] = qh[v];
             fhxL[v * NN + tt] = fh[0][v];
             fhyL[v * NN + tt] = fh[1][v];
+            if (P::kNcp) shL[v * NN + tt] = sh[v];
         }
     }
     __syncthreads();
+    predictor_finish<P, N, TEST>(m, b, t, qhiL, fhxL, fhyL, shL, tt, NN, act, cell, cell0);
+}
 
-    if (TEST) {
-        if (act) {
+
+// ---------------------------------------------------------------------------
+// Space-time-node variant of the predictor: thread (cell, i, j, l) owns the
+// state q(:, i, j, l) of one space-time node, so a cell's whole Picard sweep
+// runs at once on (N+1)^3 threads and each thread holds one node instead of a
+// time line (far fewer registers, more waves per SIMD).  Per sweep the x- and
+// y-contractions (and the Baer-Nunziato gradients) read the other nodes' F
+// (and q) of the same slice from LDS, and the time contraction reads the
+// right-hand sides rhs(:, i, j, 0..N) of the node's time line.  Every sum is
+// taken in the order of predictor_cells, so both variants give the same bits.
+// BLK threads per workgroup hold BLK / (N+1)^3 cells; smem holds
+// 2 V (N+1)^3 doubles per cell.
+// ---------------------------------------------------------------------------
+template <class P, int N, int BLK>
+struct NodeShape {
+    static constexpr int n = N + 1, NN = n * n, N3 = NN * n, V = P::V;
+    static constexpr int CPB = BLK / N3;      // cells per workgroup
+    static constexpr int R = 2 * V * N3;      // LDS doubles per cell
+    static_assert(CPB >= 1, "a workgroup must hold a whole cell");
+};
+
+template <class P, int N, int BLK, bool TEST>
+ADG_DEV void predictor_nodes(const MeshArgs &m, const StepBufs &b, const TestBufs &t, double *smem, int first,
+                             int cell0, int par, double dt)
+{
+    using S = NodeShape<P, N, BLK>;
+    constexpr int n = S::n, NN = S::NN, N3 = S::N3, V = S::V, CPB = S::CPB, R = S::R;
+    const OpPtr op = (OpPtr)b.op;
+    const int tid = threadIdx.x, lc = tid / N3, tt = tid - lc * N3;
+    const int ij = tt % NN, i = ij % n, j = ij / n, l = tt / NN;
+    const int cell = first + lc;  // local cell of the solve
+    const bool own = lc < CPB;
+    const bool act = own && cell < m.ncell;
+    double *sR = smem + (own ? lc : CPB - 1) * R;
+
+    double u[V];
+    const double *uh = TEST ? t.uh : b.uh[par];
+    const size_t gu = TEST ? (size_t)cell : (size_t)cell0 + cell;
+#pragma unroll
+    for (int v = 0; v < V; v++) u[v] = act ? uh[gu * V * NN + V * ij + v] : 0.0;
+    if (!act) {  // an admissible dummy state for idle lanes
+        if (V == 4) {
+            u[0] = 1.0;
+            u[3] = 2.5;
+        } else {
+            u[0] = 0.5; u[3] = 2.5; u[4] = 0.5; u[7] = 2.5; u[8] = 0.5;
+        }
+    }
+    const double wi = op->w[i], wj = op->w[j], wl = op->w[l];
+    double kx_i[n], kx_j[n], du_i[n], du_j[n], ik_l[n];
+#pragma unroll
+    for (int k = 0; k < n; k++) {
+        kx_i[k] = op->kxi[k + n * i];
+        kx_j[k] = op->kxi[k + n * j];
+        ik_l[k] = op->ik1[l + n * k];
+        if (P::kNcp) {
+            du_i[k] = op->dudx[i + n * k];
+            du_j[k] = op->dudx[j + n * k];
+        }
+    }
+    // the same expressions as predictor_cells for slice l of node (i, j)
+    const double sx = wl * wj * dt / m.dx, sy = wl * wi * dt / m.dy;
+    const double c0 = wi * wj * op->f0[l];
+    const double fs = wi * wj * wl * dt;
+    const double f1 = 1.0 / (wi * wj);
+
+    double q[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) q[v] = u[v];
+    double fhx[V], fhy[V], sh[V];  // time averages, valid in the l == 0 threads after the last sweep
+
+    // one Picard sweep, Euler/space_time_predictor.f90:49-84 (Baer-Nunziato/space_time_predictor.f90:54-104)
+    auto sweep = [&](auto last_tag) {
+        constexpr bool LAST = decltype(last_tag)::value;
+        double F[2 * V], rv[V], G[2 * V];
+        P::flux(m, q, F);
+        if (P::kNcp) {
+            // phase 1: q and Fx of the slice
+            if (own) {
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    sR[v * N3 + tt] = q[v];
+                    sR[(V + v) * N3 + tt] = F[v];
+                }
+            }
+            __syncthreads();
 #pragma unroll
             for (int v = 0; v < V; v++) {
-                t.qhi[(size_t)cell * V * NN + v + V * tt] = qh[v];
-                t.fhi[(size_t)cell * 2 * V * NN + v + V * (0 + 2 * tt)] = fh[0][v];
-                t.fhi[(size_t)cell * 2 * V * NN + v + V * (1 + 2 * tt)] = fh[1][v];
-                if (P::kNcp) t.shi[(size_t)cell * V * NN + v + V * tt] = sh[v];
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < n; k++) acc = acc + sR[(V + v) * N3 + k + n * j + NN * l] * kx_i[k];
+                rv[v] = c0 * u[v] - sx * acc;  // :66
+                acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < n; k++) acc = acc + sR[v * N3 + k + n * j + NN * l] * du_i[k];
+                G[v] = 1.0 / m.dx * acc;
+                if (LAST && l == 0) {  // time average of the last sweep's fluxes (:92)
+                    double a = 0.0;
+#pragma unroll
+                    for (int mm = 0; mm < n; mm++) a = a + sR[(V + v) * N3 + ij + NN * mm] * op->w[mm];
+                    fhx[v] = a;
+                }
+            }
+            __syncthreads();
+            // phase 2: Fy of the slice (q stays)
+            if (own) {
+#pragma unroll
+                for (int v = 0; v < V; v++) sR[(V + v) * N3 + tt] = F[V + v];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < n; k++) acc = acc + sR[(V + v) * N3 + i + n * k + NN * l] * kx_j[k];
+                rv[v] = rv[v] - sy * acc;  // :72
+                acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < n; k++) acc = acc + sR[v * N3 + i + n * k + NN * l] * du_j[k];
+                G[V + v] = 1.0 / m.dy * acc;
+                if (LAST && l == 0) {
+                    double a = 0.0;
+#pragma unroll
+                    for (int mm = 0; mm < n; mm++) a = a + sR[(V + v) * N3 + ij + NN * mm] * op->w[mm];
+                    fhy[v] = a;
+                }
+            }
+        } else {
+            // Fx and Fy of the slice at once
+            if (own) {
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    sR[v * N3 + tt] = F[v];
+                    sR[(V + v) * N3 + tt] = F[V + v];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < n; k++) acc = acc + sR[v * N3 + k + n * j + NN * l] * kx_i[k];
+                const double rr = c0 * u[v] - sx * acc;  // :66
+                acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < n; k++) acc = acc + sR[(V + v) * N3 + i + n * k + NN * l] * kx_j[k];
+                rv[v] = rr - sy * acc;  // :72
+                if (LAST && l == 0) {
+                    double a = 0.0, c = 0.0;
+#pragma unroll
+                    for (int mm = 0; mm < n; mm++) a = a + sR[v * N3 + ij + NN * mm] * op->w[mm];
+#pragma unroll
+                    for (int mm = 0; mm < n; mm++) c = c + sR[(V + v) * N3 + ij + NN * mm] * op->w[mm];
+                    fhx[v] = a;
+                    fhy[v] = c;
+                }
             }
         }
-    } else {
-        // volume integral, Euler/volume_integral.f90:17-28 (+ Baer-Nunziato/volume_integral.f90:36-40)
+        double src[V];
+        if (P::kNcp) {  // NCP source, Baer-Nunziato/space_time_predictor.f90:84-92
+            P::ncp(m, q, G, src);
+#pragma unroll
+            for (int v = 0; v < V; v++) rv[v] = rv[v] + fs * src[v];
+        }
+        __syncthreads();
+        // phase 3: time contraction, :79-83, q(v,ll) = f1 * sum over m of rhs(v,m) iK1(ll,m)
+        if (own) {
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                sR[v * N3 + tt] = rv[v];
+                if (P::kNcp && LAST) sR[(V + v) * N3 + tt] = src[v];
+            }
+        }
+        __syncthreads();
 #pragma unroll
         for (int v = 0; v < V; v++) {
             double acc = 0.0;
 #pragma unroll
-            for (int k = 0; k < n; k++) acc = acc + fhxL[v * NN + k + n * j] * op->kxi[i + n * k];
-            double d = 0.0 + acc * wj / m.dx;
-            acc = 0.0;
+            for (int mm = 0; mm < n; mm++) acc = acc + sR[v * N3 + ij + NN * mm] * ik_l[mm];
+            q[v] = f1 * acc;
+            if (P::kNcp && LAST && l == 0) {  // lShi (:111)
+                double a = 0.0;
 #pragma unroll
-            for (int k = 0; k < n; k++) acc = acc + fhyL[v * NN + i + n * k] * op->kxi[j + n * k];
-            d = d + acc * wi / m.dy;
-            if (P::kNcp) d = d + wi * wj * sh[v];
-            if (act) b.lduh[g * V * NN + V * tt + v] = d;
+                for (int mm = 0; mm < n; mm++) a = a + sR[(V + v) * N3 + ij + NN * mm] * op->w[mm];
+                sh[v] = a;
+            }
         }
-    }
+        __syncthreads();
+    };
+#pragma unroll 1
+    for (int it = 0; it < N; it++) sweep(std::false_type{});
+    sweep(std::true_type{});
 
-    // boundary extrapolation, Euler/space_time_predictor.f90:100-119
-    for (int task = tt; task < 4 * n; task += NN) {
-        const int f = task / n, p = task - f * n;
-        const double *cf = (f & 1) ? op->frc : op->flc;
-        double Qb[V], Fb[V];
+    // qhi = sum over l of q(l) w(l), :90
+    if (own) {
+#pragma unroll
+        for (int v = 0; v < V; v++) sR[v * N3 + tt] = q[v];
+    }
+    __syncthreads();
+    double qh[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int mm = 0; mm < n; mm++) acc = acc + sR[v * N3 + ij + NN * mm] * op->w[mm];
+        qh[v] = acc;
+    }
+    __syncthreads();
+    double *qhiL = sR, *fhxL = sR + V * NN, *fhyL = sR + 2 * V * NN, *shL = sR + 3 * V * NN;
+    if (own && l == 0) {
 #pragma unroll
         for (int v = 0; v < V; v++) {
-            double a = 0.0, c = 0.0;
-#pragma unroll
-            for (int k = 0; k < n; k++) {
-                const int idx = (f < 2) ? (k + n * p) : (p + n * k);
-                a = a + qhiL[v * NN + idx] * cf[k];
-                c = c + (f < 2 ? fhxL[v * NN + idx] : fhyL[v * NN + idx]) * cf[k];
-            }
-            Qb[v] = a;
-            Fb[v] = c;
-        }
-        if (act) {
-#pragma unroll
-            for (int v = 0; v < V; v++) {
-                if (TEST) {
-                    t.qbnd[(size_t)cell * V * 4 * n + v + V * (f + 4 * p)] = Qb[v];
-                    t.fbnd[(size_t)cell * V * 4 * n + v + V * (f + 4 * p)] = Fb[v];
-                } else {
-                    const size_t o = ((size_t)(f * V + v) * n + p) * b.G + g;
-                    b.fq[o] = Qb[v];
-                    b.ff[o] = Fb[v];
-                }
-            }
+            qhiL[v * NN + ij] = qh[v];
+            fhxL[v * NN + ij] = fhx[v];
+            fhyL[v * NN + ij] = fhy[v];
+            if (P::kNcp) shL[v * NN + ij] = sh[v];
         }
     }
-    if (TEST) return;
-
-    // get_min_max step 2: 3x3 neighbourhood, Euler/limiter.f90:147-167
-    for (int v = tt; act && v < V; v += NN) {
-        const int ci = cell % m.imax, cj = cell / m.imax;
-        const size_t row = (size_t)v * b.G + cell0;
-        double mn = b.lmin_old[row + cell], mx = b.lmax_old[row + cell];
-        for (int jj = -1; jj <= 1; jj++)
-            for (int ii = -1; ii <= 1; ii++) {
-                const int nb = nb_index(ci, ii, m.imax, m.wrap) + m.imax * nb_index(cj, jj, m.jmax, m.wrap);
-                mn = f_min(mn, b.lmin_old[row + nb]);
-                mx = f_max(mx, b.lmax_old[row + nb]);
-            }
-        b.lmin[row + cell] = mn;
-        b.lmax[row + cell] = mx;
-    }
+    __syncthreads();
+    predictor_finish<P, N, TEST>(m, b, t, qhiL, fhxL, fhyL, shL, tt, N3, act, cell, cell0);
 }
 
 template <class P, int N>
@@ -343,6 +586,35 @@ __global__ __launch_bounds__(kPredBlock<P>) void k_predictor_test(MeshArgs m, St
     using S = Shape<P, N>;
     __shared__ double smem[2 * S::CPW * S::SLOT];
     predictor_cells<P, N, true>(m, b, t, smem, blockIdx.x * S::CPW, 0, 0, dt);
+}
+
+
+template <class P, int N, int BLK>
+__global__ __launch_bounds__(BLK) void k_predictor_nodes(StepBufs b)
+{
+    using S = NodeShape<P, N, BLK>;
+    __shared__ double smem[S::CPB * S::R];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        b.counters[kCntList] = 0;
+        b.counters[kCntTroubled] = 0;
+    }
+    const int ntab = ld_const(b.meta + kTabPred);
+    for (int e = xcd_entry(blockIdx.x, gridDim.x); e < ntab; e += gridDim.x) {
+        const int2 w = ld_const(b.tab[kTabPred] + e);
+        const SolveState st = ld_const(b.state + w.x);
+        if (!st.active) continue;
+        const SolveArgs sa = ld_const(b.solves + w.x);
+        predictor_nodes<P, N, BLK, false>(sa.m, b, TestBufs{}, smem, w.y, sa.cell0, st.parity, st.dt);
+        __syncthreads();
+    }
+}
+
+template <class P, int N, int BLK>
+__global__ __launch_bounds__(BLK) void k_predictor_nodes_test(MeshArgs m, StepBufs b, TestBufs t, double dt)
+{
+    using S = NodeShape<P, N, BLK>;
+    __shared__ double smem[S::CPB * S::R];
+    predictor_nodes<P, N, BLK, true>(m, b, t, smem, blockIdx.x * S::CPB, 0, 0, dt);
 }
 
 // ---------------------------------------------------------------------------
@@ -1383,14 +1655,49 @@ __global__ __launch_bounds__(kBlock) void k_init(StepBufs b, IcSpec ic, int slot
     for (int v = 0; v < V; v++) u[v] = Q[v];
 }
 
+template <int N>
+constexpr bool node_variant_fits(int variant)
+{
+    return variant == kPredNode64 ? (N + 1) * (N + 1) * (N + 1) <= 64
+                                  : (variant == kPredNode128 ? (N + 1) * (N + 1) * (N + 1) <= 128 : true);
+}
+
 template <class P, int N>
-hipError_t launch_step_impl(const StepBufs &b, hipStream_t st, int part, const int grid[kNumTabs])
+hipError_t launch_predictor(const StepBufs &b, hipStream_t st, int variant, int grid)
+{
+    using S = Shape<P, N>;
+    switch (variant) {
+    case kPredLine:
+        hipLaunchKernelGGL((k_predictor<P, N>), dim3(grid), dim3(S::PRED_BLOCK), 0, st, b);
+        break;
+    case kPredNode64:
+        if constexpr ((N + 1) * (N + 1) * (N + 1) <= 64)
+            hipLaunchKernelGGL((k_predictor_nodes<P, N, 64>), dim3(grid), dim3(64), 0, st, b);
+        else
+            return hipErrorInvalidValue;
+        break;
+    case kPredNode128:
+        hipLaunchKernelGGL((k_predictor_nodes<P, N, 128>), dim3(grid), dim3(128), 0, st, b);
+        break;
+    case kPredNode256:
+        hipLaunchKernelGGL((k_predictor_nodes<P, N, 256>), dim3(grid), dim3(256), 0, st, b);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipSuccess;
+}
+
+template <class P, int N>
+hipError_t launch_step_impl(const StepBufs &b, hipStream_t st, int part, const int grid[kNumTabs], int variant)
 {
     using S = Shape<P, N>;
     switch (part) {
-    case kPartPredictor:
-        hipLaunchKernelGGL((k_predictor<P, N>), dim3(grid[kTabPred]), dim3(S::PRED_BLOCK), 0, st, b);
+    case kPartPredictor: {
+        const hipError_t e = launch_predictor<P, N>(b, st, variant, grid[kTabPred]);
+        if (e != hipSuccess) return e;
         break;
+    }
     case kPartFace:
         hipLaunchKernelGGL((k_face<P, N>), dim3(grid[kTabFace]), dim3(kBlock), 0, st, b);
         break;
@@ -1418,46 +1725,83 @@ hipError_t launch_step_impl(const StepBufs &b, hipStream_t st, int part, const i
 }
 
 template <class P>
-hipError_t launch_part_pde(int N, const StepBufs &b, hipStream_t st, int part, const int grid[kNumTabs])
+hipError_t launch_part_pde(int N, const StepBufs &b, hipStream_t st, int part, const int grid[kNumTabs], int variant)
 {
     switch (N) {
-    case 1: return launch_step_impl<P, 1>(b, st, part, grid);
-    case 2: return launch_step_impl<P, 2>(b, st, part, grid);
-    case 3: return launch_step_impl<P, 3>(b, st, part, grid);
-    case 4: return launch_step_impl<P, 4>(b, st, part, grid);
+    case 1: return launch_step_impl<P, 1>(b, st, part, grid, variant);
+    case 2: return launch_step_impl<P, 2>(b, st, part, grid, variant);
+    case 3: return launch_step_impl<P, 3>(b, st, part, grid, variant);
+    case 4: return launch_step_impl<P, 4>(b, st, part, grid, variant);
     default: return hipErrorInvalidValue;
     }
 }
 
 template <class P, int N>
-hipError_t launch_test_impl(const MeshArgs &m, const StepBufs &b, const TestBufs &t, double dt, hipStream_t st)
+hipError_t launch_test_impl(const MeshArgs &m, const StepBufs &b, const TestBufs &t, double dt, hipStream_t st,
+                            int variant)
 {
     using S = Shape<P, N>;
-    const int g = (m.ncell + S::CPW - 1) / S::CPW;
-    hipLaunchKernelGGL((k_predictor_test<P, N>), dim3(g), dim3(S::PRED_BLOCK), 0, st, m, b, t, dt);
+    constexpr int N3 = (N + 1) * (N + 1) * (N + 1);
+    switch (variant) {
+    case kPredLine: {
+        const int g = (m.ncell + S::CPW - 1) / S::CPW;
+        hipLaunchKernelGGL((k_predictor_test<P, N>), dim3(g), dim3(S::PRED_BLOCK), 0, st, m, b, t, dt);
+        break;
+    }
+    case kPredNode64:
+        if constexpr (N3 <= 64) {
+            const int g = (m.ncell + 64 / N3 - 1) / (64 / N3);
+            hipLaunchKernelGGL((k_predictor_nodes_test<P, N, 64>), dim3(g), dim3(64), 0, st, m, b, t, dt);
+        } else {
+            return hipErrorInvalidValue;
+        }
+        break;
+    case kPredNode128: {
+        const int g = (m.ncell + 128 / N3 - 1) / (128 / N3);
+        hipLaunchKernelGGL((k_predictor_nodes_test<P, N, 128>), dim3(g), dim3(128), 0, st, m, b, t, dt);
+        break;
+    }
+    case kPredNode256: {
+        const int g = (m.ncell + 256 / N3 - 1) / (256 / N3);
+        hipLaunchKernelGGL((k_predictor_nodes_test<P, N, 256>), dim3(g), dim3(256), 0, st, m, b, t, dt);
+        break;
+    }
+    default:
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
 template <class P>
-hipError_t launch_test_pde(int N, const MeshArgs &m, const StepBufs &b, const TestBufs &t, double dt, hipStream_t st)
+hipError_t launch_test_pde(int N, const MeshArgs &m, const StepBufs &b, const TestBufs &t, double dt, hipStream_t st,
+                           int variant)
 {
     switch (N) {
-    case 1: return launch_test_impl<P, 1>(m, b, t, dt, st);
-    case 2: return launch_test_impl<P, 2>(m, b, t, dt, st);
-    case 3: return launch_test_impl<P, 3>(m, b, t, dt, st);
-    case 4: return launch_test_impl<P, 4>(m, b, t, dt, st);
+    case 1: return launch_test_impl<P, 1>(m, b, t, dt, st, variant);
+    case 2: return launch_test_impl<P, 2>(m, b, t, dt, st, variant);
+    case 3: return launch_test_impl<P, 3>(m, b, t, dt, st, variant);
+    case 4: return launch_test_impl<P, 4>(m, b, t, dt, st, variant);
     default: return hipErrorInvalidValue;
     }
 }
 
+template <class P, int N>
+void quanta_impl(int variant, int q[kNumTabs])
+{
+    constexpr int N3 = (N + 1) * (N + 1) * (N + 1);
+    q[kTabPred] = variant == kPredLine ? Shape<P, N>::CPW
+                  : variant == kPredNode64 ? 64 / N3 : variant == kPredNode128 ? 128 / N3 : 256 / N3;
+    q[kTabUpd] = Shape<P, N>::CPWU;
+}
+
 template <class P>
-void quanta_pde(int N, int q[kNumTabs])
+void quanta_pde(int N, int variant, int q[kNumTabs])
 {
     switch (N) {
-    case 1: q[kTabPred] = Shape<P, 1>::CPW; q[kTabUpd] = Shape<P, 1>::CPWU; break;
-    case 2: q[kTabPred] = Shape<P, 2>::CPW; q[kTabUpd] = Shape<P, 2>::CPWU; break;
-    case 3: q[kTabPred] = Shape<P, 3>::CPW; q[kTabUpd] = Shape<P, 3>::CPWU; break;
-    default: q[kTabPred] = Shape<P, 4>::CPW; q[kTabUpd] = Shape<P, 4>::CPWU; break;
+    case 1: quanta_impl<P, 1>(variant, q); break;
+    case 2: quanta_impl<P, 2>(variant, q); break;
+    case 3: quanta_impl<P, 3>(variant, q); break;
+    default: quanta_impl<P, 4>(variant, q); break;
     }
     q[kTabFace] = kBlock;
     q[kTabCell] = kBlock;
@@ -1465,18 +1809,33 @@ void quanta_pde(int N, int q[kNumTabs])
 
 }  // namespace
 
-void table_quanta(int pde, int N, int quanta[kNumTabs])
+bool pred_variant_valid(int pde, int N, int variant)
+{
+    if (N < 1 || N > 4) return false;
+    if (pde == kPdeAd) return variant == kPredLine;
+    if (variant == kPredLine || variant == kPredNode128 || variant == kPredNode256) return true;
+    return variant == kPredNode64 && (N + 1) * (N + 1) * (N + 1) <= 64;
+}
+
+int pred_cells_per_block(int pde, int N, int variant)
+{
+    int q[kNumTabs];
+    table_quanta(pde, N, variant, q);
+    return q[kTabPred];
+}
+
+void table_quanta(int pde, int N, int variant, int quanta[kNumTabs])
 {
     if (pde == kPdeAd)
         table_quanta_ad(N, quanta);
     else if (pde == kPdeBn)
-        quanta_pde<BnPDE>(N, quanta);
+        quanta_pde<BnPDE>(N, variant, quanta);
     else
-        quanta_pde<EulerPDE>(N, quanta);
+        quanta_pde<EulerPDE>(N, variant, quanta);
 }
 
-hipError_t launch_part(int pde, int N, const StepBufs &b, hipStream_t st, int part, const int grid[kNumTabs],
-                       int nslot_cap)
+hipError_t launch_part(int pde, int N, int variant, const StepBufs &b, hipStream_t st, int part,
+                       const int grid[kNumTabs], int nslot_cap)
 {
     if (part == kPartDt) return launch_dt(b, st, 0.0, 0.0, 0);
     if (part == kPartAdvance || part == kPartFlip || part == kPartSeedDone) {
@@ -1485,7 +1844,8 @@ hipError_t launch_part(int pde, int N, const StepBufs &b, hipStream_t st, int pa
         return hipGetLastError();
     }
     if (pde == kPdeAd) return launch_part_ad(N, b, st, part, grid);
-    return pde == kPdeBn ? launch_part_pde<BnPDE>(N, b, st, part, grid) : launch_part_pde<EulerPDE>(N, b, st, part, grid);
+    return pde == kPdeBn ? launch_part_pde<BnPDE>(N, b, st, part, grid, variant)
+                         : launch_part_pde<EulerPDE>(N, b, st, part, grid, variant);
 }
 
 hipError_t launch_dt(const StepBufs &b, hipStream_t st, double time, double tend, int use_host_time)
@@ -1537,10 +1897,11 @@ hipError_t launch_init(int pde, int N, const StepBufs &b, const IcSpec &ic, int 
     return hipGetLastError();
 }
 
-hipError_t launch_test_predictor(int N, const MeshArgs &m, const StepBufs &b, const TestBufs &t, double dt,
-                                 hipStream_t st)
+hipError_t launch_test_predictor(int N, int variant, const MeshArgs &m, const StepBufs &b, const TestBufs &t,
+                                 double dt, hipStream_t st)
 {
-    return m.pde == kPdeBn ? launch_test_pde<BnPDE>(N, m, b, t, dt, st) : launch_test_pde<EulerPDE>(N, m, b, t, dt, st);
+    return m.pde == kPdeBn ? launch_test_pde<BnPDE>(N, m, b, t, dt, st, variant)
+                           : launch_test_pde<EulerPDE>(N, m, b, t, dt, st, variant);
 }
 
 }  // namespace aderdg

@@ -91,16 +91,25 @@ struct TestBufs {
     double *qhi, *fhi, *qbnd, *fbnd, *shi;
 };
 
-// Items (cells or face points) per work-table entry of each table, for a (PDE, N).
-void table_quanta(int pde, int N, int quanta[kNumTabs]);
+// Predictor kernels: time-line threads (one thread per spatial node holds the node's
+// whole time line) or space-time-node threads (one thread per node of the space-time
+// cell) in workgroups of 64, 128 or 256.  All give the same bits; the autotuner
+// (tune.hip) picks one per (PDE, N).  Advection-diffusion has the time-line form only.
+enum { kPredLine = 0, kPredNode64 = 1, kPredNode128 = 2, kPredNode256 = 3, kNumPredVariants = 4 };
+bool pred_variant_valid(int pde, int N, int variant);
+int pred_cells_per_block(int pde, int N, int variant);
+
+// Items (cells or face points) per work-table entry of each table, for a (PDE, N)
+// and predictor variant.
+void table_quanta(int pde, int N, int variant, int quanta[kNumTabs]);
 // the advection-diffusion kernels (ad_kernels.hip)
 void table_quanta_ad(int N, int quanta[kNumTabs]);
 hipError_t launch_part_ad(int N, const StepBufs &b, hipStream_t st, int part, const int grid[kNumTabs]);
 
 // One part of the step for every solve of the group.  grid: workgroups per table-driven
 // kernel (they loop over the table); nslot_cap: solve slots of the group.
-hipError_t launch_part(int pde, int N, const StepBufs &b, hipStream_t st, int part, const int grid[kNumTabs],
-                       int nslot_cap);
+hipError_t launch_part(int pde, int N, int variant, const StepBufs &b, hipStream_t st, int part,
+                       const int grid[kNumTabs], int nslot_cap);
 // calc_time_step of every active solve; use_host: time and tend given here instead of the solve state.
 hipError_t launch_dt(const StepBufs &b, hipStream_t st, double time, double tend, int use_host);
 // Rebuild the work tables from per-table prefix counts of entries over the solve slots
@@ -110,7 +119,7 @@ hipError_t launch_tables(const StepBufs &b, const int *prefix, int nslot, const 
 // Initial state of solve `slot` into uh[parity] from a formula (ic_spec.hpp).
 hipError_t launch_init(int pde, int N, const StepBufs &b, const IcSpec &ic, int slot, int ncell, int parity,
                        hipStream_t st);
-hipError_t launch_test_predictor(int N, const MeshArgs &m, const StepBufs &b, const TestBufs &t, double dt,
-                                 hipStream_t st);
+hipError_t launch_test_predictor(int N, int variant, const MeshArgs &m, const StepBufs &b, const TestBufs &t,
+                                 double dt, hipStream_t st);
 
 }  // namespace aderdg

@@ -1,0 +1,183 @@
+// tune.hip -- autotuner of the predictor kernel.
+//
+// The predictor has several kernel variants (step_kernels.hpp: time-line threads,
+// or space-time-node threads in workgroups of 64, 128 or 256, i.e. 1..32 cells per
+// workgroup) that give the same bits and differ in registers, LDS and waves per
+// SIMD.  The fastest depends on (PDE, N) and on the launch mode (one workgroup per
+// table entry, or a persistent grid), so the first group of a (device, PDE, N,
+// mode) times every variant on a synthetic 192 x 192 solve and keeps the fastest
+// for the rest of the process.  ADERDG_PREDICTOR_VARIANT=<k> in the environment,
+// or aderdg_tune_force(), overrides the choice (for tests and measurements).
+#include <climits>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#include "capi_common.hpp"
+#include "group.hpp"
+
+namespace aderdg {
+
+namespace {
+
+constexpr int kTuneSide = 192;
+constexpr int kTuneRounds = 3, kTuneReps = 3;
+
+struct TuneEntry {
+    int variant = kPredLine;
+    double ms[kNumPredVariants] = {-1.0, -1.0, -1.0, -1.0};
+};
+
+std::mutex g_mu;
+std::map<std::tuple<int, int, int, int>, TuneEntry> g_cache;  // (device, pde, N, persistent)
+int g_forced[3][5] = {{-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}};
+
+// The synthetic solve: the isentropic vortex (Euler) or a pressure disc (Baer-Nunziato,
+// ideal-gas phases), both admissible, timed over predictor launches at a fixed small dt.
+void synthetic(int pde, int N, aderdg_desc *d, IcSpec *ic)
+{
+    std::memset(d, 0, sizeof(*d));
+    d->pde = pde;
+    d->N = N;
+    d->imax = d->jmax = kTuneSide;
+    d->gamma = 1.4;
+    d->cfl = 0.9;
+    d->xl[0] = d->xl[1] = 0.0;
+    std::memset(ic, 0, sizeof(*ic));
+    if (pde == kPdeBn) {
+        d->xr[0] = d->xr[1] = 1.0;
+        d->bc_mode = kBcTransmissive;
+        d->bn_eos[0] = d->bn_eos[1] = 1.4;
+        ic->kind = kIcBnCircle;  // centre, radius, p inside / outside, rho_s, rho_g, phi inside / outside
+        const double p[9] = {0.5, 0.5, 0.25, 2.0, 1.0, 1.0, 1.0, 0.6, 0.4};
+        for (int k = 0; k < 9; k++) ic->p[k] = p[k];
+    } else {
+        d->xr[0] = d->xr[1] = 10.0;
+        d->bc_mode = kBcPeriodic;
+        ic->kind = kIcVortex;  // strength, centre, background velocity
+        const double p[5] = {5.0, 5.0, 5.0, 1.0, 1.0};
+        for (int k = 0; k < 5; k++) ic->p[k] = p[k];
+    }
+}
+
+// A synthetic solve in its own group, built with one predictor variant.
+struct Trial {
+    Group g;
+    bool ok = false;
+    Trial(int pde, int N, int variant, int persist_blocks, const Operators &ops, hipStream_t st)
+    {
+        aderdg_desc d;
+        IcSpec ic;
+        synthetic(pde, N, &d, &ic);
+        const int S = kTuneSide;
+        GroupCaps caps{(int64_t)S * S, (int64_t)(S + 1) * S, (int64_t)S * (S + 1), 1};
+        if (g.init(pde, N, caps, ops, persist_blocks, st, variant)) return;
+        SolveState s0{};
+        s0.dt = 1.0e-4;
+        s0.tend = 1.0e30;
+        s0.dtmin = ~0ull;
+        s0.active = 1;
+        s0.max_steps = INT_MAX;
+        int slot = -1;
+        if (g.add(mesh_from_desc(&d), s0, &slot)) return;
+        ok = launch_init(pde, N, g.b, ic, slot, S * S, 0, st) == hipSuccess && g.launch(kPartPredictor) == hipSuccess;
+    }
+};
+
+// Milliseconds per launch of kTuneReps predictor launches, or -1.
+double time_launches(const Trial &t, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
+{
+    bool ok = hipEventRecord(e0, st) == hipSuccess;
+    for (int k = 0; k < kTuneReps && ok; k++) ok = t.g.launch(kPartPredictor) == hipSuccess;
+    ok = ok && hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+    float ms = 0.0f;
+    if (!ok || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.0;
+    return ms / kTuneReps;
+}
+
+int env_variant()
+{
+    const char *e = std::getenv("ADERDG_PREDICTOR_VARIANT");
+    if (!e || !*e) return -1;
+    return std::atoi(e);
+}
+
+}  // namespace
+
+int tune_predictor(int pde, int N, int persist_blocks, const Operators &ops, hipStream_t st)
+{
+    if (pde == kPdeAd || N < 1 || N > 4) return kPredLine;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        const int f = g_forced[pde][N];
+        if (f >= 0 && pred_variant_valid(pde, N, f)) return f;
+    }
+    const int ev = env_variant();
+    if (ev >= 0 && pred_variant_valid(pde, N, ev)) return ev;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const auto key = std::make_tuple(dev, pde, N, persist_blocks > 0 ? 1 : 0);
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_cache.find(key);
+        if (it != g_cache.end()) return it->second.variant;
+    }
+    // every variant's synthetic solve set up first, then timed in interleaved rounds
+    // (the minimum over rounds), so clock ramps and cold caches hit all of them alike
+    TuneEntry t;
+    Trial *trials[kNumPredVariants] = {};
+    for (int v = 0; v < kNumPredVariants; v++)
+        if (pred_variant_valid(pde, N, v)) trials[v] = new Trial(pde, N, v, persist_blocks, ops, st);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+        for (int r = 0; r < kTuneRounds; r++)
+            for (int v = 0; v < kNumPredVariants; v++) {
+                if (!trials[v] || !trials[v]->ok) continue;
+                const double ms = time_launches(*trials[v], st, e0, e1);
+                if (ms > 0.0 && (t.ms[v] < 0.0 || ms < t.ms[v])) t.ms[v] = ms;
+            }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(st);
+    for (int v = 0; v < kNumPredVariants; v++) delete trials[v];
+    double best = -1.0;
+    for (int v = 0; v < kNumPredVariants; v++)
+        if (t.ms[v] > 0.0 && (best < 0.0 || t.ms[v] < best)) {
+            best = t.ms[v];
+            t.variant = v;
+        }
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_cache[key] = t;
+    return t.variant;
+}
+
+}  // namespace aderdg
+
+extern "C" {
+
+int aderdg_tune_force(int pde, int N, int variant)
+{
+    using namespace aderdg;
+    if (pde < 0 || pde > 2 || N < 1 || N > 4) return fail(ADERDG_EINVAL, "aderdg_tune_force: bad pde %d / N %d", pde, N);
+    if (variant >= 0 && !pred_variant_valid(pde, N, variant))
+        return fail(ADERDG_EINVAL, "aderdg_tune_force: variant %d does not exist for pde %d, N=%d", variant, pde, N);
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_forced[pde][N] = variant < 0 ? -1 : variant;
+    return 0;
+}
+
+int aderdg_tune_query(int device, int pde, int N, int persistent, int *variant, double ms[ADERDG_PREDICTOR_VARIANTS])
+{
+    using namespace aderdg;
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_cache.find(std::make_tuple(device, pde, N, persistent ? 1 : 0));
+    if (it == g_cache.end()) return fail(ADERDG_EINVAL, "aderdg_tune_query: (pde %d, N=%d) not tuned on device %d", pde, N, device);
+    if (variant) *variant = it->second.variant;
+    for (int k = 0; ms && k < ADERDG_PREDICTOR_VARIANTS; k++) ms[k] = it->second.ms[k];
+    return 0;
+}
+
+}  // extern "C"

@@ -185,6 +185,24 @@ int aderdg_predictor_cells_bn(aderdg_ctx *ctx, int ncells, double dt, const doub
 int aderdg_timing_enable(aderdg_ctx *ctx, int on);
 int aderdg_timing_get(aderdg_ctx *ctx, double ms[ADERDG_TIMING_SLOTS], int64_t launches[ADERDG_TIMING_SLOTS]);
 
+/* Predictor kernel autotuning.  No counterpart in the reference (a serial CPU
+ * program).  The predictor has ADERDG_PREDICTOR_VARIANTS kernel variants with
+ * identical results: 0 time-line threads (one thread per spatial node holds its
+ * time line), 1/2/3 space-time-node threads in workgroups of 64/128/256 (1 is
+ * absent for N=4, whose space-time cell has 125 nodes).  The first context or
+ * batch arena of a (device, pde, N, launch mode) times every variant on a
+ * synthetic 192x192 solve and keeps the fastest for the process.
+ * aderdg_tune_force(pde, N, v) makes contexts and arenas created afterwards use
+ * variant v (-1: back to the tuner); the environment variable
+ * ADERDG_PREDICTOR_VARIANT=<v> does the same for every (pde, N).
+ * aderdg_tune_query returns the tuner's choice and the milliseconds per
+ * launch it measured (-1 for variants it did not run); persistent selects
+ * batch arenas (persistent grids) rather than single contexts. */
+#define ADERDG_PREDICTOR_VARIANTS 4
+int aderdg_tune_force(int pde, int N, int variant);
+int aderdg_tune_query(int device, int pde, int N, int persistent, int *variant,
+                      double ms[ADERDG_PREDICTOR_VARIANTS]);
+
 /* Algorithmic floating-point operations of one cell-step of the predictor
  * launch (predictor + volume integral) and of the update launch, counted as
  * + - * / sqrt = 1 (the convention of the oracle's op counter).  For
