@@ -396,7 +396,7 @@ class Solver:
         return list(ms), list(ln)
 
 
-PREDICTOR_VARIANTS = ("time-line", "node-64", "node-128", "node-256")
+PREDICTOR_VARIANTS = ("time-line", "node-64", "node-128", "node-256", "time-line-lean")
 
 
 def tune_force(pde, N, variant, exact=False):
@@ -422,7 +422,7 @@ def pred_variant_valid(pde, N, variant):
     k = PDE[pde] if isinstance(pde, str) else int(pde)
     if k == 2:
         return variant == 0
-    return variant in (0, 2, 3) or (variant == 1 and (N + 1) ** 3 <= 64)
+    return variant in (0, 2, 3, 4) or (variant == 1 and (N + 1) ** 3 <= 64)
 
 
 def solve_footprint(pde, N, imax, jmax, exact=False):

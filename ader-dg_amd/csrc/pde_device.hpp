@@ -88,7 +88,8 @@ struct EulerPDE {
     static constexpr bool kNcp = false;
 
     // PDEFlux, Euler/pde.f90:46-69.  F[0..3] x-flux, F[4..7] y-flux.
-    ADG_DEV static void flux(const MeshArgs &m, const double *Q, double *F)
+    template <class M>
+    ADG_DEV static void flux(const M &m, const double *Q, double *F)
     {
         const double g = m.gamma;
         const double irho = 1.0 / Q[0];
@@ -104,7 +105,8 @@ struct EulerPDE {
     }
 
     // maxval(abs(Lambda)) of PDEEigenvalues, Euler/pde.f90:75-91, normal (nx, ny)
-    ADG_DEV static double maxabs_eig(const MeshArgs &m, const double *Q, double nx, double ny)
+    template <class M>
+    ADG_DEV static double maxabs_eig(const M &m, const double *Q, double nx, double ny)
     {
         const double g = m.gamma;
         const double u = (Q[1] * nx + Q[2] * ny) / Q[0];
@@ -119,7 +121,8 @@ struct EulerPDE {
     }
 
     // RusanovFlux, Euler/pde.f90:97-114.  One flux serves both sides: Dm = Dp.
-    ADG_DEV static void face_flux(const MeshArgs &m, const double *QL, const double *FL, const double *QR,
+    template <class M>
+    ADG_DEV static void face_flux(const M &m, const double *QL, const double *FL, const double *QR,
                                   const double *FR, double nx, double ny, double *Dm, double *Dp)
     {
         const double smax = f_max(maxabs_eig(m, QL, nx, ny), maxabs_eig(m, QR, nx, ny));
@@ -131,7 +134,8 @@ struct EulerPDE {
     }
 
     // PDEAssurePositivity, Euler/pde.f90:120-145: true if the state is rejected
-    ADG_DEV static bool inadmissible(const MeshArgs &m, const double *Q)
+    template <class M>
+    ADG_DEV static bool inadmissible(const M &m, const double *Q)
     {
         const double irho = 1.0 / Q[0];
         const double p = (m.gamma - 1.0) * (Q[3] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2]) * irho);
@@ -150,7 +154,8 @@ struct BnPDE {
     static constexpr bool kNcp = true;
 
     // PDEFlux, Baer-Nunziato/pde.f90:74-119.  F[0..8] x-flux, F[9..17] y-flux.
-    ADG_DEV static void flux(const MeshArgs &m, const double *Q, double *F)
+    template <class M>
+    ADG_DEV static void flux(const M &m, const double *Q, double *F)
     {
         const double *e = m.eos;
         const double phi_s = Q[8], phi_g = 1.0 - Q[8];
@@ -181,7 +186,8 @@ struct BnPDE {
     }
 
     // maxval(abs(Lambda)) of PDEEigenvalues, Baer-Nunziato/pde.f90:125-144 (via PDECons2Prim :8-43)
-    ADG_DEV static double maxabs_eig(const MeshArgs &m, const double *Q, double nx, double ny)
+    template <class M>
+    ADG_DEV static double maxabs_eig(const M &m, const double *Q, double nx, double ny)
     {
         const double *e = m.eos;
         const double phi_s = Q[8], phi_g = 1.0 - Q[8];
@@ -206,7 +212,8 @@ struct BnPDE {
     }
 
     // p_g, u_s, v_s as in PDEMatrixB / PDENCP, Baer-Nunziato/pde.f90:163-176, 207-219
-    ADG_DEV static double gas_pressure(const MeshArgs &m, const double *Q, double &u_s, double &v_s)
+    template <class M>
+    ADG_DEV static double gas_pressure(const M &m, const double *Q, double &u_s, double &v_s)
     {
         const double *e = m.eos;
         const double phi_g = 1.0 - Q[8];
@@ -220,7 +227,8 @@ struct BnPDE {
 
     // Column 9 of the Roe matrix: 3-point Gauss-Legendre path integral of PDEMatrixB
     // (RoeMatrix, Baer-Nunziato/pde.f90:296-320; PDEMatrixB :150-186).  The other columns are 0.
-    ADG_DEV static void roe_col9(const MeshArgs &m, const double *Qa, const double *Qb, double nx, double ny,
+    template <class M>
+    ADG_DEV static void roe_col9(const M &m, const double *Qa, const double *Qb, double nx, double ny,
                                  double *B9)
     {
         const double s15 = __builtin_sqrt(15.0);
@@ -261,7 +269,8 @@ struct BnPDE {
     }
 
     // RusanovFlux (path-conservative), Baer-Nunziato/pde.f90:326-353
-    ADG_DEV static void face_flux(const MeshArgs &m, const double *QL, const double *FL, const double *QR,
+    template <class M>
+    ADG_DEV static void face_flux(const M &m, const double *QL, const double *FL, const double *QR,
                                   const double *FR, double nx, double ny, double *Dm, double *Dp)
     {
         const double smax = f_max(maxabs_eig(m, QL, nx, ny), maxabs_eig(m, QR, nx, ny));
@@ -279,7 +288,8 @@ struct BnPDE {
     }
 
     // PDEAssurePositivity, Baer-Nunziato/pde.f90:237-266: true if the state is rejected
-    ADG_DEV static bool inadmissible(const MeshArgs &m, const double *Q)
+    template <class M>
+    ADG_DEV static bool inadmissible(const M &m, const double *Q)
     {
         const double *e = m.eos;
         const double phi_s = Q[8], phi_g = 1.0 - Q[8];
@@ -294,7 +304,8 @@ struct BnPDE {
     }
 
     // PDEFusedSrcNCP = -PDENCP, Baer-Nunziato/pde.f90:277-290, 192-231.  Gx, Gy: gradient components.
-    ADG_DEV static void ncp(const MeshArgs &m, const double *Q, const double *G, double *src)
+    template <class M>
+    ADG_DEV static void ncp(const M &m, const double *Q, const double *G, double *src)
     {
         double u_s, v_s;
         const double p_g = gas_pressure(m, Q, u_s, v_s);

@@ -82,8 +82,8 @@ ADG_DEV double dmr_shock_x(const MeshArgs &m, double time) { return m.dmr_a + m.
 // cell from its time averages in LDS (qhiL, fhxL, fhyL, shL: [v][j][i]; shL is
 // read for Baer-Nunziato only), spread over the cell's nthr threads.  TEST:
 // write the averages and boundary values in the reference layouts instead.
-template <class P, int N, bool TEST>
-ADG_DEV void predictor_finish(const MeshArgs &m, const StepBufs &b, const TestBufs &t, const double *qhiL,
+template <class P, int N, bool TEST, class M>
+ADG_DEV void predictor_finish(const M &m, const StepBufs &b, const TestBufs &t, const double *qhiL,
                               const double *fhxL, const double *fhyL, const double *shL, int tt, int nthr,
                               bool act, int cell, int cell0)
 {
@@ -340,6 +340,196 @@ ADG_DEV void predictor_cells(const MeshArgs &m, const StepBufs &b, const TestBuf
 
 
 // ---------------------------------------------------------------------------
+// Lean time-line variant: the thread/node mapping of predictor_cells with a
+// smaller register footprint.  The right-hand side of slice l replaces q(:, l)
+// in the time line once the slice is done (after the sweep the time line holds
+// rhs(:, 0..N) and the iK1 contraction runs in place, in the same summation
+// order), the initial state and the operator rows live in LDS, the flux
+// exchange is single-buffered (two barriers per slice) and the last sweep,
+// which also accumulates the time averages, is a separate copy of the loop.
+// smem: CPW cells x (V NN state + SLOT exchange) + the Kxi and dudx matrices.
+// ---------------------------------------------------------------------------
+template <class P, int N>
+struct LeanShape {
+    using S = Shape<P, N>;
+    static constexpr int CELL = S::V * S::NN + S::SLOT;   // LDS doubles per cell
+    static constexpr int LDS = S::CPW * CELL + 2 * S::NN;  // + Kxi, dudx
+};
+
+template <class P, int N, bool TEST, class M>
+ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, double *smem, int first,
+                            int cell0, int par, double dt)
+{
+    using S = Shape<P, N>;
+    using L = LeanShape<P, N>;
+    constexpr int n = S::n, NN = S::NN, CPW = S::CPW, V = S::V;
+    const OpPtr op = (OpPtr)b.op;
+    const int tid = threadIdx.x, lc = tid / NN, tt = tid - lc * NN;
+    const int i = tt % n, j = tt / n;
+    const int cell = first + lc;  // local cell of the solve
+    const bool own = lc < CPW;
+    const bool act = own && cell < m.ncell;
+    double *sC = smem + (own ? lc : CPW - 1) * L::CELL;
+    double *sU = sC;            // [v][tt] initial state
+    double *buf = sC + V * NN;  // Fx [v][j][i], Fy [v][i][j], q [v][tt]
+    double *sK = smem + CPW * L::CELL, *sD = sK + NN;
+    for (int k = tid; k < NN; k += blockDim.x) {
+        sK[k] = op->kxi[k];
+        sD[k] = op->dudx[k];
+    }
+    {
+        const double *uh = TEST ? t.uh : b.uh[par];
+        const size_t gu = TEST ? (size_t)cell : (size_t)cell0 + cell;
+        double u[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) u[v] = act ? uh[gu * V * NN + V * tt + v] : 0.0;
+        if (!act) {  // an admissible dummy state for idle lanes
+            if (V == 4) {
+                u[0] = 1.0;
+                u[3] = 2.5;
+            } else {
+                u[0] = 0.5; u[3] = 2.5; u[4] = 0.5; u[7] = 2.5; u[8] = 0.5;
+            }
+        }
+        if (own) {
+#pragma unroll
+            for (int v = 0; v < V; v++) sU[v * NN + tt] = u[v];
+        }
+    }
+    __syncthreads();
+    const double wi = op->w[i], wj = op->w[j];
+    const double f1 = 1.0 / (wi * wj);  // :81
+    double q[V][n];
+#pragma unroll
+    for (int v = 0; v < V; v++)
+#pragma unroll
+        for (int l = 0; l < n; l++) q[v][l] = sU[v * NN + tt];
+    double fh[2][V], sh[V];
+
+    // one Picard sweep, Euler/space_time_predictor.f90:49-84 (Baer-Nunziato/space_time_predictor.f90:54-104)
+    auto sweep = [&](auto last_tag) {
+        constexpr bool LAST = decltype(last_tag)::value;
+        if (LAST) {
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                fh[0][v] = 0.0;
+                fh[1][v] = 0.0;
+                sh[v] = 0.0;
+            }
+        }
+#pragma unroll 1
+        for (int l = 0; l < n; l++) {
+            double Q[V];
+#pragma unroll
+            for (int v = 0; v < V; v++) Q[v] = q[v][0];
+            {
+                double F[2 * V];
+                P::flux(m, Q, F);
+                if (own) {
+#pragma unroll
+                    for (int v = 0; v < V; v++) {
+                        buf[v * NN + j * n + i] = F[v];
+                        buf[V * NN + v * NN + i * n + j] = F[V + v];
+                        if (P::kNcp) buf[2 * V * NN + v * NN + tt] = Q[v];
+                    }
+                }
+                if (LAST) {  // time average of the last sweep's fluxes (:92)
+                    const double wl = op->w[l];
+#pragma unroll
+                    for (int v = 0; v < V; v++) {
+                        fh[0][v] = fh[0][v] + F[v] * wl;
+                        fh[1][v] = fh[1][v] + F[V + v] * wl;
+                    }
+                }
+            }
+            __syncthreads();
+            const double wl = op->w[l];
+            const double sx = wl * wj * dt / m.dx, sy = wl * wi * dt / m.dy;
+            const double c0 = wi * wj * op->f0[l];
+            double rv[V], G[2 * V];
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < n; k++) acc = acc + buf[v * NN + j * n + k] * sK[k + n * i];
+                const double rr = c0 * sU[v * NN + tt] - sx * acc;  // :66
+                acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < n; k++) acc = acc + buf[V * NN + v * NN + i * n + k] * sK[k + n * j];
+                rv[v] = rr - sy * acc;  // :72
+                if (P::kNcp) {          // gradients, Baer-Nunziato/space_time_predictor.f90:72,79
+                    acc = 0.0;
+#pragma unroll
+                    for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + k + n * j] * sD[i + n * k];
+                    G[v] = 1.0 / m.dx * acc;
+                    acc = 0.0;
+#pragma unroll
+                    for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + i + n * k] * sD[j + n * k];
+                    G[V + v] = 1.0 / m.dy * acc;
+                }
+            }
+            __syncthreads();  // the exchange buffer is free for the next slice
+            if (P::kNcp) {  // NCP source, Baer-Nunziato/space_time_predictor.f90:84-92
+                double src[V];
+                P::ncp(m, Q, G, src);
+                const double f = wi * wj * wl * dt;
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    rv[v] = rv[v] + f * src[v];
+                    if (LAST) sh[v] = sh[v] + src[v] * wl;  // lShi (:111)
+                }
+            }
+            // slice l is done: its right-hand side takes its place, and the time line rotates
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+#pragma unroll
+                for (int ll = 0; ll + 1 < n; ll++) q[v][ll] = q[v][ll + 1];
+                q[v][n - 1] = rv[v];
+            }
+        }
+        // time contraction, :79-83: q_new(v,ll) = f1 * sum over m of rhs(v,m) iK1(ll,m)
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            double qn[n];
+#pragma unroll
+            for (int ll = 0; ll < n; ll++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int mm = 0; mm < n; mm++) acc = acc + q[v][mm] * op->ik1[ll + n * mm];
+                qn[ll] = acc;
+            }
+#pragma unroll
+            for (int ll = 0; ll < n; ll++) q[v][ll] = f1 * qn[ll];
+        }
+    };
+#pragma unroll 1
+    for (int it = 0; it < N; it++) sweep(std::false_type{});
+    sweep(std::true_type{});
+
+    double qh[V];  // :90
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int l = 0; l < n; l++) acc = acc + q[v][l] * op->w[l];
+        qh[v] = acc;
+    }
+    double *qhiL = buf, *fhxL = buf + V * NN, *fhyL = sU, *shL = P::kNcp ? buf + 2 * V * NN : sU;
+    __syncthreads();
+    if (own) {
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            qhiL[v * NN + tt] = qh[v];
+            fhxL[v * NN + tt] = fh[0][v];
+            fhyL[v * NN + tt] = fh[1][v];
+            if (P::kNcp) shL[v * NN + tt] = sh[v];
+        }
+    }
+    __syncthreads();
+    predictor_finish<P, N, TEST>(m, b, t, qhiL, fhxL, fhyL, shL, tt, NN, act, cell, cell0);
+}
+
+// ---------------------------------------------------------------------------
 // Space-time-node variant of the predictor: thread (cell, i, j, l) owns the
 // state q(:, i, j, l) of one space-time node, so a cell's whole Picard sweep
 // runs at once on (N+1)^3 threads and each thread holds one node instead of a
@@ -588,6 +778,34 @@ __global__ __launch_bounds__(kPredBlock<P>) void k_predictor_test(MeshArgs m, St
     predictor_cells<P, N, true>(m, b, t, smem, blockIdx.x * S::CPW, 0, 0, dt);
 }
 
+
+// at least two waves per SIMD: the lean layout is meant to fit 256 registers
+template <class P, int N>
+__global__ __launch_bounds__(kPredBlock<P>) __attribute__((amdgpu_waves_per_eu(2))) void k_predictor_lean(StepBufs b)
+{
+    __shared__ double smem[LeanShape<P, N>::LDS];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        b.counters[kCntList] = 0;
+        b.counters[kCntTroubled] = 0;
+    }
+    const int ntab = ld_const(b.meta + kTabPred);
+    for (int e = xcd_entry(blockIdx.x, gridDim.x); e < ntab; e += gridDim.x) {
+        const int2 w = ld_const(b.tab[kTabPred] + e);
+        const SolveState st = ld_const(b.state + w.x);
+        if (!st.active) continue;
+        // the solve's constants stay in the constant address space, read where used
+        const ADG_CONST SolveArgs *sa = (const ADG_CONST SolveArgs *)(b.solves + w.x);
+        predictor_lean<P, N, false>(sa->m, b, TestBufs{}, smem, w.y, sa->cell0, st.parity, st.dt);
+        __syncthreads();
+    }
+}
+
+template <class P, int N>
+__global__ __launch_bounds__(kPredBlock<P>) void k_predictor_lean_test(MeshArgs m, StepBufs b, TestBufs t, double dt)
+{
+    __shared__ double smem[LeanShape<P, N>::LDS];
+    predictor_lean<P, N, true>(m, b, t, smem, blockIdx.x * Shape<P, N>::CPW, 0, 0, dt);
+}
 
 template <class P, int N, int BLK>
 __global__ __launch_bounds__(BLK) void k_predictor_nodes(StepBufs b)
@@ -1682,6 +1900,9 @@ hipError_t launch_predictor(const StepBufs &b, hipStream_t st, int variant, int 
     case kPredNode256:
         hipLaunchKernelGGL((k_predictor_nodes<P, N, 256>), dim3(grid), dim3(256), 0, st, b);
         break;
+    case kPredLean:
+        hipLaunchKernelGGL((k_predictor_lean<P, N>), dim3(grid), dim3(S::PRED_BLOCK), 0, st, b);
+        break;
     default:
         return hipErrorInvalidValue;
     }
@@ -1766,6 +1987,11 @@ hipError_t launch_test_impl(const MeshArgs &m, const StepBufs &b, const TestBufs
         hipLaunchKernelGGL((k_predictor_nodes_test<P, N, 256>), dim3(g), dim3(256), 0, st, m, b, t, dt);
         break;
     }
+    case kPredLean: {
+        const int g = (m.ncell + S::CPW - 1) / S::CPW;
+        hipLaunchKernelGGL((k_predictor_lean_test<P, N>), dim3(g), dim3(S::PRED_BLOCK), 0, st, m, b, t, dt);
+        break;
+    }
     default:
         return hipErrorInvalidValue;
     }
@@ -1789,7 +2015,7 @@ template <class P, int N>
 void quanta_impl(int variant, int q[kNumTabs])
 {
     constexpr int N3 = (N + 1) * (N + 1) * (N + 1);
-    q[kTabPred] = variant == kPredLine ? Shape<P, N>::CPW
+    q[kTabPred] = (variant == kPredLine || variant == kPredLean) ? Shape<P, N>::CPW
                   : variant == kPredNode64 ? 64 / N3 : variant == kPredNode128 ? 128 / N3 : 256 / N3;
     q[kTabUpd] = Shape<P, N>::CPWU;
 }
@@ -1813,7 +2039,7 @@ bool pred_variant_valid(int pde, int N, int variant)
 {
     if (N < 1 || N > 4) return false;
     if (pde == kPdeAd) return variant == kPredLine;
-    if (variant == kPredLine || variant == kPredNode128 || variant == kPredNode256) return true;
+    if (variant == kPredLine || variant == kPredNode128 || variant == kPredNode256 || variant == kPredLean) return true;
     return variant == kPredNode64 && (N + 1) * (N + 1) * (N + 1) <= 64;
 }
 

@@ -92,10 +92,11 @@ struct TestBufs {
 };
 
 // Predictor kernels: time-line threads (one thread per spatial node holds the node's
-// whole time line) or space-time-node threads (one thread per node of the space-time
-// cell) in workgroups of 64, 128 or 256.  All give the same bits; the autotuner
+// whole time line; kPredLean: the same mapping with a smaller register footprint) or
+// space-time-node threads (one thread per node of the space-time cell) in workgroups
+// of 64, 128 or 256.  All give the same bits; the autotuner
 // (tune.hip) picks one per (PDE, N).  Advection-diffusion has the time-line form only.
-enum { kPredLine = 0, kPredNode64 = 1, kPredNode128 = 2, kPredNode256 = 3, kNumPredVariants = 4 };
+enum { kPredLine = 0, kPredNode64 = 1, kPredNode128 = 2, kPredNode256 = 3, kPredLean = 4, kNumPredVariants = 5 };
 bool pred_variant_valid(int pde, int N, int variant);
 int pred_cells_per_block(int pde, int N, int variant);
 

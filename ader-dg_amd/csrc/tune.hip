@@ -1,8 +1,8 @@
 // tune.hip -- autotuner of the predictor kernel.
 //
-// The predictor has several kernel variants (step_kernels.hpp: time-line threads,
-// or space-time-node threads in workgroups of 64, 128 or 256, i.e. 1..32 cells per
-// workgroup) that give the same bits and differ in registers, LDS and waves per
+// The predictor has several kernel variants (step_kernels.hpp: time-line threads
+// in two register layouts, or space-time-node threads in workgroups of 64, 128 or
+// 256, i.e. 1..32 cells per workgroup) that give the same bits and differ in registers, LDS and waves per
 // SIMD.  The fastest depends on (PDE, N) and on the launch mode (one workgroup per
 // table entry, or a persistent grid), so the first group of a (device, PDE, N,
 // mode) times every variant on a synthetic 192 x 192 solve and keeps the fastest
@@ -27,7 +27,7 @@ constexpr int kTuneRounds = 3, kTuneReps = 3;
 
 struct TuneEntry {
     int variant = kPredLine;
-    double ms[kNumPredVariants] = {-1.0, -1.0, -1.0, -1.0};
+    double ms[kNumPredVariants] = {-1.0, -1.0, -1.0, -1.0, -1.0};
 };
 
 std::mutex g_mu;

@@ -189,7 +189,9 @@ int aderdg_timing_get(aderdg_ctx *ctx, double ms[ADERDG_TIMING_SLOTS], int64_t l
  * program).  The predictor has ADERDG_PREDICTOR_VARIANTS kernel variants with
  * identical results: 0 time-line threads (one thread per spatial node holds its
  * time line), 1/2/3 space-time-node threads in workgroups of 64/128/256 (1 is
- * absent for N=4, whose space-time cell has 125 nodes).  The first context or
+ * absent for N=4, whose space-time cell has 125 nodes), 4 time-line threads
+ * with a smaller register footprint (right-hand sides kept in the time line,
+ * state and operators in LDS).  Advection-diffusion has variant 0 only.  The first context or
  * batch arena of a (device, pde, N, launch mode) times every variant on a
  * synthetic 192x192 solve and keeps the fastest for the process.
  * aderdg_tune_force(pde, N, v) makes contexts and arenas created afterwards use
@@ -198,7 +200,7 @@ int aderdg_timing_get(aderdg_ctx *ctx, double ms[ADERDG_TIMING_SLOTS], int64_t l
  * aderdg_tune_query returns the tuner's choice and the milliseconds per
  * launch it measured (-1 for variants it did not run); persistent selects
  * batch arenas (persistent grids) rather than single contexts. */
-#define ADERDG_PREDICTOR_VARIANTS 4
+#define ADERDG_PREDICTOR_VARIANTS 5
 int aderdg_tune_force(int pde, int N, int variant);
 int aderdg_tune_query(int device, int pde, int N, int persistent, int *variant,
                       double ms[ADERDG_PREDICTOR_VARIANTS]);

@@ -1,8 +1,10 @@
 """Every predictor kernel variant against the oracle, bit for bit (exact build).
 
 The predictor has several kernel forms with the same arithmetic: time-line
-threads (one thread per spatial node, variant 0) and space-time-node threads in
-workgroups of 64, 128 or 256 (variants 1-3). The autotuner picks one per
+threads (one thread per spatial node, variant 0; variant 4 the same mapping with
+the right-hand sides kept in the time line and state/operators in LDS) and
+space-time-node threads in workgroups of 64, 128 or 256 (variants 1-3). The
+autotuner picks one per
 (PDE, N) by speed, so each one is checked here, forced with aderdg_tune_force:
 the reference argument list of ader_space_time_predictor on random cells,
 whole steps with the limiter active through a single context, and a batch
@@ -16,7 +18,7 @@ from helpers import BN_EOS_STIFF, random_bn_cells, random_cells, smooth_mesh
 from pyoracle import BC_DMR, BC_PERIODIC, BC_TRANSMISSIVE, Oracle
 
 pytestmark = pytest.mark.gpu
-CASES = [(pde, N, v) for pde in ("euler", "bn") for N in (1, 2, 3, 4) for v in range(4)
+CASES = [(pde, N, v) for pde in ("euler", "bn") for N in (1, 2, 3, 4) for v in range(len(aderdg.PREDICTOR_VARIANTS))
          if aderdg.pred_variant_valid(pde, N, v)]
 
 
@@ -97,7 +99,7 @@ def test_step_variant_bitwise(pde, N, v):
 
 def test_batch_forced_variants_bitwise():
     """A batch arena per (PDE, N) built with a forced non-default variant (persistent grid)."""
-    forced = {("euler", 2): 2, ("euler", 4): 3, ("bn", 2): 1, ("bn", 3): 2}
+    forced = {("euler", 2): 2, ("euler", 4): 4, ("bn", 2): 1, ("bn", 3): 4}
     for (pde, N), v in forced.items():
         aderdg.tune_force(pde, N, v, exact=True)
     b = aderdg.Batch(exact=True)
