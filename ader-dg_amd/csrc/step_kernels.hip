@@ -66,7 +66,8 @@ struct Shape {
     static constexpr int UPD_LDS = UPD_PART + NPART;
 };
 
-ADG_DEV double dmr_shock_x(const MeshArgs &m, double time) { return m.dmr_a + m.dmr_b * time; }
+template <class M>
+ADG_DEV double dmr_shock_x(const M &m, double time) { return m.dmr_a + m.dmr_b * time; }
 
 
 
@@ -851,8 +852,8 @@ __global__ __launch_bounds__(kBlock) void k_face(StepBufs b)
         const int2 w = ld_const(b.tab[kTabFace] + e);
         const SolveState st = ld_const(b.state + w.x);
         if (!st.active) continue;
-        const SolveArgs sa = ld_const(b.solves + w.x);
-        const MeshArgs &m = sa.m;
+        const ADG_CONST SolveArgs &sa = *(const ADG_CONST SolveArgs *)(b.solves + w.x);
+        const auto &m = sa.m;
         const int I = m.imax, J = m.jmax;
         const long nfx = (long)(I + 1) * J, nfy = (long)I * (J + 1);
         const long t = (long)w.y + threadIdx.x;
@@ -987,8 +988,8 @@ ADG_DEV void cell_ranges_stage1(OpPtr op, const double *uhL, double *tyL, double
 }
 
 // Second passes; returns true if a subcell average is inadmissible (Euler/limiter.f90:227-241)
-template <class P, int N>
-ADG_DEV bool cell_ranges_stage2(OpPtr op, const MeshArgs &m, const double *tyL,
+template <class P, int N, class M>
+ADG_DEV bool cell_ranges_stage2(OpPtr op, const M &m, const double *tyL,
                                 const double *byL, double *limL, double *lobL, int tt, int nthr)
 {
     using S = Shape<P, N>;
@@ -1050,8 +1051,8 @@ ADG_DEV bool dmp_violated(const double r[6], double lmn, double lmx)
 }
 
 // time-step candidate of one node, Euler/calc_time_step.f90:29-42
-template <class P>
-ADG_DEV double node_dt(const MeshArgs &m, const double *Q)
+template <class P, class M>
+ADG_DEV double node_dt(const M &m, const double *Q)
 {
     double denom = 0.0;
     denom = denom + P::maxabs_eig(m, Q, 1.0, 0.0) / m.dx;
@@ -1083,8 +1084,8 @@ __global__ __launch_bounds__(kUpdBlock<P>) void k_update(StepBufs b)
     const int2 w = ld_const(b.tab[kTabUpd] + e);
     const SolveState st = ld_const(b.state + w.x);
     if (SEED ? !st.seed : !st.active) continue;
-    const SolveArgs sa = ld_const(b.solves + w.x);
-    const MeshArgs &m = sa.m;
+    const ADG_CONST SolveArgs &sa = *(const ADG_CONST SolveArgs *)(b.solves + w.x);
+    const auto &m = sa.m;
     const int par = st.parity;
     const int cell = w.y + lc;                 // local cell of the solve
     const size_t g = (size_t)sa.cell0 + cell;  // cell of the group arrays
@@ -1228,8 +1229,8 @@ __global__ __launch_bounds__(kBlock) void k_spread(StepBufs b)
         const int2 w = ld_const(b.tab[kTabCell] + e);
         const SolveState st = ld_const(b.state + w.x);
         if (!st.active) continue;
-        const SolveArgs sa = ld_const(b.solves + w.x);
-        const MeshArgs &m = sa.m;
+        const ADG_CONST SolveArgs &sa = *(const ADG_CONST SolveArgs *)(b.solves + w.x);
+        const auto &m = sa.m;
         const int par = st.parity;
         const int cell = w.y + threadIdx.x;  // local cell of the solve
         const int g = sa.cell0 + cell;
@@ -1274,8 +1275,8 @@ __global__ __launch_bounds__(kBlock) void k_spread(StepBufs b)
 
 // Common tail of both subcell kernels: put_subcell_data (Euler/limiter.f90:86-96),
 // re-DMP of cells marked 2 (:619-638), next-step limiter seeds and dt candidate.
-template <class P, int N>
-ADG_DEV void subcell_finish(const MeshArgs &m, const StepBufs &b, size_t cell, int par, int flag, const double *sSub,
+template <class P, int N, class M>
+ADG_DEV void subcell_finish(const M &m, const StepBufs &b, size_t cell, int par, int flag, const double *sSub,
                             double *sUhy, double *sUh, double *sTy, double *sBy, double *sLim, double *sLob, double *sDt,
                             int *sFlag)
 {
@@ -1381,8 +1382,8 @@ __global__ __launch_bounds__(64) void k_subcell(StepBufs b)
     for (int item = blockIdx.x; item < count; item += gridDim.x) {
         const int gcell = b.list[item];
         const int slot = ld_const(b.cell_solve + gcell);
-        const SolveArgs sa = ld_const(b.solves + slot);
-        const MeshArgs &m = sa.m;
+        const ADG_CONST SolveArgs &sa = *(const ADG_CONST SolveArgs *)(b.solves + slot);
+        const auto &m = sa.m;
         const SolveState st = ld_const(b.state + slot);
         const int par = st.parity;
         const double dt = st.dt, time = st.time;
@@ -1597,8 +1598,8 @@ __global__ __launch_bounds__(64) void k_subcell_bn(StepBufs b)
     for (int item = blockIdx.x; item < count; item += gridDim.x) {
         const int gcell = b.list[item];
         const int slot = ld_const(b.cell_solve + gcell);
-        const SolveArgs sa = ld_const(b.solves + slot);
-        const MeshArgs &m = sa.m;
+        const ADG_CONST SolveArgs &sa = *(const ADG_CONST SolveArgs *)(b.solves + slot);
+        const auto &m = sa.m;
         const SolveState st = ld_const(b.state + slot);
         const int par = st.parity;
         const double dt = st.dt;
@@ -1857,8 +1858,8 @@ __global__ __launch_bounds__(kBlock) void k_init(StepBufs b, IcSpec ic, int slot
 {
 #pragma clang fp contract(off)
     constexpr int V = P::V, n = N + 1, NN = n * n;
-    const SolveArgs sa = ld_const(b.solves + slot);
-    const MeshArgs &m = sa.m;
+    const ADG_CONST SolveArgs &sa = *(const ADG_CONST SolveArgs *)(b.solves + slot);
+    const auto &m = sa.m;
     const OpPtr op = (OpPtr)b.op;
     const long t = blockIdx.x * (long)kBlock + threadIdx.x;
     if (t >= (long)m.ncell * NN) return;

@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per k_predictor launch, from rocprofv3 --pmc FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0),
 # scaled by a 256 MiB device copy; profiles/hbm_traffic_*.md holds the runs.
-PREDICTOR_TRAFFIC_BYTES = {"euler_vortex_512_n4": 965.7e6}
+PREDICTOR_TRAFFIC_BYTES = {"euler_vortex_512_n4": 965.7e6, "bn_ensemble_256_n3": 40.73e9}
 
 PLAIN = {
     # name: (ic, N, imax, jmax, xl, xr, bc)
@@ -324,8 +324,13 @@ def run_ensemble(args, specs):
     value = dof_step * K / batched_s
     dom = max(per_group, key=lambda N: per_group[N]["predictor"] or 0.0)
     fl_pred, _ = aderdg.flops_per_cell(dom, "bn")
+    tuned = {N: aderdg.tune_query("bn", N, True) for N in per_group}
+    kname = {0: "k_predictor", 1: "k_predictor_nodes<.., 64>", 2: "k_predictor_nodes<.., 128>",
+             3: "k_predictor_nodes<.., 256>", 4: "k_predictor_lean"}
+    var = tuned[dom][0] if tuned[dom] else 0
     roof = roofline_of(fl_pred, per_group[dom]["cells"], per_group[dom]["predictor"],
-                       "k_predictor<BnPDE,%d>" % dom, PREDICTOR_TRAFFIC_BYTES.get("%s_n%d" % (args.workload, dom)))
+                       "%s<BnPDE,%d>" % (kname[var], dom),
+                       PREDICTOR_TRAFFIC_BYTES.get("%s_n%d" % (args.workload, dom)))
     out = {
         "metric": "element-DOF updates/s batched vs plain; p50/p99 solve latency at fixed QPS",
         "value": value, "unit": "element-DOF updates/s", "n_gpus": 1, "steps": K, "warmup": W,
@@ -337,6 +342,9 @@ def run_ensemble(args, specs):
         "plain_value": dof_step * K / plain_s if plain_s else None,
         "batched_vs_plain": (plain_s / batched_s) if plain_s else None,
         "roofline": roof,
+        "predictor_autotune": {str(N): {"variant": aderdg.PREDICTOR_VARIANTS[t[0]],
+                                        "ms_per_launch_192x192": dict(zip(aderdg.PREDICTOR_VARIANTS, t[1]))}
+                               for N, t in tuned.items() if t},
     }
     return out
 
