@@ -46,14 +46,29 @@ ADG_DEV int xcd_entry(int blk, int nblk)
 // second argument wins and a NaN first argument loses.
 ADG_DEV double f_max(double a, double b) { return a > b ? a : b; }
 ADG_DEV double f_min(double a, double b) { return a < b ? a : b; }
-// One element of MAXVAL/MINVAL, which skip NaN elements; start from NaN.
-ADG_DEV double fold_max(double m, double e)
+// One element of MAXVAL/MINVAL, which skip NaN elements; start from NaN.  IEEE
+// maxNum/minNum (v_max_f64 / v_min_f64) skip a NaN operand the same way; the two
+// differ only in which zero a fold of +0 and -0 returns, which no comparison or
+// difference downstream can see.
+ADG_DEV double fold_max(double m, double e) { return __builtin_fmax(m, e); }
+ADG_DEV double fold_min(double m, double e) { return __builtin_fmin(m, e); }
+
+// a / b.  The exact build divides (IEEE, the reference's arithmetic); the
+// product build multiplies by a Newton-Raphson-refined hardware reciprocal
+// (v_rcp_f64 and two refinements r <- r + r (1 - b r)) and corrects the product
+// once (q <- q + r (a - b q)), which lands within an ulp or two of the quotient
+// for the finite, non-zero b of admissible states.
+ADG_DEV double rdiv(double a, double b)
 {
-    return __builtin_isnan(e) ? m : ((__builtin_isnan(m) || e > m) ? e : m);
-}
-ADG_DEV double fold_min(double m, double e)
-{
-    return __builtin_isnan(e) ? m : ((__builtin_isnan(m) || e < m) ? e : m);
+#if defined(ADERDG_EXACT) || !defined(__HIP_DEVICE_COMPILE__)
+    return a / b;
+#else
+    double r = __builtin_amdgcn_rcp(b);
+    r = __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
+    r = __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
+    const double q = a * r;
+    return __builtin_fma(__builtin_fma(-b, q, a), r, q);
+#endif
 }
 
 // minmod, Euler/limiter.f90:384-405
@@ -92,7 +107,7 @@ struct EulerPDE {
     ADG_DEV static void flux(const M &m, const double *Q, double *F)
     {
         const double g = m.gamma;
-        const double irho = 1.0 / Q[0];
+        const double irho = rdiv(1.0, Q[0]);
         const double p = (g - 1.0) * (Q[3] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2]) * irho);
         F[0] = Q[1];
         F[1] = irho * Q[1] * Q[1] + p;
@@ -109,9 +124,9 @@ struct EulerPDE {
     ADG_DEV static double maxabs_eig(const M &m, const double *Q, double nx, double ny)
     {
         const double g = m.gamma;
-        const double u = (Q[1] * nx + Q[2] * ny) / Q[0];
-        const double p = (g - 1.0) * (Q[3] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2]) / Q[0]);
-        const double c = __builtin_sqrt(g * p / Q[0]);
+        const double u = rdiv(Q[1] * nx + Q[2] * ny, Q[0]);
+        const double p = (g - 1.0) * (Q[3] - rdiv(0.5 * (Q[1] * Q[1] + Q[2] * Q[2]), Q[0]));
+        const double c = __builtin_sqrt(rdiv(g * p, Q[0]));
         double mx = __builtin_nan("");
         mx = fold_max(mx, __builtin_fabs(u - c));
         mx = fold_max(mx, __builtin_fabs(u));
@@ -137,7 +152,7 @@ struct EulerPDE {
     template <class M>
     ADG_DEV static bool inadmissible(const M &m, const double *Q)
     {
-        const double irho = 1.0 / Q[0];
+        const double irho = rdiv(1.0, Q[0]);
         const double p = (m.gamma - 1.0) * (Q[3] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2]) * irho);
         return (Q[0] <= 1.0e-12) || (p <= 1.0e-12);
     }
@@ -159,10 +174,10 @@ struct BnPDE {
     {
         const double *e = m.eos;
         const double phi_s = Q[8], phi_g = 1.0 - Q[8];
-        const double rho_s = Q[0] / phi_s, rho_g = Q[4] / phi_g;
-        const double u_s = Q[1] / Q[0], v_s = Q[2] / Q[0];
-        const double u_g = Q[5] / Q[4], v_g = Q[6] / Q[4];
-        const double E_s = Q[3] / phi_s, E_g = Q[7] / phi_g;
+        const double rho_s = rdiv(Q[0], phi_s), rho_g = rdiv(Q[4], phi_g);
+        const double u_s = rdiv(Q[1], Q[0]), v_s = rdiv(Q[2], Q[0]);
+        const double u_g = rdiv(Q[5], Q[4]), v_g = rdiv(Q[6], Q[4]);
+        const double E_s = rdiv(Q[3], phi_s), E_g = rdiv(Q[7], phi_g);
         const double p_s = (e[0] - 1.0) * (E_s - 0.5 * rho_s * (u_s * u_s + v_s * v_s)) - e[0] * e[2];
         const double p_g = (e[1] - 1.0) * (E_g - 0.5 * rho_g * (u_g * u_g + v_g * v_g)) - e[1] * e[3];
         F[0] = phi_s * rho_s * u_s;
@@ -191,16 +206,16 @@ struct BnPDE {
     {
         const double *e = m.eos;
         const double phi_s = Q[8], phi_g = 1.0 - Q[8];
-        const double rho_s = Q[0] / phi_s, rho_g = Q[4] / phi_g;
-        const double u_s = Q[1] / Q[0], v_s = Q[2] / Q[0];
-        const double u_g = Q[5] / Q[4], v_g = Q[6] / Q[4];
-        const double E_s = Q[3] / phi_s, E_g = Q[7] / phi_g;
+        const double rho_s = rdiv(Q[0], phi_s), rho_g = rdiv(Q[4], phi_g);
+        const double u_s = rdiv(Q[1], Q[0]), v_s = rdiv(Q[2], Q[0]);
+        const double u_g = rdiv(Q[5], Q[4]), v_g = rdiv(Q[6], Q[4]);
+        const double E_s = rdiv(Q[3], phi_s), E_g = rdiv(Q[7], phi_g);
         const double p_s = (e[0] - 1.0) * (E_s - 0.5 * rho_s * (u_s * u_s + v_s * v_s)) - e[0] * e[2];
         const double p_g = (e[1] - 1.0) * (E_g - 0.5 * rho_g * (u_g * u_g + v_g * v_g)) - e[1] * e[3];
         const double un_s = u_s * nx + v_s * ny;
         const double un_g = u_g * nx + v_g * ny;
-        const double a_s = __builtin_sqrt(e[0] * (p_s + e[2]) / rho_s);
-        const double a_g = __builtin_sqrt(e[1] * (p_g + e[3]) / rho_g);
+        const double a_s = __builtin_sqrt(rdiv(e[0] * (p_s + e[2]), rho_s));
+        const double a_g = __builtin_sqrt(rdiv(e[1] * (p_g + e[3]), rho_g));
         double mx = __builtin_nan("");
         mx = fold_max(mx, __builtin_fabs(un_g - a_g));
         mx = fold_max(mx, __builtin_fabs(un_g + a_g));
@@ -217,11 +232,11 @@ struct BnPDE {
     {
         const double *e = m.eos;
         const double phi_g = 1.0 - Q[8];
-        const double rho_g = Q[4] / phi_g;
-        u_s = Q[1] / Q[0];
-        v_s = Q[2] / Q[0];
-        const double u_g = Q[5] / Q[4], v_g = Q[6] / Q[4];
-        const double E_g = Q[7] / phi_g;
+        const double rho_g = rdiv(Q[4], phi_g);
+        u_s = rdiv(Q[1], Q[0]);
+        v_s = rdiv(Q[2], Q[0]);
+        const double u_g = rdiv(Q[5], Q[4]), v_g = rdiv(Q[6], Q[4]);
+        const double E_g = rdiv(Q[7], phi_g);
         return (e[1] - 1.0) * (E_g - 0.5 * rho_g * (u_g * u_g + v_g * v_g)) - e[1] * e[3];
     }
 
@@ -293,10 +308,10 @@ struct BnPDE {
     {
         const double *e = m.eos;
         const double phi_s = Q[8], phi_g = 1.0 - Q[8];
-        const double rho_s = Q[0] / phi_s, rho_g = Q[4] / phi_g;
-        const double u_s = Q[1] / Q[0], v_s = Q[2] / Q[0];
-        const double u_g = Q[5] / Q[4], v_g = Q[6] / Q[4];
-        const double E_s = Q[3] / phi_s, E_g = Q[7] / phi_g;
+        const double rho_s = rdiv(Q[0], phi_s), rho_g = rdiv(Q[4], phi_g);
+        const double u_s = rdiv(Q[1], Q[0]), v_s = rdiv(Q[2], Q[0]);
+        const double u_g = rdiv(Q[5], Q[4]), v_g = rdiv(Q[6], Q[4]);
+        const double E_s = rdiv(Q[3], phi_s), E_g = rdiv(Q[7], phi_g);
         const double p_s = (e[0] - 1.0) * (E_s - 0.5 * rho_s * (u_s * u_s + v_s * v_s)) - e[0] * e[2];
         const double p_g = (e[1] - 1.0) * (E_g - 0.5 * rho_g * (u_g * u_g + v_g * v_g)) - e[1] * e[3];
         return rho_s <= 1.0e-12 || rho_g <= 1.0e-12 || (p_s + e[2]) <= 1.0e-12 || (p_g + e[3]) <= 1.0e-12 ||
