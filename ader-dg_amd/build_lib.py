@@ -4,8 +4,9 @@ Produces, in ader-dg_amd/lib/:
   libaderdg_amd.so        floating-point contraction on (FMA), the product build
   libaderdg_amd_exact.so  contraction off: bit-identical to the reference's
                           expression order, used by the parity tests
-  aderdg_euler            the Fortran driver (fortran/aderdg_main.f90) linked
-                          against libaderdg_amd.so, when flang is available
+  aderdg_euler, aderdg_ad the Fortran drivers (fortran/aderdg_main.f90,
+                          fortran/aderdg_ad_main.f90) linked against
+                          libaderdg_amd.so, when flang is available
 """
 import concurrent.futures
 import os
@@ -67,19 +68,26 @@ def build_library(exact=False, compiled=False):
     return out
 
 
+# Fortran drivers: (executable, main program) per reference time loop
+DRIVERS = (("aderdg_euler", "aderdg_main.f90"), ("aderdg_ad", "aderdg_ad_main.f90"))
+
+
 def build_fortran_driver(exact=False):
     fc = shutil.which("flang") or shutil.which("amdflang")
     if fc is None:
-        print("flang not found: Fortran driver not built", flush=True)
+        print("flang not found: Fortran drivers not built", flush=True)
         return None
     fdir = os.path.join(HERE, "fortran")
-    out = os.path.join(LIB, "aderdg_euler_exact" if exact else "aderdg_euler")
-    mods = os.path.join(LIB, "fmod")
-    os.makedirs(mods, exist_ok=True)
-    srcs = [os.path.join(fdir, f) for f in ("aderdg_hip.f90", "aderdg_main.f90")]
-    _run([fc, "-O2", "-fdefault-real-8", "-J", mods] + srcs +
-         ["-L" + LIB, "-laderdg_amd_exact" if exact else "-laderdg_amd", "-Wl,-rpath,$ORIGIN", "-o", out])
-    return out
+    outs = []
+    for name, main in DRIVERS:
+        out = os.path.join(LIB, name + ("_exact" if exact else ""))
+        mods = os.path.join(LIB, "fmod", name + ("_exact" if exact else ""))
+        os.makedirs(mods, exist_ok=True)
+        srcs = [os.path.join(fdir, f) for f in ("aderdg_hip.f90", main)]
+        _run([fc, "-O2", "-fdefault-real-8", "-J", mods] + srcs +
+             ["-L" + LIB, "-laderdg_amd_exact" if exact else "-laderdg_amd", "-Wl,-rpath,$ORIGIN", "-o", out])
+        outs.append(out)
+    return outs
 
 
 def build_all():

@@ -1097,6 +1097,17 @@ __global__ __launch_bounds__(kUpdBlock<P>) void k_update(StepBufs b)
     double *spart = suh;            // [source][tt], after the maps are read
     double *sred = suh + S::UPD_PART; // [slot]
     if (own && tt == 0) sflag[lc] = 0;
+    // the DMP bounds of this step (lmin/lmax of get_min_max), loaded now so their
+    // latency overlaps the update instead of following the range reduction
+    constexpr int KV = (V + NN - 1) / NN;  // variables per thread in the DMP pass
+    double pmn[KV], pmx[KV];
+#pragma unroll
+    for (int k = 0; k < KV; k++) {
+        const int v = tt + k * NN;
+        const bool ok = !SEED && act && v < V;
+        pmn[k] = ok ? b.lmin[(size_t)v * b.G + g] : 0.0;
+        pmx[k] = ok ? b.lmax[(size_t)v * b.G + g] : 0.0;
+    }
 
     double un[V];
     if (SEED || !act) {
@@ -1188,20 +1199,22 @@ __global__ __launch_bounds__(kUpdBlock<P>) void k_update(StepBufs b)
         sred[slot] = r;
     }
     __syncthreads();
-    for (int v = tt; act && v < V; v += NN) {
+#pragma unroll
+    for (int k = 0; k < KV; k++) {
+        const int v = tt + k * NN;
+        if (!act || v >= V) continue;
         const double r[6] = {sred[0 * V + v], sred[1 * V + v], sred[2 * V + v],
                              sred[3 * V + v], sred[4 * V + v], sred[5 * V + v]};
-        if (!SEED && dmp_violated(r, b.lmin[(size_t)v * b.G + g], b.lmax[(size_t)v * b.G + g]))
-            atomicOr(&sflag[lc], 2);
+        if (!SEED && dmp_violated(r, pmn[k], pmx[k])) atomicOr(&sflag[lc], 2);
         // next step's get_min_max step 1 for a cell without limiter, Euler/limiter.f90:123-132
         double mn = f_min(f_min(r[0], r[2]), r[4]), mx = f_max(f_max(r[1], r[3]), r[5]);
         if (SEED && b.status[par][g] != 0) {  // limited cell: range of its subcell data, :135-139
             const double *L = b.lh[par] + g * V * SS;
             mn = __builtin_nan("");
             mx = mn;
-            for (int k = 0; k < SS; k++) {
-                mn = fold_min(mn, L[v + V * k]);
-                mx = fold_max(mx, L[v + V * k]);
+            for (int kk = 0; kk < SS; kk++) {
+                mn = fold_min(mn, L[v + V * kk]);
+                mx = fold_max(mx, L[v + V * kk]);
             }
         }
         b.lmin_old[(size_t)v * b.G + g] = mn;

@@ -36,7 +36,21 @@ module aderdg_hip
         integer(c_int) :: ad_fixed_predictor = 0, ad_pad = 0
     end type
 
+    ! formula initial conditions (aderdg_ic of include/aderdg.h)
+    integer(c_int), parameter :: ADERDG_IC_VORTEX = 1, ADERDG_IC_QUADRANT = 2, ADERDG_IC_DMR = 3, &
+                                 ADERDG_IC_BN_CIRCLE = 4, ADERDG_IC_BN_DISC = 5, ADERDG_IC_SINE = 6
+    type, bind(C) :: aderdg_ic_t
+        integer(c_int) :: kind = 0, pad = 0
+        real(c_double) :: p(24) = 0.0d0
+    end type
+
     interface
+        integer(c_int) function aderdg_ic_eval(desc, ic, uh) bind(C, name="aderdg_ic_eval")
+            import :: c_int, c_double, aderdg_desc_t, aderdg_ic_t
+            type(aderdg_desc_t), intent(in) :: desc
+            type(aderdg_ic_t), intent(in) :: ic
+            real(c_double), intent(out) :: uh(*)
+        end function
         integer(c_int) function aderdg_create(desc, ctx) bind(C, name="aderdg_create")
             import :: c_int, c_ptr, aderdg_desc_t
             type(aderdg_desc_t), intent(in) :: desc

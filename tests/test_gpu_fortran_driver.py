@@ -74,3 +74,65 @@ def test_driver_vortex_fma(tmp_path):
                                         uh0.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
     ref = oracle_run(N, I, J, (0.0, 0.0), (10.0, 10.0), BC_PERIODIC, uh0, steps, 1e9)
     assert np.linalg.norm((uh - ref).ravel()) <= 1e-12 * np.linalg.norm(ref.ravel())
+
+
+AD_PARAMS = """{cx}
+{cy}
+{a}
+{imax}
+{jmax}
+0.0
+0.0
+1.0
+1.0
+{tend}
+{b1}
+{b2}
+{b3}
+{b4}
+{wi}
+"""
+
+
+def run_ad_driver(tmp_path, exe, N, ic, fused, steps, imax, jmax, c, a, bc, tend, wi=100000):
+    (tmp_path / "params.ini").write_text(AD_PARAMS.format(cx=c[0], cy=c[1], a=a, imax=imax, jmax=jmax, tend=tend,
+                                                          b1=bc[0], b2=bc[1], b3=bc[2], b4=bc[3], wi=wi))
+    dump = tmp_path / "uh.bin"
+    print("driver", exe, N, ic, fused, steps, flush=True)
+    r = subprocess.run([os.path.join(LIB, exe), str(N), str(ic), str(fused), str(steps), str(dump)], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Total CPU time" in r.stdout
+    return np.fromfile(dump, dtype=np.float64).reshape((1, N + 1, N + 1, imax, jmax), order="F"), r.stdout
+
+
+def ad_oracle(N, imax, jmax, c, a, bc, fused, uh0, steps, tend):
+    o = Oracle(N, imax, jmax, (0.0, 0.0), (1.0, 1.0), 1, pde=2, ad=dict(c=c, a=a, bc=bc, fixed=fused))
+    o.set_uh(uh0)
+    o.set_scalars(0.0, 0.0, tend)
+    for _ in range(steps):
+        o.step()
+    return o.get_uh()
+
+
+@pytest.mark.parametrize("fused", [0, 1])
+def test_ad_driver_sine_bitwise(tmp_path, fused):
+    """The advection-diffusion driver (Advection-Diffusion/main.f90 call order) with the sine
+    initial condition, periodic / Neumann boundaries: bit for bit against the oracle."""
+    N, I, J, steps, c, a, bc = 3, 20, 16, 7, (1.0, 0.5), 0.01, (3, 3, 1, 1)
+    uh, out = run_ad_driver(tmp_path, "aderdg_ad_exact", N, 1, fused, steps, I, J, c, a, bc, 1.0, wi=3)
+    uh0 = aderdg.ic_eval(aderdg.ic_sine(1.0, 1.0, 1.0), N, I, J, (0.0, 0.0), (1.0, 1.0), pde="ad")
+    np.testing.assert_array_equal(uh, ad_oracle(N, I, J, c, a, bc, fused, uh0, steps, 1.0))
+    vtk = sorted(p.name for p in tmp_path.glob("Sol-*.vtk"))
+    assert vtk[0] == "Sol-00000000.vtk" and "Sol-00000006.vtk" in vtk
+
+
+def test_ad_driver_reference_params(tmp_path):
+    """The reference's own Advection-Diffusion/params.ini case (u0 = 0, c = (1, 0), a = 1,
+    Dirichlet left / bottom / top, Neumann right) for 20 steps: the Dirichlet sin(pi y) data,
+    evaluated on the device, is within 1e-13 relative L2 of the oracle."""
+    N, I, J, steps, c, a, bc = 3, 50, 50, 20, (1.0, 0.0), 1.0, (2, 1, 2, 2)
+    uh, out = run_ad_driver(tmp_path, "aderdg_ad_exact", N, 0, 0, steps, I, J, c, a, bc, 2.0)
+    ref = ad_oracle(N, I, J, c, a, bc, 0, np.zeros((1, N + 1, N + 1, I, J), order="F"), steps, 2.0)
+    assert np.abs(ref).max() > 1e-3  # the boundary data has entered the domain
+    assert np.linalg.norm((uh - ref).ravel()) <= 1e-13 * np.linalg.norm(ref.ravel())
