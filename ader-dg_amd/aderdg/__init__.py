@@ -216,6 +216,9 @@ def load_library(exact=False):
         "aderdg_server_destroy": [vp],
         "aderdg_batch_timing_get": [vp, ctypes.c_int, ctypes.c_int, _dp, ctypes.POINTER(i64)],
         "aderdg_tune_force": [ctypes.c_int, ctypes.c_int, ctypes.c_int],
+        "aderdg_tune_force_kernel": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+        "aderdg_tune_query_kernel": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_int), _dp],
         "aderdg_tune_query": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp],
     }
     for name, args in bsig.items():
@@ -399,21 +402,26 @@ class Solver:
 PREDICTOR_VARIANTS = ("time-line", "node-64", "node-128", "node-256", "time-line-lean")
 
 
-def tune_force(pde, N, variant, exact=False):
-    """Predictor kernel variant for contexts / arenas created afterwards (-1: the autotuner)."""
+UPDATE_VARIANTS = ("recompute", "shared")
+_KERNELS = {"predictor": (0, PREDICTOR_VARIANTS), "update": (1, UPDATE_VARIANTS)}
+
+
+def tune_force(pde, N, variant, exact=False, kernel="predictor"):
+    """Kernel variant for contexts / arenas created afterwards (-1: the autotuner)."""
     lib = load_library(exact)
     k = PDE[pde] if isinstance(pde, str) else int(pde)
-    if lib.aderdg_tune_force(k, N, int(variant)) != 0:
+    if lib.aderdg_tune_force_kernel(_KERNELS[kernel][0], k, N, int(variant)) != 0:
         raise AderdgError(lib.aderdg_last_error().decode())
 
 
-def tune_query(pde, N, persistent, device=0, exact=False):
+def tune_query(pde, N, persistent, device=0, exact=False, kernel="predictor"):
     """(chosen variant, ms per launch of each variant) the autotuner measured, or None."""
     lib = load_library(exact)
     k = PDE[pde] if isinstance(pde, str) else int(pde)
+    kid, names = _KERNELS[kernel]
     v = ctypes.c_int()
-    ms = (ctypes.c_double * len(PREDICTOR_VARIANTS))()
-    if lib.aderdg_tune_query(device, k, N, 1 if persistent else 0, ctypes.byref(v), ms) != 0:
+    ms = (ctypes.c_double * len(names))()
+    if lib.aderdg_tune_query_kernel(kid, device, k, N, 1 if persistent else 0, ctypes.byref(v), ms) != 0:
         return None
     return v.value, list(ms)
 

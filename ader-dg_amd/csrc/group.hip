@@ -144,7 +144,7 @@ void Group::footprint(int pde, int N, int64_t *per_cell, int64_t *per_xface, int
 }
 
 int Group::init(int pde_, int N_, const GroupCaps &c, const Operators &ops_, int persist_blocks, hipStream_t st,
-                 int variant_)
+                 int variant_, int upd_)
 {
     pde = pde_;
     N = N_;
@@ -156,8 +156,16 @@ int Group::init(int pde_, int N_, const GroupCaps &c, const Operators &ops_, int
     st_ = st;
     if (c.cells < 1 || c.slots < 1 || c.cells > (1LL << 30) || c.xfaces > (1LL << 30) || c.yfaces > (1LL << 30))
         return ADERDG_EINVAL;
-    variant = variant_ >= 0 ? variant_ : tune_predictor(pde, N, persist_blocks, ops_, st);
-    if (!pred_variant_valid(pde, N, variant)) return ADERDG_EINVAL;
+    if (variant_ < 0 || upd_ < 0) {
+        int tp = kPredLine, tu = kUpdShared;
+        tune_kernels(pde, N, persist_blocks, ops_, st, &tp, &tu);
+        variant = variant_ >= 0 ? variant_ : tp;
+        upd = upd_ >= 0 ? upd_ : tu;
+    } else {
+        variant = variant_;
+        upd = upd_;
+    }
+    if (!pred_variant_valid(pde, N, variant) || upd < 0 || upd >= kNumUpdVariants) return ADERDG_EINVAL;
     table_quanta(pde, N, variant, quanta);
     const int64_t items[kNumTabs] = {c.cells, c.cells, (c.xfaces + c.yfaces) * n, c.cells};
     int64_t max_entries = 0;
@@ -377,7 +385,7 @@ int Group::read_states(SolveState *out)
 
 hipError_t Group::launch(int part) const
 {
-    return launch_part(pde, N, variant, b, st_, part, grid, caps.slots);
+    return launch_part(pde, N, variant, upd, b, st_, part, grid, caps.slots);
 }
 
 }  // namespace aderdg

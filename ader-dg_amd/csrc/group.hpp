@@ -64,6 +64,7 @@ public:
     GroupCaps caps{};
     Operators ops{};
     int variant = kPredLine;  // predictor kernel variant (step_kernels.hpp)
+    int upd = kUpdShared;     // update kernel variant
     int quanta[kNumTabs] = {};
     int grid[kNumTabs] = {};
     int64_t tab_cap[kNumTabs] = {};
@@ -74,7 +75,7 @@ public:
     // (0: one per table entry of a full arena); variant: predictor kernel variant,
     // or -1 for the autotuner's choice (tune.hip).
     int init(int pde, int N, const GroupCaps &caps, const Operators &ops, int persist_blocks, hipStream_t st,
-             int variant = -1);
+             int variant = -1, int upd = -1);
     void release();
 
     // Place a solve; returns 0, or ADERDG_EFULL when no slot or range is free.
@@ -116,8 +117,8 @@ private:
     int dalloc(T **p, size_t count);
 };
 
-// The autotuner's predictor variant for a (PDE, N) and launch mode on the current
-// device (tune.hip); times the variants the first time it is asked.
-int tune_predictor(int pde, int N, int persist_blocks, const Operators &ops, hipStream_t st);
+// The autotuner's predictor and update variants for a (PDE, N) and launch mode on
+// the current device (tune.hip); times the variants the first time it is asked.
+void tune_kernels(int pde, int N, int persist_blocks, const Operators &ops, hipStream_t st, int *pred, int *upd);
 
 }  // namespace aderdg

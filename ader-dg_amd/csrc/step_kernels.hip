@@ -56,14 +56,8 @@ struct Shape {
     static constexpr int CPWU = UPD_BLOCK / NN;   // cells per update workgroup
     // predictor LDS per cell and time slice: Fx, Fy (and q for the NCP gradients)
     static constexpr int SLOT = (P::kNcp ? 3 : 2) * V * NN;
-    // per-cell LDS of the update: [suh | limy | loby], reused for the per-thread partials
-    // (nodal, Lobatto, subcell min, subcell max per v, dt) once the maps are read, then the
-    // reduced ranges (min,max) of nodal / Lobatto / subcell values per v and dt
+    // range slots of the update per cell: (min, max) of nodal / Lobatto / subcell values per v, dt
     static constexpr int NPART = 6 * V + 1;
-    static constexpr int NSRC = 4 * V + 1;
-    static constexpr int UPD_MAPS = V * NN + V * n * s + V * NN;
-    static constexpr int UPD_PART = (UPD_MAPS > NSRC * NN ? UPD_MAPS : NSRC * NN);
-    static constexpr int UPD_LDS = UPD_PART + NPART;
 };
 
 template <class M>
@@ -1061,18 +1055,28 @@ ADG_DEV double node_dt(const M &m, const double *Q)
 }
 
 // ---------------------------------------------------------------------------
-// surface integral + element update + DMP (SEED = false), or only the
-// per-cell limiter seeds of a freshly uploaded solution (SEED = true).
-// Thread (cell, i, j) updates node (i,j).  The ranges the DMP needs (min/max
-// of nodal, Lobatto and subcell values) are folded per thread while the maps
-// are evaluated and reduced across the cell's threads afterwards.
+// Shared-map variant of the update (the same results as k_update): the first
+// passes of the subcell and Lobatto maps (limy, loby) are formed once per cell
+// in LDS by the cell's threads, the per-thread (min, max) partials reuse that
+// LDS once the maps are read, and the cell's threads reduce them.  More LDS per
+// cell, fewer FMAs than k_update: faster where N is large relative to nVar.
 // ---------------------------------------------------------------------------
+template <class P, int N>
+struct SharedUpdShape {
+    using S = Shape<P, N>;
+    static constexpr int NSRC = 4 * S::V + 1;  // nodal, Lobatto, subcell min, subcell max per v, dt
+    static constexpr int MAPS = S::V * S::NN + S::V * S::n * S::s + S::V * S::NN;
+    static constexpr int PART = MAPS > NSRC * S::NN ? MAPS : NSRC * S::NN;
+    static constexpr int LDS = PART + S::NPART;
+};
+
 template <class P, int N, bool SEED>
-__global__ __launch_bounds__(kUpdBlock<P>) void k_update(StepBufs b)
+__global__ __launch_bounds__(kUpdBlock<P>) void k_update_shared(StepBufs b)
 {
     using S = Shape<P, N>;
+    using U = SharedUpdShape<P, N>;
     constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, CPW = S::CPWU, V = S::V, NP = S::NPART;
-    __shared__ double lds[CPW][S::UPD_LDS];
+    __shared__ double lds[CPW][U::LDS];
     __shared__ int sflag[CPW];
     const OpPtr op = (OpPtr)b.op;
     const int tid = threadIdx.x, lc = tid / NN, tt = tid - lc * NN;
@@ -1095,7 +1099,7 @@ __global__ __launch_bounds__(kUpdBlock<P>) void k_update(StepBufs b)
     double *sty = suh + V * NN;     // limy [v][a + n q]
     double *sby = sty + V * n * s;  // loby
     double *spart = suh;            // [source][tt], after the maps are read
-    double *sred = suh + S::UPD_PART; // [slot]
+    double *sred = suh + U::PART;   // [slot]
     if (own && tt == 0) sflag[lc] = 0;
     // the DMP bounds of this step (lmin/lmax of get_min_max), loaded now so their
     // latency overlaps the update instead of following the range reduction
@@ -1221,6 +1225,234 @@ __global__ __launch_bounds__(kUpdBlock<P>) void k_update(StepBufs b)
         b.lmax_old[(size_t)v * b.G + g] = mx;
     }
     if (act && tt == NN - 1) b.cell_dt[g] = sred[6 * V];
+    __syncthreads();
+    if (!SEED && act && tt == 0) {
+        const int tr = sflag[lc] != 0;
+        b.rc[g] = tr;
+        if (tr) atomicAdd(&b.counters[kCntTroubled], 1);
+    }
+    __syncthreads();
+    }
+}
+
+// Order-preserving 64-bit key of a double that is not NaN: unsigned key order is
+// the numeric order, with -0 below +0, so an LDS atomic min/max of keys takes the
+// min/max of the values independently of the order the threads arrive in.
+ADG_DEV unsigned long long ord_key(double x)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+// Inverse of ord_key; the untouched slot values (~0 for a min, 0 for a max) give NaN,
+// the result of MINVAL/MAXVAL folds that met only NaN.
+ADG_DEV double key_val(unsigned long long k)
+{
+    if (k == ~0ull || k == 0ull) return __builtin_nan("");
+    const unsigned long long u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __longlong_as_double((long long)u);
+}
+ADG_DEV void key_min(unsigned long long *slot, double x)
+{
+    if (!__builtin_isnan(x)) atomicMin(slot, ord_key(x));
+}
+ADG_DEV void key_max(unsigned long long *slot, double x)
+{
+    if (!__builtin_isnan(x)) atomicMax(slot, ord_key(x));
+}
+
+// ---------------------------------------------------------------------------
+// surface integral + element update + DMP (SEED = false), or only the
+// per-cell limiter seeds of a freshly uploaded solution (SEED = true).
+// Thread (cell, i, j) updates node (i,j).  The ranges the DMP needs (min/max
+// of nodal, Lobatto and subcell values) come from the cell's nodal values in
+// LDS: each thread forms the first (y) pass of the separable subcell and
+// Lobatto maps for the columns its points need, in the order of
+// get_subcell_data / get_lobatto_data, and folds its values into per-cell
+// slots with LDS atomics on order-preserving keys.  Only the nodal values and
+// the slots live in LDS (1.6 KB per Baer-Nunziato N=3 cell).
+// ---------------------------------------------------------------------------
+template <class P, int N, bool SEED>
+__global__ __launch_bounds__(kUpdBlock<P>) void k_update(StepBufs b)
+{
+    using S = Shape<P, N>;
+    constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, CPW = S::CPWU, V = S::V, NP = S::NPART;
+    __shared__ double ssuh[CPW][V * NN];
+    __shared__ unsigned long long skey[CPW][NP];
+    __shared__ int sflag[CPW];
+    const OpPtr op = (OpPtr)b.op;
+    const int tid = threadIdx.x, lc = tid / NN, tt = tid - lc * NN;
+    const int i = tt % n, j = tt / n;
+    const bool own = lc < CPW;
+    const int lcs = own ? lc : CPW - 1;
+    const int ntab = ld_const(b.meta + kTabUpd);
+    for (int e = xcd_entry(blockIdx.x, gridDim.x); e < ntab; e += gridDim.x) {
+    const int2 w = ld_const(b.tab[kTabUpd] + e);
+    const SolveState st = ld_const(b.state + w.x);
+    if (SEED ? !st.seed : !st.active) continue;
+    const ADG_CONST SolveArgs &sa = *(const ADG_CONST SolveArgs *)(b.solves + w.x);
+    const auto &m = sa.m;
+    const int par = st.parity;
+    const int cell = w.y + lc;                 // local cell of the solve
+    const size_t g = (size_t)sa.cell0 + cell;  // cell of the group arrays
+    const bool act = own && cell < m.ncell;
+    const double *uh_cur = b.uh[par];
+    double *suh = &ssuh[lcs][0];
+    unsigned long long *key = &skey[lcs][0];  // [kind * V + v], kind 0/1 nodal min/max,
+                                              // 2/3 Lobatto, 4/5 subcell; [6V] dt
+    if (own) {
+        if (tt == 0) sflag[lc] = 0;
+        for (int slot = tt; slot < NP; slot += NN)
+            key[slot] = (slot < 6 * V && ((slot / V) & 1)) ? 0ull : ~0ull;
+    }
+    // the DMP bounds of this step (lmin/lmax of get_min_max), loaded now so their
+    // latency overlaps the update instead of following the range reduction
+    constexpr int KV = (V + NN - 1) / NN;  // variables per thread in the DMP pass
+    double pmn[KV], pmx[KV];
+#pragma unroll
+    for (int k = 0; k < KV; k++) {
+        const int v = tt + k * NN;
+        const bool ok = !SEED && act && v < V;
+        pmn[k] = ok ? b.lmin[(size_t)v * b.G + g] : 0.0;
+        pmx[k] = ok ? b.lmax[(size_t)v * b.G + g] : 0.0;
+    }
+
+    double un[V];
+    if (SEED || !act) {
+#pragma unroll
+        for (int v = 0; v < V; v++) un[v] = act ? uh_cur[g * V * NN + V * tt + v] : (v == 0 || v == 3 ? 1.0 : 0.0);
+    } else {
+        const int ci = cell % m.imax, cj = cell / m.imax;
+        const size_t nfx = b.GX, nfy = b.GY;
+        const size_t fl = sa.fx0 + ci + (size_t)(m.imax + 1) * cj, fb = sa.fy0 + ci + (size_t)m.imax * cj;
+        const int sp = P::kNcp ? 1 : 0;  // right/top faces take D+ (Baer-Nunziato/main.f90:68)
+        const double wi = op->w[i], wj = op->w[j], dt = st.dt;
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            double d = b.lduh[g * V * NN + V * tt + v];
+            const double uo = uh_cur[g * V * NN + V * tt + v];
+            const double Fl = b.xflux[((size_t)(0 * V + v) * n + j) * nfx + fl];
+            const double Fr = b.xflux[((size_t)(sp * V + v) * n + j) * nfx + fl + 1];
+            const double Gb = b.yflux[((size_t)(0 * V + v) * n + i) * nfy + fb];
+            const double Gt = b.yflux[((size_t)(sp * V + v) * n + i) * nfy + fb + m.imax];
+            // surface integral, Euler/surface_integral.f90:182-196
+            d = d - wj / m.dx * (Fr * op->frc[i] - Fl * op->flc[i]);
+            d = d - wi / m.dy * (Gt * op->frc[j] - Gb * op->flc[j]);
+            d = d / (wi * wj);  // element update, Euler/element_update.f90:218-226
+            un[v] = uo + dt * d;
+            b.uh[par ^ 1][g * V * NN + V * tt + v] = un[v];
+        }
+    }
+    if (own) {
+#pragma unroll
+        for (int v = 0; v < V; v++) suh[v * NN + tt] = un[v];
+    }
+    const double dtc = act ? node_dt<P>(m, un) : __builtin_nan("");
+    __syncthreads();  // nodal values and slot initial values in place
+    bool bad = false;
+    if (act) {
+        // nodal ranges and the time-step candidate (Euler/calc_time_step.f90, limiter.f90:123-132)
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            key_min(&key[0 * V + v], un[v]);
+            key_max(&key[1 * V + v], un[v]);
+        }
+        key_min(&key[6 * V], dtc);
+        {  // get_lobatto_data :56-66 at this thread's node (p, q) = (i, j)
+            double ub_q[n], ub_p[n];
+#pragma unroll
+            for (int k = 0; k < n; k++) {
+                ub_q[k] = op->u2b[j + n * k];
+                ub_p[k] = op->u2b[i + n * k];
+            }
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int a = 0; a < n; a++) {
+                    double ly = 0.0;  // loby(a, q), :56-58
+#pragma unroll
+                    for (int bb = 0; bb < n; bb++) ly = ly + suh[v * NN + a + n * bb] * ub_q[bb];
+                    acc = acc + ly * ub_p[a];  // :64-66
+                }
+                key_min(&key[2 * V + v], acc);
+                key_max(&key[3 * V + v], acc);
+                __builtin_amdgcn_sched_barrier(0);  // one variable's loads in flight at a time
+            }
+        }
+        // get_subcell_data :25-35 and positivity :227-241, two subcell points per pass
+        double l0[V], l1[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) l0[v] = l1[v] = __builtin_nan("");
+        for (int t0 = tt; t0 < SS; t0 += 2 * NN) {
+            const int t1 = t0 + NN;
+            const bool two = t1 < SS;
+            const int p0 = t0 % s, q0 = t0 / s, p1 = two ? t1 % s : p0, q1 = two ? t1 / s : q0;
+            double cq0[n], cq1[n], cp0[n], cp1[n];
+#pragma unroll
+            for (int k = 0; k < n; k++) {
+                cq0[k] = op->u2l[q0 + s * k];
+                cq1[k] = op->u2l[q1 + s * k];
+                cp0[k] = op->u2l[p0 + s * k];
+                cp1[k] = op->u2l[p1 + s * k];
+            }
+            double L0[V], L1[V];
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+                for (int a = 0; a < n; a++) {
+                    double y0 = 0.0, y1 = 0.0;  // limy(a, q), :25-27
+#pragma unroll
+                    for (int bb = 0; bb < n; bb++) {
+                        const double u = suh[v * NN + a + n * bb];
+                        y0 = y0 + u * cq0[bb];
+                        y1 = y1 + u * cq1[bb];
+                    }
+                    a0 = a0 + y0 * cp0[a];  // :33-35
+                    a1 = a1 + y1 * cp1[a];
+                }
+                L0[v] = a0;
+                L1[v] = a1;
+                l0[v] = fold_min(l0[v], a0);
+                l1[v] = fold_max(l1[v], a0);
+                if (two) {
+                    l0[v] = fold_min(l0[v], a1);
+                    l1[v] = fold_max(l1[v], a1);
+                }
+                __builtin_amdgcn_sched_barrier(0);  // one variable's loads in flight at a time
+            }
+            bad = bad || P::inadmissible(m, L0) || (two && P::inadmissible(m, L1));
+        }
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            key_min(&key[4 * V + v], l0[v]);
+            key_max(&key[5 * V + v], l1[v]);
+        }
+    }
+    if (bad && !SEED) atomicOr(&sflag[lc], 1);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KV; k++) {
+        const int v = tt + k * NN;
+        if (!act || v >= V) continue;
+        const double r[6] = {key_val(key[0 * V + v]), key_val(key[1 * V + v]), key_val(key[2 * V + v]),
+                             key_val(key[3 * V + v]), key_val(key[4 * V + v]), key_val(key[5 * V + v])};
+        if (!SEED && dmp_violated(r, pmn[k], pmx[k])) atomicOr(&sflag[lc], 2);
+        // next step's get_min_max step 1 for a cell without limiter, Euler/limiter.f90:123-132
+        double mn = f_min(f_min(r[0], r[2]), r[4]), mx = f_max(f_max(r[1], r[3]), r[5]);
+        if (SEED && b.status[par][g] != 0) {  // limited cell: range of its subcell data, :135-139
+            const double *L = b.lh[par] + g * V * SS;
+            mn = __builtin_nan("");
+            mx = mn;
+            for (int kk = 0; kk < SS; kk++) {
+                mn = fold_min(mn, L[v + V * kk]);
+                mx = fold_max(mx, L[v + V * kk]);
+            }
+        }
+        b.lmin_old[(size_t)v * b.G + g] = mn;
+        b.lmax_old[(size_t)v * b.G + g] = mx;
+    }
+    if (act && tt == NN - 1) b.cell_dt[g] = key_val(key[6 * V]);
     __syncthreads();
     if (!SEED && act && tt == 0) {
         const int tr = sflag[lc] != 0;
@@ -1924,7 +2156,8 @@ hipError_t launch_predictor(const StepBufs &b, hipStream_t st, int variant, int 
 }
 
 template <class P, int N>
-hipError_t launch_step_impl(const StepBufs &b, hipStream_t st, int part, const int grid[kNumTabs], int variant)
+hipError_t launch_step_impl(const StepBufs &b, hipStream_t st, int part, const int grid[kNumTabs], int variant,
+                            int upd)
 {
     using S = Shape<P, N>;
     switch (part) {
@@ -1937,10 +2170,16 @@ hipError_t launch_step_impl(const StepBufs &b, hipStream_t st, int part, const i
         hipLaunchKernelGGL((k_face<P, N>), dim3(grid[kTabFace]), dim3(kBlock), 0, st, b);
         break;
     case kPartUpdate:
-        hipLaunchKernelGGL((k_update<P, N, false>), dim3(grid[kTabUpd]), dim3(S::UPD_BLOCK), 0, st, b);
+        if (upd == kUpdShared)
+            hipLaunchKernelGGL((k_update_shared<P, N, false>), dim3(grid[kTabUpd]), dim3(S::UPD_BLOCK), 0, st, b);
+        else
+            hipLaunchKernelGGL((k_update<P, N, false>), dim3(grid[kTabUpd]), dim3(S::UPD_BLOCK), 0, st, b);
         break;
     case kPartSeed:
-        hipLaunchKernelGGL((k_update<P, N, true>), dim3(grid[kTabUpd]), dim3(S::UPD_BLOCK), 0, st, b);
+        if (upd == kUpdShared)
+            hipLaunchKernelGGL((k_update_shared<P, N, true>), dim3(grid[kTabUpd]), dim3(S::UPD_BLOCK), 0, st, b);
+        else
+            hipLaunchKernelGGL((k_update<P, N, true>), dim3(grid[kTabUpd]), dim3(S::UPD_BLOCK), 0, st, b);
         break;
     case kPartSpread:
         hipLaunchKernelGGL(k_spread, dim3(grid[kTabCell]), dim3(kBlock), 0, st, b);
@@ -1960,13 +2199,14 @@ hipError_t launch_step_impl(const StepBufs &b, hipStream_t st, int part, const i
 }
 
 template <class P>
-hipError_t launch_part_pde(int N, const StepBufs &b, hipStream_t st, int part, const int grid[kNumTabs], int variant)
+hipError_t launch_part_pde(int N, const StepBufs &b, hipStream_t st, int part, const int grid[kNumTabs], int variant,
+                           int upd)
 {
     switch (N) {
-    case 1: return launch_step_impl<P, 1>(b, st, part, grid, variant);
-    case 2: return launch_step_impl<P, 2>(b, st, part, grid, variant);
-    case 3: return launch_step_impl<P, 3>(b, st, part, grid, variant);
-    case 4: return launch_step_impl<P, 4>(b, st, part, grid, variant);
+    case 1: return launch_step_impl<P, 1>(b, st, part, grid, variant, upd);
+    case 2: return launch_step_impl<P, 2>(b, st, part, grid, variant, upd);
+    case 3: return launch_step_impl<P, 3>(b, st, part, grid, variant, upd);
+    case 4: return launch_step_impl<P, 4>(b, st, part, grid, variant, upd);
     default: return hipErrorInvalidValue;
     }
 }
@@ -2074,7 +2314,7 @@ void table_quanta(int pde, int N, int variant, int quanta[kNumTabs])
         quanta_pde<EulerPDE>(N, variant, quanta);
 }
 
-hipError_t launch_part(int pde, int N, int variant, const StepBufs &b, hipStream_t st, int part,
+hipError_t launch_part(int pde, int N, int variant, int upd, const StepBufs &b, hipStream_t st, int part,
                        const int grid[kNumTabs], int nslot_cap)
 {
     if (part == kPartDt) return launch_dt(b, st, 0.0, 0.0, 0);
@@ -2084,8 +2324,8 @@ hipError_t launch_part(int pde, int N, int variant, const StepBufs &b, hipStream
         return hipGetLastError();
     }
     if (pde == kPdeAd) return launch_part_ad(N, b, st, part, grid);
-    return pde == kPdeBn ? launch_part_pde<BnPDE>(N, b, st, part, grid, variant)
-                         : launch_part_pde<EulerPDE>(N, b, st, part, grid, variant);
+    return pde == kPdeBn ? launch_part_pde<BnPDE>(N, b, st, part, grid, variant, upd)
+                         : launch_part_pde<EulerPDE>(N, b, st, part, grid, variant, upd);
 }
 
 hipError_t launch_dt(const StepBufs &b, hipStream_t st, double time, double tend, int use_host_time)

@@ -98,6 +98,11 @@ struct TestBufs {
 // (tune.hip) picks one per (PDE, N).  Advection-diffusion has the time-line form only.
 enum { kPredLine = 0, kPredNode64 = 1, kPredNode128 = 2, kPredNode256 = 3, kPredLean = 4, kNumPredVariants = 5 };
 bool pred_variant_valid(int pde, int N, int variant);
+// Update kernels: kUpdRecompute (k_update: each thread forms the first passes of the
+// subcell and Lobatto maps it needs; LDS atomics reduce the ranges) or kUpdShared
+// (k_update_shared: the first passes shared in LDS, partials reduced by the cell's
+// threads).  The same results; the autotuner picks one per (PDE, N).
+enum { kUpdRecompute = 0, kUpdShared = 1, kNumUpdVariants = 2 };
 int pred_cells_per_block(int pde, int N, int variant);
 
 // Items (cells or face points) per work-table entry of each table, for a (PDE, N)
@@ -109,7 +114,7 @@ hipError_t launch_part_ad(int N, const StepBufs &b, hipStream_t st, int part, co
 
 // One part of the step for every solve of the group.  grid: workgroups per table-driven
 // kernel (they loop over the table); nslot_cap: solve slots of the group.
-hipError_t launch_part(int pde, int N, int variant, const StepBufs &b, hipStream_t st, int part,
+hipError_t launch_part(int pde, int N, int variant, int upd, const StepBufs &b, hipStream_t st, int part,
                        const int grid[kNumTabs], int nslot_cap);
 // calc_time_step of every active solve; use_host: time and tend given here instead of the solve state.
 hipError_t launch_dt(const StepBufs &b, hipStream_t st, double time, double tend, int use_host);

@@ -1,4 +1,4 @@
-// tune.hip -- autotuner of the predictor kernel.
+// tune.hip -- autotuner of the predictor and update kernels.
 //
 // The predictor has several kernel variants (step_kernels.hpp: time-line threads
 // in two register layouts, or space-time-node threads in workgroups of 64, 128 or
@@ -6,8 +6,10 @@
 // SIMD.  The fastest depends on (PDE, N) and on the launch mode (one workgroup per
 // table entry, or a persistent grid), so the first group of a (device, PDE, N,
 // mode) times every variant on a synthetic 192 x 192 solve and keeps the fastest
-// for the rest of the process.  ADERDG_PREDICTOR_VARIANT=<k> in the environment,
-// or aderdg_tune_force(), overrides the choice (for tests and measurements).
+// for the rest of the process; the update kernel's two variants (step_kernels.hpp)
+// are timed the same way on the synthetic solve of the fastest predictor.
+// ADERDG_PREDICTOR_VARIANT=<k> / ADERDG_UPDATE_VARIANT=<k> in the environment, or
+// aderdg_tune_force_kernel(), override the choice (for tests and measurements).
 #include <climits>
 #include <cstdlib>
 #include <cstring>
@@ -26,13 +28,16 @@ constexpr int kTuneSide = 192;
 constexpr int kTuneRounds = 3, kTuneReps = 3;
 
 struct TuneEntry {
-    int variant = kPredLine;
-    double ms[kNumPredVariants] = {-1.0, -1.0, -1.0, -1.0, -1.0};
+    int pred = kPredLine, upd = kUpdShared;
+    double pms[kNumPredVariants] = {-1.0, -1.0, -1.0, -1.0, -1.0};
+    double ums[kNumUpdVariants] = {-1.0, -1.0};
 };
 
 std::mutex g_mu;
 std::map<std::tuple<int, int, int, int>, TuneEntry> g_cache;  // (device, pde, N, persistent)
-int g_forced[3][5] = {{-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}};
+// forced variants per kernel (0 predictor, 1 update), pde and N; -1 = the tuner's choice
+int g_forced[2][3][5] = {{{-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}},
+                         {{-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}}};
 
 // The synthetic solve: the isentropic vortex (Euler) or a pressure disc (Baer-Nunziato,
 // ideal-gas phases), both admissible, timed over predictor launches at a fixed small dt.
@@ -73,7 +78,7 @@ struct Trial {
         synthetic(pde, N, &d, &ic);
         const int S = kTuneSide;
         GroupCaps caps{(int64_t)S * S, (int64_t)(S + 1) * S, (int64_t)S * (S + 1), 1};
-        if (g.init(pde, N, caps, ops, persist_blocks, st, variant)) return;
+        if (g.init(pde, N, caps, ops, persist_blocks, st, variant, kUpdShared)) return;
         SolveState s0{};
         s0.dt = 1.0e-4;
         s0.tend = 1.0e30;
@@ -86,98 +91,145 @@ struct Trial {
     }
 };
 
-// Milliseconds per launch of kTuneReps predictor launches, or -1.
-double time_launches(const Trial &t, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
+// Milliseconds per launch of kTuneReps launches of one part, or -1.
+double time_launches(const Trial &t, int part, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     bool ok = hipEventRecord(e0, st) == hipSuccess;
-    for (int k = 0; k < kTuneReps && ok; k++) ok = t.g.launch(kPartPredictor) == hipSuccess;
+    for (int k = 0; k < kTuneReps && ok; k++) ok = t.g.launch(part) == hipSuccess;
     ok = ok && hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
     float ms = 0.0f;
     if (!ok || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.0;
     return ms / kTuneReps;
 }
 
-int env_variant()
+int env_variant(const char *name)
 {
-    const char *e = std::getenv("ADERDG_PREDICTOR_VARIANT");
+    const char *e = std::getenv(name);
     if (!e || !*e) return -1;
     return std::atoi(e);
 }
 
+bool upd_variant_valid(int v) { return v >= 0 && v < kNumUpdVariants; }
+
 }  // namespace
 
-int tune_predictor(int pde, int N, int persist_blocks, const Operators &ops, hipStream_t st)
+void tune_kernels(int pde, int N, int persist_blocks, const Operators &ops, hipStream_t st, int *pred, int *upd)
 {
-    if (pde == kPdeAd || N < 1 || N > 4) return kPredLine;
+    *pred = kPredLine;
+    *upd = kUpdShared;
+    if (pde == kPdeAd || N < 1 || N > 4) return;
+    int fp, fu;
     {
         std::lock_guard<std::mutex> lk(g_mu);
-        const int f = g_forced[pde][N];
-        if (f >= 0 && pred_variant_valid(pde, N, f)) return f;
+        fp = g_forced[0][pde][N];
+        fu = g_forced[1][pde][N];
     }
-    const int ev = env_variant();
-    if (ev >= 0 && pred_variant_valid(pde, N, ev)) return ev;
+    if (fp < 0) fp = env_variant("ADERDG_PREDICTOR_VARIANT");
+    if (fu < 0) fu = env_variant("ADERDG_UPDATE_VARIANT");
+    const bool have_p = fp >= 0 && pred_variant_valid(pde, N, fp), have_u = upd_variant_valid(fu);
+    if (have_p && have_u) {
+        *pred = fp;
+        *upd = fu;
+        return;
+    }
     int dev = 0;
     (void)hipGetDevice(&dev);
     const auto key = std::make_tuple(dev, pde, N, persist_blocks > 0 ? 1 : 0);
+    TuneEntry t;
+    bool cached = false;
     {
         std::lock_guard<std::mutex> lk(g_mu);
         auto it = g_cache.find(key);
-        if (it != g_cache.end()) return it->second.variant;
-    }
-    // every variant's synthetic solve set up first, then timed in interleaved rounds
-    // (the minimum over rounds), so clock ramps and cold caches hit all of them alike
-    TuneEntry t;
-    Trial *trials[kNumPredVariants] = {};
-    for (int v = 0; v < kNumPredVariants; v++)
-        if (pred_variant_valid(pde, N, v)) trials[v] = new Trial(pde, N, v, persist_blocks, ops, st);
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
-        for (int r = 0; r < kTuneRounds; r++)
-            for (int v = 0; v < kNumPredVariants; v++) {
-                if (!trials[v] || !trials[v]->ok) continue;
-                const double ms = time_launches(*trials[v], st, e0, e1);
-                if (ms > 0.0 && (t.ms[v] < 0.0 || ms < t.ms[v])) t.ms[v] = ms;
-            }
-    }
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
-    (void)hipStreamSynchronize(st);
-    for (int v = 0; v < kNumPredVariants; v++) delete trials[v];
-    double best = -1.0;
-    for (int v = 0; v < kNumPredVariants; v++)
-        if (t.ms[v] > 0.0 && (best < 0.0 || t.ms[v] < best)) {
-            best = t.ms[v];
-            t.variant = v;
+        if (it != g_cache.end()) {
+            t = it->second;
+            cached = true;
         }
-    std::lock_guard<std::mutex> lk(g_mu);
-    g_cache[key] = t;
-    return t.variant;
+    }
+    if (!cached) {
+        // every predictor variant's synthetic solve set up first, then timed in interleaved
+        // rounds (the minimum over rounds), so clock ramps and cold caches hit all alike;
+        // then both update variants on the synthetic solve of the fastest predictor
+        Trial *trials[kNumPredVariants] = {};
+        for (int v = 0; v < kNumPredVariants; v++)
+            if (pred_variant_valid(pde, N, v)) trials[v] = new Trial(pde, N, v, persist_blocks, ops, st);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+            for (int r = 0; r < kTuneRounds; r++)
+                for (int v = 0; v < kNumPredVariants; v++) {
+                    if (!trials[v] || !trials[v]->ok) continue;
+                    const double ms = time_launches(*trials[v], kPartPredictor, st, e0, e1);
+                    if (ms > 0.0 && (t.pms[v] < 0.0 || ms < t.pms[v])) t.pms[v] = ms;
+                }
+            double best = -1.0;
+            for (int v = 0; v < kNumPredVariants; v++)
+                if (t.pms[v] > 0.0 && (best < 0.0 || t.pms[v] < best)) {
+                    best = t.pms[v];
+                    t.pred = v;
+                }
+            Trial *tr = trials[t.pred];
+            if (tr && tr->ok && tr->g.launch(kPartFace) == hipSuccess) {
+                for (int r = 0; r < kTuneRounds; r++)
+                    for (int u = 0; u < kNumUpdVariants; u++) {
+                        tr->g.upd = u;
+                        const double ms = time_launches(*tr, kPartUpdate, st, e0, e1);
+                        if (ms > 0.0 && (t.ums[u] < 0.0 || ms < t.ums[u])) t.ums[u] = ms;
+                    }
+                best = -1.0;
+                for (int u = 0; u < kNumUpdVariants; u++)
+                    if (t.ums[u] > 0.0 && (best < 0.0 || t.ums[u] < best)) {
+                        best = t.ums[u];
+                        t.upd = u;
+                    }
+            }
+        }
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipStreamSynchronize(st);
+        for (int v = 0; v < kNumPredVariants; v++) delete trials[v];
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_cache[key] = t;
+    }
+    *pred = have_p ? fp : t.pred;
+    *upd = have_u ? fu : t.upd;
 }
 
 }  // namespace aderdg
 
 extern "C" {
 
-int aderdg_tune_force(int pde, int N, int variant)
+int aderdg_tune_force_kernel(int kernel, int pde, int N, int variant)
 {
     using namespace aderdg;
+    if (kernel < 0 || kernel > 1) return fail(ADERDG_EINVAL, "aderdg_tune_force_kernel: kernel %d is not 0 or 1", kernel);
     if (pde < 0 || pde > 2 || N < 1 || N > 4) return fail(ADERDG_EINVAL, "aderdg_tune_force: bad pde %d / N %d", pde, N);
-    if (variant >= 0 && !pred_variant_valid(pde, N, variant))
-        return fail(ADERDG_EINVAL, "aderdg_tune_force: variant %d does not exist for pde %d, N=%d", variant, pde, N);
+    if (variant >= 0 && !(kernel == 0 ? pred_variant_valid(pde, N, variant) : upd_variant_valid(variant)))
+        return fail(ADERDG_EINVAL, "aderdg_tune_force: variant %d does not exist for kernel %d, pde %d, N=%d", variant,
+                    kernel, pde, N);
     std::lock_guard<std::mutex> lk(g_mu);
-    g_forced[pde][N] = variant < 0 ? -1 : variant;
+    g_forced[kernel][pde][N] = variant < 0 ? -1 : variant;
+    return 0;
+}
+
+int aderdg_tune_force(int pde, int N, int variant) { return aderdg_tune_force_kernel(0, pde, N, variant); }
+
+int aderdg_tune_query_kernel(int kernel, int device, int pde, int N, int persistent, int *variant, double *ms)
+{
+    using namespace aderdg;
+    if (kernel < 0 || kernel > 1) return fail(ADERDG_EINVAL, "aderdg_tune_query_kernel: kernel %d is not 0 or 1", kernel);
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_cache.find(std::make_tuple(device, pde, N, persistent ? 1 : 0));
+    if (it == g_cache.end())
+        return fail(ADERDG_EINVAL, "aderdg_tune_query: (pde %d, N=%d) not tuned on device %d", pde, N, device);
+    const TuneEntry &t = it->second;
+    if (variant) *variant = kernel == 0 ? t.pred : t.upd;
+    const int nv = kernel == 0 ? kNumPredVariants : kNumUpdVariants;
+    for (int k = 0; ms && k < nv; k++) ms[k] = kernel == 0 ? t.pms[k] : t.ums[k];
     return 0;
 }
 
 int aderdg_tune_query(int device, int pde, int N, int persistent, int *variant, double ms[ADERDG_PREDICTOR_VARIANTS])
 {
-    using namespace aderdg;
-    std::lock_guard<std::mutex> lk(g_mu);
-    auto it = g_cache.find(std::make_tuple(device, pde, N, persistent ? 1 : 0));
-    if (it == g_cache.end()) return fail(ADERDG_EINVAL, "aderdg_tune_query: (pde %d, N=%d) not tuned on device %d", pde, N, device);
-    if (variant) *variant = it->second.variant;
-    for (int k = 0; ms && k < ADERDG_PREDICTOR_VARIANTS; k++) ms[k] = it->second.ms[k];
-    return 0;
+    return aderdg_tune_query_kernel(0, device, pde, N, persistent, variant, ms);
 }
 
 }  // extern "C"

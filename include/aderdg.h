@@ -204,6 +204,14 @@ int aderdg_timing_get(aderdg_ctx *ctx, double ms[ADERDG_TIMING_SLOTS], int64_t l
 int aderdg_tune_force(int pde, int N, int variant);
 int aderdg_tune_query(int device, int pde, int N, int persistent, int *variant,
                       double ms[ADERDG_PREDICTOR_VARIANTS]);
+/* The same for either tuned kernel: kernel 0 the predictor (variants as above),
+ * kernel 1 the update (ADERDG_UPDATE_VARIANTS variants with identical results:
+ * 0 each thread forms the first passes of the subcell and Lobatto maps it needs
+ * and LDS atomics reduce the ranges, 1 those passes shared in LDS and the ranges
+ * reduced by the cell's threads).  ms holds the variant count of the kernel. */
+#define ADERDG_UPDATE_VARIANTS 2
+int aderdg_tune_force_kernel(int kernel, int pde, int N, int variant);
+int aderdg_tune_query_kernel(int kernel, int device, int pde, int N, int persistent, int *variant, double *ms);
 
 /* Algorithmic floating-point operations of one cell-step of the predictor
  * launch (predictor + volume integral) and of the update launch, counted as

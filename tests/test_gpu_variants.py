@@ -32,6 +32,7 @@ def auto_variant():
     for pde in ("euler", "bn"):
         for N in (1, 2, 3, 4):
             aderdg.tune_force(pde, N, -1, exact=True)
+            aderdg.tune_force(pde, N, -1, exact=True, kernel="update")
 
 
 @pytest.mark.parametrize("pde,N,v", CASES)
@@ -77,6 +78,20 @@ def test_step_variant_bitwise(pde, N, v):
     Baer-Nunziato pressure disc; uh, dt, nRecompute and the limiter state each step."""
     log("steps %s N=%d variant %d" % (pde, N, v))
     aderdg.tune_force(pde, N, v, exact=True)
+    _steps_bitwise(pde, N)
+
+
+@pytest.mark.parametrize("pde,N,u", [(pde, N, u) for pde in ("euler", "bn") for N in (1, 2, 3, 4)
+                                     for u in range(len(aderdg.UPDATE_VARIANTS))])
+def test_update_variant_bitwise(pde, N, u):
+    """Each update kernel variant (first map passes recomputed per thread with LDS-atomic
+    ranges, or shared in LDS), forced, over the same three limited steps."""
+    log("steps %s N=%d update variant %d" % (pde, N, u))
+    aderdg.tune_force(pde, N, u, exact=True, kernel="update")
+    _steps_bitwise(pde, N)
+
+
+def _steps_bitwise(pde, N):
     c = _case(pde, N)
     eos = BN_EOS_STIFF if pde == "bn" else None
     s = aderdg.Solver(N, c["I"], c["J"], (0.0, 0.0), c["xr"], c["bc"], exact=True, pde=pde, eos=eos)
