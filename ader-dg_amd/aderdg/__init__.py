@@ -72,7 +72,7 @@ class _ServerStats(ctypes.Structure):
                [("busy_seconds", ctypes.c_double)]
 
 
-IC_KIND = {"vortex": 1, "quadrant": 2, "dmr": 3, "bn_circle": 4, "bn_disc": 5, "sine": 6}
+IC_KIND = {"vortex": 1, "quadrant": 2, "dmr": 3, "bn_circle": 4, "bn_disc": 5, "sine": 6, "bn_riemann": 7}
 
 
 def ic_spec(kind, *params):
@@ -102,6 +102,23 @@ def ic_dmr():
     pi = 4.0 * math.atan(1.0)
     return ic_spec("dmr", 1.0 / 6.0, math.tan(pi / 3.0), 8.0, 8.25 * math.cos(pi / 6.0), -8.25 * math.sin(pi / 6.0),
                    116.5, 1.4, 0.0, 0.0, 1.0)
+
+
+def ic_of_ictype(pde, ictype, xl, xr, exact=False):
+    """The formula of a reference ICType (Euler 4, 5, 11; Baer-Nunziato 1, 2, 3, 7)."""
+    lib = load_library(exact)
+    k = PDE[pde] if isinstance(pde, str) else int(pde)
+    ic = _Ic()
+    a = np.array(xl, dtype=np.float64)
+    b = np.array(xr, dtype=np.float64)
+    if lib.aderdg_ic_of_ictype(k, int(ictype), _p(a), _p(b), ctypes.byref(ic)) != 0:
+        raise AderdgError("no formula for pde %s ICType %d" % (pde, ictype))
+    return ic
+
+
+def ic_bn_riemann(v_left, v_right, x0=0.0):
+    """Baer-Nunziato primitive states v_left where x <= x0, v_right elsewhere."""
+    return ic_spec("bn_riemann", x0, *v_left, *v_right)
 
 
 def ic_sine(amplitude=1.0, kx=1.0, ky=1.0):
@@ -216,6 +233,8 @@ def load_library(exact=False):
         "aderdg_server_destroy": [vp],
         "aderdg_batch_timing_get": [vp, ctypes.c_int, ctypes.c_int, _dp, ctypes.POINTER(i64)],
         "aderdg_tune_force": [ctypes.c_int, ctypes.c_int, ctypes.c_int],
+        "aderdg_seed_limiter": [vp, ctypes.POINTER(_Ic)],
+        "aderdg_ic_of_ictype": [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.POINTER(_Ic)],
         "aderdg_tune_force_kernel": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
         "aderdg_tune_query_kernel": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_int), _dp],
@@ -387,6 +406,10 @@ class Solver:
             return dict(qhi=qhi, Fhi=fhi, Shi=shi, qbnd=qb, Fbnd=fb)
         self._check(self.lib.aderdg_predictor_cells(self.h, nc, dt, _p(luh), _p(qhi), _p(fhi), _p(qb), _p(fb)))
         return dict(qhi=qhi, Fhi=fhi, qbnd=qb, Fbnd=fb)
+
+    def seed_limiter(self, ic):
+        """Troubled-cell seeding of the uploaded projection of formula ic (end of initialize)."""
+        self._check(self.lib.aderdg_seed_limiter(self.h, ctypes.byref(ic)))
 
     def timing(self, on):
         self._check(self.lib.aderdg_timing_enable(self.h, 1 if on else 0))

@@ -4,9 +4,9 @@ Produces, in ader-dg_amd/lib/:
   libaderdg_amd.so        floating-point contraction on (FMA), the product build
   libaderdg_amd_exact.so  contraction off: bit-identical to the reference's
                           expression order, used by the parity tests
-  aderdg_euler, aderdg_ad the Fortran drivers (fortran/aderdg_main.f90,
-                          fortran/aderdg_ad_main.f90) linked against
-                          libaderdg_amd.so, when flang is available
+  aderdg_euler, aderdg_bn, the Fortran drivers (fortran/aderdg_main.f90,
+  aderdg_ad               fortran/aderdg_bn_main.f90, fortran/aderdg_ad_main.f90)
+                          linked against libaderdg_amd.so, when flang is available
 """
 import concurrent.futures
 import os
@@ -69,7 +69,7 @@ def build_library(exact=False, compiled=False):
 
 
 # Fortran drivers: (executable, main program) per reference time loop
-DRIVERS = (("aderdg_euler", "aderdg_main.f90"), ("aderdg_ad", "aderdg_ad_main.f90"))
+DRIVERS = (("aderdg_euler", "aderdg_main.f90"), ("aderdg_bn", "aderdg_bn_main.f90"), ("aderdg_ad", "aderdg_ad_main.f90"))
 
 
 def build_fortran_driver(exact=False):

@@ -415,6 +415,21 @@ int aderdg_set_limiter(aderdg_ctx *c, const int32_t *status, const double *Lh)
     return sync(c);
 }
 
+int aderdg_seed_limiter(aderdg_ctx *c, const aderdg_ic *ic)
+{
+    if (!c || !ic) return fail(ADERDG_EINVAL, "aderdg_seed_limiter: null argument");
+    if (!c->have_uh) return fail(ADERDG_ESTATE, "aderdg_seed_limiter: upload uh first");
+    if (c->desc.pde == ADERDG_PDE_AD) return fail(ADERDG_EINVAL, "aderdg_seed_limiter: advection-diffusion has no limiter");
+    if (!ic_fits(c->desc.pde, ic->kind)) return fail(ADERDG_EINVAL, "aderdg_seed_limiter: formula %d does not fit the system", ic->kind);
+    IcSpec s{};
+    s.kind = ic->kind;
+    for (int k = 0; k < kIcParams; k++) s.p[k] = ic->p[k];
+    HIPCHK(launch_seed_ic(c->desc.pde, c->N, c->g.b, s, c->slot, (int)c->ncell, c->stream));
+    int r = seed(c);
+    if (r) return r;
+    return sync(c);
+}
+
 int aderdg_get_limiter(aderdg_ctx *c, int32_t *status, int32_t *recompute, double *Lh, double *lmin, double *lmax)
 {
     if (!c) return fail(ADERDG_EINVAL, "aderdg_get_limiter: null context");

@@ -30,7 +30,7 @@
 
 namespace aderdg {
 
-enum { kIcVortex = 1, kIcQuadrant = 2, kIcDmr = 3, kIcBnCircle = 4, kIcBnDisc = 5, kIcSine = 6 };
+enum { kIcVortex = 1, kIcQuadrant = 2, kIcDmr = 3, kIcBnCircle = 4, kIcBnDisc = 5, kIcSine = 6, kIcBnRiemann = 7 };
 constexpr int kIcParams = 24;
 
 struct IcSpec {
@@ -66,7 +66,7 @@ ADG_HD void ic_bn_cons(const double *eos, const double *V, double *Q)
 // Whether a formula kind belongs to a system (0 Euler, 1 Baer-Nunziato, 2 advection-diffusion).
 ADG_HD bool ic_fits(int pde, int kind)
 {
-    if (pde == 1) return kind == kIcBnCircle || kind == kIcBnDisc;
+    if (pde == 1) return kind == kIcBnCircle || kind == kIcBnDisc || kind == kIcBnRiemann;
     if (pde == 2) return kind == kIcSine;
     return kind == kIcVortex || kind == kIcQuadrant || kind == kIcDmr;
 }
@@ -79,6 +79,10 @@ ADG_HD void ic_field(const IcSpec &ic, double gamma, const double *eos, double x
     if (ic.kind == kIcSine) {
         const double pi = 3.141592653589793;
         Q[0] = p[0] * sin(2.0 * pi * p[1] * x) * sin(2.0 * pi * p[2] * y);
+        return;
+    }
+    if (ic.kind == kIcBnRiemann) {  // two primitive states split at x = p[0], Baer-Nunziato/initialize.f90:314-403
+        ic_bn_cons(eos, p + (x <= p[0] ? 1 : 10), Q);
         return;
     }
     if (ic.kind == kIcBnDisc) {

@@ -44,6 +44,28 @@ bool ic_of_ictype(int ictype, const double *xl, const double *xr, IcSpec *ic)
     return true;
 }
 
+// The Baer-Nunziato test problems of Baer-Nunziato/initialize.f90:314-403 (ICType 1, 2,
+// 3 and 7): one-dimensional Riemann problems, the left state where x <= 0.  Primitive
+// states (rho_s, u_s, v_s, p_s, rho_g, u_g, v_g, p_g, phi_s).
+bool bn_ic_of_ictype(int ictype, IcSpec *ic)
+{
+    static const double states[4][2][9] = {
+        {{1.0, 0.0, 0.0, 1.0, 0.5, 0.0, 0.0, 1.0, 0.4}, {2.0, 0.0, 0.0, 2.0, 1.5, 0.0, 0.0, 2.0, 0.8}},
+        {{800.0, 0.0, 0.0, 500.0, 1.5, 0.0, 0.0, 2.0, 0.4}, {1000.0, 0.0, 0.0, 600.0, 1.0, 0.0, 0.0, 1.0, 0.3}},
+        {{1.0, 0.9, 0.0, 2.5, 1.0, 0.0, 0.0, 1.0, 0.9}, {1.0, 0.0, 0.0, 1.0, 1.2, 1.0, 0.0, 2.0, 0.2}},
+        {{800.0, 1.0, 0.0, 1.0, 2.0, 1.0, 0.0, 1.0, 0.99}, {1000.0, 1.0, 0.0, 1.0, 1.0, 1.0, 0.0, 1.0, 0.01}}};
+    const int k = ictype == 1 ? 0 : ictype == 2 ? 1 : ictype == 3 ? 2 : ictype == 7 ? 3 : -1;
+    *ic = IcSpec{};
+    if (k < 0) return false;
+    ic->kind = kIcBnRiemann;
+    ic->p[0] = 0.0;
+    for (int v = 0; v < 9; v++) {
+        ic->p[1 + v] = states[k][0][v];
+        ic->p[10 + v] = states[k][1][v];
+    }
+    return true;
+}
+
 // uh(V,n,n,IMAX,JMAX) of an IcSpec on the host.
 int ic_fill(const IcSpec &ic, int N, int V, int imax, int jmax, const double *xl, const double *xr, double gamma,
             const double *eos, double *uh)
@@ -72,6 +94,20 @@ extern "C" int aderdg_initial_condition(int ictype, int N, int imax, int jmax, c
     aderdg::IcSpec ic;
     if (!aderdg::ic_of_ictype(ictype, xl, xr, &ic)) return ADERDG_EINVAL;
     return aderdg::ic_fill(ic, N, 4, imax, jmax, xl, xr, gamma, nullptr, uh);
+}
+
+extern "C" int aderdg_ic_of_ictype(int pde, int ictype, const double *xl, const double *xr, aderdg_ic *out)
+{
+    if (!xl || !xr || !out) return ADERDG_EINVAL;
+    aderdg::IcSpec s{};
+    const bool ok = pde == ADERDG_PDE_EULER ? aderdg::ic_of_ictype(ictype, xl, xr, &s)
+                    : pde == ADERDG_PDE_BN  ? aderdg::bn_ic_of_ictype(ictype, &s)
+                                            : false;
+    if (!ok) return ADERDG_EINVAL;
+    out->kind = s.kind;
+    out->pad = 0;
+    for (int k = 0; k < aderdg::kIcParams; k++) out->p[k] = s.p[k];
+    return 0;
 }
 
 extern "C" int aderdg_ic_eval(const aderdg_desc *d, const aderdg_ic *ic, double *uh)

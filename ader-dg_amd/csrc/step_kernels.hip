@@ -2098,6 +2098,79 @@ __global__ __launch_bounds__(kBlock) void k_tables(StepBufs b, const int *prefix
 }
 
 // Initial state of one solve from a formula: one thread per (cell, node).
+// ---------------------------------------------------------------------------
+// Troubled-cell seeding of a formula initial condition, the end of initialize
+// (Euler/initialize.f90:274-303, Baer-Nunziato/initialize.f90:272-301): a cell
+// whose projected state fails the DMP against its own nodal range with the
+// relaxed margin 0.1 (DMP(..., 1.0e-1), Euler/limiter.f90 DMP), or the positivity
+// check of its subcell values, starts limited, with its subcell averages taken
+// from the formula at the subcell barycentres corner + xilimbary * dx
+// (xilimbary(i) = 0.5/nSubLim + (i-1)/nSubLim, Euler/initialize.f90:211-213).  The
+// reference writes x0 for the corner without setting it; the corner is meant.
+// One 64-thread workgroup per cell of solve `slot`, solution in uh[parity].
+// ---------------------------------------------------------------------------
+template <class P, int N>
+__global__ __launch_bounds__(64) void k_seed_ic(StepBufs b, IcSpec ic, int slot, int ncell)
+{
+    using S = Shape<P, N>;
+    constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, V = S::V;
+    __shared__ double sUh[V * NN], sTy[V * n * s], sBy[V * NN], sLim[V * SS], sLob[V * NN];
+    __shared__ int sFail;
+    const OpPtr op = (OpPtr)b.op;
+    const int lane = threadIdx.x;
+    const ADG_CONST SolveArgs &sa = *(const ADG_CONST SolveArgs *)(b.solves + slot);
+    const auto &m = sa.m;
+    const int par = ld_const(b.state + slot).parity;
+    for (int cell = blockIdx.x; cell < ncell; cell += gridDim.x) {
+        const size_t g = (size_t)sa.cell0 + cell;
+        const double *u = b.uh[par] + g * V * NN;
+        for (int k = lane; k < V * NN; k += 64) {
+            const int v = k / NN, node = k - v * NN;
+            sUh[v * NN + node] = u[v + V * node];
+        }
+        if (lane == 0) sFail = 0;
+        __syncthreads();
+        cell_ranges_stage1<P, N>(op, sUh, sTy, sBy, lane, 64);
+        __syncthreads();
+        if (cell_ranges_stage2<P, N>(op, m, sTy, sBy, sLim, sLob, lane, 64)) atomicOr(&sFail, 1);
+        __syncthreads();
+        for (int v = lane; v < V; v += 64) {
+            double umn = __builtin_nan(""), umx = umn, smn = umn, smx = umn, bmn = umn, bmx = umn;
+            for (int k = 0; k < NN; k++) {
+                umn = fold_min(umn, sUh[v * NN + k]);
+                umx = fold_max(umx, sUh[v * NN + k]);
+                bmn = fold_min(bmn, sLob[v * NN + k]);
+                bmx = fold_max(bmx, sLob[v * NN + k]);
+            }
+            for (int k = 0; k < SS; k++) {
+                smn = fold_min(smn, sLim[v * SS + k]);
+                smx = fold_max(smx, sLim[v * SS + k]);
+            }
+            // lmin = min(minval(uh), minval(lim), minval(lob)) against the nodal range, margin max(1e-4, 1e-3 range, 0.1)
+            const double mn = f_min(f_min(umn, smn), bmn), mx = f_max(f_max(umx, smx), bmx);
+            const double ldiff = f_max(f_max(1.0e-4, 1.0e-3 * (umx - umn)), 1.0e-1);
+            if (mn < umn - ldiff || mx > umx + ldiff) atomicOr(&sFail, 2);
+        }
+        __syncthreads();
+        if (sFail) {
+            if (lane == 0) b.status[par][g] = 1;
+            const int ci = cell % m.imax, cj = cell / m.imax;
+            const double cx = (m.xl[0] + ((double)ci + 0.5) * m.dx) - 0.5 * m.dx;
+            const double cy = (m.xl[1] + ((double)cj + 0.5) * m.dy) - 0.5 * m.dy;
+            for (int k = lane; k < SS; k += 64) {
+                const int ii = k % s, jj = k / s;
+                const double bx = 0.0 + 0.5 / (double)s + (double)ii * 1.0 / (double)s;
+                const double by = 0.0 + 0.5 / (double)s + (double)jj * 1.0 / (double)s;
+                double Q[V];
+                ic_field(ic, m.gamma, m.eos, cx + bx * m.dx, cy + by * m.dy, Q);
+#pragma unroll
+                for (int v = 0; v < V; v++) b.lh[par][g * V * SS + v + V * k] = Q[v];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 template <class P, int N>
 __global__ __launch_bounds__(kBlock) void k_init(StepBufs b, IcSpec ic, int slot, int parity)
 {
@@ -2341,6 +2414,31 @@ hipError_t launch_tables(const StepBufs &b, const int *prefix, int nslot, const 
     g = g < 1 ? 1 : (g > 4096 ? 4096 : g);
     hipLaunchKernelGGL(k_tables, dim3(g), dim3(kBlock), 0, st, b, prefix, nslot,
                        make_int4(quanta[0], quanta[1], quanta[2], quanta[3]));
+    return hipGetLastError();
+}
+
+hipError_t launch_seed_ic(int pde, int N, const StepBufs &b, const IcSpec &ic, int slot, int ncell, hipStream_t st)
+{
+    const dim3 grid((unsigned)(ncell < 65536 ? ncell : 65536)), block(64);
+    if (pde == kPdeBn) {
+        switch (N) {
+        case 1: hipLaunchKernelGGL((k_seed_ic<BnPDE, 1>), grid, block, 0, st, b, ic, slot, ncell); break;
+        case 2: hipLaunchKernelGGL((k_seed_ic<BnPDE, 2>), grid, block, 0, st, b, ic, slot, ncell); break;
+        case 3: hipLaunchKernelGGL((k_seed_ic<BnPDE, 3>), grid, block, 0, st, b, ic, slot, ncell); break;
+        case 4: hipLaunchKernelGGL((k_seed_ic<BnPDE, 4>), grid, block, 0, st, b, ic, slot, ncell); break;
+        default: return hipErrorInvalidValue;
+        }
+    } else if (pde == kPdeEuler) {
+        switch (N) {
+        case 1: hipLaunchKernelGGL((k_seed_ic<EulerPDE, 1>), grid, block, 0, st, b, ic, slot, ncell); break;
+        case 2: hipLaunchKernelGGL((k_seed_ic<EulerPDE, 2>), grid, block, 0, st, b, ic, slot, ncell); break;
+        case 3: hipLaunchKernelGGL((k_seed_ic<EulerPDE, 3>), grid, block, 0, st, b, ic, slot, ncell); break;
+        case 4: hipLaunchKernelGGL((k_seed_ic<EulerPDE, 4>), grid, block, 0, st, b, ic, slot, ncell); break;
+        default: return hipErrorInvalidValue;
+        }
+    } else {
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -122,6 +122,9 @@ hipError_t launch_dt(const StepBufs &b, hipStream_t st, double time, double tend
 // (prefix[k * (nslot + 1) + s], device memory); max_entries bounds every table's length.
 hipError_t launch_tables(const StepBufs &b, const int *prefix, int nslot, const int quanta[kNumTabs], int max_entries,
                          hipStream_t st);
+// Troubled-cell seeding of solve `slot` from its current solution and the formula it
+// was projected from (the end of initialize): status and Lh of the current parity.
+hipError_t launch_seed_ic(int pde, int N, const StepBufs &b, const IcSpec &ic, int slot, int ncell, hipStream_t st);
 // Initial state of solve `slot` into uh[parity] from a formula (ic_spec.hpp).
 hipError_t launch_init(int pde, int N, const StepBufs &b, const IcSpec &ic, int slot, int ncell, int parity,
                        hipStream_t st);

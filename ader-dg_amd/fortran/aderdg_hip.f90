@@ -38,7 +38,8 @@ module aderdg_hip
 
     ! formula initial conditions (aderdg_ic of include/aderdg.h)
     integer(c_int), parameter :: ADERDG_IC_VORTEX = 1, ADERDG_IC_QUADRANT = 2, ADERDG_IC_DMR = 3, &
-                                 ADERDG_IC_BN_CIRCLE = 4, ADERDG_IC_BN_DISC = 5, ADERDG_IC_SINE = 6
+                                 ADERDG_IC_BN_CIRCLE = 4, ADERDG_IC_BN_DISC = 5, ADERDG_IC_SINE = 6, &
+                                 ADERDG_IC_BN_RIEMANN = 7
     type, bind(C) :: aderdg_ic_t
         integer(c_int) :: kind = 0, pad = 0
         real(c_double) :: p(24) = 0.0d0
@@ -50,6 +51,17 @@ module aderdg_hip
             type(aderdg_desc_t), intent(in) :: desc
             type(aderdg_ic_t), intent(in) :: ic
             real(c_double), intent(out) :: uh(*)
+        end function
+        integer(c_int) function aderdg_ic_of_ictype(pde, ictype, xl, xr, ic) bind(C, name="aderdg_ic_of_ictype")
+            import :: c_int, c_double, aderdg_ic_t
+            integer(c_int), value :: pde, ictype
+            real(c_double), intent(in) :: xl(2), xr(2)
+            type(aderdg_ic_t), intent(out) :: ic
+        end function
+        integer(c_int) function aderdg_seed_limiter(ctx, ic) bind(C, name="aderdg_seed_limiter")
+            import :: c_int, c_ptr, aderdg_ic_t
+            type(c_ptr), value :: ctx
+            type(aderdg_ic_t), intent(in) :: ic
         end function
         integer(c_int) function aderdg_create(desc, ctx) bind(C, name="aderdg_create")
             import :: c_int, c_ptr, aderdg_desc_t

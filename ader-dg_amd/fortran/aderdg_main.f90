@@ -4,7 +4,8 @@
 ! (positional, Euler/initialize.f90:21-38) from the working directory and runs
 ! the time loop of Euler/main.f90:24-111 with the same call order, each
 ! per-cell loop of the reference replaced by one whole-mesh call through
-! aderdg_hip.  Output: the reference's VTK of cell-average primitive
+! aderdg_hip.  Initialization ends as the reference's does: cells whose
+! projected initial state fails the relaxed DMP start limited.  Output: the reference's VTK of cell-average primitive
 ! variables every WriteInterval steps (Euler/write_data.f90:8-95) and the
 ! "Total CPU time" line.
 !
@@ -24,6 +25,7 @@ program aderdg_euler
     implicit none
 
     type(aderdg_desc_t) :: desc
+    type(aderdg_ic_t) :: ic
     type(c_ptr) :: ctx
     integer :: N, IMAX, JMAX, bL, bR, bB, bT, ICType, WriteInterval, timestep, maxsteps, nargs
     integer(c_int32_t) :: nRecompute
@@ -98,6 +100,9 @@ program aderdg_euler
     call aderdg_check(aderdg_create(desc, ctx), 'initialize')
     call aderdg_check(aderdg_initial_condition(ICType, N, IMAX, JMAX, xL, xR, GAMMA, uh), 'initial condition')
     call aderdg_check(aderdg_upload_uh(ctx, uh), 'upload')
+    ! troubled-cell seeding of the projected initial state (Euler/initialize.f90:274-303)
+    call aderdg_check(aderdg_ic_of_ictype(ADERDG_PDE_EULER, ICType, xL, xR, ic), 'initial condition')
+    call aderdg_check(aderdg_seed_limiter(ctx, ic), 'initial limiter')
 
     time = 0.0d0
     dt = 0.0d0

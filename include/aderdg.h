@@ -129,6 +129,7 @@ int aderdg_download_uh(aderdg_ctx *ctx, double *uh);
  * elsewhere), lmin/lmax(nVar,IMAX,JMAX) of the last step's get_min_max. Any
  * output pointer may be NULL. */
 int aderdg_set_limiter(aderdg_ctx *ctx, const int32_t *status, const double *Lh);
+
 int aderdg_get_limiter(aderdg_ctx *ctx, int32_t *status, int32_t *recompute, double *Lh, double *lmin,
                        double *lmax);
 
@@ -254,7 +255,7 @@ typedef struct aderdg_batch aderdg_batch;
  *                      Baer-Nunziato/initialize.f90:314-390)
  *   SINE      (advection-diffusion) amplitude, kx, ky: A sin(2 pi kx x) sin(2 pi ky y) */
 enum aderdg_ic_kind { ADERDG_IC_VORTEX = 1, ADERDG_IC_QUADRANT = 2, ADERDG_IC_DMR = 3, ADERDG_IC_BN_CIRCLE = 4,
-                      ADERDG_IC_BN_DISC = 5, ADERDG_IC_SINE = 6 };
+                      ADERDG_IC_BN_DISC = 5, ADERDG_IC_SINE = 6, ADERDG_IC_BN_RIEMANN = 7 };
 typedef struct {
     int kind;
     int pad;
@@ -263,6 +264,21 @@ typedef struct {
 
 /* Host evaluation of an initial condition into uh(nVar,N+1,N+1,IMAX,JMAX). */
 int aderdg_ic_eval(const aderdg_desc *solve, const aderdg_ic *ic, double *uh);
+
+/* The formula of a reference ICType: Euler 4, 5, 11 (Euler/params.ini:17-85), or the
+ * Baer-Nunziato one-dimensional Riemann problems 1, 2, 3, 7
+ * (Baer-Nunziato/initialize.f90:314-403; ADERDG_IC_BN_RIEMANN: split at x = p[0],
+ * primitive states p[1..9] left, p[10..18] right). */
+int aderdg_ic_of_ictype(int pde, int ictype, const double xl[2], const double xr[2], aderdg_ic *out);
+
+/* The troubled-cell seeding at the end of initialize (Euler/initialize.f90:274-303,
+ * Baer-Nunziato/initialize.f90:272-301): after the formula initial condition ic has
+ * been projected and uploaded, every cell whose projection fails the DMP against its
+ * own nodal range with margin 0.1, or the positivity check of its subcell values,
+ * starts limited with its subcell averages taken from ic at the subcell barycentres
+ * (the reference's uninitialised x0 read as the cell corner).  Euler and
+ * Baer-Nunziato contexts. */
+int aderdg_seed_limiter(aderdg_ctx *ctx, const aderdg_ic *ic);
 
 typedef struct {
     int device;          /* HIP device ordinal                                   */

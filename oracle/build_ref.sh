@@ -67,14 +67,17 @@ build_variant() {
 # The Euler subcell recompute applies Double-Mach-Reflection ghost states on the
 # left, bottom and top boundaries unconditionally (Euler/limiter.f90:491-522).
 # Guarding them with ICType == 11 (DMR) leaves DMR runs unchanged and lets the
-# harness drive the other boundary modes through the same routine.
-EULER_DMR_GUARD=(
+# harness drive the other boundary modes through the same routine.  The
+# troubled-cell seeding at t=0 uses the uninitialised x0 (Euler/initialize.f90:297);
+# it gets the cell corner, as in the Baer-Nunziato build below.
+EULER_FIXES=(
   'limiter.f90|s/^( *)if \(i \.eq\. 1\) then ! Left/\1if (i .eq. 1 .and. ICType .eq. 11) then ! Left/'
   'limiter.f90|s/^( *)if \(j \.eq\. 1\) then ! Bottom/\1if (j .eq. 1 .and. ICType .eq. 11) then ! Bottom/'
   'limiter.f90|s/^( *)if \(j \.eq\. JMAX\) then ! Top/\1if (j .eq. JMAX .and. ICType .eq. 11) then ! Top/'
+  'initialize.f90|s#xGP = x0 \+ \(/ xilimbary#xGP = corner + (/ xilimbary#'
 )
 for n in 1 2 3 4; do
-    build_variant euler Euler "$n" ref_harness_euler.f90 "${EULER_DMR_GUARD[@]}"
+    build_variant euler Euler "$n" ref_harness_euler.f90 "${EULER_FIXES[@]}"
 done
 
 # Baer-Nunziato: two memory-safety fixes.  Gm is allocated with the x-face shape

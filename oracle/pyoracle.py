@@ -322,7 +322,9 @@ class RefEuler:
             cls._libs[N] = lib
         return cls._libs[N]
 
-    def __init__(self, N, imax, jmax, xl, xr, bc_mode, gamma=1.4, tend=1.0e9):
+    def __init__(self, N, imax, jmax, xl, xr, bc_mode, gamma=1.4, tend=1.0e9, ictype=11, keep_seed=False):
+        """keep_seed: keep the limiter state initialize leaves (the troubled-cell seeding of
+        the initial condition, Euler/initialize.f90:274-303) instead of all cells unlimited."""
         self.N, self.n, self.s, self.V = N, N + 1, 2 * N + 1, 4
         self.imax, self.jmax = imax, jmax
         self.bc_mode = bc_mode
@@ -330,7 +332,8 @@ class RefEuler:
         code = 3 if bc_mode == BC_PERIODIC else 1
         txt = PARAMS_TEMPLATE.format(gamma=repr(float(gamma)), imax=imax, jmax=jmax, xl0=repr(float(xl[0])),
                                      xl1=repr(float(xl[1])), xr0=repr(float(xr[0])), xr1=repr(float(xr[1])),
-                                     bl=code, br=code, bb=code, bt=code, ic=11, tend=repr(float(tend)), wi=10 ** 8)
+                                     bl=code, br=code, bb=code, bt=code, ic=int(ictype), tend=repr(float(tend)),
+                                     wi=10 ** 8)
         cwd = os.getcwd()
         with tempfile.TemporaryDirectory() as d:
             with open(os.path.join(d, "params.ini"), "w") as f:
@@ -340,7 +343,8 @@ class RefEuler:
                 L.ref_init()
             finally:
                 os.chdir(cwd)
-        L.ref_reset_limiter()
+        if not keep_seed:
+            L.ref_reset_limiter()
         L.ref_set_scalars(0.0, 0.0, float(tend), float(gamma), bc_mode)
         self.lib_ = L
 
@@ -479,7 +483,7 @@ BN_PARAMS_TEMPLATE = """{gs}   ! GAMMA_S
 1   ! bR
 1   ! bB
 1   ! bT
-1   ! IC
+{ic}   ! IC
 {tend}   ! tend
 {wi}   ! WriteInterval
 """
@@ -504,7 +508,7 @@ class RefBN(RefEuler):
             cls._libs[N] = lib
         return cls._libs[N]
 
-    def __init__(self, N, imax, jmax, xl, xr, eos, tend=1.0e9):
+    def __init__(self, N, imax, jmax, xl, xr, eos, tend=1.0e9, ictype=1, keep_seed=False):
         self.N, self.n, self.s, self.V = N, N + 1, 2 * N + 1, 9
         self.imax, self.jmax = imax, jmax
         self.bc_mode = BC_TRANSMISSIVE
@@ -513,7 +517,7 @@ class RefBN(RefEuler):
         txt = BN_PARAMS_TEMPLATE.format(gs=repr(float(eos[0])), gg=repr(float(eos[1])), ps=repr(float(eos[2])),
                                         pg=repr(float(eos[3])), imax=imax, jmax=jmax, xl0=repr(float(xl[0])),
                                         xl1=repr(float(xl[1])), xr0=repr(float(xr[0])), xr1=repr(float(xr[1])),
-                                        tend=repr(float(tend)), wi=10 ** 8)
+                                        ic=int(ictype), tend=repr(float(tend)), wi=10 ** 8)
         cwd = os.getcwd()
         with tempfile.TemporaryDirectory() as d:
             with open(os.path.join(d, "baer_nunziato_params.ini"), "w") as f:
@@ -523,7 +527,8 @@ class RefBN(RefEuler):
                 L.ref_init()
             finally:
                 os.chdir(cwd)
-        L.ref_reset_limiter()
+        if not keep_seed:
+            L.ref_reset_limiter()
         L.ref_set_scalars(0.0, 0.0, float(tend), _p(self.eos), BC_TRANSMISSIVE)
         self.lib_ = L
 

@@ -144,3 +144,18 @@ def test_ad_sine_ic_and_footprint():
     np.testing.assert_allclose(u[0], want, rtol=0, atol=1e-15)
     f = aderdg.solve_footprint
     assert 0 < f("ad", 3, 64, 64) < f("euler", 3, 64, 64)
+
+
+def test_bn_riemann_catalogue_matches_reference_initialize():
+    """The Baer-Nunziato ICType 1, 2, 3, 7 formulas (aderdg_ic_of_ictype) projected on the host
+    equal the reference initialize's uh bit for bit (oracle/_ref)."""
+    from pyoracle import RefBN
+    N, I, J, xl, xr, eos = 2, 20, 3, (-0.5, -0.05), (0.5, 0.05), (1.4, 1.4, 0.0, 0.0)
+    if not RefBN.available(N):
+        pytest.skip("reference build absent (oracle/build_ref.sh)")
+    for ictype in (1, 2, 3, 7):
+        ref = RefBN(N, I, J, xl, xr, eos, ictype=ictype)
+        ic = aderdg.ic_of_ictype("bn", ictype, xl, xr)
+        np.testing.assert_array_equal(aderdg.ic_eval(ic, N, I, J, xl, xr, pde="bn", eos=eos), ref.get_uh())
+    with pytest.raises(aderdg.AderdgError):
+        aderdg.ic_of_ictype("bn", 4, xl, xr)

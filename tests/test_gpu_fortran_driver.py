@@ -9,7 +9,7 @@ import pytest
 
 import aderdg
 from aderdg import ics
-from pyoracle import BC_DMR, BC_PERIODIC, Oracle
+from pyoracle import BC_DMR, BC_PERIODIC, Oracle, RefBN, RefEuler
 
 pytestmark = pytest.mark.gpu
 LIB = aderdg.LIBDIR
@@ -54,12 +54,64 @@ def oracle_run(N, imax, jmax, xl, xr, mode, uh0, steps, tend):
 
 
 def test_driver_dmr_bitwise(tmp_path):
+    """The Euler driver on the reference's Double Mach Reflection, initialization
+    included (projection and troubled-cell seeding), against the reference program's
+    own routines run the same way (oracle/_ref)."""
     N, I, J, steps = 2, 32, 8, 6
+    if not RefEuler.available(N):
+        pytest.skip("reference build absent (oracle/build_ref.sh)")
     uh, out = run_driver(tmp_path, "aderdg_euler_exact", N, steps, I, J, (0.0, 0.0), (4.0, 1.0), 11, 1, 0.2, wi=3)
-    uh0 = ics.initial_uh("dmr", aderdg.operators(N)["xi"], I, J, (0.0, 0.0), (4.0, 1.0))
-    np.testing.assert_array_equal(uh, oracle_run(N, I, J, (0.0, 0.0), (4.0, 1.0), BC_DMR, uh0, steps, 0.2))
+    ref = RefEuler(N, I, J, (0.0, 0.0), (4.0, 1.0), BC_DMR, tend=0.2, ictype=11, keep_seed=True)
+    assert ref.get_limiter()["status"].sum() > 0
+    for _ in range(steps):
+        ref.step()
+    np.testing.assert_array_equal(uh, ref.get_uh())
     vtk = sorted(p.name for p in tmp_path.glob("DMR-*.vtk"))
     assert vtk[0] == "DMR-00000000.vtk" and "DMR-00000003.vtk" in vtk
+
+
+BN_PARAMS = """1.4
+1.4
+0.0
+0.0
+{imax}
+{jmax}
+-0.5
+-0.05
+0.5
+0.05
+1
+1
+1
+1
+{ic}
+{tend}
+{wi}
+"""
+
+
+@pytest.mark.parametrize("ictype", [1, 3])
+def test_bn_driver_bitwise(tmp_path, ictype):
+    """The Baer-Nunziato driver (Baer-Nunziato/main.f90 call order) on the reference's
+    Riemann problems, initialization included, against the reference routines."""
+    N, I, J, steps = 2, 41, 4, 5
+    if not RefBN.available(N):
+        pytest.skip("reference build absent (oracle/build_ref.sh)")
+    (tmp_path / "baer_nunziato_params.ini").write_text(BN_PARAMS.format(imax=I, jmax=J, ic=ictype, tend=0.1, wi=2))
+    dump = tmp_path / "uh.bin"
+    print("driver aderdg_bn_exact", ictype, flush=True)
+    r = subprocess.run([os.path.join(LIB, "aderdg_bn_exact"), str(N), str(steps), str(dump)], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Total CPU time" in r.stdout and "nRecomp" in r.stdout
+    uh = np.fromfile(dump, dtype=np.float64).reshape((9, N + 1, N + 1, I, J), order="F")
+    ref = RefBN(N, I, J, (-0.5, -0.05), (0.5, 0.05), (1.4, 1.4, 0.0, 0.0), tend=0.1, ictype=ictype, keep_seed=True)
+    for _ in range(steps):
+        ref.step()
+    np.testing.assert_array_equal(uh, ref.get_uh())
+    base = {1: "RP1D1", 3: "RP1D3"}[ictype]
+    vtk = sorted(p.name for p in tmp_path.glob(base + "-*.vtk"))
+    assert vtk[0] == base + "-00000000.vtk" and base + "-00000004.vtk" in vtk
 
 
 def test_driver_vortex_fma(tmp_path):
