@@ -45,6 +45,7 @@ struct orc_ctx {
     double ad_c[2], ad_a;
     int ad_bc[4], ad_fixed;
     double *gradQbnd; /* (nVar, nDim, 4, n) per cell */
+    double *margin;   /* when recording: the DMP margin of each cell's candidate in the last step */
 };
 
 /* PNPMTable of Euler/ader_dg.f90:28 */
@@ -618,7 +619,7 @@ void orc_destroy(orc_ctx *c)
     free(c->uh); free(c->olduh); free(c->qhi); free(c->Fhi); free(c->duh); free(c->qbnd); free(c->Fbnd);
     free(c->Fu); free(c->Gu); free(c->recompute); free(c->status); free(c->oldstatus); free(c->Lh);
     free(c->NewLh); free(c->lmin); free(c->lmax); free(c->lmin_old); free(c->lmax_old);
-    free(c->Fp); free(c->Gp); free(c->Shi); free(c->gradQbnd);
+    free(c->Fp); free(c->Gp); free(c->Shi); free(c->gradQbnd); free(c->margin);
     free(c);
 }
 
@@ -659,6 +660,20 @@ void orc_get_scalars(const orc_ctx *c, double *dt, double *time, double *tend)
 }
 
 static size_t ncell(const orc_ctx *c) { return (size_t)c->imax * c->jmax; }
+
+/* Test support: record the DMP margin (orc_dmp_margin, < 0 where the cell is troubled) of
+ * every cell's candidate solution during each step; orc_get_margins copies the last step's. */
+void orc_record_margins(orc_ctx *c, int on)
+{
+    free(c->margin);
+    c->margin = on ? (double *)calloc(ncell(c), sizeof(double)) : NULL;
+}
+
+void orc_get_margins(const orc_ctx *c, double *out)
+{
+    if (c->margin) memcpy(out, c->margin, ncell(c) * sizeof(double));
+}
+
 
 void orc_set_uh(orc_ctx *c, const double *uh) { memcpy(c->uh, uh, c->V * c->n * c->n * ncell(c) * sizeof(double)); }
 void orc_get_uh(const orc_ctx *c, double *uh) { memcpy(uh, c->uh, c->V * c->n * c->n * ncell(c) * sizeof(double)); }
@@ -2011,6 +2026,7 @@ int orc_step(orc_ctx *c)
         }
     for (size_t k = 0; k < nc; k++) {
         orc_update(c, &c->uh[blk * k], &c->duh[blk * k]);
+        if (c->margin) c->margin[k] = orc_dmp_margin(c, &c->uh[blk * k], &c->lmin[V * k], &c->lmax[V * k], 0.0);
         if (!orc_dmp(c, &c->uh[blk * k], &c->lmin[V * k], &c->lmax[V * k], 0.0)) {
             c->recompute[k] = 1;
             nrec++;

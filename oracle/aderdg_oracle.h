@@ -58,6 +58,8 @@ void orc_put_subcell_data(orc_ctx *c, double *luh, const double *lim);
 int orc_dmp(orc_ctx *c, const double *luh, const double *lmin, const double *lmax, double argmax);
 /* signed DMP margin (negative = troubled), used for the near-threshold rule */
 double orc_dmp_margin(orc_ctx *c, const double *luh, const double *lmin, const double *lmax, double argmax);
+void orc_record_margins(orc_ctx *c, int on);
+void orc_get_margins(const orc_ctx *c, double *out);
 
 /* Baer-Nunziato (pde = ORC_PDE_BN, eos = GAMMA_S, GAMMA_G, PI_S, PI_G) */
 void orc_predictor_bn(orc_ctx *c, double *lqhi, double *lFhi, double *lShi, double *lqbnd, double *lFbnd,

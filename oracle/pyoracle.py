@@ -60,6 +60,8 @@ class Oracle:
             lib.orc_step.argtypes = [ctypes.c_void_p]
             lib.orc_dmp.restype = ctypes.c_int
             lib.orc_dmp.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double]
+            lib.orc_record_margins.argtypes = [ctypes.c_void_p, ctypes.c_int]
+            lib.orc_get_margins.argtypes = [ctypes.c_void_p, _dp]
             lib.orc_dmp_margin.restype = ctypes.c_double
             lib.orc_dmp_margin.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double]
             lib.orc_set_scalars.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double]
@@ -267,6 +269,16 @@ class Oracle:
     def dmp(self, luh, lmin, lmax, argmax):
         a = [np.asfortranarray(x, dtype=np.float64) for x in (luh, lmin, lmax)]
         return bool(self.lib().orc_dmp(self.h, *[_p(x) for x in a], argmax))
+
+    def record_margins(self, on=True):
+        """Record each cell's DMP margin (< 0: troubled) during every following step."""
+        self.lib().orc_record_margins(self.h, 1 if on else 0)
+
+    def margins(self):
+        """(IMAX, JMAX) DMP margins of the last step's candidate solution."""
+        m = np.zeros((self.imax, self.jmax), order="F")
+        self.lib().orc_get_margins(self.h, _p(m))
+        return m
 
     def dmp_margin(self, luh, lmin, lmax, argmax):
         a = [np.asfortranarray(x, dtype=np.float64) for x in (luh, lmin, lmax)]
