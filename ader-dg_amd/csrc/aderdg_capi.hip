@@ -191,7 +191,7 @@ int launch(aderdg_ctx *c, int part, double time = 0.0, double tend = 0.0, int us
     }
     hipError_t e;
     if (part == kPartDt)
-        e = launch_dt(c->g.b, c->stream, time, tend, use_host);
+        e = launch_dt(c->g.b, c->stream, time, tend, use_host, c->g.dt_grid());
     else
         e = c->g.launch(part);
     if (e != hipSuccess) return fail(ADERDG_EHIP, "kernel launch (part %d): %s", part, hipGetErrorString(e));

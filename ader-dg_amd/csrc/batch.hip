@@ -169,7 +169,7 @@ int Batch::build_graph()
     hipError_t e = hipSuccess;
     static const int parts[] = {kPartPredictor, kPartFace, kPartUpdate, kPartSpread, kPartSubcell, kPartAdvance};
     for (auto &g : groups) {
-        if (e == hipSuccess) e = launch_dt(g->b, stream, 0.0, 0.0, 0);
+        if (e == hipSuccess) e = launch_dt(g->b, stream, 0.0, 0.0, 0, g->dt_grid());
         for (int p : parts)
             if (e == hipSuccess) e = g->launch(p);
     }
@@ -223,7 +223,7 @@ int Batch::step(int nsteps)
                         }
                     }
                     HIPCHK(hipEventRecord(e0, stream));
-                    hipError_t e = parts[p] == kPartDt ? launch_dt(groups[gi]->b, stream, 0.0, 0.0, 0)
+                    hipError_t e = parts[p] == kPartDt ? launch_dt(groups[gi]->b, stream, 0.0, 0.0, 0, groups[gi]->dt_grid())
                                                        : groups[gi]->launch(parts[p]);
                     if (e != hipSuccess) return fail(ADERDG_EHIP, "aderdg_batch_step: launch: %s", hipGetErrorString(e));
                     HIPCHK(hipEventRecord(e1, stream));

@@ -97,6 +97,7 @@ public:
     hipStream_t stream() const { return st_; }
 
     hipError_t launch(int part) const;
+    int dt_grid() const { return grid[kTabCell] < kDtGrid ? grid[kTabCell] : kDtGrid; }
     // Bytes of device memory per cell / x-face / y-face and per slot of a (PDE, N).
     static void footprint(int pde, int N, int64_t *per_cell, int64_t *per_xface, int64_t *per_yface,
                           int64_t *per_slot);

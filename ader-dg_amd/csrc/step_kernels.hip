@@ -2390,7 +2390,7 @@ void table_quanta(int pde, int N, int variant, int quanta[kNumTabs])
 hipError_t launch_part(int pde, int N, int variant, int upd, const StepBufs &b, hipStream_t st, int part,
                        const int grid[kNumTabs], int nslot_cap)
 {
-    if (part == kPartDt) return launch_dt(b, st, 0.0, 0.0, 0);
+    if (part == kPartDt) return launch_dt(b, st, 0.0, 0.0, 0, grid[kTabCell]);
     if (part == kPartAdvance || part == kPartFlip || part == kPartSeedDone) {
         const int g = (nslot_cap + kBlock - 1) / kBlock;
         hipLaunchKernelGGL(k_solves, dim3(g > 0 ? g : 1), dim3(kBlock), 0, st, b, part);
@@ -2401,9 +2401,10 @@ hipError_t launch_part(int pde, int N, int variant, int upd, const StepBufs &b, 
                          : launch_part_pde<EulerPDE>(N, b, st, part, grid, variant, upd);
 }
 
-hipError_t launch_dt(const StepBufs &b, hipStream_t st, double time, double tend, int use_host_time)
+hipError_t launch_dt(const StepBufs &b, hipStream_t st, double time, double tend, int use_host_time, int grid)
 {
-    hipLaunchKernelGGL(k_dt, dim3(kDtGrid), dim3(kBlock), 0, st, b, time, tend, use_host_time);
+    grid = grid < 1 ? 1 : (grid > kDtGrid ? kDtGrid : grid);
+    hipLaunchKernelGGL(k_dt, dim3(grid), dim3(kBlock), 0, st, b, time, tend, use_host_time);
     return hipGetLastError();
 }
 

@@ -117,7 +117,9 @@ hipError_t launch_part_ad(int N, const StepBufs &b, hipStream_t st, int part, co
 hipError_t launch_part(int pde, int N, int variant, int upd, const StepBufs &b, hipStream_t st, int part,
                        const int grid[kNumTabs], int nslot_cap);
 // calc_time_step of every active solve; use_host: time and tend given here instead of the solve state.
-hipError_t launch_dt(const StepBufs &b, hipStream_t st, double time, double tend, int use_host);
+// grid: workgroups (at most kDtGrid; the cell table's length is enough, and fewer
+// workgroups means fewer finalisation tickets on one counter).
+hipError_t launch_dt(const StepBufs &b, hipStream_t st, double time, double tend, int use_host, int grid = kDtGrid);
 // Rebuild the work tables from per-table prefix counts of entries over the solve slots
 // (prefix[k * (nslot + 1) + s], device memory); max_entries bounds every table's length.
 hipError_t launch_tables(const StepBufs &b, const int *prefix, int nslot, const int quanta[kNumTabs], int max_entries,
