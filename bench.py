@@ -20,7 +20,7 @@ advanced one step (nVar*(N+1)^2 per cell-step).  Inputs are resident in HBM
 before the timed region starts; uploads are not timed.
 
 Workloads `euler_vortex_512_n4` and `euler_quadrant_1024_n3` time one plain
-solve.  `ad_sine_64_n3` runs the linear advection-diffusion problem (periodic
+solve; `euler_quadrant_1024_n3_t0.3` runs the quadrant to its end time T=0.3.  `ad_sine_64_n3` runs the linear advection-diffusion problem (periodic
 [0,1]^2, 64x64 cells, N=3, c=(1,1), diffusivity 0.01, u0 = sin 2 pi x sin 2 pi y,
 CFL 0.9) to T=1 with the reference's viscous-only predictor flux and with the
 fused advective + viscous one, and reports each run's L2 error against the exact
@@ -50,6 +50,8 @@ PLAIN = {
     "euler_vortex_512_n4": ("vortex", 4, 512, 512, (0.0, 0.0), (10.0, 10.0), "periodic"),
     "euler_quadrant_1024_n3": ("quadrant", 3, 1024, 1024, (0.0, 0.0), (1.0, 1.0), "transmissive"),
 }
+# plain workloads run to an end time: name -> (plain workload, end time)
+PLAIN_TO_T = {"euler_quadrant_1024_n3_t0.3": ("euler_quadrant_1024_n3", 0.3)}
 # name: (N, cells per side, advection velocity, diffusivity, end time)
 AD = {"ad_sine_64_n3": (3, 64, (1.0, 1.0), 0.01, 1.0)}
 ENSEMBLES = {"bn_ensemble_256": dict(count=256, seed=2026)}
@@ -161,12 +163,69 @@ def cpu_ensemble(specs, target_seconds, procs):
 
 
 # ---------------------------------------------------------------------------
+def predictor_name(pde, N, persistent):
+    """Kernel name of the predictor variant the autotuner chose."""
+    import aderdg
+    names = {0: "k_predictor", 1: "k_predictor_nodes<.., 64>", 2: "k_predictor_nodes<.., 128>",
+             3: "k_predictor_nodes<.., 256>", 4: "k_predictor_lean"}
+    t = aderdg.tune_query(pde, N, persistent)
+    return "%s<%sPDE,%d>" % (names[t[0] if t else 0], "Bn" if pde == "bn" else "Euler", N)
+
+
 def roofline_of(flop_per_cell, cells, ms_per_launch, kernel, traffic=None):
     achieved = flop_per_cell * cells / (ms_per_launch * 1e-3) / 1e12
     return {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic, "traffic_unit": "bytes/launch",
             "kernel": kernel, "flop_per_cell": flop_per_cell, "cells_per_launch": cells,
             "note": "FP64 vector ALU; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s"}
+
+
+def run_plain_to_t(args):
+    """A plain workload to its end time (the whole run timed, graph replays); a few
+    steps of the final state launched kernel by kernel give the per-kernel times."""
+    import aderdg
+    from aderdg import ics
+    base, tend = PLAIN_TO_T[args.workload]
+    ic, N, I, J, xl, xr, bc = PLAIN[base]
+    uh0 = ics.initial_uh(ic, aderdg.operators(N)["xi"], I, J, xl, xr)
+    s = aderdg.Solver(N, I, J, xl, xr, bc)
+    s.upload(uh0)
+    if args.warmup:
+        s.run(args.warmup, 0.0, tend)
+        s.upload(uh0)
+    t, steps, dt, troubled, last_log = 0.0, 0, 0.0, 0, 0
+    t0 = time.perf_counter()
+    while t < tend:
+        # graph replays in chunks that stop short of tend (dt changes slowly), then single
+        # steps, so that no step runs after the solve has reached its end time
+        k = 1 if dt <= 0.0 else max(1, min(500, int(0.9 * (tend - t) / dt)))
+        t, dt, troubled = s.run(k, t, tend)
+        steps += k
+        if steps - last_log >= 1000:
+            last_log = steps
+            log("t = %.6g after %d steps (%.1f s), nRecompute %d" % (t, steps, time.perf_counter() - t0, troubled))
+    el = time.perf_counter() - t0
+    log("t = %.6g after %d steps in %.2f s" % (t, steps, el))
+    uh = s.download()
+    s.upload(uh)
+    s.timing(True)
+    s.run(5, 0.0, 1e30)
+    ms, launches = s.timing_get()
+    s.close()
+    ncell = I * J
+    per = {name: (ms[k] / launches[k] if launches[k] else None) for k, name in enumerate(aderdg.TIMING_SLOTS)}
+    log("per-kernel average ms (final state):", json.dumps(per))
+    fl_pred, _ = aderdg.flops_per_cell(N)
+    return {
+        "metric": "element-DOF updates/s (plain single-instance Euler step, to the end time)",
+        "value": steps * ncell * 4 * (N + 1) ** 2 / el, "unit": "element-DOF updates/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": el / steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (analytic initial condition)",
+        "config": {"workload": args.workload, "pde": "euler", "ic": ic, "N": N, "cells": [I, J], "bc": bc,
+                   "tend": tend, "seconds_to_tend": el, "last_nRecompute": troubled},
+        "roofline": roofline_of(fl_pred, ncell, per["predictor"], predictor_name("euler", N, False))
+        if per["predictor"] else None,
+    }
 
 
 def run_plain(args):
@@ -190,7 +249,7 @@ def run_plain(args):
     log("per-kernel average ms:", json.dumps(per))
     log("simulated time %.6g after %d steps, last dt %.6g, nRecompute %d" % (t1_sim, args.steps, dt, nrec))
     fl_pred, _ = aderdg.flops_per_cell(N)
-    roof = roofline_of(fl_pred, ncell, per["predictor"], "k_predictor<EulerPDE,%d>" % N,
+    roof = roofline_of(fl_pred, ncell, per["predictor"], predictor_name("euler", N, False),
                        PREDICTOR_TRAFFIC_BYTES.get(args.workload)) if per["predictor"] else None
     return {
         "metric": "element-DOF updates/s (plain single-instance Euler step)",
@@ -325,11 +384,7 @@ def run_ensemble(args, specs):
     dom = max(per_group, key=lambda N: per_group[N]["predictor"] or 0.0)
     fl_pred, _ = aderdg.flops_per_cell(dom, "bn")
     tuned = {N: aderdg.tune_query("bn", N, True) for N in per_group}
-    kname = {0: "k_predictor", 1: "k_predictor_nodes<.., 64>", 2: "k_predictor_nodes<.., 128>",
-             3: "k_predictor_nodes<.., 256>", 4: "k_predictor_lean"}
-    var = tuned[dom][0] if tuned[dom] else 0
-    roof = roofline_of(fl_pred, per_group[dom]["cells"], per_group[dom]["predictor"],
-                       "%s<BnPDE,%d>" % (kname[var], dom),
+    roof = roofline_of(fl_pred, per_group[dom]["cells"], per_group[dom]["predictor"], predictor_name("bn", dom, True),
                        PREDICTOR_TRAFFIC_BYTES.get("%s_n%d" % (args.workload, dom)))
     out = {
         "metric": "element-DOF updates/s batched vs plain; p50/p99 solve latency at fixed QPS",
@@ -448,7 +503,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20 ensemble, 200 plain)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (default 2 ensemble, 10 plain)")
-    ap.add_argument("--workload", default="bn_ensemble_256", choices=sorted(PLAIN) + sorted(AD) + sorted(ENSEMBLES) + sorted(SERVING))
+    ap.add_argument("--workload", default="bn_ensemble_256", choices=sorted(PLAIN) + sorted(PLAIN_TO_T) + sorted(AD) + sorted(ENSEMBLES) + sorted(SERVING))
     ap.add_argument("--serving-requests", type=int, default=400,
                     help="ensemble workload: requests of the serving stream timed beside it (0: none)")
     ap.add_argument("--count", type=int, default=None, help="ensemble size (default 256)")
@@ -503,6 +558,10 @@ def main():
         if args.serving_requests:
             sspecs = ics.serving_requests(**dict(SERVING["serving_4000"], count=args.serving_requests))
             out["serving"] = run_serving(sspecs, (0.8,))
+    elif args.workload in PLAIN_TO_T:
+        out = run_plain_to_t(args)
+        if not args.no_cpu:
+            cpu = cpu_plain(PLAIN_TO_T[args.workload][0], args.cpu_seconds)
     elif args.workload in AD:
         out = run_ad(args)
         if not args.no_cpu:
