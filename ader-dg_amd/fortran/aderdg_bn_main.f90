@@ -17,8 +17,8 @@
 !   dump file  raw float64 uh(9,N+1,N+1,IMAX,JMAX) written at the end
 !
 ! ICType 1, 2, 3, 7: the reference's one-dimensional Riemann problems.  Boundary
-! codes: 1 (transmissive) or 3 (periodic) on all four sides; the reference's
-! inflow (0) and reflective (2) codes are not provided.
+! codes: 1 (transmissive) on all four sides, the faces the reference computes for
+! any codes; the codes' effect on its limiter neighbour table alone is not provided.
 
 program aderdg_bn
     use iso_c_binding
@@ -92,12 +92,13 @@ program aderdg_bn
     desc%xr = xR
     desc%cfl = 0.9d0
     desc%bn_eos = eos
-    if (bL == 3 .and. bR == 3 .and. bB == 3 .and. bT == 3) then
-        desc%bc_mode = ADERDG_BC_PERIODIC
-    else if (bL == 1 .and. bR == 1 .and. bB == 1 .and. bT == 1) then
+    ! the reference's face loop is transmissive on every side whatever the codes
+    ! (Baer-Nunziato/riemann_problem.f90:9-80); other codes only change its limiter
+    ! neighbour table, a mix this driver does not reproduce
+    if (bL == 1 .and. bR == 1 .and. bB == 1 .and. bT == 1) then
         desc%bc_mode = ADERDG_BC_TRANSMISSIVE
     else
-        print *, 'ERROR: boundary codes ', bL, bR, bB, bT, ': all 1 (transmissive) or all 3 (periodic) are provided'
+        print *, 'ERROR: boundary codes ', bL, bR, bB, bT, ': code 1 (transmissive) on all sides is provided'
         stop 1
     end if
 
