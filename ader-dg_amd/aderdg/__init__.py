@@ -206,6 +206,9 @@ def load_library(exact=False):
         "aderdg_batch_create": [ctypes.POINTER(_BatchDesc), ctypes.POINTER(ctypes.c_void_p)],
         "aderdg_batch_destroy": [vp],
         "aderdg_batch_reserve": [vp, ctypes.c_int, ctypes.c_int, i64, ctypes.c_int],
+        "aderdg_batch_reserve_limited": [vp, ctypes.c_int, ctypes.c_int, i64, ctypes.c_int, ctypes.c_double],
+        "aderdg_batch_error": [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+        "aderdg_server_reserve_limited": [vp, ctypes.c_int, ctypes.c_int, i64, ctypes.c_int, ctypes.c_double],
         "aderdg_batch_add": [vp, ctypes.POINTER(_Desc), _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                              ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
         "aderdg_batch_step": [vp, ctypes.c_int],
@@ -246,6 +249,8 @@ def load_library(exact=False):
         f.restype = ctypes.c_int
     lib.aderdg_solve_footprint.restype = i64
     lib.aderdg_solve_footprint.argtypes = [ctypes.c_int] * 4
+    lib.aderdg_solve_footprint_limited.restype = i64
+    lib.aderdg_solve_footprint_limited.argtypes = [ctypes.c_int] * 4 + [ctypes.c_double]
     lib.aderdg_batch_reserved_bytes.restype = i64
     lib.aderdg_batch_reserved_bytes.argtypes = [vp]
     lib.aderdg_flops_predictor.restype = ctypes.c_double
@@ -456,10 +461,14 @@ def pred_variant_valid(pde, N, variant):
     return variant in (0, 2, 3, 4) or (variant == 1 and (N + 1) ** 3 <= 64)
 
 
-def solve_footprint(pde, N, imax, jmax, exact=False):
-    """Device bytes one solve of this shape occupies in a batch arena."""
+def solve_footprint(pde, N, imax, jmax, exact=False, limited_fraction=None):
+    """Device bytes one solve of this shape occupies in a batch arena (reserved with
+    this limited-cell pool fraction, or one pool slot per cell)."""
     k = PDE[pde] if isinstance(pde, str) else int(pde)
-    return int(load_library(exact).aderdg_solve_footprint(k, N, imax, jmax))
+    lib = load_library(exact)
+    if limited_fraction is None:
+        return int(lib.aderdg_solve_footprint(k, N, imax, jmax))
+    return int(lib.aderdg_solve_footprint_limited(k, N, imax, jmax, float(limited_fraction)))
 
 
 class ArenaFull(AderdgError):
@@ -480,6 +489,7 @@ class Batch:
     """
 
     EFULL = -6
+    ECAPACITY = -5
 
     def __init__(self, exact=False, device=0, persist_blocks=0):
         self.lib = load_library(exact)
@@ -508,9 +518,15 @@ class Batch:
         except Exception:
             pass
 
-    def reserve(self, pde, N, max_cells, max_solves):
+    def reserve(self, pde, N, max_cells, max_solves, limited_fraction=None):
+        """Arena of one (pde, N); limited_fraction: limited-cell pool slots per cell of a
+        solve (aderdg_batch_reserve_limited), default one per cell."""
         k = PDE[pde] if isinstance(pde, str) else int(pde)
-        self._check(self.lib.aderdg_batch_reserve(self.h, k, N, int(max_cells), int(max_solves)))
+        if limited_fraction is None:
+            self._check(self.lib.aderdg_batch_reserve(self.h, k, N, int(max_cells), int(max_solves)))
+        else:
+            self._check(self.lib.aderdg_batch_reserve_limited(self.h, k, N, int(max_cells), int(max_solves),
+                                                              float(limited_fraction)))
 
     def reserved_bytes(self):
         return int(self.lib.aderdg_batch_reserved_bytes(self.h))
@@ -571,6 +587,12 @@ class Batch:
         rc = np.zeros((I, J), dtype=np.int32, order="F")
         self._check(self.lib.aderdg_batch_limiter(self.h, i, st.ctypes.data_as(_ip), rc.ctypes.data_as(_ip)))
         return st, rc
+
+    def error(self, i):
+        """0, or Batch.ECAPACITY when the solve ran out of limited-cell pool slots."""
+        e = ctypes.c_int()
+        self._check(self.lib.aderdg_batch_error(self.h, i, ctypes.byref(e)))
+        return e.value
 
     def remove(self, i):
         self._check(self.lib.aderdg_batch_remove(self.h, i))
@@ -642,9 +664,13 @@ class Server:
         except Exception:
             pass
 
-    def reserve(self, pde, N, max_cells, max_solves):
+    def reserve(self, pde, N, max_cells, max_solves, limited_fraction=None):
         k = PDE[pde] if isinstance(pde, str) else int(pde)
-        self._check(self.lib.aderdg_server_reserve(self.h, k, N, int(max_cells), int(max_solves)))
+        if limited_fraction is None:
+            self._check(self.lib.aderdg_server_reserve(self.h, k, N, int(max_cells), int(max_solves)))
+        else:
+            self._check(self.lib.aderdg_server_reserve_limited(self.h, k, N, int(max_cells), int(max_solves),
+                                                               float(limited_fraction)))
 
     def start(self):
         self._check(self.lib.aderdg_server_start(self.h))

@@ -405,10 +405,23 @@ int aderdg_set_limiter(aderdg_ctx *c, const int32_t *status, const double *Lh)
     if (!c->have_uh) return fail(ADERDG_ESTATE, "aderdg_set_limiter: upload uh first");
     const size_t nc = c->ncell, blk = (size_t)c->V * c->s * c->s;
     const int par = c->st.parity;
-    std::vector<int32_t> st(nc);
-    for (size_t k = 0; k < nc; k++) st[k] = status[k] ? 1 : 0;
+    // limited cells take the pool slots of the solve in cell order
+    const SolveArgs &sa = c->g.args(c->slot);
+    std::vector<int32_t> st(nc), slot(nc, -1);
+    std::vector<double> pool;
+    for (size_t k = 0; k < nc; k++) {
+        st[k] = status[k] ? 1 : 0;
+        if (!st[k]) continue;
+        slot[k] = sa.lh0 + (int)(pool.size() / blk);
+        pool.insert(pool.end(), Lh + k * blk, Lh + (k + 1) * blk);
+    }
+    if (pool.size() > (size_t)sa.lhcap * blk)
+        return fail(ADERDG_ECAPACITY, "aderdg_set_limiter: %zu limited cells, pool of %d", pool.size() / blk, sa.lhcap);
     HIPCHK(hipMemcpyAsync(c->g.b.status[par], st.data(), nc * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->g.b.lh[par], Lh, nc * blk * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->g.b.lh_slot[par], slot.data(), nc * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    if (!pool.empty())
+        HIPCHK(hipMemcpyAsync(c->g.b.lh[par] + (size_t)sa.lh0 * blk, pool.data(), pool.size() * sizeof(double),
+                              hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     int r = seed(c);
     if (r) return r;
@@ -435,11 +448,19 @@ int aderdg_get_limiter(aderdg_ctx *c, int32_t *status, int32_t *recompute, doubl
     if (!c) return fail(ADERDG_EINVAL, "aderdg_get_limiter: null context");
     const size_t nc = c->ncell, blk = (size_t)c->V * c->s * c->s, V = c->V;
     const int par = c->st.parity;
-    std::vector<int32_t> st(nc);
+    const SolveArgs &sa = c->g.args(c->slot);
+    std::vector<int32_t> st(nc), slot;
+    std::vector<double> pool;
     HIPCHK(hipMemcpyAsync(st.data(), c->g.b.status[par], nc * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     if (recompute)
         HIPCHK(hipMemcpyAsync(recompute, c->g.b.rc_final, nc * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-    if (Lh) HIPCHK(hipMemcpyAsync(Lh, c->g.b.lh[par], nc * blk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (Lh && sa.lhcap > 0) {  // the solve's pool segment and the slots of its limited cells
+        slot.resize(nc);
+        pool.resize((size_t)sa.lhcap * blk);
+        HIPCHK(hipMemcpyAsync(slot.data(), c->g.b.lh_slot[par], nc * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(pool.data(), c->g.b.lh[par] + (size_t)sa.lh0 * blk, pool.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, c->stream));
+    }
     std::vector<double> mn, mx;
     if (lmin) {
         mn.resize(V * nc);
@@ -453,7 +474,13 @@ int aderdg_get_limiter(aderdg_ctx *c, int32_t *status, int32_t *recompute, doubl
     if (r) return r;
     for (size_t k = 0; k < nc; k++) {
         if (status) status[k] = st[k];
-        if (Lh && !st[k]) std::memset(Lh + k * blk, 0, blk * sizeof(double));
+        if (Lh) {
+            const int ls = st[k] && !slot.empty() ? slot[k] - sa.lh0 : -1;
+            if (ls >= 0 && ls < sa.lhcap)
+                std::memcpy(Lh + k * blk, pool.data() + (size_t)ls * blk, blk * sizeof(double));
+            else
+                std::memset(Lh + k * blk, 0, blk * sizeof(double));
+        }
         for (size_t v = 0; v < V; v++) {
             if (lmin) lmin[v + V * k] = mn[v * nc + k];
             if (lmax) lmax[v + V * k] = mx[v * nc + k];

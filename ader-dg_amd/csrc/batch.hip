@@ -67,14 +67,22 @@ int Batch::group_of(int pde, int N) const
     return -1;
 }
 
-int64_t Batch::footprint(int pde, int N, int imax, int jmax)
+int64_t Batch::footprint(int pde, int N, int imax, int jmax, double lh_fraction)
 {
-    int64_t pc, px, py, ps;
-    Group::footprint(pde, N, &pc, &px, &py, &ps);
-    return pc * imax * jmax + px * (int64_t)(imax + 1) * jmax + py * (int64_t)imax * (jmax + 1) + ps;
+    int64_t pc, px, py, ps, pl;
+    Group::footprint(pde, N, &pc, &px, &py, &ps, &pl);
+    const int64_t nc = (int64_t)imax * jmax;
+    return pc * nc + px * (int64_t)(imax + 1) * jmax + py * (int64_t)imax * (jmax + 1) + ps +
+           pl * Group::lh_slots(pde, nc, lh_fraction);
 }
 
-int Batch::reserve(int pde, int N, int64_t max_cells, int max_solves)
+double Batch::lh_fraction(int pde, int N) const
+{
+    const int g = group_of(pde, N);
+    return g < 0 ? 1.0 : groups[g]->caps.lh_fraction;
+}
+
+int Batch::reserve(int pde, int N, int64_t max_cells, int max_solves, double lh_fraction)
 {
     if (pde != ADERDG_PDE_EULER && pde != ADERDG_PDE_BN && pde != ADERDG_PDE_AD)
         return fail(ADERDG_EINVAL, "aderdg_batch_reserve: bad pde %d", pde);
@@ -82,12 +90,14 @@ int Batch::reserve(int pde, int N, int64_t max_cells, int max_solves)
     if (max_cells < 1 || max_cells > (1LL << 28) || max_solves < 1 || max_solves > (1 << 20))
         return fail(ADERDG_EINVAL, "aderdg_batch_reserve: bad capacity (%lld cells, %d solves)", (long long)max_cells,
                     max_solves);
+    if (!(lh_fraction > 0.0 && lh_fraction <= 1.0))
+        return fail(ADERDG_EINVAL, "aderdg_batch_reserve: limited-cell fraction %g outside (0, 1]", lh_fraction);
     if (group_of(pde, N) >= 0) return fail(ADERDG_EINVAL, "aderdg_batch_reserve: (pde %d, N %d) already reserved", pde, N);
     Operators ops;
     if (!build_operators(N, ops)) return fail(ADERDG_ESINGULAR, "aderdg_batch_reserve: operator construction failed");
     // faces: (IMAX+1)*JMAX = cells + JMAX <= cells * 9/8 when IMAX >= 8, and alike for y
     const int64_t faces = max_cells + max_cells / 8 + 1;
-    const GroupCaps caps{max_cells, faces, faces, max_solves};
+    const GroupCaps caps{max_cells, faces, faces, max_solves, lh_fraction};
     auto g = std::make_unique<Group>();
     int r = g->init(pde, N, caps, ops, persist, stream);
     if (r) {
@@ -99,9 +109,9 @@ int Batch::reserve(int pde, int N, int64_t max_cells, int max_solves)
         g->release();
         return fail(ADERDG_EHIP, "aderdg_batch_reserve: hipHostMalloc failed");
     }
-    int64_t pc, px, py, psl;
-    Group::footprint(pde, N, &pc, &px, &py, &psl);
-    reserved += pc * max_cells + (px + py) * faces + psl * max_solves;
+    int64_t pc, px, py, psl, pl;
+    Group::footprint(pde, N, &pc, &px, &py, &psl, &pl);
+    reserved += pc * max_cells + (px + py) * faces + psl * max_solves + pl * Group::lh_pool(pde, max_cells, max_solves, lh_fraction);
     groups.push_back(std::move(g));
     pinned_states.push_back(ps);
     ms.emplace_back(ADERDG_TIMING_SLOTS, 0.0);
@@ -298,6 +308,16 @@ int Batch::limiter(int id, int32_t *status, int32_t *recompute)
     return 0;
 }
 
+int Batch::error(int id, int *err)
+{
+    auto it = solves.find(id);
+    if (it == solves.end()) return fail(ADERDG_EINVAL, "aderdg_batch_error: unknown id %d", id);
+    int r = snapshot();
+    if (r) return r;
+    *err = pinned_states[it->second.group][it->second.slot].failed ? ADERDG_ECAPACITY : 0;
+    return 0;
+}
+
 int Batch::remove(int id)
 {
     auto it = solves.find(id);
@@ -346,12 +366,30 @@ int aderdg_batch_reserve(aderdg_batch *b, int pde, int N, int64_t max_cells, int
     return b->eng.reserve(pde, N, max_cells, max_solves);
 }
 
-int64_t aderdg_solve_footprint(int pde, int N, int imax, int jmax)
+int aderdg_batch_reserve_limited(aderdg_batch *b, int pde, int N, int64_t max_cells, int max_solves,
+                                 double limited_fraction)
+{
+    if (!b) return fail(ADERDG_EINVAL, "aderdg_batch_reserve_limited: null batch");
+    return b->eng.reserve(pde, N, max_cells, max_solves, limited_fraction);
+}
+
+int64_t aderdg_solve_footprint_limited(int pde, int N, int imax, int jmax, double limited_fraction)
 {
     if ((pde != ADERDG_PDE_EULER && pde != ADERDG_PDE_BN && pde != ADERDG_PDE_AD) || N < 1 || N > kMaxDegree ||
-        imax < 1 || jmax < 1)
+        imax < 1 || jmax < 1 || !(limited_fraction > 0.0 && limited_fraction <= 1.0))
         return -1;
-    return Batch::footprint(pde, N, imax, jmax);
+    return Batch::footprint(pde, N, imax, jmax, limited_fraction);
+}
+
+int64_t aderdg_solve_footprint(int pde, int N, int imax, int jmax)
+{
+    return aderdg_solve_footprint_limited(pde, N, imax, jmax, 1.0);
+}
+
+int aderdg_batch_error(aderdg_batch *b, int id, int *error)
+{
+    if (!b || !error) return fail(ADERDG_EINVAL, "aderdg_batch_error: null argument");
+    return b->eng.error(id, error);
 }
 
 int64_t aderdg_batch_reserved_bytes(aderdg_batch *b) { return b ? b->eng.reserved : -1; }

@@ -43,7 +43,8 @@ public:
 
     int init(int device, int persist);
     void release();
-    int reserve(int pde, int N, int64_t max_cells, int max_solves);
+    // lh_fraction: limited-cell pool slots per cell of a solve (GroupCaps)
+    int reserve(int pde, int N, int64_t max_cells, int max_solves, double lh_fraction = 1.0);
     // initial state from the host array uh, or computed on the device from ic
     int add(const aderdg_desc *d, const double *uh, const aderdg_ic *ic, double time, double tend, int max_steps,
             int stop_at_tend, int *id);
@@ -55,9 +56,11 @@ public:
     const SolveState *state_of(int id) const;  // from the last snapshot
     int download(int id, double *uh);
     int limiter(int id, int32_t *status, int32_t *recompute);
+    int error(int id, int *err);  // ADERDG_ECAPACITY: the solve ran out of limited-cell pool slots
     int remove(int id);
     int group_of(int pde, int N) const;
-    static int64_t footprint(int pde, int N, int imax, int jmax);
+    static int64_t footprint(int pde, int N, int imax, int jmax, double lh_fraction = 1.0);
+    double lh_fraction(int pde, int N) const;  // of the (pde, N) arena, 1 if none
 
 private:
     int build_graph();

@@ -2,6 +2,7 @@
 #include "group.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "../../include/aderdg.h"
@@ -129,18 +130,35 @@ int Group::dalloc(T **p, size_t count)
     return 0;
 }
 
-void Group::footprint(int pde, int N, int64_t *per_cell, int64_t *per_xface, int64_t *per_yface, int64_t *per_slot)
+void Group::footprint(int pde, int N, int64_t *per_cell, int64_t *per_xface, int64_t *per_yface, int64_t *per_slot,
+                      int64_t *per_lh)
 {
     const int64_t V = pde == kPdeBn ? 9 : (pde == kPdeAd ? 1 : 4), n = N + 1, s = 2 * N + 1;
     const int64_t sides = pde == kPdeBn ? 2 : 1, ffc = pde == kPdeAd ? 2 : V;
-    // uh x2, lduh, fq + ff, lmin/lmax old and new, cell_dt, Lh x2; ints: rc, rc_final, list, status x2, cell_solve
-    const int64_t dbl = 3 * V * n * n + 4 * (V + ffc) * n + 4 * V + 1 + 2 * V * s * s;
-    const int64_t ints = 6;
+    // uh x2, lduh, fq + ff, lmin/lmax old and new, cell_dt; ints: rc, rc_final, list, status x2,
+    // lh_slot x2, cell_solve
+    const int64_t dbl = 3 * V * n * n + 4 * (V + ffc) * n + 4 * V + 1;
+    const int64_t ints = 8;
     // work-table entries are a few bytes per cell; counted per cell at one entry per 4 cells
     if (per_cell) *per_cell = 8 * dbl + 4 * ints + 4 * 8 / 4;
     if (per_xface) *per_xface = 8 * sides * V * n;
     if (per_yface) *per_yface = 8 * sides * V * n;
-    if (per_slot) *per_slot = (int64_t)(sizeof(SolveArgs) + sizeof(SolveState)) + 4 * 8 * kNumTabs;
+    if (per_slot) *per_slot = (int64_t)(sizeof(SolveArgs) + sizeof(SolveState)) + 4 * 8 * kNumTabs + 4;
+    if (per_lh) *per_lh = 2 * 8 * V * s * s;  // Lh and NewLh
+}
+
+int64_t Group::lh_slots(int pde, int64_t ncell, double fraction)
+{
+    if (pde == kPdeAd) return 0;  // no limiter
+    if (!(fraction < 1.0)) return ncell;
+    return std::min<int64_t>(ncell, (int64_t)std::ceil(fraction * (double)ncell));
+}
+
+int64_t Group::lh_pool(int pde, int64_t cells, int slots, double fraction)
+{
+    // every solve's share rounds up by less than one slot
+    if (pde == kPdeAd) return 0;
+    return std::min<int64_t>(cells, lh_slots(pde, cells, fraction) + (fraction < 1.0 ? slots : 0));
 }
 
 int Group::init(int pde_, int N_, const GroupCaps &c, const Operators &ops_, int persist_blocks, hipStream_t st,
@@ -154,7 +172,8 @@ int Group::init(int pde_, int N_, const GroupCaps &c, const Operators &ops_, int
     caps = c;
     ops = ops_;
     st_ = st;
-    if (c.cells < 1 || c.slots < 1 || c.cells > (1LL << 30) || c.xfaces > (1LL << 30) || c.yfaces > (1LL << 30))
+    if (c.cells < 1 || c.slots < 1 || c.cells > (1LL << 30) || c.xfaces > (1LL << 30) || c.yfaces > (1LL << 30) ||
+        !(c.lh_fraction > 0.0 && c.lh_fraction <= 1.0))
         return ADERDG_EINVAL;
     if (variant_ < 0 || upd_ < 0) {
         int tp = kPredLine, tu = kUpdShared;
@@ -177,6 +196,7 @@ int Group::init(int pde_, int N_, const GroupCaps &c, const Operators &ops_, int
     }
     if (max_entries > (1LL << 30)) return ADERDG_EINVAL;
     const size_t G = c.cells, NN = (size_t)n * n, SS = (size_t)s * s, sides = pde == kPdeBn ? 2 : 1;
+    const int64_t pool = lh_pool(pde, c.cells, c.slots, c.lh_fraction);
     b = StepBufs{};
     b.G = (int)c.cells;
     b.GX = (int)std::max<int64_t>(c.xfaces, 1);
@@ -199,7 +219,8 @@ int Group::init(int pde_, int N_, const GroupCaps &c, const Operators &ops_, int
     for (int k = 0; k < 2; k++) {
         r = r ? r : dalloc(&b.uh[k], V * NN * G);
         r = r ? r : dalloc(&b.status[k], G);
-        r = r ? r : dalloc(&b.lh[k], V * SS * G);
+        r = r ? r : dalloc(&b.lh[k], V * SS * (size_t)pool);
+        r = r ? r : dalloc(&b.lh_slot[k], G);
     }
     r = r ? r : dalloc(&b.lduh, V * NN * G);
     r = r ? r : dalloc(&b.fq, 4 * V * n * G);
@@ -215,6 +236,7 @@ int Group::init(int pde_, int N_, const GroupCaps &c, const Operators &ops_, int
     r = r ? r : dalloc(&b.rc_final, G);
     r = r ? r : dalloc(&b.list, G);
     r = r ? r : dalloc(&b.counters, (size_t)kNumCounters);
+    r = r ? r : dalloc(&b.lh_count, (size_t)c.slots);
     if (r) return r;
     b.op = opd;
     b.solves = sad;
@@ -227,10 +249,12 @@ int Group::init(int pde_, int N_, const GroupCaps &c, const Operators &ops_, int
     cell_alloc_.reset(c.cells);
     fx_alloc_.reset(c.xfaces);
     fy_alloc_.reset(c.yfaces);
+    lh_alloc_.reset(pool);
     nslot_ = 0;
     cells_in_use_ = 0;
     if (hipMemcpy(opd, &ops, sizeof(Operators), hipMemcpyHostToDevice) != hipSuccess) return ADERDG_EHIP;
     if (hipMemset(b.counters, 0, kNumCounters * sizeof(int)) != hipSuccess) return ADERDG_EHIP;
+    if (hipMemset(b.lh_count, 0, c.slots * sizeof(int)) != hipSuccess) return ADERDG_EHIP;
     if (hipMemset(b.state, 0, c.slots * sizeof(SolveState)) != hipSuccess) return ADERDG_EHIP;
     if (hipMemset(meta, 0, kNumMeta * sizeof(int)) != hipSuccess) return ADERDG_EHIP;
     for (int k = 0; k < 2; k++)
@@ -286,7 +310,8 @@ int Group::add(const MeshArgs &m, const SolveState &st0, int *slot)
         }
     if (sl < 0) return ADERDG_EFULL;
     const int64_t nfx = (int64_t)(m.imax + 1) * m.jmax, nfy = (int64_t)m.imax * (m.jmax + 1);
-    int64_t c0 = 0, x0 = 0, y0 = 0;
+    const int64_t nlh = lh_slots(pde, m.ncell, caps.lh_fraction);
+    int64_t c0 = 0, x0 = 0, y0 = 0, l0 = 0;
     if (!cell_alloc_.alloc(m.ncell, &c0)) return ADERDG_EFULL;
     if (!fx_alloc_.alloc(nfx, &x0)) {
         cell_alloc_.release(c0, m.ncell);
@@ -297,12 +322,20 @@ int Group::add(const MeshArgs &m, const SolveState &st0, int *slot)
         fx_alloc_.release(x0, nfx);
         return ADERDG_EFULL;
     }
+    if (!lh_alloc_.alloc(nlh, &l0)) {
+        cell_alloc_.release(c0, m.ncell);
+        fx_alloc_.release(x0, nfx);
+        fy_alloc_.release(y0, nfy);
+        return ADERDG_EFULL;
+    }
     SolveArgs &sa = solves_[sl];
     sa = SolveArgs{};
     sa.m = m;
     sa.cell0 = (int)c0;
     sa.fx0 = (int)x0;
     sa.fy0 = (int)y0;
+    sa.lh0 = (int)l0;
+    sa.lhcap = (int)nlh;
     used_[sl] = 1;
     nslot_ = std::max(nslot_, sl + 1);
     cells_in_use_ += m.ncell;
@@ -314,7 +347,8 @@ int Group::add(const MeshArgs &m, const SolveState &st0, int *slot)
         hipMemsetAsync(b.rc + c0, 0, sizeof(int) * m.ncell, st_) != hipSuccess ||
         hipMemsetAsync(b.rc_final + c0, 0, sizeof(int) * m.ncell, st_) != hipSuccess ||
         hipMemsetAsync(b.status[0] + c0, 0, sizeof(int) * m.ncell, st_) != hipSuccess ||
-        hipMemsetAsync(b.status[1] + c0, 0, sizeof(int) * m.ncell, st_) != hipSuccess)
+        hipMemsetAsync(b.status[1] + c0, 0, sizeof(int) * m.ncell, st_) != hipSuccess ||
+        hipMemsetAsync(b.lh_count + sl, 0, sizeof(int), st_) != hipSuccess)
         return ADERDG_EHIP;
     int r = rebuild_tables();
     if (r) return r;
@@ -331,6 +365,7 @@ int Group::remove(int sl)
     cell_alloc_.release(solves_[sl].cell0, m.ncell);
     fx_alloc_.release(solves_[sl].fx0, (int64_t)(m.imax + 1) * m.jmax);
     fy_alloc_.release(solves_[sl].fy0, (int64_t)m.imax * (m.jmax + 1));
+    lh_alloc_.release(solves_[sl].lh0, solves_[sl].lhcap);
     cells_in_use_ -= m.ncell;
     used_[sl] = 0;
     while (nslot_ > 0 && !used_[nslot_ - 1]) nslot_--;

@@ -56,6 +56,9 @@ private:
 struct GroupCaps {
     int64_t cells, xfaces, yfaces;
     int slots;
+    // limited-cell pool: a solve of C cells gets ceil(lh_fraction * C) slots of subcell
+    // averages (1: one per cell, a solve can never run out)
+    double lh_fraction = 1.0;
 };
 
 class Group {
@@ -83,6 +86,7 @@ public:
     int remove(int slot);
     const MeshArgs &mesh(int slot) const { return solves_[slot].m; }
     int cell0(int slot) const { return solves_[slot].cell0; }
+    const SolveArgs &args(int slot) const { return solves_[slot]; }
     bool used(int slot) const { return slot >= 0 && slot < caps.slots && used_[slot]; }
     int nslot() const { return nslot_; }
     int64_t cells_in_use() const { return cells_in_use_; }
@@ -98,16 +102,20 @@ public:
 
     hipError_t launch(int part) const;
     int dt_grid() const { return grid[kTabCell] < kDtGrid ? grid[kTabCell] : kDtGrid; }
-    // Bytes of device memory per cell / x-face / y-face and per slot of a (PDE, N).
+    // Bytes of device memory per cell / x-face / y-face / solve slot / limited-cell pool
+    // slot of a (PDE, N).
     static void footprint(int pde, int N, int64_t *per_cell, int64_t *per_xface, int64_t *per_yface,
-                          int64_t *per_slot);
+                          int64_t *per_slot, int64_t *per_lh);
+    // Pool slots of a solve of ncell cells, and of a group, at a pool fraction.
+    static int64_t lh_slots(int pde, int64_t ncell, double fraction);
+    static int64_t lh_pool(int pde, int64_t cells, int slots, double fraction);
 
 private:
     hipStream_t st_ = nullptr;
     std::vector<void *> allocs_;
     std::vector<SolveArgs> solves_;
     std::vector<char> used_;
-    RangeAlloc cell_alloc_, fx_alloc_, fy_alloc_;
+    RangeAlloc cell_alloc_, fx_alloc_, fy_alloc_, lh_alloc_;
     int nslot_ = 0;
     int64_t cells_in_use_ = 0;
     int *prefix_dev_ = nullptr;

@@ -168,8 +168,12 @@ int retire(aderdg_server *s, int k)
         const double lat = std::chrono::duration<double>(Clock::now() - r.submitted).count();
         std::lock_guard<std::mutex> lk(s->mu);
         if (r.req.keep_result) s->results[r.ticket] = std::move(res);
-        s->done.push_back({r.ticket, lat, 0, st.steps, st.time < r.req.tend ? 0 : 1});
-        s->stats.completed++;
+        // a solve that ran out of limited-cell pool slots ends failed (ADERDG_ECAPACITY)
+        s->done.push_back({r.ticket, lat, st.failed ? ADERDG_ECAPACITY : 0, st.steps, st.time < r.req.tend ? 0 : 1});
+        if (st.failed)
+            s->stats.failed++;
+        else
+            s->stats.completed++;
         s->stats.running--;
         s->stats.dof_updates += dof;
         s->stats.bytes_in_use -= r.bytes;
@@ -254,11 +258,12 @@ int aderdg_server_create(const aderdg_server_desc *d, aderdg_server **out)
     return 0;
 }
 
-int aderdg_server_reserve(aderdg_server *s, int pde, int N, int64_t max_cells, int max_solves)
+int aderdg_server_reserve_limited(aderdg_server *s, int pde, int N, int64_t max_cells, int max_solves,
+                                  double limited_fraction)
 {
     if (!s) return fail(ADERDG_EINVAL, "aderdg_server_reserve: null server");
     if (s->started) return fail(ADERDG_ESTATE, "aderdg_server_reserve: server already started");
-    int r = s->batch.reserve(pde, N, max_cells, max_solves);
+    int r = s->batch.reserve(pde, N, max_cells, max_solves, limited_fraction);
     if (r) return r;
     for (int k = 0; k < 2; k++) {
         SolveState *p = nullptr;
@@ -268,6 +273,11 @@ int aderdg_server_reserve(aderdg_server *s, int pde, int N, int64_t max_cells, i
         s->snap[k].push_back(p);
     }
     return 0;
+}
+
+int aderdg_server_reserve(aderdg_server *s, int pde, int N, int64_t max_cells, int max_solves)
+{
+    return aderdg_server_reserve_limited(s, pde, N, max_cells, max_solves, 1.0);
 }
 
 int aderdg_server_start(aderdg_server *s)
@@ -291,7 +301,8 @@ int aderdg_server_submit(aderdg_server *s, const aderdg_request *req, int64_t *t
     Request q;
     q.req = *req;
     q.submitted = Clock::now();
-    q.bytes = Batch::footprint(req->solve.pde, req->solve.N, req->solve.imax, req->solve.jmax);
+    q.bytes = Batch::footprint(req->solve.pde, req->solve.N, req->solve.imax, req->solve.jmax,
+                               s->batch.lh_fraction(req->solve.pde, req->solve.N));
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) return fail(ADERDG_ESTATE, "aderdg_server_submit: server stopped (%d)", s->worker_error);
     q.ticket = s->next_ticket++;

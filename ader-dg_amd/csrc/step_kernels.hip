@@ -1035,6 +1035,13 @@ ADG_DEV void group_ranges(const double *uhL, const double *lobL, const double *l
     r[0] = u0; r[1] = u1; r[2] = b0; r[3] = b1; r[4] = l0; r[5] = l1;
 }
 
+// Pool slot of the subcell averages of a cell (StepBufs::lh), or -1 for a cell without
+// limiter (and for a limited cell of a solve whose pool segment ran out: that solve has failed).
+ADG_DEV int lim_slot(const StepBufs &b, int par, size_t cell)
+{
+    return b.status[par][cell] != 0 ? b.lh_slot[par][cell] : -1;
+}
+
 // DMP range test of one variable, Euler/limiter.f90:198-223 (argmax = 0): true if violated
 ADG_DEV bool dmp_violated(const double r[6], double lmn, double lmx)
 {
@@ -1212,8 +1219,9 @@ __global__ __launch_bounds__(kUpdBlock<P>) void k_update_shared(StepBufs b)
         if (!SEED && dmp_violated(r, pmn[k], pmx[k])) atomicOr(&sflag[lc], 2);
         // next step's get_min_max step 1 for a cell without limiter, Euler/limiter.f90:123-132
         double mn = f_min(f_min(r[0], r[2]), r[4]), mx = f_max(f_max(r[1], r[3]), r[5]);
-        if (SEED && b.status[par][g] != 0) {  // limited cell: range of its subcell data, :135-139
-            const double *L = b.lh[par] + g * V * SS;
+        const int ls = SEED ? lim_slot(b, par, g) : -1;
+        if (ls >= 0) {  // limited cell: range of its subcell data, :135-139
+            const double *L = b.lh[par] + (size_t)ls * V * SS;
             mn = __builtin_nan("");
             mx = mn;
             for (int kk = 0; kk < SS; kk++) {
@@ -1440,8 +1448,9 @@ __global__ __launch_bounds__(kUpdBlock<P>) void k_update(StepBufs b)
         if (!SEED && dmp_violated(r, pmn[k], pmx[k])) atomicOr(&sflag[lc], 2);
         // next step's get_min_max step 1 for a cell without limiter, Euler/limiter.f90:123-132
         double mn = f_min(f_min(r[0], r[2]), r[4]), mx = f_max(f_max(r[1], r[3]), r[5]);
-        if (SEED && b.status[par][g] != 0) {  // limited cell: range of its subcell data, :135-139
-            const double *L = b.lh[par] + g * V * SS;
+        const int ls = SEED ? lim_slot(b, par, g) : -1;
+        if (ls >= 0) {  // limited cell: range of its subcell data, :135-139
+            const double *L = b.lh[par] + (size_t)ls * V * SS;
             mn = __builtin_nan("");
             mx = mn;
             for (int kk = 0; kk < SS; kk++) {
@@ -1521,7 +1530,8 @@ __global__ __launch_bounds__(kBlock) void k_spread(StepBufs b)
 // Common tail of both subcell kernels: put_subcell_data (Euler/limiter.f90:86-96),
 // re-DMP of cells marked 2 (:619-638), next-step limiter seeds and dt candidate.
 template <class P, int N, class M>
-ADG_DEV void subcell_finish(const M &m, const StepBufs &b, size_t cell, int par, int flag, const double *sSub,
+ADG_DEV void subcell_finish(const M &m, const StepBufs &b, size_t cell, int sol, int lh0, int lhcap, int par, int flag,
+                            const double *sSub,
                             double *sUhy, double *sUh, double *sTy, double *sBy, double *sLim, double *sLob, double *sDt,
                             int *sFlag)
 {
@@ -1592,9 +1602,16 @@ ADG_DEV void subcell_finish(const M &m, const StepBufs &b, size_t cell, int par,
         b.cell_dt[cell] = d;
         b.status[par ^ 1][cell] = status;
     }
-    if (status) {  // NewLh, and Lh = NewLh at update_limiter (:287)
-        double *L = b.lh[par ^ 1] + cell * V * SS;
-        for (int k = lane; k < V * SS; k += 64) L[k] = sSub[(k % V) * SS + k / V];
+    if (status) {  // NewLh in a pool slot of the solve, and Lh = NewLh at update_limiter (:287)
+        int nth = 0;
+        if (lane == 0) nth = atomicAdd(b.lh_count + sol, 1);
+        nth = __shfl(nth, 0);
+        const int ls = nth < lhcap ? lh0 + nth : -1;  // none left: the solve ends failed after this step
+        if (lane == 0) b.lh_slot[par ^ 1][cell] = ls;
+        if (ls >= 0) {
+            double *L = b.lh[par ^ 1] + (size_t)ls * V * SS;
+            for (int k = lane; k < V * SS; k += 64) L[k] = sSub[(k % V) * SS + k / V];
+        }
     }
     __syncthreads();
 }
@@ -1644,11 +1661,11 @@ __global__ __launch_bounds__(64) void k_subcell(StepBufs b)
         nbk[4] = sa.cell0 + ci + m.imax * nb_index(cj, 1, m.jmax, m.wrap);
         int stk[5];
 #pragma unroll
-        for (int k = 0; k < 5; k++) stk[k] = b.status[par][nbk[k]];
+        for (int k = 0; k < 5; k++) stk[k] = lim_slot(b, par, nbk[k]);  // pool slot, -1: no limiter
         // subcell data of olduh for unlimited blocks, first pass (Euler/limiter.f90:25-27, :476-477)
         for (int task = lane; task < 5 * n * s; task += 64) {
             const int blk = task / (n * s), r = task - blk * n * s, a = r % n, qq = r / n;
-            if (stk[blk] != 0) continue;
+            if (stk[blk] >= 0) continue;
             const double *u = b.uh[par] + (size_t)nbk[blk] * V * NN;
 #pragma unroll
             for (int v = 0; v < V; v++) {
@@ -1676,7 +1693,7 @@ __global__ __launch_bounds__(64) void k_subcell(StepBufs b)
                 continue;
             }
             double val[V];
-            if (stk[blk] == 0) {
+            if (stk[blk] < 0) {
 #pragma unroll
                 for (int v = 0; v < V; v++) {
                     double acc = 0.0;
@@ -1685,7 +1702,7 @@ __global__ __launch_bounds__(64) void k_subcell(StepBufs b)
                     val[v] = acc;
                 }
             } else {
-                const double *L = b.lh[par] + (size_t)nbk[blk] * V * SS;
+                const double *L = b.lh[par] + (size_t)stk[blk] * V * SS;
 #pragma unroll
                 for (int v = 0; v < V; v++) val[v] = L[v + V * (p + s * qq)];
             }
@@ -1804,7 +1821,8 @@ __global__ __launch_bounds__(64) void k_subcell(StepBufs b)
         }
 #undef S0
         __syncthreads();
-        subcell_finish<P, N>(m, b, gcell, par, flag, sSub, sUhy, sUh, sTy, sBy, sLim, sLob, sDt, &sFlag);
+        subcell_finish<P, N>(m, b, gcell, slot, sa.lh0, sa.lhcap, par, flag, sSub, sUhy, sUh, sTy, sBy, sLim, sLob,
+                             sDt, &sFlag);
     }
 }
 
@@ -1857,7 +1875,7 @@ __global__ __launch_bounds__(64) void k_subcell_bn(StepBufs b)
             const int blk = task / (n * s), r = task - blk * n * s, a = r % n, qq = r / n;
             const int di = blk % 3 - 1, dj = blk / 3 - 1;
             const int nb = sa.cell0 + nb_index(ci, di, m.imax, m.wrap) + m.imax * nb_index(cj, dj, m.jmax, m.wrap);
-            if (b.status[par][nb] != 0) continue;
+            if (lim_slot(b, par, nb) >= 0) continue;
             const double *u = b.uh[par] + (size_t)nb * V * NN;
 #pragma unroll
             for (int v = 0; v < V; v++) {
@@ -1874,7 +1892,8 @@ __global__ __launch_bounds__(64) void k_subcell_bn(StepBufs b)
             const int p = ii - 1 - di * s, qq = jj - 1 - dj * s;
             const int blk = (di + 1) + 3 * (dj + 1);
             const int nb = sa.cell0 + nb_index(ci, di, m.imax, m.wrap) + m.imax * nb_index(cj, dj, m.jmax, m.wrap);
-            if (b.status[par][nb] == 0) {
+            const int ls = lim_slot(b, par, nb);
+            if (ls < 0) {
 #pragma unroll
                 for (int v = 0; v < V; v++) {
                     double acc = 0.0;
@@ -1883,7 +1902,7 @@ __global__ __launch_bounds__(64) void k_subcell_bn(StepBufs b)
                     sS0[v * GG + task] = acc;
                 }
             } else {
-                const double *L = b.lh[par] + (size_t)nb * V * SS;
+                const double *L = b.lh[par] + (size_t)ls * V * SS;
 #pragma unroll
                 for (int v = 0; v < V; v++) sS0[v * GG + task] = L[v + V * (p + s * qq)];
             }
@@ -1991,7 +2010,8 @@ __global__ __launch_bounds__(64) void k_subcell_bn(StepBufs b)
 #undef HIX
         __syncthreads();
         double *sTy = sR0, *sBy = sTy + V * n * s, *sLim = sBy + V * NN, *sLob = sLim + V * SS;
-        subcell_finish<P, N>(m, b, gcell, par, flag, sSub, sUhy, sUh, sTy, sBy, sLim, sLob, sDt, &sFlag);
+        subcell_finish<P, N>(m, b, gcell, slot, sa.lh0, sa.lhcap, par, flag, sSub, sUhy, sUh, sTy, sBy, sLim, sLob,
+                             sDt, &sFlag);
     }
 }
 
@@ -2051,14 +2071,25 @@ __global__ __launch_bounds__(kBlock) void k_dt(StepBufs b, double host_time, dou
 // Per-solve bookkeeping after a step: kPartAdvance (time = time + dt, count the
 // step, switch buffers, end the solve at its step count or end time),
 // kPartFlip (switch buffers only: update_limiter, Euler/limiter.f90:272-296),
-// kPartSeedDone (the seeds of freshly uploaded solutions are in place).
+// kPartSeedDone (the seeds of freshly uploaded solutions are in place).  All three
+// also end the step's count of limited-cell pool slots.
 __global__ __launch_bounds__(kBlock) void k_solves(StepBufs b, int part)
 {
     const int ns = b.meta[kMetaSolves];
     for (int k = blockIdx.x * kBlock + threadIdx.x; k < ns; k += gridDim.x * kBlock) {
         SolveState &st = b.state[k];
+        // limited-cell pool: the slots the step (or the seeding) took; more than the solve's
+        // segment holds ends the solve as failed (its last solution is whole, the subcell
+        // averages of some limited cells are not kept); the count restarts for the next step
+        const int taken = b.lh_count[k];
+        b.lh_count[k] = 0;
+        const bool over = taken > ld_const(b.solves + k).lhcap;
         if (part == kPartSeedDone) {
             st.seed = 0;
+            if (over) {
+                st.failed = 1;
+                st.active = 0;
+            }
             continue;
         }
         if (!st.active) continue;
@@ -2067,6 +2098,10 @@ __global__ __launch_bounds__(kBlock) void k_solves(StepBufs b, int part)
             st.time = st.time + st.dt;
             st.steps += 1;
             if (st.steps >= st.max_steps || (st.stop_at_tend && !(st.time < st.tend))) st.active = 0;
+        }
+        if (over) {
+            st.failed = 1;
+            st.active = 0;
         }
     }
 }
@@ -2153,7 +2188,14 @@ __global__ __launch_bounds__(64) void k_seed_ic(StepBufs b, IcSpec ic, int slot,
         }
         __syncthreads();
         if (sFail) {
-            if (lane == 0) b.status[par][g] = 1;
+            int nth = 0;
+            if (lane == 0) nth = atomicAdd(b.lh_count + slot, 1);
+            nth = __shfl(nth, 0);
+            const int ls = nth < sa.lhcap ? sa.lh0 + nth : -1;  // none left: kPartSeedDone fails the solve
+            if (lane == 0) {
+                b.status[par][g] = 1;
+                b.lh_slot[par][g] = ls;
+            }
             const int ci = cell % m.imax, cj = cell / m.imax;
             const double cx = (m.xl[0] + ((double)ci + 0.5) * m.dx) - 0.5 * m.dx;
             const double cy = (m.xl[1] + ((double)cj + 0.5) * m.dy) - 0.5 * m.dy;
@@ -2164,7 +2206,8 @@ __global__ __launch_bounds__(64) void k_seed_ic(StepBufs b, IcSpec ic, int slot,
                 double Q[V];
                 ic_field(ic, m.gamma, m.eos, cx + bx * m.dx, cy + by * m.dy, Q);
 #pragma unroll
-                for (int v = 0; v < V; v++) b.lh[par][g * V * SS + v + V * k] = Q[v];
+                for (int v = 0; v < V; v++)
+                    if (ls >= 0) b.lh[par][(size_t)ls * V * SS + v + V * k] = Q[v];
             }
         }
         __syncthreads();

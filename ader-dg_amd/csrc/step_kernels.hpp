@@ -50,6 +50,7 @@ struct SolveArgs {
     MeshArgs m;
     int cell0;      // first cell in the group's cell arrays
     int fx0, fy0;   // first x-face ((IMAX+1)*JMAX of them) and y-face (IMAX*(JMAX+1))
+    int lh0, lhcap; // the solve's segment of the limited-cell pool: first slot, slots
     int pad;
 };
 
@@ -62,7 +63,8 @@ struct SolveState {
     int seed;                  // 1 until the limiter seeds of a fresh solution are computed
     int steps, max_steps;      // steps taken, and the count that ends the solve
     int stop_at_tend;          // 1: the solve also ends once time >= tend
-    int pad[2];
+    int failed;                // 1: a step limited more cells than the solve's pool segment holds
+    int pad;
 };
 
 // Device buffers of a group.  Cell arrays are strided by G (cell capacity), face
@@ -82,7 +84,13 @@ struct StepBufs {
     double *lmin_old, *lmax_old, *lmin, *lmax, *cell_dt;  // [v][cell], [cell]
     int *rc, *rc_final, *list, *counters;
     int *status[2];
-    double *lh[2];              // [cell][q][p][v] subcell averages of limited cells
+    // The limiter's subcell averages (the reference's Lh(V,s,s) of tLimiter, allocated per
+    // limited cell) in a pool of slots: lh[par] + slot * V * s * s, slot = lh_slot[par][cell]
+    // for a cell with status[par][cell] = 1.  A solve owns the slots [lh0, lh0 + lhcap); a
+    // step takes them in the order its limited cells finish (lh_count[solve] counts them).
+    double *lh[2];              // [slot][q][p][v]
+    int *lh_slot[2];            // [cell]; -1: no slot was left (the solve has failed)
+    int *lh_count;              // [solve slot] pool slots taken since the last step ended
 };
 
 // Outputs of the standalone predictor (reference layouts per cell; shi only for Baer-Nunziato).
@@ -126,6 +134,7 @@ hipError_t launch_tables(const StepBufs &b, const int *prefix, int nslot, const 
                          hipStream_t st);
 // Troubled-cell seeding of solve `slot` from its current solution and the formula it
 // was projected from (the end of initialize): status and Lh of the current parity.
+// Seeded cells take pool slots as a step's limited cells do (kPartSeedDone ends the count).
 hipError_t launch_seed_ic(int pde, int N, const StepBufs &b, const IcSpec &ic, int slot, int ncell, hipStream_t st);
 // Initial state of solve `slot` into uh[parity] from a formula (ic_spec.hpp).
 hipError_t launch_init(int pde, int N, const StepBufs &b, const IcSpec &ic, int slot, int ncell, int parity,

@@ -291,8 +291,20 @@ int aderdg_batch_destroy(aderdg_batch *b);
 /* Arena for up to max_cells cells in up to max_solves solves of one (pde, N).
  * Face capacity is sized for solves of at least 8 cells per side. */
 int aderdg_batch_reserve(aderdg_batch *b, int pde, int N, int64_t max_cells, int max_solves);
+/* The same with a smaller limited-cell pool.  The reference allocates the subcell
+ * averages Lh(nVar,2N+1,2N+1) per limited cell (Euler/limiter.f90:302-322,
+ * Baer-Nunziato/limiter.f90:301-321, :575) and frees them when the cell is no
+ * longer limited (Baer-Nunziato/limiter.f90:271-295); an arena keeps them in a pool of slots, and a
+ * solve of C cells owns ceil(limited_fraction * C) of them (aderdg_batch_reserve:
+ * limited_fraction = 1, one per cell).  A step that limits more cells than its
+ * solve owns ends that solve: its solution is that of the step, and
+ * aderdg_batch_error reports ADERDG_ECAPACITY.  0 < limited_fraction <= 1. */
+int aderdg_batch_reserve_limited(aderdg_batch *b, int pde, int N, int64_t max_cells, int max_solves,
+                                 double limited_fraction);
 /* Device bytes a solve of this shape occupies in an arena (admission by HBM footprint). */
 int64_t aderdg_solve_footprint(int pde, int N, int imax, int jmax);
+/* ... in an arena reserved with aderdg_batch_reserve_limited. */
+int64_t aderdg_solve_footprint_limited(int pde, int N, int imax, int jmax, double limited_fraction);
 /* Device bytes reserved by all arenas of the batch. */
 int64_t aderdg_batch_reserved_bytes(aderdg_batch *b);
 /* Add a solve: descriptor (ops must be NULL: built-in operators), initial
@@ -317,6 +329,9 @@ int aderdg_batch_finished(aderdg_batch *b, int *ids, int max_ids, int *count);
 int aderdg_batch_download(aderdg_batch *b, int id, double *uh);
 /* Limiter status and recompute flags of a solve's last step (synchronous). */
 int aderdg_batch_limiter(aderdg_batch *b, int id, int32_t *status, int32_t *recompute);
+/* *error = 0, or ADERDG_ECAPACITY when the solve stopped because it limited more
+ * cells than its share of the limited-cell pool (synchronous). */
+int aderdg_batch_error(aderdg_batch *b, int id, int *error);
 /* Free a solve's slot and ranges. */
 int aderdg_batch_remove(aderdg_batch *b, int id);
 /* Solves placed and cells in use over all arenas. */
@@ -358,14 +373,18 @@ typedef struct {
 
 typedef struct {
     int64_t submitted, admitted, completed, failed, queued, running;
-    int64_t dof_updates;  /* nVar (N+1)^2 cells x steps of completed requests     */
+    int64_t dof_updates;  /* nVar (N+1)^2 cells x steps of finished requests      */
     int64_t bytes_in_use, peak_bytes;
     double busy_seconds;  /* worker time with solves on the device                */
 } aderdg_server_stats;
 
 int aderdg_server_create(const aderdg_server_desc *desc, aderdg_server **out);
-/* Arenas, before aderdg_server_start (aderdg_batch_reserve). */
+/* Arenas, before aderdg_server_start (aderdg_batch_reserve, aderdg_batch_reserve_limited).
+ * A request whose solve runs out of limited-cell pool slots finishes with status
+ * ADERDG_ECAPACITY. */
 int aderdg_server_reserve(aderdg_server *s, int pde, int N, int64_t max_cells, int max_solves);
+int aderdg_server_reserve_limited(aderdg_server *s, int pde, int N, int64_t max_cells, int max_solves,
+                                  double limited_fraction);
 int aderdg_server_start(aderdg_server *s);
 /* Queue a request; *ticket identifies it in poll and result. */
 int aderdg_server_submit(aderdg_server *s, const aderdg_request *req, int64_t *ticket);

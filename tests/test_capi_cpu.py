@@ -97,6 +97,27 @@ def test_solve_footprint_host():
             assert floor < per_cell < 1.25 * floor + 512, (pde, N, per_cell, floor)
 
 
+def test_solve_footprint_limited_pool_host():
+    """A limited-cell pool of fraction f holds ceil(f * cells) subcell blocks (Lh and
+    NewLh, nVar (2N+1)^2 doubles each) per solve in place of one per cell; f = 1 is the
+    plain footprint; f outside (0, 1] is refused."""
+    import math
+    f = aderdg.solve_footprint
+    for pde, V in (("euler", 4), ("bn", 9)):
+        for N in (1, 3):
+            s = 2 * N + 1
+            I, J = 37, 23
+            full = f(pde, N, I, J)
+            assert f(pde, N, I, J, limited_fraction=1.0) == full
+            for frac in (0.5, 0.1, 1e-9):
+                saved = (I * J - math.ceil(frac * I * J)) * 2 * 8 * V * s * s
+                assert f(pde, N, I, J, limited_fraction=frac) == full - saved, (pde, N, frac)
+            assert f(pde, N, I, J, limited_fraction=0.0) == -1
+            assert f(pde, N, I, J, limited_fraction=1.5) == -1
+    # advection-diffusion keeps no subcell data
+    assert f("ad", 3, 40, 40, limited_fraction=0.1) == f("ad", 3, 40, 40)
+
+
 def test_formula_ic_host_matches_python():
     """aderdg_ic_eval (the formula library the device also evaluates) against the
     Python initial conditions: bitwise where no transcendental function enters."""
