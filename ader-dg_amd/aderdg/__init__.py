@@ -21,7 +21,27 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBDIR = os.path.join(os.path.dirname(HERE), "lib")
 
-BC = {"dmr": 0, "periodic": 1, "transmissive": 2}
+BC = {"dmr": 0, "periodic": 1, "transmissive": 2, "sides": 3}
+# per-side boundary codes of ADERDG_BC_SIDES, the reference's params.ini convention
+BC_CODE = {"inflow": 0, "transmissive": 1, "reflective": 2, "periodic": 3}
+
+
+class Sides:
+    """Boundary code per side (left, right, bottom, top): 0 inflow, 1 transmissive,
+    2 reflective, 3 periodic (in pairs), as bL, bR, bB, bT of Euler/params.ini:8-11;
+    inflow: the conserved ghost state of inflow sides."""
+
+    def __init__(self, left, right, bottom, top, inflow=None):
+        codes = [BC_CODE[c] if isinstance(c, str) else int(c) for c in (left, right, bottom, top)]
+        self.codes = tuple(codes)
+        self.inflow = None if inflow is None else tuple(float(x) for x in inflow)
+
+    def __repr__(self):
+        return "Sides%r" % (self.codes + ((self.inflow,) if self.inflow else ()))
+
+
+def sides(left, right, bottom, top, inflow=None):
+    return Sides(left, right, bottom, top, inflow)
 PDE = {"euler": 0, "bn": 1, "ad": 2}
 NVAR = {0: 4, 1: 9, 2: 1}
 # kernels timed by aderdg_timing_get, in its slot order
@@ -45,7 +65,8 @@ class _Desc(ctypes.Structure):
                 ("gamma", ctypes.c_double), ("cfl", ctypes.c_double), ("exact", ctypes.c_int),
                 ("device", ctypes.c_int), ("ops", _Ops), ("bn_eos", ctypes.c_double * 4),
                 ("ad_c", ctypes.c_double * 2), ("ad_a", ctypes.c_double), ("ad_bc", ctypes.c_int * 4),
-                ("ad_fixed_predictor", ctypes.c_int), ("ad_pad", ctypes.c_int)]
+                ("ad_fixed_predictor", ctypes.c_int), ("ad_pad", ctypes.c_int), ("bc", ctypes.c_int * 4),
+                ("bc_inflow", ctypes.c_double * 9)]
 
 
 class _BatchDesc(ctypes.Structure):
@@ -153,7 +174,14 @@ def _desc(pde, N, imax, jmax, xl, xr, bc, gamma, cfl, device, eos, ad=None):
         d.ad_fixed_predictor = int(ad.get("fixed", 0))
     d.xl[0], d.xl[1] = xl
     d.xr[0], d.xr[1] = xr
-    d.bc_mode = BC[bc] if isinstance(bc, str) else int(bc)
+    if isinstance(bc, Sides):
+        d.bc_mode = BC["sides"]
+        for k in range(4):
+            d.bc[k] = bc.codes[k]
+        for k, x in enumerate(bc.inflow or ()):
+            d.bc_inflow[k] = x
+    else:
+        d.bc_mode = BC[bc] if isinstance(bc, str) else int(bc)
     d.gamma, d.cfl, d.exact, d.device = gamma, cfl, 0, device
     return d
 

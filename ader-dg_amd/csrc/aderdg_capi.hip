@@ -75,7 +75,22 @@ int check_desc(const aderdg_desc *d, const char *who)
     if (d->N < 1 || d->N > kMaxDegree) return fail(ADERDG_EINVAL, "%s: N=%d outside 1..%d", who, d->N, kMaxDegree);
     if (d->imax < 1 || d->jmax < 1 || (long long)d->imax * d->jmax > (1LL << 28))
         return fail(ADERDG_EINVAL, "%s: bad mesh %d x %d", who, d->imax, d->jmax);
-    if (d->bc_mode < 0 || d->bc_mode > 2) return fail(ADERDG_EINVAL, "%s: bad bc_mode %d", who, d->bc_mode);
+    if (d->bc_mode < 0 || d->bc_mode > 3) return fail(ADERDG_EINVAL, "%s: bad bc_mode %d", who, d->bc_mode);
+    if (d->pde != ADERDG_PDE_AD && d->bc_mode == ADERDG_BC_SIDES) {
+        for (int k = 0; k < 4; k++)
+            if (d->bc[k] < ADERDG_BC_INFLOW || d->bc[k] > ADERDG_BC_WRAP)
+                return fail(ADERDG_EINVAL, "%s: boundary code %d of side %d outside 0..3", who, d->bc[k], k);
+        if ((d->bc[0] == ADERDG_BC_WRAP) != (d->bc[1] == ADERDG_BC_WRAP) ||
+            (d->bc[2] == ADERDG_BC_WRAP) != (d->bc[3] == ADERDG_BC_WRAP))
+            return fail(ADERDG_EINVAL, "%s: periodic boundaries come in pairs (codes %d %d %d %d)", who, d->bc[0],
+                        d->bc[1], d->bc[2], d->bc[3]);
+        const int V = d->pde == ADERDG_PDE_BN ? 9 : 4;
+        for (int k = 0; k < 4; k++)
+            if (d->bc[k] == ADERDG_BC_INFLOW)
+                for (int v = 0; v < V; v++)
+                    if (!std::isfinite(d->bc_inflow[v]))
+                        return fail(ADERDG_EINVAL, "%s: inflow state component %d is not finite", who, v);
+    }
     if (!(d->xr[0] > d->xl[0]) || !(d->xr[1] > d->xl[1])) return fail(ADERDG_EINVAL, "%s: empty domain", who);
     if (d->pde == ADERDG_PDE_EULER && !(d->gamma > 1.0)) return fail(ADERDG_EINVAL, "%s: gamma must exceed 1", who);
     if (!(d->cfl > 0.0)) return fail(ADERDG_EINVAL, "%s: cfl must be positive", who);
@@ -92,7 +107,13 @@ MeshArgs mesh_from_desc(const aderdg_desc *d)
     m.jmax = d->jmax;
     m.ncell = d->imax * d->jmax;
     m.bc_mode = d->bc_mode;
-    m.wrap = d->bc_mode == ADERDG_BC_PERIODIC;
+    for (int k = 0; k < 4; k++)
+        m.side[k] = d->bc_mode == ADERDG_BC_SIDES      ? d->bc[k]
+                    : d->bc_mode == ADERDG_BC_PERIODIC ? kSideWrap
+                                                       : kSideTransmit;
+    m.wrap = (m.side[0] == kSideWrap ? 1 : 0) | (m.side[2] == kSideWrap ? 2 : 0);
+    if (d->bc_mode == ADERDG_BC_DMR) m.wrap = 0;
+    for (int v = 0; v < 9; v++) m.qin[v] = d->bc_inflow[v];
     m.xl[0] = d->xl[0];
     m.xl[1] = d->xl[1];
     m.adv[0] = d->ad_c[0];

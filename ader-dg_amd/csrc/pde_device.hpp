@@ -101,6 +101,8 @@ struct AdPDE {
 struct EulerPDE {
     static constexpr int V = 4;
     static constexpr bool kNcp = false;
+    // x- and y-momentum components (a reflective side negates the normal one)
+    static constexpr unsigned kMomX = 1u << 1, kMomY = 1u << 2;
 
     // PDEFlux, Euler/pde.f90:46-69.  F[0..3] x-flux, F[4..7] y-flux.
     template <class M>
@@ -167,6 +169,8 @@ struct EulerPDE {
 struct BnPDE {
     static constexpr int V = 9;
     static constexpr bool kNcp = true;
+    // x- and y-momentum components of both phases
+    static constexpr unsigned kMomX = (1u << 1) | (1u << 5), kMomY = (1u << 2) | (1u << 6);
 
     // PDEFlux, Baer-Nunziato/pde.f90:74-119.  F[0..8] x-flux, F[9..17] y-flux.
     template <class M>
@@ -337,5 +341,59 @@ struct BnPDE {
         src[8] = -un;
     }
 };
+
+// Ghost face state and normal flux of a non-periodic boundary side (codes of
+// MeshArgs::side): inflow takes m.qin and its flux along (nx, ny) (the DMR inflow
+// arithmetic, Euler/riemann_problem.f90:186-193); transmissive the inside state
+// (Baer-Nunziato/riemann_problem.f90:26-34); reflective the inside state with the
+// normal momentum of every phase negated and every other flux component negated
+// (the DMR wall, Euler/riemann_problem.f90:225-228).
+template <class P, class M>
+ADG_DEV void side_ghost(const M &m, int code, double nx, double ny, const double *Qi, const double *Fi, double *Q,
+                        double *F)
+{
+    constexpr int V = P::V;
+    if (code == kSideInflow) {
+        double Fbc[2 * V];
+#pragma unroll
+        for (int v = 0; v < V; v++) Q[v] = m.qin[v];
+        P::flux(m, Q, Fbc);
+#pragma unroll
+        for (int v = 0; v < V; v++) F[v] = 0.0 + Fbc[v] * nx + Fbc[V + v] * ny;
+        return;
+    }
+    const unsigned mom = code == kSideReflect ? (nx != 0.0 ? P::kMomX : P::kMomY) : 0u;
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        const bool normal = (mom >> v) & 1u;
+        Q[v] = normal ? -Qi[v] : Qi[v];
+        F[v] = (mom && !normal) ? -Fi[v] : Fi[v];
+    }
+}
+
+// A ghost subcell of the subcell recompute outside side gx (-1 left, +1 right, 0
+// none) and gy (-1 bottom, +1 top), taken from the clamped or wrapped neighbour
+// table: inflow replaces it by m.qin, reflective negates its normal momentum; x
+// sides first (the DMR ghosts, Euler/limiter.f90:491-522).  Variable v of the
+// subcell is val[v * stride].
+template <class P, class M>
+ADG_DEV void side_subcell(const M &m, int gx, int gy, double *val, int stride = 1)
+{
+    constexpr int V = P::V;
+    for (int a = 0; a < 2; a++) {
+        const int g = a == 0 ? gx : gy;
+        if (g == 0) continue;
+        const int code = m.side[a == 0 ? (g < 0 ? 0 : 1) : (g < 0 ? 2 : 3)];
+        const unsigned mom = a == 0 ? P::kMomX : P::kMomY;
+        if (code == kSideInflow) {
+#pragma unroll
+            for (int v = 0; v < V; v++) val[v * stride] = m.qin[v];
+        } else if (code == kSideReflect) {
+#pragma unroll
+            for (int v = 0; v < V; v++)
+                if ((mom >> v) & 1u) val[v * stride] = -val[v * stride];
+        }
+    }
+}
 
 }  // namespace aderdg

@@ -155,7 +155,7 @@ ADG_DEV void predictor_finish(const M &m, const StepBufs &b, const TestBufs &t, 
         double mn = b.lmin_old[row + cell], mx = b.lmax_old[row + cell];
         for (int jj = -1; jj <= 1; jj++)
             for (int ii = -1; ii <= 1; ii++) {
-                const int nb = nb_index(ci, ii, m.imax, m.wrap) + m.imax * nb_index(cj, jj, m.jmax, m.wrap);
+                const int nb = nb_index(ci, ii, m.imax, m.wrap & 1) + m.imax * nb_index(cj, jj, m.jmax, m.wrap >> 1);
                 mn = f_min(mn, b.lmin_old[row + nb]);
                 mx = f_max(mx, b.lmax_old[row + nb]);
             }
@@ -881,10 +881,10 @@ __global__ __launch_bounds__(kBlock) void k_face(StepBufs b)
             if (fi > 0) ld(1, (fi - 1) + (long)I * j, QL, FL);
             if (fi < I) ld(0, fi + (long)I * j, QR, FR);
             if (fi == 0) {
-                if (m.bc_mode == kBcPeriodic) {
+                if (!dmr && m.side[0] == kSideWrap) {
                     ld(1, (I - 1) + (long)I * j, QL, FL);
                 } else if (!dmr) {
-                    copy(QR, FR, QL, FL);
+                    side_ghost<P>(m, m.side[0], 1.0, 0.0, QR, FR, QL, FL);
                 } else {  // DMR inflow, Euler/riemann_problem.f90:186-193
                     double Qbc[V], Fbc[2 * V];
 #pragma unroll
@@ -898,10 +898,12 @@ __global__ __launch_bounds__(kBlock) void k_face(StepBufs b)
                 }
             }
             if (fi == I) {
-                if (m.bc_mode == kBcPeriodic)
+                if (!dmr && m.side[1] == kSideWrap)
                     ld(0, (long)I * j, QR, FR);
+                else if (!dmr)
+                    side_ghost<P>(m, m.side[1], 1.0, 0.0, QL, FL, QR, FR);
                 else
-                    copy(QL, FL, QR, FR);  // transmissive, also the DMR right boundary (:195-198)
+                    copy(QL, FL, QR, FR);  // the DMR right boundary is transmissive (:195-198)
             }
             P::face_flux(m, QL, FL, QR, FR, 1.0, 0.0, Dm, Dp);
         } else {
@@ -909,8 +911,10 @@ __global__ __launch_bounds__(kBlock) void k_face(StepBufs b)
             if (fj > 0) ld(3, i + (long)I * (fj - 1), QL, FL);
             if (fj < J) ld(2, i + (long)I * fj, QR, FR);
             if (fj == 0) {
-                if (m.bc_mode == kBcPeriodic) {
+                if (!dmr && m.side[2] == kSideWrap) {
                     ld(3, i + (long)I * (J - 1), QL, FL);
+                } else if (!dmr) {
+                    side_ghost<P>(m, m.side[2], 0.0, 1.0, QR, FR, QL, FL);
                 } else {
                     copy(QR, FR, QL, FL);
                     if (dmr && !((double)(i + 1) * m.dx <= 1.0 / 6.0)) {  // reflective, :225-228
@@ -922,10 +926,10 @@ __global__ __launch_bounds__(kBlock) void k_face(StepBufs b)
                 }
             }
             if (fj == J) {
-                if (m.bc_mode == kBcPeriodic) {
+                if (!dmr && m.side[3] == kSideWrap) {
                     ld(2, i, QR, FR);
                 } else if (!dmr) {
-                    copy(QL, FL, QR, FR);
+                    side_ghost<P>(m, m.side[3], 0.0, 1.0, QL, FL, QR, FR);
                 } else {  // DMR top: exact shock position, :233-245
                     const bool pre = (double)(i + 1) * m.dx <= dmr_shock_x(m, time);
                     double Qbc[V], Fbc[2 * V];
@@ -1496,10 +1500,10 @@ __global__ __launch_bounds__(kBlock) void k_spread(StepBufs b)
                 // a face neighbour marked 1 marks this cell 2 (Euler/limiter.f90:336-350; the
                 // face-neighbour relation of the clamped/wrapped table is symmetric)
                 const int ci = cell % m.imax, cj = cell / m.imax;
-                const int nbs[4] = {nb_index(ci, -1, m.imax, m.wrap) + m.imax * cj,
-                                    nb_index(ci, 1, m.imax, m.wrap) + m.imax * cj,
-                                    ci + m.imax * nb_index(cj, -1, m.jmax, m.wrap),
-                                    ci + m.imax * nb_index(cj, 1, m.jmax, m.wrap)};
+                const int nbs[4] = {nb_index(ci, -1, m.imax, m.wrap & 1) + m.imax * cj,
+                                    nb_index(ci, 1, m.imax, m.wrap & 1) + m.imax * cj,
+                                    ci + m.imax * nb_index(cj, -1, m.jmax, m.wrap >> 1),
+                                    ci + m.imax * nb_index(cj, 1, m.jmax, m.wrap >> 1)};
 #pragma unroll
                 for (int f = 0; f < 4; f++)
                     if (nbs[f] != cell && b.rc[sa.cell0 + nbs[f]] == 1) flag = 2;
@@ -1655,10 +1659,10 @@ __global__ __launch_bounds__(64) void k_subcell(StepBufs b)
         const int ci = cell % m.imax, cj = cell / m.imax;
         int nbk[5];  // cells of the group arrays
         nbk[0] = gcell;
-        nbk[1] = sa.cell0 + nb_index(ci, -1, m.imax, m.wrap) + m.imax * cj;
-        nbk[2] = sa.cell0 + nb_index(ci, 1, m.imax, m.wrap) + m.imax * cj;
-        nbk[3] = sa.cell0 + ci + m.imax * nb_index(cj, -1, m.jmax, m.wrap);
-        nbk[4] = sa.cell0 + ci + m.imax * nb_index(cj, 1, m.jmax, m.wrap);
+        nbk[1] = sa.cell0 + nb_index(ci, -1, m.imax, m.wrap & 1) + m.imax * cj;
+        nbk[2] = sa.cell0 + nb_index(ci, 1, m.imax, m.wrap & 1) + m.imax * cj;
+        nbk[3] = sa.cell0 + ci + m.imax * nb_index(cj, -1, m.jmax, m.wrap >> 1);
+        nbk[4] = sa.cell0 + ci + m.imax * nb_index(cj, 1, m.jmax, m.wrap >> 1);
         int stk[5];
 #pragma unroll
         for (int k = 0; k < 5; k++) stk[k] = lim_slot(b, par, nbk[k]);  // pool slot, -1: no limiter
@@ -1716,6 +1720,10 @@ __global__ __launch_bounds__(64) void k_subcell(StepBufs b)
 #pragma unroll
                     for (int v = 0; v < V; v++) val[v] = pre ? m.qa[v] : m.qb[v];
                 }
+            } else {  // the side codes (inflow / reflective) on ghosts outside the domain
+                const int gx = (ci == 0 && ii == 0) ? -1 : ((ci == m.imax - 1 && ii == s + 1) ? 1 : 0);
+                const int gy = (cj == 0 && jj == 0) ? -1 : ((cj == m.jmax - 1 && jj == s + 1) ? 1 : 0);
+                side_subcell<P>(m, gx, gy, val);
             }
 #pragma unroll
             for (int v = 0; v < V; v++) sS0[v * HH + task] = val[v];
@@ -1874,7 +1882,7 @@ __global__ __launch_bounds__(64) void k_subcell_bn(StepBufs b)
         for (int task = lane; task < 9 * n * s; task += 64) {
             const int blk = task / (n * s), r = task - blk * n * s, a = r % n, qq = r / n;
             const int di = blk % 3 - 1, dj = blk / 3 - 1;
-            const int nb = sa.cell0 + nb_index(ci, di, m.imax, m.wrap) + m.imax * nb_index(cj, dj, m.jmax, m.wrap);
+            const int nb = sa.cell0 + nb_index(ci, di, m.imax, m.wrap & 1) + m.imax * nb_index(cj, dj, m.jmax, m.wrap >> 1);
             if (lim_slot(b, par, nb) >= 0) continue;
             const double *u = b.uh[par] + (size_t)nb * V * NN;
 #pragma unroll
@@ -1891,7 +1899,7 @@ __global__ __launch_bounds__(64) void k_subcell_bn(StepBufs b)
             const int di = ii < 1 ? -1 : (ii > s ? 1 : 0), dj = jj < 1 ? -1 : (jj > s ? 1 : 0);
             const int p = ii - 1 - di * s, qq = jj - 1 - dj * s;
             const int blk = (di + 1) + 3 * (dj + 1);
-            const int nb = sa.cell0 + nb_index(ci, di, m.imax, m.wrap) + m.imax * nb_index(cj, dj, m.jmax, m.wrap);
+            const int nb = sa.cell0 + nb_index(ci, di, m.imax, m.wrap & 1) + m.imax * nb_index(cj, dj, m.jmax, m.wrap >> 1);
             const int ls = lim_slot(b, par, nb);
             if (ls < 0) {
 #pragma unroll
@@ -1906,6 +1914,10 @@ __global__ __launch_bounds__(64) void k_subcell_bn(StepBufs b)
 #pragma unroll
                 for (int v = 0; v < V; v++) sS0[v * GG + task] = L[v + V * (p + s * qq)];
             }
+            // the side codes (inflow / reflective) on ghosts outside the domain
+            const int gx = (ci == 0 && di < 0) ? -1 : ((ci == m.imax - 1 && di > 0) ? 1 : 0);
+            const int gy = (cj == 0 && dj < 0) ? -1 : ((cj == m.jmax - 1 && dj > 0) ? 1 : 0);
+            if (gx || gy) side_subcell<P>(m, gx, gy, sS0 + task, GG);
         }
         __syncthreads();
 #define S0(v, ii, jj) sS0[(v) * GG + ((ii) + 1) + G * ((jj) + 1)]

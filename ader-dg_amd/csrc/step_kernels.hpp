@@ -22,7 +22,9 @@ constexpr int kDtGrid = 256;       // workgroups of the time-step reduction
 constexpr int kSubcellGrid = 4096; // persistent workgroups of the subcell recompute
 
 enum { kPdeEuler = 0, kPdeBn = 1, kPdeAd = 2 };
-enum { kBcDmr = 0, kBcPeriodic = 1, kBcTransmissive = 2 };
+enum { kBcDmr = 0, kBcPeriodic = 1, kBcTransmissive = 2, kBcSides = 3 };
+// boundary code of one side (the reference's params.ini convention, Euler/params.ini:8-11)
+enum { kSideInflow = 0, kSideTransmit = 1, kSideReflect = 2, kSideWrap = 3 };
 enum { kCntList = 0, kCntTroubled = 1, kCntDtBlocks = 2, kNumCounters = 4 };
 enum { kPartDt = 0, kPartPredictor = 1, kPartUpdate = 2, kPartSpread = 3, kPartSubcell = 4, kPartAdvance = 5,
        kPartSeed = 6, kPartFace = 7, kPartFlip = 8, kPartSeedDone = 9 };
@@ -33,9 +35,11 @@ enum { kMetaSolves = kNumTabs, kNumMeta = kNumTabs + 1 };
 
 // Mesh and equation constants of one solve.
 struct MeshArgs {
-    int pde, imax, jmax, ncell, bc_mode, wrap;
+    int pde, imax, jmax, ncell, bc_mode;
+    int wrap;                  // neighbour table: bit 0 wraps in x, bit 1 in y (periodic sides)
     int ad_fixed;              // advection-diffusion: predictor flux (0 viscous only, 1 fused)
     int ad_bc[4];              // advection-diffusion bC(1..4): 1 Neumann, 2 Dirichlet, 3 periodic
+    int side[4];               // Euler / Baer-Nunziato: left, right, bottom, top code (kSide*); DMR: unused
     int pad;
     double dx, dy, gamma, cflpnpm;
     double adv[2], diff;       // advection-diffusion EQN%cx, EQN%cy, EQN%a
@@ -43,6 +47,7 @@ struct MeshArgs {
     double eos[4];             // Baer-Nunziato: GAMMA_S, GAMMA_G, PI_S, PI_G
     double dmr_a, dmr_b;       // DMR shock position x_sw = dmr_a + dmr_b * t (Euler/riemann_problem.f90:1-8)
     double qa[4], qb[4];       // DMR inflow / pre-shock conserved states
+    double qin[9];             // conserved ghost state of kSideInflow sides
 };
 
 // Static description of one solve of a group.

@@ -14,7 +14,8 @@ module aderdg_hip
     implicit none
 
     integer(c_int), parameter :: ADERDG_PDE_EULER = 0, ADERDG_PDE_BN = 1, ADERDG_PDE_AD = 2
-    integer(c_int), parameter :: ADERDG_BC_DMR = 0, ADERDG_BC_PERIODIC = 1, ADERDG_BC_TRANSMISSIVE = 2
+    integer(c_int), parameter :: ADERDG_BC_DMR = 0, ADERDG_BC_PERIODIC = 1, ADERDG_BC_TRANSMISSIVE = 2, &
+                                 ADERDG_BC_SIDES = 3
 
     type, bind(C) :: aderdg_ops_t
         type(c_ptr) :: wGPN = c_null_ptr, Kxi = c_null_ptr, iK1 = c_null_ptr, F0 = c_null_ptr
@@ -34,6 +35,8 @@ module aderdg_hip
         real(c_double) :: ad_c(2) = 0.0d0, ad_a = 0.0d0                   ! EQN%cx, EQN%cy, EQN%a
         integer(c_int) :: ad_bc(4) = 3                                     ! bC(1..4)
         integer(c_int) :: ad_fixed_predictor = 0, ad_pad = 0
+        integer(c_int) :: bc(4) = 1                                        ! bL, bR, bB, bT with ADERDG_BC_SIDES
+        real(c_double) :: bc_inflow(9) = 0.0d0                             ! conserved inflow state
     end type
 
     ! formula initial conditions (aderdg_ic of include/aderdg.h)

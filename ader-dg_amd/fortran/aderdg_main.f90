@@ -16,8 +16,9 @@
 !
 ! ICType: 4 isentropic vortex, 5 Lax-Liu Riemann problem 3, 11 Double Mach
 ! Reflection (Euler/params.ini:17-85).  Boundaries: ICType 11 uses the DMR
-! boundaries of the reference; otherwise code 3 on all sides is periodic and
-! anything else transmissive.
+! boundaries of the reference; otherwise the codes bL, bR, bB, bT apply per side:
+! 1 transmissive, 2 reflective, 3 periodic (in pairs).  Inflow (0) needs a state,
+! which the reference defines for the DMR only.
 
 program aderdg_euler
     use iso_c_binding
@@ -88,10 +89,13 @@ program aderdg_euler
     desc%cfl = 0.9d0
     if (ICType == 11) then
         desc%bc_mode = ADERDG_BC_DMR
-    else if (bL == 3 .and. bR == 3 .and. bB == 3 .and. bT == 3) then
-        desc%bc_mode = ADERDG_BC_PERIODIC
     else
-        desc%bc_mode = ADERDG_BC_TRANSMISSIVE
+        if (min(bL, bR, bB, bT) < 1 .or. max(bL, bR, bB, bT) > 3) then
+            print *, 'ERROR: boundary codes ', bL, bR, bB, bT, ': 1, 2 or 3 outside the DMR'
+            stop 1
+        end if
+        desc%bc_mode = ADERDG_BC_SIDES
+        desc%bc = (/ bL, bR, bB, bT /)
     end if
 
     allocate(uh(4, N+1, N+1, IMAX, JMAX), wGPN(N+1))

@@ -46,7 +46,23 @@ enum aderdg_pde { ADERDG_PDE_EULER = 0, ADERDG_PDE_BN = 1, ADERDG_PDE_AD = 2 };
  * TRANSMISSIVE copies the interior state into the ghost face state
  * (Euler/riemann_problem.f90:50-53); the limiter neighbour table clamps to the
  * cell itself (Euler/initialize.f90:231-237). */
-enum aderdg_bc { ADERDG_BC_DMR = 0, ADERDG_BC_PERIODIC = 1, ADERDG_BC_TRANSMISSIVE = 2 };
+/* Boundaries of the Euler and Baer-Nunziato systems.  DMR: the Double Mach
+ * Reflection set of Euler/riemann_problem.f90:10-116 (Euler only).  PERIODIC /
+ * TRANSMISSIVE: every side.  SIDES: a code per side in aderdg_desc.bc, the
+ * reference's params.ini convention (Euler/params.ini:8-11): */
+enum aderdg_bc { ADERDG_BC_DMR = 0, ADERDG_BC_PERIODIC = 1, ADERDG_BC_TRANSMISSIVE = 2, ADERDG_BC_SIDES = 3 };
+/*   0 inflow: the ghost state is aderdg_desc.bc_inflow, its flux the state's flux
+ *     (the DMR inflow, Euler/riemann_problem.f90:186-193);
+ *   1 transmissive: the ghost is the inside face state and flux
+ *     (Baer-Nunziato/riemann_problem.f90:26-34);
+ *   2 reflective: the inside face state with the normal momentum of every phase
+ *     negated, and every flux component but the normal momentum negated (the DMR
+ *     wall, Euler/riemann_problem.f90:225-228);
+ *   3 periodic: the opposite side (left with right, bottom with top, in pairs).
+ * The subcell recompute's ghost subcells follow the same rule: the neighbour
+ * table's clamp (or wrap) gives them, then inflow replaces and reflective
+ * mirrors them (Euler/limiter.f90:491-522), x sides before y sides. */
+enum aderdg_bc_code { ADERDG_BC_INFLOW = 0, ADERDG_BC_TRANSMIT = 1, ADERDG_BC_REFLECT = 2, ADERDG_BC_WRAP = 3 };
 
 typedef struct aderdg_ctx aderdg_ctx;
 
@@ -88,6 +104,9 @@ typedef struct {
                          the viscous flux (Advection-Diffusion/space_time_predictor.f90:78);
                          1: PDEFusedViscConvFlux there                    */
     int ad_pad;
+    int bc[4];        /* bc_mode SIDES: left, right, bottom, top codes
+                         (enum aderdg_bc_code)                            */
+    double bc_inflow[9]; /* conserved ghost state of inflow sides (nVar values) */
 } aderdg_desc;
 
 /* Setup: replaces the allocation and operator part of `initialize`

@@ -46,6 +46,8 @@ struct orc_ctx {
     int ad_bc[4], ad_fixed;
     double *gradQbnd; /* (nVar, nDim, 4, n) per cell */
     double *margin;   /* when recording: the DMP margin of each cell's candidate in the last step */
+    int side[4];      /* boundary code per side (orc_set_sides); from bc_mode otherwise */
+    double qin[VMAX]; /* conserved inflow state of code-0 sides */
 };
 
 /* PNPMTable of Euler/ader_dg.f90:28 */
@@ -576,6 +578,7 @@ orc_ctx *orc_create(int pde, int N, int imax, int jmax, const double xl[2], cons
     c->imax = imax;
     c->jmax = jmax;
     c->bc_mode = bc_mode;
+    for (int k = 0; k < 4; k++) c->side[k] = bc_mode == ORC_BC_PERIODIC ? 3 : 1;
     c->gamma = eos[0];
     if (pde == ORC_PDE_BN)
         for (int k = 0; k < 4; k++) c->eos[k] = eos[k];
@@ -1095,17 +1098,92 @@ double orc_dmp_margin(orc_ctx *c, const double *luh, const double *lmin, const d
 #define UH(c, arr, ci, cj) (&(arr)[(size_t)(c)->V * (c)->n * (c)->n * ((size_t)(ci) + (size_t)(c)->imax * (cj))])
 #define BND(c, arr, ci, cj) (&(arr)[(size_t)(c)->V * 4 * (c)->n * ((size_t)(ci) + (size_t)(c)->imax * (cj))])
 
-/* neighbour table of Euler/initialize.f90:226-258: clamp, periodic wrap when the mode is periodic */
+/* neighbour table of Euler/initialize.f90:226-258: clamp, wrap along an axis whose
+ * sides are periodic */
 static void neighbor(const orc_ctx *c, int i, int j, int di, int dj, int *in, int *jn)
 {
     int a = i + di, b = j + dj;
-    int per = c->bc_mode == ORC_BC_PERIODIC;
-    if (a < 0) a = per ? c->imax - 1 : 0;
-    if (a > c->imax - 1) a = per ? 0 : c->imax - 1;
-    if (b < 0) b = per ? c->jmax - 1 : 0;
-    if (b > c->jmax - 1) b = per ? 0 : c->jmax - 1;
+    const int dmr = c->bc_mode == ORC_BC_DMR;
+    const int px = !dmr && c->side[0] == 3, py = !dmr && c->side[2] == 3;
+    if (a < 0) a = px ? c->imax - 1 : 0;
+    if (a > c->imax - 1) a = px ? 0 : c->imax - 1;
+    if (b < 0) b = py ? c->jmax - 1 : 0;
+    if (b > c->jmax - 1) b = py ? 0 : c->jmax - 1;
     *in = a;
     *jn = b;
+}
+
+void orc_set_sides(orc_ctx *c, const int codes[4], const double *qin)
+{
+    c->bc_mode = ORC_BC_SIDES;
+    for (int k = 0; k < 4; k++) c->side[k] = codes[k];
+    for (int v = 0; v < c->V; v++) c->qin[v] = qin ? qin[v] : 0.0;
+}
+
+/* the momentum component of direction d (0 x, 1 y) of either phase */
+static int normal_momentum(const orc_ctx *c, int v, int d)
+{
+    return v == 1 + d || (c->V == 9 && v == 5 + d);
+}
+
+/* Ghost state and normal flux of a non-periodic side (orc_set_sides): inflow takes qin
+ * and its flux along nv, computed as the DMR inflow (Euler/riemann_problem.f90:186-193);
+ * transmissive copies the inside face (Baer-Nunziato/riemann_problem.f90:26-34);
+ * reflective copies it with the normal momentum negated and the other flux
+ * components negated, as the DMR wall (Euler/riemann_problem.f90:225-228). */
+static void side_ghost(const orc_ctx *c, int code, const double *nv, const double *qi, const double *fi, double *Q,
+                       double *F)
+{
+    const int V = c->V;
+    if (code == 0) {
+        double Fbc[2 * VMAX];
+        for (int v = 0; v < V; v++) Q[v] = c->qin[v];
+        pde_flux(c, Q, Fbc);
+        for (int v = 0; v < V; v++) F[v] = 0.0 + Fbc[v] * nv[0] + Fbc[v + V] * nv[1];
+        return;
+    }
+    for (int v = 0; v < V; v++) {
+        Q[v] = qi[v];
+        F[v] = fi[v];
+    }
+    if (code == 2) {
+        const int d = nv[0] != 0.0 ? 0 : 1;
+        for (int v = 0; v < V; v++) {
+            if (normal_momentum(c, v, d))
+                Q[v] = -Q[v];
+            else
+                F[v] = -F[v];
+        }
+    }
+}
+
+/* Ghost subcells of a boundary cell (ci, cj) in the (3s)^2 block array addressed by
+ * at(v, ii, jj), ii, jj in 1-s..2s: inflow replaces, reflective mirrors the normal
+ * momentum; left and right, then bottom and top (the DMR overrides of
+ * Euler/limiter.f90:491-522 for a code per side). */
+static void side_subcells(const orc_ctx *c, int ci, int cj, double *subuh0)
+{
+    const int V = c->V, s = c->s, W = 3 * s;
+#define AT(v, ii, jj) subuh0[(v) + V * (((ii) + s - 1) + W * ((jj) + s - 1))]
+    for (int sd = 0; sd < 4; sd++) {
+        const int code = c->side[sd];
+        if (code != 0 && code != 2) continue;
+        if ((sd == 0 && ci != 0) || (sd == 1 && ci != c->imax - 1) || (sd == 2 && cj != 0) ||
+            (sd == 3 && cj != c->jmax - 1))
+            continue;
+        const int d = sd < 2 ? 0 : 1;
+        const int lo = (sd == 0 || sd == 2) ? 1 - s : s + 1, hi = (sd == 0 || sd == 2) ? 0 : 2 * s;
+        for (int b = 1 - s; b <= 2 * s; b++)
+            for (int a = lo; a <= hi; a++)
+                for (int v = 0; v < V; v++) {
+                    double *x = d == 0 ? &AT(v, a, b) : &AT(v, b, a);
+                    if (code == 0)
+                        *x = c->qin[v];
+                    else if (normal_momentum(c, v, d))
+                        *x = -*x;
+                }
+    }
+#undef AT
 }
 
 /* calc_time_step: Euler/calc_time_step.f90:9-52 */
@@ -1194,7 +1272,7 @@ static void dmr_states(const orc_ctx *c, double *Qa, double *Qb, double *xsw)
 }
 
 /* solve_riemann_problem: Euler/riemann_problem.f90:10-116 (DMR boundaries), or the
- * periodic / transmissive boundary states (interior faces identical) */
+ * boundary states of the side codes (side_ghost; interior faces identical) */
 static void riemann(orc_ctx *c)
 {
     const int n = c->n, V = c->V, I = c->imax, J = c->jmax;
@@ -1213,16 +1291,13 @@ static void riemann(orc_ctx *c)
                             QL[v] = Qa[v];
                             FL[v] = 0.0 + Fbc[v] * nv[0] + Fbc[v + V] * nv[1];
                         }
-                    } else if (c->bc_mode == ORC_BC_PERIODIC) {
+                    } else if (c->side[0] == 3) {
                         for (int v = 0; v < V; v++) {
                             QL[v] = BND(c, c->qbnd, I - 1, j)[v + V * (1 + 4 * l)];
                             FL[v] = BND(c, c->Fbnd, I - 1, j)[v + V * (1 + 4 * l)];
                         }
                     } else {
-                        for (int v = 0; v < V; v++) {
-                            QL[v] = qr[v];
-                            FL[v] = fr[v];
-                        }
+                        side_ghost(c, c->side[0], nv, qr, fr, QL, FL);
                     }
                     for (int v = 0; v < V; v++) {
                         QR[v] = qr[v];
@@ -1233,16 +1308,13 @@ static void riemann(orc_ctx *c)
                         QL[v] = BND(c, c->qbnd, I - 1, j)[v + V * (1 + 4 * l)];
                         FL[v] = BND(c, c->Fbnd, I - 1, j)[v + V * (1 + 4 * l)];
                     }
-                    if (c->bc_mode == ORC_BC_PERIODIC) {
+                    if (c->bc_mode != ORC_BC_DMR && c->side[1] == 3) {
                         for (int v = 0; v < V; v++) {
                             QR[v] = BND(c, c->qbnd, 0, j)[v + V * (0 + 4 * l)];
                             FR[v] = BND(c, c->Fbnd, 0, j)[v + V * (0 + 4 * l)];
                         }
-                    } else {
-                        for (int v = 0; v < V; v++) {
-                            QR[v] = QL[v];
-                            FR[v] = FL[v];
-                        }
+                    } else {  /* the DMR right side is transmissive (:195-198) */
+                        side_ghost(c, c->bc_mode == ORC_BC_DMR ? 1 : c->side[1], nv, QL, FL, QR, FR);
                     }
                 } else {
                     for (int v = 0; v < V; v++) {
@@ -1262,17 +1334,19 @@ static void riemann(orc_ctx *c)
                 if (j == 0) {
                     const double *qr = &BND(c, c->qbnd, i, 0)[V * (2 + 4 * k)];
                     const double *fr = &BND(c, c->Fbnd, i, 0)[V * (2 + 4 * k)];
-                    if (c->bc_mode == ORC_BC_PERIODIC) {
+                    if (c->bc_mode != ORC_BC_DMR && c->side[2] == 3) {
                         for (int v = 0; v < V; v++) {
                             QL[v] = BND(c, c->qbnd, i, J - 1)[v + V * (3 + 4 * k)];
                             FL[v] = BND(c, c->Fbnd, i, J - 1)[v + V * (3 + 4 * k)];
                         }
+                    } else if (c->bc_mode != ORC_BC_DMR) {
+                        side_ghost(c, c->side[2], nv, qr, fr, QL, FL);
                     } else {
                         for (int v = 0; v < V; v++) {
                             QL[v] = qr[v];
                             FL[v] = fr[v];
                         }
-                        if (c->bc_mode == ORC_BC_DMR && !((double)(i + 1) * c->dx[0] <= 1.0 / 6.0)) {
+                        if (!((double)(i + 1) * c->dx[0] <= 1.0 / 6.0)) {
                             QL[2] = -QL[2];
                             FL[0] = -FL[0];
                             FL[1] = -FL[1];
@@ -1295,16 +1369,13 @@ static void riemann(orc_ctx *c)
                             QR[v] = Qbc[v];
                             FR[v] = 0.0 + Fbc[v] * nv[0] + Fbc[v + V] * nv[1];
                         }
-                    } else if (c->bc_mode == ORC_BC_PERIODIC) {
+                    } else if (c->side[3] == 3) {
                         for (int v = 0; v < V; v++) {
                             QR[v] = BND(c, c->qbnd, i, 0)[v + V * (2 + 4 * k)];
                             FR[v] = BND(c, c->Fbnd, i, 0)[v + V * (2 + 4 * k)];
                         }
                     } else {
-                        for (int v = 0; v < V; v++) {
-                            QR[v] = QL[v];
-                            FR[v] = FL[v];
-                        }
+                        side_ghost(c, c->side[3], nv, QL, FL, QR, FR);
                     }
                 } else {
                     for (int v = 0; v < V; v++) {
@@ -1394,6 +1465,7 @@ static void subcell_recompute(orc_ctx *c)
                         for (int p = 1; p <= s; p++)
                             for (int v = 0; v < V; v++) S0(v, p + ii * s, q + jj * s) = src[v + V * ((p - 1) + s * (q - 1))];
                 }
+            if (c->bc_mode != ORC_BC_DMR) side_subcells(c, ci, cj, subuh0);
             if (c->bc_mode == ORC_BC_DMR) { /* :491-522 */
                 if (ci == 0)
                     for (int jj = 1 - s; jj <= 2 * s; jj++)
@@ -1500,17 +1572,24 @@ static void riemann_bn(orc_ctx *c)
     const int n = c->n, V = c->V, I = c->imax, J = c->jmax;
     double QL[9], QR[9], FL[9], FR[9];
     double nv[2] = {1.0, 0.0};
+    /* inside face states; a boundary face takes its ghost from the side code
+     * (transmissive: the copy of Baer-Nunziato/riemann_problem.f90:26-34) */
     for (int j = 0; j < J; j++)
         for (int i = 0; i <= I; i++)
             for (int l = 0; l < n; l++) {
-                const int cl = i == 0 ? 0 : i - 1, fl = i == 0 ? 0 : 1;
-                const int cr = i == I ? I - 1 : i, fr = i == I ? 1 : 0;
-                for (int v = 0; v < V; v++) {
-                    QL[v] = BND(c, c->qbnd, cl, j)[v + V * (fl + 4 * l)];
-                    FL[v] = BND(c, c->Fbnd, cl, j)[v + V * (fl + 4 * l)];
-                    QR[v] = BND(c, c->qbnd, cr, j)[v + V * (fr + 4 * l)];
-                    FR[v] = BND(c, c->Fbnd, cr, j)[v + V * (fr + 4 * l)];
-                }
+                const int wrap = c->side[0] == 3;
+                if (i > 0 || wrap)
+                    for (int v = 0; v < V; v++) {
+                        QL[v] = BND(c, c->qbnd, i > 0 ? i - 1 : I - 1, j)[v + V * (1 + 4 * l)];
+                        FL[v] = BND(c, c->Fbnd, i > 0 ? i - 1 : I - 1, j)[v + V * (1 + 4 * l)];
+                    }
+                if (i < I || wrap)
+                    for (int v = 0; v < V; v++) {
+                        QR[v] = BND(c, c->qbnd, i < I ? i : 0, j)[v + V * (0 + 4 * l)];
+                        FR[v] = BND(c, c->Fbnd, i < I ? i : 0, j)[v + V * (0 + 4 * l)];
+                    }
+                if (i == 0 && !wrap) side_ghost(c, c->side[0], nv, QR, FR, QL, FL);
+                if (i == I && !wrap) side_ghost(c, c->side[1], nv, QL, FL, QR, FR);
                 const size_t o = (size_t)V * (l + n * (i + (I + 1) * j));
                 bn_rusanov(c->eos, QL, FL, QR, FR, nv, &c->Fu[o], &c->Fp[o]);
             }
@@ -1519,14 +1598,19 @@ static void riemann_bn(orc_ctx *c)
     for (int j = 0; j <= J; j++)
         for (int i = 0; i < I; i++)
             for (int k = 0; k < n; k++) {
-                const int cl = j == 0 ? 0 : j - 1, fl = j == 0 ? 2 : 3;
-                const int cr = j == J ? J - 1 : j, fr = j == J ? 3 : 2;
-                for (int v = 0; v < V; v++) {
-                    QL[v] = BND(c, c->qbnd, i, cl)[v + V * (fl + 4 * k)];
-                    FL[v] = BND(c, c->Fbnd, i, cl)[v + V * (fl + 4 * k)];
-                    QR[v] = BND(c, c->qbnd, i, cr)[v + V * (fr + 4 * k)];
-                    FR[v] = BND(c, c->Fbnd, i, cr)[v + V * (fr + 4 * k)];
-                }
+                const int wrap = c->side[2] == 3;
+                if (j > 0 || wrap)
+                    for (int v = 0; v < V; v++) {
+                        QL[v] = BND(c, c->qbnd, i, j > 0 ? j - 1 : J - 1)[v + V * (3 + 4 * k)];
+                        FL[v] = BND(c, c->Fbnd, i, j > 0 ? j - 1 : J - 1)[v + V * (3 + 4 * k)];
+                    }
+                if (j < J || wrap)
+                    for (int v = 0; v < V; v++) {
+                        QR[v] = BND(c, c->qbnd, i, j < J ? j : 0)[v + V * (2 + 4 * k)];
+                        FR[v] = BND(c, c->Fbnd, i, j < J ? j : 0)[v + V * (2 + 4 * k)];
+                    }
+                if (j == 0 && !wrap) side_ghost(c, c->side[2], nv, QR, FR, QL, FL);
+                if (j == J && !wrap) side_ghost(c, c->side[3], nv, QL, FL, QR, FR);
                 const size_t o = (size_t)V * (k + n * (i + I * j));
                 bn_rusanov(c->eos, QL, FL, QR, FR, nv, &c->Gu[o], &c->Gp[o]);
             }
@@ -1557,8 +1641,9 @@ static void bn_rusanov_s(const double *e, double *F, double *D, const double *QL
 }
 
 /* subcell_recompute of Baer-Nunziato/limiter.f90:416-596: MUSCL-Hancock with the NCP
- * source and Roe fluctuations; slopes also on the halo ring (:486-492), no boundary
- * overrides (the neighbour table's clamp gives the ghost subcells). */
+ * source and Roe fluctuations; slopes also on the halo ring (:486-492).  The neighbour
+ * table's clamp gives the ghost subcells (the reference's transmissive sides), and
+ * inflow / reflective side codes then override them (side_subcells). */
 static void subcell_recompute_bn(orc_ctx *c)
 {
     const int V = c->V, s = c->s, I = c->imax, J = c->jmax;
@@ -1596,6 +1681,7 @@ static void subcell_recompute_bn(orc_ctx *c)
                         for (int p = 1; p <= s; p++)
                             for (int v = 0; v < V; v++) S0(v, p + ii * s, q + jj * s) = src[v + V * ((p - 1) + s * (q - 1))];
                 }
+            side_subcells(c, ci, cj, subuh0);
             /* :486-513 slopes, NCP source, half step on interior and halo ring */
             for (int jj = 0; jj <= s + 1; jj++)
                 for (int ii = 0; ii <= s + 1; ii++) {

@@ -21,7 +21,7 @@ extern "C" {
 #endif
 
 enum { ORC_PDE_EULER = 0, ORC_PDE_BN = 1, ORC_PDE_AD = 2 };
-enum { ORC_BC_DMR = 0, ORC_BC_PERIODIC = 1, ORC_BC_TRANSMISSIVE = 2 };
+enum { ORC_BC_DMR = 0, ORC_BC_PERIODIC = 1, ORC_BC_TRANSMISSIVE = 2, ORC_BC_SIDES = 3 };
 
 typedef struct orc_ctx orc_ctx;
 
@@ -82,6 +82,21 @@ void orc_get_face_fluxes(const orc_ctx *c, double *fu, double *gu);
  * 3 periodic) and the predictor flux: fixed = 0 the reference's PDEViscFlux,
  * fixed = 1 PDEFusedViscConvFlux.  Call right after orc_create. */
 void orc_set_ad(orc_ctx *c, double cx, double cy, double a, const int bc[4], int fixed);
+
+/* Euler / Baer-Nunziato boundary code per side (left, right, bottom, top), the
+ * reference's params.ini convention (Euler/params.ini:8-11): 0 inflow with the
+ * conserved state qin, 1 transmissive, 2 reflective, 3 periodic (in pairs).  The
+ * reference computes only the DMR set (Euler/riemann_problem.f90:10-116) and
+ * transmissive faces (Baer-Nunziato/riemann_problem.f90:9-80); the other codes
+ * restate those patterns per side: inflow as the DMR inflow (:186-193), reflective
+ * as the DMR wall (:225-228), and the subcell ghosts as the DMR overrides of
+ * Euler/limiter.f90:491-522 on the neighbour table's clamp.  Parity of the codes
+ * the reference lacks is pinned by invariants (tests/test_oracle_sides.py): all 1
+ * and all 3 equal the transmissive and periodic modes bit for bit, a reflective
+ * wall equals the mirror image of the doubled domain, a uniform state with inflow
+ * of the same state stays uniform, and a supersonic inflow state fills the mesh.
+ * Call right after orc_create. */
+void orc_set_sides(orc_ctx *c, const int codes[4], const double *qin);
 void orc_predictor_ad(orc_ctx *c, double *lqhi, double *lFhi, double *lqbnd, double *lgradqbnd, const double *luh);
 
 void orc_reset_opcount(void);

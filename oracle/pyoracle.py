@@ -17,7 +17,7 @@ import tempfile
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-BC_DMR, BC_PERIODIC, BC_TRANSMISSIVE = 0, 1, 2
+BC_DMR, BC_PERIODIC, BC_TRANSMISSIVE, BC_SIDES = 0, 1, 2, 3
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -66,6 +66,8 @@ class Oracle:
             lib.orc_dmp_margin.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.c_double]
             lib.orc_set_scalars.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double]
             lib.orc_get_opcount.restype = ctypes.c_double
+            lib.orc_set_sides.restype = None
+            lib.orc_set_sides.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), _dp]
             lib.orc_set_ad.restype = None
             lib.orc_set_ad.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                        ctypes.POINTER(ctypes.c_int), ctypes.c_int]
@@ -78,10 +80,14 @@ class Oracle:
             cls._lib = lib
         return cls._lib
 
-    def __init__(self, N, imax, jmax, xl, xr, bc_mode, gamma=1.4, pde=0, eos=None, ad=None):
+    def __init__(self, N, imax, jmax, xl, xr, bc_mode, gamma=1.4, pde=0, eos=None, ad=None, sides=None,
+                 inflow=None):
         """pde 0: Euler (gamma); pde 1: Baer-Nunziato (eos = GAMMA_S, GAMMA_G, PI_S, PI_G);
         pde 2: advection-diffusion, ad = dict(c=(cx, cy), a=diffusivity, bc=(bL, bR, bB, bT),
-        fixed=0/1) with the boundary codes of Advection-Diffusion/params.ini."""
+        fixed=0/1) with the boundary codes of Advection-Diffusion/params.ini.
+        sides: Euler / Baer-Nunziato codes (left, right, bottom, top) of orc_set_sides
+        (0 inflow of the conserved state `inflow`, 1 transmissive, 2 reflective, 3 periodic),
+        in place of bc_mode."""
         self.pde = pde
         self.N, self.n, self.s, self.V = N, N + 1, 2 * N + 1, {0: 4, 1: 9, 2: 1}[pde]
         self.imax, self.jmax = imax, jmax
@@ -92,6 +98,11 @@ class Oracle:
         self.h = L.orc_create(pde, N, imax, jmax, _p(xl), _p(xr), bc_mode, _p(eos))
         if not self.h:
             raise ValueError("orc_create failed")
+        if sides is not None:
+            q = np.zeros(9)
+            if inflow is not None:
+                q[:len(inflow)] = inflow
+            L.orc_set_sides(ctypes.c_void_p(self.h), (ctypes.c_int * 4)(*[int(x) for x in sides]), _p(q))
         if pde == 2:
             bc = (ctypes.c_int * 4)(*ad["bc"])
             L.orc_set_ad(ctypes.c_void_p(self.h), float(ad["c"][0]), float(ad["c"][1]), float(ad["a"]), bc,

@@ -152,6 +152,20 @@ def test_ad_descriptor_validation():
             aderdg.Solver(2, 8, 8, (0.0, 0.0), (1.0, 1.0), pde="ad", ad=bad)
 
 
+def test_side_codes_descriptor_validation():
+    """Per-side boundary codes (ADERDG_BC_SIDES): codes outside 0..3, a periodic side
+    without its opposite, and a non-finite inflow state are refused before any device
+    work."""
+    q = (1.0, 0.0, 0.0, 2.5)
+    for bad, msg in ((aderdg.sides(0, 1, 4, 1, inflow=q), "outside 0..3"),
+                     (aderdg.sides(3, 1, 1, 1), "in pairs"),
+                     (aderdg.sides(1, 1, 3, 2), "in pairs"),
+                     (aderdg.sides(0, 1, 1, 1, inflow=(1.0, float("nan"), 0.0, 2.5)), "not finite")):
+        with pytest.raises(aderdg.AderdgError, match=msg):
+            aderdg.Solver(2, 8, 8, (0.0, 0.0), (1.0, 1.0), bad)
+    assert aderdg.sides("inflow", "transmissive", "reflective", "reflective").codes == (0, 1, 2, 2)
+
+
 def test_ad_sine_ic_and_footprint():
     """The sine initial condition of the advection-diffusion case, A sin(2 pi kx x) sin(2 pi ky y)
     at the nodes, and a one-variable arena footprint below the Euler one."""
