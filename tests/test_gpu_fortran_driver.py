@@ -8,32 +8,19 @@ import numpy as np
 import pytest
 
 import aderdg
-from aderdg import ics
-from pyoracle import BC_DMR, BC_PERIODIC, Oracle, RefBN, RefEuler
+from aderdg import ics, params
+from pyoracle import BC_DMR, BC_PERIODIC, BC_TRANSMISSIVE, Oracle, RefBN, RefEuler
 
 pytestmark = pytest.mark.gpu
 LIB = aderdg.LIBDIR
 
-PARAMS = """{gamma}
-{imax}
-{jmax}
-{xl0}
-{xl1}
-{xr0}
-{xr1}
-{b}
-{b}
-{b}
-{b}
-{ic}
-{tend}
-{wi}
-"""
-
-
 def run_driver(tmp_path, exe, N, steps, imax, jmax, xl, xr, ic, bcode, tend, wi=100000):
-    (tmp_path / "params.ini").write_text(PARAMS.format(gamma=1.4, imax=imax, jmax=jmax, xl0=xl[0], xl1=xl[1],
-                                                       xr0=xr[0], xr1=xr[1], b=bcode, ic=ic, tend=tend, wi=wi))
+    """params.ini written by aderdg.params in the reference's layout; bcode: one code
+    for every side or (bL, bR, bB, bT)."""
+    b = bcode if isinstance(bcode, tuple) else (bcode,) * 4
+    params.write_params(tmp_path / "params.ini", dict(gamma=1.4, imax=imax, jmax=jmax, xl0=xl[0], xl1=xl[1], xr0=xr[0],
+                                                      xr1=xr[1], bL=b[0], bR=b[1], bB=b[2], bT=b[3], ictype=ic,
+                                                      tend=tend, write_interval=wi), "euler")
     dump = tmp_path / "uh.bin"
     print("driver", exe, N, steps, flush=True)
     r = subprocess.run([os.path.join(LIB, exe), str(N), str(steps), str(dump)], cwd=tmp_path,
@@ -70,26 +57,6 @@ def test_driver_dmr_bitwise(tmp_path):
     assert vtk[0] == "DMR-00000000.vtk" and "DMR-00000003.vtk" in vtk
 
 
-BN_PARAMS = """1.4
-1.4
-0.0
-0.0
-{imax}
-{jmax}
--0.5
--0.05
-0.5
-0.05
-1
-1
-1
-1
-{ic}
-{tend}
-{wi}
-"""
-
-
 @pytest.mark.parametrize("ictype", [1, 3])
 def test_bn_driver_bitwise(tmp_path, ictype):
     """The Baer-Nunziato driver (Baer-Nunziato/main.f90 call order) on the reference's
@@ -97,7 +64,9 @@ def test_bn_driver_bitwise(tmp_path, ictype):
     N, I, J, steps = 2, 41, 4, 5
     if not RefBN.available(N):
         pytest.skip("reference build absent (oracle/build_ref.sh)")
-    (tmp_path / "baer_nunziato_params.ini").write_text(BN_PARAMS.format(imax=I, jmax=J, ic=ictype, tend=0.1, wi=2))
+    params.write_params(tmp_path / "baer_nunziato_params.ini",
+                        dict(gamma_s=1.4, gamma_g=1.4, pi_s=0.0, pi_g=0.0, imax=I, jmax=J, xl0=-0.5, xl1=-0.05, xr0=0.5,
+                             xr1=0.05, bL=1, bR=1, bB=1, bT=1, ictype=ictype, tend=0.1, write_interval=2), "bn")
     dump = tmp_path / "uh.bin"
     print("driver aderdg_bn_exact", ictype, flush=True)
     r = subprocess.run([os.path.join(LIB, "aderdg_bn_exact"), str(N), str(steps), str(dump)], cwd=tmp_path,
@@ -114,6 +83,28 @@ def test_bn_driver_bitwise(tmp_path, ictype):
     assert vtk[0] == base + "-00000000.vtk" and base + "-00000004.vtk" in vtk
 
 
+def test_driver_quadrant_side_codes_bitwise(tmp_path):
+    """The Euler driver on the Lax-Liu quadrant (ICType 5) with walls left and right and
+    periodic bottom and top from params.ini: bit for bit against the oracle with the
+    same codes (the split lies on cell faces, so no cell starts limited)."""
+    N, I, J, steps = 2, 20, 16, 5
+    uh, out = run_driver(tmp_path, "aderdg_euler_exact", N, steps, I, J, (0.0, 0.0), (1.0, 1.0), 5, (2, 2, 3, 3), 0.3)
+    uh0 = np.zeros_like(uh, order="F")
+    lib = aderdg.load_library(True)
+    import ctypes
+    xl = (ctypes.c_double * 2)(0.0, 0.0)
+    xr = (ctypes.c_double * 2)(1.0, 1.0)
+    assert lib.aderdg_initial_condition(5, N, I, J, xl, xr, ctypes.c_double(1.4),
+                                        uh0.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
+    o = Oracle(N, I, J, (0.0, 0.0), (1.0, 1.0), BC_TRANSMISSIVE, sides=(2, 2, 3, 3))
+    o.set_uh(uh0)
+    o.set_scalars(0.0, 0.0, 0.3)
+    for _ in range(steps):
+        o.step()
+    assert o.get_limiter()["status"].sum() > 0
+    np.testing.assert_array_equal(uh, o.get_uh())
+
+
 def test_driver_vortex_fma(tmp_path):
     N, I, J, steps = 4, 16, 16, 5
     uh, out = run_driver(tmp_path, "aderdg_euler", N, steps, I, J, (0.0, 0.0), (10.0, 10.0), 4, 3, 1e9)
@@ -128,27 +119,10 @@ def test_driver_vortex_fma(tmp_path):
     assert np.linalg.norm((uh - ref).ravel()) <= 1e-12 * np.linalg.norm(ref.ravel())
 
 
-AD_PARAMS = """{cx}
-{cy}
-{a}
-{imax}
-{jmax}
-0.0
-0.0
-1.0
-1.0
-{tend}
-{b1}
-{b2}
-{b3}
-{b4}
-{wi}
-"""
-
-
 def run_ad_driver(tmp_path, exe, N, ic, fused, steps, imax, jmax, c, a, bc, tend, wi=100000):
-    (tmp_path / "params.ini").write_text(AD_PARAMS.format(cx=c[0], cy=c[1], a=a, imax=imax, jmax=jmax, tend=tend,
-                                                          b1=bc[0], b2=bc[1], b3=bc[2], b4=bc[3], wi=wi))
+    params.write_params(tmp_path / "params.ini",
+                        dict(cx=c[0], cy=c[1], a=a, imax=imax, jmax=jmax, xl0=0.0, xl1=0.0, xr0=1.0, xr1=1.0, tend=tend,
+                             bL=bc[0], bR=bc[1], bB=bc[2], bT=bc[3], write_interval=wi), "ad")
     dump = tmp_path / "uh.bin"
     print("driver", exe, N, ic, fused, steps, flush=True)
     r = subprocess.run([os.path.join(LIB, exe), str(N), str(ic), str(fused), str(steps), str(dump)], cwd=tmp_path,
