@@ -42,6 +42,9 @@ template <class P>
 constexpr int kPredBlock = P::V == 4 ? 256 : 64;
 template <class P>
 constexpr int kUpdBlock = P::V == 4 ? 128 : 64;
+// copies of the update kernel's LDS range slots per cell (k_update): 4 measured best of
+// 1, 2, 4, 8, 16 (fewer lanes meet on one address; more copies cost occupancy)
+constexpr int kUpdKeyCopies = 4;
 
 template <class P, int N>
 struct Shape {
@@ -344,10 +347,12 @@ ADG_DEV void predictor_cells(const MeshArgs &m, const StepBufs &b, const TestBuf
 // which also accumulates the time averages, is a separate copy of the loop.
 // smem: CPW cells x (V NN state + SLOT exchange) + the Kxi and dudx matrices.
 // ---------------------------------------------------------------------------
+// LDS doubles of a cell are padded by 16 so that consecutive cells start 32 banks apart: the
+// lanes of two cells in one half-wave then read the same offsets in different banks
 template <class P, int N>
 struct LeanShape {
     using S = Shape<P, N>;
-    static constexpr int CELL = S::V * S::NN + S::SLOT;   // LDS doubles per cell
+    static constexpr int CELL = S::V * S::NN + S::SLOT + 16;   // LDS doubles per cell
     static constexpr int LDS = S::CPW * CELL + 2 * S::NN;  // + Kxi, dudx
 };
 
@@ -1283,13 +1288,21 @@ ADG_DEV void key_max(unsigned long long *slot, double x)
 // slots with LDS atomics on order-preserving keys.  Only the nodal values and
 // the slots live in LDS (1.6 KB per Baer-Nunziato N=3 cell).
 // ---------------------------------------------------------------------------
+// waves per SIMD asked of k_update: Euler N=3 fits four without spilling (one more than
+// the compiler's own choice, 1.22 -> 1.08 ms on 1024x1024); elsewhere the compiler's choice
+template <class P, int N>
+constexpr int kUpdWaves = (P::V == 4 && N == 3) ? 4 : 1;
+
 template <class P, int N, bool SEED>
-__global__ __launch_bounds__(kUpdBlock<P>) void k_update(StepBufs b)
+__global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kUpdWaves<P, N>))) void k_update(StepBufs b)
 {
     using S = Shape<P, N>;
     constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, CPW = S::CPWU, V = S::V, NP = S::NPART;
+    // KC copies of every slot: the cell's threads spread their atomics over the copies
+    // (thread tt takes copy tt % KC), so at most ceil(NN / KC) lanes meet on one address
+    constexpr int KC = kUpdKeyCopies;
     __shared__ double ssuh[CPW][V * NN];
-    __shared__ unsigned long long skey[CPW][NP];
+    __shared__ unsigned long long skey[CPW][KC][NP];
     __shared__ int sflag[CPW];
     const OpPtr op = (OpPtr)b.op;
     const int tid = threadIdx.x, lc = tid / NN, tt = tid - lc * NN;
@@ -1309,13 +1322,26 @@ __global__ __launch_bounds__(kUpdBlock<P>) void k_update(StepBufs b)
     const bool act = own && cell < m.ncell;
     const double *uh_cur = b.uh[par];
     double *suh = &ssuh[lcs][0];
-    unsigned long long *key = &skey[lcs][0];  // [kind * V + v], kind 0/1 nodal min/max,
-                                              // 2/3 Lobatto, 4/5 subcell; [6V] dt
+    unsigned long long *key = &skey[lcs][tt % KC][0];  // [kind * V + v], kind 0/1 nodal min/max,
+                                                       // 2/3 Lobatto, 4/5 subcell; [6V] dt
     if (own) {
         if (tt == 0) sflag[lc] = 0;
-        for (int slot = tt; slot < NP; slot += NN)
-            key[slot] = (slot < 6 * V && ((slot / V) & 1)) ? 0ull : ~0ull;
+        for (int k = tt; k < KC * NP; k += NN) {
+            const int slot = k % NP;
+            skey[lc][k / NP][slot] = (slot < 6 * V && ((slot / V) & 1)) ? 0ull : ~0ull;
+        }
     }
+    // value of a slot: the min (max) over the copies of the order-preserving keys
+    auto slot_val = [&](int slot) {
+        const bool mx = slot < 6 * V && ((slot / V) & 1);
+        unsigned long long r = skey[lcs][0][slot];
+#pragma unroll
+        for (int c = 1; c < KC; c++) {
+            const unsigned long long k = skey[lcs][c][slot];
+            r = mx ? (k > r ? k : r) : (k < r ? k : r);
+        }
+        return key_val(r);
+    };
     // the DMP bounds of this step (lmin/lmax of get_min_max), loaded now so their
     // latency overlaps the update instead of following the range reduction
     constexpr int KV = (V + NN - 1) / NN;  // variables per thread in the DMP pass
@@ -1447,8 +1473,8 @@ __global__ __launch_bounds__(kUpdBlock<P>) void k_update(StepBufs b)
     for (int k = 0; k < KV; k++) {
         const int v = tt + k * NN;
         if (!act || v >= V) continue;
-        const double r[6] = {key_val(key[0 * V + v]), key_val(key[1 * V + v]), key_val(key[2 * V + v]),
-                             key_val(key[3 * V + v]), key_val(key[4 * V + v]), key_val(key[5 * V + v])};
+        const double r[6] = {slot_val(0 * V + v), slot_val(1 * V + v), slot_val(2 * V + v),
+                             slot_val(3 * V + v), slot_val(4 * V + v), slot_val(5 * V + v)};
         if (!SEED && dmp_violated(r, pmn[k], pmx[k])) atomicOr(&sflag[lc], 2);
         // next step's get_min_max step 1 for a cell without limiter, Euler/limiter.f90:123-132
         double mn = f_min(f_min(r[0], r[2]), r[4]), mx = f_max(f_max(r[1], r[3]), r[5]);
@@ -1465,7 +1491,7 @@ __global__ __launch_bounds__(kUpdBlock<P>) void k_update(StepBufs b)
         b.lmin_old[(size_t)v * b.G + g] = mn;
         b.lmax_old[(size_t)v * b.G + g] = mx;
     }
-    if (act && tt == NN - 1) b.cell_dt[g] = key_val(key[6 * V]);
+    if (act && tt == NN - 1) b.cell_dt[g] = slot_val(6 * V);
     __syncthreads();
     if (!SEED && act && tt == 0) {
         const int tr = sflag[lc] != 0;
