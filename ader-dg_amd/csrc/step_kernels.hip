@@ -42,6 +42,9 @@ template <class P>
 constexpr int kPredBlock = P::V == 4 ? 256 : 64;
 template <class P>
 constexpr int kUpdBlock = P::V == 4 ? 128 : 64;
+// threads of the Baer-Nunziato subcell workgroup (one listed cell each): one interior face per thread
+constexpr int kSubBnBlock = 128;
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
 // copies of the update kernel's LDS range slots per cell (k_update): 4 measured best of
 // 1, 2, 4, 8, 16 (fewer lanes meet on one address; more copies cost occupancy)
 constexpr int kUpdKeyCopies = 4;
@@ -1559,17 +1562,19 @@ __global__ __launch_bounds__(kBlock) void k_spread(StepBufs b)
 
 // Common tail of both subcell kernels: put_subcell_data (Euler/limiter.f90:86-96),
 // re-DMP of cells marked 2 (:619-638), next-step limiter seeds and dt candidate.
-template <class P, int N, class M>
+// NT: threads of the workgroup (one cell per workgroup); sNth: LDS word that broadcasts the
+// pool slot count of a limited cell to the workgroup
+template <class P, int N, int NT, class M>
 ADG_DEV void subcell_finish(const M &m, const StepBufs &b, size_t cell, int sol, int lh0, int lhcap, int par, int flag,
                             const double *sSub,
                             double *sUhy, double *sUh, double *sTy, double *sBy, double *sLim, double *sLob, double *sDt,
-                            int *sFlag)
+                            int *sFlag, int *sNth)
 {
     using S = Shape<P, N>;
     constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, V = S::V;
     const OpPtr op = (OpPtr)b.op;
     const int lane = threadIdx.x;
-    for (int task = lane; task < s * n; task += 64) {
+    for (int task = lane; task < s * n; task += NT) {
         const int p = task % s, bb = task / s;
 #pragma unroll
         for (int v = 0; v < V; v++) {
@@ -1580,7 +1585,7 @@ ADG_DEV void subcell_finish(const M &m, const StepBufs &b, size_t cell, int sol,
         }
     }
     __syncthreads();
-    for (int task = lane; task < NN; task += 64) {
+    for (int task = lane; task < NN; task += NT) {
         const int a = task % n, bb = task / n;
         double un[V];
 #pragma unroll
@@ -1596,19 +1601,19 @@ ADG_DEV void subcell_finish(const M &m, const StepBufs &b, size_t cell, int sol,
     }
     if (lane == 0) *sFlag = 0;
     __syncthreads();
-    cell_ranges_stage1<P, N>(op, sUh, sTy, sBy, lane, 64);
+    cell_ranges_stage1<P, N>(op, sUh, sTy, sBy, lane, NT);
     __syncthreads();
-    const bool bad = cell_ranges_stage2<P, N>(op, m, sTy, sBy, sLim, sLob, lane, 64);
+    const bool bad = cell_ranges_stage2<P, N>(op, m, sTy, sBy, sLim, sLob, lane, NT);
     if (bad) atomicOr(sFlag, 1);
     __syncthreads();
-    for (int v = lane; v < V; v += 64) {
+    for (int v = lane; v < V; v += NT) {
         double r[6];
         group_ranges<P, N>(sUh, sLob, sLim, v, r);
         if (dmp_violated(r, b.lmin[(size_t)v * b.G + cell], b.lmax[(size_t)v * b.G + cell])) atomicOr(sFlag, 2);
     }
     __syncthreads();
     const int status = (flag == 1) ? 1 : (*sFlag != 0 ? 1 : 0);
-    for (int v = lane; v < V; v += 64) {
+    for (int v = lane; v < V; v += NT) {
         double mn, mx;
         if (status) {  // next get_min_max uses Lh (:135-139)
             mn = __builtin_nan("");
@@ -1626,21 +1631,21 @@ ADG_DEV void subcell_finish(const M &m, const StepBufs &b, size_t cell, int sol,
         b.lmin_old[(size_t)v * b.G + cell] = mn;
         b.lmax_old[(size_t)v * b.G + cell] = mx;
     }
-    if (lane == 63) {
+    if (lane == NT - 1) {
         double d = __builtin_nan("");
         for (int k = 0; k < NN; k++) d = fold_min(d, sDt[k]);
         b.cell_dt[cell] = d;
         b.status[par ^ 1][cell] = status;
     }
     if (status) {  // NewLh in a pool slot of the solve, and Lh = NewLh at update_limiter (:287)
-        int nth = 0;
-        if (lane == 0) nth = atomicAdd(b.lh_count + sol, 1);
-        nth = __shfl(nth, 0);
+        if (lane == 0) *sNth = atomicAdd(b.lh_count + sol, 1);
+        __syncthreads();  // status is the same in every thread of the workgroup
+        const int nth = *sNth;
         const int ls = nth < lhcap ? lh0 + nth : -1;  // none left: the solve ends failed after this step
         if (lane == 0) b.lh_slot[par ^ 1][cell] = ls;
         if (ls >= 0) {
             double *L = b.lh[par ^ 1] + (size_t)ls * V * SS;
-            for (int k = lane; k < V * SS; k += 64) L[k] = sSub[(k % V) * SS + k / V];
+            for (int k = lane; k < V * SS; k += NT) L[k] = sSub[(k % V) * SS + k / V];
         }
     }
     __syncthreads();
@@ -1667,7 +1672,7 @@ __global__ __launch_bounds__(64) void k_subcell(StepBufs b)
     __shared__ double sUh[V * NN];
     __shared__ double sTy[V * n * s], sBy[V * NN], sLim[V * SS], sLob[V * NN];
     __shared__ double sDt[NN];
-    __shared__ int sFlag;
+    __shared__ int sFlag, sNth;
     const OpPtr op = (OpPtr)b.op;
     const int lane = threadIdx.x;
     const int count = b.counters[kCntList];
@@ -1855,8 +1860,8 @@ __global__ __launch_bounds__(64) void k_subcell(StepBufs b)
         }
 #undef S0
         __syncthreads();
-        subcell_finish<P, N>(m, b, gcell, slot, sa.lh0, sa.lhcap, par, flag, sSub, sUhy, sUh, sTy, sBy, sLim, sLob,
-                             sDt, &sFlag);
+        subcell_finish<P, N, 64>(m, b, gcell, slot, sa.lh0, sa.lhcap, par, flag, sSub, sUhy, sUh, sTy, sBy, sLim, sLob,
+                                 sDt, &sFlag, &sNth);
     }
 }
 
@@ -1868,30 +1873,39 @@ __global__ __launch_bounds__(64) void k_subcell(StepBufs b)
 // boundary overrides.  The half-step states w are recomputed from (S0,
 // slopes, Qt) where needed instead of being stored.
 // ---------------------------------------------------------------------------
+// Baer-Nunziato subcell_recompute over the compacted list (Baer-Nunziato/limiter.f90:416-596),
+// one workgroup of kSubBnBlock threads per listed cell.  The reference evaluates the
+// path-conservative RusanovFluxS of every interior subcell face twice (as the right face of one
+// subcell and the left face of the next, with the same arguments); here each face is evaluated
+// once, one face per thread, and the subcell updates read both sides from LDS: the same values,
+// with half the flux work and no serial loop over a subcell's four faces.
 template <int N>
-__global__ __launch_bounds__(64) void k_subcell_bn(StepBufs b)
+__global__ __launch_bounds__(kSubBnBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_subcell_bn(StepBufs b)
 {
     using P = BnPDE;
     using S = Shape<P, N>;
+    constexpr int NT = kSubBnBlock;
     constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, V = S::V;
     constexpr int G = s + 4, GG = G * G;  // subgrid index -1..s+2
     constexpr int H = s + 2, HH = H * H;  // halo ring 0..s+1
-    constexpr int LY = 9 * V * n * s, SL = 3 * V * HH;
-    constexpr int R0 = LY > SL ? LY : SL;
+    constexpr int NF = (s + 1) * s;       // x-faces (and y-faces) of the interior subcells
+    constexpr int FP = (2 * NF + NT - 1) / NT;  // faces per thread
+    constexpr int LY = 9 * V * n * s, SL = 3 * V * HH, FL = 4 * V * NF;
     constexpr int DMPB = V * n * s + V * NN + V * SS + V * NN;
-    static_assert(DMPB <= R0, "DMP scratch must fit in the slope region");
+    constexpr int R0 = cmax(cmax(LY, SL), cmax(FL, DMPB));
     __shared__ double sS0[V * GG];
-    __shared__ double sR0[R0];     // limy of 9 blocks, then slopes x / y / t, then DMP scratch
-    __shared__ double sSub[V * SS];
+    __shared__ double sR0[R0];     // limy of 9 blocks; slopes x / y / t; face F* and D; DMP scratch
+    __shared__ double sSub[V * SS];  // NCP source of the interior half-step states, then the new averages
     __shared__ double sUhy[V * s * n];
     __shared__ double sUh[V * NN];
     __shared__ double sDt[NN];
-    __shared__ int sFlag;
+    __shared__ int sFlag, sNth;
     const OpPtr op = (OpPtr)b.op;
     const int lane = threadIdx.x;
     const int count = b.counters[kCntList];
     double *sLy = sR0;
     double *slx = sR0, *sly = sR0 + V * HH, *slt = sR0 + 2 * V * HH;
+    double *sFF = sR0, *sFD = sR0 + 2 * V * NF;  // [dir][v][face]: F* and D of x- (dir 0) and y-faces
     for (int item = blockIdx.x; item < count; item += gridDim.x) {
         const int gcell = b.list[item];
         const int slot = ld_const(b.cell_solve + gcell);
@@ -1905,7 +1919,7 @@ __global__ __launch_bounds__(64) void k_subcell_bn(StepBufs b)
         const int flag = b.rc_final[gcell];
         const int ci = cell % m.imax, cj = cell / m.imax;
         // blocks (di,dj) of the 3x3 Voronoi neighbourhood, :463-479
-        for (int task = lane; task < 9 * n * s; task += 64) {
+        for (int task = lane; task < 9 * n * s; task += NT) {
             const int blk = task / (n * s), r = task - blk * n * s, a = r % n, qq = r / n;
             const int di = blk % 3 - 1, dj = blk / 3 - 1;
             const int nb = sa.cell0 + nb_index(ci, di, m.imax, m.wrap & 1) + m.imax * nb_index(cj, dj, m.jmax, m.wrap >> 1);
@@ -1920,7 +1934,7 @@ __global__ __launch_bounds__(64) void k_subcell_bn(StepBufs b)
             }
         }
         __syncthreads();
-        for (int task = lane; task < GG; task += 64) {  // subuh0 on -1..s+2
+        for (int task = lane; task < GG; task += NT) {  // subuh0 on -1..s+2
             const int ii = task % G - 1, jj = task / G - 1;
             const int di = ii < 1 ? -1 : (ii > s ? 1 : 0), dj = jj < 1 ? -1 : (jj > s ? 1 : 0);
             const int p = ii - 1 - di * s, qq = jj - 1 - dj * s;
@@ -1948,8 +1962,9 @@ __global__ __launch_bounds__(64) void k_subcell_bn(StepBufs b)
         __syncthreads();
 #define S0(v, ii, jj) sS0[(v) * GG + ((ii) + 1) + G * ((jj) + 1)]
 #define HIX(ii, jj) ((ii) + H * (jj))
-        // slopes, NCP source and half-step time derivative on 0..s+1, :486-513
-        for (int task = lane; task < HH; task += 64) {
+        // slopes, NCP source and half-step time derivative on 0..s+1, :486-513; for the
+        // interior subcells also the NCP source of the half-step state (:520-523)
+        for (int task = lane; task < HH; task += NT) {
             const int ii = task % H, jj = task / H;
             double sx[V], sy[V], Gr[2 * V], src[V], q0[V], wL[V], wR[V], FL[2 * V], FR[2 * V], Qt[V];
 #pragma unroll
@@ -1984,6 +1999,13 @@ __global__ __launch_bounds__(64) void k_subcell_bn(StepBufs b)
                 sly[v * HH + task] = sy[v];
                 slt[v * HH + task] = Qt[v];
             }
+            if (ii >= 1 && ii <= s && jj >= 1 && jj <= s) {
+#pragma unroll
+                for (int v = 0; v < V; v++) q0[v] = q0[v] + 0.5 * dt * Qt[v];
+                P::ncp(m, q0, Gr, src);
+#pragma unroll
+                for (int v = 0; v < V; v++) sSub[v * SS + (ii - 1) + s * (jj - 1)] = src[v];
+            }
         }
         __syncthreads();
         // half-step face states: w = (S0 -/+ 0.5 slope) + 0.5 dt Qt, as the reference stores them
@@ -1992,64 +2014,84 @@ __global__ __launch_bounds__(64) void k_subcell_bn(StepBufs b)
             const double w0 = right ? S0(v, ii, jj) + 0.5 * sl : S0(v, ii, jj) - 0.5 * sl;
             return w0 + 0.5 * dt * slt[v * HH + HIX(ii, jj)];
         };
-        // update of the interior subcells, :516-537: four RusanovFluxS per subcell
-        for (int task = lane; task < SS; task += 64) {
-            const int ii = task % s + 1, jj = task / s + 1;
-            double Gr[2 * V], qh[V], src[V], acc_f[V], acc_d[V];
+        // RusanovFluxS of every interior face, Baer-Nunziato/limiter.f90:359-383: x-face r joins
+        // subcells (r % (s+1), row) and (r % (s+1) + 1, row), row = r / (s+1) + 1; y-face r joins
+        // (col, r / s) and (col, r / s + 1), col = r % s + 1
+        double fF[FP][V], fD[FP][V];
+#pragma unroll
+        for (int k = 0; k < FP; k++) {
+            const int task = lane + k * NT;
+            if (task >= 2 * NF) continue;
+            const int dir = task / NF, r = task - dir * NF;
+            int iL, jL, iR, jR;
+            if (dir == 0) {
+                iL = r % (s + 1);
+                jL = r / (s + 1) + 1;
+                iR = iL + 1;
+                jR = jL;
+            } else {
+                iL = r % s + 1;
+                jL = r / s;
+                iR = iL;
+                jR = jL + 1;
+            }
+            const double nx = dir == 0 ? 1.0 : 0.0, ny = dir == 0 ? 0.0 : 1.0;
+            double QL[V], QR[V], FLf[2 * V], FRf[2 * V], B9[V], dQ[V];
 #pragma unroll
             for (int v = 0; v < V; v++) {
-                Gr[v] = slx[v * HH + HIX(ii, jj)] / ldx;
-                Gr[V + v] = sly[v * HH + HIX(ii, jj)] / ldy;
-                qh[v] = S0(v, ii, jj) + 0.5 * dt * slt[v * HH + HIX(ii, jj)];
+                QL[v] = wstate(v, iL, jL, dir, 1);
+                QR[v] = wstate(v, iR, jR, dir, 0);
             }
-            P::ncp(m, qh, Gr, src);
-            double Fp_[V], Fm_[V], Gp_[V], Gm_[V], Dpx[V], Dmx[V], Dpy[V], Dmy[V];
-            for (int fc = 0; fc < 4; fc++) {
-                double QL[V], QR[V], FL[2 * V], FR[2 * V], B9[V], dQ[V];
-                const int dir = fc < 2 ? 0 : 1;
-                const double nx = dir == 0 ? 1.0 : 0.0, ny = dir == 0 ? 0.0 : 1.0;
+            const double smax = f_max(P::maxabs_eig(m, QL, nx, ny), P::maxabs_eig(m, QR, nx, ny));
+            P::flux(m, QL, FLf);
+            P::flux(m, QR, FRf);
+            P::roe_col9(m, QL, QR, nx, ny, B9);
 #pragma unroll
-                for (int v = 0; v < V; v++) {
-                    if (fc == 0) { QL[v] = wstate(v, ii, jj, 0, 1); QR[v] = wstate(v, ii + 1, jj, 0, 0); }
-                    else if (fc == 1) { QL[v] = wstate(v, ii - 1, jj, 0, 1); QR[v] = wstate(v, ii, jj, 0, 0); }
-                    else if (fc == 2) { QL[v] = wstate(v, ii, jj, 1, 1); QR[v] = wstate(v, ii, jj + 1, 1, 0); }
-                    else { QL[v] = wstate(v, ii, jj - 1, 1, 1); QR[v] = wstate(v, ii, jj, 1, 0); }
-                }
-                // RusanovFluxS, Baer-Nunziato/limiter.f90:359-383
-                const double smax = f_max(P::maxabs_eig(m, QL, nx, ny), P::maxabs_eig(m, QR, nx, ny));
-                P::flux(m, QL, FL);
-                P::flux(m, QR, FR);
-                P::roe_col9(m, QL, QR, nx, ny, B9);
+            for (int v = 0; v < V; v++) dQ[v] = QR[v] - QL[v];
 #pragma unroll
-                for (int v = 0; v < V; v++) dQ[v] = QR[v] - QL[v];
-#pragma unroll
-                for (int v = 0; v < V; v++) {
-                    double mm = 0.0;
-                    mm = mm + (FL[v] + FR[v]) * nx;
-                    mm = mm + (FL[V + v] + FR[V + v]) * ny;
-                    acc_f[v] = 0.5 * mm - 0.5 * smax * dQ[v];
-                    acc_d[v] = P::roe_times(B9, v, dQ);
-                }
-#pragma unroll
-                for (int v = 0; v < V; v++) {
-                    if (fc == 0) { Fp_[v] = acc_f[v]; Dpx[v] = acc_d[v]; }
-                    else if (fc == 1) { Fm_[v] = acc_f[v]; Dmx[v] = acc_d[v]; }
-                    else if (fc == 2) { Gp_[v] = acc_f[v]; Dpy[v] = acc_d[v]; }
-                    else { Gm_[v] = acc_f[v]; Dmy[v] = acc_d[v]; }
-                }
+            for (int v = 0; v < V; v++) {
+                double mm = 0.0;
+                mm = mm + (FLf[v] + FRf[v]) * nx;
+                mm = mm + (FLf[V + v] + FRf[V + v]) * ny;
+                fF[k][v] = 0.5 * mm - 0.5 * smax * dQ[v];
+                fD[k][v] = P::roe_times(B9, v, dQ);
             }
+        }
+        __syncthreads();  // the slopes are read: their space takes the face values
 #pragma unroll
-            for (int v = 0; v < V; v++)
-                sSub[v * SS + task] = S0(v, ii, jj) - dt / ldx * (Fp_[v] - Fm_[v]) - dt / ldy * (Gp_[v] - Gm_[v]) -
-                                      dt / ldx * 0.5 * (Dpx[v] + Dmx[v]) - dt / ldy * 0.5 * (Dpy[v] + Dmy[v]) +
-                                      dt * src[v];
+        for (int k = 0; k < FP; k++) {
+            const int task = lane + k * NT;
+            if (task >= 2 * NF) continue;
+            const int dir = task / NF, r = task - dir * NF;
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                sFF[(dir * V + v) * NF + r] = fF[k][v];
+                sFD[(dir * V + v) * NF + r] = fD[k][v];
+            }
+        }
+        __syncthreads();
+        // update of the interior subcells, :516-537
+        for (int task = lane; task < SS; task += NT) {
+            const int ii = task % s + 1, jj = task / s + 1;
+            const int xr = ii + (s + 1) * (jj - 1), xl = xr - 1;      // right / left x-face
+            const int yt = (ii - 1) + s * jj, yb = yt - s;            // top / bottom y-face
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                const double Fp_ = sFF[v * NF + xr], Fm_ = sFF[v * NF + xl];
+                const double Gp_ = sFF[(V + v) * NF + yt], Gm_ = sFF[(V + v) * NF + yb];
+                const double Dpx = sFD[v * NF + xr], Dmx = sFD[v * NF + xl];
+                const double Dpy = sFD[(V + v) * NF + yt], Dmy = sFD[(V + v) * NF + yb];
+                const double src = sSub[v * SS + task];
+                sSub[v * SS + task] = S0(v, ii, jj) - dt / ldx * (Fp_ - Fm_) - dt / ldy * (Gp_ - Gm_) -
+                                      dt / ldx * 0.5 * (Dpx + Dmx) - dt / ldy * 0.5 * (Dpy + Dmy) + dt * src;
+            }
         }
 #undef S0
 #undef HIX
         __syncthreads();
         double *sTy = sR0, *sBy = sTy + V * n * s, *sLim = sBy + V * NN, *sLob = sLim + V * SS;
-        subcell_finish<P, N>(m, b, gcell, slot, sa.lh0, sa.lhcap, par, flag, sSub, sUhy, sUh, sTy, sBy, sLim, sLob,
-                             sDt, &sFlag);
+        subcell_finish<P, N, NT>(m, b, gcell, slot, sa.lh0, sa.lhcap, par, flag, sSub, sUhy, sUh, sTy, sBy, sLim,
+                                 sLob, sDt, &sFlag, &sNth);
     }
 }
 
@@ -2341,7 +2383,7 @@ hipError_t launch_step_impl(const StepBufs &b, hipStream_t st, int part, const i
     case kPartSubcell: {
         const int g = b.G < kSubcellGrid ? b.G : kSubcellGrid;
         if (P::kNcp)
-            hipLaunchKernelGGL((k_subcell_bn<N>), dim3(g), dim3(64), 0, st, b);
+            hipLaunchKernelGGL((k_subcell_bn<N>), dim3(g), dim3(kSubBnBlock), 0, st, b);
         else
             hipLaunchKernelGGL((k_subcell<N>), dim3(g), dim3(64), 0, st, b);
         break;
