@@ -54,7 +54,9 @@ PLAIN = {
 PLAIN_TO_T = {"euler_quadrant_1024_n3_t0.3": ("euler_quadrant_1024_n3", 0.3)}
 # name: (N, cells per side, advection velocity, diffusivity, end time)
 AD = {"ad_sine_64_n3": (3, 64, (1.0, 1.0), 0.01, 1.0)}
-ENSEMBLES = {"bn_ensemble_256": dict(count=256, seed=2026)}
+ENSEMBLES = {"bn_ensemble_256": dict(count=256, seed=2026),
+             # the same ensemble on small meshes (side 16..48), where one launch per solve cannot fill the GPU
+             "bn_ensemble_256_small": dict(count=256, seed=2026, side=(16, 48))}
 SERVING = {"serving_4000": dict(count=4000, seed=7, steps=20)}
 
 
@@ -392,7 +394,8 @@ def run_ensemble(args, specs):
         "ms_per_step": batched_s / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (seeded analytic initial conditions)",
         "config": {"workload": args.workload, "pde": "baer-nunziato", "solves": len(specs), "cells": cells,
-                   "N": sorted({s["N"] for s in specs}), "side": [64, 384], "bc": "transmissive",
+                   "N": sorted({s["N"] for s in specs}),
+                   "side": list(ENSEMBLES[args.workload].get("side", (64, 384))), "bc": "transmissive",
                    "seed": ENSEMBLES[args.workload]["seed"], "dof_per_step": dof_step},
         "plain_value": dof_step * K / plain_s if plain_s else None,
         "batched_vs_plain": (plain_s / batched_s) if plain_s else None,
@@ -511,6 +514,8 @@ def main():
     ap.add_argument("--cpu-procs", type=int, default=None, help="reference processes (default min(16, cores))")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="ensemble: skip the plain (one-by-one) leg")
+    ap.add_argument("--no-small", action="store_true",
+                    help="ensemble: skip the batched-vs-plain run of the same seeds on small meshes")
     ap.add_argument("--graph", action="store_true", help="plain: time graph replay instead of evented launches")
     ap.add_argument("--copy-probe-bytes", type=int, default=0,
                     help="after the run, copy this many bytes device to device (counter calibration)")
@@ -555,6 +560,13 @@ def main():
             cpu = cpu_ensemble(specs, args.cpu_seconds, procs)
             log("cpu baseline:", json.dumps(cpu))
         out = run_ensemble(args, specs)
+        if args.workload == "bn_ensemble_256" and not args.no_small and not args.no_plain:
+            # batched vs plain where batching matters: the same seeds on meshes of side 16..48
+            sargs = argparse.Namespace(**dict(vars(args), workload="bn_ensemble_256_small"))
+            small = run_ensemble(sargs, ics.bn_ensemble(**ENSEMBLES["bn_ensemble_256_small"]))
+            out["small_meshes"] = {k: small[k] for k in ("value", "plain_value", "batched_vs_plain", "ms_per_step")}
+            out["small_meshes"]["config"] = small["config"]
+            log("small meshes:", json.dumps(out["small_meshes"]))
         if args.serving_requests:
             sspecs = ics.serving_requests(**dict(SERVING["serving_4000"], count=args.serving_requests))
             out["serving"] = run_serving(sspecs, (0.8,))
