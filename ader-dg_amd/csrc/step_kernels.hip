@@ -1420,49 +1420,47 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
                 __builtin_amdgcn_sched_barrier(0);  // one variable's loads in flight at a time
             }
         }
-        // get_subcell_data :25-35 and positivity :227-241, two subcell points per pass
+        // get_subcell_data :25-35 and positivity :227-241 by row pieces: thread tt < NC takes
+        // row q of the subcell grid (or half of it, when 2s rows fit the cell's threads), forms
+        // the row's first-pass values limy(:, q) once for every variable and then the subcell
+        // values of its points from them (the same sums, in the same order)
         double l0[V], l1[V];
 #pragma unroll
         for (int v = 0; v < V; v++) l0[v] = l1[v] = __builtin_nan("");
-        for (int t0 = tt; t0 < SS; t0 += 2 * NN) {
-            const int t1 = t0 + NN;
-            const bool two = t1 < SS;
-            const int p0 = t0 % s, q0 = t0 / s, p1 = two ? t1 % s : p0, q1 = two ? t1 / s : q0;
-            double cq0[n], cq1[n], cp0[n], cp1[n];
+        constexpr bool HALF = 2 * s <= NN;
+        constexpr int NC = HALF ? 2 * s : s;
+        if (tt < NC) {
+            const int qq = HALF ? tt % s : tt, h = HALF ? tt / s : 0;
+            const int pa = HALF ? h * n : 0, pb = HALF ? (h ? s : n) : s;
+            double cq[n];
 #pragma unroll
-            for (int k = 0; k < n; k++) {
-                cq0[k] = op->u2l[q0 + s * k];
-                cq1[k] = op->u2l[q1 + s * k];
-                cp0[k] = op->u2l[p0 + s * k];
-                cp1[k] = op->u2l[p1 + s * k];
-            }
-            double L0[V], L1[V];
+            for (int k = 0; k < n; k++) cq[k] = op->u2l[qq + s * k];
+            double ly[V][n];  // limy(a, q), :25-27
 #pragma unroll
             for (int v = 0; v < V; v++) {
-                double a0 = 0.0, a1 = 0.0;
 #pragma unroll
-                for (int a = 0; a < n; a++) {
-                    double y0 = 0.0, y1 = 0.0;  // limy(a, q), :25-27
+                for (int a = 0; a < n; a++) ly[v][a] = 0.0;
 #pragma unroll
-                    for (int bb = 0; bb < n; bb++) {
-                        const double u = suh[v * NN + a + n * bb];
-                        y0 = y0 + u * cq0[bb];
-                        y1 = y1 + u * cq1[bb];
-                    }
-                    a0 = a0 + y0 * cp0[a];  // :33-35
-                    a1 = a1 + y1 * cp1[a];
-                }
-                L0[v] = a0;
-                L1[v] = a1;
-                l0[v] = fold_min(l0[v], a0);
-                l1[v] = fold_max(l1[v], a0);
-                if (two) {
-                    l0[v] = fold_min(l0[v], a1);
-                    l1[v] = fold_max(l1[v], a1);
-                }
-                __builtin_amdgcn_sched_barrier(0);  // one variable's loads in flight at a time
+                for (int bb = 0; bb < n; bb++)
+#pragma unroll
+                    for (int a = 0; a < n; a++) ly[v][a] = ly[v][a] + suh[v * NN + a + n * bb] * cq[bb];
             }
-            bad = bad || P::inadmissible(m, L0) || (two && P::inadmissible(m, L1));
+#pragma unroll 1
+            for (int p = pa; p < pb; p++) {
+                double cp[n], L[V];
+#pragma unroll
+                for (int k = 0; k < n; k++) cp[k] = op->u2l[p + s * k];
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    double acc = 0.0;  // :33-35
+#pragma unroll
+                    for (int a = 0; a < n; a++) acc = acc + ly[v][a] * cp[a];
+                    L[v] = acc;
+                    l0[v] = fold_min(l0[v], acc);
+                    l1[v] = fold_max(l1[v], acc);
+                }
+                bad = bad || P::inadmissible(m, L);
+            }
         }
 #pragma unroll
         for (int v = 0; v < V; v++) {
