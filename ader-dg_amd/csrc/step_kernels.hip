@@ -1307,11 +1307,24 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
     __shared__ double ssuh[CPW][V * NN];
     __shared__ unsigned long long skey[CPW][KC][NP];
     __shared__ int sflag[CPW];
+    // the operator rows the threads index by their own node or point, read from LDS
+    // instead of the constant tables (a vector memory access each)
+    __shared__ double sopw[n], sopfl[n], sopfr[n], sopu2b[NN], sopu2l[s * n];
     const OpPtr op = (OpPtr)b.op;
     const int tid = threadIdx.x, lc = tid / NN, tt = tid - lc * NN;
     const int i = tt % n, j = tt / n;
     const bool own = lc < CPW;
     const int lcs = own ? lc : CPW - 1;
+    for (int k = tid; k < s * n; k += kUpdBlock<P>) {
+        sopu2l[k] = op->u2l[k];
+        if (k < NN) sopu2b[k] = op->u2b[k];
+        if (k < n) {
+            sopw[k] = op->w[k];
+            sopfl[k] = op->flc[k];
+            sopfr[k] = op->frc[k];
+        }
+    }
+    __syncthreads();
     const int ntab = ld_const(b.meta + kTabUpd);
     for (int e = xcd_entry(blockIdx.x, gridDim.x); e < ntab; e += gridDim.x) {
     const int2 w = ld_const(b.tab[kTabUpd] + e);
@@ -1366,7 +1379,7 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
         const size_t nfx = b.GX, nfy = b.GY;
         const size_t fl = sa.fx0 + ci + (size_t)(m.imax + 1) * cj, fb = sa.fy0 + ci + (size_t)m.imax * cj;
         const int sp = P::kNcp ? 1 : 0;  // right/top faces take D+ (Baer-Nunziato/main.f90:68)
-        const double wi = op->w[i], wj = op->w[j], dt = st.dt;
+        const double wi = sopw[i], wj = sopw[j], dt = st.dt;
 #pragma unroll
         for (int v = 0; v < V; v++) {
             double d = b.lduh[g * V * NN + V * tt + v];
@@ -1376,8 +1389,8 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
             const double Gb = b.yflux[((size_t)(0 * V + v) * n + i) * nfy + fb];
             const double Gt = b.yflux[((size_t)(sp * V + v) * n + i) * nfy + fb + m.imax];
             // surface integral, Euler/surface_integral.f90:182-196
-            d = d - wj / m.dx * (Fr * op->frc[i] - Fl * op->flc[i]);
-            d = d - wi / m.dy * (Gt * op->frc[j] - Gb * op->flc[j]);
+            d = d - wj / m.dx * (Fr * sopfr[i] - Fl * sopfl[i]);
+            d = d - wi / m.dy * (Gt * sopfr[j] - Gb * sopfl[j]);
             d = d / (wi * wj);  // element update, Euler/element_update.f90:218-226
             un[v] = uo + dt * d;
             b.uh[par ^ 1][g * V * NN + V * tt + v] = un[v];
@@ -1402,8 +1415,8 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
             double ub_q[n], ub_p[n];
 #pragma unroll
             for (int k = 0; k < n; k++) {
-                ub_q[k] = op->u2b[j + n * k];
-                ub_p[k] = op->u2b[i + n * k];
+                ub_q[k] = sopu2b[j + n * k];
+                ub_p[k] = sopu2b[i + n * k];
             }
 #pragma unroll
             for (int v = 0; v < V; v++) {
@@ -1434,7 +1447,7 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
             const int pa = HALF ? h * n : 0, pb = HALF ? (h ? s : n) : s;
             double cq[n];
 #pragma unroll
-            for (int k = 0; k < n; k++) cq[k] = op->u2l[qq + s * k];
+            for (int k = 0; k < n; k++) cq[k] = sopu2l[qq + s * k];
             double ly[V][n];  // limy(a, q), :25-27
 #pragma unroll
             for (int v = 0; v < V; v++) {
@@ -1449,7 +1462,7 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
             for (int p = pa; p < pb; p++) {
                 double cp[n], L[V];
 #pragma unroll
-                for (int k = 0; k < n; k++) cp[k] = op->u2l[p + s * k];
+                for (int k = 0; k < n; k++) cp[k] = sopu2l[p + s * k];
 #pragma unroll
                 for (int v = 0; v < V; v++) {
                     double acc = 0.0;  // :33-35
