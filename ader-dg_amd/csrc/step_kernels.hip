@@ -1294,7 +1294,11 @@ ADG_DEV void key_max(unsigned long long *slot, double x)
 // waves per SIMD asked of k_update: Euler N=3 fits four without spilling (one more than
 // the compiler's own choice, 1.22 -> 1.08 ms on 1024x1024); elsewhere the compiler's choice
 template <class P, int N>
-constexpr int kUpdWaves = (P::V == 4 && N == 3) ? 4 : 1;
+constexpr int kUpdWaves = (P::V == 4 && N == 3) ? 4 : (P::V == 4 ? 1 : 2);
+// contiguous row staging of uh / lduh through LDS in k_update: Baer-Nunziato only (512x512:
+// N=3 update 0.80 -> 0.73 ms, N=2 0.48 -> 0.42 ms; the Euler updates measured slower with it)
+template <class P>
+constexpr bool kUpdRows = P::V != 4;
 
 template <class P, int N, bool SEED>
 __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kUpdWaves<P, N>))) void k_update(StepBufs b)
@@ -1304,7 +1308,14 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
     // KC copies of every slot: the cell's threads spread their atomics over the copies
     // (thread tt takes copy tt % KC), so at most ceil(NN / KC) lanes meet on one address
     constexpr int KC = kUpdKeyCopies;
+    // ROWS (Baer-Nunziato): srow holds uh (then the new solution) of the entry's cells in
+    // the layout of the group arrays ([cell][node][v]), so that global memory is read and
+    // written in whole contiguous rows, and ssuh first the entry's lduh rows; otherwise
+    // each thread reads and writes its node's values directly.  ssuh then holds the new
+    // nodal values [cell][v][node] that the range maps read.
+    constexpr bool ROWS = kUpdRows<P>;
     __shared__ double ssuh[CPW][V * NN];
+    __shared__ double srow[ROWS ? CPW : 1][ROWS ? V * NN : 1];
     __shared__ unsigned long long skey[CPW][KC][NP];
     __shared__ int sflag[CPW];
     // the operator rows the threads index by their own node or point, read from LDS
@@ -1370,7 +1381,65 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
         pmx[k] = ok ? b.lmax[(size_t)v * b.G + g] : 0.0;
     }
 
-    double un[V];
+    double un[V], dtc;
+    if constexpr (ROWS) {
+    // the surface-integral terms of this thread's node first (their loads overlap the row
+    // loads below), Euler/surface_integral.f90:182-196
+    double sx[V], sy[V];
+    const double wi = sopw[i], wj = sopw[j];
+    if (!SEED && act) {
+        const int ci = cell % m.imax, cj = cell / m.imax;
+        const size_t nfx = b.GX, nfy = b.GY;
+        const size_t fl = sa.fx0 + ci + (size_t)(m.imax + 1) * cj, fb = sa.fy0 + ci + (size_t)m.imax * cj;
+        const int sp = P::kNcp ? 1 : 0;  // right/top faces take D+ (Baer-Nunziato/main.f90:68)
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            const double Fl = b.xflux[((size_t)(0 * V + v) * n + j) * nfx + fl];
+            const double Fr = b.xflux[((size_t)(sp * V + v) * n + j) * nfx + fl + 1];
+            const double Gb = b.yflux[((size_t)(0 * V + v) * n + i) * nfy + fb];
+            const double Gt = b.yflux[((size_t)(sp * V + v) * n + i) * nfy + fb + m.imax];
+            sx[v] = wj / m.dx * (Fr * sopfr[i] - Fl * sopfl[i]);
+            sy[v] = wi / m.dy * (Gt * sopfr[j] - Gb * sopfl[j]);
+        }
+    }
+    const int nvalid = m.ncell - w.y < CPW ? m.ncell - w.y : CPW;  // cells of this entry
+    const int cnt = nvalid * V * NN;
+    const size_t base = ((size_t)sa.cell0 + w.y) * V * NN;
+    double *flat = &srow[0][0];
+#pragma unroll 2
+    for (int k = tid; k < cnt; k += kUpdBlock<P>) {
+        flat[k] = uh_cur[base + k];
+        if (!SEED) ssuh[0][k] = b.lduh[base + k];
+    }
+    __syncthreads();  // the entry's uh and lduh rows in LDS
+    if (SEED || !act) {
+#pragma unroll
+        for (int v = 0; v < V; v++) un[v] = act ? srow[lcs][V * tt + v] : (v == 0 || v == 3 ? 1.0 : 0.0);
+    } else {
+        const double dt = st.dt;
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            double d = ssuh[lc][V * tt + v];
+            const double uo = srow[lc][V * tt + v];
+            d = d - sx[v];
+            d = d - sy[v];
+            d = d / (wi * wj);  // element update, Euler/element_update.f90:218-226
+            un[v] = uo + dt * d;
+            srow[lc][V * tt + v] = un[v];  // this thread's own node, read just above
+        }
+    }
+    __syncthreads();  // every thread has read its lduh row: ssuh takes the new values
+    if (own) {
+#pragma unroll
+        for (int v = 0; v < V; v++) suh[v * NN + tt] = un[v];
+    }
+    dtc = act ? node_dt<P>(m, un) : __builtin_nan("");
+    __syncthreads();  // new nodal values and slot initial values in place
+    if (!SEED) {
+#pragma unroll 2
+        for (int k = tid; k < cnt; k += kUpdBlock<P>) b.uh[par ^ 1][base + k] = flat[k];
+    }
+    } else {
     if (SEED || !act) {
 #pragma unroll
         for (int v = 0; v < V; v++) un[v] = act ? uh_cur[g * V * NN + V * tt + v] : (v == 0 || v == 3 ? 1.0 : 0.0);
@@ -1400,8 +1469,9 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
 #pragma unroll
         for (int v = 0; v < V; v++) suh[v * NN + tt] = un[v];
     }
-    const double dtc = act ? node_dt<P>(m, un) : __builtin_nan("");
+    dtc = act ? node_dt<P>(m, un) : __builtin_nan("");
     __syncthreads();  // nodal values and slot initial values in place
+    }
     bool bad = false;
     if (act) {
         // nodal ranges and the time-step candidate (Euler/calc_time_step.f90, limiter.f90:123-132)
