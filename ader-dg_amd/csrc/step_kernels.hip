@@ -409,9 +409,50 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
         for (int l = 0; l < n; l++) q[v][l] = sU[v * NN + tt];
     double fh[2][V], sh[V];
 
-    // one Picard sweep, Euler/space_time_predictor.f90:49-84 (Baer-Nunziato/space_time_predictor.f90:54-104)
-    auto sweep = [&](auto last_tag) {
+    // fluxes of state Q at this thread's node, exchanged with the cell's threads, and the
+    // slice's x / y flux contractions ax, ay (and, Baer-Nunziato, the gradients G)
+    auto slice_terms = [&](const double *Q, double *F, double *ax, double *ay, double *G) {
+        P::flux(m, Q, F);
+        if (own) {
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                buf[v * NN + j * n + i] = F[v];
+                buf[V * NN + v * NN + i * n + j] = F[V + v];
+                if (P::kNcp) buf[2 * V * NN + v * NN + tt] = Q[v];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < n; k++) acc = acc + buf[v * NN + j * n + k] * sK[k + n * i];
+            ax[v] = acc;
+            acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < n; k++) acc = acc + buf[V * NN + v * NN + i * n + k] * sK[k + n * j];
+            ay[v] = acc;
+            if (P::kNcp) {  // gradients, Baer-Nunziato/space_time_predictor.f90:72,79
+                acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + k + n * j] * sD[i + n * k];
+                G[v] = 1.0 / m.dx * acc;
+                acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + i + n * k] * sD[j + n * k];
+                G[V + v] = 1.0 / m.dy * acc;
+            }
+        }
+        __syncthreads();  // the exchange buffer is free for the next slice
+    };
+
+    // one Picard sweep, Euler/space_time_predictor.f90:49-84 (Baer-Nunziato/space_time_predictor.f90:54-104).
+    // FIRST: the first sweep starts from the trivial initial guess, every time slice of the
+    // node holding luh (:27-35), so the fluxes, contractions and NCP source of its slices are
+    // the same numbers: they are formed once and the slices differ only by their weights.
+    auto sweep = [&](auto last_tag, auto first_tag) {
         constexpr bool LAST = decltype(last_tag)::value;
+        constexpr bool FIRST = decltype(first_tag)::value;
         if (LAST) {
 #pragma unroll
             for (int v = 0; v < V; v++) {
@@ -420,6 +461,28 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
                 sh[v] = 0.0;
             }
         }
+        if constexpr (FIRST) {
+            double Q[V], F[2 * V], G[2 * V], ax0[V], ay0[V], src0[V];
+#pragma unroll
+            for (int v = 0; v < V; v++) Q[v] = q[v][0];
+            slice_terms(Q, F, ax0, ay0, G);
+            if (P::kNcp) P::ncp(m, Q, G, src0);  // NCP source, Baer-Nunziato/space_time_predictor.f90:84-92
+            // the right-hand sides of the slices in place of the time line
+#pragma unroll
+            for (int l = 0; l < n; l++) {
+                const double wl = op->w[l];
+                const double sx = wl * wj * dt / m.dx, sy = wl * wi * dt / m.dy;
+                const double c0 = wi * wj * op->f0[l];
+                const double f = wi * wj * wl * dt;
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    const double rr = c0 * sU[v * NN + tt] - sx * ax0[v];  // :66
+                    double r = rr - sy * ay0[v];                           // :72
+                    if (P::kNcp) r = r + f * src0[v];
+                    q[v][l] = r;
+                }
+            }
+        } else {
 #pragma unroll 1
         for (int l = 0; l < n; l++) {
             double Q[V];
@@ -490,6 +553,7 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
                 q[v][n - 1] = rv[v];
             }
         }
+        }
         // time contraction, :79-83: q_new(v,ll) = f1 * sum over m of rhs(v,m) iK1(ll,m)
 #pragma unroll
         for (int v = 0; v < V; v++) {
@@ -505,9 +569,10 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
             for (int ll = 0; ll < n; ll++) q[v][ll] = f1 * qn[ll];
         }
     };
+    sweep(std::false_type{}, std::true_type{});
 #pragma unroll 1
-    for (int it = 0; it < N; it++) sweep(std::false_type{});
-    sweep(std::true_type{});
+    for (int it = 1; it < N; it++) sweep(std::false_type{}, std::false_type{});
+    sweep(std::true_type{}, std::false_type{});
 
     double qh[V];  // :90
 #pragma unroll
