@@ -223,12 +223,80 @@ ADG_DEV void predictor_cells(const MeshArgs &m, const StepBufs &b, const TestBuf
         for (int l = 0; l < n; l++) q[v][l] = u[v];
     }
     int slot = 0;
-    // Picard iterations, Euler/space_time_predictor.f90:49-84 (Baer-Nunziato/space_time_predictor.f90:54-104).
-    // Both loops stay rolled: the time line q rotates so that the current slice is always q[:,0],
-    // and the time contraction with iK1 accumulates slice by slice (summation order over m
-    // unchanged), which keeps the register footprint at two time lines.
+    {
+        // The first Picard sweep starts from the trivial initial guess (every time slice of
+        // the node holds luh, Euler/space_time_predictor.f90:27-35): the fluxes, contractions
+        // and NCP source of its slices are the same numbers, formed once here; the slices
+        // differ only by their weights.
+        double F[2 * V], G[2 * V], ax[V], ay[V], src[V];
+        P::flux(m, u, F);
+        double *buf = SM(0);
+        if (own) {
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                buf[v * NN + j * n + i] = F[v];
+                buf[V * NN + v * NN + i * n + j] = F[V + v];
+                if (P::kNcp) buf[2 * V * NN + v * NN + tt] = u[v];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < n; k++) acc = acc + buf[v * NN + j * n + k] * kx_i[k];
+            ax[v] = acc;
+            acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < n; k++) acc = acc + buf[V * NN + v * NN + i * n + k] * kx_j[k];
+            ay[v] = acc;
+            if (P::kNcp) {  // gradients, Baer-Nunziato/space_time_predictor.f90:72,79
+                acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + k + n * j] * du_i[k];
+                G[v] = 1.0 / m.dx * acc;
+                acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + i + n * k] * du_j[k];
+                G[V + v] = 1.0 / m.dy * acc;
+            }
+        }
+        if (P::kNcp) P::ncp(m, u, G, src);  // NCP source, Baer-Nunziato/space_time_predictor.f90:84-92
+        double qn[V][n];
+#pragma unroll
+        for (int v = 0; v < V; v++)
+#pragma unroll
+            for (int ll = 0; ll < n; ll++) qn[v][ll] = 0.0;
+#pragma unroll
+        for (int l = 0; l < n; l++) {
+            const double wl = op->w[l];
+            const double sx = wl * wj * dt / m.dx, sy = wl * wi * dt / m.dy;
+            const double c0 = wi * wj * op->f0[l];
+            const double f = wi * wj * wl * dt;
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                const double rr = c0 * u[v] - sx * ax[v];  // :66
+                double rv = rr - sy * ay[v];                // :72
+                if (P::kNcp) rv = rv + f * src[v];
+                // time contraction, :79-83, accumulated slice by slice
+#pragma unroll
+                for (int ll = 0; ll < n; ll++) qn[v][ll] = qn[v][ll] + rv * op->ik1[ll + n * l];
+            }
+        }
+        const double f1 = 1.0 / (wi * wj);  // :81
+#pragma unroll
+        for (int v = 0; v < V; v++)
+#pragma unroll
+            for (int ll = 0; ll < n; ll++) q[v][ll] = f1 * qn[v][ll];
+        slot = 1;  // the next exchange goes to the other buffer: SM(0) may still be read
+    }
+    // The remaining Picard iterations, Euler/space_time_predictor.f90:49-84
+    // (Baer-Nunziato/space_time_predictor.f90:54-104).  Both loops stay rolled: the time line q
+    // rotates so that the current slice is always q[:,0], and the time contraction with iK1
+    // accumulates slice by slice (summation order over m unchanged), which keeps the register
+    // footprint at two time lines.
 #pragma unroll 1
-    for (int it = 0; it <= N; it++) {
+    for (int it = 1; it <= N; it++) {
         double qn[V][n];
 #pragma unroll
         for (int v = 0; v < V; v++)
