@@ -188,10 +188,19 @@ int Group::init(int pde_, int N_, const GroupCaps &c, const Operators &ops_, int
     table_quanta(pde, N, variant, quanta);
     const int64_t items[kNumTabs] = {c.cells, c.cells, (c.xfaces + c.yfaces) * n, c.cells};
     int64_t max_entries = 0;
+    // a persistent grid never exceeds the workgroups of its kernel that fit on the CUs at once
+    int cus = 0, dev = 0;
+    if (persist_blocks > 0 && (hipGetDevice(&dev) != hipSuccess ||
+                               hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess))
+        cus = 0;
     for (int k = 0; k < kNumTabs; k++) {
         tab_cap[k] = items[k] / quanta[k] + c.slots + 1;
         max_entries = std::max(max_entries, tab_cap[k]);
-        const int64_t g = persist_blocks > 0 ? std::min<int64_t>(tab_cap[k], persist_blocks) : tab_cap[k];
+        int64_t g = persist_blocks > 0 ? std::min<int64_t>(tab_cap[k], persist_blocks) : tab_cap[k];
+        if (persist_blocks > 0 && cus > 0) {
+            const int r = resident_blocks_per_cu(pde, N, variant, upd, k);
+            if (r > 0) g = std::min<int64_t>(g, (int64_t)r * cus);
+        }
         grid[k] = (int)std::max<int64_t>(1, std::min<int64_t>(g, 1 << 30));
     }
     if (max_entries > (1LL << 30)) return ADERDG_EINVAL;

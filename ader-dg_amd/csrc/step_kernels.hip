@@ -2714,6 +2714,60 @@ int pred_cells_per_block(int pde, int N, int variant)
     return q[kTabPred];
 }
 
+namespace {
+
+template <class F>
+int resident_of(F kernel, int block)
+{
+    int nb = 0;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, 0) == hipSuccess ? nb : 0;
+}
+
+template <class P, int N>
+int resident_impl(int variant, int upd, int tab)
+{
+    using S = Shape<P, N>;
+    switch (tab) {
+    case kTabPred:
+        switch (variant) {
+        case kPredLine: return resident_of(k_predictor<P, N>, S::PRED_BLOCK);
+        case kPredNode64:
+            if constexpr ((N + 1) * (N + 1) * (N + 1) <= 64) return resident_of(k_predictor_nodes<P, N, 64>, 64);
+            return 0;
+        case kPredNode128: return resident_of(k_predictor_nodes<P, N, 128>, 128);
+        case kPredNode256: return resident_of(k_predictor_nodes<P, N, 256>, 256);
+        case kPredLean: return resident_of(k_predictor_lean<P, N>, S::PRED_BLOCK);
+        default: return 0;
+        }
+    case kTabUpd:
+        return upd == kUpdShared ? resident_of(k_update_shared<P, N, false>, S::UPD_BLOCK)
+                                 : resident_of(k_update<P, N, false>, S::UPD_BLOCK);
+    case kTabFace: return resident_of(k_face<P, N>, kBlock);
+    default: return 0;
+    }
+}
+
+template <class P>
+int resident_pde(int N, int variant, int upd, int tab)
+{
+    switch (N) {
+    case 1: return resident_impl<P, 1>(variant, upd, tab);
+    case 2: return resident_impl<P, 2>(variant, upd, tab);
+    case 3: return resident_impl<P, 3>(variant, upd, tab);
+    case 4: return resident_impl<P, 4>(variant, upd, tab);
+    default: return 0;
+    }
+}
+
+}  // namespace
+
+int resident_blocks_per_cu(int pde, int N, int variant, int upd, int tab)
+{
+    if (pde == kPdeBn) return resident_pde<BnPDE>(N, variant, upd, tab);
+    if (pde == kPdeEuler) return resident_pde<EulerPDE>(N, variant, upd, tab);
+    return 0;
+}
+
 void table_quanta(int pde, int N, int variant, int quanta[kNumTabs])
 {
     if (pde == kPdeAd)

@@ -118,6 +118,11 @@ bool pred_variant_valid(int pde, int N, int variant);
 enum { kUpdRecompute = 0, kUpdShared = 1, kNumUpdVariants = 2 };
 int pred_cells_per_block(int pde, int N, int variant);
 
+// Workgroups of the kernel that works through table `tab` that fit on one CU at once
+// (0: not known, or not a table kernel of this PDE).  A persistent grid larger than
+// this times the CU count leaves its last workgroups to a second round on the CUs.
+int resident_blocks_per_cu(int pde, int N, int variant, int upd, int tab);
+
 // Items (cells or face points) per work-table entry of each table, for a (PDE, N)
 // and predictor variant.
 void table_quanta(int pde, int N, int variant, int quanta[kNumTabs]);
