@@ -1782,12 +1782,21 @@ template <class P, int N, int NT, class M>
 ADG_DEV void subcell_finish(const M &m, const StepBufs &b, size_t cell, int sol, int lh0, int lhcap, int par, int flag,
                             const double *sSub,
                             double *sUhy, double *sUh, double *sTy, double *sBy, double *sLim, double *sLob, double *sDt,
-                            int *sFlag, int *sNth)
+                            double *sRng, int *sFlag, int *sNth)
 {
     using S = Shape<P, N>;
     constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, V = S::V;
     const OpPtr op = (OpPtr)b.op;
     const int lane = threadIdx.x;
+    // this step's DMP bounds of the cell, loaded now so their latency overlaps the maps
+    constexpr int KV = (V + NT - 1) / NT;
+    double pmn[KV], pmx[KV];
+#pragma unroll
+    for (int k = 0; k < KV; k++) {
+        const int v = lane + k * NT;
+        pmn[k] = v < V ? b.lmin[(size_t)v * b.G + cell] : 0.0;
+        pmx[k] = v < V ? b.lmax[(size_t)v * b.G + cell] : 0.0;
+    }
     for (int task = lane; task < s * n; task += NT) {
         const int p = task % s, bb = task / s;
 #pragma unroll
@@ -1820,35 +1829,46 @@ ADG_DEV void subcell_finish(const M &m, const StepBufs &b, size_t cell, int sol,
     const bool bad = cell_ranges_stage2<P, N>(op, m, sTy, sBy, sLim, sLob, lane, NT);
     if (bad) atomicOr(sFlag, 1);
     __syncthreads();
-    for (int v = lane; v < V; v += NT) {
-        double r[6];
-        group_ranges<P, N>(sUh, sLob, sLim, v, r);
-        if (dmp_violated(r, b.lmin[(size_t)v * b.G + cell], b.lmax[(size_t)v * b.G + cell])) atomicOr(sFlag, 2);
+    // every range the rest needs, one fold per thread (each in the order of group_ranges):
+    // sRng[kind * V + v], kind 0/1 nodal min/max, 2/3 Lobatto, 4/5 subcell, 6/7 the new
+    // subcell averages (a limited cell's next ranges, :135-139); sRng[8V] the time-step minimum
+    for (int t = lane; t <= 8 * V; t += NT) {
+        double r = __builtin_nan("");
+        if (t == 8 * V) {
+            for (int k = 0; k < NN; k++) r = fold_min(r, sDt[k]);
+        } else {
+            const int kind = t / V, v = t - kind * V, grp = kind >> 1;
+            const bool mx = kind & 1;
+            const double *src = grp == 0 ? sUh + v * NN : (grp == 1 ? sLob + v * NN : (grp == 2 ? sLim + v * SS : sSub + v * SS));
+            const int cnt = grp < 2 ? NN : SS;
+            for (int k = 0; k < cnt; k++) r = mx ? fold_max(r, src[k]) : fold_min(r, src[k]);
+        }
+        sRng[t] = r;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KV; k++) {
+        const int v = lane + k * NT;
+        if (v >= V) continue;
+        const double r[6] = {sRng[v], sRng[V + v], sRng[2 * V + v], sRng[3 * V + v], sRng[4 * V + v], sRng[5 * V + v]};
+        if (dmp_violated(r, pmn[k], pmx[k])) atomicOr(sFlag, 2);
     }
     __syncthreads();
     const int status = (flag == 1) ? 1 : (*sFlag != 0 ? 1 : 0);
     for (int v = lane; v < V; v += NT) {
         double mn, mx;
         if (status) {  // next get_min_max uses Lh (:135-139)
-            mn = __builtin_nan("");
-            mx = mn;
-            for (int k = 0; k < SS; k++) {
-                mn = fold_min(mn, sSub[v * SS + k]);
-                mx = fold_max(mx, sSub[v * SS + k]);
-            }
+            mn = sRng[6 * V + v];
+            mx = sRng[7 * V + v];
         } else {
-            double r[6];
-            group_ranges<P, N>(sUh, sLob, sLim, v, r);
-            mn = f_min(f_min(r[0], r[2]), r[4]);
-            mx = f_max(f_max(r[1], r[3]), r[5]);
+            mn = f_min(f_min(sRng[v], sRng[2 * V + v]), sRng[4 * V + v]);
+            mx = f_max(f_max(sRng[V + v], sRng[3 * V + v]), sRng[5 * V + v]);
         }
         b.lmin_old[(size_t)v * b.G + cell] = mn;
         b.lmax_old[(size_t)v * b.G + cell] = mx;
     }
     if (lane == NT - 1) {
-        double d = __builtin_nan("");
-        for (int k = 0; k < NN; k++) d = fold_min(d, sDt[k]);
-        b.cell_dt[cell] = d;
+        b.cell_dt[cell] = sRng[8 * V];
         b.status[par ^ 1][cell] = status;
     }
     if (status) {  // NewLh in a pool slot of the solve, and Lh = NewLh at update_limiter (:287)
@@ -1886,6 +1906,7 @@ __global__ __launch_bounds__(64) void k_subcell(StepBufs b)
     __shared__ double sUh[V * NN];
     __shared__ double sTy[V * n * s], sBy[V * NN], sLim[V * SS], sLob[V * NN];
     __shared__ double sDt[NN];
+    __shared__ double sRng[8 * V + 1];
     __shared__ int sFlag, sNth;
     const OpPtr op = (OpPtr)b.op;
     const int lane = threadIdx.x;
@@ -2075,7 +2096,7 @@ __global__ __launch_bounds__(64) void k_subcell(StepBufs b)
 #undef S0
         __syncthreads();
         subcell_finish<P, N, 64>(m, b, gcell, slot, sa.lh0, sa.lhcap, par, flag, sSub, sUhy, sUh, sTy, sBy, sLim, sLob,
-                                 sDt, &sFlag, &sNth);
+                                 sDt, sRng, &sFlag, &sNth);
     }
 }
 
@@ -2113,6 +2134,7 @@ __global__ __launch_bounds__(kSubBnBlock) __attribute__((amdgpu_waves_per_eu(2))
     __shared__ double sUhy[V * s * n];
     __shared__ double sUh[V * NN];
     __shared__ double sDt[NN];
+    __shared__ double sRng[8 * V + 1];
     __shared__ int sFlag, sNth;
     const OpPtr op = (OpPtr)b.op;
     const int lane = threadIdx.x;
@@ -2305,7 +2327,7 @@ __global__ __launch_bounds__(kSubBnBlock) __attribute__((amdgpu_waves_per_eu(2))
         __syncthreads();
         double *sTy = sR0, *sBy = sTy + V * n * s, *sLim = sBy + V * NN, *sLob = sLim + V * SS;
         subcell_finish<P, N, NT>(m, b, gcell, slot, sa.lh0, sa.lhcap, par, flag, sSub, sUhy, sUh, sTy, sBy, sLim,
-                                 sLob, sDt, &sFlag, &sNth);
+                                 sLob, sDt, sRng, &sFlag, &sNth);
     }
 }
 
