@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per k_predictor launch, from rocprofv3 --pmc FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0),
 # scaled by a 256 MiB device copy; profiles/hbm_traffic_*.md holds the runs.
-PREDICTOR_TRAFFIC_BYTES = {"euler_vortex_512_n4": 965.7e6, "bn_ensemble_256_n3": 32.51e9}
+PREDICTOR_TRAFFIC_BYTES = {"euler_vortex_512_n4": 965.7e6, "bn_ensemble_256_n3": 34.76e9}
 
 PLAIN = {
     # name: (ic, N, imax, jmax, xl, xr, bc)
