@@ -1404,13 +1404,15 @@ ADG_DEV double key_val(unsigned long long k)
     const unsigned long long u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
     return __longlong_as_double((long long)u);
 }
+// A NaN folds in as the slot's untouched value (~0 for a min, 0 for a max), which leaves
+// the slot as it is: no branch around the atomic.
 ADG_DEV void key_min(unsigned long long *slot, double x)
 {
-    if (!__builtin_isnan(x)) atomicMin(slot, ord_key(x));
+    atomicMin(slot, __builtin_isnan(x) ? ~0ull : ord_key(x));
 }
 ADG_DEV void key_max(unsigned long long *slot, double x)
 {
-    if (!__builtin_isnan(x)) atomicMax(slot, ord_key(x));
+    atomicMax(slot, __builtin_isnan(x) ? 0ull : ord_key(x));
 }
 
 // ---------------------------------------------------------------------------
