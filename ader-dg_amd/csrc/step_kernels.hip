@@ -1420,11 +1420,13 @@ ADG_DEV void key_max(unsigned long long *slot, double x)
 // per-cell limiter seeds of a freshly uploaded solution (SEED = true).
 // Thread (cell, i, j) updates node (i,j).  The ranges the DMP needs (min/max
 // of nodal, Lobatto and subcell values) come from the cell's nodal values in
-// LDS: each thread forms the first (y) pass of the separable subcell and
-// Lobatto maps for the columns its points need, in the order of
-// get_subcell_data / get_lobatto_data, and folds its values into per-cell
-// slots with LDS atomics on order-preserving keys.  Only the nodal values and
-// the slots live in LDS (1.6 KB per Baer-Nunziato N=3 cell).
+// LDS: each thread forms its node's Lobatto value and, for a row (or half row)
+// of the subcell grid, the row's first-pass values and then its points, in the
+// order of get_subcell_data / get_lobatto_data, and folds its values into
+// per-cell slots (four copies each) with LDS atomics on order-preserving keys.
+// The operator rows the threads index live in LDS as well, and for
+// Baer-Nunziato the entry's uh and lduh rows are staged there (row loads and
+// stores instead of node-strided ones).
 // ---------------------------------------------------------------------------
 // waves per SIMD asked of k_update: Euler N=3 fits four without spilling (one more than
 // the compiler's own choice, 1.22 -> 1.08 ms on 1024x1024); elsewhere the compiler's choice
