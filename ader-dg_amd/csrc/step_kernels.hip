@@ -1455,6 +1455,7 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
     __shared__ double srow[ROWS ? CPW : 1][ROWS ? V * NN : 1];
     __shared__ unsigned long long skey[CPW][KC][NP];
     __shared__ int sflag[CPW];
+    __shared__ double sdt[CPW][NN];  // time-step candidates of the nodes
     // the operator rows the threads index by their own node or point, read from LDS
     // instead of the constant tables (a vector memory access each)
     __shared__ double sopw[n], sopfl[n], sopfr[n], sopu2b[NN], sopu2l[s * n];
@@ -1571,6 +1572,7 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
         for (int v = 0; v < V; v++) suh[v * NN + tt] = un[v];
     }
     dtc = act ? node_dt<P>(m, un) : __builtin_nan("");
+    if (own) sdt[lc][tt] = dtc;
     __syncthreads();  // new nodal values and slot initial values in place
     if (!SEED) {
 #pragma unroll 2
@@ -1607,17 +1609,12 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
         for (int v = 0; v < V; v++) suh[v * NN + tt] = un[v];
     }
     dtc = act ? node_dt<P>(m, un) : __builtin_nan("");
+    if (own) sdt[lc][tt] = dtc;
     __syncthreads();  // nodal values and slot initial values in place
     }
     bool bad = false;
     if (act) {
-        // nodal ranges and the time-step candidate (Euler/calc_time_step.f90, limiter.f90:123-132)
-#pragma unroll
-        for (int v = 0; v < V; v++) {
-            key_min(&key[0 * V + v], un[v]);
-            key_max(&key[1 * V + v], un[v]);
-        }
-        key_min(&key[6 * V], dtc);
+        // (the nodal ranges and the time-step minimum are folded from LDS below)
         {  // get_lobatto_data :56-66 at this thread's node (p, q) = (i, j)
             double ub_q[n], ub_p[n];
 #pragma unroll
@@ -1694,8 +1691,14 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
     for (int k = 0; k < KV; k++) {
         const int v = tt + k * NN;
         if (!act || v >= V) continue;
-        const double r[6] = {slot_val(0 * V + v), slot_val(1 * V + v), slot_val(2 * V + v),
-                             slot_val(3 * V + v), slot_val(4 * V + v), slot_val(5 * V + v)};
+        // nodal range, Euler/limiter.f90:123-132, from the new nodal values in LDS
+        double u0 = __builtin_nan(""), u1 = u0;
+#pragma unroll
+        for (int k2 = 0; k2 < NN; k2++) {
+            u0 = fold_min(u0, suh[v * NN + k2]);
+            u1 = fold_max(u1, suh[v * NN + k2]);
+        }
+        const double r[6] = {u0, u1, slot_val(2 * V + v), slot_val(3 * V + v), slot_val(4 * V + v), slot_val(5 * V + v)};
         if (!SEED && dmp_violated(r, pmn[k], pmx[k])) atomicOr(&sflag[lc], 2);
         // next step's get_min_max step 1 for a cell without limiter, Euler/limiter.f90:123-132
         double mn = f_min(f_min(r[0], r[2]), r[4]), mx = f_max(f_max(r[1], r[3]), r[5]);
@@ -1712,7 +1715,12 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
         b.lmin_old[(size_t)v * b.G + g] = mn;
         b.lmax_old[(size_t)v * b.G + g] = mx;
     }
-    if (act && tt == NN - 1) b.cell_dt[g] = slot_val(6 * V);
+    if (act && tt == NN - 1) {  // the cell's time-step candidate, Euler/calc_time_step.f90:29-42
+        double d = __builtin_nan("");
+#pragma unroll
+        for (int k2 = 0; k2 < NN; k2++) d = fold_min(d, sdt[lc][k2]);
+        b.cell_dt[g] = d;
+    }
     __syncthreads();
     if (!SEED && act && tt == 0) {
         const int tr = sflag[lc] != 0;
