@@ -1456,6 +1456,8 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
     __shared__ unsigned long long skey[CPW][KC][NP];
     __shared__ int sflag[CPW];
     __shared__ double sdt[CPW][NN];  // time-step candidates of the nodes
+    // Lobatto values [v][node] of the cells (Baer-Nunziato: in srow once its rows are stored)
+    __shared__ double slob[ROWS ? 1 : CPW][ROWS ? 1 : V * NN];
     // the operator rows the threads index by their own node or point, read from LDS
     // instead of the constant tables (a vector memory access each)
     __shared__ double sopw[n], sopfl[n], sopfr[n], sopu2b[NN], sopu2l[s * n];
@@ -1487,6 +1489,11 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
     const bool act = own && cell < m.ncell;
     const double *uh_cur = b.uh[par];
     double *suh = &ssuh[lcs][0];
+    double *lobL;
+    if constexpr (ROWS)
+        lobL = &srow[lcs][0];
+    else
+        lobL = &slob[lcs][0];
     unsigned long long *key = &skey[lcs][tt % KC][0];  // [kind * V + v], kind 0/1 nodal min/max,
                                                        // 2/3 Lobatto, 4/5 subcell; [6V] dt
     if (own) {
@@ -1578,6 +1585,7 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
 #pragma unroll 2
         for (int k = tid; k < cnt; k += kUpdBlock<P>) b.uh[par ^ 1][base + k] = flat[k];
     }
+    __syncthreads();  // the rows are read: srow takes the Lobatto values
     } else {
     if (SEED || !act) {
 #pragma unroll
@@ -1632,8 +1640,7 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
                     for (int bb = 0; bb < n; bb++) ly = ly + suh[v * NN + a + n * bb] * ub_q[bb];
                     acc = acc + ly * ub_p[a];  // :64-66
                 }
-                key_min(&key[2 * V + v], acc);
-                key_max(&key[3 * V + v], acc);
+                lobL[v * NN + tt] = acc;
                 __builtin_amdgcn_sched_barrier(0);  // one variable's loads in flight at a time
             }
         }
@@ -1698,7 +1705,13 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
             u0 = fold_min(u0, suh[v * NN + k2]);
             u1 = fold_max(u1, suh[v * NN + k2]);
         }
-        const double r[6] = {u0, u1, slot_val(2 * V + v), slot_val(3 * V + v), slot_val(4 * V + v), slot_val(5 * V + v)};
+        double b0 = __builtin_nan(""), b1 = b0;  // Lobatto range, :64-66
+#pragma unroll
+        for (int k2 = 0; k2 < NN; k2++) {
+            b0 = fold_min(b0, lobL[v * NN + k2]);
+            b1 = fold_max(b1, lobL[v * NN + k2]);
+        }
+        const double r[6] = {u0, u1, b0, b1, slot_val(4 * V + v), slot_val(5 * V + v)};
         if (!SEED && dmp_violated(r, pmn[k], pmx[k])) atomicOr(&sflag[lc], 2);
         // next step's get_min_max step 1 for a cell without limiter, Euler/limiter.f90:123-132
         double mn = f_min(f_min(r[0], r[2]), r[4]), mx = f_max(f_max(r[1], r[3]), r[5]);
