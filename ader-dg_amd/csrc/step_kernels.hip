@@ -1441,7 +1441,8 @@ template <class P, int N, bool SEED>
 __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kUpdWaves<P, N>))) void k_update(StepBufs b)
 {
     using S = Shape<P, N>;
-    constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, CPW = S::CPWU, V = S::V, NP = S::NPART;
+    constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, CPW = S::CPWU, V = S::V;
+    constexpr int NP = 2 * V;  // LDS range slots per cell: subcell min (v), max (V + v)
     // KC copies of every slot: the cell's threads spread their atomics over the copies
     // (thread tt takes copy tt % KC), so at most ceil(NN / KC) lanes meet on one address
     constexpr int KC = kUpdKeyCopies;
@@ -1494,18 +1495,17 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
         lobL = &srow[lcs][0];
     else
         lobL = &slob[lcs][0];
-    unsigned long long *key = &skey[lcs][tt % KC][0];  // [kind * V + v], kind 0/1 nodal min/max,
-                                                       // 2/3 Lobatto, 4/5 subcell; [6V] dt
+    unsigned long long *key = &skey[lcs][tt % KC][0];  // [v] subcell min, [V + v] subcell max
     if (own) {
         if (tt == 0) sflag[lc] = 0;
         for (int k = tt; k < KC * NP; k += NN) {
             const int slot = k % NP;
-            skey[lc][k / NP][slot] = (slot < 6 * V && ((slot / V) & 1)) ? 0ull : ~0ull;
+            skey[lc][k / NP][slot] = slot >= V ? 0ull : ~0ull;
         }
     }
     // value of a slot: the min (max) over the copies of the order-preserving keys
     auto slot_val = [&](int slot) {
-        const bool mx = slot < 6 * V && ((slot / V) & 1);
+        const bool mx = slot >= V;
         unsigned long long r = skey[lcs][0][slot];
 #pragma unroll
         for (int c = 1; c < KC; c++) {
@@ -1688,8 +1688,8 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
         }
 #pragma unroll
         for (int v = 0; v < V; v++) {
-            key_min(&key[4 * V + v], l0[v]);
-            key_max(&key[5 * V + v], l1[v]);
+            key_min(&key[v], l0[v]);
+            key_max(&key[V + v], l1[v]);
         }
     }
     if (bad && !SEED) atomicOr(&sflag[lc], 1);
@@ -1711,7 +1711,7 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
             b0 = fold_min(b0, lobL[v * NN + k2]);
             b1 = fold_max(b1, lobL[v * NN + k2]);
         }
-        const double r[6] = {u0, u1, b0, b1, slot_val(4 * V + v), slot_val(5 * V + v)};
+        const double r[6] = {u0, u1, b0, b1, slot_val(v), slot_val(V + v)};
         if (!SEED && dmp_violated(r, pmn[k], pmx[k])) atomicOr(&sflag[lc], 2);
         // next step's get_min_max step 1 for a cell without limiter, Euler/limiter.f90:123-132
         double mn = f_min(f_min(r[0], r[2]), r[4]), mx = f_max(f_max(r[1], r[3]), r[5]);
