@@ -1452,6 +1452,9 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
     // each thread reads and writes its node's values directly.  ssuh then holds the new
     // nodal values [cell][v][node] that the range maps read.
     constexpr bool ROWS = kUpdRows<P>;
+    // FACES (ROWS with at most four cells per workgroup): the entry's face lines are staged
+    // in LDS, one lane per value, instead of each node loading its own row's values
+    constexpr bool FACES = ROWS && CPW <= 4;
     __shared__ double ssuh[CPW][V * NN];
     __shared__ double srow[ROWS ? CPW : 1][ROWS ? V * NN : 1];
     __shared__ unsigned long long skey[CPW][KC][NP];
@@ -1459,6 +1462,7 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
     __shared__ double sdt[CPW][NN];  // time-step candidates of the nodes
     // Lobatto values [v][node] of the cells (Baer-Nunziato: in srow once its rows are stored)
     __shared__ double slob[ROWS ? 1 : CPW][ROWS ? 1 : V * NN];
+    __shared__ double sface[FACES ? 4 * V * n * CPW : 1];  // the entry's face lines
     // the operator rows the threads index by their own node or point, read from LDS
     // instead of the constant tables (a vector memory access each)
     __shared__ double sopw[n], sopfl[n], sopfr[n], sopu2b[NN], sopu2l[s * n];
@@ -1528,11 +1532,12 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
 
     double un[V], dtc;
     if constexpr (ROWS) {
+    const int nvalid = m.ncell - w.y < CPW ? m.ncell - w.y : CPW;  // cells of this entry
+    const double wi = sopw[i], wj = sopw[j];
     // the surface-integral terms of this thread's node first (their loads overlap the row
     // loads below), Euler/surface_integral.f90:182-196
     double sx[V], sy[V];
-    const double wi = sopw[i], wj = sopw[j];
-    if (!SEED && act) {
+    if (!FACES && !SEED && act) {
         const int ci = cell % m.imax, cj = cell / m.imax;
         const size_t nfx = b.GX, nfy = b.GY;
         const size_t fl = sa.fx0 + ci + (size_t)(m.imax + 1) * cj, fb = sa.fy0 + ci + (size_t)m.imax * cj;
@@ -1547,7 +1552,33 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
             sy[v] = wi / m.dy * (Gt * sopfr[j] - Gb * sopfl[j]);
         }
     }
-    const int nvalid = m.ncell - w.y < CPW ? m.ncell - w.y : CPW;  // cells of this entry
+    // FACES: the face lines of the entry's cells in LDS, one lane per value: a cell's n
+    // nodes of a row (column) read the same left/right (bottom/top) value, which the loads
+    // above issue n times.  sface[side][v][p][cell]; sides left, right, bottom, top.
+    if (FACES && !SEED) {
+        const size_t nfx = b.GX, nfy = b.GY;
+        const int sp = P::kNcp ? 1 : 0;  // right/top faces take D+ (Baer-Nunziato/main.f90:68)
+        const int ci0 = w.y % m.imax, cj0 = w.y / m.imax;
+#pragma unroll 2
+        for (int k = tid; k < 4 * V * n * CPW; k += kUpdBlock<P>) {
+            const int c = k % CPW, r = k / CPW, p = r % n, fv = r / n, v = fv % V, f = fv / V;
+            if (c >= nvalid) continue;
+            int ci = ci0 + c, cj = cj0;
+            while (ci >= m.imax) {  // the entry's cells may run on into the next row(s)
+                ci -= m.imax;
+                cj++;
+            }
+            double x;
+            if (f < 2) {
+                const size_t fl = sa.fx0 + ci + (size_t)(m.imax + 1) * cj;
+                x = b.xflux[((size_t)((f ? sp : 0) * V + v) * n + p) * nfx + fl + f];
+            } else {
+                const size_t fb = sa.fy0 + ci + (size_t)m.imax * cj;
+                x = b.yflux[((size_t)((f == 3 ? sp : 0) * V + v) * n + p) * nfy + fb + (f == 3 ? m.imax : 0)];
+            }
+            sface[k] = x;
+        }
+    }
     const int cnt = nvalid * V * NN;
     const size_t base = ((size_t)sa.cell0 + w.y) * V * NN;
     double *flat = &srow[0][0];
@@ -1556,14 +1587,23 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
         flat[k] = uh_cur[base + k];
         if (!SEED) ssuh[0][k] = b.lduh[base + k];
     }
-    __syncthreads();  // the entry's uh and lduh rows in LDS
+    __syncthreads();  // the entry's uh and lduh rows (and face lines) in LDS
     if (SEED || !act) {
 #pragma unroll
         for (int v = 0; v < V; v++) un[v] = act ? srow[lcs][V * tt + v] : (v == 0 || v == 3 ? 1.0 : 0.0);
     } else {
         const double dt = st.dt;
+        constexpr int FS = V * n * CPW;  // doubles per side in sface
 #pragma unroll
         for (int v = 0; v < V; v++) {
+            if constexpr (FACES) {  // surface-integral terms, Euler/surface_integral.f90:182-196
+                const double Fl = sface[0 * FS + (v * n + j) * CPW + lc];
+                const double Fr = sface[1 * FS + (v * n + j) * CPW + lc];
+                const double Gb = sface[2 * FS + (v * n + i) * CPW + lc];
+                const double Gt = sface[3 * FS + (v * n + i) * CPW + lc];
+                sx[v] = wj / m.dx * (Fr * sopfr[i] - Fl * sopfl[i]);
+                sy[v] = wi / m.dy * (Gt * sopfr[j] - Gb * sopfl[j]);
+            }
             double d = ssuh[lc][V * tt + v];
             const double uo = srow[lc][V * tt + v];
             d = d - sx[v];
