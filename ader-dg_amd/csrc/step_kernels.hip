@@ -1555,28 +1555,26 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
     // FACES: the face lines of the entry's cells in LDS, one lane per value: a cell's n
     // nodes of a row (column) read the same left/right (bottom/top) value, which the loads
     // above issue n times.  sface[side][v][p][cell]; sides left, right, bottom, top.
+    // Lane tid takes cell c = tid % CPW of the entry and the values r = tid / CPW + k FG of
+    // its lines (r = (side V + v) n + p), so its faces are located once.
     if (FACES && !SEED) {
-        const size_t nfx = b.GX, nfy = b.GY;
-        const int sp = P::kNcp ? 1 : 0;  // right/top faces take D+ (Baer-Nunziato/main.f90:68)
-        const int ci0 = w.y % m.imax, cj0 = w.y / m.imax;
-#pragma unroll 2
-        for (int k = tid; k < 4 * V * n * CPW; k += kUpdBlock<P>) {
-            const int c = k % CPW, r = k / CPW, p = r % n, fv = r / n, v = fv % V, f = fv / V;
-            if (c >= nvalid) continue;
-            int ci = ci0 + c, cj = cj0;
-            while (ci >= m.imax) {  // the entry's cells may run on into the next row(s)
-                ci -= m.imax;
-                cj++;
+        constexpr int FG = kUpdBlock<P> / CPW;  // lanes per cell
+        const int c = tid % CPW;
+        if (tid < FG * CPW && c < nvalid) {
+            const size_t nfx = b.GX, nfy = b.GY;
+            const int sp = P::kNcp ? 1 : 0;  // right/top faces take D+ (Baer-Nunziato/main.f90:68)
+            const int lcell = w.y + c, ci = lcell % m.imax, cj = lcell / m.imax;
+            const size_t fl = sa.fx0 + ci + (size_t)(m.imax + 1) * cj, fb = sa.fy0 + ci + (size_t)m.imax * cj;
+#pragma unroll 3
+            for (int r = tid / CPW; r < 4 * V * n; r += FG) {
+                const int p = r % n, fv = r / n, v = fv % V, f = fv / V;
+                double x;
+                if (f < 2)
+                    x = b.xflux[((size_t)((f ? sp : 0) * V + v) * n + p) * nfx + fl + f];
+                else
+                    x = b.yflux[((size_t)((f == 3 ? sp : 0) * V + v) * n + p) * nfy + fb + (f == 3 ? m.imax : 0)];
+                sface[r * CPW + c] = x;
             }
-            double x;
-            if (f < 2) {
-                const size_t fl = sa.fx0 + ci + (size_t)(m.imax + 1) * cj;
-                x = b.xflux[((size_t)((f ? sp : 0) * V + v) * n + p) * nfx + fl + f];
-            } else {
-                const size_t fb = sa.fy0 + ci + (size_t)m.imax * cj;
-                x = b.yflux[((size_t)((f == 3 ? sp : 0) * V + v) * n + p) * nfy + fb + (f == 3 ? m.imax : 0)];
-            }
-            sface[k] = x;
         }
     }
     const int cnt = nvalid * V * NN;
