@@ -1455,8 +1455,8 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
     // FACES (ROWS with at most four cells per workgroup): the entry's face lines are staged
     // in LDS, one lane per value, instead of each node loading its own row's values
     constexpr bool FACES = ROWS && CPW <= 4;
-    __shared__ double ssuh[CPW][V * NN];
-    __shared__ double srow[ROWS ? CPW : 1][ROWS ? V * NN : 1];
+    __shared__ __align__(16) double ssuh[CPW][V * NN];
+    __shared__ __align__(16) double srow[ROWS ? CPW : 1][ROWS ? V * NN : 1];
     __shared__ unsigned long long skey[CPW][KC][NP];
     __shared__ int sflag[CPW];
     __shared__ double sdt[CPW][NN];  // time-step candidates of the nodes
@@ -1580,10 +1580,23 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
     const int cnt = nvalid * V * NN;
     const size_t base = ((size_t)sa.cell0 + w.y) * V * NN;
     double *flat = &srow[0][0];
+    // PAIRS: a cell's row has an even number of doubles, so the rows of every entry start
+    // 16-byte aligned and move as double pairs
+    constexpr bool PAIRS = (V * NN) % 2 == 0;
+    if constexpr (PAIRS) {
+        const double2 *u2 = (const double2 *)(uh_cur + base), *l2 = (const double2 *)(b.lduh + base);
+        double2 *f2 = (double2 *)flat, *s2 = (double2 *)&ssuh[0][0];
 #pragma unroll 2
-    for (int k = tid; k < cnt; k += kUpdBlock<P>) {
-        flat[k] = uh_cur[base + k];
-        if (!SEED) ssuh[0][k] = b.lduh[base + k];
+        for (int k = tid; k < cnt / 2; k += kUpdBlock<P>) {
+            f2[k] = u2[k];
+            if (!SEED) s2[k] = l2[k];
+        }
+    } else {
+#pragma unroll 2
+        for (int k = tid; k < cnt; k += kUpdBlock<P>) {
+            flat[k] = uh_cur[base + k];
+            if (!SEED) ssuh[0][k] = b.lduh[base + k];
+        }
     }
     __syncthreads();  // the entry's uh and lduh rows (and face lines) in LDS
     if (SEED || !act) {
@@ -1620,8 +1633,15 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
     if (own) sdt[lc][tt] = dtc;
     __syncthreads();  // new nodal values and slot initial values in place
     if (!SEED) {
+        if constexpr (PAIRS) {
+            double2 *o2 = (double2 *)(b.uh[par ^ 1] + base);
+            const double2 *f2 = (const double2 *)flat;
 #pragma unroll 2
-        for (int k = tid; k < cnt; k += kUpdBlock<P>) b.uh[par ^ 1][base + k] = flat[k];
+            for (int k = tid; k < cnt / 2; k += kUpdBlock<P>) o2[k] = f2[k];
+        } else {
+#pragma unroll 2
+            for (int k = tid; k < cnt; k += kUpdBlock<P>) b.uh[par ^ 1][base + k] = flat[k];
+        }
     }
     __syncthreads();  // the rows are read: srow takes the Lobatto values
     } else {
