@@ -1581,21 +1581,38 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
     const size_t base = ((size_t)sa.cell0 + w.y) * V * NN;
     double *flat = &srow[0][0];
     // PAIRS: a cell's row has an even number of doubles, so the rows of every entry start
-    // 16-byte aligned and move as double pairs
+    // 16-byte aligned and move as double pairs.  Otherwise the rows move as pairs from the
+    // first 16-byte boundary: h leading doubles (0 or 1) before it, and one trailing double
+    // when the rest is odd.
     constexpr bool PAIRS = (V * NN) % 2 == 0;
+    const int h = PAIRS ? 0 : (int)(base & 1), np = (cnt - h) / 2;
     if constexpr (PAIRS) {
         const double2 *u2 = (const double2 *)(uh_cur + base), *l2 = (const double2 *)(b.lduh + base);
         double2 *f2 = (double2 *)flat, *s2 = (double2 *)&ssuh[0][0];
 #pragma unroll 2
-        for (int k = tid; k < cnt / 2; k += kUpdBlock<P>) {
+        for (int k = tid; k < np; k += kUpdBlock<P>) {
             f2[k] = u2[k];
             if (!SEED) s2[k] = l2[k];
         }
     } else {
+        const double2 *u2 = (const double2 *)(uh_cur + base + h), *l2 = (const double2 *)(b.lduh + base + h);
+        double *sl = &ssuh[0][0];
 #pragma unroll 2
-        for (int k = tid; k < cnt; k += kUpdBlock<P>) {
-            flat[k] = uh_cur[base + k];
-            if (!SEED) ssuh[0][k] = b.lduh[base + k];
+        for (int k = tid; k < np; k += kUpdBlock<P>) {
+            const double2 x = u2[k];
+            flat[h + 2 * k] = x.x;
+            flat[h + 2 * k + 1] = x.y;
+            if (!SEED) {
+                const double2 y = l2[k];
+                sl[h + 2 * k] = y.x;
+                sl[h + 2 * k + 1] = y.y;
+            }
+        }
+        // the odd ends: the leading double (h = 1) and the trailing one ((cnt - h) odd)
+        const int e = tid == 0 ? 0 : cnt - 1;
+        if ((tid == 0 && h) || (tid == 1 && ((cnt - h) & 1))) {
+            flat[e] = uh_cur[base + e];
+            if (!SEED) sl[e] = b.lduh[base + e];
         }
     }
     __syncthreads();  // the entry's uh and lduh rows (and face lines) in LDS
@@ -1639,8 +1656,11 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
 #pragma unroll 2
             for (int k = tid; k < cnt / 2; k += kUpdBlock<P>) o2[k] = f2[k];
         } else {
+            double2 *o2 = (double2 *)(b.uh[par ^ 1] + base + h);
 #pragma unroll 2
-            for (int k = tid; k < cnt; k += kUpdBlock<P>) b.uh[par ^ 1][base + k] = flat[k];
+            for (int k = tid; k < np; k += kUpdBlock<P>) o2[k] = make_double2(flat[h + 2 * k], flat[h + 2 * k + 1]);
+            const int e = tid == 0 ? 0 : cnt - 1;
+            if ((tid == 0 && h) || (tid == 1 && ((cnt - h) & 1))) b.uh[par ^ 1][base + e] = flat[e];
         }
     }
     __syncthreads();  // the rows are read: srow takes the Lobatto values
