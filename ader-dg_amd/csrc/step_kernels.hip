@@ -1425,8 +1425,9 @@ ADG_DEV void key_max(unsigned long long *slot, double x)
 // order of get_subcell_data / get_lobatto_data, and folds its values into
 // per-cell slots (four copies each) with LDS atomics on order-preserving keys.
 // The operator rows the threads index live in LDS as well, and for
-// Baer-Nunziato the entry's uh and lduh rows are staged there (row loads and
-// stores instead of node-strided ones).
+// Baer-Nunziato the entry's uh and lduh rows are staged there (16-byte row
+// loads and stores instead of node-strided ones), and at N >= 3 its face lines
+// (one lane per value instead of one load per node).
 // ---------------------------------------------------------------------------
 // waves per SIMD asked of k_update: Euler N=3 fits four without spilling (one more than
 // the compiler's own choice, 1.22 -> 1.08 ms on 1024x1024); elsewhere the compiler's choice
