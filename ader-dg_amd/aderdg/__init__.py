@@ -152,10 +152,13 @@ def ic_bn_disc(v_in, v_out, radius=0.2, xc=0.5, yc=0.5):
     return ic_spec("bn_disc", xc, yc, radius, *v_in, *v_out)
 
 
-def ic_bn_circle(radius=0.2, pratio=1.0e3, xc=0.5, yc=0.5):
+def ic_bn_circle(radius=0.2, pratio=1.0e3, xc=0.5, yc=0.5, phi_eps=None):
+    """Circle of ics.bn_circle: solid volume fraction 1 - phi_eps inside, phi_eps outside
+    (default ics.BN_PHI_EPS)."""
     from .ics import BN_DENSITY, BN_P_OUT, BN_PHI_EPS
+    e = BN_PHI_EPS if phi_eps is None else float(phi_eps)
     return ic_spec("bn_circle", xc, yc, radius, pratio * BN_P_OUT, BN_P_OUT, BN_DENSITY[0], BN_DENSITY[1],
-                   1.0 - BN_PHI_EPS, BN_PHI_EPS)
+                   1.0 - e, e)
 
 
 def _desc(pde, N, imax, jmax, xl, xr, bc, gamma, cfl, device, eos, ad=None):
@@ -245,6 +248,8 @@ def load_library(exact=False):
         "aderdg_batch_finished": [vp, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
         "aderdg_batch_download": [vp, ctypes.c_int, _dp],
         "aderdg_batch_limiter": [vp, ctypes.c_int, _ip, _ip],
+        "aderdg_batch_limiter_state": [vp, ctypes.c_int, _ip, _ip, _dp, _dp, _dp],
+        "aderdg_batch_health": [vp, ctypes.c_int, ctypes.POINTER(i64), ctypes.POINTER(i64)],
         "aderdg_batch_remove": [vp, ctypes.c_int],
         "aderdg_batch_usage": [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(i64)],
         "aderdg_batch_timing_enable": [vp, ctypes.c_int],
@@ -615,6 +620,25 @@ class Batch:
         rc = np.zeros((I, J), dtype=np.int32, order="F")
         self._check(self.lib.aderdg_batch_limiter(self.h, i, st.ctypes.data_as(_ip), rc.ctypes.data_as(_ip)))
         return st, rc
+
+    def limiter_state(self, i):
+        """status, recompute, Lh, lmin, lmax of a solve, in the layouts of Solver.get_limiter."""
+        V, n, _, I, J = self.shapes[i]
+        s = 2 * n - 1
+        st = np.zeros((I, J), dtype=np.int32, order="F")
+        rc = np.zeros((I, J), dtype=np.int32, order="F")
+        lh = np.zeros((V, s, s, I, J), order="F")
+        mn = np.zeros((V, I, J), order="F")
+        mx = np.zeros((V, I, J), order="F")
+        self._check(self.lib.aderdg_batch_limiter_state(self.h, i, st.ctypes.data_as(_ip), rc.ctypes.data_as(_ip),
+                                                        _p(lh), _p(mn), _p(mx)))
+        return dict(status=st, recompute=rc, Lh=lh, lmin=mn, lmax=mx)
+
+    def health(self, i):
+        """(non-finite DOFs of the current solution, troubled cells of the last step)."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.aderdg_batch_health(self.h, i, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def error(self, i):
         """0, or Batch.ECAPACITY when the solve ran out of limited-cell pool slots."""

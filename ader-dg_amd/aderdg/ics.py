@@ -114,6 +114,12 @@ def cell_averages(uh, w):
 BN_DENSITY = (1000.0, 1.0)   # rho_s, rho_g on both sides of the interface
 BN_P_OUT = 1.0e5             # pressure outside the circle (both phases)
 BN_PHI_EPS = 1.0e-6          # volume fraction of the vanishing phase
+# Minority volume fraction of the stable interface (solid 0.7 inside, 0.3 outside).  With
+# 1e-6 / 1-1e-6 the reference scheme drives the vanishing phase's velocity up until the
+# time step collapses within ~30 steps (a 1e-16 perturbation of the initial state grows
+# past 1e-12 within two steps); with 0.3 / 0.7 the same equation of state, radii and
+# pressure ratios advance stably (>= 120 steps at N = 2, 3) and stay well-conditioned.
+BN_PHI_STABLE = 0.3
 
 
 def bn_prim2cons(eos, V):
@@ -132,9 +138,9 @@ def bn_prim2cons(eos, V):
     return Q
 
 
-def bn_circle(X, Y, eos, radius=0.2, pratio=1.0e3, xc=0.5, yc=0.5):
-    """Circular material interface at rest: solid-dominated (phi_s = 1 - 1e-6) inside the
-    circle at pressure pratio * 1e5, gas-dominated (phi_s = 1e-6) outside at 1e5, both
+def bn_circle(X, Y, eos, radius=0.2, pratio=1.0e3, xc=0.5, yc=0.5, phi_eps=BN_PHI_EPS):
+    """Circular material interface at rest: solid-dominated (phi_s = 1 - phi_eps) inside the
+    circle at pressure pratio * 1e5, gas-dominated (phi_s = phi_eps) outside at 1e5, both
     phases at rest with densities BN_DENSITY.  Not at reference HEAD."""
     inside = (X - xc) ** 2 + (Y - yc) ** 2 <= radius * radius
     V = np.zeros((9,) + X.shape)
@@ -143,7 +149,7 @@ def bn_circle(X, Y, eos, radius=0.2, pratio=1.0e3, xc=0.5, yc=0.5):
     p = np.where(inside, pratio * BN_P_OUT, BN_P_OUT)
     V[3] = p
     V[7] = p
-    V[8] = np.where(inside, 1.0 - BN_PHI_EPS, BN_PHI_EPS)
+    V[8] = np.where(inside, 1.0 - phi_eps, phi_eps)
     return bn_prim2cons(eos, V)
 
 
@@ -175,11 +181,12 @@ def bn_disc(X, Y, eos, v_in, v_out, radius=0.2, xc=0.5, yc=0.5):
 BN_EOS_ENSEMBLE = (4.4, 1.4, 6.0e8, 0.0)  # GAMMA_S, GAMMA_G, PI_S, PI_G: stiffened-gas solid, ideal gas
 
 
-def bn_ensemble(count=256, seed=2026, side=(64, 384), degrees=(2, 3)):
+def bn_ensemble(count=256, seed=2026, side=(64, 384), degrees=(2, 3), phi_eps=BN_PHI_EPS):
     """Seeded ensemble of Baer-Nunziato circle problems on [0,1]^2, transmissive:
     mesh side log-uniform in `side`, degree uniform over `degrees`, radius
-    U(0.1, 0.3), pressure ratio log-uniform in [1e3, 1e4].  Returns specs
-    (dicts of N, imax, jmax, radius, pratio); initial_uh_bn builds each state."""
+    U(0.1, 0.3), pressure ratio log-uniform in [1e3, 1e4], minority volume
+    fraction phi_eps.  Returns specs (dicts of N, imax, jmax, radius, pratio,
+    phi_eps); initial_uh_bn builds each state."""
     rng = np.random.default_rng(seed)
     lo, hi = side
     out = []
@@ -187,7 +194,7 @@ def bn_ensemble(count=256, seed=2026, side=(64, 384), degrees=(2, 3)):
         s = int(round(lo * (hi / lo) ** rng.uniform()))
         N = int(rng.choice(degrees))
         out.append(dict(N=N, imax=s, jmax=s, radius=float(rng.uniform(0.1, 0.3)),
-                        pratio=float(10.0 ** rng.uniform(3.0, 4.0))))
+                        pratio=float(10.0 ** rng.uniform(3.0, 4.0)), phi_eps=float(phi_eps)))
     return out
 
 
@@ -225,8 +232,9 @@ def serving_requests(count=4000, seed=7, steps=20, side=(32, 512), degrees=(2, 3
     vortices on [0,10]^2 periodic with random strength, centre and background
     velocity, half four-quadrant Riemann problems on [0,1]^2 transmissive with the
     Lax-Liu 3 states scaled by U(0.8,1.2) and a random split point), 30%
-    Baer-Nunziato discs (the two states of the reference's Riemann problem 1 or
-    2 inside / outside a circle of random radius and centre, ideal-gas phases);
+    Baer-Nunziato stiffened-gas interfaces (the ensemble's equation of state, a
+    circle of random radius U(0.1,0.3) and centre, pressure ratio log-uniform
+    1e3..1e4, volume fractions 0.7 inside / 0.3 outside);
     mesh side
     log-uniform in `side`, N uniform over `degrees`.  Each request's end time is
     `steps` initial time steps (estimated from its initial state), so requests of
@@ -266,12 +274,13 @@ def serving_requests(count=4000, seed=7, steps=20, side=(32, 512), degrees=(2, 3
         else:
             radius = rng.uniform(0.1, 0.3)
             xc, yc = rng.uniform(0.4, 0.6, 2)
-            v_in, v_out = BN_RP_STATES[int(rng.integers(1, 3))]
+            pratio = 10.0 ** rng.uniform(3.0, 4.0)
             xl, xr = (0.0, 0.0), (1.0, 1.0)
             X, Y = np.meshgrid((np.arange(n_est) + 0.5) / n_est, (np.arange(n_est) + 0.5) / n_est)
-            q = bn_disc(X, Y, BN_EOS_REFERENCE, v_in, v_out, radius, xc, yc)
-            params = (xc, yc, radius) + tuple(v_in) + tuple(v_out)
-            req = dict(pde="bn", bc="transmissive", kind="bn_disc", params=params, eos=BN_EOS_REFERENCE)
+            q = bn_circle(X, Y, BN_EOS_ENSEMBLE, radius, pratio, xc, yc, phi_eps=BN_PHI_STABLE)
+            params = (xc, yc, radius, pratio * BN_P_OUT, BN_P_OUT, BN_DENSITY[0], BN_DENSITY[1],
+                      1.0 - BN_PHI_STABLE, BN_PHI_STABLE)
+            req = dict(pde="bn", bc="transmissive", kind="bn_circle", params=params, eos=BN_EOS_ENSEMBLE)
         dt = _dt_estimate(req["pde"], N, s, xl, xr, q, req["eos"])
         req.update(N=N, side=s, xl=xl, xr=xr, tend=float(steps * dt), steps=steps)
         out.append(req)

@@ -16,6 +16,31 @@
 
 using namespace aderdg;
 
+namespace {
+
+// Health of one solve: out[0] += non-finite DOFs of uh, out[1] += troubled cells
+// (recompute 1 of the last step).  Grid-stride over the solve's DOFs and cells.
+__global__ void k_solve_health(const double *__restrict__ uh, const int *__restrict__ rc_final, long long ndof,
+                               int ncell, unsigned long long *out)
+{
+    unsigned long long bad = 0, troubled = 0;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < ndof; k += stride)
+        bad += __builtin_isfinite(uh[k]) ? 0ull : 1ull;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < ncell; k += stride)
+        troubled += rc_final[k] == 1 ? 1ull : 0ull;
+    for (int o = 32; o > 0; o >>= 1) {
+        bad += __shfl_down(bad, o);
+        troubled += __shfl_down(troubled, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (bad) atomicAdd(out, bad);
+        if (troubled) atomicAdd(out + 1, troubled);
+    }
+}
+
+}  // namespace
+
 namespace aderdg {
 
 int Batch::init(int device_, int persist_)
@@ -51,6 +76,8 @@ void Batch::release()
     for (auto p : pinned_states)
         if (p) (void)hipHostFree(p);
     pinned_states.clear();
+    if (health_buf) (void)hipFree(health_buf);
+    health_buf = nullptr;
     hipEvent_t evs[] = {upload_done, removed, states_ready};
     for (auto e : evs)
         if (e) (void)hipEventDestroy(e);
@@ -292,7 +319,7 @@ int Batch::download(int id, double *uh)
     return 0;
 }
 
-int Batch::limiter(int id, int32_t *status, int32_t *recompute)
+int Batch::limiter(int id, int32_t *status, int32_t *recompute, double *Lh, double *lmin, double *lmax)
 {
     auto it = solves.find(id);
     if (it == solves.end()) return fail(ADERDG_EINVAL, "aderdg_batch_limiter: unknown id %d", id);
@@ -301,10 +328,48 @@ int Batch::limiter(int id, int32_t *status, int32_t *recompute)
     Group &g = *groups[it->second.group];
     const int sl = it->second.slot;
     const int par = pinned_states[it->second.group][sl].parity;
-    const size_t c0 = g.cell0(sl), nc = g.mesh(sl).ncell;
-    if (status) HIPCHK(hipMemcpyAsync(status, g.b.status[par] + c0, nc * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    const size_t c0 = g.cell0(sl), nc = g.mesh(sl).ncell, V = g.V, G = g.b.G;
+    const size_t blk = (size_t)g.V * g.s * g.s;
+    const SolveArgs &sa = g.args(sl);
+    std::vector<int32_t> st(nc), slot;
+    std::vector<double> pool, mn, mx;
+    HIPCHK(hipMemcpyAsync(st.data(), g.b.status[par] + c0, nc * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
     if (recompute) HIPCHK(hipMemcpyAsync(recompute, g.b.rc_final + c0, nc * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    if (Lh && sa.lhcap > 0) {  // the solve's pool segment and the slots of its limited cells
+        slot.resize(nc);
+        pool.resize((size_t)sa.lhcap * blk);
+        HIPCHK(hipMemcpyAsync(slot.data(), g.b.lh_slot[par] + c0, nc * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(pool.data(), g.b.lh[par] + (size_t)sa.lh0 * blk, pool.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, stream));
+    }
+    // lmin/lmax are [v][cell] planes over the arena
+    for (size_t v = 0; v < V && (lmin || lmax); v++) {
+        if (lmin) {
+            mn.resize(V * nc);
+            HIPCHK(hipMemcpyAsync(mn.data() + v * nc, g.b.lmin + v * G + c0, nc * sizeof(double), hipMemcpyDeviceToHost,
+                                  stream));
+        }
+        if (lmax) {
+            mx.resize(V * nc);
+            HIPCHK(hipMemcpyAsync(mx.data() + v * nc, g.b.lmax + v * G + c0, nc * sizeof(double), hipMemcpyDeviceToHost,
+                                  stream));
+        }
+    }
     HIPCHK(hipStreamSynchronize(stream));
+    for (size_t k = 0; k < nc; k++) {
+        if (status) status[k] = st[k];
+        if (Lh) {
+            const int ls = st[k] && !slot.empty() ? slot[k] - sa.lh0 : -1;
+            if (ls >= 0 && ls < sa.lhcap)
+                std::memcpy(Lh + k * blk, pool.data() + (size_t)ls * blk, blk * sizeof(double));
+            else
+                std::memset(Lh + k * blk, 0, blk * sizeof(double));
+        }
+        for (size_t v = 0; v < V; v++) {
+            if (lmin) lmin[v + V * k] = mn[v * nc + k];
+            if (lmax) lmax[v + V * k] = mx[v * nc + k];
+        }
+    }
     return 0;
 }
 
@@ -315,6 +380,31 @@ int Batch::error(int id, int *err)
     int r = snapshot();
     if (r) return r;
     *err = pinned_states[it->second.group][it->second.slot].failed ? ADERDG_ECAPACITY : 0;
+    return 0;
+}
+
+int Batch::health(int id, int64_t *nonfinite, int64_t *troubled)
+{
+    auto it = solves.find(id);
+    if (it == solves.end()) return fail(ADERDG_EINVAL, "aderdg_batch_health: unknown id %d", id);
+    int r = snapshot();
+    if (r) return r;
+    if (!health_buf) HIPCHK(hipMalloc((void **)&health_buf, 2 * sizeof(unsigned long long)));
+    Group &g = *groups[it->second.group];
+    const int sl = it->second.slot;
+    const int par = pinned_states[it->second.group][sl].parity;
+    const size_t c0 = g.cell0(sl), nc = g.mesh(sl).ncell, blk = (size_t)g.V * g.n * g.n;
+    HIPCHK(hipMemsetAsync(health_buf, 0, 2 * sizeof(unsigned long long), stream));
+    const long long ndof = (long long)(nc * blk);
+    const int blocks = (int)std::min<long long>(1024, (ndof + 255) / 256);
+    hipLaunchKernelGGL(k_solve_health, dim3(blocks), dim3(256), 0, stream, g.b.uh[par] + c0 * blk,
+                       g.b.rc_final + c0, ndof, (int)nc, health_buf);
+    HIPCHK(hipGetLastError());
+    unsigned long long h[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(h, health_buf, sizeof(h), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    if (nonfinite) *nonfinite = (int64_t)h[0];
+    if (troubled) *troubled = (int64_t)h[1];
     return 0;
 }
 
@@ -460,6 +550,19 @@ int aderdg_batch_limiter(aderdg_batch *b, int id, int32_t *status, int32_t *reco
 {
     if (!b) return fail(ADERDG_EINVAL, "aderdg_batch_limiter: null batch");
     return b->eng.limiter(id, status, recompute);
+}
+
+int aderdg_batch_limiter_state(aderdg_batch *b, int id, int32_t *status, int32_t *recompute, double *Lh,
+                               double *lmin, double *lmax)
+{
+    if (!b) return fail(ADERDG_EINVAL, "aderdg_batch_limiter_state: null batch");
+    return b->eng.limiter(id, status, recompute, Lh, lmin, lmax);
+}
+
+int aderdg_batch_health(aderdg_batch *b, int id, int64_t *nonfinite, int64_t *troubled)
+{
+    if (!b) return fail(ADERDG_EINVAL, "aderdg_batch_health: null batch");
+    return b->eng.health(id, nonfinite, troubled);
 }
 
 int aderdg_batch_remove(aderdg_batch *b, int id)

@@ -55,8 +55,13 @@ public:
     int snapshot();
     const SolveState *state_of(int id) const;  // from the last snapshot
     int download(int id, double *uh);
-    int limiter(int id, int32_t *status, int32_t *recompute);
+    // Limiter state of a solve in host layout (see aderdg_get_limiter); outputs may be null.
+    int limiter(int id, int32_t *status, int32_t *recompute, double *Lh = nullptr, double *lmin = nullptr,
+                double *lmax = nullptr);
     int error(int id, int *err);  // ADERDG_ECAPACITY: the solve ran out of limited-cell pool slots
+    // Non-finite DOFs of a solve's current solution and its troubled cells of the last step
+    int health(int id, int64_t *nonfinite, int64_t *troubled);
+    unsigned long long *health_buf = nullptr;
     int remove(int id);
     int group_of(int pde, int N) const;
     static int64_t footprint(int pde, int N, int imax, int jmax, double lh_fraction = 1.0);

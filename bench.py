@@ -19,6 +19,15 @@ the timed steps, a DOF update being one coefficient (v,i,j) of one cell
 advanced one step (nVar*(N+1)^2 per cell-step).  Inputs are resident in HBM
 before the timed region starts; uploads are not timed.
 
+The line's `config.state` reports what the timed steps did to the solves: the
+time step of the last timed step (min / median / max over the solves), the
+solves holding non-finite values, the troubled-cell fraction.  The configured
+interface (volume fractions 1e-6 / 1-1e-6) collapses under the reference scheme:
+its time step falls by 10-20 orders of magnitude within ~30 steps.  The default
+run therefore also times `bn_ensemble_256_stable` (`stable`): the same seeds,
+meshes, degrees, equation of state, radii and pressure ratios with volume
+fractions 0.3 / 0.7, which the reference scheme advances stably.
+
 Workloads `euler_vortex_512_n4` and `euler_quadrant_1024_n3` time one plain
 solve; `euler_quadrant_1024_n3_t0.3` runs the quadrant to its end time T=0.3.  `ad_sine_64_n3` runs the linear advection-diffusion problem (periodic
 [0,1]^2, 64x64 cells, N=3, c=(1,1), diffusivity 0.01, u0 = sin 2 pi x sin 2 pi y,
@@ -56,8 +65,22 @@ PLAIN_TO_T = {"euler_quadrant_1024_n3_t0.3": ("euler_quadrant_1024_n3", 0.3)}
 AD = {"ad_sine_64_n3": (3, 64, (1.0, 1.0), 0.01, 1.0)}
 ENSEMBLES = {"bn_ensemble_256": dict(count=256, seed=2026),
              # the same ensemble on small meshes (side 16..48), where one launch per solve cannot fill the GPU
-             "bn_ensemble_256_small": dict(count=256, seed=2026, side=(16, 48))}
+             "bn_ensemble_256_small": dict(count=256, seed=2026, side=(16, 48)),
+             # the same ensemble with volume fractions 0.3 / 0.7 (ics.BN_PHI_STABLE), which advances stably
+             "bn_ensemble_256_stable": dict(count=256, seed=2026, phi_eps="stable")}
+
+
+def ensemble_specs(name, count=None):
+    from aderdg import ics
+    cfg = dict(ENSEMBLES[name])
+    if count:
+        cfg["count"] = count
+    if cfg.get("phi_eps") == "stable":
+        cfg["phi_eps"] = ics.BN_PHI_STABLE
+    return ics.bn_ensemble(**cfg)
 SERVING = {"serving_4000": dict(count=4000, seed=7, steps=20)}
+SERVING_MIX = ("35% Euler isentropic vortex, 35% Euler four-quadrant Riemann problem, 30% Baer-Nunziato "
+               "stiffened-gas interface (circle, volume fractions 0.3 / 0.7)")
 
 
 def log(*a):
@@ -126,42 +149,79 @@ def cpu_ad(wl, target_seconds):
 
 
 def _cpu_bn_worker(args):
-    """One reference Baer-Nunziato solve of the ensemble distribution on a 64x64 mesh."""
+    """One reference Baer-Nunziato solve of an ensemble member on its own mesh: steps until
+    target_seconds have passed (at least one step)."""
     spec, target_seconds = args
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import aderdg
     from aderdg import ics
     import pyoracle
-    N = spec["N"]
+    N, I, J = spec["N"], spec["imax"], spec["jmax"]
     eos = ics.BN_EOS_ENSEMBLE
-    uh = ics.initial_uh_bn(aderdg.operators(N)["xi"], 64, 64, (0.0, 0.0), (1.0, 1.0), eos,
-                           radius=spec["radius"], pratio=spec["pratio"])
+    uh = ics.initial_uh_bn(aderdg.operators(N)["xi"], I, J, (0.0, 0.0), (1.0, 1.0), eos,
+                           radius=spec["radius"], pratio=spec["pratio"], phi_eps=spec["phi_eps"])
     if pyoracle.RefBN.available(N):
-        s, kind = pyoracle.RefBN(N, 64, 64, (0.0, 0.0), (1.0, 1.0), eos), "reference"
+        s, kind = pyoracle.RefBN(N, I, J, (0.0, 0.0), (1.0, 1.0), eos), "reference"
     else:
-        s, kind = pyoracle.Oracle(N, 64, 64, (0.0, 0.0), (1.0, 1.0), pyoracle.BC_TRANSMISSIVE, pde=1, eos=eos), "port"
+        s, kind = pyoracle.Oracle(N, I, J, (0.0, 0.0), (1.0, 1.0), pyoracle.BC_TRANSMISSIVE, pde=1, eos=eos), "port"
     s.set_uh(uh)
     s.set_scalars(0.0, 0.0, 1e9)
     steps, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < target_seconds:
+    while steps == 0 or time.perf_counter() - t0 < target_seconds:
         s.step()
         steps += 1
-    return steps * 64 * 64 * 9 * (N + 1) ** 2, time.perf_counter() - t0, kind
+    return N, I, steps * I * J * 9 * (N + 1) ** 2, time.perf_counter() - t0, kind
 
 
-def cpu_ensemble(specs, target_seconds, procs):
+def host_cpu():
+    """CPU model of this host, and the cores this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return model, avail
+
+
+# Reference processes of a CPU baseline: one per core, at most 16 -- the host cores a
+# one-GPU job is given on the measurement hosts, whatever the machine has in total.
+CPU_PROC_CAP = 16
+
+
+def cpu_ensemble(specs, target_seconds, procs, seed=5):
     """One reference process per core (the reference is single-threaded), each on one
-    ensemble member's initial condition sampled on a 64x64 mesh; runs before any GPU use."""
-    sample = [specs[k % len(specs)] for k in range(procs)]
-    with mp.get_context("fork").Pool(procs) as pool:
+    ensemble member drawn at random, on that member's own mesh; runs before any GPU use.
+    value = the processes' DOF/s summed (the host running `procs` members at once)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    pick = rng.choice(len(specs), size=min(procs, len(specs)), replace=False)
+    sample = [specs[int(k)] for k in pick]
+    with mp.get_context("fork").Pool(len(sample)) as pool:
         res = pool.map(_cpu_bn_worker, [(s, target_seconds) for s in sample])
-    value = sum(d / e for d, e, _ in res)
-    kinds = {k for _, _, k in res}
-    return {"value": value, "unit": "element-DOF updates/s", "cores": procs,
-            "kind": "reference" if kinds == {"reference"} else "port",
-            "sample": "%d processes, one per core, each one ensemble member (N=%s) on 64x64 cells for %.0f s"
-                      % (procs, "/".join(str(s["N"]) for s in sample[:4]) + ("..." if procs > 4 else ""),
-                         target_seconds)}
+    value = sum(d / e for _, _, d, e, _ in res)
+    kinds = {k for *_, k in res}
+    model, avail = host_cpu()
+    per_n = {}
+    for N, _, d, e, _ in res:
+        a = per_n.setdefault(str(N), [0.0, 0])
+        a[0] += d / e
+        a[1] += 1
+    return {"value": value, "unit": "element-DOF updates/s", "cores": len(sample), "kind":
+            "reference" if kinds == {"reference"} else "port",
+            "cpu_model": model, "cores_available": avail, "cores_used": len(sample),
+            "per_core_dof_per_s_by_N": {N: v / c for N, (v, c) in per_n.items()},
+            "sample": "%d processes, one per core (at most %d: the host cores of a one-GPU job), each one "
+                      "randomly drawn ensemble member on its own mesh (sides %s, N %s) for >= %.0f s and at "
+                      "least one step" % (len(sample), CPU_PROC_CAP, "/".join(str(s["imax"]) for s in sample),
+                                          "/".join(str(s["N"]) for s in sample), target_seconds)}
 
 
 # ---------------------------------------------------------------------------
@@ -349,6 +409,30 @@ def run_ad(args):
     }
 
 
+def ensemble_state(b, ids, specs, steps_done):
+    """What the steps so far did to the solves: every solve took steps_done steps and none
+    failed; the time step of its last step; solves with non-finite DOFs; troubled cells."""
+    import numpy as np
+    dts, times, bad_solves, bad_dofs, troubled = [], [], 0, 0, 0
+    for i in ids:
+        q = b.query(i)
+        assert q["steps"] == steps_done and b.error(i) == 0, (i, q, steps_done)
+        nf, tr = b.health(i)
+        dts.append(q["dt"])
+        times.append(q["time"])
+        bad_solves += 1 if (nf or not np.isfinite(q["dt"])) else 0
+        bad_dofs += nf
+        troubled += tr
+    cells = sum(s["imax"] * s["jmax"] for s in specs)
+    d = np.array(dts)
+    fin = d[np.isfinite(d)]
+    return {"steps": steps_done, "last_dt_min": float(fin.min()) if fin.size else None,
+            "last_dt_median": float(np.median(fin)) if fin.size else None,
+            "last_dt_max": float(fin.max()) if fin.size else None,
+            "time_median": float(np.median(times)), "solves_nonfinite": bad_solves, "dofs_nonfinite": int(bad_dofs),
+            "troubled_fraction": troubled / cells}
+
+
 def run_ensemble(args, specs):
     import aderdg
     from aderdg import ics
@@ -357,7 +441,7 @@ def run_ensemble(args, specs):
 
     def uh_of(sp):
         return ics.initial_uh_bn(xis[sp["N"]], sp["imax"], sp["jmax"], (0.0, 0.0), (1.0, 1.0), eos,
-                                 radius=sp["radius"], pratio=sp["pratio"])
+                                 radius=sp["radius"], pratio=sp["pratio"], phi_eps=sp["phi_eps"])
 
     K, W, T = args.steps, args.warmup, 2
     dof_step = sum(9 * (s["N"] + 1) ** 2 * s["imax"] * s["jmax"] for s in specs)
@@ -395,6 +479,8 @@ def run_ensemble(args, specs):
     b.step(K)
     b.sync()
     batched_s = time.perf_counter() - t0
+    state = ensemble_state(b, ids, specs, W + K)
+    log("ensemble state after the timed steps:", json.dumps(state))
     b.timing(True)
     b.step(T)
     b.sync()
@@ -405,6 +491,9 @@ def run_ensemble(args, specs):
         per_group[N] = dict(cells=gcells, **{name: (ms[k] / ln[k] if ln[k] else None)
                                               for k, name in enumerate(aderdg.TIMING_SLOTS)})
     done = b.finished()
+    for i in ids:
+        q = b.query(i)
+        assert q["steps"] == W + K + T and not q["active"] and b.error(i) == 0, (i, q)
     b.close()
     log("batched per-kernel average ms per arena:", json.dumps(per_group))
     assert len(done) == len(ids), "every solve should have stopped after W+K+T steps"
@@ -424,7 +513,8 @@ def run_ensemble(args, specs):
         "config": {"workload": args.workload, "pde": "baer-nunziato", "solves": len(specs), "cells": cells,
                    "N": sorted({s["N"] for s in specs}),
                    "side": list(ENSEMBLES[args.workload].get("side", (64, 384))), "bc": "transmissive",
-                   "seed": ENSEMBLES[args.workload]["seed"], "dof_per_step": dof_step},
+                   "seed": ENSEMBLES[args.workload]["seed"], "dof_per_step": dof_step,
+                   "volume_fractions": [specs[0]["phi_eps"], 1.0 - specs[0]["phi_eps"]], "state": state},
         "plain_value": dof_step * K / plain_s if plain_s else None,
         "batched_vs_plain": (plain_s / batched_s) if plain_s else None,
         "roofline": roof,
@@ -535,15 +625,19 @@ def main():
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20 ensemble, 200 plain)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (default 2 ensemble, 10 plain)")
     ap.add_argument("--workload", default="bn_ensemble_256", choices=sorted(PLAIN) + sorted(PLAIN_TO_T) + sorted(AD) + sorted(ENSEMBLES) + sorted(SERVING))
-    ap.add_argument("--serving-requests", type=int, default=400,
-                    help="ensemble workload: requests of the serving stream timed beside it (0: none)")
+    ap.add_argument("--serving-requests", type=int, default=1000,
+                    help="ensemble workload: requests of the serving stream timed beside it at 0.5x, 0.8x and 1.0x "
+                         "of saturation (0: none)")
     ap.add_argument("--count", type=int, default=None, help="ensemble size (default 256)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--cpu-procs", type=int, default=None, help="reference processes (default min(16, cores))")
+    ap.add_argument("--cpu-procs", type=int, default=None,
+                    help="reference processes (default: one per available core, at most %d)" % CPU_PROC_CAP)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="ensemble: skip the plain (one-by-one) leg")
     ap.add_argument("--no-small", action="store_true",
                     help="ensemble: skip the batched-vs-plain run of the same seeds on small meshes")
+    ap.add_argument("--no-stable", action="store_true",
+                    help="ensemble: skip the run of the same seeds with volume fractions 0.3 / 0.7")
     ap.add_argument("--graph", action="store_true", help="plain: time graph replay instead of evented launches")
     ap.add_argument("--copy-probe-bytes", type=int, default=0,
                     help="after the run, copy this many bytes device to device (counter calibration)")
@@ -572,32 +666,38 @@ def main():
                "scaling": "weak", "vs_baseline": None, "dtype": "f64",
                "data": "synthetic (seeded formula initial conditions)",
                "config": {"workload": args.workload, "requests": cfg["count"], "seed": cfg["seed"],
-                          "mix": "35% Euler vortex, 35% Euler quadrant, 30% Baer-Nunziato disc (reference Riemann states 1/2)",
+                          "mix": SERVING_MIX,
                           "side": [32, 512], "N": [2, 3, 4], "steps_per_request": cfg["steps"]},
                "serving": sv}
         print(json.dumps(out), flush=True)
         return
     if ens:
         from aderdg import ics
-        cfg = dict(ENSEMBLES[args.workload])
-        if args.count:
-            cfg["count"] = args.count
-        specs = ics.bn_ensemble(**cfg)
+        specs = ensemble_specs(args.workload, args.count)
         if not args.no_cpu:  # before any GPU use: the workers are forked
-            procs = args.cpu_procs or max(1, min(16, os.cpu_count() or 1))
+            procs = args.cpu_procs or max(1, min(CPU_PROC_CAP, host_cpu()[1]))
             cpu = cpu_ensemble(specs, args.cpu_seconds, procs)
             log("cpu baseline:", json.dumps(cpu))
         out = run_ensemble(args, specs)
+        if args.workload == "bn_ensemble_256" and not args.no_stable:
+            # the same seeds on a state the reference scheme advances stably, batched and plain
+            sargs = argparse.Namespace(**dict(vars(args), workload="bn_ensemble_256_stable"))
+            st = run_ensemble(sargs, ensemble_specs("bn_ensemble_256_stable", args.count))
+            out["stable"] = {k: st[k] for k in ("value", "plain_value", "batched_vs_plain", "ms_per_step", "roofline")}
+            out["stable"]["config"] = st["config"]
+            log("stable interface:", json.dumps(out["stable"]))
         if args.workload == "bn_ensemble_256" and not args.no_small and not args.no_plain:
             # batched vs plain where batching matters: the same seeds on meshes of side 16..48
             sargs = argparse.Namespace(**dict(vars(args), workload="bn_ensemble_256_small"))
-            small = run_ensemble(sargs, ics.bn_ensemble(**ENSEMBLES["bn_ensemble_256_small"]))
+            small = run_ensemble(sargs, ensemble_specs("bn_ensemble_256_small"))
             out["small_meshes"] = {k: small[k] for k in ("value", "plain_value", "batched_vs_plain", "ms_per_step")}
             out["small_meshes"]["config"] = small["config"]
             log("small meshes:", json.dumps(out["small_meshes"]))
         if args.serving_requests:
-            sspecs = ics.serving_requests(**dict(SERVING["serving_4000"], count=args.serving_requests))
-            out["serving"] = run_serving(sspecs, (0.8,))
+            cfg = dict(SERVING["serving_4000"], count=args.serving_requests)
+            out["serving"] = run_serving(ics.serving_requests(**cfg), (0.5, 0.8, 1.0))
+            out["serving"]["config"] = {"requests": cfg["count"], "seed": cfg["seed"], "mix": SERVING_MIX,
+                                        "side": [32, 512], "N": [2, 3, 4], "steps_per_request": cfg["steps"]}
     elif args.workload in PLAIN_TO_T:
         out = run_plain_to_t(args)
         if not args.no_cpu:

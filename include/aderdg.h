@@ -348,6 +348,18 @@ int aderdg_batch_finished(aderdg_batch *b, int *ids, int max_ids, int *count);
 int aderdg_batch_download(aderdg_batch *b, int id, double *uh);
 /* Limiter status and recompute flags of a solve's last step (synchronous). */
 int aderdg_batch_limiter(aderdg_batch *b, int id, int32_t *status, int32_t *recompute);
+/* The whole limiter state of a solve, in the layouts of aderdg_get_limiter: status,
+ * recompute, Lh(nVar,2N+1,2N+1,IMAX,JMAX) of limited cells (0 elsewhere) and
+ * lmin/lmax(nVar,IMAX,JMAX) of the last step (synchronous; any output may be NULL).
+ * Replaces reading Limiter(i,j)%status/%Lh/%lmin/%lmax of module ader_dg
+ * (Euler/ader_dg.f90:73-79) for one solve of a batch. */
+int aderdg_batch_limiter_state(aderdg_batch *b, int id, int32_t *status, int32_t *recompute, double *Lh,
+                               double *lmin, double *lmax);
+/* Health of a solve (synchronous, counted on the device): *nonfinite = DOFs of the current
+ * solution that are NaN or infinite, *troubled = cells with recompute 1 in the last step
+ * (the reference's nRecompute, Euler/main.f90:86-92).  No counterpart in the reference:
+ * for monitors of many resident solves that cannot download every solution. */
+int aderdg_batch_health(aderdg_batch *b, int id, int64_t *nonfinite, int64_t *troubled);
 /* *error = 0, or ADERDG_ECAPACITY when the solve stopped because it limited more
  * cells than its share of the limited-cell pool (synchronous). */
 int aderdg_batch_error(aderdg_batch *b, int id, int *error);

@@ -37,6 +37,10 @@ def oracle_path():
     return os.path.join(HERE, "_build", "liboracle.so")
 
 
+def oracle_fma_path():
+    return os.path.join(HERE, "_build", "liboracle_fma.so")
+
+
 def ref_path(pde, N):
     return os.path.join(HERE, "_ref", "libref_%s_n%d.so" % (pde, N))
 
@@ -319,6 +323,41 @@ PARAMS_TEMPLATE = """{gamma}   ! GAMMA
 {tend}   ! tend
 {wi}   ! WriteInterval
 """
+
+
+class OracleFMA(Oracle):
+    """The same C restatement compiled with FMA contraction (oracle/Makefile): the reference
+    algorithm under a second, equally correct rounding order.  Its distance from Oracle on a
+    state measures how far that state amplifies rounding differences; it is never a
+    reference for parity on its own."""
+
+    _lib = None
+
+    @classmethod
+    def available(cls):
+        if not os.path.exists(oracle_fma_path()):
+            return False
+        try:
+            with open("/proc/cpuinfo") as f:
+                flags = f.read()
+        except OSError:
+            return False
+        return " fma " in flags and " avx2 " in flags
+
+    @classmethod
+    def lib(cls):
+        if cls._lib is None:
+            global oracle_path
+            plain = oracle_path
+            saved = Oracle._lib
+            try:
+                oracle_path = oracle_fma_path
+                Oracle._lib = None
+                cls._lib = Oracle.lib.__func__(Oracle)
+            finally:
+                oracle_path = plain
+                Oracle._lib = saved
+        return cls._lib
 
 
 class RefEuler:
