@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -30,6 +31,19 @@ struct TimedLaunch {
     int part;
     hipEvent_t a, b;
 };
+
+// Staging of aderdg_predictor_cells_noalloc: a file-scope device array of 16 MiB, one copy on
+// every GPU the library runs on, held until the process ends.  In host code its name is only a
+// handle: the array's address on a device comes from hipGetSymbolAddress, resolved once per
+// device by scratch_resolve (called from aderdg_create with the context's device current) and
+// kept in g_scratch.  Every no-allocation call on a GPU shares that GPU's copy.
+constexpr size_t kScratchDoubles = (size_t)2 << 20;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxScratchDevices] = {};
+
+int scratch_resolve(int device);
 
 // DMR states of Euler/riemann_problem.f90:165-175, PDEPrim2Cons of Euler/pde.f90:29-40
 void prim2cons(double g, const double *V, double *Q)
@@ -54,6 +68,26 @@ int fail(int code, const char *fmt, ...)
     g_err = buf;
     return code;
 }
+
+}  // namespace aderdg
+
+namespace {
+int scratch_resolve(int device)
+{
+    if (device < 0 || device >= kMaxScratchDevices)
+        return aderdg::fail(ADERDG_EINVAL, "device %d outside 0..%d", device, kMaxScratchDevices - 1);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[device]) return 0;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return aderdg::fail(ADERDG_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    g_scratch[device] = static_cast<double *>(p);
+    return 0;
+}
+}  // namespace
+
+namespace aderdg {
 
 int check_desc(const aderdg_desc *d, const char *who)
 {
@@ -331,6 +365,7 @@ int aderdg_create(const aderdg_desc *d, aderdg_ctx **out)
         return code;
     };
     if (hipSetDevice(d->device) != hipSuccess) return bail(fail(ADERDG_EHIP, "hipSetDevice(%d) failed", d->device));
+    if ((r = scratch_resolve(d->device))) return bail(r);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(ADERDG_EHIP, "hipStreamCreate failed"));
     const GroupCaps caps{c->ncell, (int64_t)(d->imax + 1) * d->jmax, (int64_t)d->imax * (d->jmax + 1), 1};
@@ -675,6 +710,53 @@ int aderdg_predictor_cells_bn(aderdg_ctx *c, int ncells, double dt, const double
         return fail(ADERDG_EINVAL, "aderdg_predictor_cells_bn: bad argument");
     if (c->desc.pde != ADERDG_PDE_BN) return fail(ADERDG_EINVAL, "aderdg_predictor_cells_bn: not a Baer-Nunziato context");
     return predictor_cells(c, ncells, dt, luh, lqhi, lFhi, lShi, lqbnd, lFbnd);
+}
+
+int aderdg_predictor_cells_noalloc(aderdg_ctx *c, int ncells, double dt, const double *luh, double *lqhi, double *lFhi,
+                                   double *lShi, double *lqbnd, double *lFbnd)
+{
+    if (!c || !luh || !lqhi || !lFhi || !lqbnd || !lFbnd || ncells < 1)
+        return fail(ADERDG_EINVAL, "aderdg_predictor_cells_noalloc: bad argument");
+    const bool bn = c->desc.pde == ADERDG_PDE_BN;
+    if (!bn && c->desc.pde != ADERDG_PDE_EULER)
+        return fail(ADERDG_EINVAL, "aderdg_predictor_cells_noalloc: Euler or Baer-Nunziato contexts only");
+    if (bn && !lShi) return fail(ADERDG_EINVAL, "aderdg_predictor_cells_noalloc: lShi is required for Baer-Nunziato");
+    const size_t n = c->n, V = c->V, nc = (size_t)ncells;
+    // uh, qhi, Shi (V n^2 each), Fhi (2 V n^2), qbnd and Fbnd (4 V n each) per cell
+    const size_t per_cell = V * n * n * 5 + 8 * V * n;
+    const size_t max_cells = kScratchDoubles / per_cell;
+    if (nc > max_cells)
+        return fail(ADERDG_ECAPACITY, "aderdg_predictor_cells_noalloc: %d cells exceed the %zu that the %zu-byte "
+                    "staging array holds for nVar %zu, N %d", ncells, max_cells, kScratchDoubles * sizeof(double), V,
+                    c->N);
+    const int dev = c->desc.device;
+    double *base = (dev >= 0 && dev < kMaxScratchDevices) ? g_scratch[dev] : nullptr;
+    if (!base) return fail(ADERDG_ESTATE, "aderdg_predictor_cells_noalloc: staging array of device %d not resolved", dev);
+    if (hipSetDevice(dev) != hipSuccess) return fail(ADERDG_EHIP, "hipSetDevice(%d) failed", dev);
+    double *d_uh = base, *d_q = d_uh + V * n * n * nc, *d_f = d_q + V * n * n * nc, *d_qb = d_f + 2 * V * n * n * nc;
+    double *d_fb = d_qb + 4 * V * n * nc, *d_s = d_fb + 4 * V * n * nc;
+    // every output element of the ncells cells is written by the kernel: nothing needs clearing
+    hipError_t e = hipMemcpyAsync(d_uh, luh, V * n * n * nc * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    MeshArgs m = c->g.mesh(c->slot);
+    m.ncell = ncells;
+    TestBufs t{d_uh, d_q, d_f, d_qb, d_fb, d_s};
+    if (e == hipSuccess) e = launch_test_predictor(c->N, c->g.variant, m, c->g.b, t, dt, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(lqhi, d_q, V * n * n * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(lFhi, d_f, 2 * V * n * n * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(lqbnd, d_qb, 4 * V * n * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(lFbnd, d_fb, 4 * V * n * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && bn)
+        e = hipMemcpyAsync(lShi, d_s, V * n * n * nc * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(ADERDG_EHIP, "aderdg_predictor_cells_noalloc: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int aderdg_predictor_cells_noalloc_max(int pde, int N)
+{
+    if ((pde != ADERDG_PDE_EULER && pde != ADERDG_PDE_BN) || N < 1 || N > kMaxDegree) return 0;
+    const size_t n = N + 1, V = pde == ADERDG_PDE_BN ? 9 : 4;
+    return (int)(kScratchDoubles / (V * n * n * 5 + 8 * V * n));
 }
 
 int aderdg_timing_enable(aderdg_ctx *c, int on)

@@ -196,6 +196,21 @@ int aderdg_predictor_cells(aderdg_ctx *ctx, int ncells, double dt, const double 
 int aderdg_predictor_cells_bn(aderdg_ctx *ctx, int ncells, double dt, const double *luh, double *lqhi, double *lFhi,
                               double *lShi, double *lqbnd, double *lFbnd);
 
+/* aderdg_predictor_cells / aderdg_predictor_cells_bn without any device allocation: no
+ * counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc): the device staging of
+ * the cells goes through a file-scope array of 16 MiB held on every GPU the library runs on,
+ * whose address is resolved when a context is created on that GPU.  Same arguments and
+ * results (bit for bit) as the allocating forms; lShi is required for a Baer-Nunziato context
+ * and ignored (may be NULL) for Euler.  ncells is limited to
+ * aderdg_predictor_cells_noalloc_max(pde, N) (e.g. 3,177 Euler or 1,412 Baer-Nunziato cells at
+ * N = 4); more returns ADERDG_ECAPACITY.  All no-allocation calls on one GPU share its array:
+ * calls that overlap in time on one GPU (other threads, other contexts) race. */
+int aderdg_predictor_cells_noalloc(aderdg_ctx *ctx, int ncells, double dt, const double *luh, double *lqhi,
+                                   double *lFhi, double *lShi, double *lqbnd, double *lFbnd);
+/* Largest ncells aderdg_predictor_cells_noalloc accepts for (pde, N); 0 for other systems. */
+int aderdg_predictor_cells_noalloc_max(int pde, int N);
+
 /* Kernel timing: when enabled, the kernels of each step are bracketed with
  * hipEvents on the context stream.  Returns accumulated milliseconds and
  * launch counts per kernel: [0] dt reduction, [1] predictor, [2] surface
