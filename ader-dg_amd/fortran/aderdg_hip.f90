@@ -102,6 +102,12 @@ module aderdg_hip
             type(c_ptr), value :: ctx
             real(c_double), intent(out) :: uh(*)
         end function
+        integer(c_int) function aderdg_set_limiter(ctx, status, lh) bind(C, name="aderdg_set_limiter")
+            import :: c_int, c_ptr, c_int32_t, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), intent(in) :: status(*)
+            real(c_double), intent(in) :: lh(*)
+        end function
         integer(c_int) function aderdg_get_limiter(ctx, status, recompute, lh, lmin, lmax) &
                 bind(C, name="aderdg_get_limiter")
             import :: c_int, c_ptr

@@ -117,7 +117,10 @@ program aderdg_euler
 
     do while (time < tend .and. timestep < maxsteps)
 
-        if (mod(timestep, WriteInterval) .eq. 0) call write_data()
+        if (mod(timestep, WriteInterval) .eq. 0) then
+            call write_data()
+            call plot_troubled_cells()
+        end if
 
         call aderdg_check(aderdg_calc_time_step(ctx, time, tend, dt), 'calc_time_step')
 
@@ -135,9 +138,12 @@ program aderdg_euler
 
     print *, ' nRecomp = ', nRecompute, ' t = ', time, 'dt = ', dt
     call write_data()
+    call plot_troubled_cells()
 
     call CPU_TIME(tCPU2)
     print *, ' Total CPU time = ', tCPU2 - tCPU1
+
+    call analyze_soln()
 
     if (len_trim(dumpfile) > 0) then
         call aderdg_check(aderdg_download_uh(ctx, uh), 'download')
@@ -148,6 +154,92 @@ program aderdg_euler
     call aderdg_check(aderdg_destroy(ctx), 'destroy')
 
 contains
+
+    ! Troubled cells of the last step (recompute 1 -> 2.0, 2 -> 1.0, else 0.0) as an unstructured
+    ! VTK grid, the format and loop order of plot_troubled_cells (Euler/write_data.f90:101-192).
+    ! Before the first step no cell is flagged.
+    subroutine plot_troubled_cells()
+        integer :: i, j, k
+        integer(c_int32_t), allocatable, target :: rc(:,:)
+        real(c_double) :: dxv(2)
+        character(len=200) :: FileName
+        integer, parameter :: out_unit = 25
+        if (WriteInterval <= 0) return
+        allocate(rc(IMAX, JMAX))
+        rc = 0
+        if (timestep .gt. 0) call aderdg_check(aderdg_get_limiter(ctx, c_null_ptr, c_loc(rc), c_null_ptr, c_null_ptr, &
+                                                                  c_null_ptr), 'plot_troubled_cells')
+        dxv = (xR - xL) / (/ dble(IMAX), dble(JMAX) /)
+        write(FileName, '(a,a1,i8.8,a)') 'Tcells', '-', timestep, '.vtk'
+        open(unit=out_unit, file=FileName)
+        write(out_unit, '(a)') '# vtk DataFile Version 2.0'
+        write(out_unit, '(a)') '2D Euler'
+        write(out_unit, '(a)') 'ASCII'
+        write(out_unit, '(a)') 'DATASET UNSTRUCTURED_GRID'
+        write(out_unit, '(a)') 'FIELD FieldData 1'
+        write(out_unit, '(a)') 'TIME 1 1 double'
+        write(out_unit, '(e15.9)') time
+        write(out_unit, '(a, i10, a)') 'POINTS ', (IMAX+1)*(JMAX+1), ' double'
+        do j = 1, JMAX+1
+            do i = 1, IMAX+1
+                write(out_unit, '(e15.7, e15.7, e15.7)') xL(1) + (i-1)*dxv(1), xL(2) + (j-1)*dxv(2), 0.0d0
+            end do
+        end do
+        write(out_unit, '(a, i10, i10)') 'CELLS ', IMAX*JMAX, IMAX*JMAX*5
+        do j = 1, JMAX
+            do i = 1, IMAX
+                k = i + (IMAX+1)*(j-1)   ! node number of the cell's lower left corner
+                write(out_unit, '(i10, i10, i10, i10, i10)') 4, k-1, k, k+IMAX+1, k+IMAX
+            end do
+        end do
+        write(out_unit, '(a, i10)') 'CELL_TYPES ', IMAX*JMAX
+        do k = 1, IMAX*JMAX
+            write(out_unit, '(i5)') 9
+        end do
+        write(out_unit, '(a, i10)') 'CELL_DATA', IMAX*JMAX
+        write(out_unit, '(a)') 'SCALARS TCells double 1'
+        write(out_unit, '(a)') 'LOOKUP_TABLE default'
+        do j = 1, JMAX
+            do i = 1, IMAX
+                if (rc(i,j) .eq. 1) then
+                    write(out_unit, '(e15.7)') 2.0d0
+                else if (rc(i,j) .eq. 2) then
+                    write(out_unit, '(e15.7)') 1.0d0
+                else
+                    write(out_unit, '(e15.7)') 0.0d0
+                end if
+            end do
+        end do
+        close(out_unit)
+        deallocate(rc)
+    end subroutine plot_troubled_cells
+
+    ! analyze_soln (Euler/analyze_soln.f90:10-56): the largest difference over the cells between
+    ! the density cell average of the solution and that of the initial condition at the Gauss
+    ! nodes, summed in the reference's order
+    subroutine analyze_soln()
+        integer :: i, j, k, l
+        real(c_double), allocatable :: ex(:,:,:,:,:), error_density(:,:)
+        real(c_double) :: rho_ave, rho_ave_exact
+        allocate(ex(4, N+1, N+1, IMAX, JMAX), error_density(IMAX, JMAX))
+        call aderdg_check(aderdg_initial_condition(ICType, N, IMAX, JMAX, xL, xR, GAMMA, ex), 'analyze_soln')
+        call aderdg_check(aderdg_download_uh(ctx, uh), 'download')
+        do j = 1, JMAX
+            do i = 1, IMAX
+                rho_ave = 0.0d0
+                rho_ave_exact = 0.0d0
+                do l = 1, N+1
+                    do k = 1, N+1
+                        rho_ave_exact = rho_ave_exact + wGPN(l)*wGPN(k)*ex(1, k, l, i, j)
+                        rho_ave = rho_ave + wGPN(l)*wGPN(k)*uh(1, k, l, i, j)
+                    end do
+                end do
+                error_density(i,j) = abs(rho_ave_exact - rho_ave)
+            end do
+        end do
+        print *, maxval(error_density)
+        deallocate(ex, error_density)
+    end subroutine analyze_soln
 
     ! VTK of the cell-average primitive variables, the format of Euler/write_data.f90:37-93
     subroutine write_data()

@@ -5,6 +5,12 @@
 #   oracle/_ref/libref_<pde>_n<N>.so   reference routines + the ref_harness_<pde>.f90
 #                                      bind(C) harness (golden vectors, CPU baseline)
 #   oracle/_ref/aderdg_<pde>_n<N>      the reference's own program (main.f90)
+#   oracle/_ref/refmain_euler_n<N>     (mode `refmain`, after the HIP library is built) the
+#                                      reference's Euler main.f90 with the maintainer's edit of
+#                                      oracle/refmain_edit.py, linked with the reference routines,
+#                                      ader-dg_amd/fortran/aderdg_hip.f90 + aderdg_refglue.f90 and
+#                                      libaderdg_amd_exact.so: the drop-in run of the reference
+#                                      program on the GPU
 #
 # TEST INFRASTRUCTURE ONLY: the product never links or loads these files.
 #
@@ -28,11 +34,11 @@ mkdir -p "$OUT"
 WORK=$(mktemp -d /tmp/aderdg_ref.XXXXXX)
 trap 'rm -rf "$WORK"' EXIT
 
-# build_variant <pde tag> <reference dir> <N> <harness file> [sed expressions for memory-safety fixes]
-build_variant() {
-    local tag=$1 dir=$2 n=$3 harness=$4
-    shift 4
-    local w="$WORK/${tag}_n${n}"
+# prepare <work dir> <reference dir> <N> [sed expressions "file|expr" for the fixes]:
+# scratch copy of the reference sources with the degree set and the fixes applied
+prepare() {
+    local w=$1 dir=$2 n=$3
+    shift 3
     mkdir -p "$w"
     cp "$REF/$dir"/*.f90 "$w/"
     sed -i -E "s/^( *integer, parameter *:: *N *= *)[0-9]+/\1${n}/" "$w/ader_dg.f90"
@@ -47,10 +53,25 @@ build_variant() {
         fi
         rm "$w/$file.orig"
     done
+}
+
+# the reference routines of a prepared directory (everything but the module and main.f90)
+routines() {
     local srcs=""
-    for f in "$w"/*.f90; do
-        case $(basename "$f") in ader_dg.f90|main.f90) ;; *) srcs="$srcs $f";; esac
+    for f in "$1"/*.f90; do
+        case $(basename "$f") in ader_dg.f90|main.f90|main_gpu.f90) ;; *) srcs="$srcs $f";; esac
     done
+    echo "$srcs"
+}
+
+# build_variant <pde tag> <reference dir> <N> <harness file> [fixes]
+build_variant() {
+    local tag=$1 dir=$2 n=$3 harness=$4
+    shift 4
+    local w="$WORK/${tag}_n${n}"
+    prepare "$w" "$dir" "$n" "$@"
+    local srcs
+    srcs=$(routines "$w")
     (cd "$w" && $FC $FFLAGS -c ader_dg.f90 && \
         for f in $srcs; do $FC $FFLAGS -c "$f"; done && \
         $FC $FFLAGS -c "$HERE/$harness" && \
@@ -62,6 +83,31 @@ build_variant() {
         -Wl,-Bsymbolic -Wl,--exclude-libs,ALL
     $FC $FFLAGS -o "$OUT/aderdg_${tag}_n${n}" "$w/ader_dg.o" $objs "$w/main.o"
     echo "built $OUT/libref_${tag}_n${n}.so"
+}
+
+# build_refmain <N> [fixes]: the reference Euler program with the maintainer's edit, on the GPU
+build_refmain() {
+    local n=$1
+    shift
+    local w="$WORK/refmain_n${n}" pkg="$HERE/../ader-dg_amd"
+    if [ ! -e "$pkg/lib/libaderdg_amd_exact.so" ]; then
+        echo "build_ref.sh: $pkg/lib/libaderdg_amd_exact.so missing; build the HIP library first" >&2
+        exit 1
+    fi
+    prepare "$w" Euler "$n" "$@"
+    python3 "$HERE/refmain_edit.py" < "$w/main.f90" > "$w/main_gpu.f90"
+    local srcs
+    srcs=$(routines "$w")
+    (cd "$w" && $FC $FFLAGS -c ader_dg.f90 && \
+        for f in $srcs; do $FC $FFLAGS -c "$f"; done && \
+        $FC $FFLAGS -c "$pkg/fortran/aderdg_hip.f90" && \
+        $FC $FFLAGS -c "$pkg/fortran/aderdg_refglue.f90" && \
+        $FC $FFLAGS -c main_gpu.f90)
+    local objs=""
+    for f in $srcs; do objs="$objs $w/$(basename "${f%.f90}").o"; done
+    $FC $FFLAGS -o "$OUT/refmain_euler_n${n}" "$w/ader_dg.o" $objs "$w/aderdg_hip.o" "$w/aderdg_refglue.o" \
+        "$w/main_gpu.o" -L"$pkg/lib" -laderdg_amd_exact -Wl,-rpath,'$ORIGIN/../../ader-dg_amd/lib'
+    echo "built $OUT/refmain_euler_n${n}"
 }
 
 # The Euler subcell recompute applies Double-Mach-Reflection ghost states on the
@@ -76,6 +122,12 @@ EULER_FIXES=(
   'limiter.f90|s/^( *)if \(j \.eq\. JMAX\) then ! Top/\1if (j .eq. JMAX .and. ICType .eq. 11) then ! Top/'
   'initialize.f90|s#xGP = x0 \+ \(/ xilimbary#xGP = corner + (/ xilimbary#'
 )
+if [ "${1:-all}" = refmain ]; then
+    for n in 2 3; do
+        build_refmain "$n" "${EULER_FIXES[@]}"
+    done
+    exit 0
+fi
 for n in 1 2 3 4; do
     build_variant euler Euler "$n" ref_harness_euler.f90 "${EULER_FIXES[@]}"
 done

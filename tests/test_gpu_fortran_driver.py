@@ -162,3 +162,42 @@ def test_ad_driver_reference_params(tmp_path):
     ref = ad_oracle(N, I, J, c, a, bc, 0, np.zeros((1, N + 1, N + 1, I, J), order="F"), steps, 2.0)
     assert np.abs(ref).max() > 1e-3  # the boundary data has entered the domain
     assert np.linalg.norm((uh - ref).ravel()) <= 1e-13 * np.linalg.norm(ref.ravel())
+
+
+def test_driver_dmr_outputs_match_reference_program(tmp_path):
+    """The driver's output on the Double Mach Reflection -- DMR-*.vtk (write_data),
+    Tcells-*.vtk (plot_troubled_cells, Euler/write_data.f90:101-192), the per-step lines and
+    analyze_soln's error (Euler/analyze_soln.f90:10-56) -- byte for byte against the reference
+    program (oracle/_ref/aderdg_euler_n2) run on the host with the same params.ini.  Not
+    compared: the CPU time, and two values the reference reads from memory it never sets
+    (main's nRecompute at the first print, Euler/main.f90:13,35, and the recompute map written
+    at timestep 0, Euler/initialize.f90:158), which the driver prints as 0."""
+    import re
+    N, I, J, tend = 2, 32, 8, 5.0e-3
+    from pyoracle import HERE as oracle_dir
+    exe_ref = os.path.join(oracle_dir, "_ref", "aderdg_euler_n%d" % N)
+    if not os.path.exists(exe_ref):
+        pytest.skip("reference program absent (oracle/build_ref.sh)")
+    a, b = tmp_path / "reference", tmp_path / "driver"
+    for d in (a, b):
+        d.mkdir()
+    _, out_drv = run_driver(b, "aderdg_euler_exact", N, 1 << 30, I, J, (0.0, 0.0), (4.0, 1.0), 11, 1, tend, wi=3)
+    params.write_params(a / "params.ini", dict(gamma=1.4, imax=I, jmax=J, xl0=0.0, xl1=0.0, xr0=4.0, xr1=1.0, bL=1,
+                                               bR=1, bB=1, bT=1, ictype=11, tend=tend, write_interval=3), "euler")
+    r = subprocess.run([exe_ref], cwd=a, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    keep = lambda out: [line for line in out.splitlines() if "Total CPU time" not in line]
+    la, lb = keep(r.stdout), keep(out_drv)
+    k = next(i for i, line in enumerate(la) if "nRecomp" in line)
+    la[k] = re.sub(r"nRecomp = +-?\d+", "nRecomp = ?", la[k])
+    lb[k] = re.sub(r"nRecomp = +-?\d+", "nRecomp = ?", lb[k])
+    assert lb == la
+    names = sorted(p.name for p in a.glob("*.vtk"))
+    assert names == sorted(p.name for p in b.glob("*.vtk")) and "Tcells-00000003.vtk" in names
+    for name in names:
+        ra, rb = (a / name).read_bytes(), (b / name).read_bytes()
+        if name == "Tcells-00000000.vtk":
+            ra, rb = ra.split(b"LOOKUP_TABLE default")[0], rb.split(b"LOOKUP_TABLE default")[0]
+        assert ra == rb, name
+    print("driver: %d stdout lines and %d VTK files identical to the reference program" % (len(la), len(names)),
+          flush=True)
