@@ -78,6 +78,12 @@ void Batch::release()
     pinned_states.clear();
     if (health_buf) (void)hipFree(health_buf);
     health_buf = nullptr;
+    for (auto e : join) (void)hipEventDestroy(e);
+    join.clear();
+    if (fork) (void)hipEventDestroy(fork);
+    fork = nullptr;
+    for (auto b : branch) (void)hipStreamDestroy(b);
+    branch.clear();
     hipEvent_t evs[] = {upload_done, removed, states_ready};
     for (auto e : evs)
         if (e) (void)hipEventDestroy(e);
@@ -202,13 +208,32 @@ int Batch::build_graph()
     if (graph) (void)hipGraphExecDestroy(graph);
     graph = nullptr;
     hipGraph_t gr = nullptr;
+    // arenas are independent: one graph branch each, so one arena's compute-bound predictor
+    // can run beside another's memory-bound face and update passes and fill its tails
+    while (branch.size() + 1 < groups.size()) {
+        hipStream_t b = nullptr;
+        hipEvent_t j = nullptr;
+        HIPCHK(hipStreamCreateWithFlags(&b, hipStreamNonBlocking));
+        branch.push_back(b);
+        HIPCHK(hipEventCreateWithFlags(&j, hipEventDisableTiming));
+        join.push_back(j);
+    }
+    if (!fork) HIPCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
     HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
     static const int parts[] = {kPartPredictor, kPartFace, kPartUpdate, kPartSpread, kPartSubcell, kPartAdvance};
-    for (auto &g : groups) {
-        if (e == hipSuccess) e = launch_dt(g->b, stream, 0.0, 0.0, 0, g->dt_grid());
+    if (groups.size() > 1) e = hipEventRecord(fork, stream);
+    for (size_t k = 1; k < groups.size() && e == hipSuccess; k++) e = hipStreamWaitEvent(branch[k - 1], fork, 0);
+    for (size_t k = 0; k < groups.size(); k++) {
+        const Group &g = *groups[k];
+        hipStream_t s = k == 0 ? stream : branch[k - 1];
+        if (e == hipSuccess) e = launch_dt(g.b, s, 0.0, 0.0, 0, g.dt_grid());
         for (int p : parts)
-            if (e == hipSuccess) e = g->launch(p);
+            if (e == hipSuccess) e = g.launch_on(p, s);
+    }
+    for (size_t k = 1; k < groups.size() && e == hipSuccess; k++) {
+        e = hipEventRecord(join[k - 1], branch[k - 1]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(stream, join[k - 1], 0);
     }
     hipError_t e2 = hipStreamEndCapture(stream, &gr);
     if (e != hipSuccess || e2 != hipSuccess) {

@@ -28,6 +28,11 @@ public:
     int64_t reserved = 0;
     hipGraphExec_t graph = nullptr;
     bool graph_stale = true;
+    // the step graph forks one branch per arena: arena 0 on `stream`, arena k on branch[k-1]
+    // (default priority, like every stream here), joined back into `stream` at the end
+    std::vector<hipStream_t> branch;
+    std::vector<hipEvent_t> join;
+    hipEvent_t fork = nullptr;
     hipEvent_t upload_done = nullptr, removed = nullptr, states_ready = nullptr;
     // per-kernel timing (steps launched part by part with events instead of the graph)
     struct Timed {

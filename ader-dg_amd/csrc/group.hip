@@ -432,4 +432,9 @@ hipError_t Group::launch(int part) const
     return launch_part(pde, N, variant, upd, b, st_, part, grid, caps.slots);
 }
 
+hipError_t Group::launch_on(int part, hipStream_t s) const
+{
+    return launch_part(pde, N, variant, upd, b, s, part, grid, caps.slots);
+}
+
 }  // namespace aderdg

@@ -101,6 +101,7 @@ public:
     hipStream_t stream() const { return st_; }
 
     hipError_t launch(int part) const;
+    hipError_t launch_on(int part, hipStream_t s) const;  // the same launch on another stream
     int dt_grid() const { return grid[kTabCell] < kDtGrid ? grid[kTabCell] : kDtGrid; }
     // Bytes of device memory per cell / x-face / y-face / solve slot / limited-cell pool
     // slot of a (PDE, N).

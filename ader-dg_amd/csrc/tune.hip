@@ -5,9 +5,10 @@
 // 256, i.e. 1..32 cells per workgroup) that give the same bits and differ in registers, LDS and waves per
 // SIMD.  The fastest depends on (PDE, N) and on the launch mode (one workgroup per
 // table entry, or a persistent grid), so the first group of a (device, PDE, N,
-// mode) times every variant on a synthetic 192 x 192 solve and keeps the fastest
+// mode) times every variant on a synthetic 256 x 256 solve and keeps the fastest
 // for the rest of the process; the update kernel's two variants (step_kernels.hpp)
-// are timed the same way on the synthetic solve of the fastest predictor.
+// are timed the same way, each in a group of its own (so each runs on the persistent
+// grid sized for its own kernel) behind the fastest predictor.
 // ADERDG_PREDICTOR_VARIANT=<k> / ADERDG_UPDATE_VARIANT=<k> in the environment, or
 // aderdg_tune_force_kernel(), override the choice (for tests and measurements).
 #include <climits>
@@ -24,7 +25,7 @@ namespace aderdg {
 
 namespace {
 
-constexpr int kTuneSide = 192;
+constexpr int kTuneSide = 256;
 constexpr int kTuneRounds = 3, kTuneReps = 3;
 
 struct TuneEntry {
@@ -67,18 +68,18 @@ void synthetic(int pde, int N, aderdg_desc *d, IcSpec *ic)
     }
 }
 
-// A synthetic solve in its own group, built with one predictor variant.
+// A synthetic solve in its own group, built with one predictor and one update variant.
 struct Trial {
     Group g;
     bool ok = false;
-    Trial(int pde, int N, int variant, int persist_blocks, const Operators &ops, hipStream_t st)
+    Trial(int pde, int N, int variant, int persist_blocks, const Operators &ops, hipStream_t st, int upd = kUpdShared)
     {
         aderdg_desc d;
         IcSpec ic;
         synthetic(pde, N, &d, &ic);
         const int S = kTuneSide;
         GroupCaps caps{(int64_t)S * S, (int64_t)(S + 1) * S, (int64_t)S * (S + 1), 1};
-        if (g.init(pde, N, caps, ops, persist_blocks, st, variant, kUpdShared)) return;
+        if (g.init(pde, N, caps, ops, persist_blocks, st, variant, upd)) return;
         SolveState s0{};
         s0.dt = 1.0e-4;
         s0.tend = 1.0e30;
@@ -166,21 +167,29 @@ void tune_kernels(int pde, int N, int persist_blocks, const Operators &ops, hipS
                     best = t.pms[v];
                     t.pred = v;
                 }
-            Trial *tr = trials[t.pred];
-            if (tr && tr->ok && tr->g.launch(kPartFace) == hipSuccess) {
-                for (int r = 0; r < kTuneRounds; r++)
-                    for (int u = 0; u < kNumUpdVariants; u++) {
-                        tr->g.upd = u;
-                        const double ms = time_launches(*tr, kPartUpdate, st, e0, e1);
-                        if (ms > 0.0 && (t.ums[u] < 0.0 || ms < t.ums[u])) t.ums[u] = ms;
-                    }
-                best = -1.0;
-                for (int u = 0; u < kNumUpdVariants; u++)
-                    if (t.ums[u] > 0.0 && (best < 0.0 || t.ums[u] < best)) {
-                        best = t.ums[u];
-                        t.upd = u;
-                    }
+            for (int v = 0; v < kNumPredVariants; v++) {
+                delete trials[v];
+                trials[v] = nullptr;
             }
+            Trial *ut[kNumUpdVariants] = {};
+            for (int u = 0; u < kNumUpdVariants; u++) {
+                ut[u] = new Trial(pde, N, t.pred, persist_blocks, ops, st, u);
+                if (ut[u]->ok && ut[u]->g.launch(kPartFace) != hipSuccess) ut[u]->ok = false;
+            }
+            for (int r = 0; r < kTuneRounds; r++)
+                for (int u = 0; u < kNumUpdVariants; u++) {
+                    if (!ut[u]->ok) continue;
+                    const double ms = time_launches(*ut[u], kPartUpdate, st, e0, e1);
+                    if (ms > 0.0 && (t.ums[u] < 0.0 || ms < t.ums[u])) t.ums[u] = ms;
+                }
+            best = -1.0;
+            for (int u = 0; u < kNumUpdVariants; u++)
+                if (t.ums[u] > 0.0 && (best < 0.0 || t.ums[u] < best)) {
+                    best = t.ums[u];
+                    t.upd = u;
+                }
+            (void)hipStreamSynchronize(st);
+            for (int u = 0; u < kNumUpdVariants; u++) delete ut[u];
         }
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);

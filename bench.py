@@ -519,10 +519,10 @@ def run_ensemble(args, specs):
         "batched_vs_plain": (plain_s / batched_s) if plain_s else None,
         "roofline": roof,
         "predictor_autotune": {str(N): {"variant": aderdg.PREDICTOR_VARIANTS[t[0]],
-                                        "ms_per_launch_192x192": dict(zip(aderdg.PREDICTOR_VARIANTS, t[1]))}
+                                        "ms_per_launch_256x256": dict(zip(aderdg.PREDICTOR_VARIANTS, t[1]))}
                                for N, t in tuned.items() if t},
         "update_autotune": {str(N): {"variant": aderdg.UPDATE_VARIANTS[t[0]],
-                                     "ms_per_launch_192x192": dict(zip(aderdg.UPDATE_VARIANTS, t[1]))}
+                                     "ms_per_launch_256x256": dict(zip(aderdg.UPDATE_VARIANTS, t[1]))}
                             for N, t in ((N, aderdg.tune_query("bn", N, True, kernel="update")) for N in per_group)
                             if t},
     }

@@ -228,7 +228,7 @@ int aderdg_timing_get(aderdg_ctx *ctx, double ms[ADERDG_TIMING_SLOTS], int64_t l
  * with a smaller register footprint (right-hand sides kept in the time line,
  * state and operators in LDS).  Advection-diffusion has variant 0 only.  The first context or
  * batch arena of a (device, pde, N, launch mode) times every variant on a
- * synthetic 192x192 solve and keeps the fastest for the process.
+ * synthetic 256x256 solve and keeps the fastest for the process.
  * aderdg_tune_force(pde, N, v) makes contexts and arenas created afterwards use
  * variant v (-1: back to the tuner); the environment variable
  * ADERDG_PREDICTOR_VARIANT=<v> does the same for every (pde, N).
