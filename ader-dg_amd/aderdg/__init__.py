@@ -250,6 +250,7 @@ def load_library(exact=False):
         "aderdg_batch_limiter": [vp, ctypes.c_int, _ip, _ip],
         "aderdg_batch_limiter_state": [vp, ctypes.c_int, _ip, _ip, _dp, _dp, _dp],
         "aderdg_batch_health": [vp, ctypes.c_int, ctypes.POINTER(i64), ctypes.POINTER(i64)],
+        "aderdg_probe_peaks": [ctypes.c_int, _dp, _dp, _dp],
         "aderdg_batch_remove": [vp, ctypes.c_int],
         "aderdg_batch_usage": [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(i64)],
         "aderdg_batch_timing_enable": [vp, ctypes.c_int],
@@ -492,6 +493,15 @@ def pred_variant_valid(pde, N, variant):
     if k == 2:
         return variant == 0
     return variant in (0, 2, 3, 4) or (variant == 1 and (N + 1) ** 3 <= 64)
+
+
+def probe_peaks(device=0, exact=False):
+    """Measured (FP64 TFLOP/s, HBM stream-read GB/s, HBM copy GB/s) of the device."""
+    lib = load_library(exact)
+    f, r, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    if lib.aderdg_probe_peaks(device, ctypes.byref(f), ctypes.byref(r), ctypes.byref(c)) != 0:
+        raise AderdgError(lib.aderdg_last_error().decode())
+    return f.value, r.value, c.value
 
 
 def solve_footprint(pde, N, imax, jmax, exact=False, limited_fraction=None):

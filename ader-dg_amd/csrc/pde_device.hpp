@@ -71,6 +71,20 @@ ADG_DEV double rdiv(double a, double b)
 #endif
 }
 
+// a / b where b stays the same over a loop and rb = 1 / b was formed once before it.  The
+// exact build divides (the reference's rounding); the product build multiplies by the
+// reciprocal, within an ulp or two of the quotient, which takes the division out of the loop.
+ADG_DEV double div_inv(double a, double b, double rb)
+{
+#if defined(ADERDG_EXACT) || !defined(__HIP_DEVICE_COMPILE__)
+    (void)rb;
+    return a / b;
+#else
+    (void)b;
+    return a * rb;
+#endif
+}
+
 // minmod, Euler/limiter.f90:384-405
 ADG_DEV double minmod(double a, double b)
 {

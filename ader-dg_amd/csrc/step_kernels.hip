@@ -177,6 +177,7 @@ ADG_DEV void predictor_cells(const MeshArgs &m, const StepBufs &b, const TestBuf
                              int cell0, int par, double dt)
 {
     using S = Shape<P, N>;
+    const double rdx = 1.0 / m.dx, rdy = 1.0 / m.dy;  // per solve; div_inv, gradients
     constexpr int n = S::n, NN = S::NN, CPW = S::CPW, V = S::V, SLOT = S::SLOT;
     const OpPtr op = (OpPtr)b.op;
     const int tid = threadIdx.x, lc = tid / NN, tt = tid - lc * NN;
@@ -254,11 +255,11 @@ ADG_DEV void predictor_cells(const MeshArgs &m, const StepBufs &b, const TestBuf
                 acc = 0.0;
 #pragma unroll
                 for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + k + n * j] * du_i[k];
-                G[v] = 1.0 / m.dx * acc;
+                G[v] = rdx * acc;
                 acc = 0.0;
 #pragma unroll
                 for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + i + n * k] * du_j[k];
-                G[V + v] = 1.0 / m.dy * acc;
+                G[V + v] = rdy * acc;
             }
         }
         if (P::kNcp) P::ncp(m, u, G, src);  // NCP source, Baer-Nunziato/space_time_predictor.f90:84-92
@@ -270,7 +271,7 @@ ADG_DEV void predictor_cells(const MeshArgs &m, const StepBufs &b, const TestBuf
 #pragma unroll
         for (int l = 0; l < n; l++) {
             const double wl = op->w[l];
-            const double sx = wl * wj * dt / m.dx, sy = wl * wi * dt / m.dy;
+            const double sx = div_inv(wl * wj * dt, m.dx, rdx), sy = div_inv(wl * wi * dt, m.dy, rdy);
             const double c0 = wi * wj * op->f0[l];
             const double f = wi * wj * wl * dt;
 #pragma unroll
@@ -326,7 +327,7 @@ ADG_DEV void predictor_cells(const MeshArgs &m, const StepBufs &b, const TestBuf
                 }
             }
             __syncthreads();
-            const double sx = wl * wj * dt / m.dx, sy = wl * wi * dt / m.dy;
+            const double sx = div_inv(wl * wj * dt, m.dx, rdx), sy = div_inv(wl * wi * dt, m.dy, rdy);
             const double c0 = wi * wj * op->f0[l];
             double G[2 * V], rv[V];
 #pragma unroll
@@ -343,11 +344,11 @@ ADG_DEV void predictor_cells(const MeshArgs &m, const StepBufs &b, const TestBuf
                     acc = 0.0;
 #pragma unroll
                     for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + k + n * j] * du_i[k];
-                    G[v] = 1.0 / m.dx * acc;
+                    G[v] = rdx * acc;
                     acc = 0.0;
 #pragma unroll
                     for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + i + n * k] * du_j[k];
-                    G[V + v] = 1.0 / m.dy * acc;
+                    G[V + v] = rdy * acc;
                 }
             }
             if (P::kNcp) {  // NCP source, Baer-Nunziato/space_time_predictor.f90:84-92
@@ -434,6 +435,7 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
                             int cell0, int par, double dt)
 {
     using S = Shape<P, N>;
+    const double rdx = 1.0 / m.dx, rdy = 1.0 / m.dy;  // per solve; div_inv, gradients
     using L = LeanShape<P, N>;
     constexpr int n = S::n, NN = S::NN, CPW = S::CPW, V = S::V;
     const OpPtr op = (OpPtr)b.op;
@@ -506,11 +508,11 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
                 acc = 0.0;
 #pragma unroll
                 for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + k + n * j] * sD[i + n * k];
-                G[v] = 1.0 / m.dx * acc;
+                G[v] = rdx * acc;
                 acc = 0.0;
 #pragma unroll
                 for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + i + n * k] * sD[j + n * k];
-                G[V + v] = 1.0 / m.dy * acc;
+                G[V + v] = rdy * acc;
             }
         }
         __syncthreads();  // the exchange buffer is free for the next slice
@@ -541,7 +543,7 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
 #pragma unroll
             for (int l = 0; l < n; l++) {
                 const double wl = op->w[l];
-                const double sx = wl * wj * dt / m.dx, sy = wl * wi * dt / m.dy;
+                const double sx = div_inv(wl * wj * dt, m.dx, rdx), sy = div_inv(wl * wi * dt, m.dy, rdy);
                 const double c0 = wi * wj * op->f0[l];
                 const double f = wi * wj * wl * dt;
 #pragma unroll
@@ -580,7 +582,7 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
             }
             __syncthreads();
             const double wl = op->w[l];
-            const double sx = wl * wj * dt / m.dx, sy = wl * wi * dt / m.dy;
+            const double sx = div_inv(wl * wj * dt, m.dx, rdx), sy = div_inv(wl * wi * dt, m.dy, rdy);
             const double c0 = wi * wj * op->f0[l];
             double rv[V], G[2 * V];
 #pragma unroll
@@ -597,11 +599,11 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
                     acc = 0.0;
 #pragma unroll
                     for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + k + n * j] * sD[i + n * k];
-                    G[v] = 1.0 / m.dx * acc;
+                    G[v] = rdx * acc;
                     acc = 0.0;
 #pragma unroll
                     for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + i + n * k] * sD[j + n * k];
-                    G[V + v] = 1.0 / m.dy * acc;
+                    G[V + v] = rdy * acc;
                 }
             }
             __syncthreads();  // the exchange buffer is free for the next slice
@@ -692,6 +694,7 @@ ADG_DEV void predictor_nodes(const MeshArgs &m, const StepBufs &b, const TestBuf
                              int cell0, int par, double dt)
 {
     using S = NodeShape<P, N, BLK>;
+    const double rdx = 1.0 / m.dx, rdy = 1.0 / m.dy;  // per solve; div_inv, gradients
     constexpr int n = S::n, NN = S::NN, N3 = S::N3, V = S::V, CPB = S::CPB, R = S::R;
     const OpPtr op = (OpPtr)b.op;
     const int tid = threadIdx.x, lc = tid / N3, tt = tid - lc * N3;
@@ -727,7 +730,7 @@ ADG_DEV void predictor_nodes(const MeshArgs &m, const StepBufs &b, const TestBuf
         }
     }
     // the same expressions as predictor_cells for slice l of node (i, j)
-    const double sx = wl * wj * dt / m.dx, sy = wl * wi * dt / m.dy;
+    const double sx = div_inv(wl * wj * dt, m.dx, rdx), sy = div_inv(wl * wi * dt, m.dy, rdy);
     const double c0 = wi * wj * op->f0[l];
     const double fs = wi * wj * wl * dt;
     const double f1 = 1.0 / (wi * wj);
@@ -761,7 +764,7 @@ ADG_DEV void predictor_nodes(const MeshArgs &m, const StepBufs &b, const TestBuf
                 acc = 0.0;
 #pragma unroll
                 for (int k = 0; k < n; k++) acc = acc + sR[v * N3 + k + n * j + NN * l] * du_i[k];
-                G[v] = 1.0 / m.dx * acc;
+                G[v] = rdx * acc;
                 if (LAST && l == 0) {  // time average of the last sweep's fluxes (:92)
                     double a = 0.0;
 #pragma unroll
@@ -785,7 +788,7 @@ ADG_DEV void predictor_nodes(const MeshArgs &m, const StepBufs &b, const TestBuf
                 acc = 0.0;
 #pragma unroll
                 for (int k = 0; k < n; k++) acc = acc + sR[v * N3 + i + n * k + NN * l] * du_j[k];
-                G[V + v] = 1.0 / m.dy * acc;
+                G[V + v] = rdy * acc;
                 if (LAST && l == 0) {
                     double a = 0.0;
 #pragma unroll

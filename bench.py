@@ -248,23 +248,70 @@ def first_sweep_skipped(pde, N):
     return (n - 1) * per_slice
 
 
+PEAKS = None  # measured (FP64 TFLOP/s, HBM read GB/s, HBM copy GB/s) of this run's GPU
+
+
 def roofline_of(flop_per_cell, cells, ms_per_launch, kernel, traffic=None, skipped_per_cell=0.0):
     """achieved: the reference algorithm's operations (the oracle's op count) per second;
     executed_achieved: the operations the kernel issues per second."""
     achieved = flop_per_cell * cells / (ms_per_launch * 1e-3) / 1e12
     executed = flop_per_cell - skipped_per_cell
-    return {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+    extra = {}
+    if PEAKS:
+        extra = {"peak_measured": PEAKS[0], "frac_measured": achieved / PEAKS[0],
+                 "peak_measured_note": "FP64 FMA rate of eight independent chains per thread on this GPU "
+                                       "(aderdg_probe_peaks)"}
+    # "mfma" is the compute roofline of the line's contract; its FP64 peak equals the vector ALU's
+    # on MI355X, and the kernel runs on the vector ALU (compute_unit)
+    return {"bound": "mfma", "compute_unit": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", **extra,
             "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic, "traffic_unit": "bytes/launch",
             "kernel": kernel, "flop_per_cell": flop_per_cell, "cells_per_launch": cells,
             "executed_flop_per_cell": executed,
             "executed_achieved": executed * cells / (ms_per_launch * 1e-3) / 1e12,
-            "note": "FP64 vector ALU; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s. achieved and "
-                    "frac count the reference algorithm's operations; the time-line predictors form the first "
-                    "Picard sweep's flux terms once for all time slices, so they issue executed_flop_per_cell"}
+            "note": "FP64 vector ALU (no MFMA: the per-cell contractions are 4x4..5x5 and run on the VALU); "
+                    "MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s. achieved and frac count the "
+                    "reference algorithm's operations; the time-line predictors form the first Picard sweep's "
+                    "flux terms once for all time slices, so they issue executed_flop_per_cell"}
 
 
 def skipped_for(kernel, pde, N):
     return first_sweep_skipped(pde, N) if kernel.startswith(("k_predictor<", "k_predictor_lean<")) else 0.0
+
+
+def algorithmic_bytes(pde, N):
+    """HBM bytes per cell-step each kernel must move at least (FP64, ping-pong state):
+    predictor reads uh (and its 3x3 ranges), writes lduh, the face extrapolations fq/ff and
+    the ranges; face reads fq/ff of both neighbours and writes the fluxes (one x and one y
+    face per cell; Baer-Nunziato two fluctuation sides); update reads uh, lduh, four face
+    lines and the ranges, writes uh', the next ranges and the time-step candidate."""
+    V = 9 if pde == "bn" else 4
+    n = N + 1
+    sides = 2 if pde == "bn" else 1
+    return {"predictor": 8 * (V * n * n + 2 * V) + 8 * (V * n * n + 2 * 4 * V * n + 2 * V),
+            "face": 2 * 8 * (2 * 2 * V * n) + 2 * 8 * (sides * V * n),
+            "update": 8 * (2 * V * n * n + 4 * V * n + 2 * V) + 8 * (V * n * n + 2 * V + 1)}
+
+
+def kernel_table(pde, per_group, peaks):
+    """Per arena and kernel: ms per launch, algorithmic GB/s and its fraction of HBM peaks."""
+    out = {}
+    for N, g in per_group.items():
+        ab = algorithmic_bytes(pde, int(N))
+        row = {}
+        for k in ("predictor", "face", "update", "subcell", "dt", "spread"):
+            ms = g.get(k)
+            if not ms:
+                continue
+            r = {"ms": ms}
+            if k in ab:
+                gbs = ab[k] * g["cells"] / (ms * 1e-3) / 1e9
+                r.update(algorithmic_bytes_per_cell=ab[k], algorithmic_gbs=gbs, frac_hbm_peak=gbs / HBM_PEAK_GBS)
+                if peaks:
+                    r["frac_measured_read_peak"] = gbs / peaks[1]
+            row[k] = r
+        out[str(N)] = row
+    return out
 
 
 def run_plain_to_t(args):
@@ -518,6 +565,7 @@ def run_ensemble(args, specs):
         "plain_value": dof_step * K / plain_s if plain_s else None,
         "batched_vs_plain": (plain_s / batched_s) if plain_s else None,
         "roofline": roof,
+        "kernels": kernel_table("bn", per_group, PEAKS),
         "predictor_autotune": {str(N): {"variant": aderdg.PREDICTOR_VARIANTS[t[0]],
                                         "ms_per_launch_256x256": dict(zip(aderdg.PREDICTOR_VARIANTS, t[1]))}
                                for N, t in tuned.items() if t},
@@ -533,10 +581,11 @@ def run_ensemble(args, specs):
 # serving: Poisson arrivals into the continuous-batching server
 # ---------------------------------------------------------------------------
 SERVING_CELLS = {"euler": 2_000_000, "bn": 1_500_000}  # arena cells per (pde, N)
-# Step cap of a request, in planned steps.  A request's end time is its planned steps times the
-# time step estimated from its initial state; the Baer-Nunziato interfaces accelerate their gas
-# phase, and on the finest meshes the time step falls up to ~20x below that estimate.
-STEP_CAP = 10
+# Step cap of a request, in planned steps: a safety net only.  A request's end time is its planned
+# steps times the time step estimated from its initial state; the Baer-Nunziato interfaces
+# accelerate their gas phase, and at N = 4 and on the finest meshes the time step falls 10-20x
+# below that estimate, so such requests take that many more steps to reach their end time.
+STEP_CAP = 50
 
 
 def serving_setup(specs):
@@ -609,6 +658,17 @@ def run_serving(specs, factors, seed=11):
     return out
 
 
+def probe():
+    """Measure this GPU's FP64 FMA and HBM stream peaks once (aderdg_probe_peaks)."""
+    global PEAKS
+    import aderdg
+    try:
+        PEAKS = aderdg.probe_peaks()
+        log("measured peaks: FP64 %.1f TFLOP/s, HBM read %.0f GB/s, copy %.0f GB/s" % PEAKS)
+    except aderdg.AderdgError as e:
+        log("peak probe failed: %s" % e)
+
+
 def copy_probe(nbytes):
     """One device-to-device copy of known size (the HIP runtime's copy kernel): the
     FETCH_SIZE / WRITE_SIZE it reads under rocprofv3 --pmc calibrate the counters."""
@@ -631,7 +691,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20 ensemble, 200 plain)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (default 2 ensemble, 10 plain)")
     ap.add_argument("--workload", default="bn_ensemble_256", choices=sorted(PLAIN) + sorted(PLAIN_TO_T) + sorted(AD) + sorted(ENSEMBLES) + sorted(SERVING))
-    ap.add_argument("--serving-requests", type=int, default=1000,
+    ap.add_argument("--serving-requests", type=int, default=400,
                     help="ensemble workload: requests of the serving stream timed beside it at 0.5x, 0.8x and 1.0x "
                          "of saturation (0: none)")
     ap.add_argument("--count", type=int, default=None, help="ensemble size (default 256)")
@@ -684,6 +744,7 @@ def main():
             procs = args.cpu_procs or max(1, min(CPU_PROC_CAP, host_cpu()[1]))
             cpu = cpu_ensemble(specs, args.cpu_seconds, procs)
             log("cpu baseline:", json.dumps(cpu))
+        probe()
         out = run_ensemble(args, specs)
         if args.workload == "bn_ensemble_256" and not args.no_stable:
             # the same seeds on a state the reference scheme advances stably, batched and plain
@@ -705,6 +766,7 @@ def main():
             out["serving"]["config"] = {"requests": cfg["count"], "seed": cfg["seed"], "mix": SERVING_MIX,
                                         "side": [32, 512], "N": [2, 3, 4], "steps_per_request": cfg["steps"]}
     elif args.workload in PLAIN_TO_T:
+        probe()
         out = run_plain_to_t(args)
         if not args.no_cpu:
             cpu = cpu_plain(PLAIN_TO_T[args.workload][0], args.cpu_seconds)
@@ -713,11 +775,14 @@ def main():
         if not args.no_cpu:
             cpu = cpu_ad(args.workload, args.cpu_seconds)
     else:
+        probe()
         out = run_plain(args)
         if not args.no_cpu:
             cpu = cpu_plain(args.workload, args.cpu_seconds)
     if cpu is not None:
         out["cpu_baseline"] = cpu
+    if PEAKS:
+        out["measured_peaks"] = {"fp64_tflops": PEAKS[0], "hbm_read_gbs": PEAKS[1], "hbm_copy_gbs": PEAKS[2]}
     if args.copy_probe_bytes:
         copy_probe(args.copy_probe_bytes)
     print(json.dumps(out), flush=True)

@@ -449,6 +449,13 @@ int aderdg_server_stats_get(aderdg_server *s, aderdg_server_stats *st);
 /* Stop the worker (queued requests are dropped) and free everything. */
 int aderdg_server_destroy(aderdg_server *s);
 
+/* Measured peaks of a device (no counterpart in the reference; the denominators of the
+ * benchmark's roofline fractions beside the datasheet's 78.6 TFLOP/s and 8 TB/s):
+ * FP64 FMA rate of eight independent chains per thread (TFLOP/s), and HBM bandwidth of a
+ * 1 GiB stream read and of a 1 GiB copy (GB/s, read plus write bytes).  Allocates 2 GiB
+ * for the duration of the call. */
+int aderdg_probe_peaks(int device, double *fp64_tflops, double *hbm_read_gbs, double *hbm_copy_gbs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,3 +46,25 @@ def test_more_than_one_gpu_is_refused():
                        text=True, timeout=120)
     assert p.returncode == 2
     assert "one GPU" in p.stderr
+
+
+def test_algorithmic_bytes_model():
+    """Per cell-step byte counts of the kernel table: Baer-Nunziato N=3 update reads uh and
+    lduh (2 x 144 doubles), four face lines (4 x 36) and the ranges (18), writes uh' (144),
+    the next ranges (18) and the time-step candidate (1)."""
+    b = bench.algorithmic_bytes("bn", 3)
+    assert b["update"] == 8 * (2 * 144 + 4 * 36 + 18) + 8 * (144 + 18 + 1)
+    assert b["face"] == 2 * 8 * (2 * 2 * 36) + 2 * 8 * (2 * 36)
+    e = bench.algorithmic_bytes("euler", 4)
+    assert e["face"] == 2 * 8 * (2 * 2 * 20) + 2 * 8 * 20
+
+
+def test_kernel_table_and_measured_peaks(monkeypatch):
+    monkeypatch.setattr(bench, "PEAKS", (56.6, 6300.0, 4800.0))
+    t = bench.kernel_table("bn", {3: dict(cells=1000, update=2.0, face=1.0)}, bench.PEAKS)
+    u = t["3"]["update"]
+    assert u["algorithmic_gbs"] == pytest.approx(bench.algorithmic_bytes("bn", 3)["update"] * 1000 / 2e-3 / 1e9)
+    assert u["frac_hbm_peak"] == pytest.approx(u["algorithmic_gbs"] / bench.HBM_PEAK_GBS)
+    assert u["frac_measured_read_peak"] == pytest.approx(u["algorithmic_gbs"] / 6300.0)
+    r = bench.roofline_of(154576.0, 1000, 2.0, "k_predictor_lean<BnPDE,3>")
+    assert r["frac_measured"] == pytest.approx(r["achieved"] / 56.6) and r["bound"] == "mfma"
