@@ -188,6 +188,10 @@ void tune_kernels(int pde, int N, int persist_blocks, const Operators &ops, hipS
                     best = t.ums[u];
                     t.upd = u;
                 }
+            // the shared variant only when it wins by 5%: on the synthetic solve the two tie
+            // within 2% for Euler N = 2, 3, while on 512^2 meshes recompute is up to 38% faster
+            if (t.upd == kUpdShared && t.ums[kUpdRecompute] > 0.0 && t.ums[kUpdShared] > 0.95 * t.ums[kUpdRecompute])
+                t.upd = kUpdRecompute;
             (void)hipStreamSynchronize(st);
             for (int u = 0; u < kNumUpdVariants; u++) delete ut[u];
         }

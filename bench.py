@@ -79,8 +79,9 @@ def ensemble_specs(name, count=None):
         cfg["phi_eps"] = ics.BN_PHI_STABLE
     return ics.bn_ensemble(**cfg)
 SERVING = {"serving_4000": dict(count=4000, seed=7, steps=20)}
-SERVING_MIX = ("35% Euler isentropic vortex, 35% Euler four-quadrant Riemann problem, 30% Baer-Nunziato "
-               "stiffened-gas interface (circle, volume fractions 0.3 / 0.7)")
+SERVING_MIX = ("35% Euler isentropic vortex, 35% Euler four-quadrant Riemann problem (end time 20 initial time "
+               "steps), 30% Baer-Nunziato stiffened-gas interface (circle, volume fractions 0.3 / 0.7, end time 5 "
+               "initial time steps)")
 
 
 def log(*a):
@@ -691,7 +692,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20 ensemble, 200 plain)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (default 2 ensemble, 10 plain)")
     ap.add_argument("--workload", default="bn_ensemble_256", choices=sorted(PLAIN) + sorted(PLAIN_TO_T) + sorted(AD) + sorted(ENSEMBLES) + sorted(SERVING))
-    ap.add_argument("--serving-requests", type=int, default=400,
+    ap.add_argument("--serving-requests", type=int, default=1000,
                     help="ensemble workload: requests of the serving stream timed beside it at 0.5x, 0.8x and 1.0x "
                          "of saturation (0: none)")
     ap.add_argument("--count", type=int, default=None, help="ensemble size (default 256)")
