@@ -441,14 +441,18 @@ ADG_DEV double quad_dot4(double x, const double *c)
 }
 
 // ---------------------------------------------------------------------------
-// LDS doubles of a cell are padded by 16 so that consecutive cells start 32 banks apart: the
-// lanes of two cells in one half-wave then read the same offsets in different banks
-// (a sweep of pads 0..24 on MI355X left the Baer-Nunziato N=2, N=3 and Euler N=3 predictors
-// within 5% of each other; 16 was the fastest for Baer-Nunziato N=3)
+// LDS doubles of a cell are padded so that the cell's region starts NN doubles (mod 32) after
+// the previous cell's: a lane's node-indexed element (tt) then lies 8 bytes after the previous
+// lane's across cell boundaries, so any 32 consecutive lanes of a wave touch 32 distinct bank
+// pairs (64 banks of 4 bytes).  At N=3 this is the pad of 16 that a sweep of even pads 0..24
+// found best; at N=2 (nine lanes per cell) it is 5: the Baer-Nunziato N=2 predictor
+// 1.248 -> 1.17 ms on 4 x 384^2 meshes, where even pads had left 2-way conflicts.
 template <class P, int N>
 struct LeanShape {
     using S = Shape<P, N>;
-    static constexpr int CELL = S::V * S::NN + S::SLOT + 16;   // LDS doubles per cell
+    static constexpr int BODY = S::V * S::NN + S::SLOT;
+    static constexpr int PAD = ((S::NN - BODY) % 32 + 32) % 32;
+    static constexpr int CELL = BODY + PAD;                // LDS doubles per cell
     static constexpr int LDS = S::CPW * CELL + 2 * S::NN;  // + Kxi, dudx
 };
 
@@ -511,6 +515,25 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
             gxr[k] = sD[i + n * k];
         }
     }
+    // OPR (n <= 3): the operator rows of this lane's contractions held in registers for the
+    // whole kernel (4 n doubles) instead of re-read from LDS at every slice: halves the LDS
+    // reads of the contractions (Baer-Nunziato N=2 predictor -2%, with the pad above -7.5%)
+    constexpr bool OPR = !DPPX && n <= 3;
+    double okx[n], oky[n], ogx[n], ogy[n];
+    if constexpr (OPR) {
+#pragma unroll
+        for (int k = 0; k < n; k++) {
+            okx[k] = sK[k + n * i];
+            oky[k] = sK[k + n * j];
+            ogx[k] = sD[i + n * k];
+            ogy[k] = sD[j + n * k];
+        }
+    }
+    // operator entries of the x / y flux contractions and the x / y gradients
+    auto opkx = [&](int k) { if constexpr (OPR) return okx[k]; else return sK[k + n * i]; };
+    auto opky = [&](int k) { if constexpr (OPR) return oky[k]; else return sK[k + n * j]; };
+    auto opgx = [&](int k) { if constexpr (OPR) return ogx[k]; else return sD[i + n * k]; };
+    auto opgy = [&](int k) { if constexpr (OPR) return ogy[k]; else return sD[j + n * k]; };
     double q[V][n];
 #pragma unroll
     for (int v = 0; v < V; v++)
@@ -540,12 +563,12 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
             double acc = 0.0;
             if constexpr (!DPPX) {
 #pragma unroll
-                for (int k = 0; k < n; k++) acc = acc + buf[v * NN + j * n + k] * sK[k + n * i];
+                for (int k = 0; k < n; k++) acc = acc + buf[v * NN + j * n + k] * opkx(k);
                 ax[v] = acc;
             }
             acc = 0.0;
 #pragma unroll
-            for (int k = 0; k < n; k++) acc = acc + buf[V * NN + v * NN + i * n + k] * sK[k + n * j];
+            for (int k = 0; k < n; k++) acc = acc + buf[V * NN + v * NN + i * n + k] * opky(k);
             ay[v] = acc;
             if (P::kNcp) {  // gradients, Baer-Nunziato/space_time_predictor.f90:72,79
                 acc = 0.0;
@@ -553,12 +576,12 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
                     acc = quad_dot4(Q[v], gxr);
                 } else {
 #pragma unroll
-                    for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + k + n * j] * sD[i + n * k];
+                    for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + k + n * j] * opgx(k);
                 }
                 G[v] = rdx * acc;
                 acc = 0.0;
 #pragma unroll
-                for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + i + n * k] * sD[j + n * k];
+                for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + i + n * k] * opgy(k);
                 G[V + v] = rdy * acc;
             }
         }
@@ -643,12 +666,12 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
                     acc = axd[v];
                 } else {
 #pragma unroll
-                    for (int k = 0; k < n; k++) acc = acc + buf[v * NN + j * n + k] * sK[k + n * i];
+                    for (int k = 0; k < n; k++) acc = acc + buf[v * NN + j * n + k] * opkx(k);
                 }
                 const double rr = c0 * sU[v * NN + tt] - sx * acc;  // :66
                 acc = 0.0;
 #pragma unroll
-                for (int k = 0; k < n; k++) acc = acc + buf[V * NN + v * NN + i * n + k] * sK[k + n * j];
+                for (int k = 0; k < n; k++) acc = acc + buf[V * NN + v * NN + i * n + k] * opky(k);
                 rv[v] = rr - sy * acc;  // :72
                 if (P::kNcp) {          // gradients, Baer-Nunziato/space_time_predictor.f90:72,79
                     acc = 0.0;
@@ -656,12 +679,12 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
                         acc = quad_dot4(Q[v], gxr);
                     } else {
 #pragma unroll
-                        for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + k + n * j] * sD[i + n * k];
+                        for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + k + n * j] * opgx(k);
                     }
                     G[v] = rdx * acc;
                     acc = 0.0;
 #pragma unroll
-                    for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + i + n * k] * sD[j + n * k];
+                    for (int k = 0; k < n; k++) acc = acc + buf[2 * V * NN + v * NN + i + n * k] * opgy(k);
                     G[V + v] = rdy * acc;
                 }
             }
