@@ -50,9 +50,12 @@ sys.path.insert(0, os.path.join(ROOT, "ader-dg_amd"))
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix) dense peak
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per k_predictor launch, from rocprofv3 --pmc FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0),
-# scaled by a 256 MiB device copy; profiles/hbm_traffic_*.md holds the runs.
-PREDICTOR_TRAFFIC_BYTES = {"euler_vortex_512_n4": 965.7e6, "bn_ensemble_256_n3": 34.76e9}
+# HBM bytes per launch of the dominant predictor (k_predictor_lean), from two rocprofv3 --pmc
+# passes: read = 32/64/128 x TCC_EA0_RDREQ_{32B,64B,128B}_sum (FETCH_SIZE counts a 128-byte
+# request as 64 bytes on this GPU), write = WRITE_SIZE KiB x 1024; both checked against the 1 GiB
+# read and copy of --probe-only.  profiles/hbm_counter_calibration.md and profiles/hbm_traffic_*.md
+# hold the runs.
+PREDICTOR_TRAFFIC_BYTES = {"euler_vortex_512_n4": 789.5e6, "bn_ensemble_256_n3": 43.69e9}
 
 PLAIN = {
     # name: (ic, N, imax, jmax, xl, xr, bc)
@@ -227,12 +230,18 @@ def cpu_ensemble(specs, target_seconds, procs, seed=5):
 
 # ---------------------------------------------------------------------------
 def predictor_name(pde, N, persistent):
-    """Kernel name of the predictor variant the autotuner chose."""
+    """Kernel name of the predictor variant in use: the one ADERDG_PREDICTOR_VARIANT pins
+    (the library then skips the autotuner), else the autotuner's choice."""
     import aderdg
     names = {0: "k_predictor", 1: "k_predictor_nodes<.., 64>", 2: "k_predictor_nodes<.., 128>",
              3: "k_predictor_nodes<.., 256>", 4: "k_predictor_lean"}
-    t = aderdg.tune_query(pde, N, persistent)
-    return "%s<%sPDE,%d>" % (names[t[0] if t else 0], "Bn" if pde == "bn" else "Euler", N)
+    pinned = os.environ.get("ADERDG_PREDICTOR_VARIANT", "")
+    if pinned.strip().isdigit() and aderdg.pred_variant_valid(pde, N, int(pinned)):
+        v = int(pinned)
+    else:
+        t = aderdg.tune_query(pde, N, persistent)
+        v = t[0] if t else 0
+    return "%s<%sPDE,%d>" % (names[v], "Bn" if pde == "bn" else "Euler", N)
 
 
 def first_sweep_skipped(pde, N):
@@ -670,22 +679,6 @@ def probe():
         log("peak probe failed: %s" % e)
 
 
-def copy_probe(nbytes):
-    """One device-to-device copy of known size (the HIP runtime's copy kernel): the
-    FETCH_SIZE / WRITE_SIZE it reads under rocprofv3 --pmc calibrate the counters."""
-    import ctypes
-    hip = ctypes.CDLL("libamdhip64.so")
-    a, b = ctypes.c_void_p(), ctypes.c_void_p()
-    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
-    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
-    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
-    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
-    assert hip.hipDeviceSynchronize() == 0
-    hip.hipFree(a)
-    hip.hipFree(b)
-    log("copy probe: %d bytes device to device" % nbytes)
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -706,13 +699,18 @@ def main():
     ap.add_argument("--no-stable", action="store_true",
                     help="ensemble: skip the run of the same seeds with volume fractions 0.3 / 0.7")
     ap.add_argument("--graph", action="store_true", help="plain: time graph replay instead of evented launches")
-    ap.add_argument("--copy-probe-bytes", type=int, default=0,
-                    help="after the run, copy this many bytes device to device (counter calibration)")
+    ap.add_argument("--probe-only", action="store_true",
+                    help="only measure the GPU's FP64 and HBM peaks (aderdg_probe_peaks: a 1 GiB stream read and "
+                         "a 1 GiB copy of known size, the calibration of the HBM counters) and print them")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; --gpus %d is not supported "
               "(independent solves shard across GPUs without communication)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.probe_only:
+        probe()
+        print(json.dumps({"measured_peaks": PEAKS}), flush=True)
+        return
     ens = args.workload in ENSEMBLES
     if args.steps is None:
         args.steps = 20 if ens else 200
@@ -784,8 +782,6 @@ def main():
         out["cpu_baseline"] = cpu
     if PEAKS:
         out["measured_peaks"] = {"fp64_tflops": PEAKS[0], "hbm_read_gbs": PEAKS[1], "hbm_copy_gbs": PEAKS[2]}
-    if args.copy_probe_bytes:
-        copy_probe(args.copy_probe_bytes)
     print(json.dumps(out), flush=True)
 
 
