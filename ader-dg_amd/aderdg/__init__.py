@@ -224,6 +224,10 @@ def load_library(exact=False):
         "aderdg_run": [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp, _ip],
         "aderdg_predictor_cells": [vp, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp, _dp],
         "aderdg_predictor_cells_bn": [vp, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp, _dp, _dp],
+        "aderdg_volume_integral_cells": [vp, ctypes.c_int, _dp, _dp, _dp],
+        "aderdg_surface_integral_cells": [vp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp],
+        "aderdg_element_update_cells": [vp, ctypes.c_int, ctypes.c_double, _dp, _dp],
+        "aderdg_dmp_cells": [vp, ctypes.c_int, _ip, _dp, _dp, _dp, ctypes.c_double],
         "aderdg_timing_enable": [vp, ctypes.c_int],
         "aderdg_timing_get": [vp, _dp, ctypes.POINTER(ctypes.c_int64)],
         "aderdg_operators": [ctypes.c_int] + [_dp] * 11,
@@ -445,6 +449,51 @@ class Solver:
             return dict(qhi=qhi, Fhi=fhi, Shi=shi, qbnd=qb, Fbnd=fb)
         self._check(self.lib.aderdg_predictor_cells(self.h, nc, dt, _p(luh), _p(qhi), _p(fhi), _p(qb), _p(fb)))
         return dict(qhi=qhi, Fhi=fhi, qbnd=qb, Fbnd=fb)
+
+    def _cells(self, a, shape):
+        """a as a Fortran-ordered float64 array of shape + (ncells,)."""
+        a = np.asfortranarray(a, dtype=np.float64)
+        nc = a.size // int(np.prod(shape))
+        return np.asfortranarray(a.reshape(tuple(shape) + (nc,), order="F")), nc
+
+    def volume_integral_cells(self, lFhi, lShi=None):
+        """volume_integral(lduh, lFhi[, lShi]) of a batch of cells: returns lduh (V,n,n,ncells)."""
+        V, n = self.V, self.n
+        fhi, nc = self._cells(lFhi, (V, 2, n, n))
+        shi = self._cells(lShi, (V, n, n))[0] if lShi is not None else None
+        lduh = np.zeros((V, n, n, nc), order="F")
+        self._check(self.lib.aderdg_volume_integral_cells(self.h, nc, _p(lduh), _p(fhi),
+                                                          _p(shi) if shi is not None else None))
+        return lduh
+
+    def surface_integral_cells(self, lduh, lFl, lFr, lGb, lGt):
+        """surface_integral(lduh, lFl, lFr, lGb, lGt) of a batch of cells: returns the new lduh."""
+        V, n = self.V, self.n
+        d, nc = self._cells(np.array(lduh, copy=True), (V, n, n))
+        d = d.copy(order="F")
+        f = [self._cells(x, (V, n))[0] for x in (lFl, lFr, lGb, lGt)]
+        self._check(self.lib.aderdg_surface_integral_cells(self.h, nc, _p(d), *[_p(x) for x in f]))
+        return d
+
+    def element_update_cells(self, dt, luh, lduh):
+        """element_update(luh, lduh) with dt: returns (luh, lduh) after the update."""
+        V, n = self.V, self.n
+        u, nc = self._cells(luh, (V, n, n))
+        d = self._cells(lduh, (V, n, n))[0]
+        u, d = u.copy(order="F"), d.copy(order="F")
+        self._check(self.lib.aderdg_element_update_cells(self.h, nc, dt, _p(u), _p(d)))
+        return u, d
+
+    def dmp_cells(self, arguh, lmin, lmax, argmax=0.0):
+        """DMP(dmpresult, arguh, argLimiter, argmax) of a batch of cells (lmin, lmax: (V, ncells)):
+        returns dmpresult as a bool array (True where the cell passes)."""
+        V, n = self.V, self.n
+        u, nc = self._cells(arguh, (V, n, n))
+        lo = self._cells(lmin, (V,))[0]
+        hi = self._cells(lmax, (V,))[0]
+        r = np.zeros(nc, dtype=np.int32)
+        self._check(self.lib.aderdg_dmp_cells(self.h, nc, r.ctypes.data_as(_ip), _p(u), _p(lo), _p(hi), argmax))
+        return r != 0
 
     def seed_limiter(self, ic):
         """Troubled-cell seeding of the uploaded projection of formula ic (end of initialize)."""

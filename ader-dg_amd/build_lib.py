@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ADERDG_ARCH", "gfx950")
 
 SOURCES = ["operators.cpp", "initial.cpp", "step_kernels.hip", "ad_kernels.hip", "group.hip", "tune.hip", "aderdg_capi.hip",
-           "batch.hip", "server.hip", "probe.hip"]
+           "batch.hip", "server.hip", "probe.hip", "percell.hip"]
 COMMON = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-I" + os.path.join(ROOT, "include")]

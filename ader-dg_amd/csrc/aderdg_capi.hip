@@ -8,14 +8,17 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/aderdg.h"
 #include "capi_common.hpp"
 #include "group.hpp"
 #include "operators.hpp"
+#include "percell.hpp"
 #include "step_kernels.hpp"
 
 using namespace aderdg;
@@ -757,6 +760,137 @@ int aderdg_predictor_cells_noalloc_max(int pde, int N)
     if ((pde != ADERDG_PDE_EULER && pde != ADERDG_PDE_BN) || N < 1 || N > kMaxDegree) return 0;
     const size_t n = N + 1, V = pde == ADERDG_PDE_BN ? 9 : 4;
     return (int)(kScratchDoubles / (V * n * n * 5 + 8 * V * n));
+}
+
+namespace {
+// Device arrays of one per-cell call, freed when the call returns.
+struct CellArrays {
+    std::vector<void *> held;
+    ~CellArrays()
+    {
+        for (void *p : held) (void)hipFree(p);
+    }
+    void *bytes(size_t count)
+    {
+        void *p = nullptr;
+        if (hipMalloc(&p, count > 0 ? count : 8) != hipSuccess) return nullptr;
+        held.push_back(p);
+        return p;
+    }
+    double *doubles(size_t count) { return static_cast<double *>(bytes(count * sizeof(double))); }
+    int *ints(size_t count) { return static_cast<int *>(bytes(count * sizeof(int))); }
+};
+
+// Common checks of the per-cell entries: an Euler or Baer-Nunziato context and ncells >= 1.
+int cells_check(aderdg_ctx *c, int ncells, const char *who)
+{
+    if (!c || ncells < 1) return fail(ADERDG_EINVAL, "%s: bad argument", who);
+    if (c->desc.pde != ADERDG_PDE_EULER && c->desc.pde != ADERDG_PDE_BN)
+        return fail(ADERDG_EINVAL, "%s: Euler or Baer-Nunziato contexts only", who);
+    if (hipSetDevice(c->desc.device) != hipSuccess) return fail(ADERDG_EHIP, "%s: hipSetDevice failed", who);
+    return 0;
+}
+
+hipError_t up(double *d, const double *h, size_t count, hipStream_t st)
+{
+    return hipMemcpyAsync(d, h, count * sizeof(double), hipMemcpyHostToDevice, st);
+}
+
+// Launch kind on the uploaded arrays, read back the outputs (host, device, doubles) and wait.
+int cells_run(aderdg_ctx *c, int kind, int ncells, const CellBufs &d, double dt, double argmax, hipError_t e,
+              std::initializer_list<std::tuple<void *, const void *, size_t>> outs, const char *who)
+{
+    const MeshArgs m = c->g.mesh(c->slot);
+    if (e == hipSuccess) e = launch_cells(kind, m, c->g.b.op, c->N, ncells, d, dt, argmax, c->stream);
+    for (const auto &o : outs)
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(std::get<0>(o), std::get<1>(o), std::get<2>(o), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(ADERDG_EHIP, "%s: %s", who, hipGetErrorString(e));
+    return 0;
+}
+}  // namespace
+
+int aderdg_volume_integral_cells(aderdg_ctx *c, int ncells, double *lduh, const double *lFhi, const double *lShi)
+{
+    const char *who = "aderdg_volume_integral_cells";
+    if (int r = cells_check(c, ncells, who)) return r;
+    const bool bn = c->desc.pde == ADERDG_PDE_BN;
+    if (!lduh || !lFhi || (bn && !lShi)) return fail(ADERDG_EINVAL, "%s: null array%s", who, bn ? " (lShi is required)" : "");
+    const size_t blk = (size_t)c->V * c->n * c->n * ncells;
+    CellArrays a;
+    CellBufs d;
+    d.lduh = a.doubles(blk);
+    double *fhi = a.doubles(2 * blk), *shi = bn ? a.doubles(blk) : nullptr;
+    if (!d.lduh || !fhi || (bn && !shi)) return fail(ADERDG_EHIP, "%s: device allocation failed", who);
+    d.fhi = fhi;
+    d.shi = shi;
+    hipError_t e = up(fhi, lFhi, 2 * blk, c->stream);
+    if (e == hipSuccess && bn) e = up(shi, lShi, blk, c->stream);
+    return cells_run(c, kCellVolume, ncells, d, 0.0, 0.0, e, {{lduh, d.lduh, blk * sizeof(double)}}, who);
+}
+
+int aderdg_surface_integral_cells(aderdg_ctx *c, int ncells, double *lduh, const double *lFl, const double *lFr,
+                                  const double *lGb, const double *lGt)
+{
+    const char *who = "aderdg_surface_integral_cells";
+    if (int r = cells_check(c, ncells, who)) return r;
+    if (!lduh || !lFl || !lFr || !lGb || !lGt) return fail(ADERDG_EINVAL, "%s: null array", who);
+    const size_t blk = (size_t)c->V * c->n * c->n * ncells, fln = (size_t)c->V * c->n * ncells;
+    CellArrays a;
+    CellBufs d;
+    d.lduh = a.doubles(blk);
+    double *f[4];
+    for (double *&p : f) p = a.doubles(fln);
+    if (!d.lduh || !f[0] || !f[1] || !f[2] || !f[3]) return fail(ADERDG_EHIP, "%s: device allocation failed", who);
+    d.fl = f[0];
+    d.fr = f[1];
+    d.gb = f[2];
+    d.gt = f[3];
+    hipError_t e = up(d.lduh, lduh, blk, c->stream);
+    const double *h[4] = {lFl, lFr, lGb, lGt};
+    for (int k = 0; k < 4 && e == hipSuccess; k++) e = up(f[k], h[k], fln, c->stream);
+    return cells_run(c, kCellSurface, ncells, d, 0.0, 0.0, e, {{lduh, d.lduh, blk * sizeof(double)}}, who);
+}
+
+int aderdg_element_update_cells(aderdg_ctx *c, int ncells, double dt, double *luh, double *lduh)
+{
+    const char *who = "aderdg_element_update_cells";
+    if (int r = cells_check(c, ncells, who)) return r;
+    if (!luh || !lduh) return fail(ADERDG_EINVAL, "%s: null array", who);
+    const size_t blk = (size_t)c->V * c->n * c->n * ncells;
+    CellArrays a;
+    CellBufs d;
+    d.luh = a.doubles(blk);
+    d.lduh = a.doubles(blk);
+    if (!d.luh || !d.lduh) return fail(ADERDG_EHIP, "%s: device allocation failed", who);
+    hipError_t e = up(d.luh, luh, blk, c->stream);
+    if (e == hipSuccess) e = up(d.lduh, lduh, blk, c->stream);
+    return cells_run(c, kCellUpdate, ncells, d, dt, 0.0, e,
+                     {{luh, d.luh, blk * sizeof(double)}, {lduh, d.lduh, blk * sizeof(double)}}, who);
+}
+
+int aderdg_dmp_cells(aderdg_ctx *c, int ncells, int32_t *dmpresult, const double *arguh, const double *lmin,
+                     const double *lmax, double argmax)
+{
+    const char *who = "aderdg_dmp_cells";
+    if (int r = cells_check(c, ncells, who)) return r;
+    if (!dmpresult || !arguh || !lmin || !lmax) return fail(ADERDG_EINVAL, "%s: null array", who);
+    const size_t V = c->V, n = c->n, s = c->s, nc = ncells, blk = V * n * n * nc;
+    CellArrays a;
+    CellBufs d;
+    d.luh = a.doubles(blk);
+    double *lo = a.doubles(V * nc), *hi = a.doubles(V * nc);
+    d.work = a.doubles(V * (s * s + n * n) * nc);
+    d.result = a.ints(nc);
+    if (!d.luh || !lo || !hi || !d.work || !d.result) return fail(ADERDG_EHIP, "%s: device allocation failed", who);
+    d.lmin = lo;
+    d.lmax = hi;
+    static_assert(sizeof(int) == sizeof(int32_t), "dmpresult is read back as int");
+    hipError_t e = up(d.luh, arguh, blk, c->stream);
+    if (e == hipSuccess) e = up(lo, lmin, V * nc, c->stream);
+    if (e == hipSuccess) e = up(hi, lmax, V * nc, c->stream);
+    return cells_run(c, kCellDmp, ncells, d, 0.0, argmax, e, {{dmpresult, d.result, nc * sizeof(int)}}, who);
 }
 
 int aderdg_timing_enable(aderdg_ctx *c, int on)

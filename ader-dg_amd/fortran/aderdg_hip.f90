@@ -137,6 +137,49 @@ module aderdg_hip
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx
         end function
+        ! per-cell forms of the element routines, reference argument lists (ncells cells)
+        integer(c_int) function aderdg_predictor_cells(ctx, ncells, dt, luh, lqhi, lFhi, lqbnd, lFbnd) &
+                bind(C, name="aderdg_predictor_cells")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: ncells
+            real(c_double), value :: dt
+            real(c_double), intent(in) :: luh(*)
+            real(c_double), intent(out) :: lqhi(*), lFhi(*), lqbnd(*), lFbnd(*)
+        end function
+        integer(c_int) function aderdg_volume_integral_cells(ctx, ncells, lduh, lFhi, lShi) &
+                bind(C, name="aderdg_volume_integral_cells")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, lShi
+            integer(c_int), value :: ncells
+            real(c_double), intent(out) :: lduh(*)
+            real(c_double), intent(in) :: lFhi(*)
+        end function
+        integer(c_int) function aderdg_surface_integral_cells(ctx, ncells, lduh, lFl, lFr, lGb, lGt) &
+                bind(C, name="aderdg_surface_integral_cells")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: ncells
+            real(c_double), intent(inout) :: lduh(*)
+            real(c_double), intent(in) :: lFl(*), lFr(*), lGb(*), lGt(*)
+        end function
+        integer(c_int) function aderdg_element_update_cells(ctx, ncells, dt, luh, lduh) &
+                bind(C, name="aderdg_element_update_cells")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: ncells
+            real(c_double), value :: dt
+            real(c_double), intent(inout) :: luh(*), lduh(*)
+        end function
+        integer(c_int) function aderdg_dmp_cells(ctx, ncells, dmpresult, arguh, lmin, lmax, argmax) &
+                bind(C, name="aderdg_dmp_cells")
+            import :: c_int, c_ptr, c_int32_t, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: ncells
+            integer(c_int32_t), intent(out) :: dmpresult(*)
+            real(c_double), intent(in) :: arguh(*), lmin(*), lmax(*)
+            real(c_double), value :: argmax
+        end function
     end interface
 
 contains

@@ -35,20 +35,7 @@ contains
         integer(c_int32_t), allocatable :: st(:,:)
         real(c_double), allocatable :: lh(:,:,:,:,:)
         integer :: i, j
-        desc%pde = ADERDG_PDE_EULER
-        desc%N = N
-        desc%imax = IMAX
-        desc%jmax = JMAX
-        desc%xl = xL
-        desc%xr = xR
-        desc%gamma = EQN%GAMMA
-        desc%cfl = CFL
-        if (ICType .eq. 11) then   ! the Double Mach Reflection set of Euler/riemann_problem.f90:10-116
-            desc%bc_mode = ADERDG_BC_DMR
-        else                       ! params.ini codes per side (Euler/params.ini:8-11)
-            desc%bc_mode = ADERDG_BC_SIDES
-            desc%bc = (/ bL, bR, bB, bT /)
-        end if
+        call aderdg_ref_desc(desc)
         call aderdg_check(aderdg_create(desc, ctx), 'aderdg_create')
         call aderdg_check(aderdg_upload_uh(ctx, uh), 'aderdg_upload_uh')
         allocate(st(IMAX, JMAX), lh(nVar, nSubLimV(1), nSubLimV(2), IMAX, JMAX))
@@ -64,6 +51,26 @@ contains
         end do
         call aderdg_check(aderdg_set_limiter(ctx, st, lh), 'aderdg_set_limiter')
         deallocate(st, lh)
+    end subroutine
+
+    ! The library's solve descriptor for what initialize read from params.ini: mesh, gamma,
+    ! CFL and the boundaries (the DMR set for ICType 11, else the codes per side)
+    subroutine aderdg_ref_desc(desc)
+        type(aderdg_desc_t), intent(out) :: desc
+        desc%pde = ADERDG_PDE_EULER
+        desc%N = N
+        desc%imax = IMAX
+        desc%jmax = JMAX
+        desc%xl = xL
+        desc%xr = xR
+        desc%gamma = EQN%GAMMA
+        desc%cfl = CFL
+        if (ICType .eq. 11) then   ! the Double Mach Reflection set of Euler/riemann_problem.f90:10-116
+            desc%bc_mode = ADERDG_BC_DMR
+        else                       ! params.ini codes per side (Euler/params.ini:8-11)
+            desc%bc_mode = ADERDG_BC_SIDES
+            desc%bc = (/ bL, bR, bB, bT /)
+        end if
     end subroutine
 
     ! calc_time_step (Euler/calc_time_step.f90:9-52): dt from the device-resident solution

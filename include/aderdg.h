@@ -196,6 +196,37 @@ int aderdg_predictor_cells(aderdg_ctx *ctx, int ncells, double dt, const double 
 int aderdg_predictor_cells_bn(aderdg_ctx *ctx, int ncells, double dt, const double *luh, double *lqhi, double *lFhi,
                               double *lShi, double *lqbnd, double *lFbnd);
 
+/* Per-cell forms of the other element routines, with the reference argument lists, for
+ * callers that keep the reference's loops over cells (the symbol-compatible Fortran shim
+ * ader-dg_amd/fortran/aderdg_percell.f90).  ncells independent cells, host arrays in the
+ * reference's per-cell layout repeated cell after cell; synchronous.  Euler and
+ * Baer-Nunziato contexts; the context supplies the operators, mesh spacing and equation of
+ * state.  In the build without contraction every result is bit-identical to the reference
+ * routine's. */
+
+/* volume_integral(lduh, lFhi) (Euler/volume_integral.f90:8-36) and its Baer-Nunziato form
+ * volume_integral(lduh, lFhi, lShi) (Baer-Nunziato/volume_integral.f90): lduh (V,n,n) per
+ * cell out; lFhi (V,2,n,n) per cell in; lShi (V,n,n) per cell, required for Baer-Nunziato,
+ * ignored (may be NULL) for Euler. */
+int aderdg_volume_integral_cells(aderdg_ctx *ctx, int ncells, double *lduh, const double *lFhi, const double *lShi);
+
+/* surface_integral(lduh, lFl, lFr, lGb, lGt) (Euler/surface_integral.f90:8-39): lduh (V,n,n)
+ * per cell in and out; the left, right, bottom and top face fluxes (V,n) per cell in. */
+int aderdg_surface_integral_cells(aderdg_ctx *ctx, int ncells, double *lduh, const double *lFl, const double *lFr,
+                                  const double *lGb, const double *lGt);
+
+/* element_update(luh, lduh) (Euler/element_update.f90:8-30) with the module's dt given:
+ * lduh is divided by the Gauss weights in place and luh advanced by dt * lduh. */
+int aderdg_element_update_cells(aderdg_ctx *ctx, int ncells, double dt, double *luh, double *lduh);
+
+/* DMP(dmpresult, arguh, argLimiter, argmax) (Euler/limiter.f90:179-245; Baer-Nunziato/limiter.f90
+ * with its PDEAssurePositivity): dmpresult[c] = 1 where the cell passes (the reference's
+ * .true.), 0 where it is troubled; arguh (V,n,n) per cell; lmin, lmax (V) per cell are the
+ * argLimiter%lmin / %lmax the reference reads (tLimiter, Euler/ader_dg.f90:73-79, is not
+ * C-interoperable: the shim passes those two members). */
+int aderdg_dmp_cells(aderdg_ctx *ctx, int ncells, int32_t *dmpresult, const double *arguh, const double *lmin,
+                     const double *lmax, double argmax);
+
 /* aderdg_predictor_cells / aderdg_predictor_cells_bn without any device allocation: no
  * counterpart in the reference.  For callers that cannot allocate device memory (code under
  * stream capture, tight serving loops, harnesses that forbid hipMalloc): the device staging of

@@ -11,6 +11,12 @@
 #                                      ader-dg_amd/fortran/aderdg_hip.f90 + aderdg_refglue.f90 and
 #                                      libaderdg_amd_exact.so: the drop-in run of the reference
 #                                      program on the GPU
+#   oracle/_ref/percell_euler_n<N>     (same mode) the reference's Euler main.f90 as it is, linked
+#                                      with ader-dg_amd/fortran/aderdg_percell.f90 in place of the
+#                                      reference's space_time_predictor, volume_integral,
+#                                      surface_integral, element_update and DMP: the per-cell tier of
+#                                      the drop-in (the reference's DMP symbol in limiter.o is made
+#                                      weak, so the shim's DMP serves every caller)
 #
 # TEST INFRASTRUCTURE ONLY: the product never links or loads these files.
 #
@@ -110,6 +116,34 @@ build_refmain() {
     echo "built $OUT/refmain_euler_n${n}"
 }
 
+# build_percell <N> [fixes]: the unedited reference Euler program over the per-cell shim
+build_percell() {
+    local n=$1
+    shift
+    local w="$WORK/percell_n${n}" pkg="$HERE/../ader-dg_amd"
+    prepare "$w" Euler "$n" "$@"
+    local srcs=""
+    for f in $(routines "$w"); do
+        case $(basename "$f") in
+            space_time_predictor.f90|volume_integral.f90|surface_integral.f90|element_update.f90) ;;
+            *) srcs="$srcs $f";;
+        esac
+    done
+    (cd "$w" && $FC $FFLAGS -c ader_dg.f90 && \
+        for f in $srcs; do $FC $FFLAGS -c "$f"; done && \
+        $FC $FFLAGS -c "$pkg/fortran/aderdg_hip.f90" && \
+        $FC $FFLAGS -c "$pkg/fortran/aderdg_refglue.f90" && \
+        $FC $FFLAGS -c "$pkg/fortran/aderdg_percell.f90" && \
+        $FC $FFLAGS -c main.f90)
+    objcopy --weaken-symbol=dmp_ "$w/limiter.o"
+    local objs=""
+    for f in $srcs; do objs="$objs $w/$(basename "${f%.f90}").o"; done
+    $FC $FFLAGS -o "$OUT/percell_euler_n${n}" "$w/ader_dg.o" $objs "$w/aderdg_hip.o" "$w/aderdg_refglue.o" \
+        "$w/aderdg_percell.o" "$w/main.o" -L"$pkg/lib" -laderdg_amd_exact \
+        -Wl,-rpath,'$ORIGIN/../../ader-dg_amd/lib'
+    echo "built $OUT/percell_euler_n${n}"
+}
+
 # The Euler subcell recompute applies Double-Mach-Reflection ghost states on the
 # left, bottom and top boundaries unconditionally (Euler/limiter.f90:491-522).
 # Guarding them with ICType == 11 (DMR) leaves DMR runs unchanged and lets the
@@ -125,6 +159,7 @@ EULER_FIXES=(
 if [ "${1:-all}" = refmain ]; then
     for n in 2 3; do
         build_refmain "$n" "${EULER_FIXES[@]}"
+        build_percell "$n" "${EULER_FIXES[@]}"
     done
     exit 0
 fi
