@@ -1375,8 +1375,11 @@ __global__ __launch_bounds__(kUpdBlock<P>) void k_update_shared(StepBufs b)
             d = d - wi / m.dy * (Gt * op->frc[j] - Gb * op->flc[j]);
             d = d / (wi * wj);  // element update, Euler/element_update.f90:218-226
             un[v] = uo + dt * d;
-            b.uh[par ^ 1][g * V * NN + V * tt + v] = un[v];
         }
+        // stored once every value is formed: a store to uh[par ^ 1] between the loads would
+        // hold the next variable's loads behind it (the compiler cannot rule out aliasing)
+#pragma unroll
+        for (int v = 0; v < V; v++) b.uh[par ^ 1][g * V * NN + V * tt + v] = un[v];
     }
     if (own) {
 #pragma unroll
@@ -1622,8 +1625,37 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
     if constexpr (ROWS) {
     const int nvalid = m.ncell - w.y < CPW ? m.ncell - w.y : CPW;  // cells of this entry
     const double wi = sopw[i], wj = sopw[j];
-    // the surface-integral terms of this thread's node first (their loads overlap the row
-    // loads below), Euler/surface_integral.f90:182-196
+    // Every load of the entry is issued before the first of them is used, so that the entry
+    // waits for one memory round trip, not one per loop trip: the uh / lduh rows and (FACES)
+    // the face lines into registers, then into LDS.  PAIRS: a cell's row has an even number
+    // of doubles, so the rows of every entry start 16-byte aligned and move as double pairs.
+    // Otherwise the rows move as pairs from the first 16-byte boundary: h leading doubles
+    // (0 or 1) before it, and one trailing double when the rest is odd.
+    constexpr bool PAIRS = (V * NN) % 2 == 0;
+    constexpr int KR = ((CPW * V * NN) / 2 + kUpdBlock<P> - 1) / kUpdBlock<P>;  // row pairs per lane
+    const int cnt = nvalid * V * NN;
+    const size_t base = ((size_t)sa.cell0 + w.y) * V * NN;
+    const int h = PAIRS ? 0 : (int)(base & 1), np = (cnt - h) / 2;
+    double2 ru[KR], rl[KR];
+    {
+        const double2 *u2 = (const double2 *)(uh_cur + base + h), *l2 = (const double2 *)(b.lduh + base + h);
+#pragma unroll
+        for (int q = 0; q < KR; q++) {
+            const int k = min(tid + q * kUpdBlock<P>, np - 1);  // lanes past the end reload the last pair
+            ru[q] = u2[k];
+            if (!SEED) rl[q] = l2[k];
+        }
+    }
+    // the odd ends of unpaired rows: the leading double (h = 1) and the trailing one
+    const int ee = tid == 0 ? 0 : cnt - 1;
+    const bool end_lane = !PAIRS && ((tid == 0 && h) || (tid == 1 && ((cnt - h) & 1)));
+    double eu = 0.0, el = 0.0;
+    if (end_lane) {
+        eu = uh_cur[base + ee];
+        if (!SEED) el = b.lduh[base + ee];
+    }
+    // the surface-integral terms of this thread's node (one load per node and value),
+    // Euler/surface_integral.f90:182-196
     double sx[V], sy[V];
     if (!FACES && !SEED && act) {
         const int ci = cell % m.imax, cj = cell / m.imax;
@@ -1643,64 +1675,59 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
     // FACES: the face lines of the entry's cells in LDS, one lane per value: a cell's n
     // nodes of a row (column) read the same left/right (bottom/top) value, which the loads
     // above issue n times.  sface[side][v][p][cell]; sides left, right, bottom, top.
-    // Lane tid takes cell c = tid % CPW of the entry and the values r = tid / CPW + k FG of
+    // Lane tid takes cell c = tid % CPW of the entry and the values r = tid / CPW + q FG of
     // its lines (r = (side V + v) n + p), so its faces are located once.
-    if (FACES && !SEED) {
-        constexpr int FG = kUpdBlock<P> / CPW;  // lanes per cell
-        const int c = tid % CPW;
-        if (tid < FG * CPW && c < nvalid) {
-            const size_t nfx = b.GX, nfy = b.GY;
-            const int sp = P::kNcp ? 1 : 0;  // right/top faces take D+ (Baer-Nunziato/main.f90:68)
-            const int lcell = w.y + c, ci = lcell % m.imax, cj = lcell / m.imax;
-            const size_t fl = sa.fx0 + ci + (size_t)(m.imax + 1) * cj, fb = sa.fy0 + ci + (size_t)m.imax * cj;
-#pragma unroll 3
-            for (int r = tid / CPW; r < 4 * V * n; r += FG) {
+    constexpr int FG = kUpdBlock<P> / CPW;                            // lanes per cell
+    constexpr int FR = FACES ? (4 * V * n + FG - 1) / FG : 1;         // face values per lane
+    const int fc = tid % CPW;
+    const bool face_lane = FACES && !SEED && tid < FG * CPW && fc < nvalid;
+    double xs[FR];
+    if (face_lane) {
+        const size_t nfx = b.GX, nfy = b.GY;
+        const int sp = P::kNcp ? 1 : 0;  // right/top faces take D+ (Baer-Nunziato/main.f90:68)
+        const int lcell = w.y + fc, ci = lcell % m.imax, cj = lcell / m.imax;
+        const size_t fl = sa.fx0 + ci + (size_t)(m.imax + 1) * cj, fb = sa.fy0 + ci + (size_t)m.imax * cj;
+#pragma unroll
+        for (int q = 0; q < FR; q++) {
+            const int r = tid / CPW + q * FG;
+            if (r < 4 * V * n) {
                 const int p = r % n, fv = r / n, v = fv % V, f = fv / V;
-                double x;
                 if (f < 2)
-                    x = b.xflux[((size_t)((f ? sp : 0) * V + v) * n + p) * nfx + fl + f];
+                    xs[q] = b.xflux[((size_t)((f ? sp : 0) * V + v) * n + p) * nfx + fl + f];
                 else
-                    x = b.yflux[((size_t)((f == 3 ? sp : 0) * V + v) * n + p) * nfy + fb + (f == 3 ? m.imax : 0)];
-                sface[r * CPW + c] = x;
+                    xs[q] = b.yflux[((size_t)((f == 3 ? sp : 0) * V + v) * n + p) * nfy + fb + (f == 3 ? m.imax : 0)];
             }
         }
     }
-    const int cnt = nvalid * V * NN;
-    const size_t base = ((size_t)sa.cell0 + w.y) * V * NN;
-    double *flat = &srow[0][0];
-    // PAIRS: a cell's row has an even number of doubles, so the rows of every entry start
-    // 16-byte aligned and move as double pairs.  Otherwise the rows move as pairs from the
-    // first 16-byte boundary: h leading doubles (0 or 1) before it, and one trailing double
-    // when the rest is odd.
-    constexpr bool PAIRS = (V * NN) % 2 == 0;
-    const int h = PAIRS ? 0 : (int)(base & 1), np = (cnt - h) / 2;
-    if constexpr (PAIRS) {
-        const double2 *u2 = (const double2 *)(uh_cur + base), *l2 = (const double2 *)(b.lduh + base);
-        double2 *f2 = (double2 *)flat, *s2 = (double2 *)&ssuh[0][0];
-#pragma unroll 2
-        for (int k = tid; k < np; k += kUpdBlock<P>) {
-            f2[k] = u2[k];
-            if (!SEED) s2[k] = l2[k];
-        }
-    } else {
-        const double2 *u2 = (const double2 *)(uh_cur + base + h), *l2 = (const double2 *)(b.lduh + base + h);
-        double *sl = &ssuh[0][0];
-#pragma unroll 2
-        for (int k = tid; k < np; k += kUpdBlock<P>) {
-            const double2 x = u2[k];
-            flat[h + 2 * k] = x.x;
-            flat[h + 2 * k + 1] = x.y;
+    // now into LDS, unguarded so that the compiler cannot sink each load into a guarded
+    // store: a lane past the end writes the last pair again, the value its owner writes
+    double *flat = &srow[0][0], *sl = &ssuh[0][0];
+#pragma unroll
+    for (int q = 0; q < KR; q++) {
+        const int k = min(tid + q * kUpdBlock<P>, np - 1);
+        if constexpr (PAIRS) {
+            ((double2 *)flat)[k] = ru[q];
+            if (!SEED) ((double2 *)sl)[k] = rl[q];
+        } else {
+            flat[h + 2 * k] = ru[q].x;
+            flat[h + 2 * k + 1] = ru[q].y;
             if (!SEED) {
-                const double2 y = l2[k];
-                sl[h + 2 * k] = y.x;
-                sl[h + 2 * k + 1] = y.y;
+                sl[h + 2 * k] = rl[q].x;
+                sl[h + 2 * k + 1] = rl[q].y;
             }
         }
-        // the odd ends: the leading double (h = 1) and the trailing one ((cnt - h) odd)
-        const int e = tid == 0 ? 0 : cnt - 1;
-        if ((tid == 0 && h) || (tid == 1 && ((cnt - h) & 1))) {
-            flat[e] = uh_cur[base + e];
-            if (!SEED) sl[e] = b.lduh[base + e];
+    }
+    if (end_lane) {
+        flat[ee] = eu;
+        if (!SEED) sl[ee] = el;
+    }
+    // the face lines into LDS after the rows: the row stores wait for loads issued after the
+    // face loads, so that both arrive in one round trip
+    if (face_lane) {
+#pragma unroll
+        for (int q = 0; q < FR; q++) {
+            const int r = tid / CPW + q * FG;
+            if (r < 4 * V * n) sface[r * CPW + fc] = xs[q];
         }
     }
     __syncthreads();  // the entry's uh and lduh rows (and face lines) in LDS
@@ -1775,8 +1802,11 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
             d = d - wi / m.dy * (Gt * sopfr[j] - Gb * sopfl[j]);
             d = d / (wi * wj);  // element update, Euler/element_update.f90:218-226
             un[v] = uo + dt * d;
-            b.uh[par ^ 1][g * V * NN + V * tt + v] = un[v];
         }
+        // stored once every value is formed: a store to uh[par ^ 1] between the loads would
+        // hold the next variable's loads behind it (the compiler cannot rule out aliasing)
+#pragma unroll
+        for (int v = 0; v < V; v++) b.uh[par ^ 1][g * V * NN + V * tt + v] = un[v];
     }
     if (own) {
 #pragma unroll
