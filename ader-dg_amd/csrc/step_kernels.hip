@@ -159,7 +159,9 @@ ADG_DEV void predictor_finish(const M &m, const StepBufs &b, const TestBufs &t, 
         const int ci = cell % m.imax, cj = cell / m.imax;
         const size_t row = (size_t)v * b.G + cell0;
         double mn = b.lmin_old[row + cell], mx = b.lmax_old[row + cell];
+#pragma unroll
         for (int jj = -1; jj <= 1; jj++)
+#pragma unroll
             for (int ii = -1; ii <= 1; ii++) {
                 const int nb = nb_index(ci, ii, m.imax, m.wrap & 1) + m.imax * nb_index(cj, jj, m.jmax, m.wrap >> 1);
                 mn = f_min(mn, b.lmin_old[row + nb]);
@@ -2357,6 +2359,7 @@ __global__ __launch_bounds__(kSubBnBlock) __attribute__((amdgpu_waves_per_eu(2))
     __shared__ double sDt[NN];
     __shared__ double sRng[8 * V + 1];
     __shared__ int sFlag, sNth;
+    __shared__ int sNb[9], sLs[9];  // the 3x3 neighbourhood: group cell and pool slot (-1 unlimited)
     const OpPtr op = (OpPtr)b.op;
     const int lane = threadIdx.x;
     const int count = b.counters[kCntList];
@@ -2375,13 +2378,21 @@ __global__ __launch_bounds__(kSubBnBlock) __attribute__((amdgpu_waves_per_eu(2))
         const int cell = gcell - sa.cell0;  // local cell of the solve
         const int flag = b.rc_final[gcell];
         const int ci = cell % m.imax, cj = cell / m.imax;
+        // the neighbourhood's cells and limiter slots once per cell, status and slot loaded
+        // side by side (one memory round trip, not two per task of the passes below)
+        if (lane < 9) {
+            const int di = lane % 3 - 1, dj = lane / 3 - 1;
+            const int nb = sa.cell0 + nb_index(ci, di, m.imax, m.wrap & 1) + m.imax * nb_index(cj, dj, m.jmax, m.wrap >> 1);
+            const int stt = b.status[par][nb], ls = b.lh_slot[par][nb];
+            sNb[lane] = nb;
+            sLs[lane] = stt != 0 ? ls : -1;  // lim_slot
+        }
+        __syncthreads();
         // blocks (di,dj) of the 3x3 Voronoi neighbourhood, :463-479
         for (int task = lane; task < 9 * n * s; task += NT) {
             const int blk = task / (n * s), r = task - blk * n * s, a = r % n, qq = r / n;
-            const int di = blk % 3 - 1, dj = blk / 3 - 1;
-            const int nb = sa.cell0 + nb_index(ci, di, m.imax, m.wrap & 1) + m.imax * nb_index(cj, dj, m.jmax, m.wrap >> 1);
-            if (lim_slot(b, par, nb) >= 0) continue;
-            const double *u = b.uh[par] + (size_t)nb * V * NN;
+            if (sLs[blk] >= 0) continue;
+            const double *u = b.uh[par] + (size_t)sNb[blk] * V * NN;
 #pragma unroll
             for (int v = 0; v < V; v++) {
                 double acc = 0.0;
@@ -2396,8 +2407,7 @@ __global__ __launch_bounds__(kSubBnBlock) __attribute__((amdgpu_waves_per_eu(2))
             const int di = ii < 1 ? -1 : (ii > s ? 1 : 0), dj = jj < 1 ? -1 : (jj > s ? 1 : 0);
             const int p = ii - 1 - di * s, qq = jj - 1 - dj * s;
             const int blk = (di + 1) + 3 * (dj + 1);
-            const int nb = sa.cell0 + nb_index(ci, di, m.imax, m.wrap & 1) + m.imax * nb_index(cj, dj, m.jmax, m.wrap >> 1);
-            const int ls = lim_slot(b, par, nb);
+            const int ls = sLs[blk];
             if (ls < 0) {
 #pragma unroll
                 for (int v = 0; v < V; v++) {
