@@ -510,7 +510,7 @@ class Solver:
         return list(ms), list(ln)
 
 
-PREDICTOR_VARIANTS = ("time-line", "node-64", "node-128", "node-256", "time-line-lean")
+PREDICTOR_VARIANTS = ("time-line", "node-64", "node-128", "node-256", "time-line-lean", "time-line-mfma")
 
 
 UPDATE_VARIANTS = ("recompute", "shared")
@@ -537,10 +537,14 @@ def tune_query(pde, N, persistent, device=0, exact=False, kernel="predictor"):
     return v.value, list(ms)
 
 
-def pred_variant_valid(pde, N, variant):
+def pred_variant_valid(pde, N, variant, exact=True):
+    """Whether predictor variant `variant` exists for (pde, N) in the exact (parity) build or
+    the product build: the matrix-core variant 5 is N=3 of the product build only."""
     k = PDE[pde] if isinstance(pde, str) else int(pde)
     if k == 2:
         return variant == 0
+    if variant == 5:
+        return N == 3 and not exact
     return variant in (0, 2, 3, 4) or (variant == 1 and (N + 1) ** 3 <= 64)
 
 

@@ -40,6 +40,12 @@ namespace {
 // workgroup sizes: Baer-Nunziato keeps three times the data per node
 template <class P>
 constexpr int kPredBlock = P::V == 4 ? 256 : 64;
+// the matrix-core predictor sums in the matrix unit's order: product build only
+#if defined(ADERDG_EXACT)
+constexpr bool kMfmaBuild = false;
+#else
+constexpr bool kMfmaBuild = true;
+#endif
 template <class P>
 constexpr int kUpdBlock = P::V == 4 ? 128 : 64;
 // threads of the Baer-Nunziato subcell workgroup (one listed cell each): one interior face per thread
@@ -443,6 +449,33 @@ ADG_DEV double quad_dot4(double x, const double *c)
 }
 
 // ---------------------------------------------------------------------------
+// Matrix-core contractions (product build, N = 3).  v_mfma_f64_4x4x4_4b_f64 multiplies four
+// independent 4x4 blocks per wave; the block of lane L is (L >> 2) & 3, and within it
+//   A[m][k] sits in lane 16 k + 4 blk + m,  B[k][n] in lane 16 k + 4 blk + n,
+//   D[m][n] in lane 16 m + 4 blk + n
+// (measured on gfx950 with one-hot operands).  A cell of 16 nodes is one block when lane
+// 16 j + 4 blk + i holds node (i, j): then B = a nodal field X(i, j) contracts over j
+// (D = Op . X, Op in A), and the lane-transposed field X(j, i) in A contracts over i
+// (D = X^T . Op, Op in B), each landing in the node layout again.
+ADG_DEV double mfma4(double a, double b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, 0.0, 0, 0, 0);
+#else
+    return a * b;  // the host pass only type-checks device code
+#endif
+}
+// x of lane addr / 4 of this wave (ds_bpermute on both halves: through the LDS crossbar, no
+// LDS storage and no barrier)
+ADG_DEV double lane_get(double x, int addr)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)b);
+    const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
+// ---------------------------------------------------------------------------
 // LDS doubles of a cell are padded so that the cell's region starts NN doubles (mod 32) after
 // the previous cell's: a lane's node-indexed element (tt) then lies 8 bytes after the previous
 // lane's across cell boundaries, so any 32 consecutive lanes of a wave touch 32 distinct bank
@@ -458,7 +491,13 @@ struct LeanShape {
     static constexpr int LDS = S::CPW * CELL + 2 * S::NN;  // + Kxi, dudx
 };
 
-template <class P, int N, bool TEST, class M>
+// MF (N = 3, product build): the x / y contractions and gradients of each slice run on the
+// matrix cores (mfma4 above) instead of DPP broadcasts and LDS reads: lane 16 j + 4 b + i of
+// wave w holds node (i, j) of local cell 4 w + b, the fields contracted over i are transposed
+// across the lanes of the cell by ds_bpermute, and no LDS exchange or barrier remains in a
+// slice.  The sums over k run inside the matrix core (their rounding differs from the
+// ordered FMA chains by ulps).
+template <class P, int N, bool TEST, bool MF, class M>
 ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, double *smem, int first,
                             int cell0, int par, double dt)
 {
@@ -466,8 +505,11 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
     const double rdx = 1.0 / m.dx, rdy = 1.0 / m.dy;  // per solve; div_inv, gradients
     using L = LeanShape<P, N>;
     constexpr int n = S::n, NN = S::NN, CPW = S::CPW, V = S::V;
+    static_assert(!MF || (n == 4 && kPredBlock<P> % 64 == 0), "matrix-core contractions need 4 x 4 nodes per cell");
     const OpPtr op = (OpPtr)b.op;
-    const int tid = threadIdx.x, lc = tid / NN, tt = tid - lc * NN;
+    const int tid = threadIdx.x;
+    const int lc = MF ? 4 * (tid >> 6) + ((tid >> 2) & 3) : tid / NN;
+    const int tt = MF ? (tid & 3) + 4 * ((tid >> 4) & 3) : tid - lc * NN;
     const int i = tt % n, j = tt / n;
     const int cell = first + lc;  // local cell of the solve
     const bool own = lc < CPW;
@@ -508,7 +550,16 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
     // values in the same order, so the same bits.  Every lane of the workgroup owns a node.
     // (Baer-Nunziato N=3 predictor 2.73 -> 2.58 ms, Euler N=3 1.88 -> 1.75 ms on 4 x 384^2 /
     // 4 x 512^2 meshes.)
-    constexpr bool DPPX = n == 4 && CPW * NN == kPredBlock<P>;
+    constexpr bool DPPX = !MF && n == 4 && CPW * NN == kPredBlock<P>;
+    // MF: this lane's entries of the operands: kxi(j, i) serves as B of the x contraction and
+    // as A of the y contraction, dudx(i, j) as B of the x gradient and A of the y gradient;
+    // xaddr is the ds_bpermute address of the lane holding node (j, i) of the same cell
+    double mk = 0.0, md = 0.0;
+    const int xaddr = (((tid & 3) << 4) | (tid & 12) | ((tid >> 4) & 3)) << 2;
+    if constexpr (MF) {
+        mk = sK[j + n * i];
+        md = sD[i + n * j];
+    }
     double kx[4], gxr[4];
     if constexpr (DPPX) {
 #pragma unroll
@@ -520,7 +571,7 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
     // OPR (n <= 3): the operator rows of this lane's contractions held in registers for the
     // whole kernel (4 n doubles) instead of re-read from LDS at every slice: halves the LDS
     // reads of the contractions (Baer-Nunziato N=2 predictor -2%, with the pad above -7.5%)
-    constexpr bool OPR = !DPPX && n <= 3;
+    constexpr bool OPR = !MF && !DPPX && n <= 3;
     double okx[n], oky[n], ogx[n], ogy[n];
     if constexpr (OPR) {
 #pragma unroll
@@ -547,6 +598,18 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
     // slice's x / y flux contractions ax, ay (and, Baer-Nunziato, the gradients G)
     auto slice_terms = [&](const double *Q, double *F, double *ax, double *ay, double *G) {
         P::flux(m, Q, F);
+        if constexpr (MF) {
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                ax[v] = mfma4(lane_get(F[v], xaddr), mk);
+                ay[v] = mfma4(mk, F[V + v]);
+                if (P::kNcp) {  // gradients, Baer-Nunziato/space_time_predictor.f90:72,79
+                    G[v] = rdx * mfma4(lane_get(Q[v], xaddr), md);
+                    G[V + v] = rdy * mfma4(md, Q[v]);
+                }
+            }
+            return;
+        }
         if (own) {
 #pragma unroll
             for (int v = 0; v < V; v++) {
@@ -626,6 +689,42 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
                     q[v][l] = r;
                 }
             }
+        } else if constexpr (MF) {
+#pragma unroll 1
+        for (int l = 0; l < n; l++) {
+            double Q[V], F[2 * V], ax[V], ay[V], G[2 * V], rv[V];
+#pragma unroll
+            for (int v = 0; v < V; v++) Q[v] = q[v][0];
+            slice_terms(Q, F, ax, ay, G);
+            const double wl = op->w[l];
+            if (LAST) {  // time average of the last sweep's fluxes (:92)
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    fh[0][v] = fh[0][v] + F[v] * wl;
+                    fh[1][v] = fh[1][v] + F[V + v] * wl;
+                }
+            }
+            const double sx = div_inv(wl * wj * dt, m.dx, rdx), sy = div_inv(wl * wi * dt, m.dy, rdy);
+            const double c0 = wi * wj * op->f0[l];
+#pragma unroll
+            for (int v = 0; v < V; v++) rv[v] = (c0 * sU[v * NN + tt] - sx * ax[v]) - sy * ay[v];  // :66, :72
+            if (P::kNcp) {  // NCP source, Baer-Nunziato/space_time_predictor.f90:84-92
+                double src[V];
+                P::ncp(m, Q, G, src);
+                const double f = wi * wj * wl * dt;
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    rv[v] = rv[v] + f * src[v];
+                    if (LAST) sh[v] = sh[v] + src[v] * wl;  // lShi (:111)
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+#pragma unroll
+                for (int ll = 0; ll + 1 < n; ll++) q[v][ll] = q[v][ll + 1];
+                q[v][n - 1] = rv[v];
+            }
+        }
         } else {
 #pragma unroll 1
         for (int l = 0; l < n; l++) {
@@ -1020,7 +1119,7 @@ __global__ __launch_bounds__(kPredBlock<P>) __attribute__((amdgpu_waves_per_eu(2
         if (!st.active) continue;
         // the solve's constants stay in the constant address space, read where used
         const ADG_CONST SolveArgs *sa = (const ADG_CONST SolveArgs *)(b.solves + w.x);
-        predictor_lean<P, N, false>(sa->m, b, TestBufs{}, smem, w.y, sa->cell0, st.parity, st.dt);
+        predictor_lean<P, N, false, false>(sa->m, b, TestBufs{}, smem, w.y, sa->cell0, st.parity, st.dt);
         __syncthreads();
     }
 }
@@ -1029,7 +1128,34 @@ template <class P, int N>
 __global__ __launch_bounds__(kPredBlock<P>) void k_predictor_lean_test(MeshArgs m, StepBufs b, TestBufs t, double dt)
 {
     __shared__ double smem[LeanShape<P, N>::LDS];
-    predictor_lean<P, N, true>(m, b, t, smem, blockIdx.x * Shape<P, N>::CPW, 0, 0, dt);
+    predictor_lean<P, N, true, false>(m, b, t, smem, blockIdx.x * Shape<P, N>::CPW, 0, 0, dt);
+}
+
+// the lean predictor with matrix-core contractions (N = 3, product build)
+template <class P, int N>
+__global__ __launch_bounds__(kPredBlock<P>) __attribute__((amdgpu_waves_per_eu(2))) void k_predictor_mfma(StepBufs b)
+{
+    __shared__ double smem[LeanShape<P, N>::LDS];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        b.counters[kCntList] = 0;
+        b.counters[kCntTroubled] = 0;
+    }
+    const int ntab = ld_const(b.meta + kTabPred);
+    for (int e = xcd_entry(blockIdx.x, gridDim.x); e < ntab; e += gridDim.x) {
+        const int2 w = ld_const(b.tab[kTabPred] + e);
+        const SolveState st = ld_const(b.state + w.x);
+        if (!st.active) continue;
+        const ADG_CONST SolveArgs *sa = (const ADG_CONST SolveArgs *)(b.solves + w.x);
+        predictor_lean<P, N, false, true>(sa->m, b, TestBufs{}, smem, w.y, sa->cell0, st.parity, st.dt);
+        __syncthreads();
+    }
+}
+
+template <class P, int N>
+__global__ __launch_bounds__(kPredBlock<P>) void k_predictor_mfma_test(MeshArgs m, StepBufs b, TestBufs t, double dt)
+{
+    __shared__ double smem[LeanShape<P, N>::LDS];
+    predictor_lean<P, N, true, true>(m, b, t, smem, blockIdx.x * Shape<P, N>::CPW, 0, 0, dt);
 }
 
 template <class P, int N, int BLK>
@@ -2812,6 +2938,12 @@ hipError_t launch_predictor(const StepBufs &b, hipStream_t st, int variant, int 
     case kPredLean:
         hipLaunchKernelGGL((k_predictor_lean<P, N>), dim3(grid), dim3(S::PRED_BLOCK), 0, st, b);
         break;
+    case kPredMfma:
+        if constexpr (kMfmaBuild && N == 3)
+            hipLaunchKernelGGL((k_predictor_mfma<P, N>), dim3(grid), dim3(S::PRED_BLOCK), 0, st, b);
+        else
+            return hipErrorInvalidValue;
+        break;
     default:
         return hipErrorInvalidValue;
     }
@@ -2909,6 +3041,14 @@ hipError_t launch_test_impl(const MeshArgs &m, const StepBufs &b, const TestBufs
         hipLaunchKernelGGL((k_predictor_lean_test<P, N>), dim3(g), dim3(S::PRED_BLOCK), 0, st, m, b, t, dt);
         break;
     }
+    case kPredMfma:
+        if constexpr (kMfmaBuild && N == 3) {
+            const int g = (m.ncell + S::CPW - 1) / S::CPW;
+            hipLaunchKernelGGL((k_predictor_mfma_test<P, N>), dim3(g), dim3(S::PRED_BLOCK), 0, st, m, b, t, dt);
+        } else {
+            return hipErrorInvalidValue;
+        }
+        break;
     default:
         return hipErrorInvalidValue;
     }
@@ -2932,7 +3072,7 @@ template <class P, int N>
 void quanta_impl(int variant, int q[kNumTabs])
 {
     constexpr int N3 = (N + 1) * (N + 1) * (N + 1);
-    q[kTabPred] = (variant == kPredLine || variant == kPredLean) ? Shape<P, N>::CPW
+    q[kTabPred] = (variant == kPredLine || variant == kPredLean || variant == kPredMfma) ? Shape<P, N>::CPW
                   : variant == kPredNode64 ? 64 / N3 : variant == kPredNode128 ? 128 / N3 : 256 / N3;
     q[kTabUpd] = Shape<P, N>::CPWU;
 }
@@ -2956,6 +3096,7 @@ bool pred_variant_valid(int pde, int N, int variant)
 {
     if (N < 1 || N > 4) return false;
     if (pde == kPdeAd) return variant == kPredLine;
+    if (variant == kPredMfma) return kMfmaBuild && N == 3;
     if (variant == kPredLine || variant == kPredNode128 || variant == kPredNode256 || variant == kPredLean) return true;
     return variant == kPredNode64 && (N + 1) * (N + 1) * (N + 1) <= 64;
 }
@@ -2990,6 +3131,9 @@ int resident_impl(int variant, int upd, int tab)
         case kPredNode128: return resident_of(k_predictor_nodes<P, N, 128>, 128);
         case kPredNode256: return resident_of(k_predictor_nodes<P, N, 256>, 256);
         case kPredLean: return resident_of(k_predictor_lean<P, N>, S::PRED_BLOCK);
+        case kPredMfma:
+            if constexpr (kMfmaBuild && N == 3) return resident_of(k_predictor_mfma<P, N>, S::PRED_BLOCK);
+            return 0;
         default: return 0;
         }
     case kTabUpd:

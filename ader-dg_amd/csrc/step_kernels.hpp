@@ -108,8 +108,13 @@ struct TestBufs {
 // whole time line; kPredLean: the same mapping with a smaller register footprint) or
 // space-time-node threads (one thread per node of the space-time cell) in workgroups
 // of 64, 128 or 256.  All give the same bits; the autotuner
-// (tune.hip) picks one per (PDE, N).  Advection-diffusion has the time-line form only.
-enum { kPredLine = 0, kPredNode64 = 1, kPredNode128 = 2, kPredNode256 = 3, kPredLean = 4, kNumPredVariants = 5 };
+// (tune.hip) picks one per (PDE, N).  kPredMfma (N = 3, product build only): the lean
+// mapping with the slice contractions on the matrix cores, within rounding of the others.
+// Advection-diffusion has the time-line form only.
+enum {
+    kPredLine = 0, kPredNode64 = 1, kPredNode128 = 2, kPredNode256 = 3, kPredLean = 4, kPredMfma = 5,
+    kNumPredVariants = 6
+};
 bool pred_variant_valid(int pde, int N, int variant);
 // Update kernels: kUpdRecompute (k_update: each thread forms the first passes of the
 // subcell and Lobatto maps it needs; LDS atomics reduce the ranges) or kUpdShared

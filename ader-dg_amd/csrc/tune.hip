@@ -30,7 +30,7 @@ constexpr int kTuneRounds = 3, kTuneReps = 3;
 
 struct TuneEntry {
     int pred = kPredLine, upd = kUpdShared;
-    double pms[kNumPredVariants] = {-1.0, -1.0, -1.0, -1.0, -1.0};
+    double pms[kNumPredVariants] = {-1.0, -1.0, -1.0, -1.0, -1.0, -1.0};
     double ums[kNumUpdVariants] = {-1.0, -1.0};
 };
 

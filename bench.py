@@ -234,9 +234,9 @@ def predictor_name(pde, N, persistent):
     (the library then skips the autotuner), else the autotuner's choice."""
     import aderdg
     names = {0: "k_predictor", 1: "k_predictor_nodes<.., 64>", 2: "k_predictor_nodes<.., 128>",
-             3: "k_predictor_nodes<.., 256>", 4: "k_predictor_lean"}
+             3: "k_predictor_nodes<.., 256>", 4: "k_predictor_lean", 5: "k_predictor_mfma"}
     pinned = os.environ.get("ADERDG_PREDICTOR_VARIANT", "")
-    if pinned.strip().isdigit() and aderdg.pred_variant_valid(pde, N, int(pinned)):
+    if pinned.strip().isdigit() and aderdg.pred_variant_valid(pde, N, int(pinned), exact=False):
         v = int(pinned)
     else:
         t = aderdg.tune_query(pde, N, persistent)
@@ -246,7 +246,7 @@ def predictor_name(pde, N, persistent):
 
 def first_sweep_skipped(pde, N):
     """Operations per cell-step of the reference algorithm that the time-line predictor
-    kernels (k_predictor, k_predictor_lean) do not issue: the first Picard sweep starts from
+    kernels (k_predictor, k_predictor_lean, k_predictor_mfma) do not issue: the first Picard sweep starts from
     the trivial initial guess (every time slice holds luh), so slices 1..N repeat slice 0's
     flux, flux contractions and (Baer-Nunziato) gradients and NCP source, which the kernels
     form once.  Same convention and constants as aderdg_flops_predictor."""
@@ -272,21 +272,30 @@ def roofline_of(flop_per_cell, cells, ms_per_launch, kernel, traffic=None, skipp
                  "peak_measured_note": "FP64 FMA rate of eight independent chains per thread on this GPU "
                                        "(aderdg_probe_peaks)"}
     # "mfma" is the compute roofline of the line's contract; its FP64 peak equals the vector ALU's
-    # on MI355X, and the kernel runs on the vector ALU (compute_unit)
-    return {"bound": "mfma", "compute_unit": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+    # on MI355X.  The matrix-core predictor runs the slice contractions on the matrix cores and the
+    # pointwise flux / NCP work on the vector ALU (compute_unit)
+    mfma = "mfma" in kernel
+    if mfma:
+        unit_note = ("FP64 vector ALU for the pointwise flux and NCP terms, FP64 matrix cores "
+                     "(v_mfma_f64_4x4x4_4b_f64) for the per-slice 4x4 contractions")
+    else:
+        unit_note = "FP64 vector ALU (no MFMA: the per-cell contractions are 4x4..5x5 and run on the VALU)"
+    return {"bound": "mfma", "compute_unit": "fp64-valu+mfma" if mfma else "fp64-valu", "achieved": achieved,
+            "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s", **extra,
             "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic, "traffic_unit": "bytes/launch",
             "kernel": kernel, "flop_per_cell": flop_per_cell, "cells_per_launch": cells,
             "executed_flop_per_cell": executed,
             "executed_achieved": executed * cells / (ms_per_launch * 1e-3) / 1e12,
-            "note": "FP64 vector ALU (no MFMA: the per-cell contractions are 4x4..5x5 and run on the VALU); "
+            "note": unit_note + "; "
                     "MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s. achieved and frac count the "
                     "reference algorithm's operations; the time-line predictors form the first Picard sweep's "
                     "flux terms once for all time slices, so they issue executed_flop_per_cell"}
 
 
 def skipped_for(kernel, pde, N):
-    return first_sweep_skipped(pde, N) if kernel.startswith(("k_predictor<", "k_predictor_lean<")) else 0.0
+    return first_sweep_skipped(pde, N) if kernel.startswith(("k_predictor<", "k_predictor_lean<",
+                                                             "k_predictor_mfma<")) else 0.0
 
 
 def algorithmic_bytes(pde, N):

@@ -257,7 +257,9 @@ int aderdg_timing_get(aderdg_ctx *ctx, double ms[ADERDG_TIMING_SLOTS], int64_t l
  * time line), 1/2/3 space-time-node threads in workgroups of 64/128/256 (1 is
  * absent for N=4, whose space-time cell has 125 nodes), 4 time-line threads
  * with a smaller register footprint (right-hand sides kept in the time line,
- * state and operators in LDS).  Advection-diffusion has variant 0 only.  The first context or
+ * state and operators in LDS), 5 (N=3, libaderdg_amd.so only) variant 4 with the
+ * per-slice contractions on the FP64 matrix cores, equal to the others within
+ * rounding.  Advection-diffusion has variant 0 only.  The first context or
  * batch arena of a (device, pde, N, launch mode) times every variant on a
  * synthetic 256x256 solve and keeps the fastest for the process.
  * aderdg_tune_force(pde, N, v) makes contexts and arenas created afterwards use
@@ -266,7 +268,7 @@ int aderdg_timing_get(aderdg_ctx *ctx, double ms[ADERDG_TIMING_SLOTS], int64_t l
  * aderdg_tune_query returns the tuner's choice and the milliseconds per
  * launch it measured (-1 for variants it did not run); persistent selects
  * batch arenas (persistent grids) rather than single contexts. */
-#define ADERDG_PREDICTOR_VARIANTS 5
+#define ADERDG_PREDICTOR_VARIANTS 6
 int aderdg_tune_force(int pde, int N, int variant);
 int aderdg_tune_query(int device, int pde, int N, int persistent, int *variant,
                       double ms[ADERDG_PREDICTOR_VARIANTS]);
