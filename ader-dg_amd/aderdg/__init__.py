@@ -45,7 +45,7 @@ def sides(left, right, bottom, top, inflow=None):
 PDE = {"euler": 0, "bn": 1, "ad": 2}
 NVAR = {0: 4, 1: 9, 2: 1}
 # kernels timed by aderdg_timing_get, in its slot order
-TIMING_SLOTS = ("dt", "predictor", "update", "spread", "subcell", "face")
+TIMING_SLOTS = ("dt", "predictor", "update", "spread", "subcell", "face", "ranges")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -539,12 +539,12 @@ def tune_query(pde, N, persistent, device=0, exact=False, kernel="predictor"):
 
 def pred_variant_valid(pde, N, variant, exact=True):
     """Whether predictor variant `variant` exists for (pde, N) in the exact (parity) build or
-    the product build: the matrix-core variant 5 is N=3 of the product build only."""
+    the product build: the matrix-core variant 5 is N<=3 of the product build only."""
     k = PDE[pde] if isinstance(pde, str) else int(pde)
     if k == 2:
         return variant == 0
     if variant == 5:
-        return N == 3 and not exact
+        return N <= 3 and not exact
     return variant in (0, 2, 3, 4) or (variant == 1 and (N + 1) ** 3 <= 64)
 
 

@@ -376,6 +376,7 @@ hipError_t launch_ad_impl(const StepBufs &b, hipStream_t st, int part, const int
         break;
     case kPartSpread:
     case kPartSubcell:
+    case kPartRanges:
         return hipSuccess;  // no limiter in the advection-diffusion solver
     default:
         return hipErrorInvalidValue;

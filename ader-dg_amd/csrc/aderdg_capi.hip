@@ -226,13 +226,14 @@ hipEvent_t get_event(aderdg_ctx *c)
 
 int timing_slot(int part)
 {
-    switch (part) {  // dt, predictor, update, spread, subcell, face
+    switch (part) {  // dt, predictor, update, spread, subcell, face, ranges
     case kPartDt: return 0;
     case kPartPredictor: return 1;
     case kPartUpdate: return 2;
     case kPartSpread: return 3;
     case kPartSubcell: return 4;
     case kPartFace: return 5;
+    case kPartRanges: return 6;
     default: return -1;
     }
 }
@@ -301,6 +302,7 @@ int seed(aderdg_ctx *c)
 int step_parts(aderdg_ctx *c)
 {
     int r;
+    if ((r = launch(c, kPartRanges))) return r;
     if ((r = launch(c, kPartPredictor))) return r;
     if ((r = launch(c, kPartFace))) return r;
     if ((r = launch(c, kPartUpdate))) return r;
@@ -573,7 +575,8 @@ int aderdg_space_time_predictor(aderdg_ctx *c)
 {
     if (!c) return fail(ADERDG_EINVAL, "aderdg_space_time_predictor: null context");
     if (!c->have_uh) return fail(ADERDG_ESTATE, "aderdg_space_time_predictor: no solution uploaded");
-    return launch(c, kPartPredictor);
+    const int r = launch(c, kPartRanges);
+    return r ? r : launch(c, kPartPredictor);
 }
 
 int aderdg_riemann_surface_update(aderdg_ctx *c, int32_t *nrec)

@@ -221,7 +221,8 @@ int Batch::build_graph()
     if (!fork) HIPCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
     HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
-    static const int parts[] = {kPartPredictor, kPartFace, kPartUpdate, kPartSpread, kPartSubcell, kPartAdvance};
+    static const int parts[] = {kPartRanges, kPartPredictor, kPartFace, kPartUpdate, kPartSpread, kPartSubcell,
+                                kPartAdvance};
     if (groups.size() > 1) e = hipEventRecord(fork, stream);
     for (size_t k = 1; k < groups.size() && e == hipSuccess; k++) e = hipStreamWaitEvent(branch[k - 1], fork, 0);
     for (size_t k = 0; k < groups.size(); k++) {
@@ -268,13 +269,14 @@ int Batch::step(int nsteps)
     if (nsteps < 0) return fail(ADERDG_EINVAL, "aderdg_batch_step: negative step count");
     if (groups.empty()) return fail(ADERDG_ESTATE, "aderdg_batch_step: nothing reserved");
     if (timing) {  // the graph's launches one by one, each between two events
-        // timing slots of include/aderdg.h: dt, predictor, update, spread, subcell, face; -1 untimed
-        static const int parts[] = {kPartDt, kPartPredictor, kPartFace, kPartUpdate, kPartSpread, kPartSubcell,
-                                    kPartAdvance};
-        static const int slots[] = {0, 1, 5, 2, 3, 4, -1};
+        // timing slots of include/aderdg.h: dt, predictor, update, spread, subcell, face, ranges; -1 untimed
+        static const int parts[] = {kPartDt, kPartRanges, kPartPredictor, kPartFace, kPartUpdate, kPartSpread,
+                                    kPartSubcell, kPartAdvance};
+        static const int slots[] = {0, 6, 1, 5, 2, 3, 4, -1};
+        constexpr int nparts = sizeof parts / sizeof parts[0];
         for (int k = 0; k < nsteps; k++)
             for (size_t gi = 0; gi < groups.size(); gi++)
-                for (int p = 0; p < 7; p++) {
+                for (int p = 0; p < nparts; p++) {
                     hipEvent_t e0 = nullptr, e1 = nullptr;
                     for (hipEvent_t *e : {&e0, &e1}) {
                         if (!free_events.empty()) {

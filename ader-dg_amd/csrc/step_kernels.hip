@@ -3,8 +3,9 @@
 //
 // Per time step (the order of Euler/main.f90:24-100, Baer-Nunziato/main.f90:24-100):
 //   k_dt          calc_time_step: per-solve min-reduction of per-cell candidates
+//   k_ranges      the 3x3 pass of get_min_max (the DMP bounds of the step)
 //   k_predictor   ader_space_time_predictor + volume_integral + face
-//                 extrapolation + the 3x3 pass of get_min_max
+//                 extrapolation
 //   k_face        solve_riemann_problem: one thread per face point
 //   k_update      surface_integral + element_update + DMP, and the next
 //                 step's per-cell get_min_max pass and time-step candidates
@@ -85,7 +86,7 @@ ADG_DEV double dmr_shock_x(const M &m, double time) { return m.dmr_a + m.dmr_b *
 // contraction with iK1 is thread-local.  Baer-Nunziato also exchanges q for
 // the gradients of the non-conservative product.
 // ---------------------------------------------------------------------------
-// Volume integral, boundary extrapolation and the 3x3 pass of get_min_max of one
+// Volume integral and boundary extrapolation of one
 // cell from its time averages in LDS (qhiL, fhxL, fhyL, shL: [v][j][i]; shL is
 // read for Baer-Nunziato only), spread over the cell's nthr threads.  TEST:
 // write the averages and boundary values in the reference layouts instead.
@@ -157,24 +158,6 @@ ADG_DEV void predictor_finish(const M &m, const StepBufs &b, const TestBufs &t, 
                 }
             }
         }
-    }
-    if (TEST) return;
-
-    // get_min_max step 2: 3x3 neighbourhood, Euler/limiter.f90:147-167
-    for (int v = tt; act && v < V; v += nthr) {
-        const int ci = cell % m.imax, cj = cell / m.imax;
-        const size_t row = (size_t)v * b.G + cell0;
-        double mn = b.lmin_old[row + cell], mx = b.lmax_old[row + cell];
-#pragma unroll
-        for (int jj = -1; jj <= 1; jj++)
-#pragma unroll
-            for (int ii = -1; ii <= 1; ii++) {
-                const int nb = nb_index(ci, ii, m.imax, m.wrap & 1) + m.imax * nb_index(cj, jj, m.jmax, m.wrap >> 1);
-                mn = f_min(mn, b.lmin_old[row + nb]);
-                mx = f_max(mx, b.lmax_old[row + nb]);
-            }
-        b.lmin[row + cell] = mn;
-        b.lmax[row + cell] = mx;
     }
 }
 
@@ -449,7 +432,7 @@ ADG_DEV double quad_dot4(double x, const double *c)
 }
 
 // ---------------------------------------------------------------------------
-// Matrix-core contractions (product build, N = 3).  v_mfma_f64_4x4x4_4b_f64 multiplies four
+// Matrix-core contractions (product build).  v_mfma_f64_4x4x4_4b_f64 multiplies four
 // independent 4x4 blocks per wave; the block of lane L is (L >> 2) & 3, and within it
 //   A[m][k] sits in lane 16 k + 4 blk + m,  B[k][n] in lane 16 k + 4 blk + n,
 //   D[m][n] in lane 16 m + 4 blk + n
@@ -491,13 +474,7 @@ struct LeanShape {
     static constexpr int LDS = S::CPW * CELL + 2 * S::NN;  // + Kxi, dudx
 };
 
-// MF (N = 3, product build): the x / y contractions and gradients of each slice run on the
-// matrix cores (mfma4 above) instead of DPP broadcasts and LDS reads: lane 16 j + 4 b + i of
-// wave w holds node (i, j) of local cell 4 w + b, the fields contracted over i are transposed
-// across the lanes of the cell by ds_bpermute, and no LDS exchange or barrier remains in a
-// slice.  The sums over k run inside the matrix core (their rounding differs from the
-// ordered FMA chains by ulps).
-template <class P, int N, bool TEST, bool MF, class M>
+template <class P, int N, bool TEST, class M>
 ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, double *smem, int first,
                             int cell0, int par, double dt)
 {
@@ -505,11 +482,8 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
     const double rdx = 1.0 / m.dx, rdy = 1.0 / m.dy;  // per solve; div_inv, gradients
     using L = LeanShape<P, N>;
     constexpr int n = S::n, NN = S::NN, CPW = S::CPW, V = S::V;
-    static_assert(!MF || (n == 4 && kPredBlock<P> % 64 == 0), "matrix-core contractions need 4 x 4 nodes per cell");
     const OpPtr op = (OpPtr)b.op;
-    const int tid = threadIdx.x;
-    const int lc = MF ? 4 * (tid >> 6) + ((tid >> 2) & 3) : tid / NN;
-    const int tt = MF ? (tid & 3) + 4 * ((tid >> 4) & 3) : tid - lc * NN;
+    const int tid = threadIdx.x, lc = tid / NN, tt = tid - lc * NN;
     const int i = tt % n, j = tt / n;
     const int cell = first + lc;  // local cell of the solve
     const bool own = lc < CPW;
@@ -550,16 +524,7 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
     // values in the same order, so the same bits.  Every lane of the workgroup owns a node.
     // (Baer-Nunziato N=3 predictor 2.73 -> 2.58 ms, Euler N=3 1.88 -> 1.75 ms on 4 x 384^2 /
     // 4 x 512^2 meshes.)
-    constexpr bool DPPX = !MF && n == 4 && CPW * NN == kPredBlock<P>;
-    // MF: this lane's entries of the operands: kxi(j, i) serves as B of the x contraction and
-    // as A of the y contraction, dudx(i, j) as B of the x gradient and A of the y gradient;
-    // xaddr is the ds_bpermute address of the lane holding node (j, i) of the same cell
-    double mk = 0.0, md = 0.0;
-    const int xaddr = (((tid & 3) << 4) | (tid & 12) | ((tid >> 4) & 3)) << 2;
-    if constexpr (MF) {
-        mk = sK[j + n * i];
-        md = sD[i + n * j];
-    }
+    constexpr bool DPPX = n == 4 && CPW * NN == kPredBlock<P>;
     double kx[4], gxr[4];
     if constexpr (DPPX) {
 #pragma unroll
@@ -571,7 +536,7 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
     // OPR (n <= 3): the operator rows of this lane's contractions held in registers for the
     // whole kernel (4 n doubles) instead of re-read from LDS at every slice: halves the LDS
     // reads of the contractions (Baer-Nunziato N=2 predictor -2%, with the pad above -7.5%)
-    constexpr bool OPR = !MF && !DPPX && n <= 3;
+    constexpr bool OPR = !DPPX && n <= 3;
     double okx[n], oky[n], ogx[n], ogy[n];
     if constexpr (OPR) {
 #pragma unroll
@@ -598,18 +563,6 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
     // slice's x / y flux contractions ax, ay (and, Baer-Nunziato, the gradients G)
     auto slice_terms = [&](const double *Q, double *F, double *ax, double *ay, double *G) {
         P::flux(m, Q, F);
-        if constexpr (MF) {
-#pragma unroll
-            for (int v = 0; v < V; v++) {
-                ax[v] = mfma4(lane_get(F[v], xaddr), mk);
-                ay[v] = mfma4(mk, F[V + v]);
-                if (P::kNcp) {  // gradients, Baer-Nunziato/space_time_predictor.f90:72,79
-                    G[v] = rdx * mfma4(lane_get(Q[v], xaddr), md);
-                    G[V + v] = rdy * mfma4(md, Q[v]);
-                }
-            }
-            return;
-        }
         if (own) {
 #pragma unroll
             for (int v = 0; v < V; v++) {
@@ -689,42 +642,6 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
                     q[v][l] = r;
                 }
             }
-        } else if constexpr (MF) {
-#pragma unroll 1
-        for (int l = 0; l < n; l++) {
-            double Q[V], F[2 * V], ax[V], ay[V], G[2 * V], rv[V];
-#pragma unroll
-            for (int v = 0; v < V; v++) Q[v] = q[v][0];
-            slice_terms(Q, F, ax, ay, G);
-            const double wl = op->w[l];
-            if (LAST) {  // time average of the last sweep's fluxes (:92)
-#pragma unroll
-                for (int v = 0; v < V; v++) {
-                    fh[0][v] = fh[0][v] + F[v] * wl;
-                    fh[1][v] = fh[1][v] + F[V + v] * wl;
-                }
-            }
-            const double sx = div_inv(wl * wj * dt, m.dx, rdx), sy = div_inv(wl * wi * dt, m.dy, rdy);
-            const double c0 = wi * wj * op->f0[l];
-#pragma unroll
-            for (int v = 0; v < V; v++) rv[v] = (c0 * sU[v * NN + tt] - sx * ax[v]) - sy * ay[v];  // :66, :72
-            if (P::kNcp) {  // NCP source, Baer-Nunziato/space_time_predictor.f90:84-92
-                double src[V];
-                P::ncp(m, Q, G, src);
-                const double f = wi * wj * wl * dt;
-#pragma unroll
-                for (int v = 0; v < V; v++) {
-                    rv[v] = rv[v] + f * src[v];
-                    if (LAST) sh[v] = sh[v] + src[v] * wl;  // lShi (:111)
-                }
-            }
-#pragma unroll
-            for (int v = 0; v < V; v++) {
-#pragma unroll
-                for (int ll = 0; ll + 1 < n; ll++) q[v][ll] = q[v][ll + 1];
-                q[v][n - 1] = rv[v];
-            }
-        }
         } else {
 #pragma unroll 1
         for (int l = 0; l < n; l++) {
@@ -850,6 +767,317 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
     }
     __syncthreads();
     predictor_finish<P, N, TEST>(m, b, t, qhiL, fhxL, fhyL, shL, tt, NN, act, cell, cell0);
+}
+
+// ---------------------------------------------------------------------------
+// Matrix-core predictor (product build, N = 2 and 3).  Thread mapping: lane 16 j + 4 b + i of
+// wave w holds the time line of node (i, j) of local cell 4 w + b, so each cell is one block of
+// v_mfma_f64_4x4x4_4b_f64 (mfma4 above).  At N = 2 the 3 x 3 nodes sit in a 4 x 4 block whose
+// padding lanes carry an admissible dummy state that stays put (no flux terms) and meet only
+// zero operator entries.  Per time slice the x / y flux contractions and the Baer-Nunziato
+// gradients are matrix products (fields contracted over i are transposed across the cell's
+// lanes by ds_bpermute); after the last sweep the volume integral and the boundary
+// extrapolation are matrix products too, and every lane writes its own lduh node and its own
+// (face, point) of fq / ff: nothing is exchanged through LDS and no barrier is needed.  LDS
+// only holds the initial state: in the persistent kernel each wave's four cells, copied from
+// uh by the asynchronous global-to-LDS loads of gfx950 while the wave works on its previous
+// entry (two buffers, entries alternate), so no entry waits on HBM for its input.  The
+// matrix core sums each 4-term contraction in its own order (measured: the order of an FMA
+// chain over k), within the product build's parity.
+// ---------------------------------------------------------------------------
+// s_waitcnt immediate that waits for vmcnt(0) only (expcnt and lgkmcnt at their maxima)
+constexpr int kWaitVmcnt0 = 0x0F70;
+// the admissible state idle and padding lanes carry (the dummy of every predictor variant)
+template <class P>
+ADG_DEV void mf_dummy(double *u)
+{
+#pragma unroll
+    for (int v = 0; v < P::V; v++) u[v] = 0.0;
+    if (P::V == 4) {
+        u[0] = 1.0;
+        u[3] = 2.5;
+    } else {
+        u[0] = 0.5; u[3] = 2.5; u[4] = 0.5; u[7] = 2.5; u[8] = 0.5;
+    }
+}
+
+template <class P, int N>
+struct MfShape {
+    static constexpr int n = N + 1, NN = n * n, V = P::V;
+    static constexpr int CPW = kPredBlock<P> / 16;  // cells per workgroup: four per wave
+    static constexpr int LDS = CPW * V * 16;        // test kernel: initial state per cell, [v][16 lanes]
+    static constexpr int CELLS = 4 * V * NN;        // doubles of a wave's four cells, uh layout
+    // bytes per lane of one global-to-LDS load: 16 where every cell starts 16-byte aligned
+    static constexpr int CHUNK = (V * NN) % 2 == 0 ? 16 : 4;
+    static constexpr int LOADS = (CELLS * 8 + 64 * CHUNK - 1) / (64 * CHUNK);  // per wave and entry
+    static_assert(n <= 4, "a cell's nodes must fit one 4 x 4 block");
+};
+
+// Issue the asynchronous copy of the first `ncells` (<= 4) cells of a wave's entry from uh
+// (src, the entry's first cell) into dst (LDS, the uh layout); completes with vmcnt.
+template <class P, int N>
+ADG_DEV void mf_prefetch(const double *src, double *dst, int ncells)
+{
+    using S = MfShape<P, N>;
+    const int lane = threadIdx.x & 63;
+    const int bytes = ncells * S::V * S::NN * 8;
+    const char *s = (const char *)src;
+    char *d = (char *)dst;
+#pragma unroll
+    for (int r = 0; r < S::LOADS; r++) {
+        const int off = (r * 64 + lane) * S::CHUNK;
+        if (off < bytes)
+            __builtin_amdgcn_global_load_lds((const void *)(s + off),
+                                             (__attribute__((address_space(3))) void *)(d + r * 64 * S::CHUNK),
+                                             S::CHUNK, 0, 0);
+    }
+}
+
+// D = A B + C of the four blocks (mfma4 with an accumulator)
+ADG_DEV double mfma4c(double a, double b, double c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
+#else
+    return a * b + c;  // the host pass only type-checks device code
+#endif
+}
+
+struct NoHook {
+    ADG_DEV void operator()() const {}
+};
+
+// Operator entries the matrix-core predictor indexes per lane, staged in LDS by the persistent
+// kernel (offsets in doubles): kxi, dudx (n x n), w, flc, frc (n), then the dummy state.
+enum { kSopKxi = 0, kSopDudx = 16, kSopW = 32, kSopFlc = 36, kSopFrc = 40, kSopPad = 44 };
+template <class P>
+constexpr int kSopSize = kSopPad + P::V;
+
+// PF: the initial state comes from ubuf, the wave's four cells in LDS (uh layout) whose
+// asynchronous copy was issued earlier, and the per-lane operator entries from sop (LDS, the
+// layout above); after() runs once the initial state is read (the caller's next prefetch).
+// Otherwise (test kernel) both come from global memory, the state staged in smem.
+template <class P, int N, bool TEST, bool PF, class M, class After = NoHook>
+ADG_DEV void predictor_mf(const M &m, const StepBufs &b, const TestBufs &t, double *smem, int first, int cell0,
+                          int par, double dt, const double *ubuf = nullptr, const double *sop = nullptr,
+                          After after = {})
+{
+    using S = MfShape<P, N>;
+    constexpr int n = S::n, NN = S::NN, V = S::V;
+    const double rdx = 1.0 / m.dx, rdy = 1.0 / m.dy;  // per solve; div_inv, gradients
+    const OpPtr op = (OpPtr)b.op;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int i = lane & 3, j = lane >> 4, lc = 4 * (tid >> 6) + ((lane >> 2) & 3);
+    const bool real = i < n && j < n;  // a node of the cell, not padding
+    const int tt = real ? i + n * j : 0;  // node index in the reference order
+    const int cell = first + lc;  // local cell of the solve
+    const bool act = real && cell < m.ncell;
+    const bool cellok = cell < m.ncell;
+    // ds_bpermute address of the lane holding node (j, i) of the same cell
+    const int xaddr = ((i << 4) | (lane & 12) | j) << 2;
+    auto kxi = [&](int k) { if constexpr (PF) return sop[kSopKxi + k]; else return op->kxi[k]; };
+    auto dudx = [&](int k) { if constexpr (PF) return sop[kSopDudx + k]; else return op->dudx[k]; };
+    auto wq = [&](int k) { if constexpr (PF) return sop[kSopW + k]; else return op->w[k]; };
+    // operand entries of this lane (zero outside the cell's n x n block):
+    //   mk  = kxi(j, i): B of the x flux contraction, A of the y flux contraction
+    //   md  = dudx(i, j): B of the x gradient, A of the y gradient
+    // (the volume integral's kxi(i, j) and the boundary coefficients are read at the end)
+    double mk = 0.0, md = 0.0;
+    if (real) {
+        mk = kxi(j + n * i);
+        md = dudx(i + n * j);
+    }
+    // padding lanes: unit weights and no flux terms, so their time line stays the dummy state
+    const double wi = real ? wq(i) : 1.0, wj = real ? wq(j) : 1.0;
+    const double f1 = 1.0 / (wi * wj);  // :81
+    const double tx = real ? dt : 0.0;  // the time step in the flux and source terms
+    // this lane's initial state, element v at sU[SS * v]
+    constexpr int SS = PF ? 1 : 16;
+    const double *sU;
+    double q[V][n];
+    if constexpr (PF) {
+        // the prefetched cells; idle and padding lanes read the dummy state
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        sU = act ? ubuf + (((lane >> 2) & 3) * NN + tt) * V : sop + kSopPad;
+#pragma unroll
+        for (int v = 0; v < V; v++)
+#pragma unroll
+            for (int l = 0; l < n; l++) q[v][l] = sU[v];
+    } else {
+        double *sw = smem + lc * V * 16 + (lane & 3) + 4 * (lane >> 4);  // [v][16] of this cell
+        const double *uh = TEST ? t.uh : b.uh[par];
+        const size_t gu = TEST ? (size_t)cell : (size_t)cell0 + cell;
+        double u[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) u[v] = act ? uh[gu * V * NN + V * tt + v] : 0.0;
+        if (!act) mf_dummy<P>(u);  // idle and padding lanes
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            sw[16 * v] = u[v];
+#pragma unroll
+            for (int l = 0; l < n; l++) q[v][l] = u[v];
+        }
+        sU = sw;
+    }
+    after();
+    double fh[2][V], sh[V];
+
+    // fluxes of state Q at this lane's node, the slice's flux contractions ax, ay and
+    // (Baer-Nunziato) the gradients G, Euler/space_time_predictor.f90:60-72,
+    // Baer-Nunziato/space_time_predictor.f90:66-80
+    auto slice_terms = [&](const double *Q, double *F, double *ax, double *ay, double *G) {
+        P::flux(m, Q, F);
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            ax[v] = mfma4(lane_get(F[v], xaddr), mk);
+            ay[v] = mfma4(mk, F[V + v]);
+            if (P::kNcp) {
+                G[v] = rdx * mfma4(lane_get(Q[v], xaddr), md);
+                G[V + v] = rdy * mfma4(md, Q[v]);
+            }
+        }
+    };
+    // one Picard sweep, Euler/space_time_predictor.f90:49-84 (Baer-Nunziato/space_time_predictor.f90:54-104);
+    // FIRST: from the trivial initial guess every slice holds luh (:27-35), so the slices share
+    // their flux terms, formed once, and differ only by their weights
+    auto sweep = [&](auto last_tag, auto first_tag) {
+        constexpr bool LAST = decltype(last_tag)::value;
+        constexpr bool FIRST = decltype(first_tag)::value;
+        if (LAST) {
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                fh[0][v] = 0.0;
+                fh[1][v] = 0.0;
+                sh[v] = 0.0;
+            }
+        }
+        if constexpr (FIRST) {
+            double Q[V], F[2 * V], G[2 * V], ax[V], ay[V], src[V];
+#pragma unroll
+            for (int v = 0; v < V; v++) Q[v] = q[v][0];
+            slice_terms(Q, F, ax, ay, G);
+            if (P::kNcp) P::ncp(m, Q, G, src);  // :84-92
+#pragma unroll
+            for (int l = 0; l < n; l++) {
+                const double wl = op->w[l];
+                const double sx = div_inv(wl * wj * tx, m.dx, rdx), sy = div_inv(wl * wi * tx, m.dy, rdy);
+                const double c0 = wi * wj * op->f0[l];
+                const double f = wi * wj * wl * tx;
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    double r = (c0 * sU[SS * v] - sx * ax[v]) - sy * ay[v];  // :66, :72
+                    if (P::kNcp) r = r + f * src[v];
+                    q[v][l] = r;
+                }
+            }
+        } else {
+#pragma unroll 1
+            for (int l = 0; l < n; l++) {
+                double Q[V], F[2 * V], ax[V], ay[V], G[2 * V], rv[V];
+#pragma unroll
+                for (int v = 0; v < V; v++) Q[v] = q[v][0];
+                slice_terms(Q, F, ax, ay, G);
+                const double wl = op->w[l];
+                if (LAST) {  // time average of the last sweep's fluxes (:92)
+#pragma unroll
+                    for (int v = 0; v < V; v++) {
+                        fh[0][v] = fh[0][v] + F[v] * wl;
+                        fh[1][v] = fh[1][v] + F[V + v] * wl;
+                    }
+                }
+                const double sx = div_inv(wl * wj * tx, m.dx, rdx), sy = div_inv(wl * wi * tx, m.dy, rdy);
+                const double c0 = wi * wj * op->f0[l];
+#pragma unroll
+                for (int v = 0; v < V; v++) rv[v] = (c0 * sU[SS * v] - sx * ax[v]) - sy * ay[v];  // :66, :72
+                if (P::kNcp) {  // NCP source, :84-92
+                    double src[V];
+                    P::ncp(m, Q, G, src);
+                    const double f = wi * wj * wl * tx;
+#pragma unroll
+                    for (int v = 0; v < V; v++) {
+                        rv[v] = rv[v] + f * src[v];
+                        if (LAST) sh[v] = sh[v] + src[v] * wl;  // lShi (:111)
+                    }
+                }
+                // slice l is done: its right-hand side takes its place, the time line rotates
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+#pragma unroll
+                    for (int ll = 0; ll + 1 < n; ll++) q[v][ll] = q[v][ll + 1];
+                    q[v][n - 1] = rv[v];
+                }
+            }
+        }
+        // time contraction, :79-83: q_new(v,ll) = f1 * sum over m of rhs(v,m) iK1(ll,m)
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            double qn[n];
+#pragma unroll
+            for (int ll = 0; ll < n; ll++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int mm = 0; mm < n; mm++) acc = acc + q[v][mm] * op->ik1[ll + n * mm];
+                qn[ll] = acc;
+            }
+#pragma unroll
+            for (int ll = 0; ll < n; ll++) q[v][ll] = f1 * qn[ll];
+        }
+    };
+    sweep(std::false_type{}, std::true_type{});
+#pragma unroll 1
+    for (int it = 1; it < N; it++) sweep(std::false_type{}, std::false_type{});
+    sweep(std::true_type{}, std::false_type{});
+
+    // time averages (:90), volume integral (Euler/volume_integral.f90:17-28, Baer-Nunziato/
+    // volume_integral.f90:36-40) and boundary extrapolation (Euler/space_time_predictor.f90:100-119):
+    // lane (i, j) keeps node (i, j) of lduh and point j of face i (left, right, bottom, top)
+    const size_t g = (size_t)cell0 + cell;
+    const int fc = i, pt = j;
+    const bool facelane = cellok && pt < n;
+    //   mkt = kxi(i, j): B of the volume integral's x part, A of its y part
+    //   mlr / mbt: flc / frc (j) in columns i = 0, 1 / i = 2, 3 (B of the boundary extrapolation)
+    double mkt = 0.0, mlr = 0.0, mbt = 0.0;
+    if (real) mkt = kxi(i + n * j);
+    if (j < n) {
+        double cf;
+        if constexpr (PF)
+            cf = sop[((i & 1) ? kSopFrc : kSopFlc) + j];
+        else
+            cf = (i & 1) ? op->frc[j] : op->flc[j];
+        mlr = i < 2 ? cf : 0.0;
+        mbt = i < 2 ? 0.0 : cf;
+    }
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        double qh = 0.0;
+#pragma unroll
+        for (int l = 0; l < n; l++) qh = qh + q[v][l] * op->w[l];
+        const double fxt = lane_get(fh[0][v], xaddr), qht = lane_get(qh, xaddr);
+        const double qb = mfma4c(qh, mbt, mfma4(qht, mlr));
+        const double fb = mfma4c(fh[1][v], mbt, mfma4(fxt, mlr));
+        if (TEST) {
+            if (act) {
+                t.qhi[(size_t)cell * V * NN + v + V * tt] = qh;
+                t.fhi[(size_t)cell * 2 * V * NN + v + V * (0 + 2 * tt)] = fh[0][v];
+                t.fhi[(size_t)cell * 2 * V * NN + v + V * (1 + 2 * tt)] = fh[1][v];
+                if (P::kNcp) t.shi[(size_t)cell * V * NN + v + V * tt] = sh[v];
+            }
+            if (facelane) {
+                t.qbnd[(size_t)cell * V * 4 * n + v + V * (fc + 4 * pt)] = qb;
+                t.fbnd[(size_t)cell * V * 4 * n + v + V * (fc + 4 * pt)] = fb;
+            }
+        } else {
+            double d = 0.0 + mfma4(fxt, mkt) * wj / m.dx;
+            d = d + mfma4(mkt, fh[1][v]) * wi / m.dy;
+            if (P::kNcp) d = d + wi * wj * sh[v];
+            if (act) b.lduh[g * V * NN + V * tt + v] = d;
+            if (facelane) {
+                const size_t o = ((size_t)(fc * V + v) * n + pt) * b.G + g;
+                b.fq[o] = qb;
+                b.ff[o] = fb;
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1119,7 +1347,7 @@ __global__ __launch_bounds__(kPredBlock<P>) __attribute__((amdgpu_waves_per_eu(2
         if (!st.active) continue;
         // the solve's constants stay in the constant address space, read where used
         const ADG_CONST SolveArgs *sa = (const ADG_CONST SolveArgs *)(b.solves + w.x);
-        predictor_lean<P, N, false, false>(sa->m, b, TestBufs{}, smem, w.y, sa->cell0, st.parity, st.dt);
+        predictor_lean<P, N, false>(sa->m, b, TestBufs{}, smem, w.y, sa->cell0, st.parity, st.dt);
         __syncthreads();
     }
 }
@@ -1128,34 +1356,93 @@ template <class P, int N>
 __global__ __launch_bounds__(kPredBlock<P>) void k_predictor_lean_test(MeshArgs m, StepBufs b, TestBufs t, double dt)
 {
     __shared__ double smem[LeanShape<P, N>::LDS];
-    predictor_lean<P, N, true, false>(m, b, t, smem, blockIdx.x * Shape<P, N>::CPW, 0, 0, dt);
+    predictor_lean<P, N, true>(m, b, t, smem, blockIdx.x * Shape<P, N>::CPW, 0, 0, dt);
 }
 
-// the lean predictor with matrix-core contractions (N = 3, product build)
+// the matrix-core predictor (N <= 3, product build)
 template <class P, int N>
 __global__ __launch_bounds__(kPredBlock<P>) __attribute__((amdgpu_waves_per_eu(2))) void k_predictor_mfma(StepBufs b)
 {
-    __shared__ double smem[LeanShape<P, N>::LDS];
+    using S = MfShape<P, N>;
+    constexpr int WAVES = kPredBlock<P> / 64;
+    // two buffers of each wave's four cells, alternating between the entries of the loop (a wave
+    // only touches its own: no barrier in the loop), then the operator entries and dummy state
+    __shared__ double lds[2 * WAVES * S::CELLS + kSopSize<P>];
+    double *sop = lds + 2 * WAVES * S::CELLS;
+    {
+        constexpr int n = S::n;
+        const OpPtr op = (OpPtr)b.op;
+        const int k = threadIdx.x;
+        if (k < n * n) {
+            sop[kSopKxi + k] = op->kxi[k];
+            sop[kSopDudx + k] = op->dudx[k];
+        }
+        if (k < n) {
+            sop[kSopW + k] = op->w[k];
+            sop[kSopFlc + k] = op->flc[k];
+            sop[kSopFrc + k] = op->frc[k];
+        }
+        if (k == 0) {
+            double u[P::V];
+            mf_dummy<P>(u);
+#pragma unroll
+            for (int v = 0; v < P::V; v++) sop[kSopPad + v] = u[v];
+        }
+    }
+    __syncthreads();
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         b.counters[kCntList] = 0;
         b.counters[kCntTroubled] = 0;
     }
+    const int wv = threadIdx.x >> 6;
     const int ntab = ld_const(b.meta + kTabPred);
-    for (int e = xcd_entry(blockIdx.x, gridDim.x); e < ntab; e += gridDim.x) {
-        const int2 w = ld_const(b.tab[kTabPred] + e);
-        const SolveState st = ld_const(b.state + w.x);
-        if (!st.active) continue;
+    // entry e's table row and solve state, read one entry ahead
+    int2 w = make_int2(0, 0);
+    SolveState st{};
+    auto fetch = [&](int e) {
+        w = ld_const(b.tab[kTabPred] + e);
+        st = ld_const(b.state + w.x);
+    };
+    // the copy of this wave's four cells of the fetched entry into buffer k
+    auto prefetch = [&](int k) {
+        if (!st.active) return;
         const ADG_CONST SolveArgs *sa = (const ADG_CONST SolveArgs *)(b.solves + w.x);
-        predictor_lean<P, N, false, true>(sa->m, b, TestBufs{}, smem, w.y, sa->cell0, st.parity, st.dt);
-        __syncthreads();
+        const int c = w.y + 4 * wv, nc = sa->m.ncell - c;
+        if (nc > 0)
+            mf_prefetch<P, N>(b.uh[st.parity] + ((size_t)sa->cell0 + c) * P::V * S::NN,
+                              lds + (k * WAVES + wv) * S::CELLS, nc < 4 ? nc : 4);
+    };
+    int k = 0;
+    int e = xcd_entry(blockIdx.x, gridDim.x);
+    if (e < ntab) {
+        fetch(e);
+        prefetch(0);
+    }
+    for (; e < ntab; e += gridDim.x, k ^= 1) {
+        const int2 wc = w;
+        const SolveState sc = st;
+        // the next entry's cells go to the other buffer, whose last reader was the previous entry;
+        // issued once this entry's copy has landed (the wait for it counts every load before it)
+        const int en = e + (int)gridDim.x;
+        if (en < ntab) fetch(en);
+        auto next = [&]() {
+            if (en < ntab) prefetch(k ^ 1);
+        };
+        if (!sc.active) {
+            next();
+            continue;
+        }
+        const ADG_CONST SolveArgs *sa = (const ADG_CONST SolveArgs *)(b.solves + wc.x);
+        predictor_mf<P, N, false, true>(sa->m, b, TestBufs{}, nullptr, wc.y, sa->cell0, sc.parity, sc.dt,
+                                        lds + (k * WAVES + wv) * S::CELLS, sop, next);
     }
 }
 
 template <class P, int N>
 __global__ __launch_bounds__(kPredBlock<P>) void k_predictor_mfma_test(MeshArgs m, StepBufs b, TestBufs t, double dt)
 {
-    __shared__ double smem[LeanShape<P, N>::LDS];
-    predictor_lean<P, N, true, true>(m, b, t, smem, blockIdx.x * Shape<P, N>::CPW, 0, 0, dt);
+    __shared__ double smem[MfShape<P, N>::LDS];
+    predictor_mf<P, N, true, false>(m, b, t, smem, blockIdx.x * MfShape<P, N>::CPW, 0, 0, dt);
 }
 
 template <class P, int N, int BLK>
@@ -2694,6 +2981,45 @@ __global__ __launch_bounds__(kSubBnBlock) __attribute__((amdgpu_waves_per_eu(2))
 // doubles order like their bits; NaN candidates are skipped as MINVAL skips
 // them); the last workgroup to finish turns the minima into dt.  No host sync,
 // no memset.
+// get_min_max step 2, Euler/limiter.f90:147-167: the DMP bounds of every cell, the min / max over
+// its 3x3 neighbourhood of the first-pass ranges (lmin_old / lmax_old, left by the previous
+// step's update and subcell passes or by the seed).  One thread per cell, consecutive cells on
+// consecutive lanes, so every load is coalesced and the neighbour rows come from L2; in the
+// predictor kernels this gather sat at the end of each entry with its latency exposed.
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_ranges(StepBufs b)
+{
+    const int ntab = ld_const(b.meta + kTabCell);
+    for (int e = xcd_entry(blockIdx.x, gridDim.x); e < ntab; e += gridDim.x) {
+        const int2 w = ld_const(b.tab[kTabCell] + e);
+        if (!ld_const(b.state + w.x).active) continue;
+        const ADG_CONST SolveArgs *sa = (const ADG_CONST SolveArgs *)(b.solves + w.x);
+        const MeshArgs m = sa->m;
+        const int cell = w.y + (int)threadIdx.x;
+        if (cell >= m.ncell) continue;
+        const int ci = cell % m.imax, cj = cell / m.imax;
+        int nb[9];
+#pragma unroll
+        for (int jj = -1; jj <= 1; jj++)
+#pragma unroll
+            for (int ii = -1; ii <= 1; ii++)
+                nb[3 * (jj + 1) + ii + 1] =
+                    nb_index(ci, ii, m.imax, m.wrap & 1) + m.imax * nb_index(cj, jj, m.jmax, m.wrap >> 1);
+#pragma unroll
+        for (int v = 0; v < P::V; v++) {
+            const size_t row = (size_t)v * b.G + sa->cell0;
+            double mn = b.lmin_old[row + cell], mx = b.lmax_old[row + cell];
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                mn = f_min(mn, b.lmin_old[row + nb[k]]);
+                mx = f_max(mx, b.lmax_old[row + nb[k]]);
+            }
+            b.lmin[row + cell] = mn;
+            b.lmax[row + cell] = mx;
+        }
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_dt(StepBufs b, double host_time, double host_tend, int use_host_time)
 {
     __shared__ double red[kBlock];
@@ -2939,7 +3265,7 @@ hipError_t launch_predictor(const StepBufs &b, hipStream_t st, int variant, int 
         hipLaunchKernelGGL((k_predictor_lean<P, N>), dim3(grid), dim3(S::PRED_BLOCK), 0, st, b);
         break;
     case kPredMfma:
-        if constexpr (kMfmaBuild && N == 3)
+        if constexpr (kMfmaBuild && N <= 3)
             hipLaunchKernelGGL((k_predictor_mfma<P, N>), dim3(grid), dim3(S::PRED_BLOCK), 0, st, b);
         else
             return hipErrorInvalidValue;
@@ -2956,6 +3282,9 @@ hipError_t launch_step_impl(const StepBufs &b, hipStream_t st, int part, const i
 {
     using S = Shape<P, N>;
     switch (part) {
+    case kPartRanges:
+        hipLaunchKernelGGL((k_ranges<P>), dim3(grid[kTabCell]), dim3(kBlock), 0, st, b);
+        break;
     case kPartPredictor: {
         const hipError_t e = launch_predictor<P, N>(b, st, variant, grid[kTabPred]);
         if (e != hipSuccess) return e;
@@ -3042,8 +3371,8 @@ hipError_t launch_test_impl(const MeshArgs &m, const StepBufs &b, const TestBufs
         break;
     }
     case kPredMfma:
-        if constexpr (kMfmaBuild && N == 3) {
-            const int g = (m.ncell + S::CPW - 1) / S::CPW;
+        if constexpr (kMfmaBuild && N <= 3) {
+            const int g = (m.ncell + MfShape<P, N>::CPW - 1) / MfShape<P, N>::CPW;
             hipLaunchKernelGGL((k_predictor_mfma_test<P, N>), dim3(g), dim3(S::PRED_BLOCK), 0, st, m, b, t, dt);
         } else {
             return hipErrorInvalidValue;
@@ -3072,7 +3401,14 @@ template <class P, int N>
 void quanta_impl(int variant, int q[kNumTabs])
 {
     constexpr int N3 = (N + 1) * (N + 1) * (N + 1);
-    q[kTabPred] = (variant == kPredLine || variant == kPredLean || variant == kPredMfma) ? Shape<P, N>::CPW
+    if constexpr (N <= 3) {
+        if (variant == kPredMfma) {
+            q[kTabPred] = MfShape<P, N>::CPW;
+            q[kTabUpd] = Shape<P, N>::CPWU;
+            return;
+        }
+    }
+    q[kTabPred] = (variant == kPredLine || variant == kPredLean) ? Shape<P, N>::CPW
                   : variant == kPredNode64 ? 64 / N3 : variant == kPredNode128 ? 128 / N3 : 256 / N3;
     q[kTabUpd] = Shape<P, N>::CPWU;
 }
@@ -3096,7 +3432,7 @@ bool pred_variant_valid(int pde, int N, int variant)
 {
     if (N < 1 || N > 4) return false;
     if (pde == kPdeAd) return variant == kPredLine;
-    if (variant == kPredMfma) return kMfmaBuild && N == 3;
+    if (variant == kPredMfma) return kMfmaBuild && N <= 3;
     if (variant == kPredLine || variant == kPredNode128 || variant == kPredNode256 || variant == kPredLean) return true;
     return variant == kPredNode64 && (N + 1) * (N + 1) * (N + 1) <= 64;
 }
@@ -3132,7 +3468,7 @@ int resident_impl(int variant, int upd, int tab)
         case kPredNode256: return resident_of(k_predictor_nodes<P, N, 256>, 256);
         case kPredLean: return resident_of(k_predictor_lean<P, N>, S::PRED_BLOCK);
         case kPredMfma:
-            if constexpr (kMfmaBuild && N == 3) return resident_of(k_predictor_mfma<P, N>, S::PRED_BLOCK);
+            if constexpr (kMfmaBuild && N <= 3) return resident_of(k_predictor_mfma<P, N>, S::PRED_BLOCK);
             return 0;
         default: return 0;
         }

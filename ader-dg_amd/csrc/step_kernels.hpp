@@ -26,8 +26,9 @@ enum { kBcDmr = 0, kBcPeriodic = 1, kBcTransmissive = 2, kBcSides = 3 };
 // boundary code of one side (the reference's params.ini convention, Euler/params.ini:8-11)
 enum { kSideInflow = 0, kSideTransmit = 1, kSideReflect = 2, kSideWrap = 3 };
 enum { kCntList = 0, kCntTroubled = 1, kCntDtBlocks = 2, kNumCounters = 4 };
+// kPartRanges (the DMP bounds of the step) runs before every kPartPredictor
 enum { kPartDt = 0, kPartPredictor = 1, kPartUpdate = 2, kPartSpread = 3, kPartSubcell = 4, kPartAdvance = 5,
-       kPartSeed = 6, kPartFace = 7, kPartFlip = 8, kPartSeedDone = 9 };
+       kPartSeed = 6, kPartFace = 7, kPartFlip = 8, kPartSeedDone = 9, kPartRanges = 10 };
 // work tables: items per entry are cells (predictor, update, cell) or face points (face)
 enum { kTabPred = 0, kTabUpd = 1, kTabFace = 2, kTabCell = 3, kNumTabs = 4 };
 // device meta words: [0, kNumTabs) table lengths, then the number of solve slots in use
@@ -108,8 +109,9 @@ struct TestBufs {
 // whole time line; kPredLean: the same mapping with a smaller register footprint) or
 // space-time-node threads (one thread per node of the space-time cell) in workgroups
 // of 64, 128 or 256.  All give the same bits; the autotuner
-// (tune.hip) picks one per (PDE, N).  kPredMfma (N = 3, product build only): the lean
-// mapping with the slice contractions on the matrix cores, within rounding of the others.
+// (tune.hip) picks one per (PDE, N).  kPredMfma (N <= 3, product build only): time-line
+// threads, one cell per 16 lanes, the contractions, volume integral and boundary
+// extrapolation on the matrix cores; within rounding of the others.
 // Advection-diffusion has the time-line form only.
 enum {
     kPredLine = 0, kPredNode64 = 1, kPredNode128 = 2, kPredNode256 = 3, kPredLean = 4, kPredMfma = 5,

@@ -300,14 +300,16 @@ def skipped_for(kernel, pde, N):
 
 def algorithmic_bytes(pde, N):
     """HBM bytes per cell-step each kernel must move at least (FP64, ping-pong state):
-    predictor reads uh (and its 3x3 ranges), writes lduh, the face extrapolations fq/ff and
-    the ranges; face reads fq/ff of both neighbours and writes the fluxes (one x and one y
-    face per cell; Baer-Nunziato two fluctuation sides); update reads uh, lduh, four face
-    lines and the ranges, writes uh', the next ranges and the time-step candidate."""
+    ranges reads the cell's first-pass ranges (its neighbours' come from cache) and writes
+    the 3x3 ranges; predictor reads uh, writes lduh and the face extrapolations fq/ff; face
+    reads fq/ff of both neighbours and writes the fluxes (one x and one y face per cell;
+    Baer-Nunziato two fluctuation sides); update reads uh, lduh, four face lines and the
+    ranges, writes uh', the next ranges and the time-step candidate."""
     V = 9 if pde == "bn" else 4
     n = N + 1
     sides = 2 if pde == "bn" else 1
-    return {"predictor": 8 * (V * n * n + 2 * V) + 8 * (V * n * n + 2 * 4 * V * n + 2 * V),
+    return {"ranges": 8 * (2 * V) + 8 * (2 * V),
+            "predictor": 8 * (V * n * n) + 8 * (V * n * n + 2 * 4 * V * n),
             "face": 2 * 8 * (2 * 2 * V * n) + 2 * 8 * (sides * V * n),
             "update": 8 * (2 * V * n * n + 4 * V * n + 2 * V) + 8 * (V * n * n + 2 * V + 1)}
 
@@ -318,7 +320,7 @@ def kernel_table(pde, per_group, peaks):
     for N, g in per_group.items():
         ab = algorithmic_bytes(pde, int(N))
         row = {}
-        for k in ("predictor", "face", "update", "subcell", "dt", "spread"):
+        for k in ("predictor", "face", "update", "subcell", "dt", "spread", "ranges"):
             ms = g.get(k)
             if not ms:
                 continue

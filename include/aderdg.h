@@ -246,8 +246,9 @@ int aderdg_predictor_cells_noalloc_max(int pde, int N);
  * hipEvents on the context stream.  Returns accumulated milliseconds and
  * launch counts per kernel: [0] dt reduction, [1] predictor, [2] surface
  * integral + update + DMP, [3] spread/compact, [4] subcell recompute,
- * [5] face Riemann fluxes. */
-#define ADERDG_TIMING_SLOTS 6
+ * [5] face Riemann fluxes, [6] the 3x3 neighbourhood ranges of get_min_max
+ * (the DMP bounds of the step). */
+#define ADERDG_TIMING_SLOTS 7
 int aderdg_timing_enable(aderdg_ctx *ctx, int on);
 int aderdg_timing_get(aderdg_ctx *ctx, double ms[ADERDG_TIMING_SLOTS], int64_t launches[ADERDG_TIMING_SLOTS]);
 
@@ -257,9 +258,9 @@ int aderdg_timing_get(aderdg_ctx *ctx, double ms[ADERDG_TIMING_SLOTS], int64_t l
  * time line), 1/2/3 space-time-node threads in workgroups of 64/128/256 (1 is
  * absent for N=4, whose space-time cell has 125 nodes), 4 time-line threads
  * with a smaller register footprint (right-hand sides kept in the time line,
- * state and operators in LDS), 5 (N=3, libaderdg_amd.so only) variant 4 with the
- * per-slice contractions on the FP64 matrix cores, equal to the others within
- * rounding.  Advection-diffusion has variant 0 only.  The first context or
+ * state and operators in LDS), 5 (N<=3, libaderdg_amd.so only) time-line threads
+ * with the per-slice contractions, the volume integral and the boundary
+ * extrapolation on the FP64 matrix cores, equal to the others within rounding.  Advection-diffusion has variant 0 only.  The first context or
  * batch arena of a (device, pde, N, launch mode) times every variant on a
  * synthetic 256x256 solve and keeps the fastest for the process.
  * aderdg_tune_force(pde, N, v) makes contexts and arenas created afterwards use

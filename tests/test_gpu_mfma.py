@@ -1,6 +1,7 @@
-"""The matrix-core predictor (variant 5: the lean time-line kernel with the x / y flux
-contractions and the Baer-Nunziato gradients of every time slice on
-v_mfma_f64_4x4x4_4b_f64; N=3, product build) against the oracle.
+"""The matrix-core predictor (variant 5: time-line threads, one cell per 16-lane block of
+v_mfma_f64_4x4x4_4b_f64, with the x / y flux contractions and Baer-Nunziato gradients of
+every time slice, the volume integral and the boundary extrapolation on the matrix
+cores; N <= 3, cells of N < 3 padded to 4 x 4 nodes; product build) against the oracle.
 
 The matrix core sums each 4-term contraction in its own order, so the variant is
 held to the parity bar of the product build rather than to bit equality:
@@ -14,7 +15,7 @@ held to the parity bar of the product build rather than to bit equality:
   of the Euler cells);
 * whole steps of the reference loop (Euler/main.f90:24-100,
   Baer-Nunziato/main.f90:24-100) with the limiter active: the product-build long runs
-  of test_gpu_product_long at N=3 with the variant forced (DOF and cell-average
+  of test_gpu_product_long at N=2 and 3 with the variant forced (DOF and cell-average
   relative L2 <= 1e-12 each step, dt within 1e-12, flags identical away from the DMP
   threshold);
 * a batch arena on a persistent grid of three workgroups (hundreds of rounds per
@@ -43,14 +44,16 @@ def log(*a):
 def auto_variant():
     yield
     for pde in ("euler", "bn"):
-        aderdg.tune_force(pde, 3, -1, exact=False)
-        aderdg.tune_force(pde, 3, -1, exact=False, kernel="update")
+        for N in (1, 2, 3):
+            aderdg.tune_force(pde, N, -1, exact=False)
+            aderdg.tune_force(pde, N, -1, exact=False, kernel="update")
 
 
 def test_variant_exists_in_product_build_only():
     assert aderdg.pred_variant_valid("bn", 3, MFMA, exact=False)
     assert not aderdg.pred_variant_valid("bn", 3, MFMA, exact=True)
-    assert not aderdg.pred_variant_valid("euler", 2, MFMA, exact=False)
+    assert aderdg.pred_variant_valid("euler", 2, MFMA, exact=False)
+    assert not aderdg.pred_variant_valid("euler", 4, MFMA, exact=False)
     aderdg.tune_force("euler", 3, MFMA, exact=False)
     with pytest.raises(aderdg.AderdgError):
         aderdg.tune_force("euler", 3, MFMA, exact=True)
@@ -58,40 +61,40 @@ def test_variant_exists_in_product_build_only():
         aderdg.tune_force("bn", 4, MFMA, exact=False)
 
 
-def _predictor_outputs(pde, variant, luh, dt):
-    aderdg.tune_force(pde, 3, variant, exact=False)
+def _predictor_outputs(pde, N, variant, luh, dt):
+    aderdg.tune_force(pde, N, variant, exact=False)
     if pde == "bn":
-        s = aderdg.Solver(3, 4, 4, (0.0, 0.0), (1.0, 1.0), "transmissive", exact=False, pde="bn", eos=BN_EOS_STIFF)
+        s = aderdg.Solver(N, 4, 4, (0.0, 0.0), (1.0, 1.0), "transmissive", exact=False, pde="bn", eos=BN_EOS_STIFF)
     else:
-        s = aderdg.Solver(3, 4, 4, (0.0, 0.0), (1.0, 1.0), "periodic", exact=False)
+        s = aderdg.Solver(N, 4, 4, (0.0, 0.0), (1.0, 1.0), "periodic", exact=False)
     g = s.predictor_cells(luh, dt)
     s.close()
     return g
 
 
-@pytest.mark.parametrize("pde", ["euler", "bn"])
-def test_mfma_predictor_cells(pde):
-    log("matrix-core predictor cells %s N=3" % pde)
-    rng = np.random.default_rng(9301 if pde == "bn" else 9302)
+@pytest.mark.parametrize("pde,N", [(pde, N) for pde in ("euler", "bn") for N in (1, 2, 3)])
+def test_mfma_predictor_cells(pde, N):
+    log("matrix-core predictor cells %s N=%d" % (pde, N))
+    rng = np.random.default_rng(9300 + 10 * N + (1 if pde == "bn" else 2))
     if pde == "bn":
         # 45 cells: 11 full workgroups of four and a ragged last one
-        luh = random_bn_cells(rng, 3, 45, BN_EOS_STIFF)
+        luh = random_bn_cells(rng, N, 45, BN_EOS_STIFF)
         dt = 2.0e-7
-        args = (3, 4, 4, (0.0, 0.0), (1.0, 1.0), BC_TRANSMISSIVE)
+        args = (N, 4, 4, (0.0, 0.0), (1.0, 1.0), BC_TRANSMISSIVE)
         kw = dict(pde=1, eos=BN_EOS_STIFF)
         keys = ("qhi", "Fhi", "Shi", "qbnd", "Fbnd")
     else:
-        luh = random_cells(rng, 3, 45)
+        luh = random_cells(rng, N, 45)
         dt = 0.013
-        args, kw = (3, 4, 4, (0.0, 0.0), (1.0, 1.0), BC_PERIODIC), {}
+        args, kw = (N, 4, 4, (0.0, 0.0), (1.0, 1.0), BC_PERIODIC), {}
         keys = ("qhi", "Fhi", "qbnd", "Fbnd")
     if not OracleFMA.available():
         pytest.skip("oracle FMA build or an FMA-capable host missing")
     o, ofma = Oracle(*args, **kw), OracleFMA(*args, **kw)
     for x in (o, ofma):
         x.set_scalars(dt, 0.0, 1e9)
-    g = _predictor_outputs(pde, MFMA, luh, dt)
-    lean = _predictor_outputs(pde, 4, luh, dt)
+    g = _predictor_outputs(pde, N, MFMA, luh, dt)
+    lean = _predictor_outputs(pde, N, 4, luh, dt)
     ref = [o.predictor(luh[..., c]) for c in range(luh.shape[-1])]
     fma = [ofma.predictor(luh[..., c]) for c in range(luh.shape[-1])]
     for k in keys:
@@ -104,50 +107,55 @@ def test_mfma_predictor_cells(pde):
         assert e_o <= max(TOL, ENVELOPE * e_f), (k, e_o, e_f)
 
 
-@pytest.mark.parametrize("name", ["quadrant_n3_96", "bn_interface_n3_48"])
+@pytest.mark.parametrize("name", ["quadrant_n3_96", "bn_interface_n3_48", "bn_interface_n2_48"])
 def test_mfma_long_run(name):
     c = long_runs.CASES[name]
     log("matrix-core predictor, long run %s" % name)
-    aderdg.tune_force(c["pde"], 3, MFMA, exact=False)
+    aderdg.tune_force(c["pde"], c["N"], MFMA, exact=False)
     long_runs.test_product_build_long_run(name)
 
 
 def test_mfma_persistent_batch():
-    """Two Baer-Nunziato N=3 interfaces and an Euler N=3 quadrant in one arena on three
-    persistent workgroups per kernel, each equal to the same solve stepped alone (the same
-    build and variant: the same operations in the same order, so the same bits)."""
+    """Baer-Nunziato interfaces at N=2 and 3 and an Euler N=3 quadrant in one batch (an arena
+    per (PDE, N)) on three persistent workgroups per kernel, each equal to the same solve
+    stepped alone (the same build and variant: the same operations in the same order, so the
+    same bits)."""
     for pde in ("euler", "bn"):
-        aderdg.tune_force(pde, 3, MFMA, exact=False)
-        aderdg.tune_force(pde, 3, 0, exact=False, kernel="update")
-    xi = aderdg.operators(3)["xi"]
+        for N in (2, 3):
+            aderdg.tune_force(pde, N, MFMA, exact=False)
+            aderdg.tune_force(pde, N, 0, exact=False, kernel="update")
+    x3, x2 = aderdg.operators(3)["xi"], aderdg.operators(2)["xi"]
     cases = [
-        dict(pde="bn", I=64, J=64, steps=3, bc="transmissive",
-             uh=ics.initial_uh_bn(xi, 64, 64, (0.0, 0.0), (1.0, 1.0), BN_EOS_STIFF, radius=0.2, pratio=1.0e4,
+        dict(pde="bn", N=3, I=64, J=64, steps=3, bc="transmissive",
+             uh=ics.initial_uh_bn(x3, 64, 64, (0.0, 0.0), (1.0, 1.0), BN_EOS_STIFF, radius=0.2, pratio=1.0e4,
                                   phi_eps=ics.BN_PHI_STABLE)),
-        dict(pde="bn", I=48, J=40, steps=4, bc="transmissive",
-             uh=ics.initial_uh_bn(xi, 48, 40, (0.0, 0.0), (1.0, 1.0), BN_EOS_STIFF, radius=0.3, pratio=2.0e3)),
-        dict(pde="euler", I=64, J=64, steps=5, bc="transmissive",
-             uh=ics.initial_uh("quadrant", xi, 64, 64, (0.0, 0.0), (1.0, 1.0))),
+        dict(pde="bn", N=3, I=48, J=40, steps=4, bc="transmissive",
+             uh=ics.initial_uh_bn(x3, 48, 40, (0.0, 0.0), (1.0, 1.0), BN_EOS_STIFF, radius=0.3, pratio=2.0e3)),
+        dict(pde="bn", N=2, I=72, J=56, steps=4, bc="transmissive",
+             uh=ics.initial_uh_bn(x2, 72, 56, (0.0, 0.0), (1.0, 1.0), BN_EOS_STIFF, radius=0.25, pratio=3.0e3)),
+        dict(pde="euler", N=3, I=64, J=64, steps=5, bc="transmissive",
+             uh=ics.initial_uh("quadrant", x3, 64, 64, (0.0, 0.0), (1.0, 1.0))),
     ]
     b = aderdg.Batch(exact=False, persist_blocks=3)
-    for pde in ("euler", "bn"):
-        b.reserve(pde, 3, max_cells=64 * 64 * 2, max_solves=2)
+    for pde, N in (("euler", 3), ("bn", 3), ("bn", 2)):
+        b.reserve(pde, N, max_cells=72 * 64 * 2, max_solves=2)
     for c in cases:
         eos = BN_EOS_STIFF if c["pde"] == "bn" else None
-        c["id"] = b.add(c["uh"], 3, c["I"], c["J"], (0.0, 0.0), (1.0, 1.0), bc=c["bc"], pde=c["pde"], eos=eos,
+        c["id"] = b.add(c["uh"], c["N"], c["I"], c["J"], (0.0, 0.0), (1.0, 1.0), bc=c["bc"], pde=c["pde"], eos=eos,
                         max_steps=c["steps"])
     b.step(max(c["steps"] for c in cases) + 1)
     b.sync()
     for c in cases:
         eos = BN_EOS_STIFF if c["pde"] == "bn" else None
-        s = aderdg.Solver(3, c["I"], c["J"], (0.0, 0.0), (1.0, 1.0), c["bc"], exact=False, pde=c["pde"], eos=eos)
+        s = aderdg.Solver(c["N"], c["I"], c["J"], (0.0, 0.0), (1.0, 1.0), c["bc"], exact=False, pde=c["pde"],
+                          eos=eos)
         s.upload(c["uh"])
         t = 0.0
         for _ in range(c["steps"]):
             dt, _ = s.step(t, 1e9)
             t += dt
         q = b.query(c["id"])
-        log("batch %s %dx%d: %d steps, t=%.6g" % (c["pde"], c["I"], c["J"], q["steps"], q["time"]))
+        log("batch %s N=%d %dx%d: %d steps, t=%.6g" % (c["pde"], c["N"], c["I"], c["J"], q["steps"], q["time"]))
         assert q["steps"] == c["steps"] and q["time"] == t, (q, t)
         np.testing.assert_array_equal(b.download(c["id"]), s.download(), err_msg="%s uh" % c["pde"])
         la, lb = b.limiter_state(c["id"]), s.get_limiter()
