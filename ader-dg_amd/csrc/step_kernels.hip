@@ -785,6 +785,22 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
 // matrix core sums each 4-term contraction in its own order (measured: the order of an FMA
 // chain over k), within the product build's parity.
 // ---------------------------------------------------------------------------
+// Non-temporal 16-byte loads and stores (streamed data that no other workgroup of the kernel
+// reads back: their lines need not stay in L2, where they would push out shared lines)
+typedef double ntd2 __attribute__((ext_vector_type(2)));
+ADG_DEV double2 nt_load2(const double2 *p)
+{
+    const ntd2 x = __builtin_nontemporal_load((const ntd2 *)p);
+    return make_double2(x.x, x.y);
+}
+ADG_DEV void nt_store2(double2 *p, double2 v)
+{
+    ntd2 x;
+    x.x = v.x;
+    x.y = v.y;
+    __builtin_nontemporal_store(x, (ntd2 *)p);
+}
+
 // s_waitcnt immediate that waits for vmcnt(0) only (expcnt and lgkmcnt at their maxima)
 constexpr int kWaitVmcnt0 = 0x0F70;
 // the admissible state idle and padding lanes carry (the dummy of every predictor variant)
@@ -801,10 +817,16 @@ ADG_DEV void mf_dummy(double *u)
     }
 }
 
+// threads of the matrix-core predictor's workgroup (four cells per wave): four waves, so that a
+// workgroup writes each face-extrapolation plane's 128-byte line of 16 cells at once (with one
+// wave, Baer-Nunziato's two arenas ran 2-3 ms per ensemble step slower side by side)
+template <class P>
+constexpr int kMfBlock = 256;
+
 template <class P, int N>
 struct MfShape {
     static constexpr int n = N + 1, NN = n * n, V = P::V;
-    static constexpr int CPW = kPredBlock<P> / 16;  // cells per workgroup: four per wave
+    static constexpr int CPW = kMfBlock<P> / 16;  // cells per workgroup: four per wave
     static constexpr int LDS = CPW * V * 16;        // test kernel: initial state per cell, [v][16 lanes]
     static constexpr int CELLS = 4 * V * NN;        // doubles of a wave's four cells, uh layout
     // bytes per lane of one global-to-LDS load: 16 where every cell starts 16-byte aligned
@@ -1361,10 +1383,10 @@ __global__ __launch_bounds__(kPredBlock<P>) void k_predictor_lean_test(MeshArgs 
 
 // the matrix-core predictor (N <= 3, product build)
 template <class P, int N>
-__global__ __launch_bounds__(kPredBlock<P>) __attribute__((amdgpu_waves_per_eu(2))) void k_predictor_mfma(StepBufs b)
+__global__ __launch_bounds__(kMfBlock<P>) __attribute__((amdgpu_waves_per_eu(2))) void k_predictor_mfma(StepBufs b)
 {
     using S = MfShape<P, N>;
-    constexpr int WAVES = kPredBlock<P> / 64;
+    constexpr int WAVES = kMfBlock<P> / 64;
     // two buffers of each wave's four cells, alternating between the entries of the loop (a wave
     // only touches its own: no barrier in the loop), then the operator entries and dummy state
     __shared__ double lds[2 * WAVES * S::CELLS + kSopSize<P>];
@@ -1439,7 +1461,7 @@ __global__ __launch_bounds__(kPredBlock<P>) __attribute__((amdgpu_waves_per_eu(2
 }
 
 template <class P, int N>
-__global__ __launch_bounds__(kPredBlock<P>) void k_predictor_mfma_test(MeshArgs m, StepBufs b, TestBufs t, double dt)
+__global__ __launch_bounds__(kMfBlock<P>) void k_predictor_mfma_test(MeshArgs m, StepBufs b, TestBufs t, double dt)
 {
     __shared__ double smem[MfShape<P, N>::LDS];
     predictor_mf<P, N, true, false>(m, b, t, smem, blockIdx.x * MfShape<P, N>::CPW, 0, 0, dt);
@@ -2057,8 +2079,11 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
 #pragma unroll
         for (int q = 0; q < KR; q++) {
             const int k = min(tid + q * kUpdBlock<P>, np - 1);  // lanes past the end reload the last pair
-            ru[q] = u2[k];
-            if (!SEED) rl[q] = l2[k];
+            // the rows stream through: non-temporal, so that they do not evict the face lines
+            // the entries of a row share in L2 (the N=3 Baer-Nunziato update read 1.7x its
+            // algorithmic bytes with plain loads, 1.06x with these)
+            ru[q] = nt_load2(u2 + k);
+            if (!SEED) rl[q] = nt_load2(l2 + k);
         }
     }
     // the odd ends of unpaired rows: the leading double (h = 1) and the trailing one
@@ -2184,11 +2209,11 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
             double2 *o2 = (double2 *)(b.uh[par ^ 1] + base);
             const double2 *f2 = (const double2 *)flat;
 #pragma unroll 2
-            for (int k = tid; k < cnt / 2; k += kUpdBlock<P>) o2[k] = f2[k];
+            for (int k = tid; k < cnt / 2; k += kUpdBlock<P>) nt_store2(o2 + k, f2[k]);
         } else {
             double2 *o2 = (double2 *)(b.uh[par ^ 1] + base + h);
 #pragma unroll 2
-            for (int k = tid; k < np; k += kUpdBlock<P>) o2[k] = make_double2(flat[h + 2 * k], flat[h + 2 * k + 1]);
+            for (int k = tid; k < np; k += kUpdBlock<P>) nt_store2(o2 + k, make_double2(flat[h + 2 * k], flat[h + 2 * k + 1]));
             const int e = tid == 0 ? 0 : cnt - 1;
             if ((tid == 0 && h) || (tid == 1 && ((cnt - h) & 1))) b.uh[par ^ 1][base + e] = flat[e];
         }
@@ -3266,7 +3291,7 @@ hipError_t launch_predictor(const StepBufs &b, hipStream_t st, int variant, int 
         break;
     case kPredMfma:
         if constexpr (kMfmaBuild && N <= 3)
-            hipLaunchKernelGGL((k_predictor_mfma<P, N>), dim3(grid), dim3(S::PRED_BLOCK), 0, st, b);
+            hipLaunchKernelGGL((k_predictor_mfma<P, N>), dim3(grid), dim3(kMfBlock<P>), 0, st, b);
         else
             return hipErrorInvalidValue;
         break;
@@ -3373,7 +3398,7 @@ hipError_t launch_test_impl(const MeshArgs &m, const StepBufs &b, const TestBufs
     case kPredMfma:
         if constexpr (kMfmaBuild && N <= 3) {
             const int g = (m.ncell + MfShape<P, N>::CPW - 1) / MfShape<P, N>::CPW;
-            hipLaunchKernelGGL((k_predictor_mfma_test<P, N>), dim3(g), dim3(S::PRED_BLOCK), 0, st, m, b, t, dt);
+            hipLaunchKernelGGL((k_predictor_mfma_test<P, N>), dim3(g), dim3(kMfBlock<P>), 0, st, m, b, t, dt);
         } else {
             return hipErrorInvalidValue;
         }
@@ -3468,7 +3493,7 @@ int resident_impl(int variant, int upd, int tab)
         case kPredNode256: return resident_of(k_predictor_nodes<P, N, 256>, 256);
         case kPredLean: return resident_of(k_predictor_lean<P, N>, S::PRED_BLOCK);
         case kPredMfma:
-            if constexpr (kMfmaBuild && N <= 3) return resident_of(k_predictor_mfma<P, N>, S::PRED_BLOCK);
+            if constexpr (kMfmaBuild && N <= 3) return resident_of(k_predictor_mfma<P, N>, kMfBlock<P>);
             return 0;
         default: return 0;
         }

@@ -103,11 +103,22 @@ double time_launches(const Trial &t, int part, hipStream_t st, hipEvent_t e0, hi
     return ms / kTuneReps;
 }
 
-int env_variant(const char *name)
+// The variant an environment variable pins for degree N: "<v>" for every degree, or a list
+// "<N>:<v>,<N>:<v>..." per degree (-1: none for N)
+int env_variant(const char *name, int N)
 {
     const char *e = std::getenv(name);
     if (!e || !*e) return -1;
-    return std::atoi(e);
+    if (!std::strchr(e, ':')) return std::atoi(e);
+    for (const char *p = e; *p;) {
+        char *end = nullptr;
+        const long n = std::strtol(p, &end, 10);
+        if (end == p || *end != ':') return -1;
+        const long v = std::strtol(end + 1, &end, 10);
+        if (n == N) return (int)v;
+        p = *end == ',' ? end + 1 : end;
+    }
+    return -1;
 }
 
 bool upd_variant_valid(int v) { return v >= 0 && v < kNumUpdVariants; }
@@ -125,8 +136,8 @@ void tune_kernels(int pde, int N, int persist_blocks, const Operators &ops, hipS
         fp = g_forced[0][pde][N];
         fu = g_forced[1][pde][N];
     }
-    if (fp < 0) fp = env_variant("ADERDG_PREDICTOR_VARIANT");
-    if (fu < 0) fu = env_variant("ADERDG_UPDATE_VARIANT");
+    if (fp < 0) fp = env_variant("ADERDG_PREDICTOR_VARIANT", N);
+    if (fu < 0) fu = env_variant("ADERDG_UPDATE_VARIANT", N);
     const bool have_p = fp >= 0 && pred_variant_valid(pde, N, fp), have_u = upd_variant_valid(fu);
     if (have_p && have_u) {
         *pred = fp;

@@ -229,16 +229,28 @@ def cpu_ensemble(specs, target_seconds, procs, seed=5):
 
 
 # ---------------------------------------------------------------------------
+def pinned_variant(spec, N):
+    """The variant ADERDG_PREDICTOR_VARIANT pins for degree N ("<v>" or "<N>:<v>,..."), or None."""
+    spec = spec.strip()
+    if not spec:
+        return None
+    if ":" not in spec:
+        return int(spec) if spec.isdigit() else None
+    for item in spec.split(","):
+        n, _, v = item.partition(":")
+        if n.strip() == str(N) and v.strip().isdigit():
+            return int(v)
+    return None
+
+
 def predictor_name(pde, N, persistent):
     """Kernel name of the predictor variant in use: the one ADERDG_PREDICTOR_VARIANT pins
     (the library then skips the autotuner), else the autotuner's choice."""
     import aderdg
     names = {0: "k_predictor", 1: "k_predictor_nodes<.., 64>", 2: "k_predictor_nodes<.., 128>",
              3: "k_predictor_nodes<.., 256>", 4: "k_predictor_lean", 5: "k_predictor_mfma"}
-    pinned = os.environ.get("ADERDG_PREDICTOR_VARIANT", "")
-    if pinned.strip().isdigit() and aderdg.pred_variant_valid(pde, N, int(pinned), exact=False):
-        v = int(pinned)
-    else:
+    v = pinned_variant(os.environ.get("ADERDG_PREDICTOR_VARIANT", ""), N)
+    if v is None or not aderdg.pred_variant_valid(pde, N, v, exact=False):
         t = aderdg.tune_query(pde, N, persistent)
         v = t[0] if t else 0
     return "%s<%sPDE,%d>" % (names[v], "Bn" if pde == "bn" else "Euler", N)

@@ -265,7 +265,8 @@ int aderdg_timing_get(aderdg_ctx *ctx, double ms[ADERDG_TIMING_SLOTS], int64_t l
  * synthetic 256x256 solve and keeps the fastest for the process.
  * aderdg_tune_force(pde, N, v) makes contexts and arenas created afterwards use
  * variant v (-1: back to the tuner); the environment variable
- * ADERDG_PREDICTOR_VARIANT=<v> does the same for every (pde, N).
+ * ADERDG_PREDICTOR_VARIANT=<v> does the same for every (pde, N), and
+ * ADERDG_PREDICTOR_VARIANT=<N>:<v>,<N>:<v>... per degree.
  * aderdg_tune_query returns the tuner's choice and the milliseconds per
  * launch it measured (-1 for variants it did not run); persistent selects
  * batch arenas (persistent grids) rather than single contexts. */

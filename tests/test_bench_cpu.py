@@ -68,3 +68,12 @@ def test_kernel_table_and_measured_peaks(monkeypatch):
     assert u["frac_measured_read_peak"] == pytest.approx(u["algorithmic_gbs"] / 6300.0)
     r = bench.roofline_of(154576.0, 1000, 2.0, "k_predictor_lean<BnPDE,3>")
     assert r["frac_measured"] == pytest.approx(r["achieved"] / 56.6) and r["bound"] == "mfma"
+
+
+def test_pinned_variant_spec():
+    """ADERDG_PREDICTOR_VARIANT: one variant for every degree, or a per-degree list."""
+    assert bench.pinned_variant("", 3) is None
+    assert bench.pinned_variant("4", 2) == 4
+    assert bench.pinned_variant("2:4,3:5", 3) == 5
+    assert bench.pinned_variant("2:4,3:5", 2) == 4
+    assert bench.pinned_variant("2:4,3:5", 4) is None
