@@ -260,9 +260,11 @@ int aderdg_timing_get(aderdg_ctx *ctx, double ms[ADERDG_TIMING_SLOTS], int64_t l
  * with a smaller register footprint (right-hand sides kept in the time line,
  * state and operators in LDS), 5 (N<=3, libaderdg_amd.so only) time-line threads
  * with the per-slice contractions, the volume integral and the boundary
- * extrapolation on the FP64 matrix cores, equal to the others within rounding.  Advection-diffusion has variant 0 only.  The first context or
- * batch arena of a (device, pde, N, launch mode) times every variant on a
- * synthetic 256x256 solve and keeps the fastest for the process.
+ * extrapolation on the FP64 matrix cores, equal to the others within rounding.
+ * Advection-diffusion has variant 0 only.  The first context or batch arena of
+ * a (device, pde, N, launch mode) times every variant on a synthetic 256x256
+ * solve and keeps the fastest for the process (variant 5 competes at N=3 only;
+ * below it pads each cell to 4x4 nodes).
  * aderdg_tune_force(pde, N, v) makes contexts and arenas created afterwards use
  * variant v (-1: back to the tuner); the environment variable
  * ADERDG_PREDICTOR_VARIANT=<v> does the same for every (pde, N), and
