@@ -140,10 +140,14 @@ void Group::footprint(int pde, int N, int64_t *per_cell, int64_t *per_xface, int
     const int64_t dbl = 3 * V * n * n + 4 * (V + ffc) * n + 4 * V + 1;
     const int64_t ints = 8;
     // work-table entries are a few bytes per cell; counted per cell at one entry per 4 cells
-    if (per_cell) *per_cell = 8 * dbl + 4 * ints + 4 * 8 / 4;
-    if (per_xface) *per_xface = 8 * sides * V * n;
-    if (per_yface) *per_yface = 8 * sides * V * n;
-    if (per_slot) *per_slot = (int64_t)(sizeof(SolveArgs) + sizeof(SolveState)) + 4 * 8 * kNumTabs + 4;
+    const int64_t cell = 8 * dbl + 4 * ints + 4 * 8 / 4, face = 8 * sides * V * n;
+    if (per_cell) *per_cell = cell;
+    if (per_xface) *per_xface = face;
+    if (per_yface) *per_yface = face;
+    // a slot of a batch arena also holds the alignment padding of its cell and face ranges (init)
+    if (per_slot)
+        *per_slot = (int64_t)(sizeof(SolveArgs) + sizeof(SolveState)) + 4 * 8 * kNumTabs + 4 +
+                    (kAlign - 1) * (cell + 2 * face);
     if (per_lh) *per_lh = 2 * 8 * V * s * s;  // Lh and NewLh
 }
 
@@ -161,9 +165,17 @@ int64_t Group::lh_pool(int pde, int64_t cells, int slots, double fraction)
     return std::min<int64_t>(cells, lh_slots(pde, cells, fraction) + (fraction < 1.0 ? slots : 0));
 }
 
-int Group::init(int pde_, int N_, const GroupCaps &c, const Operators &ops_, int persist_blocks, hipStream_t st,
+int Group::init(int pde_, int N_, const GroupCaps &c_in, const Operators &ops_, int persist_blocks, hipStream_t st,
                  int variant_, int upd_)
 {
+    // every solve's ranges start on a multiple of kAlign items: room for the padding of each slot
+    // (a single context's solve starts at 0 and its planes keep the stride of its own cells)
+    GroupCaps c = c_in;
+    if (c.slots > 1) {
+        c.cells += (kAlign - 1) * c.slots;
+        c.xfaces += (kAlign - 1) * c.slots;
+        c.yfaces += (kAlign - 1) * c.slots;
+    }
     pde = pde_;
     N = N_;
     V = pde == kPdeBn ? 9 : (pde == kPdeAd ? 1 : 4);
@@ -205,7 +217,7 @@ int Group::init(int pde_, int N_, const GroupCaps &c, const Operators &ops_, int
     }
     if (max_entries > (1LL << 30)) return ADERDG_EINVAL;
     const size_t G = c.cells, NN = (size_t)n * n, SS = (size_t)s * s, sides = pde == kPdeBn ? 2 : 1;
-    const int64_t pool = lh_pool(pde, c.cells, c.slots, c.lh_fraction);
+    const int64_t pool = lh_pool(pde, c_in.cells, c.slots, c.lh_fraction);
     b = StepBufs{};
     b.G = (int)c.cells;
     b.GX = (int)std::max<int64_t>(c.xfaces, 1);
@@ -320,21 +332,22 @@ int Group::add(const MeshArgs &m, const SolveState &st0, int *slot)
     if (sl < 0) return ADERDG_EFULL;
     const int64_t nfx = (int64_t)(m.imax + 1) * m.jmax, nfy = (int64_t)m.imax * (m.jmax + 1);
     const int64_t nlh = lh_slots(pde, m.ncell, caps.lh_fraction);
+    const int64_t ac = span(m.ncell), ax = span(nfx), ay = span(nfy);
     int64_t c0 = 0, x0 = 0, y0 = 0, l0 = 0;
-    if (!cell_alloc_.alloc(m.ncell, &c0)) return ADERDG_EFULL;
-    if (!fx_alloc_.alloc(nfx, &x0)) {
-        cell_alloc_.release(c0, m.ncell);
+    if (!cell_alloc_.alloc(ac, &c0)) return ADERDG_EFULL;
+    if (!fx_alloc_.alloc(ax, &x0)) {
+        cell_alloc_.release(c0, ac);
         return ADERDG_EFULL;
     }
-    if (!fy_alloc_.alloc(nfy, &y0)) {
-        cell_alloc_.release(c0, m.ncell);
-        fx_alloc_.release(x0, nfx);
+    if (!fy_alloc_.alloc(ay, &y0)) {
+        cell_alloc_.release(c0, ac);
+        fx_alloc_.release(x0, ax);
         return ADERDG_EFULL;
     }
     if (!lh_alloc_.alloc(nlh, &l0)) {
-        cell_alloc_.release(c0, m.ncell);
-        fx_alloc_.release(x0, nfx);
-        fy_alloc_.release(y0, nfy);
+        cell_alloc_.release(c0, ac);
+        fx_alloc_.release(x0, ax);
+        fy_alloc_.release(y0, ay);
         return ADERDG_EFULL;
     }
     SolveArgs &sa = solves_[sl];
@@ -371,9 +384,9 @@ int Group::remove(int sl)
     const MeshArgs &m = solves_[sl].m;
     SolveState off{};  // inactive, no seed pending
     if (staging_.put(b.state + sl, &off, sizeof(SolveState), st_) != hipSuccess) return ADERDG_EHIP;
-    cell_alloc_.release(solves_[sl].cell0, m.ncell);
-    fx_alloc_.release(solves_[sl].fx0, (int64_t)(m.imax + 1) * m.jmax);
-    fy_alloc_.release(solves_[sl].fy0, (int64_t)m.imax * (m.jmax + 1));
+    cell_alloc_.release(solves_[sl].cell0, span(m.ncell));
+    fx_alloc_.release(solves_[sl].fx0, span((int64_t)(m.imax + 1) * m.jmax));
+    fy_alloc_.release(solves_[sl].fy0, span((int64_t)m.imax * (m.jmax + 1)));
     lh_alloc_.release(solves_[sl].lh0, solves_[sl].lhcap);
     cells_in_use_ -= m.ncell;
     used_[sl] = 0;

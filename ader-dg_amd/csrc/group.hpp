@@ -39,6 +39,12 @@ private:
     std::vector<hipEvent_t> free_;
 };
 
+// Solves' cell and face ranges start on multiples of kAlign items and take multiples of it:
+// 16 doubles, one 128-byte line of every [v][cell] / [v][p][face] plane, so that a workgroup of
+// 16 cells (the matrix-core predictor's) writes whole lines of the face-extrapolation planes.
+constexpr int64_t kAlign = 16;
+inline int64_t align_up(int64_t n) { return (n + kAlign - 1) / kAlign * kAlign; }
+
 // First-fit allocator of index ranges in [0, cap), coalescing on release.
 class RangeAlloc {
 public:
@@ -87,6 +93,8 @@ public:
     const MeshArgs &mesh(int slot) const { return solves_[slot].m; }
     int cell0(int slot) const { return solves_[slot].cell0; }
     const SolveArgs &args(int slot) const { return solves_[slot]; }
+    // items a solve's range takes: aligned in arenas of several slots
+    int64_t span(int64_t n) const { return caps.slots > 1 ? align_up(n) : n; }
     bool used(int slot) const { return slot >= 0 && slot < caps.slots && used_[slot]; }
     int nslot() const { return nslot_; }
     int64_t cells_in_use() const { return cells_in_use_; }
