@@ -170,11 +170,12 @@ int Group::init(int pde_, int N_, const GroupCaps &c_in, const Operators &ops_, 
 {
     // every solve's ranges start on a multiple of kAlign items: room for the padding of each slot
     // (a single context's solve starts at 0 and its planes keep the stride of its own cells)
+    // and the plane strides are multiples of kAlign too, so that every plane's lines align
     GroupCaps c = c_in;
     if (c.slots > 1) {
-        c.cells += (kAlign - 1) * c.slots;
-        c.xfaces += (kAlign - 1) * c.slots;
-        c.yfaces += (kAlign - 1) * c.slots;
+        c.cells = align_up(c.cells + (kAlign - 1) * c.slots);
+        c.xfaces = align_up(c.xfaces + (kAlign - 1) * c.slots);
+        c.yfaces = align_up(c.yfaces + (kAlign - 1) * c.slots);
     }
     pde = pde_;
     N = N_;

@@ -878,10 +878,11 @@ constexpr int kSopSize = kSopPad + P::V;
 // PF: the initial state comes from ubuf, the wave's four cells in LDS (uh layout) whose
 // asynchronous copy was issued earlier, and the per-lane operator entries from sop (LDS, the
 // layout above); after() runs once the initial state is read (the caller's next prefetch).
+// After the sweeps ubuf stages the cells' lduh, which the wave then stores as whole rows.
 // Otherwise (test kernel) both come from global memory, the state staged in smem.
 template <class P, int N, bool TEST, bool PF, class M, class After = NoHook>
 ADG_DEV void predictor_mf(const M &m, const StepBufs &b, const TestBufs &t, double *smem, int first, int cell0,
-                          int par, double dt, const double *ubuf = nullptr, const double *sop = nullptr,
+                          int par, double dt, double *ubuf = nullptr, const double *sop = nullptr,
                           After after = {})
 {
     using S = MfShape<P, N>;
@@ -1056,6 +1057,12 @@ ADG_DEV void predictor_mf(const M &m, const StepBufs &b, const TestBufs &t, doub
     const size_t g = (size_t)cell0 + cell;
     const int fc = i, pt = j;
     const bool facelane = cellok && pt < n;
+    // STAGEF: the face extrapolations of the workgroup's 16 cells are staged in the entry's LDS
+    // buffers and stored a 16-cell line of each plane at a time (each wave would write 32 bytes
+    // of every 128-byte line, which L2 fetches before merging the four waves' parts)
+    constexpr int PL = 4 * V * n;  // planes [face][v][p] of fq and of ff
+    constexpr bool STAGEF = PF && !TEST && PL * 16 <= (kMfBlock<P> / 64) * S::CELLS;
+    double qbv[STAGEF ? V : 1], fbv[STAGEF ? V : 1];
     //   mkt = kxi(i, j): B of the volume integral's x part, A of its y part
     //   mlr / mbt: flc / frc (j) in columns i = 0, 1 / i = 2, 3 (B of the boundary extrapolation)
     double mkt = 0.0, mlr = 0.0, mbt = 0.0;
@@ -1092,12 +1099,64 @@ ADG_DEV void predictor_mf(const M &m, const StepBufs &b, const TestBufs &t, doub
             double d = 0.0 + mfma4(fxt, mkt) * wj / m.dx;
             d = d + mfma4(mkt, fh[1][v]) * wi / m.dy;
             if (P::kNcp) d = d + wi * wj * sh[v];
-            if (act) b.lduh[g * V * NN + V * tt + v] = d;
-            if (facelane) {
+            if constexpr (PF) {
+                if (act) ubuf[(((lane >> 2) & 3) * NN + tt) * V + v] = d;
+            } else {
+                if (act) b.lduh[g * V * NN + V * tt + v] = d;
+            }
+            if constexpr (STAGEF) {
+                qbv[v] = qb;
+                fbv[v] = fb;
+            } else if (facelane) {
                 const size_t o = ((size_t)(fc * V + v) * n + pt) * b.G + g;
                 b.fq[o] = qb;
                 b.ff[o] = fb;
             }
+        }
+    }
+    if constexpr (PF && !TEST) {
+        // the wave's lduh rows from LDS: consecutive lanes store consecutive 16-byte pairs, so
+        // every 128-byte line is written whole by one instruction (single 8-byte stores of each
+        // node would make L2 fetch every line before merging them).  A wave's LDS writes and
+        // reads complete in order; the barrier only keeps the compiler from reordering them.
+        __builtin_amdgcn_wave_barrier();
+        const int c0w = first + 4 * (tid >> 6), nv = m.ncell - c0w < 4 ? m.ncell - c0w : 4;
+        const int cnt = nv > 0 ? nv * V * NN : 0;
+        double *dst = b.lduh + ((size_t)cell0 + c0w) * V * NN;
+        if (((uintptr_t)dst & 15) == 0) {
+            const double2 *s2 = (const double2 *)ubuf;
+            double2 *d2 = (double2 *)dst;
+            for (int k = lane; k < cnt / 2; k += 64) d2[k] = s2[k];
+            if ((cnt & 1) && lane == 0) dst[cnt - 1] = ubuf[cnt - 1];
+        } else {
+            for (int k = lane; k < cnt; k += 64) dst[k] = ubuf[k];
+        }
+    }
+    if constexpr (STAGEF) {
+        __syncthreads();  // every wave has read its lduh rows: the entry's buffers take the planes
+        double *stage = ubuf - (tid >> 6) * S::CELLS;  // the four waves' buffers of this entry
+        const int nvw = m.ncell - first < 16 ? m.ncell - first : 16;
+#pragma unroll
+        for (int pass = 0; pass < 2; pass++) {
+            double *out = pass ? b.ff : b.fq;
+            if (facelane) {
+#pragma unroll
+                for (int v = 0; v < V; v++) stage[((fc * V + v) * n + pt) * 16 + lc] = pass ? fbv[v] : qbv[v];
+            }
+            __syncthreads();
+            for (int k = tid; k < PL * 8; k += kMfBlock<P>) {
+                const int plane = k >> 3, c2 = 2 * (k & 7);
+                if (c2 >= nvw) continue;
+                double *o = out + (size_t)plane * b.G + cell0 + first + c2;
+                const double a0 = stage[plane * 16 + c2], a1 = stage[plane * 16 + c2 + 1];
+                if (c2 + 1 < nvw && ((uintptr_t)o & 15) == 0) {
+                    *(double2 *)o = make_double2(a0, a1);
+                } else {
+                    o[0] = a0;
+                    if (c2 + 1 < nvw) o[1] = a1;
+                }
+            }
+            __syncthreads();  // the stage is read before it is rewritten (or prefetched into)
         }
     }
 }
@@ -1389,7 +1448,7 @@ __global__ __launch_bounds__(kMfBlock<P>) __attribute__((amdgpu_waves_per_eu(2))
     constexpr int WAVES = kMfBlock<P> / 64;
     // two buffers of each wave's four cells, alternating between the entries of the loop (a wave
     // only touches its own: no barrier in the loop), then the operator entries and dummy state
-    __shared__ double lds[2 * WAVES * S::CELLS + kSopSize<P>];
+    __shared__ __align__(16) double lds[2 * WAVES * S::CELLS + kSopSize<P>];
     double *sop = lds + 2 * WAVES * S::CELLS;
     {
         constexpr int n = S::n;

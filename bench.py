@@ -50,12 +50,13 @@ sys.path.insert(0, os.path.join(ROOT, "ader-dg_amd"))
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix) dense peak
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per launch of the dominant predictor (k_predictor_lean), from two rocprofv3 --pmc
+# HBM bytes per launch of the dominant predictor (k_predictor_mfma at N=3, k_predictor_lean at N=4), from two rocprofv3 --pmc
 # passes: read = 32/64/128 x TCC_EA0_RDREQ_{32B,64B,128B}_sum (FETCH_SIZE counts a 128-byte
 # request as 64 bytes on this GPU), write = WRITE_SIZE KiB x 1024; both checked against the 1 GiB
 # read and copy of --probe-only.  profiles/hbm_counter_calibration.md and profiles/hbm_traffic_*.md
 # hold the runs.
-PREDICTOR_TRAFFIC_BYTES = {"euler_vortex_512_n4": 789.5e6, "bn_ensemble_256_n3": 43.69e9}
+PREDICTOR_TRAFFIC_BYTES = {"euler_vortex_512_n4": 755.3e6, "euler_quadrant_1024_n3": 2148.1e6,
+                           "bn_ensemble_256_n3": 23.30e9}
 
 PLAIN = {
     # name: (ic, N, imax, jmax, xl, xr, bc)
