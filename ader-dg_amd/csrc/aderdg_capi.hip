@@ -302,7 +302,7 @@ int seed(aderdg_ctx *c)
 int step_parts(aderdg_ctx *c)
 {
     int r;
-    if ((r = launch(c, kPartRanges))) return r;
+    if (c->g.pde != kPdeAd && (r = launch(c, kPartRanges))) return r;  // no limiter, no ranges
     if ((r = launch(c, kPartPredictor))) return r;
     if ((r = launch(c, kPartFace))) return r;
     if ((r = launch(c, kPartUpdate))) return r;
@@ -575,7 +575,7 @@ int aderdg_space_time_predictor(aderdg_ctx *c)
 {
     if (!c) return fail(ADERDG_EINVAL, "aderdg_space_time_predictor: null context");
     if (!c->have_uh) return fail(ADERDG_ESTATE, "aderdg_space_time_predictor: no solution uploaded");
-    const int r = launch(c, kPartRanges);
+    const int r = c->g.pde == kPdeAd ? 0 : launch(c, kPartRanges);
     return r ? r : launch(c, kPartPredictor);
 }
 
