@@ -994,7 +994,8 @@ ADG_DEV void predictor_mf(const M &m, const StepBufs &b, const TestBufs &t, doub
                 }
             }
         } else {
-#pragma unroll 1
+            // two slices per trip: half the time line's rotation moves (BN N=3 15.2 -> 14.6 ms)
+#pragma unroll 2
             for (int l = 0; l < n; l++) {
                 double Q[V], F[2 * V], ax[V], ay[V], G[2 * V], rv[V];
 #pragma unroll
@@ -2039,9 +2040,11 @@ __global__ __launch_bounds__(kUpdBlock<P>) __attribute__((amdgpu_waves_per_eu(kU
     // each thread reads and writes its node's values directly.  ssuh then holds the new
     // nodal values [cell][v][node] that the range maps read.
     constexpr bool ROWS = kUpdRows<P>;
-    // FACES (ROWS with at most four cells per workgroup): the entry's face lines are staged
-    // in LDS, one lane per value, instead of each node loading its own row's values
-    constexpr bool FACES = ROWS && CPW <= 4;
+    // FACES (ROWS, i.e. Baer-Nunziato, N <= 3: at most seven cells per workgroup): the entry's
+    // face lines are staged in LDS, one lane per value, instead of each node loading its own
+    // row's values (at N=2 it measured slower until the rows streamed non-temporally: then
+    // 5.4 -> 5.1 ms on the ensemble's N=2 arena)
+    constexpr bool FACES = ROWS && CPW <= 7;
     __shared__ __align__(16) double ssuh[CPW][V * NN];
     __shared__ __align__(16) double srow[ROWS ? CPW : 1][ROWS ? V * NN : 1];
     __shared__ unsigned long long skey[CPW][KC][NP];
