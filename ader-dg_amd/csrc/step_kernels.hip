@@ -994,8 +994,10 @@ ADG_DEV void predictor_mf(const M &m, const StepBufs &b, const TestBufs &t, doub
                 }
             }
         } else {
-            // two slices per trip: half the time line's rotation moves (BN N=3 15.2 -> 14.6 ms)
-#pragma unroll 2
+            // two slices per trip where n is even: half the time line's rotation moves (BN N=3
+            // 15.2 -> 14.6 ms; with three slices the remainder loop spilled, N=2 11.9 -> 28 ms)
+            constexpr int UNR = n % 2 == 0 ? 2 : 1;
+#pragma unroll UNR
             for (int l = 0; l < n; l++) {
                 double Q[V], F[2 * V], ax[V], ay[V], G[2 * V], rv[V];
 #pragma unroll
