@@ -115,6 +115,9 @@ struct AdPDE {
 struct EulerPDE {
     static constexpr int V = 4;
     static constexpr bool kNcp = false;
+    // variables whose flux is identically zero / whose gradient the NCP source reads
+    static constexpr bool flux_zero(int) { return false; }
+    static constexpr bool grad_used(int) { return false; }
     // x- and y-momentum components (a reflective side negates the normal one)
     static constexpr unsigned kMomX = 1u << 1, kMomY = 1u << 2;
 
@@ -183,6 +186,10 @@ struct EulerPDE {
 struct BnPDE {
     static constexpr int V = 9;
     static constexpr bool kNcp = true;
+    // the volume fraction (variable 8) has no flux (F[8] = F[17] = 0), and it is the only
+    // variable whose gradient the NCP source reads (ncp below: G[8], G[V + 8])
+    static constexpr bool flux_zero(int v) { return v == 8; }
+    static constexpr bool grad_used(int v) { return v == 8; }
     // x- and y-momentum components of both phases
     static constexpr unsigned kMomX = (1u << 1) | (1u << 5), kMomY = (1u << 2) | (1u << 6);
 

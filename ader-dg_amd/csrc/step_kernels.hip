@@ -560,20 +560,25 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
     double fh[2][V], sh[V];
 
     // fluxes of state Q at this thread's node, exchanged with the cell's threads, and the
-    // slice's x / y flux contractions ax, ay (and, Baer-Nunziato, the gradients G)
+    // slice's x / y flux contractions ax, ay (and, Baer-Nunziato, the gradients G).  A
+    // variable whose flux is identically zero (Baer-Nunziato's volume fraction) is neither
+    // exchanged nor contracted: its contraction is a sum of zeros, +0, so it is set to +0,
+    // the same bits.  Only the gradients the NCP source reads are formed.
     auto slice_terms = [&](const double *Q, double *F, double *ax, double *ay, double *G) {
         P::flux(m, Q, F);
         if (own) {
 #pragma unroll
             for (int v = 0; v < V; v++) {
-                if (!DPPX) buf[v * NN + j * n + i] = F[v];
-                buf[V * NN + v * NN + i * n + j] = F[V + v];
-                if (P::kNcp) buf[2 * V * NN + v * NN + tt] = Q[v];
+                if (!P::flux_zero(v)) {
+                    if (!DPPX) buf[v * NN + j * n + i] = F[v];
+                    buf[V * NN + v * NN + i * n + j] = F[V + v];
+                }
+                if (P::kNcp && P::grad_used(v)) buf[2 * V * NN + v * NN + tt] = Q[v];
             }
         }
         if constexpr (DPPX) {
 #pragma unroll
-            for (int v = 0; v < V; v++) ax[v] = quad_dot4(F[v], kx);
+            for (int v = 0; v < V; v++) ax[v] = P::flux_zero(v) ? 0.0 : quad_dot4(F[v], kx);
         }
         __syncthreads();
 #pragma unroll
@@ -581,14 +586,15 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
             double acc = 0.0;
             if constexpr (!DPPX) {
 #pragma unroll
-                for (int k = 0; k < n; k++) acc = acc + buf[v * NN + j * n + k] * opkx(k);
+                for (int k = 0; k < n && !P::flux_zero(v); k++) acc = acc + buf[v * NN + j * n + k] * opkx(k);
                 ax[v] = acc;
             }
             acc = 0.0;
 #pragma unroll
-            for (int k = 0; k < n; k++) acc = acc + buf[V * NN + v * NN + i * n + k] * opky(k);
+            for (int k = 0; k < n && !P::flux_zero(v); k++) acc = acc + buf[V * NN + v * NN + i * n + k] * opky(k);
             ay[v] = acc;
-            if (P::kNcp) {  // gradients, Baer-Nunziato/space_time_predictor.f90:72,79
+            G[v] = G[V + v] = 0.0;
+            if (P::kNcp && P::grad_used(v)) {  // gradients, Baer-Nunziato/space_time_predictor.f90:72,79
                 acc = 0.0;
                 if constexpr (DPPX) {
                     acc = quad_dot4(Q[v], gxr);
@@ -654,14 +660,16 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
                 if (own) {
 #pragma unroll
                     for (int v = 0; v < V; v++) {
-                        if (!DPPX) buf[v * NN + j * n + i] = F[v];
-                        buf[V * NN + v * NN + i * n + j] = F[V + v];
-                        if (P::kNcp) buf[2 * V * NN + v * NN + tt] = Q[v];
+                        if (!P::flux_zero(v)) {
+                            if (!DPPX) buf[v * NN + j * n + i] = F[v];
+                            buf[V * NN + v * NN + i * n + j] = F[V + v];
+                        }
+                        if (P::kNcp && P::grad_used(v)) buf[2 * V * NN + v * NN + tt] = Q[v];
                     }
                 }
                 if constexpr (DPPX) {
 #pragma unroll
-                    for (int v = 0; v < V; v++) axd[v] = quad_dot4(F[v], kx);
+                    for (int v = 0; v < V; v++) axd[v] = P::flux_zero(v) ? 0.0 : quad_dot4(F[v], kx);
                 }
                 if (LAST) {  // time average of the last sweep's fluxes (:92)
                     const double wl = op->w[l];
@@ -684,14 +692,15 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
                     acc = axd[v];
                 } else {
 #pragma unroll
-                    for (int k = 0; k < n; k++) acc = acc + buf[v * NN + j * n + k] * opkx(k);
+                    for (int k = 0; k < n && !P::flux_zero(v); k++) acc = acc + buf[v * NN + j * n + k] * opkx(k);
                 }
                 const double rr = c0 * sU[v * NN + tt] - sx * acc;  // :66
                 acc = 0.0;
 #pragma unroll
-                for (int k = 0; k < n; k++) acc = acc + buf[V * NN + v * NN + i * n + k] * opky(k);
+                for (int k = 0; k < n && !P::flux_zero(v); k++) acc = acc + buf[V * NN + v * NN + i * n + k] * opky(k);
                 rv[v] = rr - sy * acc;  // :72
-                if (P::kNcp) {          // gradients, Baer-Nunziato/space_time_predictor.f90:72,79
+                G[v] = G[V + v] = 0.0;
+                if (P::kNcp && P::grad_used(v)) {  // gradients, Baer-Nunziato/space_time_predictor.f90:72,79
                     acc = 0.0;
                     if constexpr (DPPX) {
                         acc = quad_dot4(Q[v], gxr);
@@ -952,9 +961,10 @@ ADG_DEV void predictor_mf(const M &m, const StepBufs &b, const TestBufs &t, doub
         P::flux(m, Q, F);
 #pragma unroll
         for (int v = 0; v < V; v++) {
-            ax[v] = mfma4(lane_get(F[v], xaddr), mk);
-            ay[v] = mfma4(mk, F[V + v]);
-            if (P::kNcp) {
+            ax[v] = P::flux_zero(v) ? 0.0 : mfma4(lane_get(F[v], xaddr), mk);
+            ay[v] = P::flux_zero(v) ? 0.0 : mfma4(mk, F[V + v]);
+            G[v] = G[V + v] = 0.0;
+            if (P::kNcp && P::grad_used(v)) {
                 G[v] = rdx * mfma4(lane_get(Q[v], xaddr), md);
                 G[V + v] = rdy * mfma4(md, Q[v]);
             }
