@@ -10,9 +10,9 @@ held to the parity bar of the product build rather than to bit equality:
   (Euler/space_time_predictor.f90, Baer-Nunziato/space_time_predictor.f90) on random
   admissible cells: every output within 1e-12 relative L2 of the lean variant 4 of the
   same build, and within the product build's envelope of the oracle (1e-12, or 30
-  times the distance of the oracle's FMA build, where random rough cells amplify
-  rounding: the lean variant of the product build sits 1.6e-11 from the oracle on one
-  of the Euler cells);
+  times the distance of the oracle's FMA build: random rough cells amplify rounding,
+  so that a build with FMA contraction, the oracle's FMA build included, can sit above
+  1e-12 from the oracle on them);
 * whole steps of the reference loop (Euler/main.f90:24-100,
   Baer-Nunziato/main.f90:24-100) with the limiter active: the product-build long runs
   of test_gpu_product_long at N=2 and 3 with the variant forced (DOF and cell-average
