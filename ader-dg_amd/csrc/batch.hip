@@ -230,7 +230,7 @@ int Batch::build_graph()
         hipStream_t s = k == 0 ? stream : branch[k - 1];
         if (e == hipSuccess) e = launch_dt(g.b, s, 0.0, 0.0, 0, g.dt_grid());
         for (int p : parts)
-            if (e == hipSuccess) e = g.launch_on(p, s);
+            if (e == hipSuccess) e = g.launch_on(p, s, groups.size() > 1);
     }
     for (size_t k = 1; k < groups.size() && e == hipSuccess; k++) {
         e = hipEventRecord(join[k - 1], branch[k - 1]);
@@ -268,7 +268,8 @@ int Batch::step(int nsteps)
 {
     if (nsteps < 0) return fail(ADERDG_EINVAL, "aderdg_batch_step: negative step count");
     if (groups.empty()) return fail(ADERDG_ESTATE, "aderdg_batch_step: nothing reserved");
-    if (timing) {  // the graph's launches one by one, each between two events
+    if (timing) {  // the graph's launches one by one, each between two events (each alone on the
+                   // GPU, so the matrix-core predictor takes its full grid, not the shared one)
         // timing slots of include/aderdg.h: dt, predictor, update, spread, subcell, face, ranges; -1 untimed
         static const int parts[] = {kPartDt, kPartRanges, kPartPredictor, kPartFace, kPartUpdate, kPartSpread,
                                     kPartSubcell, kPartAdvance};

@@ -202,7 +202,8 @@ int Group::init(int pde_, int N_, const GroupCaps &c_in, const Operators &ops_, 
     const int64_t items[kNumTabs] = {c.cells, c.cells, (c.xfaces + c.yfaces) * n, c.cells};
     int64_t max_entries = 0;
     // a persistent grid never exceeds the workgroups of its kernel that fit on the CUs at once
-    int cus = 0, dev = 0;
+    int dev = 0;
+    cus = 0;
     if (persist_blocks > 0 && (hipGetDevice(&dev) != hipSuccess ||
                                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess))
         cus = 0;
@@ -446,8 +447,14 @@ hipError_t Group::launch(int part) const
     return launch_part(pde, N, variant, upd, b, st_, part, grid, caps.slots);
 }
 
-hipError_t Group::launch_on(int part, hipStream_t s) const
+hipError_t Group::launch_on(int part, hipStream_t s, bool shared) const
 {
+    if (shared && part == kPartPredictor && variant == kPredMfma && cus > 0 && grid[kTabPred] > cus) {
+        int g[kNumTabs];
+        for (int k = 0; k < kNumTabs; k++) g[k] = grid[k];
+        g[kTabPred] = cus;
+        return launch_part(pde, N, variant, upd, b, s, part, g, caps.slots);
+    }
     return launch_part(pde, N, variant, upd, b, s, part, grid, caps.slots);
 }
 

@@ -76,6 +76,7 @@ public:
     int upd = kUpdShared;     // update kernel variant
     int quanta[kNumTabs] = {};
     int grid[kNumTabs] = {};
+    int cus = 0;  // compute units of the device (0: unknown)
     int64_t tab_cap[kNumTabs] = {};
     StepBufs b{};
 
@@ -109,7 +110,12 @@ public:
     hipStream_t stream() const { return st_; }
 
     hipError_t launch(int part) const;
-    hipError_t launch_on(int part, hipStream_t s) const;  // the same launch on another stream
+    // The same launch on another stream.  shared: the launch is one branch of a graph that
+    // steps several arenas side by side; the matrix-core predictor then runs one workgroup
+    // per CU (one wave per SIMD) instead of filling the CUs, so that the other arenas'
+    // passes run beside it on every CU (at one wave per SIMD it keeps about 80% of its rate:
+    // the Baer-Nunziato ensemble step 47.6 -> 46.4 ms)
+    hipError_t launch_on(int part, hipStream_t s, bool shared = false) const;
     int dt_grid() const { return grid[kTabCell] < kDtGrid ? grid[kTabCell] : kDtGrid; }
     // Bytes of device memory per cell / x-face / y-face / solve slot / limited-cell pool
     // slot of a (PDE, N).

@@ -586,6 +586,9 @@ def run_ensemble(args, specs):
     kname = predictor_name("bn", dom, True)
     roof = roofline_of(fl_pred, per_group[dom]["cells"], per_group[dom]["predictor"], kname,
                        PREDICTOR_TRAFFIC_BYTES.get("%s_n%d" % (args.workload, dom)), skipped_for(kname, "bn", dom))
+    if len(per_group) > 1 and kname.startswith("k_predictor_mfma"):
+        roof["note"] += ("; timed launched alone at full residency (two workgroups per CU); in the step graph, "
+                         "beside the other arena's branch, it runs one workgroup per CU")
     out = {
         "metric": "element-DOF updates/s batched vs plain; p50/p99 solve latency at fixed QPS",
         "value": value, "unit": "element-DOF updates/s", "n_gpus": 1, "steps": K, "warmup": W,
