@@ -115,9 +115,11 @@ struct AdPDE {
 struct EulerPDE {
     static constexpr int V = 4;
     static constexpr bool kNcp = false;
-    // variables whose flux is identically zero / whose gradient the NCP source reads
+    // variables whose flux is identically zero / whose gradient the NCP source reads / whose
+    // NCP source is identically -0 (none: no NCP)
     static constexpr bool flux_zero(int) { return false; }
     static constexpr bool grad_used(int) { return false; }
+    static constexpr bool src_zero(int) { return false; }
     // x- and y-momentum components (a reflective side negates the normal one)
     static constexpr unsigned kMomX = 1u << 1, kMomY = 1u << 2;
 
@@ -190,6 +192,9 @@ struct BnPDE {
     // variable whose gradient the NCP source reads (ncp below: G[8], G[V + 8])
     static constexpr bool flux_zero(int v) { return v == 8; }
     static constexpr bool grad_used(int v) { return v == 8; }
+    // the NCP source of the two densities is -0 (ncp below: src[0] = src[4] = -0.0); adding
+    // f * (-0) with f >= 0 leaves any value unchanged, so the predictors skip those terms
+    static constexpr bool src_zero(int v) { return v == 0 || v == 4; }
     // x- and y-momentum components of both phases
     static constexpr unsigned kMomX = (1u << 1) | (1u << 5), kMomY = (1u << 2) | (1u << 6);
 

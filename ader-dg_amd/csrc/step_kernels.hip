@@ -644,7 +644,7 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
                 for (int v = 0; v < V; v++) {
                     const double rr = c0 * sU[v * NN + tt] - sx * ax0[v];  // :66
                     double r = rr - sy * ay0[v];                           // :72
-                    if (P::kNcp) r = r + f * src0[v];
+                    if (P::kNcp && !P::src_zero(v)) r = r + f * src0[v];
                     q[v][l] = r;
                 }
             }
@@ -675,8 +675,8 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
                     const double wl = op->w[l];
 #pragma unroll
                     for (int v = 0; v < V; v++) {
-                        fh[0][v] = fh[0][v] + F[v] * wl;
-                        fh[1][v] = fh[1][v] + F[V + v] * wl;
+                        if (!P::flux_zero(v)) fh[0][v] = fh[0][v] + F[v] * wl;
+                        if (!P::flux_zero(v)) fh[1][v] = fh[1][v] + F[V + v] * wl;
                     }
                 }
             }
@@ -722,8 +722,8 @@ ADG_DEV void predictor_lean(const M &m, const StepBufs &b, const TestBufs &t, do
                 const double f = wi * wj * wl * dt;
 #pragma unroll
                 for (int v = 0; v < V; v++) {
-                    rv[v] = rv[v] + f * src[v];
-                    if (LAST) sh[v] = sh[v] + src[v] * wl;  // lShi (:111)
+                    if (!P::src_zero(v)) rv[v] = rv[v] + f * src[v];
+                    if (LAST && !P::src_zero(v)) sh[v] = sh[v] + src[v] * wl;  // lShi (:111)
                 }
             }
             // slice l is done: its right-hand side takes its place, and the time line rotates
@@ -999,7 +999,7 @@ ADG_DEV void predictor_mf(const M &m, const StepBufs &b, const TestBufs &t, doub
 #pragma unroll
                 for (int v = 0; v < V; v++) {
                     double r = (c0 * sU[SS * v] - sx * ax[v]) - sy * ay[v];  // :66, :72
-                    if (P::kNcp) r = r + f * src[v];
+                    if (P::kNcp && !P::src_zero(v)) r = r + f * src[v];
                     q[v][l] = r;
                 }
             }
@@ -1017,8 +1017,8 @@ ADG_DEV void predictor_mf(const M &m, const StepBufs &b, const TestBufs &t, doub
                 if (LAST) {  // time average of the last sweep's fluxes (:92)
 #pragma unroll
                     for (int v = 0; v < V; v++) {
-                        fh[0][v] = fh[0][v] + F[v] * wl;
-                        fh[1][v] = fh[1][v] + F[V + v] * wl;
+                        if (!P::flux_zero(v)) fh[0][v] = fh[0][v] + F[v] * wl;
+                        if (!P::flux_zero(v)) fh[1][v] = fh[1][v] + F[V + v] * wl;
                     }
                 }
                 const double sx = div_inv(wl * wj * tx, m.dx, rdx), sy = div_inv(wl * wi * tx, m.dy, rdy);
@@ -1031,8 +1031,8 @@ ADG_DEV void predictor_mf(const M &m, const StepBufs &b, const TestBufs &t, doub
                     const double f = wi * wj * wl * tx;
 #pragma unroll
                     for (int v = 0; v < V; v++) {
-                        rv[v] = rv[v] + f * src[v];
-                        if (LAST) sh[v] = sh[v] + src[v] * wl;  // lShi (:111)
+                        if (!P::src_zero(v)) rv[v] = rv[v] + f * src[v];
+                        if (LAST && !P::src_zero(v)) sh[v] = sh[v] + src[v] * wl;  // lShi (:111)
                     }
                 }
                 // slice l is done: its right-hand side takes its place, the time line rotates
