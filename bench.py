@@ -56,7 +56,7 @@ HBM_PEAK_GBS = 8000.0
 # read and copy of --probe-only.  profiles/hbm_counter_calibration.md and profiles/hbm_traffic_*.md
 # hold the runs.
 PREDICTOR_TRAFFIC_BYTES = {"euler_vortex_512_n4": 755.3e6, "euler_quadrant_1024_n3": 2148.1e6,
-                           "bn_ensemble_256_n3": 23.30e9}
+                           "bn_ensemble_256_n3": 21.24e9}
 
 PLAIN = {
     # name: (ic, N, imax, jmax, xl, xr, bc)
