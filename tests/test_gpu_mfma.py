@@ -19,7 +19,10 @@ held to the parity bar of the product build rather than to bit equality:
   relative L2 <= 1e-12 each step, dt within 1e-12, flags identical away from the DMP
   threshold);
 * a batch arena on a persistent grid of three workgroups (hundreds of rounds per
-  launch) against the same solves stepped alone with the variant forced.
+  launch) against the same solves stepped alone with the variant forced;
+* a two-arena batch on the default grid, whose step graph launches the matrix-core
+  predictor at one workgroup per CU beside the other arena's branch, against the same
+  solves stepped alone.
 """
 import numpy as np
 import pytest
@@ -163,3 +166,54 @@ def test_mfma_persistent_batch():
             np.testing.assert_array_equal(la[key], lb[key], err_msg="%s %s" % (c["pde"], key))
         s.close()
     b.close()
+
+
+def test_mfma_shared_graph_batch():
+    """Baer-Nunziato arenas at N=3 (matrix-core predictor, more work entries than CUs) and N=2
+    in one batch on the default grid: the step graph runs one branch per arena and launches
+    the matrix-core predictor at one workgroup per CU (the timed kernel-by-kernel path of the
+    same batch launches it at its full grid).  Each solve equals the same solve stepped alone,
+    bit for bit: no result depends on the grid that takes an entry."""
+    aderdg.tune_force("bn", 3, MFMA, exact=False)
+    aderdg.tune_force("bn", 2, 4, exact=False)
+    for N in (2, 3):
+        aderdg.tune_force("bn", N, 0, exact=False, kernel="update")
+    x3, x2 = aderdg.operators(3)["xi"], aderdg.operators(2)["xi"]
+    cases = [
+        dict(N=3, I=96, J=80, steps=3,
+             uh=ics.initial_uh_bn(x3, 96, 80, (0.0, 0.0), (1.0, 1.0), BN_EOS_STIFF, radius=0.25, pratio=5.0e3,
+                                  phi_eps=ics.BN_PHI_STABLE)),
+        dict(N=3, I=64, J=72, steps=2,
+             uh=ics.initial_uh_bn(x3, 64, 72, (0.0, 0.0), (1.0, 1.0), BN_EOS_STIFF, radius=0.3, pratio=2.0e3)),
+        dict(N=2, I=80, J=64, steps=3,
+             uh=ics.initial_uh_bn(x2, 80, 64, (0.0, 0.0), (1.0, 1.0), BN_EOS_STIFF, radius=0.2, pratio=4.0e3)),
+    ]
+    # 12,288 cells at N=3: 768 entries of 16 cells, more than the 256 CUs of an MI355X
+    assert sum(c["I"] * c["J"] for c in cases if c["N"] == 3) // 16 > 256
+    for timed in (False, True):
+        b = aderdg.Batch(exact=False)
+        for N in (3, 2):
+            b.reserve("bn", N, max_cells=sum(c["I"] * c["J"] for c in cases if c["N"] == N), max_solves=2)
+        ids = [b.add(c["uh"], c["N"], c["I"], c["J"], (0.0, 0.0), (1.0, 1.0), bc="transmissive", pde="bn",
+                     eos=BN_EOS_STIFF, max_steps=c["steps"]) for c in cases]
+        b.timing(timed)
+        b.step(max(c["steps"] for c in cases) + 1)
+        b.sync()
+        for i, c in zip(ids, cases):
+            s = aderdg.Solver(c["N"], c["I"], c["J"], (0.0, 0.0), (1.0, 1.0), "transmissive", exact=False, pde="bn",
+                              eos=BN_EOS_STIFF)
+            s.upload(c["uh"])
+            t = 0.0
+            for _ in range(c["steps"]):
+                dt, _ = s.step(t, 1e9)
+                t += dt
+            q = b.query(i)
+            log("%s batch N=%d %dx%d: %d steps, t=%.6g" % ("timed" if timed else "graph", c["N"], c["I"], c["J"],
+                                                          q["steps"], q["time"]))
+            assert q["steps"] == c["steps"] and q["time"] == t, (q, t)
+            np.testing.assert_array_equal(b.download(i), s.download(), err_msg="N=%d uh" % c["N"])
+            la, lb = b.limiter_state(i), s.get_limiter()
+            for key in ("status", "recompute", "Lh"):
+                np.testing.assert_array_equal(la[key], lb[key], err_msg="N=%d %s" % (c["N"], key))
+            s.close()
+        b.close()
