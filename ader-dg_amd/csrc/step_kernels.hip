@@ -1695,8 +1695,8 @@ __global__ __launch_bounds__(kBlock) void k_face(StepBufs b)
 
 // First passes of get_subcell_data / get_lobatto_data for one cell held in LDS (uhL [v][NN])
 template <class P, int N>
-ADG_DEV void cell_ranges_stage1(OpPtr op, const double *uhL, double *tyL, double *byL, int tt,
-                                int nthr)
+ADG_DEV void cell_ranges_stage1(const double *u2l, const double *u2b, const double *uhL, double *tyL, double *byL,
+                                int tt, int nthr)
 {
     using S = Shape<P, N>;
     constexpr int n = S::n, NN = S::NN, s = S::s, V = S::V;
@@ -1706,7 +1706,7 @@ ADG_DEV void cell_ranges_stage1(OpPtr op, const double *uhL, double *tyL, double
         for (int v = 0; v < V; v++) {
             double acc = 0.0;
 #pragma unroll
-            for (int bb = 0; bb < n; bb++) acc = acc + uhL[v * NN + a + n * bb] * op->u2l[qq + s * bb];
+            for (int bb = 0; bb < n; bb++) acc = acc + uhL[v * NN + a + n * bb] * u2l[qq + s * bb];
             tyL[v * n * s + a + n * qq] = acc;
         }
     }
@@ -1716,7 +1716,7 @@ ADG_DEV void cell_ranges_stage1(OpPtr op, const double *uhL, double *tyL, double
         for (int v = 0; v < V; v++) {
             double acc = 0.0;
 #pragma unroll
-            for (int bb = 0; bb < n; bb++) acc = acc + uhL[v * NN + a + n * bb] * op->u2b[qq + n * bb];
+            for (int bb = 0; bb < n; bb++) acc = acc + uhL[v * NN + a + n * bb] * u2b[qq + n * bb];
             byL[v * NN + a + n * qq] = acc;
         }
     }
@@ -1724,7 +1724,7 @@ ADG_DEV void cell_ranges_stage1(OpPtr op, const double *uhL, double *tyL, double
 
 // Second passes; returns true if a subcell average is inadmissible (Euler/limiter.f90:227-241)
 template <class P, int N, class M>
-ADG_DEV bool cell_ranges_stage2(OpPtr op, const M &m, const double *tyL,
+ADG_DEV bool cell_ranges_stage2(const double *u2l, const double *u2b, const M &m, const double *tyL,
                                 const double *byL, double *limL, double *lobL, int tt, int nthr)
 {
     using S = Shape<P, N>;
@@ -1737,7 +1737,7 @@ ADG_DEV bool cell_ranges_stage2(OpPtr op, const M &m, const double *tyL,
         for (int v = 0; v < V; v++) {
             double acc = 0.0;
 #pragma unroll
-            for (int a = 0; a < n; a++) acc = acc + tyL[v * n * s + a + n * qq] * op->u2l[p + s * a];
+            for (int a = 0; a < n; a++) acc = acc + tyL[v * n * s + a + n * qq] * u2l[p + s * a];
             limL[v * SS + task] = acc;
             L[v] = acc;
         }
@@ -1749,7 +1749,7 @@ ADG_DEV bool cell_ranges_stage2(OpPtr op, const M &m, const double *tyL,
         for (int v = 0; v < V; v++) {
             double acc = 0.0;
 #pragma unroll
-            for (int a = 0; a < n; a++) acc = acc + byL[v * NN + a + n * qq] * op->u2b[p + n * a];
+            for (int a = 0; a < n; a++) acc = acc + byL[v * NN + a + n * qq] * u2b[p + n * a];
             lobL[v * NN + task] = acc;
         }
     }
@@ -1896,7 +1896,7 @@ __global__ __launch_bounds__(kUpdBlock<P>) void k_update_shared(StepBufs b)
     }
     const double dtc = act ? node_dt<P>(m, un) : __builtin_nan("");
     __syncthreads();
-    if (own) cell_ranges_stage1<P, N>(op, suh, sty, sby, tt, NN);
+    if (own) cell_ranges_stage1<P, N>(op->u2l, op->u2b, suh, sty, sby, tt, NN);
     __syncthreads();
     // second passes folded into per-thread (min, max) of Lobatto and subcell values
     double b0[V], l0[V], l1[V];
@@ -2509,20 +2509,47 @@ __global__ __launch_bounds__(kBlock) void k_spread(StepBufs b)
     }
 }
 
+// The operator tables the subcell passes index per lane (u2l: subcell averages from nodal
+// values, l2u: nodal values from subcell averages, u2b: Lobatto values), copied to LDS once per
+// workgroup: read from global memory, each of the dozen barrier-separated passes of a cell
+// waited for them (Baer-Nunziato N=3 subcell recompute: a third of a cell's time)
+struct SubOps {
+    const double *u2l, *l2u, *u2b;
+};
+template <int N>
+struct SubOpLds {
+    static constexpr int n = N + 1, s = 2 * N + 1;
+    double u2l[s * n], l2u[n * s], u2b[n * n];
+};
+template <int N>
+ADG_DEV SubOps stage_subops(OpPtr op, SubOpLds<N> &d, int lane, int nthr)
+{
+    constexpr int n = N + 1, s = 2 * N + 1;
+    for (int k = lane; k < s * n; k += nthr) {
+        d.u2l[k] = op->u2l[k];
+        d.l2u[k] = op->l2u[k];
+    }
+    for (int k = lane; k < n * n; k += nthr) d.u2b[k] = op->u2b[k];
+    __syncthreads();
+    return SubOps{d.u2l, d.l2u, d.u2b};
+}
+
 // Common tail of both subcell kernels: put_subcell_data (Euler/limiter.f90:86-96),
 // re-DMP of cells marked 2 (:619-638), next-step limiter seeds and dt candidate.
 // NT: threads of the workgroup (one cell per workgroup); sNth: LDS word that broadcasts the
 // pool slot count of a limited cell to the workgroup
 template <class P, int N, int NT, class M>
 ADG_DEV void subcell_finish(const M &m, const StepBufs &b, size_t cell, int sol, int lh0, int lhcap, int par, int flag,
-                            const double *sSub,
-                            double *sUhy, double *sUh, double *sTy, double *sBy, double *sLim, double *sLob, double *sDt,
-                            double *sRng, int *sFlag, int *sNth)
+                            const double *sSub, double *sUh, double *sLim, double *sLob, double *sDt, double *sRng,
+                            int *sFlag, int *sNth, const SubOps &o)
 {
     using S = Shape<P, N>;
     constexpr int n = S::n, NN = S::NN, s = S::s, SS = S::SS, V = S::V;
-    const OpPtr op = (OpPtr)b.op;
     const int lane = threadIdx.x;
+    // a cell marked 1 is limited whatever the re-DMP finds: its pool slot is taken now, so
+    // that the atomic's round trip overlaps the passes below instead of ending the cell
+    int nth1 = 0;
+    if (flag == 1 && lane == 0) nth1 = atomicAdd(b.lh_count + sol, 1);
     // this step's DMP bounds of the cell, loaded now so their latency overlaps the maps
     constexpr int KV = (V + NT - 1) / NT;
     double pmn[KV], pmx[KV];
@@ -2532,51 +2559,104 @@ ADG_DEV void subcell_finish(const M &m, const StepBufs &b, size_t cell, int sol,
         pmn[k] = v < V ? b.lmin[(size_t)v * b.G + cell] : 0.0;
         pmx[k] = v < V ? b.lmax[(size_t)v * b.G + cell] : 0.0;
     }
-    for (int task = lane; task < s * n; task += NT) {
-        const int p = task % s, bb = task / s;
+    // Each pair of separable passes below runs as one pass per output point, which forms the
+    // first pass's sums it needs itself (the same sums in the same order, so the same bits):
+    // more multiplies, but one barrier and one LDS round trip fewer per pair.
+    // put_subcell_data (Euler/limiter.f90:86-96): the nodal values of the new subcell averages,
+    // one (node, variable) per thread
+    constexpr int PU = N >= 4 ? 1 : s;  // outer loops of the fused passes: unrolled, they spill at N = 4
+    for (int task = lane; task < V * NN; task += NT) {
+        const int v = task / NN, node = task - v * NN, a = node % n, bb = node / n;
+        double acc = 0.0;
+#pragma unroll PU
+        for (int p = 0; p < s; p++) {
+            double y = 0.0;  // :86-90, the first pass at (p, bb)
 #pragma unroll
-        for (int v = 0; v < V; v++) {
-            double acc = 0.0;
-#pragma unroll
-            for (int qq = 0; qq < s; qq++) acc = acc + sSub[v * SS + p + s * qq] * op->l2u[bb + n * qq];
-            sUhy[v * s * n + p + s * bb] = acc;
+            for (int qq = 0; qq < s; qq++) y = y + sSub[v * SS + p + s * qq] * o.l2u[bb + n * qq];
+            acc = acc + y * o.l2u[a + n * p];
         }
-    }
-    __syncthreads();
-    for (int task = lane; task < NN; task += NT) {
-        const int a = task % n, bb = task / n;
-        double un[V];
-#pragma unroll
-        for (int v = 0; v < V; v++) {
-            double acc = 0.0;
-#pragma unroll
-            for (int p = 0; p < s; p++) acc = acc + sUhy[v * s * n + p + s * bb] * op->l2u[a + n * p];
-            un[v] = acc;
-            sUh[v * NN + task] = acc;
-            b.uh[par ^ 1][cell * V * NN + V * task + v] = acc;
-        }
-        sDt[task] = node_dt<P>(m, un);
+        sUh[v * NN + node] = acc;
+        b.uh[par ^ 1][cell * V * NN + V * node + v] = acc;
     }
     if (lane == 0) *sFlag = 0;
     __syncthreads();
-    cell_ranges_stage1<P, N>(op, sUh, sTy, sBy, lane, NT);
-    __syncthreads();
-    const bool bad = cell_ranges_stage2<P, N>(op, m, sTy, sBy, sLim, sLob, lane, NT);
+    // the new cell's subcell data (get_subcell_data :25-35, positivity :227-241) and Lobatto
+    // data (get_lobatto_data :56-66), one point per thread, and its nodes' time-step candidates
+    bool bad = false;
+    for (int task = lane; task < SS + 2 * NN; task += NT) {
+        if (task < SS) {
+            const int p = task % s, qq = task / s;
+            double L[V];
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                double acc = 0.0;
+#pragma unroll PU
+                for (int a = 0; a < n; a++) {
+                    double ty = 0.0;  // limy(a, qq), :25-27
+#pragma unroll
+                    for (int bb = 0; bb < n; bb++) ty = ty + sUh[v * NN + a + n * bb] * o.u2l[qq + s * bb];
+                    acc = acc + ty * o.u2l[p + s * a];
+                }
+                sLim[v * SS + task] = acc;
+                L[v] = acc;
+            }
+            bad = bad || P::inadmissible(m, L);
+        } else if (task < SS + NN) {
+            const int t = task - SS, p = t % n, qq = t / n;
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                double acc = 0.0;
+#pragma unroll PU
+                for (int a = 0; a < n; a++) {
+                    double by = 0.0;  // loby(a, qq), :56-58
+#pragma unroll
+                    for (int bb = 0; bb < n; bb++) by = by + sUh[v * NN + a + n * bb] * o.u2b[qq + n * bb];
+                    acc = acc + by * o.u2b[p + n * a];
+                }
+                sLob[v * NN + t] = acc;
+            }
+        } else {
+            const int node = task - SS - NN;
+            double un[V];
+#pragma unroll
+            for (int v = 0; v < V; v++) un[v] = sUh[v * NN + node];
+            sDt[node] = node_dt<P>(m, un);
+        }
+    }
     if (bad) atomicOr(sFlag, 1);
     __syncthreads();
     // every range the rest needs, one fold per thread (each in the order of group_ranges):
     // sRng[kind * V + v], kind 0/1 nodal min/max, 2/3 Lobatto, 4/5 subcell, 6/7 the new
     // subcell averages (a limited cell's next ranges, :135-139); sRng[8V] the time-step minimum
+    // (the folds are unrolled over the compile-time counts, so that a thread's LDS reads are
+    // issued ahead of its fold chain instead of one round trip per element)
     for (int t = lane; t <= 8 * V; t += NT) {
         double r = __builtin_nan("");
         if (t == 8 * V) {
+#pragma unroll
             for (int k = 0; k < NN; k++) r = fold_min(r, sDt[k]);
         } else {
             const int kind = t / V, v = t - kind * V, grp = kind >> 1;
             const bool mx = kind & 1;
-            const double *src = grp == 0 ? sUh + v * NN : (grp == 1 ? sLob + v * NN : (grp == 2 ? sLim + v * SS : sSub + v * SS));
-            const int cnt = grp < 2 ? NN : SS;
-            for (int k = 0; k < cnt; k++) r = mx ? fold_max(r, src[k]) : fold_min(r, src[k]);
+            if (grp < 2) {
+                const double *src = grp == 0 ? sUh + v * NN : sLob + v * NN;
+                double x[NN];
+#pragma unroll
+                for (int k = 0; k < NN; k++) x[k] = src[k];
+#pragma unroll
+                for (int k = 0; k < NN; k++) r = mx ? fold_max(r, x[k]) : fold_min(r, x[k]);
+            } else {
+                const double *src = grp == 2 ? sLim + v * SS : sSub + v * SS;
+                constexpr int C = SS < 25 ? SS : 25;  // elements loaded ahead of their folds
+#pragma unroll
+                for (int k0 = 0; k0 < SS; k0 += C) {
+                    double x[C];
+#pragma unroll
+                    for (int k = 0; k < C; k++) x[k] = k0 + k < SS ? src[k0 + k] : 0.0;
+#pragma unroll
+                    for (int k = 0; k < C && k0 + k < SS; k++) r = mx ? fold_max(r, x[k]) : fold_min(r, x[k]);
+                }
+            }
         }
         sRng[t] = r;
     }
@@ -2607,7 +2687,7 @@ ADG_DEV void subcell_finish(const M &m, const StepBufs &b, size_t cell, int sol,
         b.status[par ^ 1][cell] = status;
     }
     if (status) {  // NewLh in a pool slot of the solve, and Lh = NewLh at update_limiter (:287)
-        if (lane == 0) *sNth = atomicAdd(b.lh_count + sol, 1);
+        if (lane == 0) *sNth = flag == 1 ? nth1 : atomicAdd(b.lh_count + sol, 1);
         __syncthreads();  // status is the same in every thread of the workgroup
         const int nth = *sNth;
         const int ls = nth < lhcap ? lh0 + nth : -1;  // none left: the solve ends failed after this step
@@ -2637,14 +2717,15 @@ __global__ __launch_bounds__(64) void k_subcell(StepBufs b)
     __shared__ double sFx[V * (s + 1) * s];
     __shared__ double sFy[V * s * (s + 1)];
     __shared__ double sSub[V * SS];
-    __shared__ double sUhy[V * s * n];
     __shared__ double sUh[V * NN];
-    __shared__ double sTy[V * n * s], sBy[V * NN], sLim[V * SS], sLob[V * NN];
+    __shared__ double sLim[V * SS], sLob[V * NN];
     __shared__ double sDt[NN];
     __shared__ double sRng[8 * V + 1];
     __shared__ int sFlag, sNth;
     const OpPtr op = (OpPtr)b.op;
     const int lane = threadIdx.x;
+    __shared__ SubOpLds<N> sOps;
+    const SubOps o = stage_subops<N>(op, sOps, lane, 64);
     const int count = b.counters[kCntList];
     for (int item = blockIdx.x; item < count; item += gridDim.x) {
         const int gcell = b.list[item];
@@ -2676,7 +2757,7 @@ __global__ __launch_bounds__(64) void k_subcell(StepBufs b)
             for (int v = 0; v < V; v++) {
                 double acc = 0.0;
 #pragma unroll
-                for (int bb = 0; bb < n; bb++) acc = acc + u[v + V * (a + n * bb)] * op->u2l[qq + s * bb];
+                for (int bb = 0; bb < n; bb++) acc = acc + u[v + V * (a + n * bb)] * o.u2l[qq + s * bb];
                 sLy[blk][v * n * s + a + n * qq] = acc;
             }
         }
@@ -2703,7 +2784,7 @@ __global__ __launch_bounds__(64) void k_subcell(StepBufs b)
                 for (int v = 0; v < V; v++) {
                     double acc = 0.0;
 #pragma unroll
-                    for (int a = 0; a < n; a++) acc = acc + sLy[blk][v * n * s + a + n * qq] * op->u2l[p + s * a];
+                    for (int a = 0; a < n; a++) acc = acc + sLy[blk][v * n * s + a + n * qq] * o.u2l[p + s * a];
                     val[v] = acc;
                 }
             } else {
@@ -2830,8 +2911,8 @@ __global__ __launch_bounds__(64) void k_subcell(StepBufs b)
         }
 #undef S0
         __syncthreads();
-        subcell_finish<P, N, 64>(m, b, gcell, slot, sa.lh0, sa.lhcap, par, flag, sSub, sUhy, sUh, sTy, sBy, sLim, sLob,
-                                 sDt, sRng, &sFlag, &sNth);
+        subcell_finish<P, N, 64>(m, b, gcell, slot, sa.lh0, sa.lhcap, par, flag, sSub, sUh, sLim, sLob, sDt, sRng,
+                                 &sFlag, &sNth, o);
     }
 }
 
@@ -2861,12 +2942,11 @@ __global__ __launch_bounds__(kSubBnBlock) __attribute__((amdgpu_waves_per_eu(2))
     constexpr int NF = (s + 1) * s;       // x-faces (and y-faces) of the interior subcells
     constexpr int FP = (2 * NF + NT - 1) / NT;  // faces per thread
     constexpr int LY = 9 * V * n * s, SL = 3 * V * HH, FL = 4 * V * NF;
-    constexpr int DMPB = V * n * s + V * NN + V * SS + V * NN;
+    constexpr int DMPB = V * SS + V * NN;
     constexpr int R0 = cmax(cmax(LY, SL), cmax(FL, DMPB));
     __shared__ double sS0[V * GG];
     __shared__ double sR0[R0];     // limy of 9 blocks; slopes x / y / t; face F* and D; DMP scratch
     __shared__ double sSub[V * SS];  // NCP source of the interior half-step states, then the new averages
-    __shared__ double sUhy[V * s * n];
     __shared__ double sUh[V * NN];
     __shared__ double sDt[NN];
     __shared__ double sRng[8 * V + 1];
@@ -2878,6 +2958,8 @@ __global__ __launch_bounds__(kSubBnBlock) __attribute__((amdgpu_waves_per_eu(2))
     double *sLy = sR0;
     double *slx = sR0, *sly = sR0 + V * HH, *slt = sR0 + 2 * V * HH;
     double *sFF = sR0, *sFD = sR0 + 2 * V * NF;  // [dir][v][face]: F* and D of x- (dir 0) and y-faces
+    __shared__ SubOpLds<N> sOps;
+    const SubOps o = stage_subops<N>(op, sOps, lane, NT);
     for (int item = blockIdx.x; item < count; item += gridDim.x) {
         const int gcell = b.list[item];
         const int slot = ld_const(b.cell_solve + gcell);
@@ -2909,7 +2991,7 @@ __global__ __launch_bounds__(kSubBnBlock) __attribute__((amdgpu_waves_per_eu(2))
             for (int v = 0; v < V; v++) {
                 double acc = 0.0;
 #pragma unroll
-                for (int bb = 0; bb < n; bb++) acc = acc + u[v + V * (a + n * bb)] * op->u2l[qq + s * bb];
+                for (int bb = 0; bb < n; bb++) acc = acc + u[v + V * (a + n * bb)] * o.u2l[qq + s * bb];
                 sLy[(blk * V + v) * n * s + a + n * qq] = acc;
             }
         }
@@ -2925,7 +3007,7 @@ __global__ __launch_bounds__(kSubBnBlock) __attribute__((amdgpu_waves_per_eu(2))
                 for (int v = 0; v < V; v++) {
                     double acc = 0.0;
 #pragma unroll
-                    for (int a = 0; a < n; a++) acc = acc + sLy[(blk * V + v) * n * s + a + n * qq] * op->u2l[p + s * a];
+                    for (int a = 0; a < n; a++) acc = acc + sLy[(blk * V + v) * n * s + a + n * qq] * o.u2l[p + s * a];
                     sS0[v * GG + task] = acc;
                 }
             } else {
@@ -3068,9 +3150,9 @@ __global__ __launch_bounds__(kSubBnBlock) __attribute__((amdgpu_waves_per_eu(2))
 #undef S0
 #undef HIX
         __syncthreads();
-        double *sTy = sR0, *sBy = sTy + V * n * s, *sLim = sBy + V * NN, *sLob = sLim + V * SS;
-        subcell_finish<P, N, NT>(m, b, gcell, slot, sa.lh0, sa.lhcap, par, flag, sSub, sUhy, sUh, sTy, sBy, sLim,
-                                 sLob, sDt, sRng, &sFlag, &sNth);
+        double *sLim = sR0, *sLob = sLim + V * SS;
+        subcell_finish<P, N, NT>(m, b, gcell, slot, sa.lh0, sa.lhcap, par, flag, sSub, sUh, sLim, sLob, sDt, sRng,
+                                 &sFlag, &sNth, o);
     }
 }
 
@@ -3303,9 +3385,9 @@ __global__ __launch_bounds__(64) void k_seed_ic(StepBufs b, IcSpec ic, int slot,
         }
         if (lane == 0) sFail = 0;
         __syncthreads();
-        cell_ranges_stage1<P, N>(op, sUh, sTy, sBy, lane, 64);
+        cell_ranges_stage1<P, N>(op->u2l, op->u2b, sUh, sTy, sBy, lane, 64);
         __syncthreads();
-        if (cell_ranges_stage2<P, N>(op, m, sTy, sBy, sLim, sLob, lane, 64)) atomicOr(&sFail, 1);
+        if (cell_ranges_stage2<P, N>(op->u2l, op->u2b, m, sTy, sBy, sLim, sLob, lane, 64)) atomicOr(&sFail, 1);
         __syncthreads();
         for (int v = lane; v < V; v += 64) {
             double umn = __builtin_nan(""), umx = umn, smn = umn, smx = umn, bmn = umn, bmx = umn;
