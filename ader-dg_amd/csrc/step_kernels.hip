@@ -2559,78 +2559,94 @@ ADG_DEV void subcell_finish(const M &m, const StepBufs &b, size_t cell, int sol,
         pmn[k] = v < V ? b.lmin[(size_t)v * b.G + cell] : 0.0;
         pmx[k] = v < V ? b.lmax[(size_t)v * b.G + cell] : 0.0;
     }
-    // Each pair of separable passes below runs as one pass per output point, which forms the
-    // first pass's sums it needs itself (the same sums in the same order, so the same bits):
-    // more multiplies, but one barrier and one LDS round trip fewer per pair.
-    // put_subcell_data (Euler/limiter.f90:86-96): the nodal values of the new subcell averages,
-    // one (node, variable) per thread
-    constexpr int PU = N >= 4 ? 1 : s;  // outer loops of the fused passes: unrolled, they spill at N = 4
-    for (int task = lane; task < V * NN; task += NT) {
-        const int v = task / NN, node = task - v * NN, a = node % n, bb = node / n;
-        double acc = 0.0;
-#pragma unroll PU
-        for (int p = 0; p < s; p++) {
-            double y = 0.0;  // :86-90, the first pass at (p, bb)
+    // Each pair of separable passes below runs as one pass per row: a thread forms the row's
+    // first-pass sums once and then the row's outputs from them (the same sums in the same
+    // order, so the same bits), one barrier and one LDS round trip fewer per pair.
+    // put_subcell_data (Euler/limiter.f90:86-96): the nodal values of the new subcell
+    // averages, node row bb of variable v per thread
+    for (int task = lane; task < V * n; task += NT) {
+        const int v = task / n, bb = task - v * n;
+        double y[s];  // :86-90, the first pass at (p, bb)
 #pragma unroll
-            for (int qq = 0; qq < s; qq++) y = y + sSub[v * SS + p + s * qq] * o.l2u[bb + n * qq];
-            acc = acc + y * o.l2u[a + n * p];
+        for (int p = 0; p < s; p++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int qq = 0; qq < s; qq++) acc = acc + sSub[v * SS + p + s * qq] * o.l2u[bb + n * qq];
+            y[p] = acc;
         }
-        sUh[v * NN + node] = acc;
-        b.uh[par ^ 1][cell * V * NN + V * node + v] = acc;
+#pragma unroll
+        for (int a = 0; a < n; a++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int p = 0; p < s; p++) acc = acc + y[p] * o.l2u[a + n * p];
+            sUh[v * NN + a + n * bb] = acc;
+            b.uh[par ^ 1][cell * V * NN + V * (a + n * bb) + v] = acc;
+        }
     }
     if (lane == 0) *sFlag = 0;
     __syncthreads();
-    // the new cell's subcell data (get_subcell_data :25-35, positivity :227-241) and Lobatto
-    // data (get_lobatto_data :56-66), one point per thread, and its nodes' time-step candidates
-    bool bad = false;
-    for (int task = lane; task < SS + 2 * NN; task += NT) {
-        if (task < SS) {
-            const int p = task % s, qq = task / s;
-            double L[V];
+    // the new cell's subcell data (get_subcell_data :25-35) and Lobatto data
+    // (get_lobatto_data :56-66), a row of one variable per thread, and its nodes' time-step
+    // candidates
+    for (int task = lane; task < V * s + V * n + NN; task += NT) {
+        if (task < V * s) {
+            const int v = task / s, qq = task - v * s;
+            double ty[n];  // limy(a, qq), :25-27
 #pragma unroll
-            for (int v = 0; v < V; v++) {
+            for (int a = 0; a < n; a++) {
                 double acc = 0.0;
-#pragma unroll PU
-                for (int a = 0; a < n; a++) {
-                    double ty = 0.0;  // limy(a, qq), :25-27
 #pragma unroll
-                    for (int bb = 0; bb < n; bb++) ty = ty + sUh[v * NN + a + n * bb] * o.u2l[qq + s * bb];
-                    acc = acc + ty * o.u2l[p + s * a];
-                }
-                sLim[v * SS + task] = acc;
-                L[v] = acc;
+                for (int bb = 0; bb < n; bb++) acc = acc + sUh[v * NN + a + n * bb] * o.u2l[qq + s * bb];
+                ty[a] = acc;
             }
-            bad = bad || P::inadmissible(m, L);
-        } else if (task < SS + NN) {
-            const int t = task - SS, p = t % n, qq = t / n;
 #pragma unroll
-            for (int v = 0; v < V; v++) {
+            for (int p = 0; p < s; p++) {
                 double acc = 0.0;
-#pragma unroll PU
-                for (int a = 0; a < n; a++) {
-                    double by = 0.0;  // loby(a, qq), :56-58
 #pragma unroll
-                    for (int bb = 0; bb < n; bb++) by = by + sUh[v * NN + a + n * bb] * o.u2b[qq + n * bb];
-                    acc = acc + by * o.u2b[p + n * a];
-                }
-                sLob[v * NN + t] = acc;
+                for (int a = 0; a < n; a++) acc = acc + ty[a] * o.u2l[p + s * a];
+                sLim[v * SS + p + s * qq] = acc;
+            }
+        } else if (task < V * s + V * n) {
+            const int t = task - V * s, v = t / n, qq = t - v * n;
+            double by[n];  // loby(a, qq), :56-58
+#pragma unroll
+            for (int a = 0; a < n; a++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int bb = 0; bb < n; bb++) acc = acc + sUh[v * NN + a + n * bb] * o.u2b[qq + n * bb];
+                by[a] = acc;
+            }
+#pragma unroll
+            for (int p = 0; p < n; p++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int a = 0; a < n; a++) acc = acc + by[a] * o.u2b[p + n * a];
+                sLob[v * NN + p + n * qq] = acc;
             }
         } else {
-            const int node = task - SS - NN;
+            const int node = task - V * s - V * n;
             double un[V];
 #pragma unroll
             for (int v = 0; v < V; v++) un[v] = sUh[v * NN + node];
             sDt[node] = node_dt<P>(m, un);
         }
     }
-    if (bad) atomicOr(sFlag, 1);
     __syncthreads();
     // every range the rest needs, one fold per thread (each in the order of group_ranges):
     // sRng[kind * V + v], kind 0/1 nodal min/max, 2/3 Lobatto, 4/5 subcell, 6/7 the new
     // subcell averages (a limited cell's next ranges, :135-139); sRng[8V] the time-step minimum
     // (the folds are unrolled over the compile-time counts, so that a thread's LDS reads are
     // issued ahead of its fold chain instead of one round trip per element)
-    for (int t = lane; t <= 8 * V; t += NT) {
+    // The threads past the folds check the positivity of the subcell averages (:227-241).
+    bool bad = false;
+    for (int t = lane; t < 8 * V + 1 + SS; t += NT) {
+        if (t > 8 * V) {
+            double L[V];
+#pragma unroll
+            for (int v = 0; v < V; v++) L[v] = sLim[v * SS + t - 8 * V - 1];
+            bad = bad || P::inadmissible(m, L);
+            continue;
+        }
         double r = __builtin_nan("");
         if (t == 8 * V) {
 #pragma unroll
@@ -2660,6 +2676,7 @@ ADG_DEV void subcell_finish(const M &m, const StepBufs &b, size_t cell, int sol,
         }
         sRng[t] = r;
     }
+    if (bad) atomicOr(sFlag, 1);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < KV; k++) {
