@@ -2632,49 +2632,71 @@ ADG_DEV void subcell_finish(const M &m, const StepBufs &b, size_t cell, int sol,
         }
     }
     __syncthreads();
-    // every range the rest needs, one fold per thread (each in the order of group_ranges):
-    // sRng[kind * V + v], kind 0/1 nodal min/max, 2/3 Lobatto, 4/5 subcell, 6/7 the new
-    // subcell averages (a limited cell's next ranges, :135-139); sRng[8V] the time-step minimum
-    // (the folds are unrolled over the compile-time counts, so that a thread's LDS reads are
-    // issued ahead of its fold chain instead of one round trip per element)
-    // The threads past the folds check the positivity of the subcell averages (:227-241).
+    // every range the rest needs, each fold in the order of group_ranges: sRng[kind * V + v],
+    // kind 0/1 nodal min/max, 2/3 Lobatto, 4/5 subcell, 6/7 the new subcell averages (a limited
+    // cell's next ranges, :135-139); sRng[8V] the time-step minimum.  A thread takes the min and
+    // the max of one group (two independent chains), its elements loaded ahead of the folds
+    // (compile-time counts, fully unrolled).  With two waves the first folds and the second
+    // checks the positivity of the new subcell averages (:227-241) point by point.
     bool bad = false;
-    for (int t = lane; t < 8 * V + 1 + SS; t += NT) {
-        if (t > 8 * V) {
-            double L[V];
+    auto fold_task = [&](int t) {  // t < 4V + 1: subcell groups, nodal / Lobatto groups, dt
+        if (t < 2 * V) {
+            const int grp = t / V, v = t - grp * V;
+            const double *src = grp == 0 ? sLim + v * SS : sSub + v * SS;
+            double r0 = __builtin_nan(""), r1 = r0;
+            constexpr int C = SS < 25 ? SS : 25;
 #pragma unroll
-            for (int v = 0; v < V; v++) L[v] = sLim[v * SS + t - 8 * V - 1];
-            bad = bad || P::inadmissible(m, L);
-            continue;
-        }
-        double r = __builtin_nan("");
-        if (t == 8 * V) {
+            for (int k0 = 0; k0 < SS; k0 += C) {
+                double x[C];
 #pragma unroll
-            for (int k = 0; k < NN; k++) r = fold_min(r, sDt[k]);
-        } else {
-            const int kind = t / V, v = t - kind * V, grp = kind >> 1;
-            const bool mx = kind & 1;
-            if (grp < 2) {
-                const double *src = grp == 0 ? sUh + v * NN : sLob + v * NN;
-                double x[NN];
+                for (int k = 0; k < C; k++) x[k] = k0 + k < SS ? src[k0 + k] : 0.0;
 #pragma unroll
-                for (int k = 0; k < NN; k++) x[k] = src[k];
-#pragma unroll
-                for (int k = 0; k < NN; k++) r = mx ? fold_max(r, x[k]) : fold_min(r, x[k]);
-            } else {
-                const double *src = grp == 2 ? sLim + v * SS : sSub + v * SS;
-                constexpr int C = SS < 25 ? SS : 25;  // elements loaded ahead of their folds
-#pragma unroll
-                for (int k0 = 0; k0 < SS; k0 += C) {
-                    double x[C];
-#pragma unroll
-                    for (int k = 0; k < C; k++) x[k] = k0 + k < SS ? src[k0 + k] : 0.0;
-#pragma unroll
-                    for (int k = 0; k < C && k0 + k < SS; k++) r = mx ? fold_max(r, x[k]) : fold_min(r, x[k]);
+                for (int k = 0; k < C && k0 + k < SS; k++) {
+                    r0 = fold_min(r0, x[k]);
+                    r1 = fold_max(r1, x[k]);
                 }
             }
+            sRng[(4 + 2 * grp) * V + v] = r0;
+            sRng[(5 + 2 * grp) * V + v] = r1;
+        } else if (t < 4 * V) {
+            const int u = t - 2 * V, grp = u / V, v = u - grp * V;
+            const double *src = grp == 0 ? sUh + v * NN : sLob + v * NN;
+            double x[NN], r0 = __builtin_nan(""), r1 = r0;
+#pragma unroll
+            for (int k = 0; k < NN; k++) x[k] = src[k];
+#pragma unroll
+            for (int k = 0; k < NN; k++) {
+                r0 = fold_min(r0, x[k]);
+                r1 = fold_max(r1, x[k]);
+            }
+            sRng[(2 * grp) * V + v] = r0;
+            sRng[(2 * grp + 1) * V + v] = r1;
+        } else {
+            double r = __builtin_nan("");
+#pragma unroll
+            for (int k = 0; k < NN; k++) r = fold_min(r, sDt[k]);
+            sRng[8 * V] = r;
         }
-        sRng[t] = r;
+    };
+    auto positivity_task = [&](int pt) {  // subcell point pt
+        double L[V];
+#pragma unroll
+        for (int v = 0; v < V; v++) L[v] = sLim[v * SS + pt];
+        bad = bad || P::inadmissible(m, L);
+    };
+    if constexpr (NT >= 128) {
+        if (lane < 64) {
+            for (int t = lane; t < 4 * V + 1; t += 64) fold_task(t);
+        } else {
+            for (int pt = lane - 64; pt < SS; pt += NT - 64) positivity_task(pt);
+        }
+    } else {
+        for (int t = lane; t < 4 * V + 1 + SS; t += NT) {
+            if (t < 4 * V + 1)
+                fold_task(t);
+            else
+                positivity_task(t - 4 * V - 1);
+        }
     }
     if (bad) atomicOr(sFlag, 1);
     __syncthreads();
@@ -2999,17 +3021,30 @@ __global__ __launch_bounds__(kSubBnBlock) __attribute__((amdgpu_waves_per_eu(2))
             sLs[lane] = stt != 0 ? ls : -1;  // lim_slot
         }
         __syncthreads();
-        // blocks (di,dj) of the 3x3 Voronoi neighbourhood, :463-479
-        for (int task = lane; task < 9 * n * s; task += NT) {
-            const int blk = task / (n * s), r = task - blk * n * s, a = r % n, qq = r / n;
+        // blocks (di,dj) of the 3x3 Voronoi neighbourhood, :463-479: a thread loads the n x V
+        // values of column a of a neighbour once and forms half of that column's limy rows qq
+        // (one round of loads for the workgroup instead of one per qq)
+        constexpr int QH = (s + 1) / 2;
+        for (int task = lane; task < 9 * n * 2; task += NT) {
+            const int blk = task / (2 * n), r = task - blk * 2 * n, a = r >> 1, q0 = (r & 1) * QH;
             if (sLs[blk] >= 0) continue;
             const double *u = b.uh[par] + (size_t)sNb[blk] * V * NN;
+            double x[n][V];
 #pragma unroll
-            for (int v = 0; v < V; v++) {
-                double acc = 0.0;
+            for (int bb = 0; bb < n; bb++)
 #pragma unroll
-                for (int bb = 0; bb < n; bb++) acc = acc + u[v + V * (a + n * bb)] * o.u2l[qq + s * bb];
-                sLy[(blk * V + v) * n * s + a + n * qq] = acc;
+                for (int v = 0; v < V; v++) x[bb][v] = u[v + V * (a + n * bb)];
+#pragma unroll
+            for (int dq = 0; dq < QH; dq++) {
+                const int qq = q0 + dq;
+                if (qq >= s) break;
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int bb = 0; bb < n; bb++) acc = acc + x[bb][v] * o.u2l[qq + s * bb];
+                    sLy[(blk * V + v) * n * s + a + n * qq] = acc;
+                }
             }
         }
         __syncthreads();
