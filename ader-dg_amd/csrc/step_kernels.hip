@@ -1756,26 +1756,6 @@ ADG_DEV bool cell_ranges_stage2(const double *u2l, const double *u2b, const M &m
     return bad;
 }
 
-// (minval, maxval) of nodal, Lobatto, subcell groups of variable v
-template <class P, int N>
-ADG_DEV void group_ranges(const double *uhL, const double *lobL, const double *limL, int v, double r[6])
-{
-    using S = Shape<P, N>;
-    constexpr int NN = S::NN, SS = S::SS;
-    double u0 = __builtin_nan(""), u1 = u0, b0 = u0, b1 = u0, l0 = u0, l1 = u0;
-    for (int k = 0; k < NN; k++) {
-        u0 = fold_min(u0, uhL[v * NN + k]);
-        u1 = fold_max(u1, uhL[v * NN + k]);
-        b0 = fold_min(b0, lobL[v * NN + k]);
-        b1 = fold_max(b1, lobL[v * NN + k]);
-    }
-    for (int k = 0; k < SS; k++) {
-        l0 = fold_min(l0, limL[v * SS + k]);
-        l1 = fold_max(l1, limL[v * SS + k]);
-    }
-    r[0] = u0; r[1] = u1; r[2] = b0; r[3] = b1; r[4] = l0; r[5] = l1;
-}
-
 // Pool slot of the subcell averages of a cell (StepBufs::lh), or -1 for a cell without
 // limiter (and for a limited cell of a solve whose pool segment ran out: that solve has failed).
 ADG_DEV int lim_slot(const StepBufs &b, int par, size_t cell)
@@ -2632,7 +2612,8 @@ ADG_DEV void subcell_finish(const M &m, const StepBufs &b, size_t cell, int sol,
         }
     }
     __syncthreads();
-    // every range the rest needs, each fold in the order of group_ranges: sRng[kind * V + v],
+    // every range the rest needs, each fold over its group in element order (MINVAL / MAXVAL,
+    // Euler/limiter.f90:123-132): sRng[kind * V + v],
     // kind 0/1 nodal min/max, 2/3 Lobatto, 4/5 subcell, 6/7 the new subcell averages (a limited
     // cell's next ranges, :135-139); sRng[8V] the time-step minimum.  A thread takes the min and
     // the max of one group (two independent chains), its elements loaded ahead of the folds
